@@ -1,0 +1,545 @@
+// ffi.cpp -- the gsplat.h / gsplat_ext.h C ABI.
+//
+// Mirrors crates/gsplat-ffi-c/src/lib.rs: thread-local last-error message
+// ("ok" on success, "<fn>: <detail>" on failure, :190-223), static code
+// strings (:267-283), NULL checks before any work, and an exception firewall
+// on every entry point standing in for catch_unwind (:225-255).
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "common.h"
+#include "renderer.h"
+
+struct GsplatContext {
+  std::unique_ptr<gs::Renderer> renderer;
+  GsplatCamera camera;
+};
+
+struct GsplatPlyScene {
+  gs::HostScene scene;
+};
+
+namespace {
+
+thread_local std::string g_last_error = "ok";
+
+void set_last(const std::string &m) {
+  std::string clean;
+  clean.reserve(m.size());
+  for (char c : m)
+    if (c != '\0') clean.push_back(c);
+  g_last_error = clean;
+}
+
+int32_t ok() {
+  set_last("ok");
+  return 0;
+}
+
+int32_t fail(gs::Code c, const std::string &msg) {
+  set_last(msg);
+  return (int32_t)c;
+}
+
+// catch_unwind equivalent: gs::Error keeps its code, anything else is a
+// panic (INTERNAL with the fixed text).
+template <typename F>
+int32_t guard(const char *op, F &&body) {
+  try {
+    return body();
+  } catch (const gs::Error &e) {
+    return fail(e.code, std::string(op) + ": " + e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(gs::Code::Internal, std::string(op) + ": panic caught at C ABI boundary");
+  } catch (...) {
+    try {
+      set_last(std::string(op) + ": panic caught at C ABI boundary");
+    } catch (...) {
+    }
+    return (int32_t)gs::Code::Internal;
+  }
+}
+
+int32_t null_arg(const char *op, const char *what) {
+  return fail(gs::Code::InvalidArgument, std::string(op) + ": " + what + " is null");
+}
+
+gs::PlyLimits limits_from(const GsplatPlyLimits *l) {
+  gs::PlyLimits out;
+  if (l) {
+    out.max_input_bytes = l->max_input_bytes;
+    out.max_header_bytes = l->max_header_bytes;
+    out.max_vertices = l->max_vertices;
+    out.max_vertex_properties = l->max_vertex_properties;
+    out.max_scene_bytes = l->max_scene_bytes;
+  }
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------ gsplat.h
+
+uint32_t gsplat_version_major(void) { return 0; }
+uint32_t gsplat_version_minor(void) { return 1; }
+
+const char *gsplat_error_message(int32_t code) {
+  switch (code) {
+    case 0: return "ok";
+    case 1: return "invalid argument";
+    case 2: return "not found";
+    case 3: return "parse failed";
+    case 4: return "unsupported";
+    case 5: return "scene not loaded";
+    case 100: return "internal error";
+    default: return "unknown error";
+  }
+}
+
+const char *gsplat_last_error_message(void) { return g_last_error.c_str(); }
+
+GsplatConfig gsplat_config_default(void) {
+  GsplatConfig c;
+  c.width = 1280;
+  c.height = 720;
+  c.mode = GSPLAT_RENDER_MODE_SORTED_ALPHA;
+  return c;
+}
+
+GsplatCamera gsplat_camera_default(void) { return gs::camera_default(); }
+
+// gsplat_context_create (ffi-c/src/lib.rs:325-375).
+int32_t gsplat_context_create(GsplatConfig config, GsplatContext **out_ctx) {
+  static const char *op = "gsplat_context_create";
+  return guard(op, [&]() -> int32_t {
+    if (!out_ctx) return fail(gs::Code::InvalidArgument, "gsplat_context_create: out_ctx is null");
+    *out_ctx = nullptr;
+    if (config.mode != GSPLAT_RENDER_MODE_SORTED_ALPHA)
+      return fail(gs::Code::InvalidArgument, "gsplat_context_create: unsupported render mode");
+    if (config.width == 0 || config.height == 0)
+      return fail(gs::Code::InvalidArgument, "gsplat_context_create: invalid renderer configuration");
+    auto ctx = std::make_unique<GsplatContext>();
+    ctx->renderer = std::make_unique<gs::Renderer>(config.width, config.height);
+    ctx->camera = gs::camera_default();
+    *out_ctx = ctx.release();
+    return ok();
+  });
+}
+
+void gsplat_context_destroy(GsplatContext *ctx) {
+  try {
+    delete ctx;
+  } catch (...) {
+    set_last("gsplat_context_destroy: panic caught at C ABI boundary");
+  }
+}
+
+int32_t gsplat_context_set_camera(GsplatContext *ctx, GsplatCamera camera) {
+  static const char *op = "gsplat_context_set_camera";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!gs::camera_valid(camera)) return fail(gs::Code::InvalidArgument, std::string(op) + ": invalid camera");
+    ctx->camera = camera;
+    return ok();
+  });
+}
+
+int32_t gsplat_context_set_auto_camera(GsplatContext *ctx) {
+  static const char *op = "gsplat_context_set_auto_camera";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    try {
+      ctx->camera = gs::auto_camera(ctx->renderer->scene(), ctx->renderer->width(), ctx->renderer->height());
+    } catch (const gs::Error &e) {
+      return fail(e.code, std::string(op) + ": scene is not loaded");
+    }
+    return ok();
+  });
+}
+
+int32_t gsplat_context_load_scene_path(GsplatContext *ctx, const char *path) {
+  static const char *op = "gsplat_context_load_scene_path";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!path) return null_arg(op, "path");
+    size_t len = std::strlen(path);
+    // CStr::to_str: the path must be UTF-8.
+    {
+      const unsigned char *u = reinterpret_cast<const unsigned char *>(path);
+      size_t i = 0;
+      while (i < len) {
+        unsigned char c = u[i];
+        size_t l = c < 0x80 ? 1 : (c & 0xE0) == 0xC0 ? 2 : (c & 0xF0) == 0xE0 ? 3 : (c & 0xF8) == 0xF0 ? 4 : 0;
+        bool good = l != 0 && i + l <= len;
+        for (size_t k = 1; good && k < l; ++k) good = (u[i + k] & 0xC0) == 0x80;
+        if (good && l == 2) good = c >= 0xC2;
+        if (!good) return fail(gs::Code::InvalidArgument, std::string(op) + ": path is not valid UTF-8");
+        i += l;
+      }
+    }
+    gs::HostScene scene = gs::ply_load(path, gs::PlyLimits());
+    ctx->renderer->load_scene(std::move(scene));
+    return ok();
+  });
+}
+
+int32_t gsplat_context_render_frame(GsplatContext *ctx) {
+  static const char *op = "gsplat_context_render_frame";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    ctx->renderer->render(ctx->camera);
+    return ok();
+  });
+}
+
+int32_t gsplat_context_get_stats(const GsplatContext *ctx, GsplatStats *out_stats) {
+  static const char *op = "gsplat_context_get_stats";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!out_stats) return null_arg(op, "out_stats");
+    *out_stats = ctx->renderer->stats();
+    return ok();
+  });
+}
+
+// ------------------------------------------- surface API (headless device)
+
+#define GS_SURFACE_UNSUPPORTED(op) \
+  fail(gs::Code::Unsupported, std::string(op) + ": native window surfaces are not available on this device")
+
+int32_t gsplat_surface_renderer_create_android(void *native_window, const char *path, uint32_t, uint32_t,
+                                               GsplatSurfaceRenderer **out_renderer) {
+  static const char *op = "gsplat_surface_renderer_create_android";
+  return guard(op, [&]() -> int32_t {
+    if (!out_renderer) return null_arg(op, "out_renderer");
+    *out_renderer = nullptr;
+    if (!native_window) return null_arg(op, "native_window");
+    if (!path) return null_arg(op, "path");
+    return GS_SURFACE_UNSUPPORTED(op);
+  });
+}
+
+int32_t gsplat_surface_renderer_create_android_with_geometry_path(void *native_window, const char *path, uint32_t w,
+                                                                  uint32_t h, uint32_t,
+                                                                  GsplatSurfaceRenderer **out_renderer) {
+  return gsplat_surface_renderer_create_android(native_window, path, w, h, out_renderer);
+}
+
+int32_t gsplat_surface_renderer_create_uikit(void *ui_view, void *, const char *path, uint32_t, uint32_t,
+                                             GsplatSurfaceRenderer **out_renderer) {
+  static const char *op = "gsplat_surface_renderer_create_uikit";
+  return guard(op, [&]() -> int32_t {
+    if (!out_renderer) return null_arg(op, "out_renderer");
+    *out_renderer = nullptr;
+    if (!ui_view) return null_arg(op, "ui_view");
+    if (!path) return null_arg(op, "path");
+    return GS_SURFACE_UNSUPPORTED(op);
+  });
+}
+
+int32_t gsplat_surface_renderer_create_uikit_with_geometry_path(void *ui_view, void *ui_view_controller,
+                                                                const char *path, uint32_t w, uint32_t h, uint32_t,
+                                                                GsplatSurfaceRenderer **out_renderer) {
+  return gsplat_surface_renderer_create_uikit(ui_view, ui_view_controller, path, w, h, out_renderer);
+}
+
+void gsplat_surface_renderer_destroy(GsplatSurfaceRenderer *) {}
+
+#define GS_SURFACE_ENTRY(name, ...)                                  \
+  int32_t name(__VA_ARGS__) {                                        \
+    static const char *op = #name;                                   \
+    return guard(op, [&]() -> int32_t {                              \
+      if (!renderer) return null_arg(op, "renderer");                \
+      return GS_SURFACE_UNSUPPORTED(op);                             \
+    });                                                              \
+  }
+
+GS_SURFACE_ENTRY(gsplat_surface_renderer_resize, GsplatSurfaceRenderer *renderer, uint32_t, uint32_t)
+GS_SURFACE_ENTRY(gsplat_surface_renderer_set_sort_interval, GsplatSurfaceRenderer *renderer, uint32_t)
+GS_SURFACE_ENTRY(gsplat_surface_renderer_set_geometry_path, GsplatSurfaceRenderer *renderer, uint32_t)
+GS_SURFACE_ENTRY(gsplat_surface_renderer_set_gpu_preproject, GsplatSurfaceRenderer *renderer, uint32_t)
+GS_SURFACE_ENTRY(gsplat_surface_renderer_set_gpu_preproject_double_buffer, GsplatSurfaceRenderer *renderer, uint32_t)
+GS_SURFACE_ENTRY(gsplat_surface_renderer_set_async_sort, GsplatSurfaceRenderer *renderer, uint32_t)
+GS_SURFACE_ENTRY(gsplat_surface_renderer_set_async_geometry, GsplatSurfaceRenderer *renderer, uint32_t)
+GS_SURFACE_ENTRY(gsplat_surface_renderer_set_instance_buffer_count, GsplatSurfaceRenderer *renderer, uint32_t)
+GS_SURFACE_ENTRY(gsplat_surface_renderer_set_frame_latency, GsplatSurfaceRenderer *renderer, uint32_t)
+GS_SURFACE_ENTRY(gsplat_surface_renderer_reset_camera, GsplatSurfaceRenderer *renderer)
+GS_SURFACE_ENTRY(gsplat_surface_renderer_orbit, GsplatSurfaceRenderer *renderer, float, float)
+GS_SURFACE_ENTRY(gsplat_surface_renderer_zoom, GsplatSurfaceRenderer *renderer, float)
+GS_SURFACE_ENTRY(gsplat_surface_renderer_pan, GsplatSurfaceRenderer *renderer, float, float)
+GS_SURFACE_ENTRY(gsplat_surface_renderer_render_frame, GsplatSurfaceRenderer *renderer)
+
+int32_t gsplat_surface_renderer_get_stats(const GsplatSurfaceRenderer *renderer, GsplatStats *out_stats) {
+  static const char *op = "gsplat_surface_renderer_get_stats";
+  return guard(op, [&]() -> int32_t {
+    if (!renderer) return null_arg(op, "renderer");
+    if (!out_stats) return null_arg(op, "out_stats");
+    return GS_SURFACE_UNSUPPORTED(op);
+  });
+}
+
+int32_t gsplat_surface_renderer_get_sort_stats(const GsplatSurfaceRenderer *renderer,
+                                               GsplatSurfaceSortStats *out_stats) {
+  static const char *op = "gsplat_surface_renderer_get_sort_stats";
+  return guard(op, [&]() -> int32_t {
+    if (!renderer) return null_arg(op, "renderer");
+    if (!out_stats) return null_arg(op, "out_stats");
+    return GS_SURFACE_UNSUPPORTED(op);
+  });
+}
+
+// --------------------------------------------------------- gsplat_ext.h
+
+int32_t gsplat_context_load_scene_memory(GsplatContext *ctx, const GsplatSceneView *view) {
+  static const char *op = "gsplat_context_load_scene_memory";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!view) return null_arg(op, "scene");
+    gs::HostScene s;
+    try {
+      s = gs::scene_from_view(*view);
+    } catch (const gs::Error &e) {
+      // Renderer::load_scene maps invalid buffers to InvalidScene
+      // (render-wgpu/src/lib.rs:488-490 -> InvalidArgument), except the
+      // unsupported SH degree.
+      if (e.code == gs::Code::Unsupported) throw;
+      return fail(gs::Code::InvalidArgument, std::string(op) + ": invalid scene buffers");
+    }
+    ctx->renderer->load_scene(std::move(s));
+    return ok();
+  });
+}
+
+int32_t gsplat_context_set_size(GsplatContext *ctx, uint32_t width, uint32_t height) {
+  static const char *op = "gsplat_context_set_size";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    ctx->renderer->set_size(width, height);
+    return ok();
+  });
+}
+
+int32_t gsplat_context_get_camera(const GsplatContext *ctx, GsplatCamera *out_camera) {
+  static const char *op = "gsplat_context_get_camera";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!out_camera) return null_arg(op, "out_camera");
+    *out_camera = ctx->camera;
+    return ok();
+  });
+}
+
+int32_t gsplat_context_wait(GsplatContext *ctx) {
+  static const char *op = "gsplat_context_wait";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    ctx->renderer->wait();
+    return ok();
+  });
+}
+
+int32_t gsplat_context_readback_rgba8(GsplatContext *ctx, uint8_t *out, uint64_t len) {
+  static const char *op = "gsplat_context_readback_rgba8";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!out) return null_arg(op, "out");
+    ctx->renderer->readback_rgba8(out, len);
+    return ok();
+  });
+}
+
+int32_t gsplat_context_readback_rgba32f(GsplatContext *ctx, float *out, uint64_t len) {
+  static const char *op = "gsplat_context_readback_rgba32f";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!out) return null_arg(op, "out");
+    ctx->renderer->readback_rgba32f(out, len);
+    return ok();
+  });
+}
+
+int32_t gsplat_context_get_stage_times(GsplatContext *ctx, GsplatStageTimes *out) {
+  static const char *op = "gsplat_context_get_stage_times";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!out) return null_arg(op, "out");
+    *out = ctx->renderer->stage_times();
+    return ok();
+  });
+}
+
+int32_t gsplat_context_read_depth_keys(GsplatContext *ctx, uint32_t *out, uint64_t cap) {
+  static const char *op = "gsplat_context_read_depth_keys";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!out) return null_arg(op, "out");
+    ctx->renderer->read_depth_keys(out, cap);
+    return ok();
+  });
+}
+
+int32_t gsplat_context_read_sorted_indices(GsplatContext *ctx, uint32_t *out, uint64_t cap, uint32_t *out_count) {
+  static const char *op = "gsplat_context_read_sorted_indices";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!out) return null_arg(op, "out");
+    if (!out_count) return null_arg(op, "out_count");
+    ctx->renderer->read_sorted(out, cap, out_count);
+    return ok();
+  });
+}
+
+int32_t gsplat_context_read_projection(GsplatContext *ctx, float *out, uint64_t cap) {
+  static const char *op = "gsplat_context_read_projection";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!out) return null_arg(op, "out");
+    ctx->renderer->read_projection(ctx->camera, out, cap);
+    return ok();
+  });
+}
+
+int32_t gsplat_context_read_tile_counts(GsplatContext *ctx, uint32_t *out, uint64_t cap, uint32_t *out_tiles_x,
+                                        uint32_t *out_tiles_y) {
+  static const char *op = "gsplat_context_read_tile_counts";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!out) return null_arg(op, "out");
+    if (!out_tiles_x) return null_arg(op, "out_tiles_x");
+    if (!out_tiles_y) return null_arg(op, "out_tiles_y");
+    ctx->renderer->read_tile_counts(out, cap, out_tiles_x, out_tiles_y);
+    return ok();
+  });
+}
+
+// CpuSortBackend::sort_values_by_keys (sort/src/lib.rs:36-61) on the device:
+// length mismatch cannot occur through this signature; len <= 1 is a no-op.
+int32_t gsplat_sort_values_by_keys(const uint32_t *keys, uint32_t *values, uint64_t len) {
+  static const char *op = "gsplat_sort_values_by_keys";
+  return guard(op, [&]() -> int32_t {
+    if (len <= 1) return ok();
+    if (!keys) return null_arg(op, "keys");
+    if (!values) return null_arg(op, "values");
+    if (len > 0xFFFFFFF0ull) return fail(gs::Code::Unsupported, std::string(op) + ": length exceeds 32-bit indexing");
+    if (!gs::device_present()) return fail(gs::Code::Unsupported, std::string(op) + ": gpu backend unavailable");
+    const uint32_t n = (uint32_t)len;
+    hipStream_t st;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+      return fail(gs::Code::Internal, std::string(op) + ": stream creation failed");
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+      }
+    } sg{st};
+    gs::DevBuf<uint32_t> k, v, ka, va, hist, partials;
+    gs::DevBuf<uint64_t> total;
+    k.ensure(n);
+    v.ensure(n);
+    ka.ensure(n);
+    va.ensure(n);
+    hist.ensure(gs::radix_hist_entries(n));
+    partials.ensure(gs::scan_partial_entries(gs::radix_hist_entries(n)));
+    total.ensure(1);
+    gs::SortScratch sc;
+    sc.hist = hist.get();
+    sc.hist_cap = hist.capacity();
+    sc.partials = partials.get();
+    sc.partial_cap = partials.capacity();
+    sc.total = total.get();
+    if (hipMemcpyAsync(k.get(), keys, len * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(v.get(), values, len * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+      return fail(gs::Code::Internal, std::string(op) + ": upload failed");
+    uint32_t *pk = k.get(), *pv = v.get(), *pka = ka.get(), *pva = va.get();
+    gs::radix_sort(&pk, &pv, &pka, &pva, n, 32, true, sc, st);
+    if (hipMemcpyAsync(values, pv, len * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return fail(gs::Code::Internal, std::string(op) + ": download failed");
+    return ok();
+  });
+}
+
+GsplatPlyLimits gsplat_ply_limits_default(void) {
+  gs::PlyLimits d;
+  GsplatPlyLimits l;
+  l.max_input_bytes = d.max_input_bytes;
+  l.max_header_bytes = d.max_header_bytes;
+  l.max_vertices = d.max_vertices;
+  l.max_vertex_properties = d.max_vertex_properties;
+  l.max_scene_bytes = d.max_scene_bytes;
+  return l;
+}
+
+int32_t gsplat_ply_load(const char *path, GsplatPlyScene **out_scene) {
+  static const char *op = "gsplat_ply_load";
+  return guard(op, [&]() -> int32_t {
+    if (!out_scene) return null_arg(op, "out_scene");
+    *out_scene = nullptr;
+    if (!path) return null_arg(op, "path");
+    auto s = std::make_unique<GsplatPlyScene>();
+    s->scene = gs::ply_load(path, gs::PlyLimits());
+    *out_scene = s.release();
+    return ok();
+  });
+}
+
+static int32_t ply_parse_entry(const char *op, const uint8_t *bytes, uint64_t len, const GsplatPlyLimits *limits,
+                               GsplatPlyScene **out_scene, bool text) {
+  return guard(op, [&]() -> int32_t {
+    if (!out_scene) return null_arg(op, "out_scene");
+    *out_scene = nullptr;
+    if (!bytes && len) return null_arg(op, "bytes");
+    static const uint8_t empty = 0;
+    auto s = std::make_unique<GsplatPlyScene>();
+    s->scene = gs::ply_parse(bytes ? bytes : &empty, (size_t)len, limits_from(limits), text);
+    *out_scene = s.release();
+    return ok();
+  });
+}
+
+int32_t gsplat_ply_parse_bytes(const uint8_t *bytes, uint64_t len, const GsplatPlyLimits *limits,
+                               GsplatPlyScene **out_scene) {
+  return ply_parse_entry("gsplat_ply_parse_bytes", bytes, len, limits, out_scene, false);
+}
+
+int32_t gsplat_ply_parse_text(const uint8_t *bytes, uint64_t len, const GsplatPlyLimits *limits,
+                              GsplatPlyScene **out_scene) {
+  return ply_parse_entry("gsplat_ply_parse_text", bytes, len, limits, out_scene, true);
+}
+
+int32_t gsplat_ply_scene_view(const GsplatPlyScene *scene, GsplatSceneView *out_view) {
+  static const char *op = "gsplat_ply_scene_view";
+  return guard(op, [&]() -> int32_t {
+    if (!scene) return null_arg(op, "scene");
+    if (!out_view) return null_arg(op, "out_view");
+    const gs::HostScene &s = scene->scene;
+    out_view->count = s.count;
+    out_view->sh_degree = s.sh_degree;
+    out_view->positions = s.positions.data();
+    out_view->opacity = s.opacity.data();
+    out_view->scale_xyz = s.scale_xyz.data();
+    out_view->rotation_xyzw = s.rotation_xyzw.data();
+    out_view->color_dc = s.color_dc.data();
+    out_view->sh_rest = s.has_sh_rest ? s.sh_rest.data() : nullptr;
+    return ok();
+  });
+}
+
+void gsplat_ply_free(GsplatPlyScene *scene) { delete scene; }
+
+uint32_t gsplat_device_available(void) {
+  try {
+    return gs::device_present() ? 1u : 0u;
+  } catch (...) {
+    return 0u;
+  }
+}
+
+}  // extern "C"

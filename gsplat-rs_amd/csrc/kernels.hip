@@ -1,0 +1,685 @@
+// kernels.hip -- gfx950 kernels of the SortedAlpha path.
+//
+// Compiled with -ffp-contract=off and correctly rounded f32 divide/sqrt, so
+// every expression that restates reference arithmetic (depth key,
+// projection, SH colour, tile rectangle) rounds exactly like the reference's
+// IEEE single precision; fused multiply-adds appear only where written as
+// fmaf (the per-pixel r^2 of the raster, whose operation order is defined by
+// this renderer because the reference leaves it to the GPU's interpolator).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace gs {
+
+#define GS_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw Error(Code::Internal, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+constexpr uint32_t kWave = 64;
+constexpr uint32_t kWaves = kThreads / kWave;
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const uint32_t lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (uint32_t d = 1; d < kWave; d <<= 1) {
+    uint32_t t = __shfl_up(v, d, kWave);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan across the 256 threads of a workgroup; *total = block sum.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *lds4, uint32_t *total) {
+  const uint32_t inc = wave_inclusive_scan(v);
+  const uint32_t w = threadIdx.x / kWave;
+  if (__lane_id() == kWave - 1) lds4[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kWaves; ++i) {
+    uint32_t x = lds4[i];
+    off += (i < w) ? x : 0u;
+    tot += x;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+// ---------------------------------------------------------------- projection
+
+struct Cov6 {
+  float xx, xy, xz, yy, yz, zz;
+};
+
+// covariance_quadratic / covariance_bilinear (splat_surface_direct.wgsl:95-109).
+__device__ __forceinline__ float cov_quadratic(const Cov6 &c, float r0, float r1, float r2) {
+  return r0 * r0 * c.xx + 2.0f * r0 * r1 * c.xy + 2.0f * r0 * r2 * c.xz + r1 * r1 * c.yy +
+         2.0f * r1 * r2 * c.yz + r2 * r2 * c.zz;
+}
+__device__ __forceinline__ float cov_bilinear(const Cov6 &c, float a0, float a1, float a2, float b0, float b1,
+                                              float b2) {
+  float bx = c.xx * b0 + c.xy * b1 + c.xz * b2;
+  float by = c.xy * b0 + c.yy * b1 + c.yz * b2;
+  float bz = c.xz * b0 + c.yz * b1 + c.zz * b2;
+  return a0 * bx + a1 * by + a2 * bz;
+}
+
+// One splat of eval_sh_rgb (wgsl:233-326) for one channel, in the shader's
+// left-to-right accumulation order. `rest` points at coefficient 0 of the
+// channel in the coefficient-major SoA (stride n floats per coefficient).
+template <int DEG>
+__device__ __forceinline__ float sh_channel(float dc, const float *__restrict__ rest, uint32_t n, float x, float y,
+                                            float z) {
+  const float C0 = 0.28209479177387814f;
+  float r = C0 * dc;
+  if constexpr (DEG >= 1) {
+    const float C1 = 0.4886025119029199f;
+    r = r - C1 * y * rest[0 * (size_t)n] + C1 * z * rest[1 * (size_t)n] - C1 * x * rest[2 * (size_t)n];
+  }
+  if constexpr (DEG >= 2) {
+    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+    r = r + 1.0925484305920792f * xy * rest[3 * (size_t)n] + -1.0925484305920792f * yz * rest[4 * (size_t)n] +
+        0.31539156525252005f * (2.0f * zz - xx - yy) * rest[5 * (size_t)n] +
+        -1.0925484305920792f * xz * rest[6 * (size_t)n] + 0.5462742152960396f * (xx - yy) * rest[7 * (size_t)n];
+    if constexpr (DEG >= 3) {
+      r = r + -0.5900435899266435f * y * (3.0f * xx - yy) * rest[8 * (size_t)n] +
+          2.890611442640554f * xy * z * rest[9 * (size_t)n] +
+          -0.4570457994644658f * y * (4.0f * zz - xx - yy) * rest[10 * (size_t)n] +
+          0.3731763325901154f * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * rest[11 * (size_t)n] +
+          -0.4570457994644658f * x * (4.0f * zz - xx - yy) * rest[12 * (size_t)n] +
+          1.445305721320277f * z * (xx - yy) * rest[13 * (size_t)n] +
+          -0.5900435899266435f * x * (xx - 3.0f * yy) * rest[14 * (size_t)n];
+    }
+    if constexpr (DEG >= 4) {
+      r = r + 2.5033429417967046f * xy * (xx - yy) * rest[15 * (size_t)n] +
+          -1.7701307697799304f * yz * (3.0f * xx - yy) * rest[16 * (size_t)n] +
+          0.9461746957575601f * xy * (7.0f * zz - 1.0f) * rest[17 * (size_t)n] +
+          -0.6690465435572892f * yz * (7.0f * zz - 3.0f) * rest[18 * (size_t)n] +
+          0.10578554691520431f * (zz * (35.0f * zz - 30.0f) + 3.0f) * rest[19 * (size_t)n] +
+          -0.6690465435572892f * xz * (7.0f * zz - 3.0f) * rest[20 * (size_t)n] +
+          0.47308734787878004f * (xx - yy) * (7.0f * zz - 1.0f) * rest[21 * (size_t)n] +
+          -1.7701307697799304f * xz * (xx - 3.0f * yy) * rest[22 * (size_t)n] +
+          0.6258357354491761f * (xx * (xx - 3.0f * yy) - yy * (3.0f * xx - yy)) * rest[23 * (size_t)n];
+    }
+  }
+  return r;
+}
+
+// One thread per splat over the coefficient-major SoA: depth key and cull
+// (preprocess_visible_into, render-wgpu/src/lib.rs:1263-1296), projection
+// (project_splat, wgsl:131-217), colour (eval_color, wgsl:328-334), and the
+// raster record + tile rectangle of this renderer.
+template <int DEG, bool EXPORT>
+__global__ __launch_bounds__(kThreads) void k_project(const float *__restrict__ soa, uint32_t n, FrameParams fp,
+                                                      uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                      float4 *__restrict__ rec, uint2 *__restrict__ rect,
+                                                      uint32_t *__restrict__ visible,
+                                                      float *__restrict__ export12) {
+  constexpr uint32_t PC = (DEG + 1) * (DEG + 1) - 1;
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  bool vis = false;
+  if (i < n) {
+    const size_t N = n;
+    const float relx = soa[kPx * N + i] - fp.cam[0];
+    const float rely = soa[kPy * N + i] - fp.cam[1];
+    const float relz = soa[kPz * N + i] - fp.cam[2];
+    // world_to_camera_depth_with_view_row (lib.rs:1346-1354), no FMA.
+    const float z = fp.rows[6] * relx + fp.rows[7] * rely + fp.rows[8] * relz;
+    vis = z >= fp.near_plane && z <= fp.far_plane;
+    if (!EXPORT || keys != nullptr) {
+      keys[i] = vis ? __float_as_uint(fmaxf(z, 0.0f)) : 0u;  // depth_to_key (lib.rs:973-976)
+      if (vals != nullptr) vals[i] = i;
+    }
+    bool drawn = false;
+    float cx = 0, cy = 0, ux = 0, uy = 0, vx = 0, vy = 0, alpha = 0, col[3] = {0, 0, 0};
+    if (vis && z > 1e-6f) {
+      const float x = fp.rows[0] * relx + fp.rows[1] * rely + fp.rows[2] * relz;
+      const float y = fp.rows[3] * relx + fp.rows[4] * rely + fp.rows[5] * relz;
+      const float inv_z = 1.0f / z;
+      const float x_ndc = ((x * fp.f) * inv_z) / fp.aspect;
+      const float y_ndc = (y * fp.f) * inv_z;
+      const Cov6 wc{soa[kCxx * N + i], soa[kCxy * N + i], soa[kCxz * N + i],
+                    soa[kCyy * N + i], soa[kCyz * N + i], soa[kCzz * N + i]};
+      const float *r0 = &fp.rows[0], *r1 = &fp.rows[3], *r2 = &fp.rows[6];
+      // transform_covariance_terms_to_camera (wgsl:111-120).
+      const Cov6 cc{cov_quadratic(wc, r0[0], r0[1], r0[2]),
+                    cov_bilinear(wc, r0[0], r0[1], r0[2], r1[0], r1[1], r1[2]),
+                    cov_bilinear(wc, r0[0], r0[1], r0[2], r2[0], r2[1], r2[2]),
+                    cov_quadratic(wc, r1[0], r1[1], r1[2]),
+                    cov_bilinear(wc, r1[0], r1[1], r1[2], r2[0], r2[1], r2[2]),
+                    cov_quadratic(wc, r2[0], r2[1], r2[2])};
+      const float x_cl = fminf(fmaxf(x / z, -fp.lim_x), fp.lim_x) * z;
+      const float y_cl = fminf(fmaxf(y / z, -fp.lim_y), fp.lim_y) * z;
+      const float inv_z2 = inv_z * inv_z;
+      const float j00 = fp.fx * inv_z;
+      const float j02 = -fp.fx * x_cl * inv_z2;
+      const float j11 = fp.fy * inv_z;
+      const float j12 = -fp.fy * y_cl * inv_z2;
+      const float b = j00 * j11 * cc.xy + j00 * j12 * cc.xz + j02 * j11 * cc.yz + j02 * j12 * cc.zz;
+      float a = j00 * j00 * cc.xx + 2.0f * j00 * j02 * cc.xz + j02 * j02 * cc.zz;
+      float c = j11 * j11 * cc.yy + 2.0f * j11 * j12 * cc.yz + j12 * j12 * cc.zz;
+      a = a + fp.blur_x;
+      c = c + fp.blur_y;
+      const float apco2 = (a + c) * 0.5f;
+      const float amco2 = (a - c) * 0.5f;
+      const float term = sqrtf(fmaxf(amco2 * amco2 + b * b, 0.0f));
+      const float major = fmaxf(apco2 + term, 1e-10f);
+      const float minor = fmaxf(apco2 - term, 1e-10f);
+      float udx = 1.0f, udy = 0.0f;
+      if (fabsf(b) > 1e-8f) {
+        const float px = b, py = major - a;
+        const float len2 = px * px + py * py;
+        if (len2 > 1e-20f) {
+          const float inv = 1.0f / sqrtf(len2);
+          udx = px * inv;
+          udy = py * inv;
+        }
+      } else if (a < c) {
+        udx = 0.0f;
+        udy = 1.0f;
+      }
+      float major_r = fminf(fmaxf(sqrtf(major) * 3.0f, 1e-4f), 2.0f);
+      const float minor_r = fminf(fmaxf(sqrtf(minor) * 3.0f, 1e-4f), 2.0f);
+      major_r = fminf(major_r, minor_r * 64.0f);
+      ux = udx * major_r;
+      uy = udy * major_r;
+      vx = -udy * minor_r;
+      vy = udx * minor_r;
+      const float ext_x = fabsf(ux) + fabsf(vx);
+      const float ext_y = fabsf(uy) + fabsf(vy);
+      drawn = !(x_ndc + ext_x < -1.0f || x_ndc - ext_x > 1.0f || y_ndc + ext_y < -1.0f || y_ndc - ext_y > 1.0f);
+      if (drawn) {
+        cx = x_ndc;
+        cy = y_ndc;
+        alpha = soa[kAlpha * N + i];
+        // normalize3_or_default(position - camera) (wgsl:87-93, 329).
+        float dxw = 0.0f, dyw = 0.0f, dzw = 1.0f;
+        const float len2 = relx * relx + rely * rely + relz * relz;
+        if (len2 > 1e-20f) {
+          const float inv = 1.0f / sqrtf(len2);
+          dxw = relx * inv;
+          dyw = rely * inv;
+          dzw = relz * inv;
+        }
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+          const float dc = soa[(kDcR + ch) * N + i];
+          const float s = sh_channel<DEG>(dc, soa + (size_t)(kRest0 + ch * PC) * N + i, n, dxw, dyw, dzw);
+          col[ch] = fminf(fmaxf(s + 0.5f, 0.0f), 1.0f) * alpha;
+        }
+      }
+    }
+    if (EXPORT) {
+      float *o = export12 + 12 * (size_t)i;
+      o[0] = drawn ? 1.0f : 0.0f;
+      o[1] = cx;
+      o[2] = cy;
+      o[3] = ux;
+      o[4] = uy;
+      o[5] = vx;
+      o[6] = vy;
+      o[7] = col[0];
+      o[8] = col[1];
+      o[9] = col[2];
+      o[10] = alpha;
+      o[11] = 0.0f;
+    } else {
+      uint2 rr = make_uint2(0xFFFFFFFFu, 0u);
+      if (drawn) {
+        const float cut = fminf(soa[kCut * N + i], 1.0f);
+        const float det = ux * vy - vx * uy;
+        rec[3 * (size_t)i + 0] = make_float4(cx, cy, vy / det, -vx / det);
+        rec[3 * (size_t)i + 1] = make_float4(-uy / det, ux / det, cut, alpha);
+        rec[3 * (size_t)i + 2] = make_float4(col[0], col[1], col[2], 0.0f);
+        if (cut >= 0.0f) {
+          // Bounding box of {center + A l : |l|^2 <= cut} plus one pixel.
+          const float sc = sqrtf(cut);
+          const float ex = sc * sqrtf(ux * ux + vx * vx);
+          const float ey = sc * sqrtf(uy * uy + vy * vy);
+          const float hw = 0.5f * (float)fp.width, hh = 0.5f * (float)fp.height;
+          const float cxp = (cx + 1.0f) * hw;
+          const float cyp = (1.0f - cy) * hh;
+          const float exw = ex * hw + 1.0f;
+          const float eyh = ey * hh + 1.0f;
+          const float x0f = ceilf(cxp - exw - 0.5f), x1f = floorf(cxp + exw - 0.5f);
+          const float y0f = ceilf(cyp - eyh - 0.5f), y1f = floorf(cyp + eyh - 0.5f);
+          const float wm1 = (float)(fp.width - 1), hm1 = (float)(fp.height - 1);
+          if (!(x1f < 0.0f || y1f < 0.0f || x0f > wm1 || y0f > hm1 || x0f > x1f || y0f > y1f)) {
+            const uint32_t x0 = (uint32_t)fmaxf(x0f, 0.0f) / kTile, x1 = (uint32_t)fminf(x1f, wm1) / kTile;
+            const uint32_t y0 = (uint32_t)fmaxf(y0f, 0.0f) / kTile, y1 = (uint32_t)fminf(y1f, hm1) / kTile;
+            rr = make_uint2(x0 | (y0 << 16), x1 | (y1 << 16));
+          }
+        }
+      }
+      rect[i] = rr;
+    }
+  }
+  if (!EXPORT) {
+    // Visible count: wave ballot, one LDS slot per wave, one atomic per block.
+    __shared__ uint32_t wcount[kWaves];
+    const uint64_t m = __ballot(vis);
+    if (__lane_id() == 0) wcount[threadIdx.x / kWave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (uint32_t w = 0; w < kWaves; ++w) t += wcount[w];
+      if (t) atomicAdd(visible, t);
+    }
+  }
+}
+
+// ------------------------------------------------------------------- scans
+
+// First level: each block scans kScanTile items (16 consecutive per thread)
+// into `out` and writes its sum to partials[block].
+__global__ __launch_bounds__(kThreads) void k_scan_blocks(const uint32_t *__restrict__ in,
+                                                          uint32_t *__restrict__ out, size_t n,
+                                                          uint32_t *__restrict__ partials) {
+  __shared__ uint32_t lds4[kWaves];
+  const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * 16;
+  uint32_t v[16];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const size_t p = base + k;
+    v[k] = p < n ? in[p] : 0u;
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(sum, lds4, &total);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const size_t p = base + k;
+    if (p < n) out[p] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// Second level: one block scans the partials (any count) and writes the
+// 64-bit grand total.
+__global__ __launch_bounds__(kThreads) void k_scan_partials(uint32_t *__restrict__ partials, uint32_t count,
+                                                            uint64_t *__restrict__ total64) {
+  __shared__ uint32_t lds4[kWaves];
+  uint32_t carry = 0;
+  uint64_t grand = 0;
+  for (uint32_t base = 0; base < count; base += kThreads) {
+    const uint32_t p = base + threadIdx.x;
+    const uint32_t v = p < count ? partials[p] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan(v, lds4, &tot);
+    if (p < count) partials[p] = carry + ex;
+    carry += tot;
+    grand += tot;
+  }
+  if (threadIdx.x == 0 && total64) *total64 = grand;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t *__restrict__ out, size_t n,
+                                                       const uint32_t *__restrict__ partials) {
+  const uint32_t add = partials[blockIdx.x];
+  const size_t base = (size_t)blockIdx.x * kScanTile;
+  for (uint32_t k = threadIdx.x; k < kScanTile; k += kThreads) {
+    const size_t p = base + k;
+    if (p < n) out[p] += add;
+  }
+}
+
+// -------------------------------------------------------------- radix sort
+
+// Digit of a key for this pass; `flip` = ~0 sorts descending.
+__device__ __forceinline__ uint32_t digit_of(uint32_t key, uint32_t shift, uint32_t flip) {
+  return ((key ^ flip) >> shift) & 0xFFu;
+}
+
+// Lanes of the wave holding the same 8-bit digit (8 ballots), restricted to
+// `active` lanes.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool active) {
+  uint64_t m = __ballot(active);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bb = __ballot(active && bit);
+    m &= bit ? bb : ~bb;
+  }
+  return m;
+}
+
+// Per-block digit histogram, digit-major: hist[d * blocks + block].
+__global__ __launch_bounds__(kThreads) void k_radix_upsweep(const uint32_t *__restrict__ keys, uint32_t n,
+                                                            uint32_t shift, uint32_t flip,
+                                                            uint32_t *__restrict__ hist, uint32_t blocks) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.x * kRadixTile;
+#pragma unroll 4
+  for (uint32_t r = 0; r < kRadixItems; ++r) {
+    const size_t p = base + (size_t)r * kThreads + threadIdx.x;
+    const bool active = p < n;
+    const uint32_t d = active ? digit_of(keys[p], shift, flip) : 0u;
+    const uint64_t peers = match_digit(d, active);
+    if (active && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+  }
+  __syncthreads();
+  hist[(size_t)threadIdx.x * blocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: item order inside a block is (round, wave, lane); the rank
+// of an item among equal digits is the block's running count + the counts of
+// earlier waves this round + its lane rank among peers (ballot + popcount).
+__global__ __launch_bounds__(kThreads) void k_radix_scatter(const uint32_t *__restrict__ kin,
+                                                            const uint32_t *__restrict__ vin,
+                                                            uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                            uint32_t n, uint32_t shift, uint32_t flip,
+                                                            const uint32_t *__restrict__ offsets, uint32_t blocks) {
+  __shared__ uint32_t gbase[256];
+  __shared__ uint32_t wcnt[kWaves][256];
+  const uint32_t t = threadIdx.x, w = t / kWave;
+  gbase[t] = offsets[(size_t)t * blocks + blockIdx.x];
+  uint32_t run = 0;  // running count of digit t in this block
+  const size_t base = (size_t)blockIdx.x * kRadixTile;
+  for (uint32_t r = 0; r < kRadixItems; ++r) {
+#pragma unroll
+    for (uint32_t q = 0; q < kWaves; ++q) wcnt[q][t] = 0;
+    __syncthreads();
+    const size_t p = base + (size_t)r * kThreads + t;
+    const bool active = p < n;
+    uint32_t k = 0, v = 0, d = 0;
+    if (active) {
+      k = kin[p];
+      v = vin[p];
+      d = digit_of(k, shift, flip);
+    }
+    const uint64_t peers = match_digit(d, active);
+    const uint32_t lrank = (uint32_t)__popcll(peers & lanemask_lt());
+    if (active && lrank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    // Thread t owns digit t: exclusive prefix over waves, then advance run.
+    uint32_t s = run;
+#pragma unroll
+    for (uint32_t q = 0; q < kWaves; ++q) {
+      const uint32_t c = wcnt[q][t];
+      wcnt[q][t] = s;
+      s += c;
+    }
+    run = s;
+    __syncthreads();
+    if (active) {
+      const uint32_t pos = gbase[d] + wcnt[w][d] + lrank;
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------- tile binning
+
+__device__ __forceinline__ uint32_t rect_tiles(uint2 r) {
+  if (r.x == 0xFFFFFFFFu) return 0u;
+  const uint32_t x0 = r.x & 0xFFFFu, y0 = r.x >> 16, x1 = r.y & 0xFFFFu, y1 = r.y >> 16;
+  return (x1 - x0 + 1) * (y1 - y0 + 1);
+}
+
+__global__ __launch_bounds__(kThreads) void k_tile_counts(const uint32_t *__restrict__ sorted,
+                                                          const uint32_t *__restrict__ visible,
+                                                          const uint2 *__restrict__ rect, uint32_t n,
+                                                          uint32_t *__restrict__ counts) {
+  const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t v = *visible;
+  counts[j] = j < v ? rect_tiles(rect[sorted[v - 1 - j]]) : 0u;
+}
+
+__global__ __launch_bounds__(kThreads) void k_tile_emit(const uint32_t *__restrict__ sorted,
+                                                        const uint32_t *__restrict__ visible,
+                                                        const uint2 *__restrict__ rect,
+                                                        const uint32_t *__restrict__ offsets, uint32_t n,
+                                                        uint32_t tiles_x, uint32_t *__restrict__ pair_keys,
+                                                        uint32_t *__restrict__ pair_vals) {
+  const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t v = *visible;
+  if (j >= n || j >= v) return;
+  const uint32_t s = sorted[v - 1 - j];
+  const uint2 r = rect[s];
+  if (r.x == 0xFFFFFFFFu) return;
+  const uint32_t x0 = r.x & 0xFFFFu, y0 = r.x >> 16, x1 = r.y & 0xFFFFu, y1 = r.y >> 16;
+  uint32_t o = offsets[j];
+  for (uint32_t ty = y0; ty <= y1; ++ty)
+    for (uint32_t tx = x0; tx <= x1; ++tx) {
+      pair_keys[o] = ty * tiles_x + tx;
+      pair_vals[o] = s;
+      ++o;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_tile_ranges(const uint32_t *__restrict__ keys, uint32_t m,
+                                                          uint2 *__restrict__ ranges) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t t = keys[i];
+  if (i == 0 || keys[i - 1] != t) ranges[t].x = i;
+  if (i == m - 1 || keys[i + 1] != t) ranges[t].y = i + 1;
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+
+// ------------------------------------------------------------------ raster
+
+// One workgroup per 16x16 tile, one thread per pixel. Batches of 256 records
+// are gathered into LDS and blended front to back: C += T*g*rgb_a,
+// T *= 1 - a*g, stopping when T < 1e-4 (equivalent to the reference's
+// back-to-front premultiplied "over", draw_pass.rs:75, up to that bound).
+// Coverage is r^2 <= cut (the fragment test of wgsl:358-367 precomputed per
+// splat); g = exp(-4.5 r^2) (wgsl:363).
+__global__ __launch_bounds__(kThreads) void k_raster(const float4 *__restrict__ rec,
+                                                     const uint2 *__restrict__ ranges,
+                                                     const uint32_t *__restrict__ list, uint32_t width,
+                                                     uint32_t height, uint32_t tiles_x, float4 *__restrict__ out32,
+                                                     uchar4 *__restrict__ out8) {
+  __shared__ float4 s0[kThreads], s1[kThreads], s2[kThreads];
+  const uint32_t tile = blockIdx.x;
+  const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+  const uint32_t x = tx * kTile + (threadIdx.x % kTile), y = ty * kTile + (threadIdx.x / kTile);
+  const bool inside = x < width && y < height;
+  const float pnx = (2.0f * ((float)x + 0.5f)) / (float)width - 1.0f;
+  const float pny = 1.0f - (2.0f * ((float)y + 0.5f)) / (float)height;
+  const uint2 rg = ranges[tile];
+  float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+  bool done = !inside;
+  for (uint32_t start = rg.x; start < rg.y; start += kThreads) {
+    if (__syncthreads_count(!done) == 0) break;
+    const uint32_t k = start + threadIdx.x;
+    if (k < rg.y) {
+      const size_t s = list[k];
+      s0[threadIdx.x] = rec[3 * s + 0];
+      s1[threadIdx.x] = rec[3 * s + 1];
+      s2[threadIdx.x] = rec[3 * s + 2];
+    }
+    __syncthreads();
+    const uint32_t cnt = min(kThreads, rg.y - start);
+    if (!done) {
+      for (uint32_t j = 0; j < cnt; ++j) {
+        const float4 a = s0[j];
+        const float4 b = s1[j];
+        const float dx = pnx - a.x;
+        const float dy = pny - a.y;
+        const float lx = fmaf(a.w, dy, a.z * dx);
+        const float ly = fmaf(b.y, dy, b.x * dx);
+        const float r2 = fmaf(ly, ly, lx * lx);
+        if (r2 <= b.z) {
+          const float g = __expf(-4.5f * r2);
+          const float4 c = s2[j];
+          const float wgt = T * g;
+          cr = fmaf(wgt, c.x, cr);
+          cg = fmaf(wgt, c.y, cg);
+          cb = fmaf(wgt, c.z, cb);
+          T = T * (1.0f - b.w * g);
+          if (T < 1e-4f) {
+            done = true;
+            break;
+          }
+        }
+      }
+    }
+  }
+  if (inside) {
+    const size_t o = (size_t)y * width + x;
+    const float A = 1.0f - T;
+    out32[o] = make_float4(cr, cg, cb, A);
+    auto q = [](float v) { return (unsigned char)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f); };
+    out8[o] = make_uchar4(q(cr), q(cg), q(cb), q(A));
+  }
+}
+
+uint32_t div_up(size_t a, size_t b) { return (uint32_t)((a + b - 1) / b); }
+
+template <bool EXPORT>
+void project_dispatch(uint32_t deg, const float *soa, uint32_t n, const FrameParams &fp, uint32_t *keys,
+                      uint32_t *vals, float4 *rec, uint2 *rect, uint32_t *visible, float *export12,
+                      hipStream_t st) {
+  const dim3 grid(div_up(n, kThreads)), block(kThreads);
+  switch (deg) {
+    case 0: k_project<0, EXPORT><<<grid, block, 0, st>>>(soa, n, fp, keys, vals, rec, rect, visible, export12); break;
+    case 1: k_project<1, EXPORT><<<grid, block, 0, st>>>(soa, n, fp, keys, vals, rec, rect, visible, export12); break;
+    case 2: k_project<2, EXPORT><<<grid, block, 0, st>>>(soa, n, fp, keys, vals, rec, rect, visible, export12); break;
+    case 3: k_project<3, EXPORT><<<grid, block, 0, st>>>(soa, n, fp, keys, vals, rec, rect, visible, export12); break;
+    case 4: k_project<4, EXPORT><<<grid, block, 0, st>>>(soa, n, fp, keys, vals, rec, rect, visible, export12); break;
+    default: throw Error(Code::Unsupported, "sh degree above 4");
+  }
+  GS_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+FrameParams make_frame_params(const float position[3], const float rotation_xyzw[4], float fov, float near_plane,
+                              float far_plane, uint32_t width, uint32_t height) {
+  FrameParams p{};
+  for (int a = 0; a < 3; ++a) p.cam[a] = position[a];
+  view_rows(rotation_xyzw, p.rows);
+  p.near_plane = near_plane;
+  p.far_plane = far_plane;
+  // make_surface_render_params (lib.rs:889-920) and project_splat's
+  // constants (wgsl:143-185); pow(x, 2.0) evaluated as x*x.
+  p.aspect = std::max((float)width / (float)std::max(height, 1u), 1e-6f);
+  const float tan_half = std::tan(fov * 0.5f);
+  p.f = 1.0f / tan_half;
+  p.fx = p.f / p.aspect;
+  p.fy = p.f;
+  p.lim_x = 1.3f * (tan_half * p.aspect);
+  p.lim_y = 1.3f * tan_half;
+  const float bx = 0.3f * (2.0f / std::max((float)width, 1.0f));
+  const float by = 0.3f * (2.0f / std::max((float)height, 1.0f));
+  p.blur_x = bx * bx;
+  p.blur_y = by * by;
+  p.width = width;
+  p.height = height;
+  p.tiles_x = (width + kTile - 1) / kTile;
+  p.tiles_y = (height + kTile - 1) / kTile;
+  return p;
+}
+
+void launch_project(uint32_t deg, const float *soa, uint32_t n, const FrameParams &fp, uint32_t *keys,
+                    uint32_t *vals, float4 *rec, uint2 *rect, uint32_t *visible, float *export12,
+                    hipStream_t st) {
+  if (n == 0) return;
+  if (export12)
+    project_dispatch<true>(deg, soa, n, fp, keys, vals, rec, rect, visible, export12, st);
+  else
+    project_dispatch<false>(deg, soa, n, fp, keys, vals, rec, rect, visible, nullptr, st);
+}
+
+size_t radix_hist_entries(uint32_t n) { return (size_t)256 * std::max<uint32_t>(div_up(n, kRadixTile), 1u); }
+size_t scan_partial_entries(size_t n) { return std::max<size_t>(div_up(n, kScanTile), 1); }
+
+void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, SortScratch &s, hipStream_t st) {
+  if (n == 0) {
+    launch_fill_u32(reinterpret_cast<uint32_t *>(s.total), 0u, 2, st);
+    return;
+  }
+  const uint32_t blocks = div_up(n, kScanTile);
+  if (blocks > s.partial_cap) throw Error(Code::Internal, "scan scratch too small");
+  k_scan_blocks<<<blocks, kThreads, 0, st>>>(in, out, n, s.partials);
+  k_scan_partials<<<1, kThreads, 0, st>>>(s.partials, blocks, s.total);
+  if (blocks > 1) k_scan_add<<<blocks, kThreads, 0, st>>>(out, n, s.partials);
+  GS_HIP(hipGetLastError());
+}
+
+void radix_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n,
+                uint32_t bits, bool descending, SortScratch &s, hipStream_t st) {
+  if (n <= 1 || bits == 0) return;
+  const uint32_t blocks = div_up(n, kRadixTile);
+  const size_t entries = (size_t)256 * blocks;
+  if (entries > s.hist_cap) throw Error(Code::Internal, "radix scratch too small");
+  const uint32_t flip = descending ? 0xFFFFFFFFu : 0u;
+  for (uint32_t shift = 0; shift < bits; shift += 8) {
+    k_radix_upsweep<<<blocks, kThreads, 0, st>>>(*keys, n, shift, flip, s.hist, blocks);
+    exclusive_scan(s.hist, s.hist, entries, s, st);
+    k_radix_scatter<<<blocks, kThreads, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n, shift, flip, s.hist,
+                                                  blocks);
+    GS_HIP(hipGetLastError());
+    std::swap(*keys, *keys_alt);
+    std::swap(*vals, *vals_alt);
+  }
+}
+
+void launch_tile_counts(const uint32_t *sorted, const uint32_t *visible, const uint2 *rect, uint32_t n,
+                        uint32_t *counts, hipStream_t st) {
+  if (n == 0) return;
+  k_tile_counts<<<div_up(n, kThreads), kThreads, 0, st>>>(sorted, visible, rect, n, counts);
+  GS_HIP(hipGetLastError());
+}
+
+void launch_tile_emit(const uint32_t *sorted, const uint32_t *visible, const uint2 *rect, const uint32_t *offsets,
+                      uint32_t n, uint32_t tiles_x, uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t st) {
+  if (n == 0) return;
+  k_tile_emit<<<div_up(n, kThreads), kThreads, 0, st>>>(sorted, visible, rect, offsets, n, tiles_x, pair_keys,
+                                                         pair_vals);
+  GS_HIP(hipGetLastError());
+}
+
+void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t st) {
+  if (n == 0) return;
+  k_fill_u32<<<div_up(n, kThreads), kThreads, 0, st>>>(p, value, n);
+  GS_HIP(hipGetLastError());
+}
+
+void launch_tile_ranges(const uint32_t *pair_keys, uint32_t m, uint32_t tiles, uint2 *ranges, hipStream_t st) {
+  launch_fill_u32(reinterpret_cast<uint32_t *>(ranges), 0u, 2 * (size_t)tiles, st);
+  if (m == 0) return;
+  k_tile_ranges<<<div_up(m, kThreads), kThreads, 0, st>>>(pair_keys, m, ranges);
+  GS_HIP(hipGetLastError());
+}
+
+void launch_raster(const float4 *rec, const uint2 *ranges, const uint32_t *list, uint32_t width, uint32_t height,
+                   uint32_t tiles_x, uint32_t tiles_y, float4 *out32, uchar4 *out8, hipStream_t st) {
+  const uint32_t tiles = tiles_x * tiles_y;
+  if (tiles == 0) return;
+  k_raster<<<tiles, kThreads, 0, st>>>(rec, ranges, list, width, height, tiles_x, out32, out8);
+  GS_HIP(hipGetLastError());
+}
+
+}  // namespace gs

@@ -1,0 +1,159 @@
+/*
+ * gsplat_ext.h -- additive entry points of the MI355X-native renderer.
+ *
+ * The reference v0.1 C ABI (gsplat.h) has no readback, no in-memory scene
+ * load, no per-stage timing and no batching; the reference lists in-memory
+ * scene loading as outside the v0.1 contract (README.md:167-170). Everything
+ * here is additive: no struct of gsplat.h changes layout and no existing
+ * symbol changes behaviour. Each declaration names the reference function
+ * whose behaviour it exposes.
+ *
+ * Conventions are those of gsplat.h: int32_t returns are GsplatErrorCode,
+ * gsplat_last_error_message() carries the detail, NULL handles or outputs
+ * return GSPLAT_ERROR_INVALID_ARGUMENT.
+ */
+#ifndef GSPLAT_FFI_C_GSPLAT_EXT_H
+#define GSPLAT_FFI_C_GSPLAT_EXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gsplat.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- scenes */
+
+/*
+ * Borrowed view of scene buffers in host memory, field for field the
+ * reference's SceneBuffers (crates/gsplat-core/src/lib.rs:201-213):
+ *   positions      count*3 floats, xyz
+ *   opacity        count logits (alpha = sigmoid(logit))
+ *   scale_xyz      count*3 log-scales
+ *   rotation_xyzw  count*4 quaternions (need not be unit)
+ *   color_dc       count*3 DC SH coefficients
+ *   sh_rest        count*3*((sh_degree+1)^2-1) floats, per splat channel-major
+ *                  [R c1..cK, G c1..cK, B c1..cK]; NULL iff sh_degree == 0
+ * Validation follows SceneBuffers::validate (core/src/lib.rs:224-276):
+ * non-finite values -> PARSE_FAILED, sh_degree > 4 -> UNSUPPORTED.
+ */
+typedef struct GsplatSceneView {
+  uint32_t count;
+  uint32_t sh_degree;
+  const float *positions;
+  const float *opacity;
+  const float *scale_xyz;
+  const float *rotation_xyzw;
+  const float *color_dc;
+  const float *sh_rest;
+} GsplatSceneView;
+
+/* Replaces Renderer::load_scene (render-wgpu/src/lib.rs:488-497) for a scene
+ * already in memory; the PLY path is gsplat_context_load_scene_path. */
+int32_t gsplat_context_load_scene_memory(GsplatContext *ctx, const GsplatSceneView *scene);
+
+/* Renderer::set_size (render-wgpu/src/lib.rs:339-357): 0 -> INVALID_ARGUMENT,
+ * above the 2D target limit (16384) -> UNSUPPORTED. */
+int32_t gsplat_context_set_size(GsplatContext *ctx, uint32_t width, uint32_t height);
+
+/* Current camera (after gsplat_context_set_auto_camera, for instance). */
+int32_t gsplat_context_get_camera(const GsplatContext *ctx, GsplatCamera *out_camera);
+
+/* Renderer::wait_for_gpu (render-wgpu/src/lib.rs:468-486). */
+int32_t gsplat_context_wait(GsplatContext *ctx);
+
+/* Renderer::readback_rgba8 (render-wgpu/src/lib.rs:779-795, 2899-2977):
+ * width*height*4 bytes, rows top first, premultiplied RGBA. `len` must be at
+ * least that size. Waits for the frame. */
+int32_t gsplat_context_readback_rgba8(GsplatContext *ctx, uint8_t *out, uint64_t len);
+
+/* Same image before 8-bit quantisation: width*height*4 floats. */
+int32_t gsplat_context_readback_rgba32f(GsplatContext *ctx, float *out, uint64_t len);
+
+/* Device time per stage of the last frame (hipEvent pairs on the context's
+ * stream) and the sizes that drive each stage. */
+typedef struct GsplatStageTimes {
+  float project_ms;  /* depth key + cull + projection + SH colour      */
+  float sort_ms;     /* stable descending depth sort                   */
+  float bin_ms;      /* tile counts, scan, pair emission, tile sort    */
+  float raster_ms;   /* LDS-staged front-to-back tile blend            */
+  float total_ms;    /* first event to last event                      */
+  uint32_t splat_count;    /* N                                        */
+  uint32_t visible_count;  /* V: near <= depth <= far                  */
+  uint32_t tiles_x;
+  uint32_t tiles_y;
+  uint64_t tile_pairs;     /* M: (tile, splat) pairs                   */
+} GsplatStageTimes;
+int32_t gsplat_context_get_stage_times(GsplatContext *ctx, GsplatStageTimes *out);
+
+/* ------------------------------------------------ parity/inspection hooks */
+
+/* Depth keys of the last frame for every splat, in splat order:
+ * f32 bits of max(depth,0) when near <= depth <= far, else 0
+ * (preprocess_visible_into, render-wgpu/src/lib.rs:1263-1296, and the GPU
+ * statement direct_gpu_order.wgsl:78-98). `cap` >= splat count. */
+int32_t gsplat_context_read_depth_keys(GsplatContext *ctx, uint32_t *out, uint64_t cap);
+
+/* Back-to-front draw order of the last frame: the V visible splat indices
+ * sorted by depth key descending, ties by ascending index
+ * (CpuSortBackend::sort_values_by_keys, sort/src/lib.rs:36-61). */
+int32_t gsplat_context_read_sorted_indices(GsplatContext *ctx, uint32_t *out, uint64_t cap,
+                                           uint32_t *out_count);
+
+/* Per-splat projection of the current camera, 12 floats per splat:
+ * [drawn, center.x, center.y, axis_u.x, axis_u.y, axis_v.x, axis_v.y,
+ *  r*a, g*a, b*a, a, 0] in NDC -- the quantities of project_splat and
+ * eval_color (splat_surface_direct.wgsl:131-217, 328-334); drawn = 0 for
+ * splats the shader turns into invalid_splat(). cap >= 12*N floats. */
+int32_t gsplat_context_read_projection(GsplatContext *ctx, float *out, uint64_t cap);
+
+/* Number of (tile, splat) pairs per 16x16 tile of the last frame, row-major
+ * tiles; cap >= tiles_x*tiles_y. */
+int32_t gsplat_context_read_tile_counts(GsplatContext *ctx, uint32_t *out, uint64_t cap,
+                                        uint32_t *out_tiles_x, uint32_t *out_tiles_y);
+
+/* --------------------------------------------------------- standalone sort */
+
+/* GPU statement of CpuSortBackend::sort_values_by_keys
+ * (sort/src/lib.rs:36-61): stable sort of `values` by `keys` descending;
+ * equal keys keep their input order. Host buffers; runs on the device. */
+int32_t gsplat_sort_values_by_keys(const uint32_t *keys, uint32_t *values, uint64_t len);
+
+/* ------------------------------------------------------------- PLY loader */
+
+/* Resource budgets of gsplat_io_ply::PlyLoadLimits (io-ply/src/lib.rs:28-47);
+ * defaults 1 GiB input, 1 MiB header, 5,000,000 vertices, 128 properties,
+ * 1 GiB decoded scene. */
+typedef struct GsplatPlyLimits {
+  uint64_t max_input_bytes;
+  uint64_t max_header_bytes;
+  uint64_t max_vertices;
+  uint64_t max_vertex_properties;
+  uint64_t max_scene_bytes;
+} GsplatPlyLimits;
+
+typedef struct GsplatPlyScene GsplatPlyScene;
+
+GsplatPlyLimits gsplat_ply_limits_default(void);
+/* load_ply / parse_ply_bytes_with_limits (io-ply/src/lib.rs:178-294); error
+ * codes as PlyLoadError::code (:161-176), detail "<fn>: <PlyLoadError text>". */
+int32_t gsplat_ply_load(const char *path, GsplatPlyScene **out_scene);
+int32_t gsplat_ply_parse_bytes(const uint8_t *bytes, uint64_t len, const GsplatPlyLimits *limits,
+                               GsplatPlyScene **out_scene);
+/* parse_ply_text_with_limits: additionally rejects non-ASCII formats. */
+int32_t gsplat_ply_parse_text(const uint8_t *bytes, uint64_t len, const GsplatPlyLimits *limits,
+                              GsplatPlyScene **out_scene);
+/* Borrowed view valid until gsplat_ply_free. */
+int32_t gsplat_ply_scene_view(const GsplatPlyScene *scene, GsplatSceneView *out_view);
+void gsplat_ply_free(GsplatPlyScene *scene);
+
+/* 1 when a gfx950 device is visible, else 0. */
+uint32_t gsplat_device_available(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSPLAT_FFI_C_GSPLAT_EXT_H */
