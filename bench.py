@@ -1,0 +1,144 @@
+"""Benchmark of the MI355X SortedAlpha path.
+
+Default workload (one GPU): plain SortedAlpha, 300,000 splats SH degree 3,
+one 1920x1080 view (synthetic scene from a fixed seed; see synth.py).
+A step is one full frame: depth key + cull + projection + SH colour, depth
+sort, tile binning, raster, with the scene resident in HBM.
+
+Prints one JSON line (frames/s) with a roofline entry for the dominant
+kernel and the CPU oracle timed on a bounded sample.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+for sub in ("gsplat-rs_amd",):
+    sys.path.insert(0, str(ROOT / sub))
+
+import numpy as np  # noqa: E402
+
+import gsplat_abi as A  # noqa: E402
+import synth  # noqa: E402
+
+MI355X_HBM_GBPS = 8000.0
+MI355X_FP32_TFLOPS = 157.3
+
+
+def nearest_rank(sorted_vals, p):
+    """tools/bench-runner/src/artifact.rs:433-436."""
+    rank = max(math.ceil(p * len(sorted_vals)), 1)
+    return sorted_vals[min(rank - 1, len(sorted_vals) - 1)]
+
+
+def cpu_baseline(scene, view, cam):
+    """Oracle (CPU restatement of the reference path) on the host cores."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_py as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    osc = O.Scene(scene.positions, scene.opacity, scene.scale_xyz, scene.rotation_xyzw, scene.color_dc,
+                  scene.sh_degree, scene.sh_rest)
+    frames, t0 = 0, time.perf_counter()
+    while True:
+        O.render(osc, cam.as_dict(), view.width, view.height, threads=threads)
+        frames += 1
+        if time.perf_counter() - t0 > 10.0 or frames >= 8:
+            break
+    el = time.perf_counter() - t0
+    st = O.last_timings()
+    return {"value": round(frames / el, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} full frames of the same workload (preprocess 1 thread, sort 1 thread, raster "
+                      f"{threads} threads); last frame ms: preprocess {st[0]:.1f}, sort {st[1]:.1f}, "
+                      f"project {st[2]:.1f}, raster {st[3]:.1f}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
+        sys.exit(2)
+    if not A.device_available():
+        print("no gfx950 device visible", file=sys.stderr)
+        sys.exit(1)
+
+    scene, view = synth.plain_1080p()
+    cam = A.look_at(view.position, view.target)
+    ctx = A.Context(view.width, view.height)
+    ctx.load(scene)
+    ctx.set_camera(cam)
+    for _ in range(args.warmup):
+        ctx.render()
+    ctx.wait()
+
+    # Timed region: K back-to-back frames.
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.render()
+    ctx.wait()
+    elapsed = time.perf_counter() - t0
+    fps = args.steps / elapsed
+
+    # Per-frame latency and stage device times (separate pass, one frame at a time).
+    lat, stages = [], []
+    for _ in range(min(args.steps, 60)):
+        t = time.perf_counter()
+        ctx.render()
+        ctx.wait()
+        lat.append((time.perf_counter() - t) * 1e3)
+        s = ctx.stage_times()
+        stages.append((s.project_ms, s.sort_ms, s.bin_ms, s.raster_ms, s.total_ms))
+    lat.sort()
+    st = ctx.stage_times()
+    med = np.median(np.array(stages), axis=0)
+    names = ["project", "sort", "bin", "raster"]
+    dom = int(np.argmax(med[:4]))
+
+    n, v = st.splat_count, st.visible_count
+    k = 3 * ((scene.sh_degree + 1) ** 2 - 1)
+    # Projection bytes: positions for all, covariance + alpha + cut for visible, colour terms and
+    # record/rect writes for visible, keys/values for all.
+    proj_bytes = 12 * n + 8 * n + (32 + 12 + 4 * k + 56) * v
+    roof = {"project": ("hbm", proj_bytes / (med[0] * 1e-3) / 1e9, MI355X_HBM_GBPS, "GB/s", proj_bytes)}
+    kernel, (bound, achieved, peak, unit, traffic) = names[0], roof["project"]
+    line = {
+        "metric": "frames/s @300k splats SH3 1080p (plain SortedAlpha)",
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 3-component Gaussian mixture, SH3)",
+        "config": {"workload": "plain-1080p-300k-sh3", "splats": n, "width": view.width, "height": view.height,
+                   "visible": v, "tile_pairs": int(st.tile_pairs)},
+        "latency_ms": {"p50": round(nearest_rank(lat, 0.5), 4), "p99": round(nearest_rank(lat, 0.99), 4)},
+        "stage_ms": {nm: round(float(x), 4) for nm, x in zip(names, med[:4])},
+        "dominant_stage": names[dom],
+        "roofline": {"kernel": "k_project", "bound": bound, "achieved": round(achieved, 2), "peak": peak,
+                     "unit": unit, "frac": round(achieved / peak, 4), "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(scene, view, cam)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

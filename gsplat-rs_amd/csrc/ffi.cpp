@@ -534,6 +534,18 @@ int32_t gsplat_ply_scene_view(const GsplatPlyScene *scene, GsplatSceneView *out_
 
 void gsplat_ply_free(GsplatPlyScene *scene) { delete scene; }
 
+GsplatCamera gsplat_camera_look_at(const float position[3], const float target[3], float fov, float near_plane,
+                                   float far_plane) {
+  GsplatCamera c = gs::camera_default();
+  if (!position || !target) return c;
+  for (int a = 0; a < 3; ++a) c.position[a] = position[a];
+  gs::rotation_looking_at(position, target, c.rotation_xyzw);
+  c.vertical_fov_radians = fov;
+  c.near_plane = near_plane;
+  c.far_plane = far_plane;
+  return c;
+}
+
 uint32_t gsplat_device_available(void) {
   try {
     return gs::device_present() ? 1u : 0u;

@@ -1,0 +1,404 @@
+"""ctypes binding of libgsplat_ffi_c.so (include/gsplat.h + include/gsplat_ext.h).
+
+This is the binding a Python caller of the C ABI writes; tests and bench.py
+use it. Loading fails loudly when the shared library has not been built:
+there is no fallback implementation.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libgsplat_ffi_c.so"
+
+OK = 0
+INVALID_ARGUMENT = 1
+NOT_FOUND = 2
+PARSE_FAILED = 3
+UNSUPPORTED = 4
+SCENE_NOT_LOADED = 5
+INTERNAL = 100
+
+
+class GsplatConfig(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("mode", ctypes.c_uint32)]
+
+
+class GsplatStats(ctypes.Structure):
+    _fields_ = [
+        ("frame_ms", ctypes.c_float),
+        ("preprocess_ms", ctypes.c_float),
+        ("sort_ms", ctypes.c_float),
+        ("raster_ms", ctypes.c_float),
+        ("visible_count", ctypes.c_uint32),
+        ("drawn_count", ctypes.c_uint32),
+    ]
+
+
+class GsplatSurfaceSortStats(ctypes.Structure):
+    _fields_ = [
+        ("camera_revision", ctypes.c_uint64),
+        ("applied_order_revision", ctypes.c_uint64),
+        ("scheduled_revision", ctypes.c_uint64),
+        ("completed_revision", ctypes.c_uint64),
+        ("presented_order_revision_lag", ctypes.c_uint32),
+        ("observed_result_revision_lag", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+class GsplatCamera(ctypes.Structure):
+    _fields_ = [
+        ("position", ctypes.c_float * 3),
+        ("rotation_xyzw", ctypes.c_float * 4),
+        ("vertical_fov_radians", ctypes.c_float),
+        ("near_plane", ctypes.c_float),
+        ("far_plane", ctypes.c_float),
+    ]
+
+    def as_dict(self) -> dict:
+        return dict(
+            position=list(self.position),
+            rotation_xyzw=list(self.rotation_xyzw),
+            fov=self.vertical_fov_radians,
+            near=self.near_plane,
+            far=self.far_plane,
+        )
+
+
+class GsplatSceneView(ctypes.Structure):
+    _fields_ = [
+        ("count", ctypes.c_uint32),
+        ("sh_degree", ctypes.c_uint32),
+        ("positions", ctypes.c_void_p),
+        ("opacity", ctypes.c_void_p),
+        ("scale_xyz", ctypes.c_void_p),
+        ("rotation_xyzw", ctypes.c_void_p),
+        ("color_dc", ctypes.c_void_p),
+        ("sh_rest", ctypes.c_void_p),
+    ]
+
+
+class GsplatStageTimes(ctypes.Structure):
+    _fields_ = [
+        ("project_ms", ctypes.c_float),
+        ("sort_ms", ctypes.c_float),
+        ("bin_ms", ctypes.c_float),
+        ("raster_ms", ctypes.c_float),
+        ("total_ms", ctypes.c_float),
+        ("splat_count", ctypes.c_uint32),
+        ("visible_count", ctypes.c_uint32),
+        ("tiles_x", ctypes.c_uint32),
+        ("tiles_y", ctypes.c_uint32),
+        ("tile_pairs", ctypes.c_uint64),
+    ]
+
+
+class GsplatPlyLimits(ctypes.Structure):
+    _fields_ = [
+        ("max_input_bytes", ctypes.c_uint64),
+        ("max_header_bytes", ctypes.c_uint64),
+        ("max_vertices", ctypes.c_uint64),
+        ("max_vertex_properties", ctypes.c_uint64),
+        ("max_scene_bytes", ctypes.c_uint64),
+    ]
+
+
+P = ctypes.c_void_p
+U32 = ctypes.c_uint32
+U64 = ctypes.c_uint64
+I32 = ctypes.c_int32
+F32 = ctypes.c_float
+PU32 = ctypes.POINTER(ctypes.c_uint32)
+
+# name -> (restype, argtypes); every symbol the headers declare.
+SIGNATURES = {
+    # gsplat.h
+    "gsplat_version_major": (U32, []),
+    "gsplat_version_minor": (U32, []),
+    "gsplat_error_message": (ctypes.c_char_p, [I32]),
+    "gsplat_last_error_message": (ctypes.c_char_p, []),
+    "gsplat_config_default": (GsplatConfig, []),
+    "gsplat_camera_default": (GsplatCamera, []),
+    "gsplat_context_create": (I32, [GsplatConfig, ctypes.POINTER(P)]),
+    "gsplat_context_destroy": (None, [P]),
+    "gsplat_context_set_camera": (I32, [P, GsplatCamera]),
+    "gsplat_context_set_auto_camera": (I32, [P]),
+    "gsplat_context_load_scene_path": (I32, [P, ctypes.c_char_p]),
+    "gsplat_context_render_frame": (I32, [P]),
+    "gsplat_context_get_stats": (I32, [P, ctypes.POINTER(GsplatStats)]),
+    "gsplat_surface_renderer_create_android": (I32, [P, ctypes.c_char_p, U32, U32, ctypes.POINTER(P)]),
+    "gsplat_surface_renderer_create_android_with_geometry_path": (
+        I32, [P, ctypes.c_char_p, U32, U32, U32, ctypes.POINTER(P)]),
+    "gsplat_surface_renderer_create_uikit": (I32, [P, P, ctypes.c_char_p, U32, U32, ctypes.POINTER(P)]),
+    "gsplat_surface_renderer_create_uikit_with_geometry_path": (
+        I32, [P, P, ctypes.c_char_p, U32, U32, U32, ctypes.POINTER(P)]),
+    "gsplat_surface_renderer_destroy": (None, [P]),
+    "gsplat_surface_renderer_resize": (I32, [P, U32, U32]),
+    "gsplat_surface_renderer_set_sort_interval": (I32, [P, U32]),
+    "gsplat_surface_renderer_set_geometry_path": (I32, [P, U32]),
+    "gsplat_surface_renderer_set_gpu_preproject": (I32, [P, U32]),
+    "gsplat_surface_renderer_set_gpu_preproject_double_buffer": (I32, [P, U32]),
+    "gsplat_surface_renderer_set_async_sort": (I32, [P, U32]),
+    "gsplat_surface_renderer_set_async_geometry": (I32, [P, U32]),
+    "gsplat_surface_renderer_set_instance_buffer_count": (I32, [P, U32]),
+    "gsplat_surface_renderer_set_frame_latency": (I32, [P, U32]),
+    "gsplat_surface_renderer_reset_camera": (I32, [P]),
+    "gsplat_surface_renderer_orbit": (I32, [P, F32, F32]),
+    "gsplat_surface_renderer_zoom": (I32, [P, F32]),
+    "gsplat_surface_renderer_pan": (I32, [P, F32, F32]),
+    "gsplat_surface_renderer_render_frame": (I32, [P]),
+    "gsplat_surface_renderer_get_stats": (I32, [P, ctypes.POINTER(GsplatStats)]),
+    "gsplat_surface_renderer_get_sort_stats": (I32, [P, ctypes.POINTER(GsplatSurfaceSortStats)]),
+    # gsplat_ext.h
+    "gsplat_context_load_scene_memory": (I32, [P, ctypes.POINTER(GsplatSceneView)]),
+    "gsplat_context_set_size": (I32, [P, U32, U32]),
+    "gsplat_context_get_camera": (I32, [P, ctypes.POINTER(GsplatCamera)]),
+    "gsplat_context_wait": (I32, [P]),
+    "gsplat_context_readback_rgba8": (I32, [P, P, U64]),
+    "gsplat_context_readback_rgba32f": (I32, [P, P, U64]),
+    "gsplat_context_get_stage_times": (I32, [P, ctypes.POINTER(GsplatStageTimes)]),
+    "gsplat_context_read_depth_keys": (I32, [P, P, U64]),
+    "gsplat_context_read_sorted_indices": (I32, [P, P, U64, PU32]),
+    "gsplat_context_read_projection": (I32, [P, P, U64]),
+    "gsplat_context_read_tile_counts": (I32, [P, P, U64, PU32, PU32]),
+    "gsplat_sort_values_by_keys": (I32, [P, P, U64]),
+    "gsplat_ply_limits_default": (GsplatPlyLimits, []),
+    "gsplat_ply_load": (I32, [ctypes.c_char_p, ctypes.POINTER(P)]),
+    "gsplat_ply_parse_bytes": (I32, [P, U64, ctypes.POINTER(GsplatPlyLimits), ctypes.POINTER(P)]),
+    "gsplat_ply_parse_text": (I32, [P, U64, ctypes.POINTER(GsplatPlyLimits), ctypes.POINTER(P)]),
+    "gsplat_ply_scene_view": (I32, [P, ctypes.POINTER(GsplatSceneView)]),
+    "gsplat_ply_free": (None, [P]),
+    "gsplat_device_available": (U32, []),
+    "gsplat_camera_look_at": (GsplatCamera, [ctypes.POINTER(F32), ctypes.POINTER(F32), F32, F32, F32]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `make -C gsplat-rs_amd` (or __graft_entry__.build())")
+        l = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+class GsplatError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        raise GsplatError(rc, last_error())
+
+
+def last_error() -> str:
+    return lib().gsplat_last_error_message().decode()
+
+
+def device_available() -> bool:
+    return bool(lib().gsplat_device_available())
+
+
+def camera(position, rotation_xyzw=(0.0, 0.0, 0.0, 1.0), fov=None, near=0.01, far=1000.0) -> GsplatCamera:
+    c = lib().gsplat_camera_default()
+    c.position[:] = [float(v) for v in position]
+    c.rotation_xyzw[:] = [float(v) for v in rotation_xyzw]
+    if fov is not None:
+        c.vertical_fov_radians = float(fov)
+    c.near_plane = float(near)
+    c.far_plane = float(far)
+    return c
+
+
+def look_at(position, target=(0.0, 0.0, 0.0), fov=None, near=0.01, far=1000.0) -> GsplatCamera:
+    p = (F32 * 3)(*[float(v) for v in position])
+    t = (F32 * 3)(*[float(v) for v in target])
+    if fov is None:
+        fov = lib().gsplat_camera_default().vertical_fov_radians
+    return lib().gsplat_camera_look_at(p, t, F32(fov), F32(near), F32(far))
+
+
+def scene_view(scene) -> GsplatSceneView:
+    """`scene` has positions/opacity/scale_xyz/rotation_xyzw/color_dc/sh_rest numpy float32 arrays."""
+    v = GsplatSceneView()
+    v.count = int(scene.count)
+    v.sh_degree = int(scene.sh_degree)
+    v.positions = scene.positions.ctypes.data
+    v.opacity = scene.opacity.ctypes.data
+    v.scale_xyz = scene.scale_xyz.ctypes.data
+    v.rotation_xyzw = scene.rotation_xyzw.ctypes.data
+    v.color_dc = scene.color_dc.ctypes.data
+    v.sh_rest = None if scene.sh_rest is None else scene.sh_rest.ctypes.data
+    return v
+
+
+class Context:
+    """Owns one GsplatContext (one renderer on the GPU)."""
+
+    def __init__(self, width: int = 1280, height: int = 720):
+        cfg = lib().gsplat_config_default()
+        cfg.width, cfg.height = width, height
+        h = P()
+        check(lib().gsplat_context_create(cfg, ctypes.byref(h)))
+        self.h = h
+        self.width, self.height = width, height
+        self.count = 0
+
+    def close(self):
+        if self.h:
+            lib().gsplat_context_destroy(self.h)
+            self.h = P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, scene):
+        v = scene_view(scene)
+        check(lib().gsplat_context_load_scene_memory(self.h, ctypes.byref(v)))
+        self.count = int(scene.count)
+
+    def load_path(self, path: str):
+        check(lib().gsplat_context_load_scene_path(self.h, str(path).encode()))
+
+    def set_camera(self, cam: GsplatCamera):
+        check(lib().gsplat_context_set_camera(self.h, cam))
+
+    def set_size(self, width: int, height: int):
+        check(lib().gsplat_context_set_size(self.h, width, height))
+        self.width, self.height = width, height
+
+    def get_camera(self) -> GsplatCamera:
+        c = GsplatCamera()
+        check(lib().gsplat_context_get_camera(self.h, ctypes.byref(c)))
+        return c
+
+    def render(self):
+        check(lib().gsplat_context_render_frame(self.h))
+
+    def wait(self):
+        check(lib().gsplat_context_wait(self.h))
+
+    def stats(self) -> GsplatStats:
+        s = GsplatStats()
+        check(lib().gsplat_context_get_stats(self.h, ctypes.byref(s)))
+        return s
+
+    def stage_times(self) -> GsplatStageTimes:
+        s = GsplatStageTimes()
+        check(lib().gsplat_context_get_stage_times(self.h, ctypes.byref(s)))
+        return s
+
+    def rgba8(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        check(lib().gsplat_context_readback_rgba8(self.h, out.ctypes.data, out.size))
+        return out
+
+    def rgba32f(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        check(lib().gsplat_context_readback_rgba32f(self.h, out.ctypes.data, out.size))
+        return out
+
+    def depth_keys(self, count: int) -> np.ndarray:
+        out = np.zeros(max(count, 1), np.uint32)
+        check(lib().gsplat_context_read_depth_keys(self.h, out.ctypes.data, out.size))
+        return out[:count]
+
+    def sorted_indices(self, count: int) -> np.ndarray:
+        out = np.zeros(max(count, 1), np.uint32)
+        n = ctypes.c_uint32()
+        check(lib().gsplat_context_read_sorted_indices(self.h, out.ctypes.data, out.size, ctypes.byref(n)))
+        return out[: n.value].copy()
+
+    def projection(self, count: int) -> np.ndarray:
+        out = np.zeros((max(count, 1), 12), np.float32)
+        check(lib().gsplat_context_read_projection(self.h, out.ctypes.data, out.size))
+        return out[:count]
+
+    def tile_counts(self) -> np.ndarray:
+        tx, ty = (self.width + 15) // 16, (self.height + 15) // 16
+        out = np.zeros(tx * ty, np.uint32)
+        ox, oy = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().gsplat_context_read_tile_counts(self.h, out.ctypes.data, out.size, ctypes.byref(ox),
+                                                    ctypes.byref(oy)))
+        return out.reshape(oy.value, ox.value)
+
+
+def sort_values_by_keys(keys, values) -> np.ndarray:
+    k = np.ascontiguousarray(np.asarray(keys, dtype=np.uint32))
+    v = np.ascontiguousarray(np.asarray(values, dtype=np.uint32)).copy()
+    if k.shape != v.shape:
+        raise ValueError("key/value length mismatch")
+    check(lib().gsplat_sort_values_by_keys(k.ctypes.data if k.size else None, v.ctypes.data if v.size else None,
+                                           k.size))
+    return v
+
+
+class PlyScene:
+    """A parsed PLY scene owned by the library (gsplat_ply_*)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        v = GsplatSceneView()
+        check(lib().gsplat_ply_scene_view(self.h, ctypes.byref(v)))
+        n = v.count
+        self.count = n
+        self.sh_degree = v.sh_degree
+
+        def arr(ptr, k):
+            if not ptr or n == 0:
+                return np.zeros((n, k), np.float32)
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_float)), shape=(n * k,)).reshape(
+                n, k).copy()
+
+        self.positions = arr(v.positions, 3)
+        self.opacity = arr(v.opacity, 1).reshape(n)
+        self.scale_xyz = arr(v.scale_xyz, 3)
+        self.rotation_xyzw = arr(v.rotation_xyzw, 4)
+        self.color_dc = arr(v.color_dc, 3)
+        k = 3 * ((v.sh_degree + 1) ** 2 - 1)
+        self.sh_rest = arr(v.sh_rest, k) if v.sh_rest else None
+        lib().gsplat_ply_free(self.h)
+        self.h = None
+
+
+def ply_load(path) -> PlyScene:
+    h = P()
+    check(lib().gsplat_ply_load(str(path).encode(), ctypes.byref(h)))
+    return PlyScene(h)
+
+
+def ply_parse(data: bytes, limits: GsplatPlyLimits | None = None, text: bool = False) -> PlyScene:
+    h = P()
+    buf = ctypes.create_string_buffer(data, len(data)) if data else None
+    fn = lib().gsplat_ply_parse_text if text else lib().gsplat_ply_parse_bytes
+    check(fn(ctypes.cast(buf, P) if buf is not None else None, len(data),
+             ctypes.byref(limits) if limits is not None else None, ctypes.byref(h)))
+    return PlyScene(h)
+
+
+def ply_limits_default() -> GsplatPlyLimits:
+    return lib().gsplat_ply_limits_default()

@@ -152,6 +152,12 @@ void gsplat_ply_free(GsplatPlyScene *scene);
 /* 1 when a gfx950 device is visible, else 0. */
 uint32_t gsplat_device_available(void);
 
+/* Camera at `position` looking at `target` with the reference's look-at
+ * basis and quaternion (camera_rotation_looking_at / quat_from_camera_basis,
+ * ffi-c/src/lib.rs:1677-1739); fov/near/far are taken as given. */
+GsplatCamera gsplat_camera_look_at(const float position[3], const float target[3],
+                                   float vertical_fov_radians, float near_plane, float far_plane);
+
 #ifdef __cplusplus
 }
 #endif

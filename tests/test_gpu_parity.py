@@ -1,0 +1,228 @@
+"""HIP path vs the CPU oracle, through the C ABI, on seeded inputs.
+
+Bars (from the plain-path contract): depth keys, visible set, back-to-front
+permutation and per-tile pair counts bit-exact; projected quantities
+bit-exact (both sides evaluate the reference's IEEE f32 expressions with
+contraction off and correctly rounded divide/sqrt); fp32 image within
+max-abs 2e-3 per channel; RGBA8 within 1 LSB on >= 99.9% of pixels.
+"""
+
+import math
+import pathlib
+import sys
+
+import numpy as np
+import pytest
+
+import synth
+from helpers import DEFAULT_FOV, cam_dict, lcg_keys_fast, minimal_ascii_scene, oracle_scene
+
+pytestmark = pytest.mark.gpu
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+IMG_TOL = 2e-3  # fp32 image, max-abs per channel
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def render(abi, scene, cam, w, h):
+    ctx = abi.Context(w, h)
+    ctx.load(scene)
+    ctx.set_camera(cam)
+    ctx.render()
+    return ctx
+
+
+def check_image(gpu32, gpu8, ref32, label):
+    d = np.abs(gpu32 - ref32)
+    worst = np.unravel_index(np.argmax(d), d.shape)
+    assert d.max() <= IMG_TOL, f"{label}: max-abs {d.max():.3g} at {worst}"
+    ref8 = np.floor(np.clip(ref32, 0, 1) * np.float32(255) + np.float32(0.5)).astype(np.int16)
+    off = np.abs(gpu8.astype(np.int16) - ref8).max(axis=-1)
+    frac_within = (off <= 1).mean()
+    assert frac_within >= 0.999, f"{label}: only {frac_within:.5f} of pixels within 1 LSB"
+
+
+def full_check(abi, oracle, scene, cam, w, h, label, image=True):
+    say(f"[{label}] N={scene.count} {w}x{h}")
+    ctx = render(abi, scene, cam, w, h)
+    osc = oracle_scene(oracle, scene)
+    cd = cam_dict(cam)
+    keys_ref, idx_ref = oracle.preprocess(osc, cd)
+    # depth keys in splat order: key bits for visible splats, 0 otherwise
+    keys = ctx.depth_keys(scene.count)
+    exp_keys = np.zeros(scene.count, np.uint32)
+    exp_keys[idx_ref] = keys_ref
+    assert np.array_equal(keys, exp_keys), f"{label}: depth keys differ at {np.nonzero(keys != exp_keys)[0][:10]}"
+    # permutation
+    order_ref = oracle.sort_values_by_keys(keys_ref, idx_ref)
+    order = ctx.sorted_indices(scene.count)
+    assert np.array_equal(order, order_ref), f"{label}: permutation differs"
+    st = ctx.stats()
+    assert st.visible_count == st.drawn_count == len(idx_ref)
+    # projection
+    proj = ctx.projection(scene.count)
+    proj_ref = oracle.project(osc, cd, w, h)
+    bad = np.nonzero((proj != proj_ref).any(axis=1))[0]
+    assert bad.size == 0, f"{label}: projection differs for {bad.size} splats, e.g. {bad[:5]}: " \
+                          f"{proj[bad[0]] if bad.size else ''} vs {proj_ref[bad[0]] if bad.size else ''}"
+    # per-tile pair counts
+    tc = ctx.tile_counts()
+    tc_ref = oracle.tile_counts(proj_ref, w, h)
+    assert np.array_equal(tc, tc_ref), f"{label}: tile counts differ in {(tc != tc_ref).sum()} tiles"
+    assert ctx.stage_times().tile_pairs == tc_ref.sum()
+    if image:
+        ref32, vis = oracle.render(osc, cd, w, h)
+        assert vis == len(idx_ref)
+        check_image(ctx.rgba32f(), ctx.rgba8(), ref32, label)
+    ctx.close()
+
+
+# ------------------------------------------------------------------- sort
+
+@pytest.mark.parametrize("length,seed", [(0, 42), (1, 43), (2, 44), (255, 297), (256, 298), (257, 299),
+                                         (1024, 1066), (4099, 1), (65537, 42 + 65537)])
+def test_gpu_sort_matches_reference_order(gpu, oracle, length, seed):
+    # sort/src/lib.rs:614-647
+    say(f"[sort] len={length}")
+    keys = lcg_keys_fast(length, seed)
+    vals = np.arange(length, dtype=np.uint32)
+    got = gpu.sort_values_by_keys(keys, vals)
+    assert np.array_equal(got, oracle.sort_values_by_keys(keys, vals))
+
+
+def test_gpu_sort_values_by_immutable_keys(gpu):
+    # sort/src/lib.rs:678-688
+    assert list(gpu.sort_values_by_keys([20, 5, 12, 12], [0, 1, 2, 3])) == [0, 2, 3, 1]
+    assert list(gpu.sort_values_by_keys([20, 5, 12], [0, 1, 2])) == [0, 2, 1]
+
+
+@pytest.mark.parametrize("case", ["ties", "extremes", "equal257", "large"])
+def test_gpu_sort_edge_keys(gpu, oracle, case):
+    # direct_gpu_order.rs:557-634 cases: 0 / u32::MAX keys, 257 equal keys; plus
+    # heavy ties with permuted values and 3M keys.
+    rng = np.random.default_rng(hash(case) % 1000)
+    if case == "ties":
+        keys = rng.integers(0, 3, 50000).astype(np.uint32)
+        vals = rng.permutation(50000).astype(np.uint32)
+    elif case == "extremes":
+        keys = rng.choice(np.array([0, 0xFFFFFFFF, 1, 0x80000000], np.uint32), 10007)
+        vals = np.arange(10007, dtype=np.uint32)
+    elif case == "equal257":
+        keys = np.full(257, 7, np.uint32)
+        vals = np.arange(257, dtype=np.uint32)[::-1].copy()
+    else:
+        keys = rng.integers(0, 2 ** 32, 3_000_000, dtype=np.uint64).astype(np.uint32)
+        vals = np.arange(3_000_000, dtype=np.uint32)
+    say(f"[sort] {case} n={keys.size}")
+    assert np.array_equal(gpu.sort_values_by_keys(keys, vals), oracle.sort_values_by_keys(keys, vals))
+
+
+# ------------------------------------------------------------ reference KATs
+
+def test_gpu_back_to_front_order_with_culling(gpu):
+    # render-wgpu/src/lib.rs:4062-4089
+    pos = np.array([[0, 0, 0.5], [0, 0, 2.0], [0, 0, 1.0], [0, 0, -1.0], [0, 0, 2000.0]], np.float32)
+    s = synth.Scene(pos, np.ones(5, np.float32), np.zeros((5, 3), np.float32),
+                    np.tile(np.float32([0, 0, 0, 1]), (5, 1)), np.tile(np.float32([0.2, 0.3, 0.4]), (5, 1)), 0, None)
+    ctx = render(gpu, s, gpu.lib().gsplat_camera_default(), 1280, 720)
+    assert list(ctx.sorted_indices(5)) == [1, 2, 0]
+    st = ctx.stats()
+    assert st.visible_count == 3 and st.drawn_count == 3 and st.frame_ms >= 0
+
+
+def test_gpu_conformance_sorted_alpha(gpu, oracle):
+    # crates/gsplat-render-wgpu/tests/conformance_sorted_alpha.rs:14-77
+    ctx = gpu.Context(64, 64)
+    ctx.load_path(GOLDEN / "minimal_ascii.ply")
+    ctx.render()
+    st = ctx.stats()
+    assert st.visible_count == 2 and st.drawn_count == 2 and st.frame_ms >= 0
+    rgba = ctx.rgba8()
+    assert rgba.size == 64 * 64 * 4 and (rgba[..., 3] > 0).any()
+    means = rgba.reshape(-1, 4).astype(np.float64).mean(0) / 255.0
+    assert np.all(np.abs(means - [0.4194, 0.3234, 0.3073, 0.5712]) <= 0.035), means
+    ref32, _ = oracle.render(minimal_ascii_scene(oracle), cam_dict(gpu.lib().gsplat_camera_default()), 64, 64)
+    check_image(ctx.rgba32f(), rgba, ref32, "conformance")
+
+
+# ------------------------------------------------------------ full pipeline
+
+def test_gpu_parity_plain_640x480_16k_sh0(gpu, oracle):
+    s, v = synth.plain_small()
+    full_check(gpu, oracle, s, gpu.look_at(v.position, v.target), v.width, v.height, "plain-640x480-16k")
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+def test_gpu_parity_sh_degrees(gpu, oracle, deg):
+    s = synth.tiny_scene(3000, 100 + deg, sh_degree=deg, spread=0.7, log_scale=-3.0)
+    cam = gpu.look_at((0.3, 0.5, -2.5), (0.0, 0.0, 0.0))
+    full_check(gpu, oracle, s, cam, 320, 200, f"sh{deg}")
+
+
+@pytest.mark.parametrize("w,h", [(100, 75), (17, 33), (1, 1), (257, 16)])
+def test_gpu_parity_ragged_sizes(gpu, oracle, w, h):
+    s = synth.tiny_scene(2000, 7, sh_degree=1, spread=0.6, log_scale=-2.5)
+    full_check(gpu, oracle, s, gpu.look_at((0.0, 0.0, -2.5), (0.0, 0.0, 0.0)), w, h, f"ragged-{w}x{h}")
+
+
+def test_gpu_parity_large_splats_and_near_plane(gpu, oracle):
+    # Splats behind, straddling and in front of the near plane; huge footprints.
+    s = synth.tiny_scene(1500, 9, sh_degree=2, spread=1.5, log_scale=-1.0)
+    cam = gpu.look_at((0.0, 0.2, -1.0), (0.0, 0.0, 0.0), near=0.3, far=3.0)
+    full_check(gpu, oracle, s, cam, 256, 192, "large-near")
+
+
+def test_gpu_parity_plain_1080p_300k_sh3(gpu, oracle):
+    s, v = synth.plain_1080p()
+    full_check(gpu, oracle, s, gpu.look_at(v.position, v.target), v.width, v.height, "plain-1080p-300k")
+
+
+def test_gpu_empty_scene_and_all_culled(gpu):
+    empty = synth.Scene(np.zeros((0, 3), np.float32), np.zeros(0, np.float32), np.zeros((0, 3), np.float32),
+                        np.zeros((0, 4), np.float32), np.zeros((0, 3), np.float32), 0, None)
+    ctx = render(gpu, empty, gpu.lib().gsplat_camera_default(), 48, 40)
+    assert ctx.stats().visible_count == 0
+    assert not ctx.rgba8().any()
+    s = synth.tiny_scene(500, 3)
+    s.positions[:, 2] = -5.0  # all behind the camera
+    ctx = render(gpu, s, gpu.lib().gsplat_camera_default(), 48, 40)
+    assert ctx.stats().visible_count == 0 and not ctx.rgba8().any()
+
+
+def test_gpu_render_without_scene_and_invalid_inputs(gpu):
+    ctx = gpu.Context(64, 64)
+    assert gpu.lib().gsplat_context_render_frame(ctx.h) == gpu.SCENE_NOT_LOADED
+    assert gpu.last_error() == "gsplat_context_render_frame: scene not loaded"
+    bad = gpu.lib().gsplat_camera_default()
+    bad.near_plane, bad.far_plane = 10.0, 1.0
+    assert gpu.lib().gsplat_context_set_camera(ctx.h, bad) == gpu.INVALID_ARGUMENT
+    assert gpu.lib().gsplat_context_set_auto_camera(ctx.h) == gpu.SCENE_NOT_LOADED
+    assert gpu.lib().gsplat_context_set_size(ctx.h, 0, 10) == gpu.INVALID_ARGUMENT
+    assert gpu.lib().gsplat_context_set_size(ctx.h, 16385, 10) == gpu.UNSUPPORTED
+    assert gpu.lib().gsplat_context_load_scene_path(ctx.h, b"/nonexistent.ply") == gpu.NOT_FOUND
+
+
+def test_gpu_frames_are_deterministic_and_resizable(gpu, oracle):
+    s = synth.tiny_scene(4000, 21, sh_degree=3, spread=0.5, log_scale=-3.0)
+    ctx = render(gpu, s, gpu.look_at((0.5, 0.0, -2.0), (0, 0, 0)), 160, 120)
+    a = ctx.rgba32f()
+    ctx.render()
+    assert np.array_equal(a, ctx.rgba32f())
+    ctx.set_size(96, 64)
+    ctx.render()
+    ref, _ = oracle.render(oracle_scene(oracle, s), cam_dict(ctx.get_camera()), 96, 64)
+    check_image(ctx.rgba32f(), ctx.rgba8(), ref, "resized")
+
+
+def test_gpu_auto_camera_frames_scene(gpu, oracle):
+    ctx = gpu.Context(64, 64)
+    ctx.load_path(GOLDEN / "minimal_ascii.ply")
+    assert gpu.lib().gsplat_context_set_auto_camera(ctx.h) == gpu.OK
+    c = ctx.get_camera()
+    assert c.near_plane >= 0.01 and c.far_plane >= 100.0
+    ctx.render()
+    keys, idx = oracle.preprocess(minimal_ascii_scene(oracle), cam_dict(c))
+    assert ctx.stats().visible_count == len(idx) > 0
