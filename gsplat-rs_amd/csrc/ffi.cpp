@@ -13,6 +13,7 @@
 
 #include "common.h"
 #include "renderer.h"
+#include "sort.h"
 
 struct GsplatContext {
   std::unique_ptr<gs::Renderer> renderer;
@@ -439,26 +440,17 @@ int32_t gsplat_sort_values_by_keys(const uint32_t *keys, uint32_t *values, uint6
         (void)hipStreamDestroy(s);
       }
     } sg{st};
-    gs::DevBuf<uint32_t> k, v, ka, va, hist, partials;
-    gs::DevBuf<uint64_t> total;
+    gs::DevBuf<uint32_t> k, v, ka, va, scratch;
     k.ensure(n);
     v.ensure(n);
     ka.ensure(n);
     va.ensure(n);
-    hist.ensure(gs::radix_hist_entries(n));
-    partials.ensure(gs::scan_partial_entries(gs::radix_hist_entries(n)));
-    total.ensure(1);
-    gs::SortScratch sc;
-    sc.hist = hist.get();
-    sc.hist_cap = hist.capacity();
-    sc.partials = partials.get();
-    sc.partial_cap = partials.capacity();
-    sc.total = total.get();
+    scratch.ensure(gs::onesweep_status_words(n, 32));
     if (hipMemcpyAsync(k.get(), keys, len * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(v.get(), values, len * 4, hipMemcpyHostToDevice, st) != hipSuccess)
       return fail(gs::Code::Internal, std::string(op) + ": upload failed");
     uint32_t *pk = k.get(), *pv = v.get(), *pka = ka.get(), *pva = va.get();
-    gs::radix_sort(&pk, &pv, &pka, &pva, n, 32, true, sc, st);
+    gs::onesweep_sort(&pk, &pv, &pka, &pva, n, nullptr, 32, true, scratch.get(), scratch.capacity(), st);
     if (hipMemcpyAsync(values, pv, len * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
       return fail(gs::Code::Internal, std::string(op) + ": download failed");

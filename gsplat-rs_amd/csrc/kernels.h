@@ -26,8 +26,6 @@ FrameParams make_frame_params(const float position[3], const float rotation_xyzw
 
 constexpr uint32_t kTile = 16;            // raster tile edge in pixels
 constexpr uint32_t kThreads = 256;        // threads per workgroup (4 wave64)
-constexpr uint32_t kRadixItems = 16;      // keys per thread per radix block
-constexpr uint32_t kRadixTile = kThreads * kRadixItems;
 constexpr uint32_t kScanTile = kThreads * 16;
 
 // Projected splat, 3 x float4 per splat (48 B):
@@ -46,40 +44,38 @@ void launch_project(uint32_t sh_degree, const float *soa, uint32_t n, const Fram
                     uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect, uint32_t *visible,
                     float *export12, hipStream_t stream);
 
-// Scratch for the scans and the radix sort.
-struct SortScratch {
-  uint32_t *hist = nullptr;      // 256 * blocks digit counts
-  uint32_t *partials = nullptr;  // first-level scan partials
-  uint64_t *total = nullptr;     // 64-bit scan total (device)
-  size_t hist_cap = 0;
+// Scratch of the two-level exclusive scan.
+struct ScanScratch {
+  uint32_t *partials = nullptr;  // first-level partials
+  uint64_t *total = nullptr;     // 64-bit grand total (device)
   size_t partial_cap = 0;
 };
-size_t radix_hist_entries(uint32_t n);
 size_t scan_partial_entries(size_t n);
 
 // Exclusive scan of n u32 values; the 64-bit sum goes to *scratch.total.
-void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, SortScratch &s, hipStream_t stream);
-
-// Stable LSD radix sort of (keys, vals) on the low `bits` key bits, 8 bits
-// per pass; descending order when `descending`. The sorted data ends in
-// keys[0]/vals[0] (the pointers are swapped with the alternate buffers as
-// needed).
-void radix_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n,
-                uint32_t bits, bool descending, SortScratch &s, hipStream_t stream);
+void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s, hipStream_t stream);
 
 // counts[j] = number of tiles of the j-th splat front to back, i.e. of
 // sorted[V-1-j], for j < V (V read from *visible); 0 for j >= V.
 void launch_tile_counts(const uint32_t *sorted, const uint32_t *visible, const uint2 *rect, uint32_t n,
                         uint32_t *counts, hipStream_t stream);
 
-// Writes (tile id, splat) pairs front to back at offsets[j].
-void launch_tile_emit(const uint32_t *sorted, const uint32_t *visible, const uint2 *rect,
-                      const uint32_t *offsets, uint32_t n, uint32_t tiles_x, uint32_t *pair_keys,
-                      uint32_t *pair_vals, hipStream_t stream);
+// Frame summary written on the device after the tile-count scan:
+// info[0] = V, info[1..2] = 64-bit pair total M, info[3] = 1 when M exceeds
+// the pair capacity; *m_out = min(M, capacity) drives the later kernels.
+void launch_pairs_finalize(const uint32_t *visible, const uint64_t *total, uint32_t capacity, uint32_t *m_out,
+                           uint32_t *info, hipStream_t stream);
 
-// ranges[t] = [first, last+1) of tile t in the tile-sorted pair keys.
-void launch_tile_ranges(const uint32_t *pair_keys, uint32_t m, uint32_t tiles, uint2 *ranges,
-                        hipStream_t stream);
+// Writes (tile id, splat) pairs front to back at offsets[j]; pairs at or
+// beyond `capacity` are dropped (the frame is then flagged and re-rendered).
+void launch_tile_emit(const uint32_t *sorted, const uint32_t *visible, const uint2 *rect,
+                      const uint32_t *offsets, uint32_t n, uint32_t tiles_x, uint32_t capacity,
+                      uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t stream);
+
+// ranges[t] = [first, last+1) of tile t in the tile-sorted pair keys; the
+// pair count is read from *m_dev (grid sized for `capacity`).
+void launch_tile_ranges(const uint32_t *pair_keys, const uint32_t *m_dev, uint32_t capacity, uint32_t tiles,
+                        uint2 *ranges, hipStream_t stream);
 
 // Front-to-back blend of each 16x16 tile's list with LDS-staged records.
 void launch_raster(const float4 *rec, const uint2 *ranges, const uint32_t *list, uint32_t width,

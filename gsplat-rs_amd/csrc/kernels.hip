@@ -31,11 +31,6 @@ namespace {
 constexpr uint32_t kWave = 64;
 constexpr uint32_t kWaves = kThreads / kWave;
 
-__device__ __forceinline__ uint64_t lanemask_lt() {
-  const uint32_t lane = __lane_id();
-  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
-
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
   const uint32_t lane = __lane_id();
 #pragma unroll
@@ -229,6 +224,8 @@ __global__ __launch_bounds__(kThreads) void k_project(const float *__restrict__ 
       }
     }
     if (EXPORT) {
+      // invalid_splat() (wgsl:122-129): nothing but zeros for a culled splat.
+      if (!drawn) ux = uy = vx = vy = 0.0f;
       float *o = export12 + 12 * (size_t)i;
       o[0] = drawn ? 1.0f : 0.0f;
       o[1] = cx;
@@ -344,95 +341,6 @@ __global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t *__restrict__ ou
   }
 }
 
-// -------------------------------------------------------------- radix sort
-
-// Digit of a key for this pass; `flip` = ~0 sorts descending.
-__device__ __forceinline__ uint32_t digit_of(uint32_t key, uint32_t shift, uint32_t flip) {
-  return ((key ^ flip) >> shift) & 0xFFu;
-}
-
-// Lanes of the wave holding the same 8-bit digit (8 ballots), restricted to
-// `active` lanes.
-__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool active) {
-  uint64_t m = __ballot(active);
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t bb = __ballot(active && bit);
-    m &= bit ? bb : ~bb;
-  }
-  return m;
-}
-
-// Per-block digit histogram, digit-major: hist[d * blocks + block].
-__global__ __launch_bounds__(kThreads) void k_radix_upsweep(const uint32_t *__restrict__ keys, uint32_t n,
-                                                            uint32_t shift, uint32_t flip,
-                                                            uint32_t *__restrict__ hist, uint32_t blocks) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const size_t base = (size_t)blockIdx.x * kRadixTile;
-#pragma unroll 4
-  for (uint32_t r = 0; r < kRadixItems; ++r) {
-    const size_t p = base + (size_t)r * kThreads + threadIdx.x;
-    const bool active = p < n;
-    const uint32_t d = active ? digit_of(keys[p], shift, flip) : 0u;
-    const uint64_t peers = match_digit(d, active);
-    if (active && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
-  }
-  __syncthreads();
-  hist[(size_t)threadIdx.x * blocks + blockIdx.x] = h[threadIdx.x];
-}
-
-// Stable scatter: item order inside a block is (round, wave, lane); the rank
-// of an item among equal digits is the block's running count + the counts of
-// earlier waves this round + its lane rank among peers (ballot + popcount).
-__global__ __launch_bounds__(kThreads) void k_radix_scatter(const uint32_t *__restrict__ kin,
-                                                            const uint32_t *__restrict__ vin,
-                                                            uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                            uint32_t n, uint32_t shift, uint32_t flip,
-                                                            const uint32_t *__restrict__ offsets, uint32_t blocks) {
-  __shared__ uint32_t gbase[256];
-  __shared__ uint32_t wcnt[kWaves][256];
-  const uint32_t t = threadIdx.x, w = t / kWave;
-  gbase[t] = offsets[(size_t)t * blocks + blockIdx.x];
-  uint32_t run = 0;  // running count of digit t in this block
-  const size_t base = (size_t)blockIdx.x * kRadixTile;
-  for (uint32_t r = 0; r < kRadixItems; ++r) {
-#pragma unroll
-    for (uint32_t q = 0; q < kWaves; ++q) wcnt[q][t] = 0;
-    __syncthreads();
-    const size_t p = base + (size_t)r * kThreads + t;
-    const bool active = p < n;
-    uint32_t k = 0, v = 0, d = 0;
-    if (active) {
-      k = kin[p];
-      v = vin[p];
-      d = digit_of(k, shift, flip);
-    }
-    const uint64_t peers = match_digit(d, active);
-    const uint32_t lrank = (uint32_t)__popcll(peers & lanemask_lt());
-    if (active && lrank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    // Thread t owns digit t: exclusive prefix over waves, then advance run.
-    uint32_t s = run;
-#pragma unroll
-    for (uint32_t q = 0; q < kWaves; ++q) {
-      const uint32_t c = wcnt[q][t];
-      wcnt[q][t] = s;
-      s += c;
-    }
-    run = s;
-    __syncthreads();
-    if (active) {
-      const uint32_t pos = gbase[d] + wcnt[w][d] + lrank;
-      kout[pos] = k;
-      vout[pos] = v;
-    }
-    __syncthreads();
-  }
-}
-
 // ------------------------------------------------------------- tile binning
 
 __device__ __forceinline__ uint32_t rect_tiles(uint2 r) {
@@ -451,11 +359,23 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const uint32_t *__rest
   counts[j] = j < v ? rect_tiles(rect[sorted[v - 1 - j]]) : 0u;
 }
 
+__global__ void k_pairs_finalize(const uint32_t *__restrict__ visible, const uint64_t *__restrict__ total,
+                                 uint32_t capacity, uint32_t *__restrict__ m_out, uint32_t *__restrict__ info) {
+  if (threadIdx.x != 0) return;
+  const uint64_t m = *total;
+  *m_out = m > capacity ? capacity : (uint32_t)m;
+  info[0] = *visible;
+  info[1] = (uint32_t)m;
+  info[2] = (uint32_t)(m >> 32);
+  info[3] = m > capacity ? 1u : 0u;
+}
+
 __global__ __launch_bounds__(kThreads) void k_tile_emit(const uint32_t *__restrict__ sorted,
                                                         const uint32_t *__restrict__ visible,
                                                         const uint2 *__restrict__ rect,
                                                         const uint32_t *__restrict__ offsets, uint32_t n,
-                                                        uint32_t tiles_x, uint32_t *__restrict__ pair_keys,
+                                                        uint32_t tiles_x, uint32_t capacity,
+                                                        uint32_t *__restrict__ pair_keys,
                                                         uint32_t *__restrict__ pair_vals) {
   const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
   const uint32_t v = *visible;
@@ -467,15 +387,18 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const uint32_t *__restri
   uint32_t o = offsets[j];
   for (uint32_t ty = y0; ty <= y1; ++ty)
     for (uint32_t tx = x0; tx <= x1; ++tx) {
+      if (o >= capacity) return;
       pair_keys[o] = ty * tiles_x + tx;
       pair_vals[o] = s;
       ++o;
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_tile_ranges(const uint32_t *__restrict__ keys, uint32_t m,
+__global__ __launch_bounds__(kThreads) void k_tile_ranges(const uint32_t *__restrict__ keys,
+                                                          const uint32_t *__restrict__ m_dev,
                                                           uint2 *__restrict__ ranges) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t m = *m_dev;
   if (i >= m) return;
   const uint32_t t = keys[i];
   if (i == 0 || keys[i - 1] != t) ranges[t].x = i;
@@ -612,10 +535,9 @@ void launch_project(uint32_t deg, const float *soa, uint32_t n, const FrameParam
     project_dispatch<false>(deg, soa, n, fp, keys, vals, rec, rect, visible, nullptr, st);
 }
 
-size_t radix_hist_entries(uint32_t n) { return (size_t)256 * std::max<uint32_t>(div_up(n, kRadixTile), 1u); }
 size_t scan_partial_entries(size_t n) { return std::max<size_t>(div_up(n, kScanTile), 1); }
 
-void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, SortScratch &s, hipStream_t st) {
+void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s, hipStream_t st) {
   if (n == 0) {
     launch_fill_u32(reinterpret_cast<uint32_t *>(s.total), 0u, 2, st);
     return;
@@ -628,22 +550,10 @@ void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, SortScratch &s,
   GS_HIP(hipGetLastError());
 }
 
-void radix_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n,
-                uint32_t bits, bool descending, SortScratch &s, hipStream_t st) {
-  if (n <= 1 || bits == 0) return;
-  const uint32_t blocks = div_up(n, kRadixTile);
-  const size_t entries = (size_t)256 * blocks;
-  if (entries > s.hist_cap) throw Error(Code::Internal, "radix scratch too small");
-  const uint32_t flip = descending ? 0xFFFFFFFFu : 0u;
-  for (uint32_t shift = 0; shift < bits; shift += 8) {
-    k_radix_upsweep<<<blocks, kThreads, 0, st>>>(*keys, n, shift, flip, s.hist, blocks);
-    exclusive_scan(s.hist, s.hist, entries, s, st);
-    k_radix_scatter<<<blocks, kThreads, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n, shift, flip, s.hist,
-                                                  blocks);
-    GS_HIP(hipGetLastError());
-    std::swap(*keys, *keys_alt);
-    std::swap(*vals, *vals_alt);
-  }
+void launch_pairs_finalize(const uint32_t *visible, const uint64_t *total, uint32_t capacity, uint32_t *m_out,
+                           uint32_t *info, hipStream_t st) {
+  k_pairs_finalize<<<1, 64, 0, st>>>(visible, total, capacity, m_out, info);
+  GS_HIP(hipGetLastError());
 }
 
 void launch_tile_counts(const uint32_t *sorted, const uint32_t *visible, const uint2 *rect, uint32_t n,
@@ -654,10 +564,11 @@ void launch_tile_counts(const uint32_t *sorted, const uint32_t *visible, const u
 }
 
 void launch_tile_emit(const uint32_t *sorted, const uint32_t *visible, const uint2 *rect, const uint32_t *offsets,
-                      uint32_t n, uint32_t tiles_x, uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t st) {
+                      uint32_t n, uint32_t tiles_x, uint32_t capacity, uint32_t *pair_keys, uint32_t *pair_vals,
+                      hipStream_t st) {
   if (n == 0) return;
-  k_tile_emit<<<div_up(n, kThreads), kThreads, 0, st>>>(sorted, visible, rect, offsets, n, tiles_x, pair_keys,
-                                                         pair_vals);
+  k_tile_emit<<<div_up(n, kThreads), kThreads, 0, st>>>(sorted, visible, rect, offsets, n, tiles_x, capacity,
+                                                         pair_keys, pair_vals);
   GS_HIP(hipGetLastError());
 }
 
@@ -667,10 +578,11 @@ void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t st) {
   GS_HIP(hipGetLastError());
 }
 
-void launch_tile_ranges(const uint32_t *pair_keys, uint32_t m, uint32_t tiles, uint2 *ranges, hipStream_t st) {
+void launch_tile_ranges(const uint32_t *pair_keys, const uint32_t *m_dev, uint32_t capacity, uint32_t tiles,
+                        uint2 *ranges, hipStream_t st) {
   launch_fill_u32(reinterpret_cast<uint32_t *>(ranges), 0u, 2 * (size_t)tiles, st);
-  if (m == 0) return;
-  k_tile_ranges<<<div_up(m, kThreads), kThreads, 0, st>>>(pair_keys, m, ranges);
+  if (capacity == 0) return;
+  k_tile_ranges<<<div_up(capacity, kThreads), kThreads, 0, st>>>(pair_keys, m_dev, ranges);
   GS_HIP(hipGetLastError());
 }
 

@@ -6,10 +6,14 @@
 //   bin       per-splat tile counts front to back, scan, (tile, splat) pair
 //             emission, stable radix sort by tile id, tile ranges
 //   raster    16x16-tile front-to-back blend into RGBA32F + RGBA8
-// The only host synchronisation is after the scan of tile counts (the pair
-// count sizes the pair buffers).
+// No host synchronisation inside a frame: every size after the projection
+// (V, M) is read on the device. The (tile, splat) pair buffers have a host-
+// chosen capacity; a frame whose pair count exceeds it drops the excess,
+// flags itself, and is re-rendered by wait() after the buffers grow.
 
 #include "renderer.h"
+
+#include "sort.h"
 
 #include <algorithm>
 #include <cstring>
@@ -42,6 +46,8 @@ void DevBuf<T>::ensure(size_t count) {
 
 template <typename T>
 void DevBuf<T>::release() {
+  // Work queued on any stream may still use the old allocation.
+  if (ptr_) (void)hipDeviceSynchronize();
   if (ptr_) (void)hipFree(ptr_);
   ptr_ = nullptr;
   cap_ = 0;
@@ -72,16 +78,6 @@ std::shared_ptr<DeviceScene> upload_scene(const HostScene &s, hipStream_t stream
   return d;
 }
 
-SortScratch ViewWork::scratch() {
-  SortScratch s;
-  s.hist = hist.get();
-  s.hist_cap = hist.capacity();
-  s.partials = partials.get();
-  s.partial_cap = partials.capacity();
-  s.total = total.get();
-  return s;
-}
-
 Renderer::Renderer(uint32_t width, uint32_t height) : width_(width), height_(height) {
   if (!device_present()) throw Error(Code::Unsupported, "gpu rasterizer unavailable");
   GS_HIP(hipSetDevice(0));
@@ -90,6 +86,8 @@ Renderer::Renderer(uint32_t width, uint32_t height) : width_(width), height_(hei
   GS_HIP(hipHostMalloc(reinterpret_cast<void **>(&pinned_), 4 * sizeof(uint32_t), hipHostMallocDefault));
   w_.visible.ensure(1);
   w_.total.ensure(1);
+  w_.m_dev.ensure(1);
+  w_.info.ensure(4);
   set_size(width, height);
 }
 
@@ -116,6 +114,7 @@ void Renderer::load_scene(HostScene &&scene) {
   scene_ = upload_scene(scene, stream_);
   host_scene_ = std::make_unique<HostScene>(std::move(scene));
   frame_valid_ = false;
+  w_.pair_cap = 0;
 }
 
 void Renderer::ensure_frame_buffers(uint32_t n) {
@@ -128,8 +127,8 @@ void Renderer::ensure_frame_buffers(uint32_t n) {
   w_.rect.ensure(m);
   w_.counts.ensure(m);
   w_.offsets.ensure(m);
-  w_.hist.ensure(radix_hist_entries(n));
-  w_.partials.ensure(std::max(scan_partial_entries(m), scan_partial_entries(radix_hist_entries(n))));
+  w_.partials.ensure(scan_partial_entries(m));
+  w_.sort_scratch.ensure(onesweep_status_words((uint32_t)m, 32));
 }
 
 void Renderer::ensure_target() {
@@ -140,17 +139,27 @@ void Renderer::ensure_target() {
   w_.ranges.ensure(tiles);
 }
 
-void Renderer::render(const GsplatCamera &cam) {
-  const auto t0 = std::chrono::steady_clock::now();
-  if (!scene_) throw Error(Code::SceneNotLoaded, "scene not loaded");
-  if (!camera_valid(cam)) throw Error(Code::InvalidArgument, "invalid camera");
+void Renderer::ensure_pairs(uint32_t capacity, uint32_t tile_bits) {
+  w_.pair_keys.ensure(capacity);
+  w_.pair_vals.ensure(capacity);
+  w_.pair_keys_alt.ensure(capacity);
+  w_.pair_vals_alt.ensure(capacity);
+  w_.sort_scratch.ensure(onesweep_status_words(capacity, tile_bits));
+}
+
+void Renderer::enqueue_frame(const GsplatCamera &cam) {
   const uint32_t n = scene_->count;
   ensure_frame_buffers(n);
   ensure_target();
   const FrameParams fp = make_frame_params(cam.position, cam.rotation_xyzw, cam.vertical_fov_radians,
                                            cam.near_plane, cam.far_plane, width_, height_);
   const uint32_t tiles = fp.tiles_x * fp.tiles_y;
+  uint32_t tile_bits = 1;
+  while ((1u << tile_bits) < tiles) ++tile_bits;
+  if (w_.pair_cap == 0) w_.pair_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8ull * n, 1u << 20), 1u << 30);
+  ensure_pairs(w_.pair_cap, tile_bits);
   hipStream_t st = stream_;
+  ScanScratch sc{w_.partials.get(), w_.total.get(), w_.partials.capacity()};
 
   GS_HIP(hipEventRecord(ev_[0], st));
   launch_fill_u32(w_.visible.get(), 0u, 1, st);
@@ -159,99 +168,117 @@ void Renderer::render(const GsplatCamera &cam) {
   GS_HIP(hipEventRecord(ev_[1], st));
 
   uint32_t *k = w_.keys.get(), *v = w_.vals.get(), *ka = w_.keys_alt.get(), *va = w_.vals_alt.get();
-  SortScratch sc = w_.scratch();
-  radix_sort(&k, &v, &ka, &va, n, 32, /*descending=*/true, sc, st);
-  w_.sorted_keys = k;
+  onesweep_sort(&k, &v, &ka, &va, n, nullptr, 32, /*descending=*/true, w_.sort_scratch.get(),
+                w_.sort_scratch.capacity(), st);
   w_.sorted_vals = v;
   GS_HIP(hipEventRecord(ev_[2], st));
 
-  launch_tile_counts(v, w_.visible.get(), w_.rect.get(), n, w_.counts.get(), st);
-  exclusive_scan(w_.counts.get(), w_.offsets.get(), std::max<uint32_t>(n, 1), sc, st);
-  GS_HIP(hipMemcpyAsync(pinned_, w_.visible.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  GS_HIP(hipMemcpyAsync(pinned_ + 2, w_.total.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-  GS_HIP(hipStreamSynchronize(st));
-  visible_ = n ? pinned_[0] : 0;
-  uint64_t m64;
-  std::memcpy(&m64, pinned_ + 2, sizeof(m64));
-  if (n == 0) m64 = 0;
-  if (m64 > 0xFFFFFFF0ull) throw Error(Code::Unsupported, "tile pair count exceeds 32-bit indexing");
-  const uint32_t m = (uint32_t)m64;
-  pairs_ = m;
-
-  w_.pair_keys.ensure(m);
-  w_.pair_vals.ensure(m);
-  w_.pair_keys_alt.ensure(m);
-  w_.pair_vals_alt.ensure(m);
-  w_.hist.ensure(radix_hist_entries(m));
-  w_.partials.ensure(scan_partial_entries(radix_hist_entries(m)));
-  sc = w_.scratch();
-  launch_tile_emit(v, w_.visible.get(), w_.rect.get(), w_.offsets.get(), n, fp.tiles_x, w_.pair_keys.get(),
-                   w_.pair_vals.get(), st);
+  const uint32_t n1 = std::max<uint32_t>(n, 1);
+  launch_tile_counts(v, w_.visible.get(), w_.rect.get(), n1, w_.counts.get(), st);
+  exclusive_scan(w_.counts.get(), w_.offsets.get(), n1, sc, st);
+  launch_pairs_finalize(w_.visible.get(), w_.total.get(), w_.pair_cap, w_.m_dev.get(), w_.info.get(), st);
+  launch_tile_emit(v, w_.visible.get(), w_.rect.get(), w_.offsets.get(), n, fp.tiles_x, w_.pair_cap,
+                   w_.pair_keys.get(), w_.pair_vals.get(), st);
   uint32_t *pk = w_.pair_keys.get(), *pv = w_.pair_vals.get(), *pka = w_.pair_keys_alt.get(),
            *pva = w_.pair_vals_alt.get();
-  uint32_t bits = 1;
-  while ((1u << bits) < tiles) ++bits;
-  radix_sort(&pk, &pv, &pka, &pva, m, bits, /*descending=*/false, sc, st);
-  w_.tile_keys = pk;
+  onesweep_sort(&pk, &pv, &pka, &pva, w_.pair_cap, w_.m_dev.get(), tile_bits, /*descending=*/false,
+                w_.sort_scratch.get(), w_.sort_scratch.capacity(), st);
   w_.tile_list = pv;
-  launch_tile_ranges(pk, m, tiles, w_.ranges.get(), st);
+  launch_tile_ranges(pk, w_.m_dev.get(), w_.pair_cap, tiles, w_.ranges.get(), st);
   GS_HIP(hipEventRecord(ev_[3], st));
 
   launch_raster(w_.rec.get(), w_.ranges.get(), pv, width_, height_, fp.tiles_x, fp.tiles_y, w_.img32.get(),
                 w_.img8.get(), st);
   GS_HIP(hipEventRecord(ev_[4], st));
+  GS_HIP(hipMemcpyAsync(pinned_, w_.info.get(), 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  GS_HIP(hipEventRecord(ev_[5], st));
 
   frame_w_ = width_;
   frame_h_ = height_;
   last_fp_ = fp;
+  last_cam_ = cam;
   frame_valid_ = true;
-  stats_pending_ = true;
+  frame_pending_ = true;
+}
+
+bool Renderer::absorb_frame_info() {
+  visible_ = scene_ && scene_->count ? pinned_[0] : 0;
+  pairs_ = scene_ && scene_->count ? ((uint64_t)pinned_[2] << 32) | pinned_[1] : 0;
+  const bool overflow = pinned_[3] != 0 && scene_ && scene_->count;
+  if (overflow) {
+    const uint64_t want = pairs_ + pairs_ / 4 + 4096;
+    if (want > (1ull << 30)) throw Error(Code::Unsupported, "tile pair count exceeds 2^30");
+    w_.pair_cap = (uint32_t)want;
+  }
+  return overflow;
+}
+
+void Renderer::render(const GsplatCamera &cam) {
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!scene_) throw Error(Code::SceneNotLoaded, "scene not loaded");
+  if (!camera_valid(cam)) throw Error(Code::InvalidArgument, "invalid camera");
+  // Grow the pair buffers early when the previous frame is known to have
+  // overflowed them (that frame is re-rendered by wait()).
+  if (frame_pending_ && hipEventQuery(ev_[5]) == hipSuccess) {
+    if (absorb_frame_info()) {
+      ++overflow_rerenders_;
+      enqueue_frame(last_cam_);
+      wait();
+    }
+  }
+  enqueue_frame(cam);
   host_ms_ = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 void Renderer::wait() {
-  if (stream_) GS_HIP(hipStreamSynchronize(stream_));
-}
-
-void Renderer::finish_stats() {
-  if (!stats_pending_) return;
-  GS_HIP(hipEventSynchronize(ev_[4]));
-  float ms[4] = {0, 0, 0, 0}, total = 0;
-  for (int s = 0; s < 4; ++s) GS_HIP(hipEventElapsedTime(&ms[s], ev_[s], ev_[s + 1]));
-  GS_HIP(hipEventElapsedTime(&total, ev_[0], ev_[4]));
-  // FrameStats (core/src/lib.rs:172-180): visible = drawn = V on this path
-  // (render-wgpu/src/lib.rs:565-583, 736).
-  last_stats_.frame_ms = host_ms_;
-  last_stats_.preprocess_ms = ms[0];
-  last_stats_.sort_ms = ms[1];
-  last_stats_.raster_ms = ms[2] + ms[3];
-  last_stats_.visible_count = visible_;
-  last_stats_.drawn_count = visible_;
-  last_times_.project_ms = ms[0];
-  last_times_.sort_ms = ms[1];
-  last_times_.bin_ms = ms[2];
-  last_times_.raster_ms = ms[3];
-  last_times_.total_ms = total;
-  last_times_.splat_count = scene_ ? scene_->count : 0;
-  last_times_.visible_count = visible_;
-  last_times_.tiles_x = (frame_w_ + kTile - 1) / kTile;
-  last_times_.tiles_y = (frame_h_ + kTile - 1) / kTile;
-  last_times_.tile_pairs = pairs_;
-  stats_pending_ = false;
+  if (!stream_) return;
+  for (int attempt = 0; frame_pending_; ++attempt) {
+    GS_HIP(hipEventSynchronize(ev_[5]));
+    if (absorb_frame_info()) {
+      if (attempt > 4) throw Error(Code::Internal, "tile pair buffers keep overflowing");
+      ++overflow_rerenders_;
+      enqueue_frame(last_cam_);
+      continue;
+    }
+    float ms[4] = {0, 0, 0, 0}, total = 0;
+    for (int s = 0; s < 4; ++s) GS_HIP(hipEventElapsedTime(&ms[s], ev_[s], ev_[s + 1]));
+    GS_HIP(hipEventElapsedTime(&total, ev_[0], ev_[4]));
+    // FrameStats (core/src/lib.rs:172-180): visible = drawn = V on this path
+    // (render-wgpu/src/lib.rs:565-583, 736).
+    last_stats_.frame_ms = host_ms_;
+    last_stats_.preprocess_ms = ms[0];
+    last_stats_.sort_ms = ms[1];
+    last_stats_.raster_ms = ms[2] + ms[3];
+    last_stats_.visible_count = visible_;
+    last_stats_.drawn_count = visible_;
+    last_times_.project_ms = ms[0];
+    last_times_.sort_ms = ms[1];
+    last_times_.bin_ms = ms[2];
+    last_times_.raster_ms = ms[3];
+    last_times_.total_ms = total;
+    last_times_.splat_count = scene_ ? scene_->count : 0;
+    last_times_.visible_count = visible_;
+    last_times_.tiles_x = (frame_w_ + kTile - 1) / kTile;
+    last_times_.tiles_y = (frame_h_ + kTile - 1) / kTile;
+    last_times_.tile_pairs = pairs_;
+    frame_pending_ = false;
+  }
+  GS_HIP(hipStreamSynchronize(stream_));
 }
 
 GsplatStats Renderer::stats() {
-  finish_stats();
+  wait();
   return last_stats_;
 }
 
 GsplatStageTimes Renderer::stage_times() {
-  finish_stats();
+  wait();
   return last_times_;
 }
 
 void Renderer::readback_rgba8(uint8_t *out, uint64_t len) {
   if (!frame_valid_) throw Error(Code::SceneNotLoaded, "no rendered frame to read back");
+  wait();
   const uint64_t need = (uint64_t)frame_w_ * frame_h_ * 4;
   if (len < need) throw Error(Code::InvalidArgument, "output buffer smaller than width*height*4 bytes");
   GS_HIP(hipMemcpyAsync(out, w_.img8.get(), need, hipMemcpyDeviceToHost, stream_));
@@ -260,6 +287,7 @@ void Renderer::readback_rgba8(uint8_t *out, uint64_t len) {
 
 void Renderer::readback_rgba32f(float *out, uint64_t len) {
   if (!frame_valid_) throw Error(Code::SceneNotLoaded, "no rendered frame to read back");
+  wait();
   const uint64_t need = (uint64_t)frame_w_ * frame_h_ * 4;
   if (len < need) throw Error(Code::InvalidArgument, "output buffer smaller than width*height*4 floats");
   GS_HIP(hipMemcpyAsync(out, w_.img32.get(), need * sizeof(float), hipMemcpyDeviceToHost, stream_));
@@ -285,6 +313,7 @@ void Renderer::read_depth_keys(uint32_t *out, uint64_t cap) {
 
 void Renderer::read_sorted(uint32_t *out, uint64_t cap, uint32_t *count) {
   if (!frame_valid_) throw Error(Code::SceneNotLoaded, "no rendered frame");
+  wait();
   if (cap < visible_) throw Error(Code::InvalidArgument, "output buffer smaller than the visible count");
   if (visible_)
     GS_HIP(hipMemcpyAsync(out, w_.sorted_vals, (size_t)visible_ * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
@@ -310,6 +339,7 @@ void Renderer::read_projection(const GsplatCamera &cam, float *out, uint64_t cap
 
 void Renderer::read_tile_counts(uint32_t *out, uint64_t cap, uint32_t *tx, uint32_t *ty) {
   if (!frame_valid_) throw Error(Code::SceneNotLoaded, "no rendered frame");
+  wait();
   const uint32_t txs = (frame_w_ + kTile - 1) / kTile, tys = (frame_h_ + kTile - 1) / kTile;
   const size_t tiles = (size_t)txs * tys;
   if (cap < tiles) throw Error(Code::InvalidArgument, "output buffer smaller than the tile count");

@@ -52,13 +52,12 @@ struct ViewWork {
   DevBuf<uint2> ranges;
   DevBuf<float4> img32;
   DevBuf<uchar4> img8;
-  DevBuf<uint32_t> hist, partials;
+  DevBuf<uint32_t> sort_scratch, partials;
   DevBuf<uint64_t> total;
-  DevBuf<uint32_t> visible;
-  uint32_t *sorted_keys = nullptr, *sorted_vals = nullptr;  // results of the depth sort
-  uint32_t *tile_list = nullptr;                             // tile-sorted splat ids
-  uint32_t *tile_keys = nullptr;                             // tile-sorted tile ids
-  SortScratch scratch();
+  DevBuf<uint32_t> visible, m_dev, info;
+  uint32_t pair_cap = 0;
+  uint32_t *sorted_vals = nullptr;  // back-to-front splat order of the last frame
+  uint32_t *tile_list = nullptr;    // tile-sorted splat ids of the last frame
 };
 
 class Renderer {
@@ -72,12 +71,15 @@ class Renderer {
   void load_scene(HostScene &&scene);
   const HostScene *scene() const { return host_scene_.get(); }
 
-  // Full frame for `camera`; enqueues the raster and returns once the
-  // binning sizes are known (the raster completes asynchronously).
+  // Enqueues a full frame for `camera` and returns without waiting for the
+  // device (as the reference's queue.submit, render-wgpu/src/lib.rs:2751).
   void render(const GsplatCamera &camera);
+  // Waits for the last frame; re-renders it when the tile pairs overflowed
+  // their buffers (which are then grown).
   void wait();
   GsplatStats stats();
   GsplatStageTimes stage_times();
+  uint64_t overflow_rerenders() const { return overflow_rerenders_; }
 
   void readback_rgba8(uint8_t *out, uint64_t len);
   void readback_rgba32f(float *out, uint64_t len);
@@ -87,23 +89,27 @@ class Renderer {
   void read_tile_counts(uint32_t *out, uint64_t cap, uint32_t *tx, uint32_t *ty);
 
  private:
+  void enqueue_frame(const GsplatCamera &camera);
+  bool absorb_frame_info();  // true when the frame's pairs overflowed
   void ensure_frame_buffers(uint32_t n);
   void ensure_target();
-  void finish_stats();
+  void ensure_pairs(uint32_t capacity, uint32_t tile_bits);
 
   hipStream_t stream_ = nullptr;
   uint32_t width_, height_;
   std::unique_ptr<HostScene> host_scene_;
   std::shared_ptr<DeviceScene> scene_;
   ViewWork w_;
-  hipEvent_t ev_[5] = {};
-  uint32_t *pinned_ = nullptr;  // [visible, total lo, total hi]
+  hipEvent_t ev_[6] = {};  // stage boundaries 0..4, 5 = frame summary copied
+  uint32_t *pinned_ = nullptr;  // frame summary: [V, M lo, M hi, overflow]
   bool frame_valid_ = false;
-  bool stats_pending_ = false;
+  bool frame_pending_ = false;
+  GsplatCamera last_cam_{};
   uint32_t frame_w_ = 0, frame_h_ = 0;
   FrameParams last_fp_{};
   uint32_t visible_ = 0;
   uint64_t pairs_ = 0;
+  uint64_t overflow_rerenders_ = 0;
   float host_ms_ = 0.0f;
   GsplatStats last_stats_{};
   GsplatStageTimes last_times_{};

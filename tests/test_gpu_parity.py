@@ -222,7 +222,7 @@ def test_gpu_auto_camera_frames_scene(gpu, oracle):
     ctx.load_path(GOLDEN / "minimal_ascii.ply")
     assert gpu.lib().gsplat_context_set_auto_camera(ctx.h) == gpu.OK
     c = ctx.get_camera()
-    assert c.near_plane >= 0.01 and c.far_plane >= 100.0
+    assert c.near_plane >= np.float32(0.01) and c.far_plane >= 100.0
     ctx.render()
     keys, idx = oracle.preprocess(minimal_ascii_scene(oracle), cam_dict(c))
     assert ctx.stats().visible_count == len(idx) > 0
