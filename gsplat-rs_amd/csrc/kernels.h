@@ -77,10 +77,15 @@ void launch_tile_emit(const uint32_t *sorted, const uint32_t *visible, const uin
 void launch_tile_ranges(const uint32_t *pair_keys, const uint32_t *m_dev, uint32_t capacity, uint32_t tiles,
                         uint2 *ranges, hipStream_t stream);
 
-// Front-to-back blend of each 16x16 tile's list with LDS-staged records.
-void launch_raster(const float4 *rec, const uint2 *ranges, const uint32_t *list, uint32_t width,
-                   uint32_t height, uint32_t tiles_x, uint32_t tiles_y, float4 *out32, uchar4 *out8,
-                   hipStream_t stream);
+// Front-to-back blend of each 16x16 tile's list.
+//   TileLds      one workgroup per tile, batches of 256 records staged in LDS
+//   QuadWave     one wave per 8x8 quadrant, wave-uniform record loads,
+//                quadrant culling by the splat's pixel bounds, per-wave exit
+//   QuadWaveXcd  QuadWave with tiles banded per XCD
+enum class RasterVariant : uint32_t { TileLds = 0, QuadWave = 1, QuadWaveXcd = 2 };
+void launch_raster(RasterVariant variant, const float4 *rec, const uint2 *rect, const uint2 *ranges,
+                   const uint32_t *list, uint32_t width, uint32_t height, uint32_t tiles_x, uint32_t tiles_y,
+                   float4 *out32, uchar4 *out8, hipStream_t stream);
 
 // Device fill of n u32 words (graph-safe replacement of hipMemsetAsync).
 void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t stream);

@@ -261,8 +261,9 @@ __global__ __launch_bounds__(kThreads) void k_project(const float *__restrict__ 
           const float y0f = ceilf(cyp - eyh - 0.5f), y1f = floorf(cyp + eyh - 0.5f);
           const float wm1 = (float)(fp.width - 1), hm1 = (float)(fp.height - 1);
           if (!(x1f < 0.0f || y1f < 0.0f || x0f > wm1 || y0f > hm1 || x0f > x1f || y0f > y1f)) {
-            const uint32_t x0 = (uint32_t)fmaxf(x0f, 0.0f) / kTile, x1 = (uint32_t)fminf(x1f, wm1) / kTile;
-            const uint32_t y0 = (uint32_t)fmaxf(y0f, 0.0f) / kTile, y1 = (uint32_t)fminf(y1f, hm1) / kTile;
+            // Pixel bounds (the tile rectangle is these divided by kTile).
+            const uint32_t x0 = (uint32_t)fmaxf(x0f, 0.0f), x1 = (uint32_t)fminf(x1f, wm1);
+            const uint32_t y0 = (uint32_t)fmaxf(y0f, 0.0f), y1 = (uint32_t)fminf(y1f, hm1);
             rr = make_uint2(x0 | (y0 << 16), x1 | (y1 << 16));
           }
         }
@@ -343,9 +344,11 @@ __global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t *__restrict__ ou
 
 // ------------------------------------------------------------- tile binning
 
+// rect holds the pixel bounds x0|y0<<16, x1|y1<<16 (x = ~0u: no pixel).
 __device__ __forceinline__ uint32_t rect_tiles(uint2 r) {
   if (r.x == 0xFFFFFFFFu) return 0u;
-  const uint32_t x0 = r.x & 0xFFFFu, y0 = r.x >> 16, x1 = r.y & 0xFFFFu, y1 = r.y >> 16;
+  const uint32_t x0 = (r.x & 0xFFFFu) / kTile, y0 = (r.x >> 16) / kTile;
+  const uint32_t x1 = (r.y & 0xFFFFu) / kTile, y1 = (r.y >> 16) / kTile;
   return (x1 - x0 + 1) * (y1 - y0 + 1);
 }
 
@@ -383,7 +386,8 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const uint32_t *__restri
   const uint32_t s = sorted[v - 1 - j];
   const uint2 r = rect[s];
   if (r.x == 0xFFFFFFFFu) return;
-  const uint32_t x0 = r.x & 0xFFFFu, y0 = r.x >> 16, x1 = r.y & 0xFFFFu, y1 = r.y >> 16;
+  const uint32_t x0 = (r.x & 0xFFFFu) / kTile, y0 = (r.x >> 16) / kTile;
+  const uint32_t x1 = (r.y & 0xFFFFu) / kTile, y1 = (r.y >> 16) / kTile;
   uint32_t o = offsets[j];
   for (uint32_t ty = y0; ty <= y1; ++ty)
     for (uint32_t tx = x0; tx <= x1; ++tx) {
@@ -467,6 +471,81 @@ __global__ __launch_bounds__(kThreads) void k_raster(const float4 *__restrict__ 
           }
         }
       }
+    }
+  }
+  if (inside) {
+    const size_t o = (size_t)y * width + x;
+    const float A = 1.0f - T;
+    out32[o] = make_float4(cr, cg, cb, A);
+    auto q = [](float v) { return (unsigned char)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f); };
+    out8[o] = make_uchar4(q(cr), q(cg), q(cb), q(A));
+  }
+}
+
+// One wave per 8x8 quadrant of a 16x16 tile, one workgroup per wave. The
+// wave walks its tile's list on its own: 64 list entries per vector load,
+// then per entry wave-uniform loads of the splat's pixel bounds and record;
+// a splat whose bounds miss the quadrant is skipped by the whole wave, and
+// the wave leaves as soon as all 64 of its pixels have T < 1e-4.
+// Workgroup b runs on XCD b mod 8 (round-robin dispatch); XCD_MAP gives each
+// XCD one contiguous band of tiles and keeps a tile's four quadrants on the
+// same XCD, so a tile's list and records are fetched into one L2.
+template <bool XCD_MAP>
+__global__ __launch_bounds__(kWave) void k_raster_wave(const float4 *__restrict__ rec, const uint2 *__restrict__ rect,
+                                                       const uint2 *__restrict__ ranges,
+                                                       const uint32_t *__restrict__ list, uint32_t width,
+                                                       uint32_t height, uint32_t tiles_x, uint32_t tiles,
+                                                       float4 *__restrict__ out32, uchar4 *__restrict__ out8) {
+  uint32_t tile, quad;
+  if (XCD_MAP) {
+    const uint32_t band = (tiles + 7) / 8;
+    const uint32_t k = blockIdx.x / 8;
+    tile = (blockIdx.x % 8) * band + k / 4;
+    quad = k % 4;
+  } else {
+    tile = blockIdx.x / 4;
+    quad = blockIdx.x % 4;
+  }
+  if (tile >= tiles) return;
+  const uint32_t lane = threadIdx.x;
+  const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+  const uint32_t qx0 = tx * kTile + (quad & 1u) * 8u, qy0 = ty * kTile + (quad >> 1) * 8u;
+  const uint32_t x = qx0 + (lane & 7u), y = qy0 + (lane >> 3);
+  const bool inside = x < width && y < height;
+  const float pnx = (2.0f * ((float)x + 0.5f)) / (float)width - 1.0f;
+  const float pny = 1.0f - (2.0f * ((float)y + 0.5f)) / (float)height;
+  const uint2 rg = ranges[tile];
+  float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+  bool done = !inside;
+  for (uint32_t base = rg.x; base < rg.y; base += kWave) {
+    if (__ballot(!done) == 0) break;
+    const uint32_t cnt = min(kWave, rg.y - base);
+    const uint32_t mine = lane < cnt ? list[base + lane] : 0u;
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t s = __builtin_amdgcn_readlane(mine, j);
+      const uint2 r = rect[s];
+      const uint32_t bx0 = r.x & 0xFFFFu, by0 = r.x >> 16, bx1 = r.y & 0xFFFFu, by1 = r.y >> 16;
+      if (bx1 < qx0 || bx0 > qx0 + 7u || by1 < qy0 || by0 > qy0 + 7u) continue;
+      const float4 a = rec[3 * (size_t)s + 0];
+      const float4 b = rec[3 * (size_t)s + 1];
+      if (!done) {
+        const float dx = pnx - a.x;
+        const float dy = pny - a.y;
+        const float lx = fmaf(a.w, dy, a.z * dx);
+        const float ly = fmaf(b.y, dy, b.x * dx);
+        const float r2 = fmaf(ly, ly, lx * lx);
+        if (r2 <= b.z) {
+          const float g = __expf(-4.5f * r2);
+          const float4 c = rec[3 * (size_t)s + 2];
+          const float wgt = T * g;
+          cr = fmaf(wgt, c.x, cr);
+          cg = fmaf(wgt, c.y, cg);
+          cb = fmaf(wgt, c.z, cb);
+          T = T * (1.0f - b.w * g);
+          done = T < 1e-4f;
+        }
+      }
+      if (__ballot(!done) == 0) break;
     }
   }
   if (inside) {
@@ -586,11 +665,24 @@ void launch_tile_ranges(const uint32_t *pair_keys, const uint32_t *m_dev, uint32
   GS_HIP(hipGetLastError());
 }
 
-void launch_raster(const float4 *rec, const uint2 *ranges, const uint32_t *list, uint32_t width, uint32_t height,
-                   uint32_t tiles_x, uint32_t tiles_y, float4 *out32, uchar4 *out8, hipStream_t st) {
+void launch_raster(RasterVariant variant, const float4 *rec, const uint2 *rect, const uint2 *ranges,
+                   const uint32_t *list, uint32_t width, uint32_t height, uint32_t tiles_x, uint32_t tiles_y,
+                   float4 *out32, uchar4 *out8, hipStream_t st) {
   const uint32_t tiles = tiles_x * tiles_y;
   if (tiles == 0) return;
-  k_raster<<<tiles, kThreads, 0, st>>>(rec, ranges, list, width, height, tiles_x, out32, out8);
+  switch (variant) {
+    case RasterVariant::TileLds:
+      k_raster<<<tiles, kThreads, 0, st>>>(rec, ranges, list, width, height, tiles_x, out32, out8);
+      break;
+    case RasterVariant::QuadWave:
+      k_raster_wave<false><<<4 * tiles, kWave, 0, st>>>(rec, rect, ranges, list, width, height, tiles_x, tiles,
+                                                        out32, out8);
+      break;
+    case RasterVariant::QuadWaveXcd:
+      k_raster_wave<true><<<32 * ((tiles + 7) / 8), kWave, 0, st>>>(rec, rect, ranges, list, width, height,
+                                                                     tiles_x, tiles, out32, out8);
+      break;
+  }
   GS_HIP(hipGetLastError());
 }
 

@@ -16,6 +16,7 @@
 #include "sort.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -88,6 +89,7 @@ Renderer::Renderer(uint32_t width, uint32_t height) : width_(width), height_(hei
   w_.total.ensure(1);
   w_.m_dev.ensure(1);
   w_.info.ensure(4);
+  if (const char *v = std::getenv("GSPLAT_RASTER_VARIANT")) raster_variant_ = (RasterVariant)(std::atoi(v) % 3);
   set_size(width, height);
 }
 
@@ -187,8 +189,8 @@ void Renderer::enqueue_frame(const GsplatCamera &cam) {
   launch_tile_ranges(pk, w_.m_dev.get(), w_.pair_cap, tiles, w_.ranges.get(), st);
   GS_HIP(hipEventRecord(ev_[3], st));
 
-  launch_raster(w_.rec.get(), w_.ranges.get(), pv, width_, height_, fp.tiles_x, fp.tiles_y, w_.img32.get(),
-                w_.img8.get(), st);
+  launch_raster(raster_variant_, w_.rec.get(), w_.rect.get(), w_.ranges.get(), pv, width_, height_, fp.tiles_x,
+                fp.tiles_y, w_.img32.get(), w_.img8.get(), st);
   GS_HIP(hipEventRecord(ev_[4], st));
   GS_HIP(hipMemcpyAsync(pinned_, w_.info.get(), 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   GS_HIP(hipEventRecord(ev_[5], st));

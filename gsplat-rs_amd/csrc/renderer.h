@@ -113,6 +113,7 @@ class Renderer {
   float host_ms_ = 0.0f;
   GsplatStats last_stats_{};
   GsplatStageTimes last_times_{};
+  RasterVariant raster_variant_ = RasterVariant::TileLds;
 };
 
 bool device_present();

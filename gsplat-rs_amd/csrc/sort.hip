@@ -1,18 +1,26 @@
 // sort.hip -- onesweep LSD radix sort of (u32 key, u32 value) pairs for gfx950.
 //
-// One kernel builds the digit histograms of every pass in a single read of
-// the keys; then one kernel per 8-bit digit pass ranks each 4096-key tile
-// with wave64 ballots (match-any over the 8 digit bits + popcount), publishes
-// the tile's digit counts, resolves its global digit offsets by decoupled
-// look-back over the preceding tiles, stages the tile in LDS in digit order
-// and writes it out coalesced. The sort is stable, so with keys complemented
-// (descending) and values in ascending index order it yields the reference's
-// back-to-front order: key descending, ties by ascending index
-// (CpuSortBackend::sort_values_by_keys, sort/src/lib.rs:36-61, radix.rs:21-58).
+//   k_os_hist    one read of the keys: per-workgroup digit histograms of
+//                every pass (wave64 ballot match + LDS atomics), no global
+//                atomics; also clears the look-back state of the passes
+//   k_os_reduce  sums the per-workgroup histograms
+//   k_os_pass    one launch per 8-bit digit: each 4096-key tile is ranked by
+//                its four waves independently (a wave owns 1024 consecutive
+//                keys and a private LDS histogram; ballot match-any over the
+//                8 digit bits + popcount gives the rank among equal digits),
+//                the tile publishes its digit counts, resolves its global
+//                offsets by decoupled look-back over the preceding tiles
+//                (eight predecessors polled per step), stages the tile in LDS
+//                in digit order and writes it out coalesced
+//
+// The sort is stable, so with keys complemented (descending) and values in
+// ascending index order it yields the reference's back-to-front order: key
+// descending, ties by ascending index (CpuSortBackend::sort_values_by_keys,
+// sort/src/lib.rs:36-61, radix.rs:21-58).
 //
 // Forward progress of the look-back: tiles take their index from a per-pass
-// atomic counter when they start, so every tile a block waits on has already
-// started and finishes without waiting on later tiles.
+// atomic counter when they start, so every tile a workgroup waits on has
+// already started and finishes without waiting on later tiles.
 
 #include <hip/hip_runtime.h>
 
@@ -37,6 +45,9 @@ constexpr uint32_t T = kSortThreads;
 constexpr uint32_t IPT = kSortItems;
 constexpr uint32_t TILE = kSortTile;
 constexpr uint32_t WAVES = T / 64;
+constexpr uint32_t WAVE_ITEMS = TILE / WAVES;
+constexpr uint32_t HIST_BLOCKS = 128;
+constexpr uint32_t LOOKBACK = 8;
 constexpr uint32_t FLAG_AGG = 1u << 30;  // tile count published
 constexpr uint32_t FLAG_INC = 2u << 30;  // inclusive prefix published
 constexpr uint32_t VALUE_MASK = (1u << 30) - 1;
@@ -83,17 +94,12 @@ __device__ __forceinline__ uint32_t load_count(const uint32_t *n_dev, uint32_t n
   return n_dev ? min(*n_dev, n_cap) : n_cap;
 }
 
-__global__ __launch_bounds__(T) void k_os_zero(uint32_t *__restrict__ p, size_t n) {
-  const size_t i = (size_t)blockIdx.x * T + threadIdx.x;
-  if (i < n) p[i] = 0u;
-}
-
-// Histograms of all passes in one read: LDS bins per pass, ballot-aggregated
-// LDS atomics, one global atomic per (block, pass, digit).
 __global__ __launch_bounds__(T) void k_os_hist(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ n_dev,
                                                uint32_t n_cap, uint32_t passes, uint32_t flip,
-                                               uint32_t *__restrict__ hist) {
+                                               uint32_t *__restrict__ partial, uint32_t *__restrict__ clear,
+                                               size_t clear_words) {
   __shared__ uint32_t h[4][256];
+  for (size_t i = (size_t)blockIdx.x * T + threadIdx.x; i < clear_words; i += (size_t)gridDim.x * T) clear[i] = 0u;
   for (uint32_t p = 0; p < 4; ++p) h[p][threadIdx.x] = 0;
   __syncthreads();
   const uint32_t n = load_count(n_dev, n_cap);
@@ -108,9 +114,16 @@ __global__ __launch_bounds__(T) void k_os_hist(const uint32_t *__restrict__ keys
     }
   }
   __syncthreads();
+  for (uint32_t p = 0; p < passes; ++p)
+    partial[((size_t)blockIdx.x * passes + p) * 256 + threadIdx.x] = h[p][threadIdx.x];
+}
+
+__global__ __launch_bounds__(T) void k_os_reduce(const uint32_t *__restrict__ partial, uint32_t blocks,
+                                                 uint32_t passes, uint32_t *__restrict__ hist) {
   for (uint32_t p = 0; p < passes; ++p) {
-    const uint32_t c = h[p][threadIdx.x];
-    if (c) atomicAdd(&hist[p * 256 + threadIdx.x], c);
+    uint32_t s = 0;
+    for (uint32_t b = 0; b < blocks; ++b) s += partial[((size_t)b * passes + p) * 256 + threadIdx.x];
+    hist[p * 256 + threadIdx.x] = s;
   }
 }
 
@@ -124,16 +137,18 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
   __shared__ uint32_t gofs[256];    // global start of each digit
   __shared__ uint32_t lstart[256];  // start of each digit inside this tile
   __shared__ uint32_t prefix[256];  // count of each digit in earlier tiles
-  __shared__ uint32_t wcnt[WAVES][256];
+  __shared__ uint32_t whist[WAVES][256];
   __shared__ uint32_t skeys[TILE];
   __shared__ uint32_t svals[TILE];
 
-  const uint32_t t = threadIdx.x, w = t / 64;
+  const uint32_t t = threadIdx.x, w = t / 64, lane = __lane_id();
   const uint32_t n = load_count(n_dev, n_cap);
   if (t == 0) {
     s_tile = atomicAdd(counter, 1u);
     s_trivial = 0;
   }
+#pragma unroll
+  for (uint32_t q = 0; q < WAVES; ++q) whist[q][t] = 0;
   __syncthreads();
   const uint32_t tile = s_tile;
   const size_t base = (size_t)tile * TILE;
@@ -153,34 +168,32 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
     return;
   }
 
-  // Rank: item order is (round, wave, lane) = global position order.
+  // Wave-private ranking: wave w owns tile items [w*1024, (w+1)*1024) in
+  // order, 64 per round; within a round the lanes of equal digit are ranked
+  // by lane, and the running per-digit count lives in whist[w].
   uint32_t key[IPT], val[IPT], rank[IPT];
-  uint32_t run = 0;  // running count of digit t in this tile
+  uint32_t *wh = whist[w];
 #pragma unroll
   for (uint32_t r = 0; r < IPT; ++r) {
-#pragma unroll
-    for (uint32_t q = 0; q < WAVES; ++q) wcnt[q][t] = 0;
-    __syncthreads();
-    const uint32_t j = r * T + t;
+    const uint32_t j = w * WAVE_ITEMS + r * 64 + lane;
     const bool active = j < count;
     key[r] = active ? kin[base + j] : 0u;
     val[r] = active ? vin[base + j] : 0u;
     const uint32_t d = ((key[r] ^ flip) >> shift) & 0xFFu;
     const uint64_t peers = match_digit(d, active);
     const uint32_t lrank = (uint32_t)__popcll(peers & lanemask_lt());
-    if (active && lrank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    uint32_t s = run;
+    const uint32_t before = active ? wh[d] : 0u;
+    rank[r] = active ? before + lrank : 0xFFFFFFFFu;
+    if (active && lrank == 0) wh[d] = before + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  // Thread t owns digit t: turn the four wave counts into wave offsets.
+  uint32_t run = 0;
 #pragma unroll
-    for (uint32_t q = 0; q < WAVES; ++q) {
-      const uint32_t c = wcnt[q][t];
-      wcnt[q][t] = s;
-      s += c;
-    }
-    run = s;
-    __syncthreads();
-    rank[r] = active ? wcnt[w][d] + lrank : 0xFFFFFFFFu;
-    __syncthreads();  // wcnt is cleared again by the next round
+  for (uint32_t q = 0; q < WAVES; ++q) {
+    const uint32_t c = whist[q][t];
+    whist[q][t] = run;
+    run += c;
   }
 
   // Publish this tile's count of digit t, then look back for the prefix.
@@ -191,15 +204,29 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
   } else {
     __hip_atomic_store(st, FLAG_AGG | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t excl = 0;
-    int64_t j = (int64_t)tile - 1;
+    int64_t j = (int64_t)tile - 1;  // next predecessor not yet consumed
     while (j >= 0) {
-      const uint32_t sv =
-          __hip_atomic_load(status + (size_t)j * 256 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t flag = sv & ~VALUE_MASK;
-      if (flag == 0) continue;  // predecessor has not published yet
-      excl += sv & VALUE_MASK;
-      if (flag == FLAG_INC) break;
-      --j;
+      uint32_t sv[LOOKBACK];
+#pragma unroll
+      for (uint32_t k = 0; k < LOOKBACK; ++k)
+        sv[k] = (j - (int64_t)k >= 0)
+                    ? __hip_atomic_load(status + (size_t)(j - k) * 256 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                    : FLAG_INC;
+      int64_t consumed = 0;
+      bool finished = false;
+#pragma unroll
+      for (uint32_t k = 0; k < LOOKBACK; ++k) {
+        const uint32_t flag = sv[k] & ~VALUE_MASK;
+        if (flag == 0) break;  // not published yet: poll again from here
+        excl += sv[k] & VALUE_MASK;
+        ++consumed;
+        if (flag == FLAG_INC) {
+          finished = true;
+          break;
+        }
+      }
+      if (finished) break;
+      j -= consumed;
     }
     __hip_atomic_store(st, FLAG_INC | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     prefix[t] = excl;
@@ -212,7 +239,7 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
   for (uint32_t r = 0; r < IPT; ++r) {
     if (rank[r] != 0xFFFFFFFFu) {
       const uint32_t d = ((key[r] ^ flip) >> shift) & 0xFFu;
-      const uint32_t pos = lstart[d] + rank[r];
+      const uint32_t pos = lstart[d] + whist[w][d] + rank[r];
       skeys[pos] = key[r];
       svals[pos] = val[r];
     }
@@ -229,11 +256,16 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
 
 uint32_t div_up(size_t a, size_t b) { return (uint32_t)((a + b - 1) / b); }
 
+uint32_t hist_blocks(uint32_t n_cap) { return std::max<uint32_t>(std::min<uint32_t>(div_up(n_cap, TILE), HIST_BLOCKS), 1); }
+
 }  // namespace
 
+// Layout: hist [4][256] | partial [hist_blocks][passes][256] | counters [4] |
+// status [passes][tiles][256]; counters and status are cleared per sort.
 size_t onesweep_status_words(uint32_t n_cap, uint32_t bits) {
   const uint32_t passes = std::max<uint32_t>((bits + 7) / 8, 1);
-  return 256 * 4 + 4 + (size_t)passes * std::max<uint32_t>(div_up(n_cap, TILE), 1) * 256;
+  const size_t tiles = std::max<uint32_t>(div_up(n_cap, TILE), 1);
+  return 1024 + (size_t)hist_blocks(n_cap) * passes * 256 + 4 + (size_t)passes * tiles * 256;
 }
 
 void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n_cap,
@@ -243,15 +275,16 @@ void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32
   if (n_cap > (1u << 30)) throw Error(Code::Unsupported, "sort length exceeds 2^30");
   const uint32_t passes = std::min<uint32_t>((bits + 7) / 8, 4);
   const uint32_t tiles = div_up(n_cap, TILE);
-  const size_t need = onesweep_status_words(n_cap, bits);
-  if (need > scratch_words) throw Error(Code::Internal, "onesweep scratch too small");
-  uint32_t *hist = scratch;             // [4][256]
-  uint32_t *counters = scratch + 1024;  // [4]
-  uint32_t *status = scratch + 1028;    // [passes][tiles][256]
+  const uint32_t hb = hist_blocks(n_cap);
+  if (onesweep_status_words(n_cap, bits) > scratch_words) throw Error(Code::Internal, "onesweep scratch too small");
+  uint32_t *hist = scratch;
+  uint32_t *partial = hist + 1024;
+  uint32_t *counters = partial + (size_t)hb * passes * 256;
+  uint32_t *status = counters + 4;
+  const size_t clear_words = 4 + (size_t)passes * tiles * 256;
   const uint32_t flip = descending ? 0xFFFFFFFFu : 0u;
-  k_os_zero<<<div_up(need, T), T, 0, st>>>(scratch, need);
-  const uint32_t hist_blocks = std::min<uint32_t>(tiles, 1024);
-  k_os_hist<<<hist_blocks, T, 0, st>>>(*keys, n_dev, n_cap, passes, flip, hist);
+  k_os_hist<<<hb, T, 0, st>>>(*keys, n_dev, n_cap, passes, flip, partial, counters, clear_words);
+  k_os_reduce<<<1, T, 0, st>>>(partial, hb, passes, hist);
   for (uint32_t p = 0; p < passes; ++p) {
     k_os_pass<<<tiles, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, p, flip, hist, counters + p,
                                    status + (size_t)p * tiles * 256);

@@ -226,3 +226,18 @@ def test_gpu_auto_camera_frames_scene(gpu, oracle):
     ctx.render()
     keys, idx = oracle.preprocess(minimal_ascii_scene(oracle), cam_dict(c))
     assert ctx.stats().visible_count == len(idx) > 0
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_gpu_raster_variants_match_oracle(gpu, oracle, monkeypatch, variant):
+    # Every raster variant (LDS tile, wave quadrant, wave quadrant banded by
+    # XCD) blends the same lists to the same tolerance.
+    monkeypatch.setenv("GSPLAT_RASTER_VARIANT", str(variant))
+    s = synth.tiny_scene(6000, 77, sh_degree=3, spread=0.6, log_scale=-3.2)
+    cam = gpu.look_at((0.2, -0.3, -2.2), (0.0, 0.0, 0.0))
+    w, h = 333, 217
+    say(f"[raster variant {variant}]")
+    ctx = render(gpu, s, cam, w, h)
+    ref, _ = oracle.render(oracle_scene(oracle, s), cam_dict(cam), w, h)
+    check_image(ctx.rgba32f(), ctx.rgba8(), ref, f"variant-{variant}")
+    ctx.close()
