@@ -445,7 +445,7 @@ int32_t gsplat_sort_values_by_keys(const uint32_t *keys, uint32_t *values, uint6
     v.ensure(n);
     ka.ensure(n);
     va.ensure(n);
-    scratch.ensure(gs::onesweep_status_words(n, 32));
+    scratch.ensure_zeroed(gs::onesweep_status_words(n, 32));
     if (hipMemcpyAsync(k.get(), keys, len * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(v.get(), values, len * 4, hipMemcpyHostToDevice, st) != hipSuccess)
       return fail(gs::Code::Internal, std::string(op) + ": upload failed");

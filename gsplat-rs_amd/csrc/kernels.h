@@ -78,10 +78,11 @@ void launch_tile_ranges(const uint32_t *pair_keys, const uint32_t *m_dev, uint32
                         uint2 *ranges, hipStream_t stream);
 
 // Front-to-back blend of each 16x16 tile's list.
-//   TileLds      one workgroup per tile, batches of 256 records staged in LDS
-//   QuadWave     one wave per 8x8 quadrant, wave-uniform record loads,
-//                quadrant culling by the splat's pixel bounds, per-wave exit
-//   QuadWaveXcd  QuadWave with tiles banded per XCD
+// All variants: one workgroup per tile, batches of 256 records staged in LDS.
+//   TileLds      thread = pixel in row-major tile order
+//   QuadWave     each wave owns an 8x8 quadrant and skips splats whose pixel
+//                bounds miss it
+//   QuadWaveXcd  QuadWave with the tiles banded per XCD
 enum class RasterVariant : uint32_t { TileLds = 0, QuadWave = 1, QuadWaveXcd = 2 };
 void launch_raster(RasterVariant variant, const float4 *rec, const uint2 *rect, const uint2 *ranges,
                    const uint32_t *list, uint32_t width, uint32_t height, uint32_t tiles_x, uint32_t tiles_y,

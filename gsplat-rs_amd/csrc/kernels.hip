@@ -417,20 +417,43 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
 // ------------------------------------------------------------------ raster
 
 // One workgroup per 16x16 tile, one thread per pixel. Batches of 256 records
-// are gathered into LDS and blended front to back: C += T*g*rgb_a,
-// T *= 1 - a*g, stopping when T < 1e-4 (equivalent to the reference's
-// back-to-front premultiplied "over", draw_pass.rs:75, up to that bound).
-// Coverage is r^2 <= cut (the fragment test of wgsl:358-367 precomputed per
-// splat); g = exp(-4.5 r^2) (wgsl:363).
-__global__ __launch_bounds__(kThreads) void k_raster(const float4 *__restrict__ rec,
+// are gathered into LDS (coalesced by the 256 threads) and blended front to
+// back: C += T*g*rgb_a, T *= 1 - a*g, stopping when T < 1e-4 (equivalent to
+// the reference's back-to-front premultiplied "over", draw_pass.rs:75, up to
+// that bound). Coverage is r^2 <= cut (the fragment test of wgsl:358-367
+// precomputed per splat); g = exp(-4.5 r^2) (wgsl:363).
+//   QUAD  each wave owns an 8x8 quadrant of the tile; the splat's pixel
+//         bounds are staged with its record and a wave skips, as a whole,
+//         every splat whose bounds miss its quadrant
+//   XCD   workgroup b runs on XCD b mod 8 (round-robin dispatch); tiles are
+//         banded so that each XCD renders one contiguous range of tiles and
+//         neighbouring tiles share that XCD's L2
+template <bool QUAD, bool XCD>
+__global__ __launch_bounds__(kThreads) void k_raster(const float4 *__restrict__ rec, const uint2 *__restrict__ rect,
                                                      const uint2 *__restrict__ ranges,
                                                      const uint32_t *__restrict__ list, uint32_t width,
-                                                     uint32_t height, uint32_t tiles_x, float4 *__restrict__ out32,
-                                                     uchar4 *__restrict__ out8) {
+                                                     uint32_t height, uint32_t tiles_x, uint32_t tiles,
+                                                     float4 *__restrict__ out32, uchar4 *__restrict__ out8) {
   __shared__ float4 s0[kThreads], s1[kThreads], s2[kThreads];
-  const uint32_t tile = blockIdx.x;
+  __shared__ uint2 sb[QUAD ? kThreads : 1];
+  uint32_t tile = blockIdx.x;
+  if (XCD) {
+    const uint32_t band = (tiles + 7) / 8;
+    tile = (blockIdx.x % 8) * band + blockIdx.x / 8;
+  }
+  if (tile >= tiles) return;
   const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
-  const uint32_t x = tx * kTile + (threadIdx.x % kTile), y = ty * kTile + (threadIdx.x / kTile);
+  const uint32_t t = threadIdx.x, lane = t % kWave, w = t / kWave;
+  uint32_t x, y, qx0 = 0, qy0 = 0;
+  if (QUAD) {
+    qx0 = tx * kTile + (w & 1u) * 8u;
+    qy0 = ty * kTile + (w >> 1) * 8u;
+    x = qx0 + (lane & 7u);
+    y = qy0 + (lane >> 3);
+  } else {
+    x = tx * kTile + (t % kTile);
+    y = ty * kTile + (t / kTile);
+  }
   const bool inside = x < width && y < height;
   const float pnx = (2.0f * ((float)x + 0.5f)) / (float)width - 1.0f;
   const float pny = 1.0f - (2.0f * ((float)y + 0.5f)) / (float)height;
@@ -439,17 +462,23 @@ __global__ __launch_bounds__(kThreads) void k_raster(const float4 *__restrict__ 
   bool done = !inside;
   for (uint32_t start = rg.x; start < rg.y; start += kThreads) {
     if (__syncthreads_count(!done) == 0) break;
-    const uint32_t k = start + threadIdx.x;
+    const uint32_t k = start + t;
     if (k < rg.y) {
       const size_t s = list[k];
-      s0[threadIdx.x] = rec[3 * s + 0];
-      s1[threadIdx.x] = rec[3 * s + 1];
-      s2[threadIdx.x] = rec[3 * s + 2];
+      s0[t] = rec[3 * s + 0];
+      s1[t] = rec[3 * s + 1];
+      s2[t] = rec[3 * s + 2];
+      if (QUAD) sb[t] = rect[s];
     }
     __syncthreads();
     const uint32_t cnt = min(kThreads, rg.y - start);
     if (!done) {
       for (uint32_t j = 0; j < cnt; ++j) {
+        if (QUAD) {
+          const uint2 r = sb[j];
+          if ((r.y & 0xFFFFu) < qx0 || (r.x & 0xFFFFu) > qx0 + 7u || (r.y >> 16) < qy0 || (r.x >> 16) > qy0 + 7u)
+            continue;
+        }
         const float4 a = s0[j];
         const float4 b = s1[j];
         const float dx = pnx - a.x;
@@ -471,81 +500,6 @@ __global__ __launch_bounds__(kThreads) void k_raster(const float4 *__restrict__ 
           }
         }
       }
-    }
-  }
-  if (inside) {
-    const size_t o = (size_t)y * width + x;
-    const float A = 1.0f - T;
-    out32[o] = make_float4(cr, cg, cb, A);
-    auto q = [](float v) { return (unsigned char)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f); };
-    out8[o] = make_uchar4(q(cr), q(cg), q(cb), q(A));
-  }
-}
-
-// One wave per 8x8 quadrant of a 16x16 tile, one workgroup per wave. The
-// wave walks its tile's list on its own: 64 list entries per vector load,
-// then per entry wave-uniform loads of the splat's pixel bounds and record;
-// a splat whose bounds miss the quadrant is skipped by the whole wave, and
-// the wave leaves as soon as all 64 of its pixels have T < 1e-4.
-// Workgroup b runs on XCD b mod 8 (round-robin dispatch); XCD_MAP gives each
-// XCD one contiguous band of tiles and keeps a tile's four quadrants on the
-// same XCD, so a tile's list and records are fetched into one L2.
-template <bool XCD_MAP>
-__global__ __launch_bounds__(kWave) void k_raster_wave(const float4 *__restrict__ rec, const uint2 *__restrict__ rect,
-                                                       const uint2 *__restrict__ ranges,
-                                                       const uint32_t *__restrict__ list, uint32_t width,
-                                                       uint32_t height, uint32_t tiles_x, uint32_t tiles,
-                                                       float4 *__restrict__ out32, uchar4 *__restrict__ out8) {
-  uint32_t tile, quad;
-  if (XCD_MAP) {
-    const uint32_t band = (tiles + 7) / 8;
-    const uint32_t k = blockIdx.x / 8;
-    tile = (blockIdx.x % 8) * band + k / 4;
-    quad = k % 4;
-  } else {
-    tile = blockIdx.x / 4;
-    quad = blockIdx.x % 4;
-  }
-  if (tile >= tiles) return;
-  const uint32_t lane = threadIdx.x;
-  const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
-  const uint32_t qx0 = tx * kTile + (quad & 1u) * 8u, qy0 = ty * kTile + (quad >> 1) * 8u;
-  const uint32_t x = qx0 + (lane & 7u), y = qy0 + (lane >> 3);
-  const bool inside = x < width && y < height;
-  const float pnx = (2.0f * ((float)x + 0.5f)) / (float)width - 1.0f;
-  const float pny = 1.0f - (2.0f * ((float)y + 0.5f)) / (float)height;
-  const uint2 rg = ranges[tile];
-  float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
-  bool done = !inside;
-  for (uint32_t base = rg.x; base < rg.y; base += kWave) {
-    if (__ballot(!done) == 0) break;
-    const uint32_t cnt = min(kWave, rg.y - base);
-    const uint32_t mine = lane < cnt ? list[base + lane] : 0u;
-    for (uint32_t j = 0; j < cnt; ++j) {
-      const uint32_t s = __builtin_amdgcn_readlane(mine, j);
-      const uint2 r = rect[s];
-      const uint32_t bx0 = r.x & 0xFFFFu, by0 = r.x >> 16, bx1 = r.y & 0xFFFFu, by1 = r.y >> 16;
-      if (bx1 < qx0 || bx0 > qx0 + 7u || by1 < qy0 || by0 > qy0 + 7u) continue;
-      const float4 a = rec[3 * (size_t)s + 0];
-      const float4 b = rec[3 * (size_t)s + 1];
-      if (!done) {
-        const float dx = pnx - a.x;
-        const float dy = pny - a.y;
-        const float lx = fmaf(a.w, dy, a.z * dx);
-        const float ly = fmaf(b.y, dy, b.x * dx);
-        const float r2 = fmaf(ly, ly, lx * lx);
-        if (r2 <= b.z) {
-          const float g = __expf(-4.5f * r2);
-          const float4 c = rec[3 * (size_t)s + 2];
-          const float wgt = T * g;
-          cr = fmaf(wgt, c.x, cr);
-          cg = fmaf(wgt, c.y, cg);
-          cb = fmaf(wgt, c.z, cb);
-          T = T * (1.0f - b.w * g);
-          done = T < 1e-4f;
-        }
-      }
-      if (__ballot(!done) == 0) break;
     }
   }
   if (inside) {
@@ -672,15 +626,16 @@ void launch_raster(RasterVariant variant, const float4 *rec, const uint2 *rect, 
   if (tiles == 0) return;
   switch (variant) {
     case RasterVariant::TileLds:
-      k_raster<<<tiles, kThreads, 0, st>>>(rec, ranges, list, width, height, tiles_x, out32, out8);
+      k_raster<false, false><<<tiles, kThreads, 0, st>>>(rec, rect, ranges, list, width, height, tiles_x, tiles,
+                                                         out32, out8);
       break;
     case RasterVariant::QuadWave:
-      k_raster_wave<false><<<4 * tiles, kWave, 0, st>>>(rec, rect, ranges, list, width, height, tiles_x, tiles,
+      k_raster<true, false><<<tiles, kThreads, 0, st>>>(rec, rect, ranges, list, width, height, tiles_x, tiles,
                                                         out32, out8);
       break;
     case RasterVariant::QuadWaveXcd:
-      k_raster_wave<true><<<32 * ((tiles + 7) / 8), kWave, 0, st>>>(rec, rect, ranges, list, width, height,
-                                                                     tiles_x, tiles, out32, out8);
+      k_raster<true, true><<<8 * ((tiles + 7) / 8), kThreads, 0, st>>>(rec, rect, ranges, list, width, height,
+                                                                        tiles_x, tiles, out32, out8);
       break;
   }
   GS_HIP(hipGetLastError());

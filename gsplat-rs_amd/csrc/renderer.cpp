@@ -46,6 +46,13 @@ void DevBuf<T>::ensure(size_t count) {
 }
 
 template <typename T>
+void DevBuf<T>::ensure_zeroed(size_t count) {
+  if (count <= cap_) return;
+  ensure(count);
+  GS_HIP(hipMemset(ptr_, 0, cap_ * sizeof(T)));
+}
+
+template <typename T>
 void DevBuf<T>::release() {
   // Work queued on any stream may still use the old allocation.
   if (ptr_) (void)hipDeviceSynchronize();
@@ -130,7 +137,7 @@ void Renderer::ensure_frame_buffers(uint32_t n) {
   w_.counts.ensure(m);
   w_.offsets.ensure(m);
   w_.partials.ensure(scan_partial_entries(m));
-  w_.sort_scratch.ensure(onesweep_status_words((uint32_t)m, 32));
+  w_.sort_scratch.ensure_zeroed(onesweep_status_words((uint32_t)m, 32));
 }
 
 void Renderer::ensure_target() {
@@ -146,7 +153,7 @@ void Renderer::ensure_pairs(uint32_t capacity, uint32_t tile_bits) {
   w_.pair_vals.ensure(capacity);
   w_.pair_keys_alt.ensure(capacity);
   w_.pair_vals_alt.ensure(capacity);
-  w_.sort_scratch.ensure(onesweep_status_words(capacity, tile_bits));
+  w_.sort_scratch.ensure_zeroed(onesweep_status_words(capacity, tile_bits));
 }
 
 void Renderer::enqueue_frame(const GsplatCamera &cam) {

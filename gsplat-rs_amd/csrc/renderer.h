@@ -22,6 +22,8 @@ class DevBuf {
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
   void ensure(size_t count);
+  // As ensure(), clearing the allocation to zero whenever it is (re)made.
+  void ensure_zeroed(size_t count);
   void release();
   T *get() const { return ptr_; }
   size_t capacity() const { return cap_; }

@@ -94,37 +94,62 @@ __device__ __forceinline__ uint32_t load_count(const uint32_t *n_dev, uint32_t n
   return n_dev ? min(*n_dev, n_cap) : n_cap;
 }
 
+// Each workgroup takes tiles of 4096 keys (16 per thread, all loaded before
+// any is counted), accumulates LDS histograms of every pass, and writes them
+// as its partial; the last workgroup to finish sums the partials (the
+// `arrive` counter resets itself for the next sort).
 __global__ __launch_bounds__(T) void k_os_hist(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ n_dev,
                                                uint32_t n_cap, uint32_t passes, uint32_t flip,
-                                               uint32_t *__restrict__ partial, uint32_t *__restrict__ clear,
+                                               uint32_t *__restrict__ partial, uint32_t *__restrict__ hist,
+                                               uint32_t *__restrict__ arrive, uint32_t *__restrict__ clear,
                                                size_t clear_words) {
   __shared__ uint32_t h[4][256];
+  __shared__ uint32_t s_last;
   for (size_t i = (size_t)blockIdx.x * T + threadIdx.x; i < clear_words; i += (size_t)gridDim.x * T) clear[i] = 0u;
   for (uint32_t p = 0; p < 4; ++p) h[p][threadIdx.x] = 0;
   __syncthreads();
   const uint32_t n = load_count(n_dev, n_cap);
-  const size_t stride = (size_t)gridDim.x * T;
-  for (size_t i = (size_t)blockIdx.x * T + threadIdx.x; i - threadIdx.x < n; i += stride) {
-    const bool active = i < n;
-    const uint32_t k = active ? (keys[i] ^ flip) : 0u;
-    for (uint32_t p = 0; p < passes; ++p) {
-      const uint32_t d = (k >> (8 * p)) & 0xFFu;
-      const uint64_t peers = match_digit(d, active);
-      if (active && (peers & lanemask_lt()) == 0) atomicAdd(&h[p][d], (uint32_t)__popcll(peers));
+  for (size_t base = (size_t)blockIdx.x * TILE; base < n; base += (size_t)gridDim.x * TILE) {
+    uint32_t k[IPT];
+#pragma unroll
+    for (uint32_t r = 0; r < IPT; ++r) {
+      const size_t i = base + (size_t)r * T + threadIdx.x;
+      k[r] = i < n ? (keys[i] ^ flip) : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < IPT; ++r) {
+      const bool active = base + (size_t)r * T + threadIdx.x < n;
+      for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t d = (k[r] >> (8 * p)) & 0xFFu;
+        const uint64_t peers = match_digit(d, active);
+        if (active && (peers & lanemask_lt()) == 0) atomicAdd(&h[p][d], (uint32_t)__popcll(peers));
+      }
     }
   }
   __syncthreads();
   for (uint32_t p = 0; p < passes; ++p)
     partial[((size_t)blockIdx.x * passes + p) * 256 + threadIdx.x] = h[p][threadIdx.x];
-}
-
-__global__ __launch_bounds__(T) void k_os_reduce(const uint32_t *__restrict__ partial, uint32_t blocks,
-                                                 uint32_t passes, uint32_t *__restrict__ hist) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(arrive, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
   for (uint32_t p = 0; p < passes; ++p) {
-    uint32_t s = 0;
-    for (uint32_t b = 0; b < blocks; ++b) s += partial[((size_t)b * passes + p) * 256 + threadIdx.x];
-    hist[p * 256 + threadIdx.x] = s;
+    uint32_t s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t b = 0;
+    for (; b + 8 <= gridDim.x; b += 8) {
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u)
+        s[u] += __hip_atomic_load(partial + ((size_t)(b + u) * passes + p) * 256 + threadIdx.x, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+    }
+    for (; b < gridDim.x; ++b)
+      s[0] += __hip_atomic_load(partial + ((size_t)b * passes + p) * 256 + threadIdx.x, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+    hist[p * 256 + threadIdx.x] = s[0] + s[1] + s[2] + s[3] + s[4] + s[5] + s[6] + s[7];
   }
+  if (threadIdx.x == 0) *arrive = 0;
 }
 
 __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
@@ -260,12 +285,14 @@ uint32_t hist_blocks(uint32_t n_cap) { return std::max<uint32_t>(std::min<uint32
 
 }  // namespace
 
-// Layout: hist [4][256] | partial [hist_blocks][passes][256] | counters [4] |
-// status [passes][tiles][256]; counters and status are cleared per sort.
+// Layout: arrive [4] | hist [4][256] | partial [hist_blocks][passes][256] |
+// counters [4] | status [passes][tiles][256]. `arrive` must be zero before
+// the first sort (the scratch is cleared when allocated) and resets itself;
+// counters and status are cleared by every sort.
 size_t onesweep_status_words(uint32_t n_cap, uint32_t bits) {
   const uint32_t passes = std::max<uint32_t>((bits + 7) / 8, 1);
   const size_t tiles = std::max<uint32_t>(div_up(n_cap, TILE), 1);
-  return 1024 + (size_t)hist_blocks(n_cap) * passes * 256 + 4 + (size_t)passes * tiles * 256;
+  return 4 + 1024 + (size_t)hist_blocks(n_cap) * passes * 256 + 4 + (size_t)passes * tiles * 256;
 }
 
 void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n_cap,
@@ -277,14 +304,14 @@ void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32
   const uint32_t tiles = div_up(n_cap, TILE);
   const uint32_t hb = hist_blocks(n_cap);
   if (onesweep_status_words(n_cap, bits) > scratch_words) throw Error(Code::Internal, "onesweep scratch too small");
-  uint32_t *hist = scratch;
+  uint32_t *arrive = scratch;
+  uint32_t *hist = arrive + 4;
   uint32_t *partial = hist + 1024;
   uint32_t *counters = partial + (size_t)hb * passes * 256;
   uint32_t *status = counters + 4;
   const size_t clear_words = 4 + (size_t)passes * tiles * 256;
   const uint32_t flip = descending ? 0xFFFFFFFFu : 0u;
-  k_os_hist<<<hb, T, 0, st>>>(*keys, n_dev, n_cap, passes, flip, partial, counters, clear_words);
-  k_os_reduce<<<1, T, 0, st>>>(partial, hb, passes, hist);
+  k_os_hist<<<hb, T, 0, st>>>(*keys, n_dev, n_cap, passes, flip, partial, hist, arrive, counters, clear_words);
   for (uint32_t p = 0; p < passes; ++p) {
     k_os_pass<<<tiles, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, p, flip, hist, counters + p,
                                    status + (size_t)p * tiles * 256);
