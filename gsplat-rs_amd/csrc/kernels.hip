@@ -373,6 +373,11 @@ __global__ void k_pairs_finalize(const uint32_t *__restrict__ visible, const uin
   info[3] = m > capacity ? 1u : 0u;
 }
 
+// Load-balanced expansion: workgroup b owns front-to-back positions
+// [256b, 256b+256), whose pairs are one contiguous range of the output. The
+// positions' first pair index, tile rectangle and splat id are staged in
+// LDS; then each pass writes 256 consecutive pairs, every thread finding its
+// pair's position by binary search over the staged first-pair indices.
 __global__ __launch_bounds__(kThreads) void k_tile_emit(const uint32_t *__restrict__ sorted,
                                                         const uint32_t *__restrict__ visible,
                                                         const uint2 *__restrict__ rect,
@@ -380,22 +385,63 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const uint32_t *__restri
                                                         uint32_t tiles_x, uint32_t capacity,
                                                         uint32_t *__restrict__ pair_keys,
                                                         uint32_t *__restrict__ pair_vals) {
-  const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-  const uint32_t v = *visible;
-  if (j >= n || j >= v) return;
-  const uint32_t s = sorted[v - 1 - j];
-  const uint2 r = rect[s];
-  if (r.x == 0xFFFFFFFFu) return;
-  const uint32_t x0 = (r.x & 0xFFFFu) / kTile, y0 = (r.x >> 16) / kTile;
-  const uint32_t x1 = (r.y & 0xFFFFu) / kTile, y1 = (r.y >> 16) / kTile;
-  uint32_t o = offsets[j];
-  for (uint32_t ty = y0; ty <= y1; ++ty)
-    for (uint32_t tx = x0; tx <= x1; ++tx) {
-      if (o >= capacity) return;
-      pair_keys[o] = ty * tiles_x + tx;
-      pair_vals[o] = s;
-      ++o;
+  __shared__ uint32_t s_first[kThreads + 1];
+  __shared__ uint32_t s_splat[kThreads];
+  __shared__ uint32_t s_x0[kThreads], s_y0[kThreads], s_w[kThreads];
+  const uint32_t v = min(*visible, n);
+  const uint32_t j0 = blockIdx.x * kThreads;
+  if (j0 >= v) return;
+  const uint32_t cnt = min(kThreads, v - j0);
+  const uint32_t t = threadIdx.x;
+  if (t < cnt) {
+    const uint32_t s = sorted[v - 1 - (j0 + t)];
+    const uint2 r = rect[s];
+    s_first[t] = offsets[j0 + t];
+    s_splat[t] = s;
+    if (r.x == 0xFFFFFFFFu) {
+      s_x0[t] = s_y0[t] = 0;
+      s_w[t] = 1;
+    } else {
+      const uint32_t x0 = (r.x & 0xFFFFu) / kTile, x1 = (r.y & 0xFFFFu) / kTile;
+      s_x0[t] = x0;
+      s_y0[t] = (r.x >> 16) / kTile;
+      s_w[t] = x1 - x0 + 1;
     }
+  }
+  if (t == 0) {
+    // End of the last position's pairs: next position's offset, or the
+    // last position's offset plus its own tile count.
+    const uint32_t last = j0 + cnt - 1;
+    uint32_t end;
+    if (last + 1 < v) {
+      end = offsets[last + 1];
+    } else {
+      const uint2 r = rect[sorted[v - 1 - last]];
+      uint32_t c = 0;
+      if (r.x != 0xFFFFFFFFu)
+        c = ((r.y & 0xFFFFu) / kTile - (r.x & 0xFFFFu) / kTile + 1) * ((r.y >> 16) / kTile - (r.x >> 16) / kTile + 1);
+      end = offsets[last] + c;
+    }
+    s_first[cnt] = end;
+  }
+  __syncthreads();
+  const uint32_t p_begin = s_first[0], p_end = min(s_first[cnt], capacity);
+  for (uint32_t p = p_begin + t; p < p_end; p += kThreads) {
+    // Largest i < cnt with s_first[i] <= p (positions with no pair repeat
+    // the next offset and are skipped by taking the largest).
+    uint32_t lo = 0, hi = cnt - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (s_first[mid] <= p)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const uint32_t k = p - s_first[lo];
+    const uint32_t w = s_w[lo];
+    pair_keys[p] = (s_y0[lo] + k / w) * tiles_x + s_x0[lo] + k % w;
+    pair_vals[p] = s_splat[lo];
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void k_tile_ranges(const uint32_t *__restrict__ keys,

@@ -1,16 +1,15 @@
 // sort.hip -- onesweep LSD radix sort of (u32 key, u32 value) pairs for gfx950.
 //
-//   k_os_hist    one read of the keys: per-workgroup digit histograms of
-//                every pass (wave64 ballot match + LDS atomics), no global
-//                atomics; also clears the look-back state of the passes
-//   k_os_reduce  sums the per-workgroup histograms
+//   k_os_clear   clears the histograms and the look-back state
+//   k_os_hist    one read of the keys: LDS digit histograms of every pass,
+//                added into striped global copies
 //   k_os_pass    one launch per 8-bit digit: each 4096-key tile is ranked by
 //                its four waves independently (a wave owns 1024 consecutive
 //                keys and a private LDS histogram; ballot match-any over the
 //                8 digit bits + popcount gives the rank among equal digits),
 //                the tile publishes its digit counts, resolves its global
 //                offsets by decoupled look-back over the preceding tiles
-//                (eight predecessors polled per step), stages the tile in LDS
+//                (32 predecessors polled per step), stages the tile in LDS
 //                in digit order and writes it out coalesced
 //
 // The sort is stable, so with keys complemented (descending) and values in
@@ -46,8 +45,9 @@ constexpr uint32_t IPT = kSortItems;
 constexpr uint32_t TILE = kSortTile;
 constexpr uint32_t WAVES = T / 64;
 constexpr uint32_t WAVE_ITEMS = TILE / WAVES;
-constexpr uint32_t HIST_BLOCKS = 128;
-constexpr uint32_t LOOKBACK = 8;
+constexpr uint32_t HIST_BLOCKS = 512;
+constexpr uint32_t STRIPES = 8;
+constexpr uint32_t LOOKBACK = 32;
 constexpr uint32_t FLAG_AGG = 1u << 30;  // tile count published
 constexpr uint32_t FLAG_INC = 2u << 30;  // inclusive prefix published
 constexpr uint32_t VALUE_MASK = (1u << 30) - 1;
@@ -94,18 +94,18 @@ __device__ __forceinline__ uint32_t load_count(const uint32_t *n_dev, uint32_t n
   return n_dev ? min(*n_dev, n_cap) : n_cap;
 }
 
-// Each workgroup takes tiles of 4096 keys (16 per thread, all loaded before
-// any is counted), accumulates LDS histograms of every pass, and writes them
-// as its partial; the last workgroup to finish sums the partials (the
-// `arrive` counter resets itself for the next sort).
+__global__ __launch_bounds__(T) void k_os_clear(uint32_t *__restrict__ p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * T + threadIdx.x; i < n; i += (size_t)gridDim.x * T) p[i] = 0u;
+}
+
+// Workgroups take tiles of 4096 keys (16 per thread, all loaded before any
+// is counted) and count every pass's digits with LDS atomics; each
+// workgroup then adds its non-zero bins into one of STRIPES global copies
+// (copy = workgroup mod STRIPES) to spread the global atomics.
 __global__ __launch_bounds__(T) void k_os_hist(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ n_dev,
                                                uint32_t n_cap, uint32_t passes, uint32_t flip,
-                                               uint32_t *__restrict__ partial, uint32_t *__restrict__ hist,
-                                               uint32_t *__restrict__ arrive, uint32_t *__restrict__ clear,
-                                               size_t clear_words) {
+                                               uint32_t *__restrict__ hist) {
   __shared__ uint32_t h[4][256];
-  __shared__ uint32_t s_last;
-  for (size_t i = (size_t)blockIdx.x * T + threadIdx.x; i < clear_words; i += (size_t)gridDim.x * T) clear[i] = 0u;
   for (uint32_t p = 0; p < 4; ++p) h[p][threadIdx.x] = 0;
   __syncthreads();
   const uint32_t n = load_count(n_dev, n_cap);
@@ -118,38 +118,16 @@ __global__ __launch_bounds__(T) void k_os_hist(const uint32_t *__restrict__ keys
     }
 #pragma unroll
     for (uint32_t r = 0; r < IPT; ++r) {
-      const bool active = base + (size_t)r * T + threadIdx.x < n;
-      for (uint32_t p = 0; p < passes; ++p) {
-        const uint32_t d = (k[r] >> (8 * p)) & 0xFFu;
-        const uint64_t peers = match_digit(d, active);
-        if (active && (peers & lanemask_lt()) == 0) atomicAdd(&h[p][d], (uint32_t)__popcll(peers));
-      }
+      if (base + (size_t)r * T + threadIdx.x < n)
+        for (uint32_t p = 0; p < passes; ++p) atomicAdd(&h[p][(k[r] >> (8 * p)) & 0xFFu], 1u);
     }
   }
   __syncthreads();
-  for (uint32_t p = 0; p < passes; ++p)
-    partial[((size_t)blockIdx.x * passes + p) * 256 + threadIdx.x] = h[p][threadIdx.x];
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) s_last = atomicAdd(arrive, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
+  uint32_t *dst = hist + (size_t)(blockIdx.x % STRIPES) * 1024;
   for (uint32_t p = 0; p < passes; ++p) {
-    uint32_t s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    uint32_t b = 0;
-    for (; b + 8 <= gridDim.x; b += 8) {
-#pragma unroll
-      for (uint32_t u = 0; u < 8; ++u)
-        s[u] += __hip_atomic_load(partial + ((size_t)(b + u) * passes + p) * 256 + threadIdx.x, __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
-    }
-    for (; b < gridDim.x; ++b)
-      s[0] += __hip_atomic_load(partial + ((size_t)b * passes + p) * 256 + threadIdx.x, __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
-    hist[p * 256 + threadIdx.x] = s[0] + s[1] + s[2] + s[3] + s[4] + s[5] + s[6] + s[7];
+    const uint32_t c = h[p][threadIdx.x];
+    if (c) atomicAdd(&dst[p * 256 + threadIdx.x], c);
   }
-  if (threadIdx.x == 0) *arrive = 0;
 }
 
 __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
@@ -181,7 +159,9 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
   const uint32_t count = (uint32_t)min((size_t)TILE, (size_t)n - base);
   const uint32_t shift = 8 * pass;
 
-  const uint32_t hd = hist[pass * 256 + t];
+  uint32_t hd = 0;
+#pragma unroll
+  for (uint32_t c = 0; c < STRIPES; ++c) hd += hist[c * 1024 + pass * 256 + t];
   if (hd == n) s_trivial = 1;  // every key has this digit: the pass is the identity
   gofs[t] = block_exclusive_scan(hd, lds4);
   __syncthreads();
@@ -237,18 +217,18 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
         sv[k] = (j - (int64_t)k >= 0)
                     ? __hip_atomic_load(status + (size_t)(j - k) * 256 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                     : FLAG_INC;
+      // Consume predecessors nearest first until an inclusive prefix
+      // (done) or an unpublished tile (poll again from there).
       int64_t consumed = 0;
-      bool finished = false;
+      bool finished = false, stop = false;
 #pragma unroll
       for (uint32_t k = 0; k < LOOKBACK; ++k) {
         const uint32_t flag = sv[k] & ~VALUE_MASK;
-        if (flag == 0) break;  // not published yet: poll again from here
-        excl += sv[k] & VALUE_MASK;
-        ++consumed;
-        if (flag == FLAG_INC) {
-          finished = true;
-          break;
-        }
+        const bool take = !stop && flag != 0;
+        excl += take ? (sv[k] & VALUE_MASK) : 0u;
+        consumed += take ? 1 : 0;
+        finished = finished || (take && flag == FLAG_INC);
+        stop = stop || !take || flag == FLAG_INC;
       }
       if (finished) break;
       j -= consumed;
@@ -285,14 +265,12 @@ uint32_t hist_blocks(uint32_t n_cap) { return std::max<uint32_t>(std::min<uint32
 
 }  // namespace
 
-// Layout: arrive [4] | hist [4][256] | partial [hist_blocks][passes][256] |
-// counters [4] | status [passes][tiles][256]. `arrive` must be zero before
-// the first sort (the scratch is cleared when allocated) and resets itself;
-// counters and status are cleared by every sort.
+// Layout: hist [STRIPES][4][256] | counters [4] | status [passes][tiles][256],
+// all cleared at the start of every sort.
 size_t onesweep_status_words(uint32_t n_cap, uint32_t bits) {
   const uint32_t passes = std::max<uint32_t>((bits + 7) / 8, 1);
   const size_t tiles = std::max<uint32_t>(div_up(n_cap, TILE), 1);
-  return 4 + 1024 + (size_t)hist_blocks(n_cap) * passes * 256 + 4 + (size_t)passes * tiles * 256;
+  return (size_t)STRIPES * 1024 + 4 + (size_t)passes * tiles * 256;
 }
 
 void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n_cap,
@@ -302,16 +280,14 @@ void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32
   if (n_cap > (1u << 30)) throw Error(Code::Unsupported, "sort length exceeds 2^30");
   const uint32_t passes = std::min<uint32_t>((bits + 7) / 8, 4);
   const uint32_t tiles = div_up(n_cap, TILE);
-  const uint32_t hb = hist_blocks(n_cap);
-  if (onesweep_status_words(n_cap, bits) > scratch_words) throw Error(Code::Internal, "onesweep scratch too small");
-  uint32_t *arrive = scratch;
-  uint32_t *hist = arrive + 4;
-  uint32_t *partial = hist + 1024;
-  uint32_t *counters = partial + (size_t)hb * passes * 256;
+  const size_t words = onesweep_status_words(n_cap, bits);
+  if (words > scratch_words) throw Error(Code::Internal, "onesweep scratch too small");
+  uint32_t *hist = scratch;
+  uint32_t *counters = hist + (size_t)STRIPES * 1024;
   uint32_t *status = counters + 4;
-  const size_t clear_words = 4 + (size_t)passes * tiles * 256;
   const uint32_t flip = descending ? 0xFFFFFFFFu : 0u;
-  k_os_hist<<<hb, T, 0, st>>>(*keys, n_dev, n_cap, passes, flip, partial, hist, arrive, counters, clear_words);
+  k_os_clear<<<std::min<uint32_t>(div_up(words, T), 1024), T, 0, st>>>(scratch, words);
+  k_os_hist<<<hist_blocks(n_cap), T, 0, st>>>(*keys, n_dev, n_cap, passes, flip, hist);
   for (uint32_t p = 0; p < passes; ++p) {
     k_os_pass<<<tiles, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, p, flip, hist, counters + p,
                                    status + (size_t)p * tiles * 256);
