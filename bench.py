@@ -6,7 +6,13 @@ A step is one full frame: depth key + cull + projection + SH colour, depth
 sort, tile binning, raster, with the scene resident in HBM.
 
 Prints one JSON line (frames/s) with a roofline entry for the dominant
-kernel and the CPU oracle timed on a bounded sample.
+kernel and the CPU oracle timed on a bounded sample. Beside the plain
+figure the line carries the batched forms, each measured against the plain
+one-view-per-launch path on the same GPU and the same views:
+  batched_1080p   8 cameras over the same 300k SH3 scene at 1920x1080, one
+                  batch per step
+  batched_64v     64 views over 8 resident scenes (50k..1M splats, SH3) at
+                  320x240..1920x1080, one batch per step
 """
 
 from __future__ import annotations
@@ -60,12 +66,73 @@ def cpu_baseline(scene, view, cam):
                       f"project {st[2]:.1f}, raster {st[3]:.1f}"}
 
 
+def time_batches(batch, views, steps, warmup):
+    """Frames (views) per second of `steps` back-to-back batch renders of `views`."""
+    for _ in range(warmup):
+        batch.render(views)
+    batch.wait()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.render(views)
+    batch.wait()
+    return len(views) * steps / (time.perf_counter() - t0)
+
+
+def time_one_per_launch(batch, views, passes, warmup):
+    """Frames per second rendering the same views one per launch (the plain path)."""
+    for _ in range(warmup):
+        for v in views:
+            batch.render([v])
+    batch.wait()
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        for v in views:
+            batch.render([v])
+    batch.wait()
+    return len(views) * passes / (time.perf_counter() - t0)
+
+
+def bench_batched_1080p(scene, steps, warmup):
+    rng = np.random.default_rng(synth.SEED_BASE + 1 + 500)
+    dev = A.Scene(scene)
+    views = [(dev, A.look_at(synth.sphere_point(rng, 4.0)), 1920, 1080) for _ in range(8)]
+    b = A.Batch()
+    batched = time_batches(b, views, steps, warmup)
+    st = b.stage_times()
+    plain = time_one_per_launch(A.Batch(), views, max(steps // 2, 1), warmup)
+    return {"views": len(views), "frames_per_s": round(batched, 3), "plain_frames_per_s": round(plain, 3),
+            "speedup_vs_plain": round(batched / plain, 3), "ms_per_batch": round(1e3 * len(views) / batched, 4),
+            "stage_ms": {"project": round(st.project_ms, 4), "sort": round(st.sort_ms, 4),
+                         "bin": round(st.bin_ms, 4), "raster": round(st.raster_ms, 4)},
+            "tile_pairs": int(st.tile_pairs)}
+
+
+def bench_batched_64v(steps, warmup):
+    scenes, vs = synth.batched()
+    dev = [A.Scene(s) for s in scenes]
+    del scenes
+    views = [(dev[v.scene], A.look_at(v.position, v.target), v.width, v.height) for v in vs]
+    b = A.Batch()
+    batched = time_batches(b, views, steps, warmup)
+    st = b.stage_times()
+    plain = time_one_per_launch(A.Batch(), views, max(steps // 4, 1), max(warmup // 4, 1))
+    pixels = sum(w * h for (_, _, w, h) in views)
+    return {"views": len(views), "scenes": len(dev), "frames_per_s": round(batched, 3),
+            "plain_frames_per_s": round(plain, 3), "speedup_vs_plain": round(batched / plain, 3),
+            "mpix_per_s": round(batched / len(views) * pixels / 1e6, 1),
+            "ms_per_batch": round(1e3 * len(views) / batched, 4),
+            "stage_ms": {"project": round(st.project_ms, 4), "sort": round(st.sort_ms, 4),
+                         "bin": round(st.bin_ms, 4), "raster": round(st.raster_ms, 4)},
+            "splat_slots": int(st.splat_count), "visible": int(st.visible_count), "tile_pairs": int(st.tile_pairs)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain-only", action="store_true", help="skip the batched sections")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
@@ -134,9 +201,14 @@ def main():
         "roofline": {"kernel": "k_project", "bound": bound, "achieved": round(achieved, 2), "peak": peak,
                      "unit": unit, "frac": round(achieved / peak, 4), "traffic": None},
     }
+    ctx.close()
+    if not args.plain_only:
+        line["batched_1080p"] = bench_batched_1080p(scene, max(args.steps // 8, 5), max(args.warmup // 4, 2))
+        print(f"batched_1080p {line['batched_1080p']}", file=sys.stderr, flush=True)
+        line["batched_64v"] = bench_batched_64v(max(args.steps // 20, 5), max(args.warmup // 4, 2))
+        print(f"batched_64v {line['batched_64v']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, view, cam)
-    ctx.close()
     print(json.dumps(line), flush=True)
 
 

@@ -6,10 +6,12 @@
 // on every entry point standing in for catch_unwind (:225-255).
 
 #include <cmath>
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "common.h"
 #include "renderer.h"
@@ -22,6 +24,15 @@ struct GsplatContext {
 
 struct GsplatPlyScene {
   gs::HostScene scene;
+};
+
+struct GsplatScene {
+  std::shared_ptr<gs::DeviceScene> scene;
+};
+
+struct GsplatBatch {
+  gs::Pipeline pipe;
+  float host_ms = 0.0f;
 };
 
 namespace {
@@ -79,6 +90,23 @@ gs::PlyLimits limits_from(const GsplatPlyLimits *l) {
     out.max_scene_bytes = l->max_scene_bytes;
   }
   return out;
+}
+
+// CStr::to_str: a path must be UTF-8 (ffi-c/src/lib.rs, load_scene_path).
+bool path_is_utf8(const char *path) {
+  const size_t len = std::strlen(path);
+  const unsigned char *u = reinterpret_cast<const unsigned char *>(path);
+  size_t i = 0;
+  while (i < len) {
+    const unsigned char c = u[i];
+    const size_t l = c < 0x80 ? 1 : (c & 0xE0) == 0xC0 ? 2 : (c & 0xF0) == 0xE0 ? 3 : (c & 0xF8) == 0xF0 ? 4 : 0;
+    bool good = l != 0 && i + l <= len;
+    for (size_t k = 1; good && k < l; ++k) good = (u[i + k] & 0xC0) == 0x80;
+    if (good && l == 2) good = c >= 0xC2;
+    if (!good) return false;
+    i += l;
+  }
+  return true;
 }
 
 }  // namespace
@@ -169,21 +197,7 @@ int32_t gsplat_context_load_scene_path(GsplatContext *ctx, const char *path) {
   return guard(op, [&]() -> int32_t {
     if (!ctx) return null_arg(op, "ctx");
     if (!path) return null_arg(op, "path");
-    size_t len = std::strlen(path);
-    // CStr::to_str: the path must be UTF-8.
-    {
-      const unsigned char *u = reinterpret_cast<const unsigned char *>(path);
-      size_t i = 0;
-      while (i < len) {
-        unsigned char c = u[i];
-        size_t l = c < 0x80 ? 1 : (c & 0xE0) == 0xC0 ? 2 : (c & 0xF0) == 0xE0 ? 3 : (c & 0xF8) == 0xF0 ? 4 : 0;
-        bool good = l != 0 && i + l <= len;
-        for (size_t k = 1; good && k < l; ++k) good = (u[i + k] & 0xC0) == 0x80;
-        if (good && l == 2) good = c >= 0xC2;
-        if (!good) return fail(gs::Code::InvalidArgument, std::string(op) + ": path is not valid UTF-8");
-        i += l;
-      }
-    }
+    if (!path_is_utf8(path)) return fail(gs::Code::InvalidArgument, std::string(op) + ": path is not valid UTF-8");
     gs::HostScene scene = gs::ply_load(path, gs::PlyLimits());
     ctx->renderer->load_scene(std::move(scene));
     return ok();
@@ -544,6 +558,214 @@ uint32_t gsplat_device_available(void) {
   } catch (...) {
     return 0u;
   }
+}
+
+// ------------------------------------------------------- resident scenes
+
+namespace {
+
+int32_t make_scene(gs::HostScene &&s, GsplatScene **out) {
+  if (!gs::device_present()) throw gs::Error(gs::Code::Unsupported, "gpu rasterizer unavailable");
+  gs::validate_scene(s);
+  auto h = std::make_unique<GsplatScene>();
+  h->scene = gs::upload_scene(s, nullptr);
+  *out = h.release();
+  return ok();
+}
+
+}  // namespace
+
+int32_t gsplat_scene_create(const GsplatSceneView *view, GsplatScene **out_scene) {
+  static const char *op = "gsplat_scene_create";
+  return guard(op, [&]() -> int32_t {
+    if (!out_scene) return null_arg(op, "out_scene");
+    *out_scene = nullptr;
+    if (!view) return null_arg(op, "scene");
+    gs::HostScene s;
+    try {
+      s = gs::scene_from_view(*view);
+    } catch (const gs::Error &e) {
+      if (e.code == gs::Code::Unsupported) throw;
+      return fail(gs::Code::InvalidArgument, std::string(op) + ": invalid scene buffers");
+    }
+    return make_scene(std::move(s), out_scene);
+  });
+}
+
+int32_t gsplat_scene_create_from_path(const char *path, GsplatScene **out_scene) {
+  static const char *op = "gsplat_scene_create_from_path";
+  return guard(op, [&]() -> int32_t {
+    if (!out_scene) return null_arg(op, "out_scene");
+    *out_scene = nullptr;
+    if (!path) return null_arg(op, "path");
+    if (!path_is_utf8(path)) return fail(gs::Code::InvalidArgument, std::string(op) + ": path is not valid UTF-8");
+    return make_scene(gs::ply_load(path, gs::PlyLimits()), out_scene);
+  });
+}
+
+int32_t gsplat_scene_get_info(const GsplatScene *scene, GsplatSceneInfo *out_info) {
+  static const char *op = "gsplat_scene_get_info";
+  return guard(op, [&]() -> int32_t {
+    if (!scene) return null_arg(op, "scene");
+    if (!out_info) return null_arg(op, "out_info");
+    out_info->count = scene->scene->count;
+    out_info->sh_degree = scene->scene->sh_degree;
+    out_info->device_bytes = scene->scene->bytes();
+    return ok();
+  });
+}
+
+void gsplat_scene_destroy(GsplatScene *scene) {
+  try {
+    delete scene;
+  } catch (...) {
+  }
+}
+
+// ---------------------------------------------------------------- batches
+
+int32_t gsplat_batch_create(GsplatBatch **out_batch) {
+  static const char *op = "gsplat_batch_create";
+  return guard(op, [&]() -> int32_t {
+    if (!out_batch) return null_arg(op, "out_batch");
+    *out_batch = nullptr;
+    auto b = std::make_unique<GsplatBatch>();
+    b->pipe.set_raster_variant(gs::raster_variant_from_env());
+    *out_batch = b.release();
+    return ok();
+  });
+}
+
+void gsplat_batch_destroy(GsplatBatch *batch) {
+  try {
+    delete batch;
+  } catch (...) {
+  }
+}
+
+int32_t gsplat_batch_render(GsplatBatch *batch, const GsplatViewDesc *views, uint32_t count) {
+  static const char *op = "gsplat_batch_render";
+  return guard(op, [&]() -> int32_t {
+    if (!batch) return null_arg(op, "batch");
+    if (!views) return null_arg(op, "views");
+    const auto t0 = std::chrono::steady_clock::now();
+    if (count == 0 || count > gs::kMaxViews)
+      return fail(gs::Code::InvalidArgument, std::string(op) + ": view count must be 1.." +
+                                                 std::to_string(gs::kMaxViews));
+    std::vector<gs::ViewSpec> specs;
+    specs.reserve(count);
+    for (uint32_t v = 0; v < count; ++v) {
+      const GsplatViewDesc &d = views[v];
+      const std::string at = std::string(op) + ": view " + std::to_string(v) + ": ";
+      if (!d.scene) return fail(gs::Code::InvalidArgument, at + "scene is null");
+      if (!gs::camera_valid(d.camera)) return fail(gs::Code::InvalidArgument, at + "invalid camera");
+      if (d.width == 0 || d.height == 0)
+        return fail(gs::Code::InvalidArgument, at + "invalid renderer configuration");
+      if (d.width > 16384 || d.height > 16384)
+        return fail(gs::Code::Unsupported, at + "render dimensions exceed the device 2D texture limit 16384");
+      specs.push_back(gs::ViewSpec{
+          d.scene->scene, gs::make_frame_params(d.camera.position, d.camera.rotation_xyzw,
+                                                d.camera.vertical_fov_radians, d.camera.near_plane,
+                                                d.camera.far_plane, d.width, d.height)});
+    }
+    batch->pipe.enqueue(std::move(specs));
+    batch->host_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return ok();
+  });
+}
+
+int32_t gsplat_batch_wait(GsplatBatch *batch) {
+  static const char *op = "gsplat_batch_wait";
+  return guard(op, [&]() -> int32_t {
+    if (!batch) return null_arg(op, "batch");
+    batch->pipe.wait();
+    return ok();
+  });
+}
+
+int32_t gsplat_batch_readback_rgba8(GsplatBatch *batch, uint32_t view, uint8_t *out, uint64_t len) {
+  static const char *op = "gsplat_batch_readback_rgba8";
+  return guard(op, [&]() -> int32_t {
+    if (!batch) return null_arg(op, "batch");
+    if (!out) return null_arg(op, "out");
+    batch->pipe.readback_rgba8(view, out, len);
+    return ok();
+  });
+}
+
+int32_t gsplat_batch_readback_rgba32f(GsplatBatch *batch, uint32_t view, float *out, uint64_t len) {
+  static const char *op = "gsplat_batch_readback_rgba32f";
+  return guard(op, [&]() -> int32_t {
+    if (!batch) return null_arg(op, "batch");
+    if (!out) return null_arg(op, "out");
+    batch->pipe.readback_rgba32f(view, out, len);
+    return ok();
+  });
+}
+
+int32_t gsplat_batch_get_view_stats(GsplatBatch *batch, uint32_t view, GsplatStats *out_stats) {
+  static const char *op = "gsplat_batch_get_view_stats";
+  return guard(op, [&]() -> int32_t {
+    if (!batch) return null_arg(op, "batch");
+    if (!out_stats) return null_arg(op, "out_stats");
+    const uint32_t vis = batch->pipe.visible(view);
+    const gs::FrameTimes t = batch->pipe.times();
+    out_stats->frame_ms = batch->host_ms;
+    out_stats->preprocess_ms = t.project_ms;
+    out_stats->sort_ms = t.sort_ms;
+    out_stats->raster_ms = t.bin_ms + t.raster_ms;
+    out_stats->visible_count = vis;
+    out_stats->drawn_count = vis;
+    return ok();
+  });
+}
+
+int32_t gsplat_batch_read_sorted_indices(GsplatBatch *batch, uint32_t view, uint32_t *out, uint64_t cap,
+                                         uint32_t *out_count) {
+  static const char *op = "gsplat_batch_read_sorted_indices";
+  return guard(op, [&]() -> int32_t {
+    if (!batch) return null_arg(op, "batch");
+    if (!out_count) return null_arg(op, "out_count");
+    if (!out && cap) return null_arg(op, "out");
+    batch->pipe.read_sorted(view, out, cap, out_count);
+    return ok();
+  });
+}
+
+int32_t gsplat_batch_read_tile_counts(GsplatBatch *batch, uint32_t view, uint32_t *out, uint64_t cap,
+                                      uint32_t *out_tiles_x, uint32_t *out_tiles_y) {
+  static const char *op = "gsplat_batch_read_tile_counts";
+  return guard(op, [&]() -> int32_t {
+    if (!batch) return null_arg(op, "batch");
+    if (!out) return null_arg(op, "out");
+    if (!out_tiles_x || !out_tiles_y) return null_arg(op, "out_tiles");
+    batch->pipe.read_tile_counts(view, out, cap, out_tiles_x, out_tiles_y);
+    return ok();
+  });
+}
+
+int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out) {
+  static const char *op = "gsplat_batch_get_stage_times";
+  return guard(op, [&]() -> int32_t {
+    if (!batch) return null_arg(op, "batch");
+    if (!out) return null_arg(op, "out");
+    *out = GsplatStageTimes{};
+    if (!batch->pipe.has_frame()) return ok();
+    const gs::FrameTimes t = batch->pipe.times();
+    out->project_ms = t.project_ms;
+    out->sort_ms = t.sort_ms;
+    out->bin_ms = t.bin_ms;
+    out->raster_ms = t.raster_ms;
+    out->total_ms = t.total_ms;
+    out->splat_count = (uint32_t)std::min<uint64_t>(batch->pipe.total_splats(), 0xFFFFFFFFu);
+    uint64_t vis = 0;
+    for (uint32_t v = 0; v < batch->pipe.view_count(); ++v) vis += batch->pipe.visible(v);
+    out->visible_count = (uint32_t)std::min<uint64_t>(vis, 0xFFFFFFFFu);
+    out->tiles_x = batch->pipe.total_tiles();
+    out->tiles_y = 1;
+    out->tile_pairs = batch->pipe.pairs();
+    return ok();
+  });
 }
 
 }  // extern "C"

@@ -1,4 +1,8 @@
 // kernels.h -- host launchers of the gfx950 kernels of the SortedAlpha path.
+//
+// Every per-frame kernel works on a batch of views described by a device
+// view table. A view renders one resident scene from one camera into its own
+// target. A single context frame is a batch of one view.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,22 +31,45 @@ FrameParams make_frame_params(const float position[3], const float rotation_xyzw
 constexpr uint32_t kTile = 16;            // raster tile edge in pixels
 constexpr uint32_t kThreads = 256;        // threads per workgroup (4 wave64)
 constexpr uint32_t kScanTile = kThreads * 16;
+constexpr uint32_t kSegmentAlign = 4096;  // view slot segments align to the sort tile
+constexpr uint32_t kLocalBits = 24;       // splat index bits in a sort value
+constexpr uint32_t kLocalMask = (1u << kLocalBits) - 1;
+constexpr uint32_t kMaxViews = 256;       // view id fits the value's top byte
 
-// Projected splat, 3 x float4 per splat (48 B):
+// One view of a batch. Its splats own the slot segment [seg_off,
+// seg_off + seg_len) of every per-splat array (seg_len = n rounded up to
+// kSegmentAlign), its tiles own the global tile ids [tile_off, tile_off +
+// tiles), and its target starts at pixel img_off.
+struct ViewDev {
+  FrameParams fp;
+  const float *soa;  // scene SoA (coefficient-major, see common.h)
+  uint32_t n, sh_degree;
+  uint32_t seg_off, seg_len;
+  uint32_t tile_off, tiles;
+  uint64_t img_off;
+};
+
+// Projected splat, 3 x float4 per slot (48 B):
 //   [cx, cy, i00, i01] [i10, i11, cut, alpha] [r*a, g*a, b*a, 0]
 // where (cx, cy) is the NDC centre, i = [axis_u axis_v]^-1, cut the coverage
 // bound on r^2 and (r,g,b)*a the premultiplied colour.
 constexpr uint32_t kRecordFloat4 = 3;
 
-// Projection + depth key + cull + SH colour + tile rectangle for every splat.
-// keys[i] = f32 bits of depth if near<=depth<=far else 0; vals[i] = i;
-// rect[i] = packed tile rectangle (x0|y0<<16, x1|y1<<16) or x = ~0u when
-// the splat covers no pixel; *visible += number of depth-visible splats.
-// `export12`, when non-null, receives the reference quantities (12 floats
-// per splat) instead of the record/rect outputs.
-void launch_project(uint32_t sh_degree, const float *soa, uint32_t n, const FrameParams &fp,
-                    uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect, uint32_t *visible,
-                    float *export12, hipStream_t stream);
+// Depth key, cull, projection, SH colour and pixel bounds for every slot of
+// every view whose scene has SH degree `degree` (other views are skipped):
+// keys[g] = f32 bits of depth when near <= depth <= far else 0;
+// vals[g] = view << 24 | splat index (index 0xFFFFFF for padding slots);
+// rect[g] = pixel bounds x0|y0<<16, x1|y1<<16, or x = ~0u when the splat
+// covers no pixel.
+void launch_project(uint32_t degree, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,
+                    uint32_t total_slots, uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect,
+                    hipStream_t stream);
+
+// The same projection of one view's n splats exporting, per splat, the depth
+// key (keys_out, optional) and the reference quantities of project_splat +
+// eval_color (12 floats: drawn, centre, axis_u, axis_v, premultiplied rgba).
+void launch_project_export(uint32_t degree, const ViewDev *views, uint32_t view, uint32_t n, uint32_t *keys_out,
+                           float *export12, hipStream_t stream);
 
 // Scratch of the two-level exclusive scan.
 struct ScanScratch {
@@ -55,38 +82,44 @@ size_t scan_partial_entries(size_t n);
 // Exclusive scan of n u32 values; the 64-bit sum goes to *scratch.total.
 void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s, hipStream_t stream);
 
-// counts[j] = number of tiles of the j-th splat front to back, i.e. of
-// sorted[V-1-j], for j < V (V read from *visible); 0 for j >= V.
-void launch_tile_counts(const uint32_t *sorted, const uint32_t *visible, const uint2 *rect, uint32_t n,
-                        uint32_t *counts, hipStream_t stream);
+// After the depth sort (each view's segment back to front, culled last):
+// counts_rev[total-1-g] = number of tiles of the splat in sorted slot g (0
+// for culled and padding slots), so an exclusive scan of counts_rev gives
+// every slot its first pair index with each view's pairs front to back; and
+// visible[v] = V of view v (written by the last visible slot; zero it first).
+void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
+                        const uint32_t *skeys, const uint32_t *svals, const uint2 *rect, uint32_t *counts_rev,
+                        uint32_t *visible, hipStream_t stream);
 
 // Frame summary written on the device after the tile-count scan:
-// info[0] = V, info[1..2] = 64-bit pair total M, info[3] = 1 when M exceeds
-// the pair capacity; *m_out = min(M, capacity) drives the later kernels.
-void launch_pairs_finalize(const uint32_t *visible, const uint64_t *total, uint32_t capacity, uint32_t *m_out,
-                           uint32_t *info, hipStream_t stream);
+// info[1..2] = 64-bit pair total M, info[3] = 1 when M exceeds the pair
+// capacity; *m_out = min(M, capacity) drives the later kernels.
+void launch_pairs_finalize(const uint64_t *total, uint32_t capacity, uint32_t *m_out, uint32_t *info,
+                           hipStream_t stream);
 
-// Writes (tile id, splat) pairs front to back at offsets[j]; pairs at or
-// beyond `capacity` are dropped (the frame is then flagged and re-rendered).
-void launch_tile_emit(const uint32_t *sorted, const uint32_t *visible, const uint2 *rect,
-                      const uint32_t *offsets, uint32_t n, uint32_t tiles_x, uint32_t capacity,
-                      uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t stream);
+// Writes (global tile id, record slot) pairs front to back at the scanned
+// offsets; pairs at or beyond `capacity` are dropped (the frame is then
+// flagged and re-rendered).
+void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
+                      const uint32_t *svals, const uint2 *rect, const uint32_t *counts_rev,
+                      const uint32_t *offsets_rev, uint32_t capacity, uint32_t *pair_keys, uint32_t *pair_vals,
+                      hipStream_t stream);
 
-// ranges[t] = [first, last+1) of tile t in the tile-sorted pair keys; the
-// pair count is read from *m_dev (grid sized for `capacity`).
+// ranges[t] = [first, last+1) of global tile t in the tile-sorted pair keys;
+// the pair count is read from *m_dev (grid sized for `capacity`).
 void launch_tile_ranges(const uint32_t *pair_keys, const uint32_t *m_dev, uint32_t capacity, uint32_t tiles,
                         uint2 *ranges, hipStream_t stream);
 
-// Front-to-back blend of each 16x16 tile's list.
-// All variants: one workgroup per tile, batches of 256 records staged in LDS.
-//   TileLds      thread = pixel in row-major tile order
-//   QuadWave     each wave owns an 8x8 quadrant and skips splats whose pixel
-//                bounds miss it
-//   QuadWaveXcd  QuadWave with the tiles banded per XCD
-enum class RasterVariant : uint32_t { TileLds = 0, QuadWave = 1, QuadWaveXcd = 2 };
-void launch_raster(RasterVariant variant, const float4 *rec, const uint2 *rect, const uint2 *ranges,
-                   const uint32_t *list, uint32_t width, uint32_t height, uint32_t tiles_x, uint32_t tiles_y,
-                   float4 *out32, uchar4 *out8, hipStream_t stream);
+// Front-to-back blend of every view's 16x16 tiles: one workgroup per tile,
+// batches of 256 records staged in LDS.
+//   Linear   workgroup b renders global tile b
+//   XcdBand  workgroup b runs on XCD b mod 8 (round-robin dispatch); the
+//            tiles are banded so each XCD renders one contiguous range and
+//            neighbouring tiles share that XCD's L2
+enum class RasterVariant : uint32_t { Linear = 0, XcdBand = 1 };
+void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
+                   uint32_t total_tiles, const float4 *rec, const uint2 *ranges, const uint32_t *list,
+                   float4 *img32, uchar4 *img8, hipStream_t stream);
 
 // Device fill of n u32 words (graph-safe replacement of hipMemsetAsync).
 void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t stream);

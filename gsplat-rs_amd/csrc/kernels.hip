@@ -1,17 +1,20 @@
-// kernels.hip -- gfx950 kernels of the SortedAlpha path.
+// kernels.hip -- gfx950 kernels of the SortedAlpha path, over a batch of views.
 //
 // Compiled with -ffp-contract=off and correctly rounded f32 divide/sqrt, so
 // every expression that restates reference arithmetic (depth key,
-// projection, SH colour, tile rectangle) rounds exactly like the reference's
+// projection, SH colour, pixel bounds) rounds exactly like the reference's
 // IEEE single precision; fused multiply-adds appear only where written as
 // fmaf (the per-pixel r^2 of the raster, whose operation order is defined by
 // this renderer because the reference leaves it to the GPU's interpolator).
+//
+// A view's splats occupy a slot segment aligned to kSegmentAlign, so every
+// 256-thread workgroup over slots belongs to exactly one view, and the view
+// is found once per workgroup by binary search over the segment starts.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
-#include <stdexcept>
 #include <string>
 
 #include "common.h"
@@ -57,6 +60,19 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *l
   __syncthreads();
   *total = tot;
   return off + inc - v;
+}
+
+// Index v of the last start[v] <= x (start is non-decreasing, start[0] = 0).
+__device__ __forceinline__ uint32_t find_view(const uint32_t *__restrict__ start, uint32_t nviews, uint32_t x) {
+  uint32_t lo = 0, hi = nviews - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) / 2;
+    if (start[mid] <= x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
 }
 
 // ---------------------------------------------------------------- projection
@@ -119,170 +135,183 @@ __device__ __forceinline__ float sh_channel(float dc, const float *__restrict__ 
   return r;
 }
 
-// One thread per splat over the coefficient-major SoA: depth key and cull
-// (preprocess_visible_into, render-wgpu/src/lib.rs:1263-1296), projection
-// (project_splat, wgsl:131-217), colour (eval_color, wgsl:328-334), and the
-// raster record + tile rectangle of this renderer.
-template <int DEG, bool EXPORT>
-__global__ __launch_bounds__(kThreads) void k_project(const float *__restrict__ soa, uint32_t n, FrameParams fp,
-                                                      uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                      float4 *__restrict__ rec, uint2 *__restrict__ rect,
-                                                      uint32_t *__restrict__ visible,
-                                                      float *__restrict__ export12) {
+struct Projected {
+  uint32_t key;  // depth key, 0 when culled by near/far
+  bool drawn;    // project_splat did not return invalid_splat()
+  float cx, cy, ux, uy, vx, vy, alpha;
+  float col[3];  // premultiplied rgb
+};
+
+// Depth key and cull (preprocess_visible_into, render-wgpu/src/lib.rs:
+// 1263-1296), projection (project_splat, wgsl:131-217) and colour
+// (eval_color, wgsl:328-334) of splat i of a scene.
+template <int DEG>
+__device__ __forceinline__ Projected project_one(const FrameParams &fp, const float *__restrict__ soa, uint32_t n,
+                                                 uint32_t i) {
   constexpr uint32_t PC = (DEG + 1) * (DEG + 1) - 1;
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  bool vis = false;
-  if (i < n) {
-    const size_t N = n;
-    const float relx = soa[kPx * N + i] - fp.cam[0];
-    const float rely = soa[kPy * N + i] - fp.cam[1];
-    const float relz = soa[kPz * N + i] - fp.cam[2];
-    // world_to_camera_depth_with_view_row (lib.rs:1346-1354), no FMA.
-    const float z = fp.rows[6] * relx + fp.rows[7] * rely + fp.rows[8] * relz;
-    vis = z >= fp.near_plane && z <= fp.far_plane;
-    if (!EXPORT || keys != nullptr) {
-      keys[i] = vis ? __float_as_uint(fmaxf(z, 0.0f)) : 0u;  // depth_to_key (lib.rs:973-976)
-      if (vals != nullptr) vals[i] = i;
+  const size_t N = n;
+  Projected o{};
+  const float relx = soa[kPx * N + i] - fp.cam[0];
+  const float rely = soa[kPy * N + i] - fp.cam[1];
+  const float relz = soa[kPz * N + i] - fp.cam[2];
+  // world_to_camera_depth_with_view_row (lib.rs:1346-1354), no FMA.
+  const float z = fp.rows[6] * relx + fp.rows[7] * rely + fp.rows[8] * relz;
+  const bool vis = z >= fp.near_plane && z <= fp.far_plane;
+  o.key = vis ? __float_as_uint(fmaxf(z, 0.0f)) : 0u;  // depth_to_key (lib.rs:973-976)
+  if (!(vis && z > 1e-6f)) return o;
+  const float x = fp.rows[0] * relx + fp.rows[1] * rely + fp.rows[2] * relz;
+  const float y = fp.rows[3] * relx + fp.rows[4] * rely + fp.rows[5] * relz;
+  const float inv_z = 1.0f / z;
+  const float x_ndc = ((x * fp.f) * inv_z) / fp.aspect;
+  const float y_ndc = (y * fp.f) * inv_z;
+  const Cov6 wc{soa[kCxx * N + i], soa[kCxy * N + i], soa[kCxz * N + i],
+                soa[kCyy * N + i], soa[kCyz * N + i], soa[kCzz * N + i]};
+  const float *r0 = &fp.rows[0], *r1 = &fp.rows[3], *r2 = &fp.rows[6];
+  // transform_covariance_terms_to_camera (wgsl:111-120).
+  const Cov6 cc{cov_quadratic(wc, r0[0], r0[1], r0[2]),
+                cov_bilinear(wc, r0[0], r0[1], r0[2], r1[0], r1[1], r1[2]),
+                cov_bilinear(wc, r0[0], r0[1], r0[2], r2[0], r2[1], r2[2]),
+                cov_quadratic(wc, r1[0], r1[1], r1[2]),
+                cov_bilinear(wc, r1[0], r1[1], r1[2], r2[0], r2[1], r2[2]),
+                cov_quadratic(wc, r2[0], r2[1], r2[2])};
+  const float x_cl = fminf(fmaxf(x / z, -fp.lim_x), fp.lim_x) * z;
+  const float y_cl = fminf(fmaxf(y / z, -fp.lim_y), fp.lim_y) * z;
+  const float inv_z2 = inv_z * inv_z;
+  const float j00 = fp.fx * inv_z;
+  const float j02 = -fp.fx * x_cl * inv_z2;
+  const float j11 = fp.fy * inv_z;
+  const float j12 = -fp.fy * y_cl * inv_z2;
+  const float b = j00 * j11 * cc.xy + j00 * j12 * cc.xz + j02 * j11 * cc.yz + j02 * j12 * cc.zz;
+  float a = j00 * j00 * cc.xx + 2.0f * j00 * j02 * cc.xz + j02 * j02 * cc.zz;
+  float c = j11 * j11 * cc.yy + 2.0f * j11 * j12 * cc.yz + j12 * j12 * cc.zz;
+  a = a + fp.blur_x;
+  c = c + fp.blur_y;
+  const float apco2 = (a + c) * 0.5f;
+  const float amco2 = (a - c) * 0.5f;
+  const float term = sqrtf(fmaxf(amco2 * amco2 + b * b, 0.0f));
+  const float major = fmaxf(apco2 + term, 1e-10f);
+  const float minor = fmaxf(apco2 - term, 1e-10f);
+  float udx = 1.0f, udy = 0.0f;
+  if (fabsf(b) > 1e-8f) {
+    // normalize2_or_default (wgsl:79-85), inverseSqrt as 1/sqrt.
+    const float px = b, py = major - a;
+    const float len2 = px * px + py * py;
+    if (len2 > 1e-20f) {
+      const float inv = 1.0f / sqrtf(len2);
+      udx = px * inv;
+      udy = py * inv;
     }
-    bool drawn = false;
-    float cx = 0, cy = 0, ux = 0, uy = 0, vx = 0, vy = 0, alpha = 0, col[3] = {0, 0, 0};
-    if (vis && z > 1e-6f) {
-      const float x = fp.rows[0] * relx + fp.rows[1] * rely + fp.rows[2] * relz;
-      const float y = fp.rows[3] * relx + fp.rows[4] * rely + fp.rows[5] * relz;
-      const float inv_z = 1.0f / z;
-      const float x_ndc = ((x * fp.f) * inv_z) / fp.aspect;
-      const float y_ndc = (y * fp.f) * inv_z;
-      const Cov6 wc{soa[kCxx * N + i], soa[kCxy * N + i], soa[kCxz * N + i],
-                    soa[kCyy * N + i], soa[kCyz * N + i], soa[kCzz * N + i]};
-      const float *r0 = &fp.rows[0], *r1 = &fp.rows[3], *r2 = &fp.rows[6];
-      // transform_covariance_terms_to_camera (wgsl:111-120).
-      const Cov6 cc{cov_quadratic(wc, r0[0], r0[1], r0[2]),
-                    cov_bilinear(wc, r0[0], r0[1], r0[2], r1[0], r1[1], r1[2]),
-                    cov_bilinear(wc, r0[0], r0[1], r0[2], r2[0], r2[1], r2[2]),
-                    cov_quadratic(wc, r1[0], r1[1], r1[2]),
-                    cov_bilinear(wc, r1[0], r1[1], r1[2], r2[0], r2[1], r2[2]),
-                    cov_quadratic(wc, r2[0], r2[1], r2[2])};
-      const float x_cl = fminf(fmaxf(x / z, -fp.lim_x), fp.lim_x) * z;
-      const float y_cl = fminf(fmaxf(y / z, -fp.lim_y), fp.lim_y) * z;
-      const float inv_z2 = inv_z * inv_z;
-      const float j00 = fp.fx * inv_z;
-      const float j02 = -fp.fx * x_cl * inv_z2;
-      const float j11 = fp.fy * inv_z;
-      const float j12 = -fp.fy * y_cl * inv_z2;
-      const float b = j00 * j11 * cc.xy + j00 * j12 * cc.xz + j02 * j11 * cc.yz + j02 * j12 * cc.zz;
-      float a = j00 * j00 * cc.xx + 2.0f * j00 * j02 * cc.xz + j02 * j02 * cc.zz;
-      float c = j11 * j11 * cc.yy + 2.0f * j11 * j12 * cc.yz + j12 * j12 * cc.zz;
-      a = a + fp.blur_x;
-      c = c + fp.blur_y;
-      const float apco2 = (a + c) * 0.5f;
-      const float amco2 = (a - c) * 0.5f;
-      const float term = sqrtf(fmaxf(amco2 * amco2 + b * b, 0.0f));
-      const float major = fmaxf(apco2 + term, 1e-10f);
-      const float minor = fmaxf(apco2 - term, 1e-10f);
-      float udx = 1.0f, udy = 0.0f;
-      if (fabsf(b) > 1e-8f) {
-        const float px = b, py = major - a;
-        const float len2 = px * px + py * py;
-        if (len2 > 1e-20f) {
-          const float inv = 1.0f / sqrtf(len2);
-          udx = px * inv;
-          udy = py * inv;
-        }
-      } else if (a < c) {
-        udx = 0.0f;
-        udy = 1.0f;
-      }
-      float major_r = fminf(fmaxf(sqrtf(major) * 3.0f, 1e-4f), 2.0f);
-      const float minor_r = fminf(fmaxf(sqrtf(minor) * 3.0f, 1e-4f), 2.0f);
-      major_r = fminf(major_r, minor_r * 64.0f);
-      ux = udx * major_r;
-      uy = udy * major_r;
-      vx = -udy * minor_r;
-      vy = udx * minor_r;
-      const float ext_x = fabsf(ux) + fabsf(vx);
-      const float ext_y = fabsf(uy) + fabsf(vy);
-      drawn = !(x_ndc + ext_x < -1.0f || x_ndc - ext_x > 1.0f || y_ndc + ext_y < -1.0f || y_ndc - ext_y > 1.0f);
-      if (drawn) {
-        cx = x_ndc;
-        cy = y_ndc;
-        alpha = soa[kAlpha * N + i];
-        // normalize3_or_default(position - camera) (wgsl:87-93, 329).
-        float dxw = 0.0f, dyw = 0.0f, dzw = 1.0f;
-        const float len2 = relx * relx + rely * rely + relz * relz;
-        if (len2 > 1e-20f) {
-          const float inv = 1.0f / sqrtf(len2);
-          dxw = relx * inv;
-          dyw = rely * inv;
-          dzw = relz * inv;
-        }
+  } else if (a < c) {
+    udx = 0.0f;
+    udy = 1.0f;
+  }
+  float major_r = fminf(fmaxf(sqrtf(major) * 3.0f, 1e-4f), 2.0f);
+  const float minor_r = fminf(fmaxf(sqrtf(minor) * 3.0f, 1e-4f), 2.0f);
+  major_r = fminf(major_r, minor_r * 64.0f);
+  const float ux = udx * major_r, uy = udy * major_r;
+  const float vx = -udy * minor_r, vy = udx * minor_r;
+  const float ext_x = fabsf(ux) + fabsf(vx);
+  const float ext_y = fabsf(uy) + fabsf(vy);
+  if (x_ndc + ext_x < -1.0f || x_ndc - ext_x > 1.0f || y_ndc + ext_y < -1.0f || y_ndc - ext_y > 1.0f) return o;
+  o.drawn = true;
+  o.cx = x_ndc;
+  o.cy = y_ndc;
+  o.ux = ux;
+  o.uy = uy;
+  o.vx = vx;
+  o.vy = vy;
+  o.alpha = soa[kAlpha * N + i];
+  // normalize3_or_default(position - camera) (wgsl:87-93, 329).
+  float dxw = 0.0f, dyw = 0.0f, dzw = 1.0f;
+  const float len2 = relx * relx + rely * rely + relz * relz;
+  if (len2 > 1e-20f) {
+    const float inv = 1.0f / sqrtf(len2);
+    dxw = relx * inv;
+    dyw = rely * inv;
+    dzw = relz * inv;
+  }
 #pragma unroll
-        for (int ch = 0; ch < 3; ++ch) {
-          const float dc = soa[(kDcR + ch) * N + i];
-          const float s = sh_channel<DEG>(dc, soa + (size_t)(kRest0 + ch * PC) * N + i, n, dxw, dyw, dzw);
-          col[ch] = fminf(fmaxf(s + 0.5f, 0.0f), 1.0f) * alpha;
-        }
+  for (int ch = 0; ch < 3; ++ch) {
+    const float dc = soa[(kDcR + ch) * N + i];
+    const float s = sh_channel<DEG>(dc, soa + (size_t)(kRest0 + ch * PC) * N + i, n, dxw, dyw, dzw);
+    o.col[ch] = fminf(fmaxf(s + 0.5f, 0.0f), 1.0f) * o.alpha;
+  }
+  return o;
+}
+
+template <int DEG>
+__global__ __launch_bounds__(kThreads) void k_project(const ViewDev *__restrict__ views,
+                                                      const uint32_t *__restrict__ seg_start, uint32_t nviews,
+                                                      uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                      float4 *__restrict__ rec, uint2 *__restrict__ rect) {
+  const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t v = find_view(seg_start, nviews, blockIdx.x * kThreads);
+  const ViewDev &vw = views[v];
+  if (vw.sh_degree != (uint32_t)DEG) return;
+  const uint32_t i = g - vw.seg_off;
+  if (i >= vw.n) {  // padding slot: never visible, sorts last in its view
+    keys[g] = 0u;
+    vals[g] = (v << kLocalBits) | kLocalMask;
+    rect[g] = make_uint2(0xFFFFFFFFu, 0u);
+    return;
+  }
+  const FrameParams &fp = vw.fp;
+  const Projected p = project_one<DEG>(fp, vw.soa, vw.n, i);
+  keys[g] = p.key;
+  vals[g] = (v << kLocalBits) | i;
+  uint2 rr = make_uint2(0xFFFFFFFFu, 0u);
+  if (p.drawn) {
+    const float cut = fminf(vw.soa[kCut * (size_t)vw.n + i], 1.0f);
+    const float det = p.ux * p.vy - p.vx * p.uy;
+    rec[3 * (size_t)g + 0] = make_float4(p.cx, p.cy, p.vy / det, -p.vx / det);
+    rec[3 * (size_t)g + 1] = make_float4(-p.uy / det, p.ux / det, cut, p.alpha);
+    rec[3 * (size_t)g + 2] = make_float4(p.col[0], p.col[1], p.col[2], 0.0f);
+    if (cut >= 0.0f) {
+      // Bounding box of {center + A l : |l|^2 <= cut} plus one pixel.
+      const float sc = sqrtf(cut);
+      const float ex = sc * sqrtf(p.ux * p.ux + p.vx * p.vx);
+      const float ey = sc * sqrtf(p.uy * p.uy + p.vy * p.vy);
+      const float hw = 0.5f * (float)fp.width, hh = 0.5f * (float)fp.height;
+      const float cxp = (p.cx + 1.0f) * hw;
+      const float cyp = (1.0f - p.cy) * hh;
+      const float exw = ex * hw + 1.0f;
+      const float eyh = ey * hh + 1.0f;
+      const float x0f = ceilf(cxp - exw - 0.5f), x1f = floorf(cxp + exw - 0.5f);
+      const float y0f = ceilf(cyp - eyh - 0.5f), y1f = floorf(cyp + eyh - 0.5f);
+      const float wm1 = (float)(fp.width - 1), hm1 = (float)(fp.height - 1);
+      if (!(x1f < 0.0f || y1f < 0.0f || x0f > wm1 || y0f > hm1 || x0f > x1f || y0f > y1f)) {
+        const uint32_t x0 = (uint32_t)fmaxf(x0f, 0.0f), x1 = (uint32_t)fminf(x1f, wm1);
+        const uint32_t y0 = (uint32_t)fmaxf(y0f, 0.0f), y1 = (uint32_t)fminf(y1f, hm1);
+        rr = make_uint2(x0 | (y0 << 16), x1 | (y1 << 16));
       }
     }
-    if (EXPORT) {
-      // invalid_splat() (wgsl:122-129): nothing but zeros for a culled splat.
-      if (!drawn) ux = uy = vx = vy = 0.0f;
-      float *o = export12 + 12 * (size_t)i;
-      o[0] = drawn ? 1.0f : 0.0f;
-      o[1] = cx;
-      o[2] = cy;
-      o[3] = ux;
-      o[4] = uy;
-      o[5] = vx;
-      o[6] = vy;
-      o[7] = col[0];
-      o[8] = col[1];
-      o[9] = col[2];
-      o[10] = alpha;
-      o[11] = 0.0f;
-    } else {
-      uint2 rr = make_uint2(0xFFFFFFFFu, 0u);
-      if (drawn) {
-        const float cut = fminf(soa[kCut * N + i], 1.0f);
-        const float det = ux * vy - vx * uy;
-        rec[3 * (size_t)i + 0] = make_float4(cx, cy, vy / det, -vx / det);
-        rec[3 * (size_t)i + 1] = make_float4(-uy / det, ux / det, cut, alpha);
-        rec[3 * (size_t)i + 2] = make_float4(col[0], col[1], col[2], 0.0f);
-        if (cut >= 0.0f) {
-          // Bounding box of {center + A l : |l|^2 <= cut} plus one pixel.
-          const float sc = sqrtf(cut);
-          const float ex = sc * sqrtf(ux * ux + vx * vx);
-          const float ey = sc * sqrtf(uy * uy + vy * vy);
-          const float hw = 0.5f * (float)fp.width, hh = 0.5f * (float)fp.height;
-          const float cxp = (cx + 1.0f) * hw;
-          const float cyp = (1.0f - cy) * hh;
-          const float exw = ex * hw + 1.0f;
-          const float eyh = ey * hh + 1.0f;
-          const float x0f = ceilf(cxp - exw - 0.5f), x1f = floorf(cxp + exw - 0.5f);
-          const float y0f = ceilf(cyp - eyh - 0.5f), y1f = floorf(cyp + eyh - 0.5f);
-          const float wm1 = (float)(fp.width - 1), hm1 = (float)(fp.height - 1);
-          if (!(x1f < 0.0f || y1f < 0.0f || x0f > wm1 || y0f > hm1 || x0f > x1f || y0f > y1f)) {
-            // Pixel bounds (the tile rectangle is these divided by kTile).
-            const uint32_t x0 = (uint32_t)fmaxf(x0f, 0.0f), x1 = (uint32_t)fminf(x1f, wm1);
-            const uint32_t y0 = (uint32_t)fmaxf(y0f, 0.0f), y1 = (uint32_t)fminf(y1f, hm1);
-            rr = make_uint2(x0 | (y0 << 16), x1 | (y1 << 16));
-          }
-        }
-      }
-      rect[i] = rr;
-    }
   }
-  if (!EXPORT) {
-    // Visible count: wave ballot, one LDS slot per wave, one atomic per block.
-    __shared__ uint32_t wcount[kWaves];
-    const uint64_t m = __ballot(vis);
-    if (__lane_id() == 0) wcount[threadIdx.x / kWave] = (uint32_t)__popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t t = 0;
-      for (uint32_t w = 0; w < kWaves; ++w) t += wcount[w];
-      if (t) atomicAdd(visible, t);
-    }
-  }
+  rect[g] = rr;
+}
+
+template <int DEG>
+__global__ __launch_bounds__(kThreads) void k_project_export(const ViewDev *__restrict__ views, uint32_t v,
+                                                             uint32_t *__restrict__ keys_out,
+                                                             float *__restrict__ export12) {
+  const ViewDev &vw = views[v];
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= vw.n) return;
+  const Projected p = project_one<DEG>(vw.fp, vw.soa, vw.n, i);
+  if (keys_out) keys_out[i] = p.key;
+  float *o = export12 + 12 * (size_t)i;
+  // invalid_splat() (wgsl:122-129): zeros for a splat that is not drawn.
+  o[0] = p.drawn ? 1.0f : 0.0f;
+  o[1] = p.cx;
+  o[2] = p.cy;
+  o[3] = p.drawn ? p.ux : 0.0f;
+  o[4] = p.drawn ? p.uy : 0.0f;
+  o[5] = p.drawn ? p.vx : 0.0f;
+  o[6] = p.drawn ? p.vy : 0.0f;
+  o[7] = p.col[0];
+  o[8] = p.col[1];
+  o[9] = p.col[2];
+  o[10] = p.alpha;
+  o[11] = 0.0f;
 }
 
 // ------------------------------------------------------------------- scans
@@ -344,7 +373,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t *__restrict__ ou
 
 // ------------------------------------------------------------- tile binning
 
-// rect holds the pixel bounds x0|y0<<16, x1|y1<<16 (x = ~0u: no pixel).
+// Number of 16x16 tiles of a pixel-bounds rectangle (x0|y0<<16, x1|y1<<16).
 __device__ __forceinline__ uint32_t rect_tiles(uint2 r) {
   if (r.x == 0xFFFFFFFFu) return 0u;
   const uint32_t x0 = (r.x & 0xFFFFu) / kTile, y0 = (r.x >> 16) / kTile;
@@ -352,84 +381,78 @@ __device__ __forceinline__ uint32_t rect_tiles(uint2 r) {
   return (x1 - x0 + 1) * (y1 - y0 + 1);
 }
 
-__global__ __launch_bounds__(kThreads) void k_tile_counts(const uint32_t *__restrict__ sorted,
-                                                          const uint32_t *__restrict__ visible,
-                                                          const uint2 *__restrict__ rect, uint32_t n,
-                                                          uint32_t *__restrict__ counts) {
-  const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-  if (j >= n) return;
-  const uint32_t v = *visible;
-  counts[j] = j < v ? rect_tiles(rect[sorted[v - 1 - j]]) : 0u;
+__global__ __launch_bounds__(kThreads) void k_tile_counts(const ViewDev *__restrict__ views,
+                                                          const uint32_t *__restrict__ seg_start, uint32_t nviews,
+                                                          uint32_t total, const uint32_t *__restrict__ skeys,
+                                                          const uint32_t *__restrict__ svals,
+                                                          const uint2 *__restrict__ rect,
+                                                          uint32_t *__restrict__ counts_rev,
+                                                          uint32_t *__restrict__ visible) {
+  const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t v = find_view(seg_start, nviews, blockIdx.x * kThreads);
+  const ViewDev &vw = views[v];
+  const uint32_t key = skeys[g];
+  uint32_t c = 0;
+  const uint32_t local = svals[g] - (v << kLocalBits);
+  if (key != 0u && local < vw.n) {
+    c = rect_tiles(rect[vw.seg_off + local]);
+    const uint32_t j = g - vw.seg_off;
+    if (j + 1 == vw.seg_len || skeys[g + 1] == 0u) visible[v] = j + 1;
+  }
+  counts_rev[total - 1 - g] = c;
 }
 
-__global__ void k_pairs_finalize(const uint32_t *__restrict__ visible, const uint64_t *__restrict__ total,
-                                 uint32_t capacity, uint32_t *__restrict__ m_out, uint32_t *__restrict__ info) {
+__global__ void k_pairs_finalize(const uint64_t *__restrict__ total, uint32_t capacity, uint32_t *__restrict__ m_out,
+                                 uint32_t *__restrict__ info) {
   if (threadIdx.x != 0) return;
   const uint64_t m = *total;
   *m_out = m > capacity ? capacity : (uint32_t)m;
-  info[0] = *visible;
   info[1] = (uint32_t)m;
   info[2] = (uint32_t)(m >> 32);
   info[3] = m > capacity ? 1u : 0u;
 }
 
-// Load-balanced expansion: workgroup b owns front-to-back positions
-// [256b, 256b+256), whose pairs are one contiguous range of the output. The
-// positions' first pair index, tile rectangle and splat id are staged in
-// LDS; then each pass writes 256 consecutive pairs, every thread finding its
-// pair's position by binary search over the staged first-pair indices.
-__global__ __launch_bounds__(kThreads) void k_tile_emit(const uint32_t *__restrict__ sorted,
-                                                        const uint32_t *__restrict__ visible,
+// Load-balanced expansion. Workgroup b owns sorted slots [256b, 256b+256) of
+// one view; their pairs are one contiguous range of the output, nearest slot
+// first. The slots' first pair index, tile rectangle and record slot are
+// staged in LDS in pair order; then each pass writes 256 consecutive pairs,
+// every thread finding its pair's slot by binary search.
+__global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restrict__ views,
+                                                        const uint32_t *__restrict__ seg_start, uint32_t nviews,
+                                                        uint32_t total, const uint32_t *__restrict__ svals,
                                                         const uint2 *__restrict__ rect,
-                                                        const uint32_t *__restrict__ offsets, uint32_t n,
-                                                        uint32_t tiles_x, uint32_t capacity,
+                                                        const uint32_t *__restrict__ counts_rev,
+                                                        const uint32_t *__restrict__ offsets_rev, uint32_t capacity,
                                                         uint32_t *__restrict__ pair_keys,
                                                         uint32_t *__restrict__ pair_vals) {
   __shared__ uint32_t s_first[kThreads + 1];
-  __shared__ uint32_t s_splat[kThreads];
+  __shared__ uint32_t s_slot[kThreads];
   __shared__ uint32_t s_x0[kThreads], s_y0[kThreads], s_w[kThreads];
-  const uint32_t v = min(*visible, n);
-  const uint32_t j0 = blockIdx.x * kThreads;
-  if (j0 >= v) return;
-  const uint32_t cnt = min(kThreads, v - j0);
+  const uint32_t g0 = blockIdx.x * kThreads;
+  const uint32_t v = find_view(seg_start, nviews, g0);
+  const ViewDev &vw = views[v];
   const uint32_t t = threadIdx.x;
-  if (t < cnt) {
-    const uint32_t s = sorted[v - 1 - (j0 + t)];
-    const uint2 r = rect[s];
-    s_first[t] = offsets[j0 + t];
-    s_splat[t] = s;
-    if (r.x == 0xFFFFFFFFu) {
-      s_x0[t] = s_y0[t] = 0;
-      s_w[t] = 1;
-    } else {
-      const uint32_t x0 = (r.x & 0xFFFFu) / kTile, x1 = (r.y & 0xFFFFu) / kTile;
-      s_x0[t] = x0;
-      s_y0[t] = (r.x >> 16) / kTile;
-      s_w[t] = x1 - x0 + 1;
-    }
-  }
-  if (t == 0) {
-    // End of the last position's pairs: next position's offset, or the
-    // last position's offset plus its own tile count.
-    const uint32_t last = j0 + cnt - 1;
-    uint32_t end;
-    if (last + 1 < v) {
-      end = offsets[last + 1];
-    } else {
-      const uint2 r = rect[sorted[v - 1 - last]];
-      uint32_t c = 0;
-      if (r.x != 0xFFFFFFFFu)
-        c = ((r.y & 0xFFFFu) / kTile - (r.x & 0xFFFFu) / kTile + 1) * ((r.y >> 16) / kTile - (r.x >> 16) / kTile + 1);
-      end = offsets[last] + c;
-    }
-    s_first[cnt] = end;
+  // Staged entry t is sorted slot g0 + 255 - t (nearest first).
+  const uint32_t r = total - 1 - (g0 + kThreads - 1 - t);
+  const uint32_t cnt = counts_rev[r];
+  s_first[t] = offsets_rev[r];
+  if (t == kThreads - 1) s_first[kThreads] = offsets_rev[r] + cnt;
+  if (cnt) {
+    const uint32_t slot = vw.seg_off + (svals[g0 + kThreads - 1 - t] - (v << kLocalBits));
+    const uint2 b = rect[slot];
+    const uint32_t x0 = (b.x & 0xFFFFu) / kTile, x1 = (b.y & 0xFFFFu) / kTile;
+    s_slot[t] = slot;
+    s_x0[t] = x0;
+    s_y0[t] = (b.x >> 16) / kTile;
+    s_w[t] = x1 - x0 + 1;
   }
   __syncthreads();
-  const uint32_t p_begin = s_first[0], p_end = min(s_first[cnt], capacity);
+  const uint32_t p_begin = s_first[0], p_end = min(s_first[kThreads], capacity);
+  const uint32_t tiles_x = vw.fp.tiles_x;
   for (uint32_t p = p_begin + t; p < p_end; p += kThreads) {
-    // Largest i < cnt with s_first[i] <= p (positions with no pair repeat
-    // the next offset and are skipped by taking the largest).
-    uint32_t lo = 0, hi = cnt - 1;
+    // Largest i with s_first[i] <= p: the entry whose range holds p (entries
+    // without pairs share the next entry's first index and are passed over).
+    uint32_t lo = 0, hi = kThreads - 1;
     while (lo < hi) {
       const uint32_t mid = (lo + hi + 1) / 2;
       if (s_first[mid] <= p)
@@ -439,8 +462,8 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const uint32_t *__restri
     }
     const uint32_t k = p - s_first[lo];
     const uint32_t w = s_w[lo];
-    pair_keys[p] = (s_y0[lo] + k / w) * tiles_x + s_x0[lo] + k % w;
-    pair_vals[p] = s_splat[lo];
+    pair_keys[p] = vw.tile_off + (s_y0[lo] + k / w) * tiles_x + s_x0[lo] + k % w;
+    pair_vals[p] = s_slot[lo];
   }
 }
 
@@ -468,38 +491,27 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
 // the reference's back-to-front premultiplied "over", draw_pass.rs:75, up to
 // that bound). Coverage is r^2 <= cut (the fragment test of wgsl:358-367
 // precomputed per splat); g = exp(-4.5 r^2) (wgsl:363).
-//   QUAD  each wave owns an 8x8 quadrant of the tile; the splat's pixel
-//         bounds are staged with its record and a wave skips, as a whole,
-//         every splat whose bounds miss its quadrant
-//   XCD   workgroup b runs on XCD b mod 8 (round-robin dispatch); tiles are
-//         banded so that each XCD renders one contiguous range of tiles and
-//         neighbouring tiles share that XCD's L2
-template <bool QUAD, bool XCD>
-__global__ __launch_bounds__(kThreads) void k_raster(const float4 *__restrict__ rec, const uint2 *__restrict__ rect,
+template <bool XCD>
+__global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__ views,
+                                                     const uint32_t *__restrict__ tile_start, uint32_t nviews,
+                                                     uint32_t total_tiles, const float4 *__restrict__ rec,
                                                      const uint2 *__restrict__ ranges,
-                                                     const uint32_t *__restrict__ list, uint32_t width,
-                                                     uint32_t height, uint32_t tiles_x, uint32_t tiles,
-                                                     float4 *__restrict__ out32, uchar4 *__restrict__ out8) {
+                                                     const uint32_t *__restrict__ list, float4 *__restrict__ img32,
+                                                     uchar4 *__restrict__ img8) {
   __shared__ float4 s0[kThreads], s1[kThreads], s2[kThreads];
-  __shared__ uint2 sb[QUAD ? kThreads : 1];
   uint32_t tile = blockIdx.x;
   if (XCD) {
-    const uint32_t band = (tiles + 7) / 8;
+    const uint32_t band = (total_tiles + 7) / 8;
     tile = (blockIdx.x % 8) * band + blockIdx.x / 8;
   }
-  if (tile >= tiles) return;
-  const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
-  const uint32_t t = threadIdx.x, lane = t % kWave, w = t / kWave;
-  uint32_t x, y, qx0 = 0, qy0 = 0;
-  if (QUAD) {
-    qx0 = tx * kTile + (w & 1u) * 8u;
-    qy0 = ty * kTile + (w >> 1) * 8u;
-    x = qx0 + (lane & 7u);
-    y = qy0 + (lane >> 3);
-  } else {
-    x = tx * kTile + (t % kTile);
-    y = ty * kTile + (t / kTile);
-  }
+  if (tile >= total_tiles) return;
+  const uint32_t v = find_view(tile_start, nviews, tile);
+  const ViewDev &vw = views[v];
+  const uint32_t width = vw.fp.width, height = vw.fp.height;
+  const uint32_t local = tile - vw.tile_off;
+  const uint32_t tx = local % vw.fp.tiles_x, ty = local / vw.fp.tiles_x;
+  const uint32_t t = threadIdx.x;
+  const uint32_t x = tx * kTile + (t % kTile), y = ty * kTile + (t / kTile);
   const bool inside = x < width && y < height;
   const float pnx = (2.0f * ((float)x + 0.5f)) / (float)width - 1.0f;
   const float pny = 1.0f - (2.0f * ((float)y + 0.5f)) / (float)height;
@@ -514,17 +526,11 @@ __global__ __launch_bounds__(kThreads) void k_raster(const float4 *__restrict__ 
       s0[t] = rec[3 * s + 0];
       s1[t] = rec[3 * s + 1];
       s2[t] = rec[3 * s + 2];
-      if (QUAD) sb[t] = rect[s];
     }
     __syncthreads();
     const uint32_t cnt = min(kThreads, rg.y - start);
     if (!done) {
       for (uint32_t j = 0; j < cnt; ++j) {
-        if (QUAD) {
-          const uint2 r = sb[j];
-          if ((r.y & 0xFFFFu) < qx0 || (r.x & 0xFFFFu) > qx0 + 7u || (r.y >> 16) < qy0 || (r.x >> 16) > qy0 + 7u)
-            continue;
-        }
         const float4 a = s0[j];
         const float4 b = s1[j];
         const float dx = pnx - a.x;
@@ -549,31 +555,15 @@ __global__ __launch_bounds__(kThreads) void k_raster(const float4 *__restrict__ 
     }
   }
   if (inside) {
-    const size_t o = (size_t)y * width + x;
+    const size_t o = vw.img_off + (size_t)y * width + x;
     const float A = 1.0f - T;
-    out32[o] = make_float4(cr, cg, cb, A);
-    auto q = [](float v) { return (unsigned char)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f); };
-    out8[o] = make_uchar4(q(cr), q(cg), q(cb), q(A));
+    img32[o] = make_float4(cr, cg, cb, A);
+    auto q = [](float val) { return (unsigned char)(fminf(fmaxf(val, 0.0f), 1.0f) * 255.0f + 0.5f); };
+    img8[o] = make_uchar4(q(cr), q(cg), q(cb), q(A));
   }
 }
 
 uint32_t div_up(size_t a, size_t b) { return (uint32_t)((a + b - 1) / b); }
-
-template <bool EXPORT>
-void project_dispatch(uint32_t deg, const float *soa, uint32_t n, const FrameParams &fp, uint32_t *keys,
-                      uint32_t *vals, float4 *rec, uint2 *rect, uint32_t *visible, float *export12,
-                      hipStream_t st) {
-  const dim3 grid(div_up(n, kThreads)), block(kThreads);
-  switch (deg) {
-    case 0: k_project<0, EXPORT><<<grid, block, 0, st>>>(soa, n, fp, keys, vals, rec, rect, visible, export12); break;
-    case 1: k_project<1, EXPORT><<<grid, block, 0, st>>>(soa, n, fp, keys, vals, rec, rect, visible, export12); break;
-    case 2: k_project<2, EXPORT><<<grid, block, 0, st>>>(soa, n, fp, keys, vals, rec, rect, visible, export12); break;
-    case 3: k_project<3, EXPORT><<<grid, block, 0, st>>>(soa, n, fp, keys, vals, rec, rect, visible, export12); break;
-    case 4: k_project<4, EXPORT><<<grid, block, 0, st>>>(soa, n, fp, keys, vals, rec, rect, visible, export12); break;
-    default: throw Error(Code::Unsupported, "sh degree above 4");
-  }
-  GS_HIP(hipGetLastError());
-}
 
 }  // namespace
 
@@ -604,14 +594,34 @@ FrameParams make_frame_params(const float position[3], const float rotation_xyzw
   return p;
 }
 
-void launch_project(uint32_t deg, const float *soa, uint32_t n, const FrameParams &fp, uint32_t *keys,
-                    uint32_t *vals, float4 *rec, uint2 *rect, uint32_t *visible, float *export12,
-                    hipStream_t st) {
+void launch_project(uint32_t deg, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,
+                    uint32_t total_slots, uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect, hipStream_t st) {
+  if (total_slots == 0 || nviews == 0) return;
+  const dim3 grid(div_up(total_slots, kThreads)), block(kThreads);
+  switch (deg) {
+    case 0: k_project<0><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect); break;
+    case 1: k_project<1><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect); break;
+    case 2: k_project<2><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect); break;
+    case 3: k_project<3><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect); break;
+    case 4: k_project<4><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect); break;
+    default: throw Error(Code::Unsupported, "sh degree above 4");
+  }
+  GS_HIP(hipGetLastError());
+}
+
+void launch_project_export(uint32_t deg, const ViewDev *views, uint32_t view, uint32_t n, uint32_t *keys_out,
+                           float *export12, hipStream_t st) {
   if (n == 0) return;
-  if (export12)
-    project_dispatch<true>(deg, soa, n, fp, keys, vals, rec, rect, visible, export12, st);
-  else
-    project_dispatch<false>(deg, soa, n, fp, keys, vals, rec, rect, visible, nullptr, st);
+  const dim3 grid(div_up(n, kThreads)), block(kThreads);
+  switch (deg) {
+    case 0: k_project_export<0><<<grid, block, 0, st>>>(views, view, keys_out, export12); break;
+    case 1: k_project_export<1><<<grid, block, 0, st>>>(views, view, keys_out, export12); break;
+    case 2: k_project_export<2><<<grid, block, 0, st>>>(views, view, keys_out, export12); break;
+    case 3: k_project_export<3><<<grid, block, 0, st>>>(views, view, keys_out, export12); break;
+    case 4: k_project_export<4><<<grid, block, 0, st>>>(views, view, keys_out, export12); break;
+    default: throw Error(Code::Unsupported, "sh degree above 4");
+  }
+  GS_HIP(hipGetLastError());
 }
 
 size_t scan_partial_entries(size_t n) { return std::max<size_t>(div_up(n, kScanTile), 1); }
@@ -629,25 +639,28 @@ void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s,
   GS_HIP(hipGetLastError());
 }
 
-void launch_pairs_finalize(const uint32_t *visible, const uint64_t *total, uint32_t capacity, uint32_t *m_out,
-                           uint32_t *info, hipStream_t st) {
-  k_pairs_finalize<<<1, 64, 0, st>>>(visible, total, capacity, m_out, info);
+void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
+                        const uint32_t *skeys, const uint32_t *svals, const uint2 *rect, uint32_t *counts_rev,
+                        uint32_t *visible, hipStream_t st) {
+  if (total_slots == 0) return;
+  k_tile_counts<<<total_slots / kThreads, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, skeys, svals,
+                                                               rect, counts_rev, visible);
   GS_HIP(hipGetLastError());
 }
 
-void launch_tile_counts(const uint32_t *sorted, const uint32_t *visible, const uint2 *rect, uint32_t n,
-                        uint32_t *counts, hipStream_t st) {
-  if (n == 0) return;
-  k_tile_counts<<<div_up(n, kThreads), kThreads, 0, st>>>(sorted, visible, rect, n, counts);
+void launch_pairs_finalize(const uint64_t *total, uint32_t capacity, uint32_t *m_out, uint32_t *info,
+                           hipStream_t st) {
+  k_pairs_finalize<<<1, 64, 0, st>>>(total, capacity, m_out, info);
   GS_HIP(hipGetLastError());
 }
 
-void launch_tile_emit(const uint32_t *sorted, const uint32_t *visible, const uint2 *rect, const uint32_t *offsets,
-                      uint32_t n, uint32_t tiles_x, uint32_t capacity, uint32_t *pair_keys, uint32_t *pair_vals,
+void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
+                      const uint32_t *svals, const uint2 *rect, const uint32_t *counts_rev,
+                      const uint32_t *offsets_rev, uint32_t capacity, uint32_t *pair_keys, uint32_t *pair_vals,
                       hipStream_t st) {
-  if (n == 0) return;
-  k_tile_emit<<<div_up(n, kThreads), kThreads, 0, st>>>(sorted, visible, rect, offsets, n, tiles_x, capacity,
-                                                         pair_keys, pair_vals);
+  if (total_slots == 0) return;
+  k_tile_emit<<<total_slots / kThreads, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, svals, rect,
+                                                             counts_rev, offsets_rev, capacity, pair_keys, pair_vals);
   GS_HIP(hipGetLastError());
 }
 
@@ -665,25 +678,16 @@ void launch_tile_ranges(const uint32_t *pair_keys, const uint32_t *m_dev, uint32
   GS_HIP(hipGetLastError());
 }
 
-void launch_raster(RasterVariant variant, const float4 *rec, const uint2 *rect, const uint2 *ranges,
-                   const uint32_t *list, uint32_t width, uint32_t height, uint32_t tiles_x, uint32_t tiles_y,
-                   float4 *out32, uchar4 *out8, hipStream_t st) {
-  const uint32_t tiles = tiles_x * tiles_y;
-  if (tiles == 0) return;
-  switch (variant) {
-    case RasterVariant::TileLds:
-      k_raster<false, false><<<tiles, kThreads, 0, st>>>(rec, rect, ranges, list, width, height, tiles_x, tiles,
-                                                         out32, out8);
-      break;
-    case RasterVariant::QuadWave:
-      k_raster<true, false><<<tiles, kThreads, 0, st>>>(rec, rect, ranges, list, width, height, tiles_x, tiles,
-                                                        out32, out8);
-      break;
-    case RasterVariant::QuadWaveXcd:
-      k_raster<true, true><<<8 * ((tiles + 7) / 8), kThreads, 0, st>>>(rec, rect, ranges, list, width, height,
-                                                                        tiles_x, tiles, out32, out8);
-      break;
-  }
+void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
+                   uint32_t total_tiles, const float4 *rec, const uint2 *ranges, const uint32_t *list,
+                   float4 *img32, uchar4 *img8, hipStream_t st) {
+  if (total_tiles == 0 || nviews == 0) return;
+  if (variant == RasterVariant::XcdBand)
+    k_raster<true><<<8 * div_up(total_tiles, 8), kThreads, 0, st>>>(views, tile_start, nviews, total_tiles, rec,
+                                                                     ranges, list, img32, img8);
+  else
+    k_raster<false><<<total_tiles, kThreads, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list,
+                                                       img32, img8);
   GS_HIP(hipGetLastError());
 }
 

@@ -1,0 +1,361 @@
+// pipeline.cpp -- batched frame pipeline of the SortedAlpha path.
+
+#include "pipeline.h"
+
+#include <cstring>
+
+#include "sort.h"
+
+namespace gs {
+
+namespace {
+
+constexpr size_t kTableBytes = 2 * kMaxViews * sizeof(uint32_t) + kMaxViews * sizeof(ViewDev);
+constexpr size_t kSummaryWords = 4 + kMaxViews;
+
+uint32_t bits_for(uint64_t count) {
+  uint32_t b = 1;
+  while (b < 32 && (1ull << b) < count) ++b;
+  return b;
+}
+
+uint32_t initial_pair_capacity(uint64_t splats) {
+  return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * splats, 1u << 20), 1u << 30);
+}
+
+}  // namespace
+
+bool device_present() {
+  int n = 0;
+  return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+}
+
+std::shared_ptr<DeviceScene> upload_scene(const HostScene &s, hipStream_t stream) {
+  PreparedScene p = prepare_scene(s);
+  auto d = std::make_shared<DeviceScene>();
+  d->count = p.count;
+  d->sh_degree = p.sh_degree;
+  d->soa.ensure(p.soa.size());
+  if (!p.soa.empty()) {
+    GS_HIP(hipMemcpyAsync(d->soa.get(), p.soa.data(), p.soa.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    GS_HIP(hipStreamSynchronize(stream));
+  }
+  return d;
+}
+
+uint64_t frame_working_bytes(uint64_t slots, uint64_t pixels, uint64_t tiles, uint64_t pair_cap) {
+  // keys, vals and their alternates, record (48 B), rect, counts, offsets.
+  const uint64_t per_slot = 4 * 4 + 16 * kRecordFloat4 + 8 + 4 + 4;
+  // pair keys/vals and alternates, plus the sort status words of the larger
+  // of the two sorts.
+  const uint64_t sort_words = onesweep_status_words((uint32_t)std::max<uint64_t>(slots, pair_cap), 32);
+  return per_slot * slots + 16 * pixels + 20 * pixels / 16 + 8 * tiles + 16 * pair_cap + 4 * sort_words;
+}
+
+Pipeline::Pipeline() {
+  if (!device_present()) throw Error(Code::Unsupported, "gpu rasterizer unavailable");
+  GS_HIP(hipSetDevice(0));
+  GS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  for (auto &e : ev_) GS_HIP(hipEventCreate(&e));
+  for (int r = 0; r < 2; ++r) {
+    GS_HIP(hipEventCreateWithFlags(&table_ev_[r], hipEventDisableTiming));
+    GS_HIP(hipEventRecord(table_ev_[r], stream_));
+    GS_HIP(hipHostMalloc(reinterpret_cast<void **>(&table_pinned_[r]), kTableBytes, hipHostMallocDefault));
+  }
+  GS_HIP(hipHostMalloc(reinterpret_cast<void **>(&summary_pinned_), kSummaryWords * sizeof(uint32_t),
+                       hipHostMallocDefault));
+  std::memset(summary_pinned_, 0, kSummaryWords * sizeof(uint32_t));
+  table_.ensure(kTableBytes);
+  summary_.ensure(kSummaryWords);
+  total_.ensure(1);
+  m_dev_.ensure(1);
+  export_view_.ensure(1);
+}
+
+Pipeline::~Pipeline() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto &e : ev_)
+    if (e) (void)hipEventDestroy(e);
+  for (int r = 0; r < 2; ++r) {
+    if (table_ev_[r]) (void)hipEventDestroy(table_ev_[r]);
+    if (table_pinned_[r]) (void)hipHostFree(table_pinned_[r]);
+  }
+  if (summary_pinned_) (void)hipHostFree(summary_pinned_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_rgba8) {
+  if (views.empty()) throw Error(Code::InvalidArgument, "batch has no views");
+  if (!host_rgba8.empty() && host_rgba8.size() != views.size())
+    throw Error(Code::InvalidArgument, "one host target per view expected");
+  if (views.size() > kMaxViews)
+    throw Error(Code::InvalidArgument, "batch of " + std::to_string(views.size()) + " views exceeds " +
+                                           std::to_string(kMaxViews));
+  // A finished earlier frame that overflowed only grows the pair buffers:
+  // its results are replaced by this frame.
+  if (pending_ && hipEventQuery(ev_[5]) == hipSuccess) {
+    absorb_summary();
+    pending_ = false;
+  }
+  const bool multi = views.size() > 1;
+  std::vector<ViewDev> table(views.size());
+  uint64_t slots = 0, tiles = 0, pixels = 0, splats = 0;
+  for (size_t v = 0; v < views.size(); ++v) {
+    const ViewSpec &s = views[v];
+    if (!s.scene) throw Error(Code::SceneNotLoaded, "scene not loaded");
+    const uint32_t n = s.scene->count;
+    if (multi && n > kLocalMask)
+      throw Error(Code::Unsupported, "a scene in a batch of several views holds at most " +
+                                         std::to_string(kLocalMask) + " splats");
+    ViewDev &d = table[v];
+    d.fp = s.fp;
+    d.soa = s.scene->soa.get();
+    d.n = n;
+    d.sh_degree = s.scene->sh_degree;
+    d.seg_off = (uint32_t)slots;
+    d.seg_len = (uint32_t)(((uint64_t)n + kSegmentAlign - 1) / kSegmentAlign * kSegmentAlign);
+    d.tile_off = (uint32_t)tiles;
+    d.tiles = s.fp.tiles_x * s.fp.tiles_y;
+    d.img_off = pixels;
+    slots += d.seg_len;
+    tiles += d.tiles;
+    pixels += (uint64_t)s.fp.width * s.fp.height;
+    splats += n;
+    if (slots > (1ull << 30)) throw Error(Code::Unsupported, "batch exceeds 2^30 splat slots");
+  }
+  if (tiles >= (1ull << 30)) throw Error(Code::Unsupported, "batch exceeds 2^30 tiles");
+
+  views_ = std::move(views);
+  host_rgba8_ = std::move(host_rgba8);
+  host_views_ = std::move(table);
+  total_slots_ = (uint32_t)slots;
+  total_tiles_ = (uint32_t)tiles;
+  total_pixels_ = pixels;
+  total_splats_ = splats;
+  if (pair_cap_ == 0) pair_cap_ = initial_pair_capacity(splats);
+  launch_frame();
+}
+
+void Pipeline::launch_frame() {
+  const uint32_t nviews = (uint32_t)views_.size();
+  const uint32_t slots = total_slots_, tiles = total_tiles_;
+  const uint32_t tile_bits = bits_for(tiles);
+  const size_t m = std::max<uint32_t>(slots, 1);
+  keys_.ensure(m);
+  vals_.ensure(m);
+  keys_alt_.ensure(m);
+  vals_alt_.ensure(m);
+  rec_.ensure(kRecordFloat4 * m);
+  rect_.ensure(m);
+  counts_.ensure(m);
+  offsets_.ensure(m);
+  partials_.ensure(scan_partial_entries(m));
+  img32_.ensure(total_pixels_);
+  img8_.ensure(total_pixels_);
+  ranges_.ensure(std::max<uint32_t>(tiles, 1));
+  pair_keys_.ensure(pair_cap_);
+  pair_vals_.ensure(pair_cap_);
+  pair_keys_alt_.ensure(pair_cap_);
+  pair_vals_alt_.ensure(pair_cap_);
+  sort_scratch_.ensure(std::max(onesweep_status_words((uint32_t)m, 32), onesweep_status_words(pair_cap_, tile_bits)));
+
+  hipStream_t st = stream_;
+  // View table through a pinned staging slot whose previous upload is done.
+  const int r = (int)(frame_counter_++ % 2);
+  GS_HIP(hipEventSynchronize(table_ev_[r]));
+  uint8_t *host = table_pinned_[r];
+  uint32_t *seg_start = reinterpret_cast<uint32_t *>(host);
+  uint32_t *tile_start = seg_start + kMaxViews;
+  for (uint32_t v = 0; v < nviews; ++v) {
+    seg_start[v] = host_views_[v].seg_off;
+    tile_start[v] = host_views_[v].tile_off;
+  }
+  std::memcpy(host + 2 * kMaxViews * sizeof(uint32_t), host_views_.data(), nviews * sizeof(ViewDev));
+  GS_HIP(hipMemcpyAsync(table_.get(), host, 2 * kMaxViews * sizeof(uint32_t) + nviews * sizeof(ViewDev),
+                        hipMemcpyHostToDevice, st));
+  GS_HIP(hipEventRecord(table_ev_[r], st));
+  const uint32_t *seg_start_d = reinterpret_cast<const uint32_t *>(table_.get());
+  const uint32_t *tile_start_d = seg_start_d + kMaxViews;
+  const ViewDev *views_d = reinterpret_cast<const ViewDev *>(table_.get() + 2 * kMaxViews * sizeof(uint32_t));
+  uint32_t *info = summary_.get();
+  uint32_t *visible = info + 4;
+  ScanScratch sc{partials_.get(), total_.get(), partials_.capacity()};
+
+  GS_HIP(hipEventRecord(ev_[0], st));
+  launch_fill_u32(visible, 0u, nviews, st);
+  bool degree_seen[5] = {false, false, false, false, false};
+  for (const ViewDev &d : host_views_) degree_seen[std::min<uint32_t>(d.sh_degree, 4)] = true;
+  for (uint32_t deg = 0; deg <= 4; ++deg)
+    if (degree_seen[deg])
+      launch_project(deg, views_d, seg_start_d, nviews, slots, keys_.get(), vals_.get(), rec_.get(), rect_.get(), st);
+  GS_HIP(hipEventRecord(ev_[1], st));
+
+  uint32_t *k = keys_.get(), *v = vals_.get(), *ka = keys_alt_.get(), *va = vals_alt_.get();
+  onesweep_sort(&k, &v, &ka, &va, slots, nullptr, 32, /*descending=*/true, sort_scratch_.get(),
+                sort_scratch_.capacity(), st, nviews > 1 ? 3 : -1);
+  sorted_vals_ = v;
+  GS_HIP(hipEventRecord(ev_[2], st));
+
+  launch_tile_counts(views_d, seg_start_d, nviews, slots, k, v, rect_.get(), counts_.get(), visible, st);
+  exclusive_scan(counts_.get(), offsets_.get(), slots, sc, st);
+  launch_pairs_finalize(total_.get(), pair_cap_, m_dev_.get(), info, st);
+  launch_tile_emit(views_d, seg_start_d, nviews, slots, v, rect_.get(), counts_.get(), offsets_.get(), pair_cap_,
+                   pair_keys_.get(), pair_vals_.get(), st);
+  uint32_t *pk = pair_keys_.get(), *pv = pair_vals_.get(), *pka = pair_keys_alt_.get(), *pva = pair_vals_alt_.get();
+  onesweep_sort(&pk, &pv, &pka, &pva, pair_cap_, m_dev_.get(), tile_bits, /*descending=*/false,
+                sort_scratch_.get(), sort_scratch_.capacity(), st);
+  launch_tile_ranges(pk, m_dev_.get(), pair_cap_, tiles, ranges_.get(), st);
+  GS_HIP(hipEventRecord(ev_[3], st));
+
+  launch_raster(raster_variant_, views_d, tile_start_d, nviews, tiles, rec_.get(), ranges_.get(), pv, img32_.get(),
+                img8_.get(), st);
+  GS_HIP(hipEventRecord(ev_[4], st));
+  for (size_t i = 0; i < host_rgba8_.size(); ++i) {
+    if (!host_rgba8_[i]) continue;
+    const ViewDev &d = host_views_[i];
+    GS_HIP(hipMemcpyAsync(host_rgba8_[i], img8_.get() + d.img_off, (size_t)d.fp.width * d.fp.height * 4,
+                          hipMemcpyDeviceToHost, st));
+  }
+  GS_HIP(hipMemcpyAsync(summary_pinned_, info, (4 + nviews) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  GS_HIP(hipEventRecord(ev_[5], st));
+  pending_ = true;
+  summary_valid_ = false;
+}
+
+bool Pipeline::absorb_summary() {
+  const uint32_t nviews = (uint32_t)views_.size();
+  visible_.assign(summary_pinned_ + 4, summary_pinned_ + 4 + nviews);
+  pairs_ = ((uint64_t)summary_pinned_[2] << 32) | summary_pinned_[1];
+  const bool overflow = summary_pinned_[3] != 0;
+  if (overflow) {
+    const uint64_t want = pairs_ + pairs_ / 4 + 4096;
+    if (want > (1ull << 30)) throw Error(Code::Unsupported, "tile pair count exceeds 2^30");
+    pair_cap_ = (uint32_t)want;
+  }
+  return overflow;
+}
+
+void Pipeline::wait() {
+  if (!stream_) return;
+  for (int attempt = 0; pending_; ++attempt) {
+    GS_HIP(hipEventSynchronize(ev_[5]));
+    if (absorb_summary()) {
+      if (attempt > 4) throw Error(Code::Internal, "tile pair buffers keep overflowing");
+      ++overflow_rerenders_;
+      launch_frame();
+      continue;
+    }
+    float ms[4] = {0, 0, 0, 0}, total = 0;
+    for (int s = 0; s < 4; ++s) GS_HIP(hipEventElapsedTime(&ms[s], ev_[s], ev_[s + 1]));
+    GS_HIP(hipEventElapsedTime(&total, ev_[0], ev_[4]));
+    times_ = FrameTimes{ms[0], ms[1], ms[2], ms[3], total};
+    pending_ = false;
+    summary_valid_ = true;
+  }
+  GS_HIP(hipStreamSynchronize(stream_));
+}
+
+void Pipeline::check_view(uint32_t v) const {
+  if (views_.empty()) throw Error(Code::SceneNotLoaded, "no rendered frame");
+  if (v >= views_.size())
+    throw Error(Code::InvalidArgument, "view " + std::to_string(v) + " out of range for a batch of " +
+                                           std::to_string(views_.size()));
+}
+
+const ViewDev &Pipeline::view(uint32_t v) const {
+  check_view(v);
+  return host_views_[v];
+}
+
+uint32_t Pipeline::visible(uint32_t v) {
+  check_view(v);
+  wait();
+  return visible_[v];
+}
+
+uint64_t Pipeline::pairs() {
+  wait();
+  return pairs_;
+}
+
+FrameTimes Pipeline::times() {
+  wait();
+  return times_;
+}
+
+void Pipeline::readback_rgba8(uint32_t v, uint8_t *out, uint64_t len) {
+  check_view(v);
+  wait();
+  const ViewDev &d = host_views_[v];
+  const uint64_t need = (uint64_t)d.fp.width * d.fp.height * 4;
+  if (len < need) throw Error(Code::InvalidArgument, "output buffer smaller than width*height*4 bytes");
+  GS_HIP(hipMemcpyAsync(out, img8_.get() + d.img_off, need, hipMemcpyDeviceToHost, stream_));
+  GS_HIP(hipStreamSynchronize(stream_));
+}
+
+void Pipeline::readback_rgba32f(uint32_t v, float *out, uint64_t len) {
+  check_view(v);
+  wait();
+  const ViewDev &d = host_views_[v];
+  const uint64_t need = (uint64_t)d.fp.width * d.fp.height * 4;
+  if (len < need) throw Error(Code::InvalidArgument, "output buffer smaller than width*height*4 floats");
+  GS_HIP(hipMemcpyAsync(out, img32_.get() + d.img_off, need * sizeof(float), hipMemcpyDeviceToHost, stream_));
+  GS_HIP(hipStreamSynchronize(stream_));
+}
+
+void Pipeline::read_sorted(uint32_t v, uint32_t *out, uint64_t cap, uint32_t *count) {
+  check_view(v);
+  wait();
+  const uint32_t vis = visible_[v];
+  if (cap < vis) throw Error(Code::InvalidArgument, "output buffer smaller than the visible count");
+  const ViewDev &d = host_views_[v];
+  if (vis) {
+    GS_HIP(hipMemcpyAsync(out, sorted_vals_ + d.seg_off, (size_t)vis * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          stream_));
+    GS_HIP(hipStreamSynchronize(stream_));
+    const uint32_t base = v << kLocalBits;
+    for (uint32_t i = 0; i < vis; ++i) out[i] -= base;
+  }
+  *count = vis;
+}
+
+void Pipeline::read_tile_counts(uint32_t v, uint32_t *out, uint64_t cap, uint32_t *tx, uint32_t *ty) {
+  check_view(v);
+  wait();
+  const ViewDev &d = host_views_[v];
+  if (cap < d.tiles) throw Error(Code::InvalidArgument, "output buffer smaller than the tile count");
+  std::vector<uint2> r(d.tiles);
+  if (d.tiles) {
+    GS_HIP(hipMemcpyAsync(r.data(), ranges_.get() + d.tile_off, d.tiles * sizeof(uint2), hipMemcpyDeviceToHost,
+                          stream_));
+    GS_HIP(hipStreamSynchronize(stream_));
+  }
+  for (uint32_t t = 0; t < d.tiles; ++t) out[t] = r[t].y - r[t].x;
+  *tx = d.fp.tiles_x;
+  *ty = d.fp.tiles_y;
+}
+
+void Pipeline::export_projection(const DeviceScene &scene, const FrameParams &fp, uint32_t *keys_out,
+                                 float *proj_out) {
+  const uint32_t n = scene.count;
+  if (n == 0) return;
+  ViewDev d{};
+  d.fp = fp;
+  d.soa = scene.soa.get();
+  d.n = n;
+  d.sh_degree = scene.sh_degree;
+  GS_HIP(hipStreamSynchronize(stream_));
+  GS_HIP(hipMemcpy(export_view_.get(), &d, sizeof(d), hipMemcpyHostToDevice));
+  DevBuf<uint32_t> keys;
+  DevBuf<float> proj;
+  keys.ensure(n);
+  proj.ensure(12ull * n);
+  launch_project_export(scene.sh_degree, export_view_.get(), 0, n, keys.get(), proj.get(), stream_);
+  if (keys_out)
+    GS_HIP(hipMemcpyAsync(keys_out, keys.get(), (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+  if (proj_out)
+    GS_HIP(hipMemcpyAsync(proj_out, proj.get(), 12ull * n * sizeof(float), hipMemcpyDeviceToHost, stream_));
+  GS_HIP(hipStreamSynchronize(stream_));
+}
+
+}  // namespace gs

@@ -1,0 +1,191 @@
+// pipeline.h -- device buffers, resident scenes and the batched frame
+// pipeline of the SortedAlpha path on one GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace gs {
+
+#define GS_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw Error(Code::Internal, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Grow-only device allocation. Releasing waits for the device, since work
+// queued on any stream may still use the old allocation.
+template <typename T>
+class DevBuf {
+ public:
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void ensure(size_t count) {
+    if (count <= cap_) return;
+    release();
+    const size_t want = std::max<size_t>(count, 1);
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&ptr_), want * sizeof(T));
+    if (e != hipSuccess) {
+      ptr_ = nullptr;
+      throw Error(Code::Internal, "device allocation of " + std::to_string(want * sizeof(T)) +
+                                      " bytes failed: " + hipGetErrorString(e));
+    }
+    cap_ = want;
+  }
+  // As ensure(), clearing the allocation to zero whenever it is (re)made.
+  void ensure_zeroed(size_t count) {
+    if (count <= cap_) return;
+    ensure(count);
+    GS_HIP(hipMemset(ptr_, 0, cap_ * sizeof(T)));
+  }
+  void release() {
+    if (ptr_) {
+      (void)hipDeviceSynchronize();
+      (void)hipFree(ptr_);
+    }
+    ptr_ = nullptr;
+    cap_ = 0;
+  }
+  T *get() const { return ptr_; }
+  size_t capacity() const { return cap_; }
+
+ private:
+  T *ptr_ = nullptr;
+  size_t cap_ = 0;
+};
+
+bool device_present();
+
+// Scene resident in HBM in the coefficient-major SoA layout of
+// prepare_scene(); shared by contexts, batches and the server.
+struct DeviceScene {
+  uint32_t count = 0;
+  uint32_t sh_degree = 0;
+  DevBuf<float> soa;
+  uint64_t bytes() const { return (uint64_t)soa_fields(sh_degree) * count * sizeof(float); }
+};
+std::shared_ptr<DeviceScene> upload_scene(const HostScene &s, hipStream_t stream);
+
+// Device bytes a batch frame needs beyond its scenes: per-slot buffers,
+// targets, tile ranges and the initial tile-pair buffers.
+uint64_t frame_working_bytes(uint64_t slots, uint64_t pixels, uint64_t tiles, uint64_t pair_cap);
+
+// One view of a batch: a resident scene rendered with the constants `fp`
+// (camera and target size).
+struct ViewSpec {
+  std::shared_ptr<DeviceScene> scene;
+  FrameParams fp;
+};
+
+// Stage timings of the last frame (device events), in milliseconds.
+struct FrameTimes {
+  float project_ms = 0, sort_ms = 0, bin_ms = 0, raster_ms = 0, total_ms = 0;
+};
+
+// Renders batches of views on its own stream. Per frame (Renderer::
+// render_frame, render-wgpu/src/lib.rs:723-742, for every view at once):
+//   project   depth key + cull + projection + SH colour, one thread per slot
+//   sort      one stable radix sort of (depth key, view<<24|index) over all
+//             slots, descending depth, then stably by view: every view's
+//             segment ends up back to front, ties by splat index
+//   bin       per-slot tile counts, scan, (global tile, slot) pairs front to
+//             back, stable radix sort by global tile id, tile ranges
+//   raster    one workgroup per 16x16 tile of every view
+// No host synchronisation inside a frame: the visible counts and the pair
+// count are produced and consumed on the device. The pair buffers have a
+// host-chosen capacity; a frame whose pair count exceeds it drops the excess,
+// flags itself, and is rendered again by wait() after the buffers grow.
+class Pipeline {
+ public:
+  Pipeline();
+  ~Pipeline();
+  Pipeline(const Pipeline &) = delete;
+  Pipeline &operator=(const Pipeline &) = delete;
+
+  // Enqueues one frame of `views` (1..kMaxViews) and returns without waiting
+  // for the device. The scenes stay referenced until the next enqueue. When
+  // `host_rgba8` is given, entry v (if not null) receives view v's RGBA8
+  // image by a copy queued behind the raster; it must be pinned host memory
+  // (a pageable target would make the copy synchronous) and stay valid until
+  // wait() returns.
+  void enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_rgba8 = {});
+  // Waits for the last frame, re-rendering it after an overflow.
+  void wait();
+  bool has_frame() const { return !views_.empty(); }
+
+  uint32_t view_count() const { return (uint32_t)views_.size(); }
+  const ViewDev &view(uint32_t v) const;
+  uint32_t visible(uint32_t v);  // waits
+  uint64_t pairs();              // waits
+  uint64_t total_splats() const { return total_splats_; }
+  uint32_t total_tiles() const { return total_tiles_; }
+  FrameTimes times();  // waits
+  uint64_t overflow_rerenders() const { return overflow_rerenders_; }
+  uint32_t pair_capacity() const { return pair_cap_; }
+
+  void readback_rgba8(uint32_t v, uint8_t *out, uint64_t len);
+  void readback_rgba32f(uint32_t v, float *out, uint64_t len);
+  // Back-to-front splat indices of view v (its visible set).
+  void read_sorted(uint32_t v, uint32_t *out, uint64_t cap, uint32_t *count);
+  void read_tile_counts(uint32_t v, uint32_t *out, uint64_t cap, uint32_t *tx, uint32_t *ty);
+  // Depth keys (keys_out, n entries) and/or the 12-float projection export
+  // (proj_out, 12n entries) of `scene` under `fp`, into host memory.
+  void export_projection(const DeviceScene &scene, const FrameParams &fp, uint32_t *keys_out, float *proj_out);
+
+  void set_raster_variant(RasterVariant v) { raster_variant_ = v; }
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  void launch_frame();
+  bool absorb_summary();  // true when the frame's pairs overflowed
+  void check_view(uint32_t v) const;
+
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev_[6] = {};        // stage boundaries 0..4, 5 = summary copied
+  hipEvent_t table_ev_[2] = {};  // view-table uploads out of each pinned slot
+  uint8_t *table_pinned_[2] = {};
+  uint32_t *summary_pinned_ = nullptr;  // [-, M lo, M hi, overflow, visible[kMaxViews]]
+  uint64_t frame_counter_ = 0;
+
+  DevBuf<uint8_t> table_;    // seg_start[kMaxViews] | tile_start[kMaxViews] | ViewDev[kMaxViews]
+  DevBuf<uint32_t> summary_;  // device side of summary_pinned_
+  DevBuf<uint32_t> keys_, vals_, keys_alt_, vals_alt_;
+  DevBuf<float4> rec_;
+  DevBuf<uint2> rect_;
+  DevBuf<uint32_t> counts_, offsets_;
+  DevBuf<uint32_t> pair_keys_, pair_vals_, pair_keys_alt_, pair_vals_alt_;
+  DevBuf<uint2> ranges_;
+  DevBuf<float4> img32_;
+  DevBuf<uchar4> img8_;
+  DevBuf<uint32_t> sort_scratch_, partials_, m_dev_;
+  DevBuf<uint64_t> total_;
+  DevBuf<ViewDev> export_view_;
+
+  std::vector<ViewSpec> views_;
+  std::vector<uint8_t *> host_rgba8_;
+  std::vector<ViewDev> host_views_;
+  uint32_t total_slots_ = 0, total_tiles_ = 0;
+  uint64_t total_pixels_ = 0, total_splats_ = 0;
+  uint32_t pair_cap_ = 0;
+  uint32_t *sorted_vals_ = nullptr;  // depth-sorted values of the last frame
+  bool pending_ = false;
+  bool summary_valid_ = false;
+  std::vector<uint32_t> visible_;
+  uint64_t pairs_ = 0;
+  uint64_t overflow_rerenders_ = 0;
+  FrameTimes times_{};
+  RasterVariant raster_variant_ = RasterVariant::Linear;
+};
+
+}  // namespace gs

@@ -108,6 +108,19 @@ class GsplatPlyLimits(ctypes.Structure):
     ]
 
 
+class GsplatSceneInfo(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint32), ("sh_degree", ctypes.c_uint32), ("device_bytes", ctypes.c_uint64)]
+
+
+class GsplatViewDesc(ctypes.Structure):
+    _fields_ = [
+        ("scene", ctypes.c_void_p),
+        ("camera", GsplatCamera),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+    ]
+
+
 P = ctypes.c_void_p
 U32 = ctypes.c_uint32
 U64 = ctypes.c_uint64
@@ -175,6 +188,20 @@ SIGNATURES = {
     "gsplat_ply_free": (None, [P]),
     "gsplat_device_available": (U32, []),
     "gsplat_camera_look_at": (GsplatCamera, [ctypes.POINTER(F32), ctypes.POINTER(F32), F32, F32, F32]),
+    "gsplat_scene_create": (I32, [ctypes.POINTER(GsplatSceneView), ctypes.POINTER(P)]),
+    "gsplat_scene_create_from_path": (I32, [ctypes.c_char_p, ctypes.POINTER(P)]),
+    "gsplat_scene_get_info": (I32, [P, ctypes.POINTER(GsplatSceneInfo)]),
+    "gsplat_scene_destroy": (None, [P]),
+    "gsplat_batch_create": (I32, [ctypes.POINTER(P)]),
+    "gsplat_batch_destroy": (None, [P]),
+    "gsplat_batch_render": (I32, [P, ctypes.POINTER(GsplatViewDesc), U32]),
+    "gsplat_batch_wait": (I32, [P]),
+    "gsplat_batch_readback_rgba8": (I32, [P, U32, P, U64]),
+    "gsplat_batch_readback_rgba32f": (I32, [P, U32, P, U64]),
+    "gsplat_batch_get_view_stats": (I32, [P, U32, ctypes.POINTER(GsplatStats)]),
+    "gsplat_batch_read_sorted_indices": (I32, [P, U32, P, U64, PU32]),
+    "gsplat_batch_read_tile_counts": (I32, [P, U32, P, U64, PU32, PU32]),
+    "gsplat_batch_get_stage_times": (I32, [P, ctypes.POINTER(GsplatStageTimes)]),
 }
 
 _lib = None
@@ -344,6 +371,107 @@ class Context:
         ox, oy = ctypes.c_uint32(), ctypes.c_uint32()
         check(lib().gsplat_context_read_tile_counts(self.h, out.ctypes.data, out.size, ctypes.byref(ox),
                                                     ctypes.byref(oy)))
+        return out.reshape(oy.value, ox.value)
+
+
+class Scene:
+    """A scene resident in device memory (GsplatScene), shared by batches."""
+
+    def __init__(self, scene=None, path=None):
+        h = P()
+        if path is not None:
+            check(lib().gsplat_scene_create_from_path(str(path).encode(), ctypes.byref(h)))
+        else:
+            v = scene_view(scene)
+            check(lib().gsplat_scene_create(ctypes.byref(v), ctypes.byref(h)))
+        self.h = h
+        info = GsplatSceneInfo()
+        check(lib().gsplat_scene_get_info(self.h, ctypes.byref(info)))
+        self.count, self.sh_degree, self.device_bytes = info.count, info.sh_degree, info.device_bytes
+
+    def close(self):
+        if self.h:
+            lib().gsplat_scene_destroy(self.h)
+            self.h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Owns one GsplatBatch: renders lists of (scene, camera, width, height)."""
+
+    def __init__(self):
+        h = P()
+        check(lib().gsplat_batch_create(ctypes.byref(h)))
+        self.h = h
+        self.views = []
+
+    def close(self):
+        if self.h:
+            lib().gsplat_batch_destroy(self.h)
+            self.h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, views):
+        """views: sequence of (Scene, GsplatCamera, width, height)."""
+        arr = (GsplatViewDesc * max(len(views), 1))()
+        for i, (scene, cam, w, h) in enumerate(views):
+            arr[i].scene = scene.h if scene is not None else None
+            arr[i].camera = cam
+            arr[i].width, arr[i].height = w, h
+        check(lib().gsplat_batch_render(self.h, arr, len(views)))
+        self.views = [(s, c, w, h) for (s, c, w, h) in views]
+
+    def wait(self):
+        check(lib().gsplat_batch_wait(self.h))
+
+    def _size(self, v):
+        _, _, w, h = self.views[v]
+        return w, h
+
+    def rgba8(self, v: int) -> np.ndarray:
+        w, h = self._size(v)
+        out = np.zeros((h, w, 4), np.uint8)
+        check(lib().gsplat_batch_readback_rgba8(self.h, v, out.ctypes.data, out.size))
+        return out
+
+    def rgba32f(self, v: int) -> np.ndarray:
+        w, h = self._size(v)
+        out = np.zeros((h, w, 4), np.float32)
+        check(lib().gsplat_batch_readback_rgba32f(self.h, v, out.ctypes.data, out.size))
+        return out
+
+    def view_stats(self, v: int) -> GsplatStats:
+        s = GsplatStats()
+        check(lib().gsplat_batch_get_view_stats(self.h, v, ctypes.byref(s)))
+        return s
+
+    def stage_times(self) -> GsplatStageTimes:
+        s = GsplatStageTimes()
+        check(lib().gsplat_batch_get_stage_times(self.h, ctypes.byref(s)))
+        return s
+
+    def sorted_indices(self, v: int, count: int) -> np.ndarray:
+        out = np.zeros(max(count, 1), np.uint32)
+        n = ctypes.c_uint32()
+        check(lib().gsplat_batch_read_sorted_indices(self.h, v, out.ctypes.data, out.size, ctypes.byref(n)))
+        return out[: n.value].copy()
+
+    def tile_counts(self, v: int) -> np.ndarray:
+        w, h = self._size(v)
+        out = np.zeros(((w + 15) // 16) * ((h + 15) // 16), np.uint32)
+        ox, oy = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().gsplat_batch_read_tile_counts(self.h, v, out.ctypes.data, out.size, ctypes.byref(ox),
+                                                  ctypes.byref(oy)))
         return out.reshape(oy.value, ox.value)
 
 

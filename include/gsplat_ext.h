@@ -152,6 +152,70 @@ void gsplat_ply_free(GsplatPlyScene *scene);
 /* 1 when a gfx950 device is visible, else 0. */
 uint32_t gsplat_device_available(void);
 
+/* ------------------------------------------------------- resident scenes */
+
+/*
+ * A scene uploaded once to HBM (load-time precompute of Renderer::load_scene,
+ * render-wgpu/src/lib.rs:488-497, kept on the device) and shared by any
+ * number of batches. Validation and error codes are those of
+ * gsplat_context_load_scene_memory / _load_scene_path. The scene may be
+ * destroyed while a batch that rendered it still holds its last frame: the
+ * device memory is released when no batch references it.
+ */
+typedef struct GsplatScene GsplatScene;
+
+typedef struct GsplatSceneInfo {
+  uint32_t count;
+  uint32_t sh_degree;
+  uint64_t device_bytes; /* HBM held by the scene's SoA */
+} GsplatSceneInfo;
+
+int32_t gsplat_scene_create(const GsplatSceneView *scene, GsplatScene **out_scene);
+int32_t gsplat_scene_create_from_path(const char *path, GsplatScene **out_scene);
+int32_t gsplat_scene_get_info(const GsplatScene *scene, GsplatSceneInfo *out_info);
+void gsplat_scene_destroy(GsplatScene *scene);
+
+/* ---------------------------------------------------------------- batches */
+
+/*
+ * One view of a batch: a resident scene, a camera (validated as in
+ * gsplat_context_set_camera) and a target size (as gsplat_context_set_size).
+ * Each view's result is the frame a context holding that scene, camera and
+ * size would render (Renderer::render_frame, render-wgpu/src/lib.rs:723-742),
+ * identical whatever else the batch holds.
+ */
+typedef struct GsplatViewDesc {
+  const GsplatScene *scene;
+  GsplatCamera camera;
+  uint32_t width;
+  uint32_t height;
+} GsplatViewDesc;
+
+/* Renders up to 256 views per frame on its own device stream: one projection
+ * launch, one segmented depth sort, one tile binning and one raster launch
+ * for all views. A batch is confined to one thread at a time, like a
+ * context. Scenes in a batch of more than one view hold at most 2^24-1
+ * splats each. */
+typedef struct GsplatBatch GsplatBatch;
+
+int32_t gsplat_batch_create(GsplatBatch **out_batch);
+void gsplat_batch_destroy(GsplatBatch *batch);
+/* Enqueues the frame and returns without waiting for the device. */
+int32_t gsplat_batch_render(GsplatBatch *batch, const GsplatViewDesc *views, uint32_t count);
+int32_t gsplat_batch_wait(GsplatBatch *batch);
+/* Per-view results of the last frame (wait implied); view >= count ->
+ * INVALID_ARGUMENT, no frame -> SCENE_NOT_LOADED. */
+int32_t gsplat_batch_readback_rgba8(GsplatBatch *batch, uint32_t view, uint8_t *out, uint64_t len);
+int32_t gsplat_batch_readback_rgba32f(GsplatBatch *batch, uint32_t view, float *out, uint64_t len);
+int32_t gsplat_batch_get_view_stats(GsplatBatch *batch, uint32_t view, GsplatStats *out_stats);
+int32_t gsplat_batch_read_sorted_indices(GsplatBatch *batch, uint32_t view, uint32_t *out, uint64_t cap,
+                                         uint32_t *out_count);
+int32_t gsplat_batch_read_tile_counts(GsplatBatch *batch, uint32_t view, uint32_t *out, uint64_t cap,
+                                      uint32_t *out_tiles_x, uint32_t *out_tiles_y);
+/* Whole-frame stage times: splat_count and visible_count are sums over the
+ * views, tiles_x the total tile count over the views and tiles_y 1. */
+int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out);
+
 /* Camera at `position` looking at `target` with the reference's look-at
  * basis and quaternion (camera_rotation_looking_at / quat_from_camera_basis,
  * ffi-c/src/lib.rs:1677-1739); fov/near/far are taken as given. */

@@ -228,10 +228,10 @@ def test_gpu_auto_camera_frames_scene(gpu, oracle):
     assert ctx.stats().visible_count == len(idx) > 0
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1])
 def test_gpu_raster_variants_match_oracle(gpu, oracle, monkeypatch, variant):
-    # Every raster variant (LDS tile, wave quadrant, wave quadrant banded by
-    # XCD) blends the same lists to the same tolerance.
+    # Both raster tile orders (linear, banded by XCD) blend the same lists to
+    # the same tolerance.
     monkeypatch.setenv("GSPLAT_RASTER_VARIANT", str(variant))
     s = synth.tiny_scene(6000, 77, sh_degree=3, spread=0.6, log_scale=-3.2)
     cam = gpu.look_at((0.2, -0.3, -2.2), (0.0, 0.0, 0.0))
