@@ -13,6 +13,12 @@ one-view-per-launch path on the same GPU and the same views:
                   batch per step
   batched_64v     64 views over 8 resident scenes (50k..1M splats, SH3) at
                   320x240..1920x1080, one batch per step
+and the served form (16 resident scenes totalling 10M splats, SH3):
+  served_poisson  open loop, Poisson arrivals at 5,000 requests/s; latency
+                  from each request's scheduled arrival to its RGBA8 image in
+                  pinned host memory
+  served_closed64 64 clients each submitting its next request when the last
+                  one returns
 """
 
 from __future__ import annotations
@@ -126,13 +132,108 @@ def bench_batched_64v(steps, warmup):
             "splat_slots": int(st.splat_count), "visible": int(st.visible_count), "tile_pairs": int(st.tile_pairs)}
 
 
+def pct(vals):
+    vals = sorted(vals)
+    return {"p50": round(nearest_rank(vals, 0.5), 3), "p99": round(nearest_rank(vals, 0.99), 3),
+            "max": round(vals[-1], 3)}
+
+
+def bench_served(requests, rate, closed_seconds):
+    import queue
+    import threading
+
+    scenes, reqs = synth.served(requests=requests, rate=rate)
+    dev = [A.Scene(s) for s in scenes]
+    del scenes
+    srv = A.Server(max_batch_views=64)
+    ids = [srv.add_scene(d) for d in dev]
+    cams = [(ids[v.scene], A.look_at(v.position, v.target), v.width, v.height) for (_, v) in reqs]
+    pool = queue.Queue()
+    for _ in range(128):
+        pool.put(A.PinnedImage(1920, 1080))
+    # warm the pipelines' buffers on a few batches
+    warm = [srv.submit(*cams[i % len(cams)]) for i in range(256)]
+    for t in warm:
+        srv.wait(t)
+
+    # Open loop: one thread submits at the scheduled arrival times, one waits in order.
+    tickets = queue.Queue()
+    lat, server_ms, batch_views = [], [], []
+    t0 = time.perf_counter() + 0.05
+
+    def submitter():
+        for (arrival, _), (sid, cam, w, h) in zip(reqs, cams):
+            delay = t0 + arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            buf = pool.get()
+            tickets.put((t0 + arrival, srv.submit(sid, cam, w, h, buf), buf))
+        tickets.put(None)
+
+    def waiter():
+        while True:
+            item = tickets.get()
+            if item is None:
+                return
+            due, t, buf = item
+            r = srv.wait(t)
+            lat.append((time.perf_counter() - due) * 1e3)
+            server_ms.append(r.total_ms)
+            batch_views.append(r.batch_views)
+            pool.put(buf)
+
+    th = [threading.Thread(target=submitter), threading.Thread(target=waiter)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    span = max(time.perf_counter() - t0, 1e-9)
+    open_loop = {"requests": len(reqs), "offered_per_s": rate, "completed_per_s": round(len(lat) / span, 1),
+                 "latency_ms": pct(lat), "server_ms": pct(server_ms),
+                 "mean_batch_views": round(float(np.mean(batch_views)), 2)}
+
+    # Closed loop: 64 clients.
+    done, clat = [0], []
+    lock = threading.Lock()
+    stop_at = time.perf_counter() + closed_seconds
+
+    def client(k):
+        buf = A.PinnedImage(1920, 1080)
+        i = k
+        while time.perf_counter() < stop_at:
+            sid, cam, w, h = cams[i % len(cams)]
+            ts = time.perf_counter()
+            srv.wait(srv.submit(sid, cam, w, h, buf))
+            with lock:
+                clat.append((time.perf_counter() - ts) * 1e3)
+                done[0] += 1
+            i += 64
+        buf.close()
+
+    tc = time.perf_counter()
+    th = [threading.Thread(target=client, args=(k,)) for k in range(64)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    closed = {"clients": 64, "completed_per_s": round(done[0] / (time.perf_counter() - tc), 1),
+              "latency_ms": pct(clat)}
+    st = srv.stats()
+    srv.close()
+    return {"scenes": len(dev), "splats": int(sum(d.count for d in dev)),
+            "resident_scene_bytes": int(st.resident_scene_bytes), "memory_budget_bytes": int(st.memory_budget_bytes),
+            "poisson": open_loop, "closed64": closed, "batches": int(st.batches),
+            "rejected": int(st.rejected)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--plain-only", action="store_true", help="skip the batched sections")
+    ap.add_argument("--plain-only", action="store_true", help="skip the batched and served sections")
+    ap.add_argument("--served-requests", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
@@ -207,6 +308,8 @@ def main():
         print(f"batched_1080p {line['batched_1080p']}", file=sys.stderr, flush=True)
         line["batched_64v"] = bench_batched_64v(max(args.steps // 20, 5), max(args.warmup // 4, 2))
         print(f"batched_64v {line['batched_64v']}", file=sys.stderr, flush=True)
+        line["served"] = bench_served(args.served_requests, 5000.0, 2.0)
+        print(f"served {line['served']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, view, cam)
     print(json.dumps(line), flush=True)

@@ -15,6 +15,7 @@
 
 #include "common.h"
 #include "renderer.h"
+#include "server.h"
 #include "sort.h"
 
 struct GsplatContext {
@@ -33,6 +34,10 @@ struct GsplatScene {
 struct GsplatBatch {
   gs::Pipeline pipe;
   float host_ms = 0.0f;
+};
+
+struct GsplatServer {
+  std::unique_ptr<gs::Server> server;
 };
 
 namespace {
@@ -766,6 +771,129 @@ int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out) 
     out->tile_pairs = batch->pipe.pairs();
     return ok();
   });
+}
+
+// ----------------------------------------------------------------- server
+
+GsplatServerConfig gsplat_server_config_default(void) {
+  GsplatServerConfig c;
+  c.memory_budget_bytes = 0;
+  c.max_batch_views = 64;
+  c.max_queue = 4096;
+  return c;
+}
+
+int32_t gsplat_server_create(const GsplatServerConfig *config, GsplatServer **out_server) {
+  static const char *op = "gsplat_server_create";
+  return guard(op, [&]() -> int32_t {
+    if (!out_server) return null_arg(op, "out_server");
+    *out_server = nullptr;
+    if (!config) return null_arg(op, "config");
+    gs::ServerConfig c;
+    c.memory_budget = config->memory_budget_bytes;
+    c.max_batch_views = config->max_batch_views;
+    c.max_queue = config->max_queue;
+    auto s = std::make_unique<GsplatServer>();
+    s->server = std::make_unique<gs::Server>(c);
+    *out_server = s.release();
+    return ok();
+  });
+}
+
+void gsplat_server_destroy(GsplatServer *server) {
+  try {
+    delete server;
+  } catch (...) {
+  }
+}
+
+int32_t gsplat_server_add_scene(GsplatServer *server, const GsplatScene *scene, uint32_t *out_scene_id) {
+  static const char *op = "gsplat_server_add_scene";
+  return guard(op, [&]() -> int32_t {
+    if (!server) return null_arg(op, "server");
+    if (!scene) return null_arg(op, "scene");
+    if (!out_scene_id) return null_arg(op, "out_scene_id");
+    *out_scene_id = server->server->add_scene(scene->scene);
+    return ok();
+  });
+}
+
+int32_t gsplat_server_remove_scene(GsplatServer *server, uint32_t scene_id) {
+  static const char *op = "gsplat_server_remove_scene";
+  return guard(op, [&]() -> int32_t {
+    if (!server) return null_arg(op, "server");
+    server->server->remove_scene(scene_id);
+    return ok();
+  });
+}
+
+int32_t gsplat_server_submit(GsplatServer *server, const GsplatRenderRequest *request, uint64_t *out_ticket) {
+  static const char *op = "gsplat_server_submit";
+  return guard(op, [&]() -> int32_t {
+    if (!server) return null_arg(op, "server");
+    if (!request) return null_arg(op, "request");
+    if (!out_ticket) return null_arg(op, "out_ticket");
+    *out_ticket = server->server->submit(request->scene_id, request->camera, request->width, request->height,
+                                         request->out_rgba8, request->out_len);
+    return ok();
+  });
+}
+
+int32_t gsplat_server_wait(GsplatServer *server, uint64_t ticket, GsplatRequestResult *out_result) {
+  static const char *op = "gsplat_server_wait";
+  return guard(op, [&]() -> int32_t {
+    if (!server) return null_arg(op, "server");
+    if (!out_result) return null_arg(op, "out_result");
+    const gs::RequestResult r = server->server->wait(ticket);
+    out_result->status = r.status;
+    out_result->visible_count = r.visible;
+    out_result->batch_views = r.batch_views;
+    out_result->queue_ms = r.queue_ms;
+    out_result->device_ms = r.device_ms;
+    out_result->total_ms = r.total_ms;
+    if (r.status != 0) return fail((gs::Code)r.status, std::string(op) + ": " + r.error);
+    return ok();
+  });
+}
+
+int32_t gsplat_server_get_stats(GsplatServer *server, GsplatServerStats *out_stats) {
+  static const char *op = "gsplat_server_get_stats";
+  return guard(op, [&]() -> int32_t {
+    if (!server) return null_arg(op, "server");
+    if (!out_stats) return null_arg(op, "out_stats");
+    const gs::ServerStats s = server->server->stats();
+    out_stats->submitted = s.submitted;
+    out_stats->completed = s.completed;
+    out_stats->rejected = s.rejected;
+    out_stats->batches = s.batches;
+    out_stats->views_in_batches = s.views_in_batches;
+    out_stats->resident_scene_bytes = s.resident_scene_bytes;
+    out_stats->memory_budget_bytes = s.memory_budget;
+    out_stats->scenes = s.scenes;
+    out_stats->queued = s.queued;
+    return ok();
+  });
+}
+
+void *gsplat_host_alloc(uint64_t bytes) {
+  try {
+    return gs::host_alloc(bytes);
+  } catch (const gs::Error &e) {
+    set_last(std::string("gsplat_host_alloc: ") + e.what());
+    return nullptr;
+  } catch (...) {
+    set_last("gsplat_host_alloc: panic caught at C ABI boundary");
+    return nullptr;
+  }
+}
+
+void gsplat_host_free(void *ptr) {
+  try {
+    gs::host_free(ptr);
+  } catch (const gs::Error &e) {
+    set_last(std::string("gsplat_host_free: ") + e.what());
+  } catch (...) {
+  }
 }
 
 }  // extern "C"

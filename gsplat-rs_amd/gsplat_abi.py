@@ -121,6 +121,39 @@ class GsplatViewDesc(ctypes.Structure):
     ]
 
 
+class GsplatServerConfig(ctypes.Structure):
+    _fields_ = [("memory_budget_bytes", ctypes.c_uint64), ("max_batch_views", ctypes.c_uint32),
+                ("max_queue", ctypes.c_uint32)]
+
+
+class GsplatRenderRequest(ctypes.Structure):
+    _fields_ = [
+        ("scene_id", ctypes.c_uint32),
+        ("camera", GsplatCamera),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("out_rgba8", ctypes.c_void_p),
+        ("out_len", ctypes.c_uint64),
+    ]
+
+
+class GsplatRequestResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("visible_count", ctypes.c_uint32),
+        ("batch_views", ctypes.c_uint32),
+        ("queue_ms", ctypes.c_float),
+        ("device_ms", ctypes.c_float),
+        ("total_ms", ctypes.c_float),
+    ]
+
+
+class GsplatServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "rejected", "batches", "views_in_batches",
+                                              "resident_scene_bytes", "memory_budget_bytes")] + \
+               [("scenes", ctypes.c_uint32), ("queued", ctypes.c_uint32)]
+
+
 P = ctypes.c_void_p
 U32 = ctypes.c_uint32
 U64 = ctypes.c_uint64
@@ -202,6 +235,16 @@ SIGNATURES = {
     "gsplat_batch_read_sorted_indices": (I32, [P, U32, P, U64, PU32]),
     "gsplat_batch_read_tile_counts": (I32, [P, U32, P, U64, PU32, PU32]),
     "gsplat_batch_get_stage_times": (I32, [P, ctypes.POINTER(GsplatStageTimes)]),
+    "gsplat_server_config_default": (GsplatServerConfig, []),
+    "gsplat_server_create": (I32, [ctypes.POINTER(GsplatServerConfig), ctypes.POINTER(P)]),
+    "gsplat_server_destroy": (None, [P]),
+    "gsplat_server_add_scene": (I32, [P, P, PU32]),
+    "gsplat_server_remove_scene": (I32, [P, U32]),
+    "gsplat_server_submit": (I32, [P, ctypes.POINTER(GsplatRenderRequest), ctypes.POINTER(U64)]),
+    "gsplat_server_wait": (I32, [P, U64, ctypes.POINTER(GsplatRequestResult)]),
+    "gsplat_server_get_stats": (I32, [P, ctypes.POINTER(GsplatServerStats)]),
+    "gsplat_host_alloc": (P, [U64]),
+    "gsplat_host_free": (None, [P]),
 }
 
 _lib = None
@@ -473,6 +516,84 @@ class Batch:
         check(lib().gsplat_batch_read_tile_counts(self.h, v, out.ctypes.data, out.size, ctypes.byref(ox),
                                                   ctypes.byref(oy)))
         return out.reshape(oy.value, ox.value)
+
+
+class PinnedImage:
+    """RGBA8 target in pinned host memory (gsplat_host_alloc)."""
+
+    def __init__(self, width: int, height: int):
+        self.nbytes = width * height * 4
+        self.ptr = lib().gsplat_host_alloc(self.nbytes)
+        if not self.ptr:
+            raise GsplatError(INTERNAL, last_error())
+        buf = (ctypes.c_uint8 * self.nbytes).from_address(self.ptr)
+        self.array = np.ctypeslib.as_array(buf).reshape(height, width, 4)
+
+    def close(self):
+        if self.ptr:
+            self.array = None
+            lib().gsplat_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Owns one GsplatServer (continuous batching over resident scenes)."""
+
+    def __init__(self, memory_budget_bytes: int = 0, max_batch_views: int = 64, max_queue: int = 4096):
+        cfg = lib().gsplat_server_config_default()
+        cfg.memory_budget_bytes, cfg.max_batch_views, cfg.max_queue = memory_budget_bytes, max_batch_views, max_queue
+        h = P()
+        check(lib().gsplat_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().gsplat_server_destroy(self.h)
+            self.h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_scene(self, scene: "Scene") -> int:
+        sid = ctypes.c_uint32()
+        check(lib().gsplat_server_add_scene(self.h, scene.h, ctypes.byref(sid)))
+        return sid.value
+
+    def remove_scene(self, scene_id: int):
+        check(lib().gsplat_server_remove_scene(self.h, scene_id))
+
+    def submit(self, scene_id: int, cam: GsplatCamera, width: int, height: int, out=None) -> int:
+        """`out`: None, a PinnedImage, or a C-contiguous uint8 numpy array of width*height*4 bytes."""
+        req = GsplatRenderRequest()
+        req.scene_id, req.camera, req.width, req.height = scene_id, cam, width, height
+        if isinstance(out, PinnedImage):
+            req.out_rgba8, req.out_len = out.ptr, out.nbytes
+        elif out is not None:
+            req.out_rgba8, req.out_len = out.ctypes.data, out.nbytes
+        t = ctypes.c_uint64()
+        check(lib().gsplat_server_submit(self.h, ctypes.byref(req), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, raise_on_error: bool = True) -> GsplatRequestResult:
+        r = GsplatRequestResult()
+        rc = lib().gsplat_server_wait(self.h, ticket, ctypes.byref(r))
+        if raise_on_error:
+            check(rc)
+        return r
+
+    def stats(self) -> GsplatServerStats:
+        s = GsplatServerStats()
+        check(lib().gsplat_server_get_stats(self.h, ctypes.byref(s)))
+        return s
 
 
 def sort_values_by_keys(keys, values) -> np.ndarray:

@@ -216,6 +216,78 @@ int32_t gsplat_batch_read_tile_counts(GsplatBatch *batch, uint32_t view, uint32_
  * views, tiles_x the total tile count over the views and tiles_y 1. */
 int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out);
 
+/* ----------------------------------------------------------------- server */
+
+/*
+ * Request loop over resident scenes with continuous batching: requests that
+ * arrive while batches run on the device are folded into the next batch as
+ * soon as one of its two pipelines is free, and consecutive batches overlap
+ * on the device. Each result is the frame a context would render for the
+ * request alone. Admission is by device memory (the analogue of the
+ * reference's direct_scene_preflight, render-wgpu/src/lib.rs:2202-2304, with
+ * a byte budget in place of the storage-binding limit): resident scenes plus
+ * each pipeline's batch working set stay within memory_budget_bytes. All
+ * gsplat_server_* calls are thread-safe; destroy must not race with them.
+ */
+typedef struct GsplatServer GsplatServer;
+
+typedef struct GsplatServerConfig {
+  uint64_t memory_budget_bytes; /* 0: 90% of the device memory free at create */
+  uint32_t max_batch_views;     /* views folded into one batch, 1..256 */
+  uint32_t max_queue;           /* queued requests before submit refuses */
+} GsplatServerConfig;
+
+typedef struct GsplatRenderRequest {
+  uint32_t scene_id;
+  GsplatCamera camera;
+  uint32_t width;
+  uint32_t height;
+  uint8_t *out_rgba8; /* width*height*4 bytes or NULL; valid until wait returns */
+  uint64_t out_len;
+} GsplatRenderRequest;
+
+typedef struct GsplatRequestResult {
+  int32_t status;         /* GsplatErrorCode of the request */
+  uint32_t visible_count; /* V of the view */
+  uint32_t batch_views;   /* views in the batch that rendered it */
+  float queue_ms;         /* submit -> batch start */
+  float device_ms;        /* device time of that batch */
+  float total_ms;         /* submit -> image delivered */
+} GsplatRequestResult;
+
+typedef struct GsplatServerStats {
+  uint64_t submitted;
+  uint64_t completed;
+  uint64_t rejected;
+  uint64_t batches;
+  uint64_t views_in_batches;
+  uint64_t resident_scene_bytes;
+  uint64_t memory_budget_bytes;
+  uint32_t scenes;
+  uint32_t queued;
+} GsplatServerStats;
+
+/* {0, 64, 4096} */
+GsplatServerConfig gsplat_server_config_default(void);
+int32_t gsplat_server_create(const GsplatServerConfig *config, GsplatServer **out_server);
+/* Waits for batches on the device; requests still queued end with INTERNAL. */
+void gsplat_server_destroy(GsplatServer *server);
+/* Registers a resident scene (at most 2^24-1 splats); UNSUPPORTED when the
+ * budget cannot hold it plus one 1080p view of it per pipeline. */
+int32_t gsplat_server_add_scene(GsplatServer *server, const GsplatScene *scene, uint32_t *out_scene_id);
+int32_t gsplat_server_remove_scene(GsplatServer *server, uint32_t scene_id);
+/* Queues a render. out_rgba8 inside a gsplat_host_alloc block is written by
+ * a copy queued in the batch; other memory is written after the batch. */
+int32_t gsplat_server_submit(GsplatServer *server, const GsplatRenderRequest *request, uint64_t *out_ticket);
+/* Blocks until the request is done and releases the ticket; returns the
+ * request's own status (its message in gsplat_last_error_message). */
+int32_t gsplat_server_wait(GsplatServer *server, uint64_t ticket, GsplatRequestResult *out_result);
+int32_t gsplat_server_get_stats(GsplatServer *server, GsplatServerStats *out_stats);
+
+/* Pinned host memory for server outputs (NULL on failure). */
+void *gsplat_host_alloc(uint64_t bytes);
+void gsplat_host_free(void *ptr);
+
 /* Camera at `position` looking at `target` with the reference's look-at
  * basis and quaternion (camera_rotation_looking_at / quat_from_camera_basis,
  * ffi-c/src/lib.rs:1677-1739); fov/near/far are taken as given. */

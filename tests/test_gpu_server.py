@@ -1,0 +1,147 @@
+"""Served renders (gsplat_server_*) through the C ABI.
+
+The bar: every request's image and visible count are exactly those of the
+same view rendered alone (a one-view batch), whatever the batch that carried
+it; requests arriving while batches run are folded into later batches;
+memory admission refuses scenes and requests beyond the budget.
+"""
+
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def alone(abi, dev, cam, w, h):
+    b = abi.Batch()
+    b.render([(dev, cam, w, h)])
+    img = b.rgba8(0)
+    vis = b.view_stats(0).visible_count
+    b.close()
+    return img, vis
+
+
+def workload(abi, count, seed):
+    specs = [(20000, 3), (5000, 0), (9000, 1), (40000, 3)]
+    scenes = [synth.tiny_scene(n, 8100 + i, sh_degree=d) for i, (n, d) in enumerate(specs)]
+    rng = np.random.default_rng(seed)
+    sizes = [(320, 240), (640, 480), (256, 144), (97, 61)]
+    reqs = []
+    for _ in range(count):
+        s = int(rng.integers(0, len(scenes)))
+        w, h = sizes[int(rng.integers(0, len(sizes)))]
+        reqs.append((s, abi.look_at(synth.sphere_point(rng, float(rng.uniform(2.0, 4.0)))), w, h))
+    return scenes, reqs
+
+
+def test_server_results_equal_alone(abi):
+    scenes, reqs = workload(abi, 48, 5)
+    dev = [abi.Scene(s) for s in scenes]
+    srv = abi.Server(max_batch_views=16)
+    ids = [srv.add_scene(d) for d in dev]
+    outs, tickets = [], []
+    say(f"[server burst] {len(reqs)} requests")
+    for i, (s, cam, w, h) in enumerate(reqs):
+        out = abi.PinnedImage(w, h) if i % 2 == 0 else np.zeros((h, w, 4), np.uint8)
+        outs.append(out)
+        tickets.append(srv.submit(ids[s], cam, w, h, out))
+    results = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    assert st.completed == len(reqs) and st.rejected == 0
+    assert st.batches < len(reqs), "a burst must be folded into shared batches"
+    assert max(r.batch_views for r in results) > 1
+    for i, ((s, cam, w, h), r, out) in enumerate(zip(reqs, results, outs)):
+        img, vis = alone(abi, dev[s], cam, w, h)
+        got = out.array if isinstance(out, abi.PinnedImage) else out
+        assert r.status == 0 and r.visible_count == vis, f"request {i}"
+        assert np.array_equal(got, img), f"request {i}: image differs from the view rendered alone"
+        assert r.total_ms >= r.queue_ms >= 0.0
+    srv.close()
+
+
+def test_server_concurrent_submitters(abi):
+    scenes, reqs = workload(abi, 64, 6)
+    dev = [abi.Scene(s) for s in scenes]
+    srv = abi.Server(max_batch_views=64)
+    ids = [srv.add_scene(d) for d in dev]
+    results = [None] * len(reqs)
+    outs = [abi.PinnedImage(w, h) for (_, _, w, h) in reqs]
+
+    def client(k):
+        for i in range(k, len(reqs), 8):
+            s, cam, w, h = reqs[i]
+            t = srv.submit(ids[s], cam, w, h, outs[i])
+            results[i] = srv.wait(t)
+
+    say("[server 8 client threads]")
+    th = [threading.Thread(target=client, args=(k,)) for k in range(8)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i, (s, cam, w, h) in enumerate(reqs[:16]):
+        img, vis = alone(abi, dev[s], cam, w, h)
+        assert results[i].visible_count == vis
+        assert np.array_equal(outs[i].array, img), f"request {i}"
+    assert srv.stats().completed == len(reqs)
+    srv.close()
+
+
+def test_server_memory_admission(abi):
+    small = abi.Scene(synth.tiny_scene(1000, 11))
+    big = abi.Scene(synth.tiny_scene(200000, 12))
+    # a budget that holds the small scene and its 1080p working set but not the big one
+    srv = abi.Server(memory_budget_bytes=160 << 20)
+    sid = srv.add_scene(small)
+    with pytest.raises(abi.GsplatError) as e:
+        srv.add_scene(big)
+    assert e.value.code == abi.UNSUPPORTED and "memory budget" in abi.last_error()
+    # a request whose working set exceeds the pipeline share is refused, the next one runs
+    huge = srv.submit(sid, abi.look_at((0, 0, -3)), 16384, 16384)
+    ok = srv.submit(sid, abi.look_at((0, 0, -3)), 64, 64)
+    r = srv.wait(huge, raise_on_error=False)
+    assert r.status == abi.UNSUPPORTED
+    assert srv.wait(ok).status == 0
+    assert srv.stats().rejected == 1
+    srv.close()
+
+
+def test_server_errors(abi):
+    dev = abi.Scene(synth.tiny_scene(100, 13))
+    srv = abi.Server()
+    sid = srv.add_scene(dev)
+    cam = abi.look_at((0, 0, -3))
+    for bad_call, code in [
+        (lambda: srv.submit(sid + 100, cam, 64, 64), abi.INVALID_ARGUMENT),
+        (lambda: srv.submit(sid, cam, 0, 64), abi.INVALID_ARGUMENT),
+        (lambda: srv.submit(sid, cam, 64, 64, np.zeros(10, np.uint8)), abi.INVALID_ARGUMENT),
+        (lambda: srv.wait(987654321), abi.INVALID_ARGUMENT),
+        (lambda: srv.remove_scene(sid + 100), abi.INVALID_ARGUMENT),
+    ]:
+        with pytest.raises(abi.GsplatError) as e:
+            bad_call()
+        assert e.value.code == code
+    badcam = abi.look_at((0, 0, -3))
+    badcam.vertical_fov_radians = 0.0
+    with pytest.raises(abi.GsplatError) as e:
+        srv.submit(sid, badcam, 64, 64)
+    assert e.value.code == abi.INVALID_ARGUMENT
+    with pytest.raises(abi.GsplatError):
+        abi.Server(max_batch_views=0)
+    # a removed scene no longer takes requests; queued ones still finish
+    t = srv.submit(sid, cam, 64, 64)
+    srv.remove_scene(sid)
+    assert srv.wait(t).status == 0
+    with pytest.raises(abi.GsplatError):
+        srv.submit(sid, cam, 64, 64)
+    srv.close()
