@@ -132,6 +132,118 @@ def bench_batched_64v(steps, warmup):
             "splat_slots": int(st.splat_count), "visible": int(st.visible_count), "tile_pairs": int(st.tile_pairs)}
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_plain-1080p-300k-sh3.json"
+
+
+def load_traffic():
+    """HBM bytes per launch per kernel from the committed rocprofv3 --pmc summary (FETCH_SIZE and
+    WRITE_SIZE runs, scaled by the copy calibration); {} when absent."""
+    try:
+        data = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return {}
+    return {k: int(v["bytes_per_launch"]) for k, v in data.get("kernels", {}).items()}
+
+
+def copy_calibration(mib):
+    """Plain device-to-device copies of known size (the runtime's copy kernel), for scaling the
+    FETCH_SIZE / WRITE_SIZE counters: each copy reads and writes `mib` MiB."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    for _ in range(5):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": n, "bytes_written": n, "copies": 5}}), flush=True)
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def _kernel_short(name):
+    n = name.replace("(anonymous namespace)::", "").replace("void ", "")
+    return n.split("(")[0].replace("gs::", "")
+
+
+# Kernels of each plain-frame stage (one frame launches each listed kernel one or more times).
+STAGE_KERNELS = {
+    "k_project": ["k_project<3>"],
+    "k_os_pass": ["k_os_clear", "k_os_hist", "k_os_pass"],
+    "bin(pair sort+emit)": ["k_tile_counts", "k_scan_blocks", "k_scan_partials", "k_scan_add", "k_pairs_finalize",
+                            "k_tile_emit", "k_rts_up", "k_rts_down", "k_tile_ranges"],
+    "k_raster": ["k_raster<false, false>"],
+}
+
+
+def summarize_profiles(trace_dir, fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, frames, out_dir):
+    """rocprofv3 CSVs of the plain workload -> profiles/ summaries (kernel stats, HBM traffic per frame)."""
+    import csv
+    import collections
+
+    def rows(d, suffix):
+        path = next(pathlib.Path(d).glob(f"*{suffix}"))
+        return list(csv.DictReader(open(path)))
+
+    def per_kernel(d):
+        acc = collections.defaultdict(list)
+        for r in rows(d, "counter_collection.csv"):
+            acc[_kernel_short(r["Kernel_Name"])].append(float(r["Counter_Value"]) * 1024.0)
+        return acc
+
+    cf = sum(float(r["Counter_Value"]) for r in rows(copy_fetch_dir, "counter_collection.csv")
+             if "copyBuffer" in r["Kernel_Name"]) * 1024.0
+    cw = sum(float(r["Counter_Value"]) for r in rows(copy_write_dir, "counter_collection.csv")
+             if "copyBuffer" in r["Kernel_Name"]) * 1024.0
+    copies = sum(1 for r in rows(copy_fetch_dir, "counter_collection.csv") if "copyBuffer" in r["Kernel_Name"])
+    known = copies * (1024 << 20)
+    ff, wf = known / cf, known / cw
+    fetch, write = per_kernel(fetch_dir), per_kernel(write_dir)
+    launches = {k: len(v) for k, v in fetch.items()}
+    # Frames profiled: the counting-variant frame is not a plain frame; k_project<3> runs once per frame.
+    nframes = launches.get("k_project<3>", frames)
+    kernels = {}
+    for stage, ks in STAGE_KERNELS.items():
+        # a one-kernel stage: per launch of that kernel; else per frame
+        div = launches.get(ks[0], nframes) if len(ks) == 1 else nframes
+        fb = sum(sum(fetch.get(k, [])) for k in ks) / max(div, 1) * ff
+        wb = sum(sum(write.get(k, [])) for k in ks) / max(div, 1) * wf
+        kernels[stage] = {"fetch_bytes": int(fb), "write_bytes": int(wb), "bytes_per_launch": int(fb + wb),
+                          "kernels": ks}
+    out = {"workload": "plain-1080p-300k-sh3", "frames": nframes, "fetch_scale": round(ff, 4),
+           "write_scale": round(wf, 4), "kernels": kernels,
+           "commands": ["rocprofv3 --pmc FETCH_SIZE --output-format csv -d DIR -o plain -- python3 bench.py "
+                        "--plain-only --steps 8 --warmup 2 --no-cpu-baseline",
+                        "rocprofv3 --pmc WRITE_SIZE (same)",
+                        "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE -- python3 bench.py --copy-calibration 1024"]}
+    out_dir = pathlib.Path(out_dir)
+    (out_dir / "pmc_traffic_plain-1080p-300k-sh3.json").write_text(json.dumps(out, indent=1) + "\n")
+    stats = rows(trace_dir, "kernel_stats.csv")
+    lines = ["# Kernel trace: plain-1080p-300k-sh3 (300,000 splats SH3, 1920x1080)", "",
+             "Command (from /tmp, TMPDIR=/tmp):", "",
+             "    rocprofv3 --kernel-trace --stats --output-format csv -d DIR -o plain -- \\",
+             "        python3 bench.py --plain-only --steps 30 --warmup 5 --no-cpu-baseline", "",
+             "Summarised with `python3 bench.py --summarize-profiles ...`. Calls cover warmup, timed, latency",
+             "and one counting frame (k_raster<false, true>).", "",
+             "| kernel | calls | avg us | total % |", "|---|---:|---:|---:|"]
+    for r in sorted(stats, key=lambda r: -float(r["Percentage"])):
+        lines.append(f"| {_kernel_short(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                     f"{float(r['Percentage']):.2f} |")
+    lines += ["", "# HBM traffic per frame (rocprofv3 --pmc, two runs)", "",
+              f"FETCH_SIZE scaled by {ff:.3f} and WRITE_SIZE by {wf:.3f}: the factors that make the counters read",
+              f"{copies} plain device-to-device copies of 1 GiB (`bench.py --copy-calibration 1024`) as 1 GiB read",
+              "and 1 GiB written.", "", "| stage | kernels | fetch MB | write MB | total MB |", "|---|---|---:|---:|---:|"]
+    for stage, v in kernels.items():
+        lines.append(f"| {stage} | {', '.join(v['kernels'])} | {v['fetch_bytes'] / 1e6:.1f} | "
+                     f"{v['write_bytes'] / 1e6:.1f} | {v['bytes_per_launch'] / 1e6:.1f} |")
+    (out_dir / "kernels_plain-1080p-300k-sh3.md").write_text("\n".join(lines) + "\n")
+    print(json.dumps(out))
+
+
 def pct(vals):
     vals = sorted(vals)
     return {"p50": round(nearest_rank(vals, 0.5), 3), "p99": round(nearest_rank(vals, 0.99), 3),
@@ -234,7 +346,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="skip the batched and served sections")
     ap.add_argument("--served-requests", type=int, default=2000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (PMC counter calibration)")
+    ap.add_argument("--summarize-profiles", nargs=5, metavar=("TRACE", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="write profiles/ summaries from rocprofv3 output directories and exit")
     args = ap.parse_args()
+    if args.summarize_profiles:
+        summarize_profiles(*args.summarize_profiles, frames=0, out_dir=ROOT / "profiles")
+        return
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
         sys.exit(2)
@@ -275,12 +397,52 @@ def main():
     dom = int(np.argmax(med[:4]))
 
     n, v = st.splat_count, st.visible_count
+    pairs = int(st.tile_pairs)
+    ctx.close()
+
+    # One counting frame (outside the timed region) for the blend's algorithmic work E.
+    os.environ["GSPLAT_COUNT_EVALS"] = "1"
+    cctx = A.Context(view.width, view.height)
+    cctx.load(scene)
+    cctx.set_camera(cam)
+    cctx.render()
+    cst = cctx.stage_times()
+    evals, tests = int(cst.blend_evals), int(cst.blend_tests)
+    cctx.close()
+    del os.environ["GSPLAT_COUNT_EVALS"]
+
     k = 3 * ((scene.sh_degree + 1) ** 2 - 1)
-    # Projection bytes: positions for all, covariance + alpha + cut for visible, colour terms and
-    # record/rect writes for visible, keys/values for all.
-    proj_bytes = 12 * n + 8 * n + (32 + 12 + 4 * k + 56) * v
-    roof = {"project": ("hbm", proj_bytes / (med[0] * 1e-3) / 1e9, MI355X_HBM_GBPS, "GB/s", proj_bytes)}
-    kernel, (bound, achieved, peak, unit, traffic) = names[0], roof["project"]
+    # Algorithmic work per stage (units as stated; per frame):
+    #  project  positions for all splats, covariance + alpha + cut + SH for visible, key/value writes for
+    #           every slot, 48-byte record + 8-byte rect for visible
+    #  sort     8-bit LSD: 4 passes x (key+value read and write) over N slots
+    #  raster   16 FLOP per (pixel, splat) evaluation passing coverage before the pixel's early stop
+    proj_bytes = 12 * n + 8 * n + (24 + 4 + 4 + 12 + 4 * k + 48 + 8) * v
+    sort_bytes = 4 * 16 * n
+    #  bin      rect read per visible splat, pair writes, then 2 passes x (key+value read and write) over M
+    pair_bytes = 8 * v + 8 * pairs + 2 * 16 * pairs
+    raster_flop = 16 * evals
+    traffic = load_traffic()
+    rooflines = {
+        "k_project": {"bound": "hbm", "achieved": proj_bytes / (med[0] * 1e-3) / 1e9, "peak": MI355X_HBM_GBPS,
+                      "unit": "GB/s", "algorithmic": proj_bytes},
+        "k_os_pass": {"bound": "hbm", "achieved": sort_bytes / (med[1] * 1e-3) / 1e9,
+                             "peak": MI355X_HBM_GBPS, "unit": "GB/s", "algorithmic": sort_bytes},
+        "bin(pair sort+emit)": {"bound": "hbm", "achieved": pair_bytes / (med[2] * 1e-3) / 1e9,
+                                "peak": MI355X_HBM_GBPS, "unit": "GB/s", "algorithmic": pair_bytes},
+        "k_raster": {"bound": "valu", "achieved": raster_flop / (med[3] * 1e-3) / 1e12,
+                     "peak": MI355X_FP32_TFLOPS, "unit": "TFLOP/s", "algorithmic": raster_flop},
+    }
+    for name, r in rooflines.items():
+        r["achieved"] = round(r["achieved"], 3)
+        r["frac"] = round(r["achieved"] / r["peak"], 4)
+        r["traffic"] = traffic.get(name)
+    dom_kernel = {"project": "k_project", "sort": "k_os_pass", "bin": "bin(pair sort+emit)",
+                  "raster": "k_raster"}[
+        names[dom]]
+    roof = dict(rooflines[dom_kernel])
+    roof = {"kernel": dom_kernel, "bound": roof["bound"], "achieved": roof["achieved"],
+            "peak": roof["peak"], "unit": roof["unit"], "frac": roof["frac"], "traffic": roof["traffic"]}
     line = {
         "metric": "frames/s @300k splats SH3 1080p (plain SortedAlpha)",
         "value": round(fps, 3),
@@ -295,14 +457,15 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded 3-component Gaussian mixture, SH3)",
         "config": {"workload": "plain-1080p-300k-sh3", "splats": n, "width": view.width, "height": view.height,
-                   "visible": v, "tile_pairs": int(st.tile_pairs)},
+                   "visible": v, "tile_pairs": pairs, "blend_evals": evals, "blend_tests": tests},
         "latency_ms": {"p50": round(nearest_rank(lat, 0.5), 4), "p99": round(nearest_rank(lat, 0.99), 4)},
         "stage_ms": {nm: round(float(x), 4) for nm, x in zip(names, med[:4])},
         "dominant_stage": names[dom],
-        "roofline": {"kernel": "k_project", "bound": bound, "achieved": round(achieved, 2), "peak": peak,
-                     "unit": unit, "frac": round(achieved / peak, 4), "traffic": None},
+        "mpix_per_s": round(fps * view.width * view.height / 1e6, 1),
+        "msplats_sorted_per_s": round(v / (med[1] * 1e-3) / 1e6, 1),
+        "roofline": roof,
+        "rooflines": rooflines,
     }
-    ctx.close()
     if not args.plain_only:
         line["batched_1080p"] = bench_batched_1080p(scene, max(args.steps // 8, 5), max(args.warmup // 4, 2))
         print(f"batched_1080p {line['batched_1080p']}", file=sys.stderr, flush=True)

@@ -769,6 +769,8 @@ int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out) 
     out->tiles_x = batch->pipe.total_tiles();
     out->tiles_y = 1;
     out->tile_pairs = batch->pipe.pairs();
+    out->blend_evals = t.blend_evals;
+    out->blend_tests = t.blend_tests;
     return ok();
   });
 }

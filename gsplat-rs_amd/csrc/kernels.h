@@ -31,10 +31,13 @@ FrameParams make_frame_params(const float position[3], const float rotation_xyzw
 constexpr uint32_t kTile = 16;            // raster tile edge in pixels
 constexpr uint32_t kThreads = 256;        // threads per workgroup (4 wave64)
 constexpr uint32_t kScanTile = kThreads * 16;
+constexpr uint32_t kMaxGrid = 4096;       // bound of the grids that loop over device-sized work
 constexpr uint32_t kSegmentAlign = 4096;  // view slot segments align to the sort tile
 constexpr uint32_t kLocalBits = 24;       // splat index bits in a sort value
 constexpr uint32_t kLocalMask = (1u << kLocalBits) - 1;
 constexpr uint32_t kMaxViews = 256;       // view id fits the value's top byte
+constexpr uint32_t kPairViewShift = 24;   // tile-pair key = view << 24 | tile of the view
+constexpr uint32_t kPairTileMask = (1u << kPairViewShift) - 1;
 
 // One view of a batch. Its splats own the slot segment [seg_off,
 // seg_off + seg_len) of every per-splat array (seg_len = n rounded up to
@@ -79,8 +82,10 @@ struct ScanScratch {
 };
 size_t scan_partial_entries(size_t n);
 
-// Exclusive scan of n u32 values; the 64-bit sum goes to *scratch.total.
-void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s, hipStream_t stream);
+// Exclusive scan of n u32 values (in place allowed); the 64-bit sum goes to
+// *scratch.total. With n_dev the length is min(*n_dev, n), read on the device.
+void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s, hipStream_t stream,
+                    const uint32_t *n_dev = nullptr);
 
 // After the depth sort (each view's segment back to front, culled last):
 // counts_rev[total-1-g] = number of tiles of the splat in sorted slot g (0
@@ -97,18 +102,21 @@ void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_
 void launch_pairs_finalize(const uint64_t *total, uint32_t capacity, uint32_t *m_out, uint32_t *info,
                            hipStream_t stream);
 
-// Writes (global tile id, record slot) pairs front to back at the scanned
-// offsets; pairs at or beyond `capacity` are dropped (the frame is then
-// flagged and re-rendered).
+// Writes (view << 24 | tile of the view, record slot) pairs at the scanned
+// offsets: each view's pairs form one contiguous run, front to back. Pairs at
+// or beyond `capacity` are dropped (the frame is then flagged and rendered
+// again).
 void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
                       const uint32_t *svals, const uint2 *rect, const uint32_t *counts_rev,
                       const uint32_t *offsets_rev, uint32_t capacity, uint32_t *pair_keys, uint32_t *pair_vals,
                       hipStream_t stream);
 
-// ranges[t] = [first, last+1) of global tile t in the tile-sorted pair keys;
-// the pair count is read from *m_dev (grid sized for `capacity`).
-void launch_tile_ranges(const uint32_t *pair_keys, const uint32_t *m_dev, uint32_t capacity, uint32_t tiles,
-                        uint2 *ranges, hipStream_t stream);
+// After a stable sort of the pairs on the tile bits of their keys:
+// ranges[t] = [first, last+1) of global tile t (= tile_off of the key's view
+// + the key's tile); the pair count is read from *m_dev (grid sized for
+// `capacity`).
+void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const uint32_t *m_dev, uint32_t capacity,
+                        uint32_t tiles, uint2 *ranges, hipStream_t stream);
 
 // Front-to-back blend of every view's 16x16 tiles: one workgroup per tile,
 // batches of 256 records staged in LDS.
@@ -117,9 +125,12 @@ void launch_tile_ranges(const uint32_t *pair_keys, const uint32_t *m_dev, uint32
 //            tiles are banded so each XCD renders one contiguous range and
 //            neighbouring tiles share that XCD's L2
 enum class RasterVariant : uint32_t { Linear = 0, XcdBand = 1 };
+// With `evals`, the counting variant also adds to evals[0] the number of
+// (pixel, splat) evaluations that pass coverage before each pixel's early
+// stop (the blend's algorithmic work) and to evals[1] the coverage tests made.
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
                    uint32_t total_tiles, const float4 *rec, const uint2 *ranges, const uint32_t *list,
-                   float4 *img32, uchar4 *img8, hipStream_t stream);
+                   float4 *img32, uchar4 *img8, hipStream_t stream, unsigned long long *evals = nullptr);
 
 // Device fill of n u32 words (graph-safe replacement of hipMemsetAsync).
 void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t stream);

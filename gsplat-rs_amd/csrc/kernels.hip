@@ -316,37 +316,50 @@ __global__ __launch_bounds__(kThreads) void k_project_export(const ViewDev *__re
 
 // ------------------------------------------------------------------- scans
 
+// The scan's length is n_cap, or min(*n_dev, n_cap) when n_dev is given (a
+// count produced on the device); grids are bounded and loop over scan tiles,
+// so the cost follows the live length, not the capacity.
+__device__ __forceinline__ size_t scan_length(const uint32_t *n_dev, size_t n_cap) {
+  return n_dev ? min((size_t)*n_dev, n_cap) : n_cap;
+}
+
 // First level: each block scans kScanTile items (16 consecutive per thread)
-// into `out` and writes its sum to partials[block].
+// into `out` and writes their sum to partials[tile].
 __global__ __launch_bounds__(kThreads) void k_scan_blocks(const uint32_t *__restrict__ in,
-                                                          uint32_t *__restrict__ out, size_t n,
+                                                          uint32_t *__restrict__ out, size_t n_cap,
+                                                          const uint32_t *__restrict__ n_dev,
                                                           uint32_t *__restrict__ partials) {
   __shared__ uint32_t lds4[kWaves];
-  const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * 16;
-  uint32_t v[16];
-  uint32_t sum = 0;
+  const size_t n = scan_length(n_dev, n_cap);
+  for (size_t b = blockIdx.x; b * kScanTile < n; b += gridDim.x) {
+    const size_t base = b * kScanTile + (size_t)threadIdx.x * 16;
+    uint32_t v[16];
+    uint32_t sum = 0;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const size_t p = base + k;
-    v[k] = p < n ? in[p] : 0u;
-    sum += v[k];
-  }
-  uint32_t total;
-  uint32_t run = block_exclusive_scan(sum, lds4, &total);
+    for (int k = 0; k < 16; ++k) {
+      const size_t p = base + k;
+      v[k] = p < n ? in[p] : 0u;
+      sum += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, lds4, &total);
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const size_t p = base + k;
-    if (p < n) out[p] = run;
-    run += v[k];
+    for (int k = 0; k < 16; ++k) {
+      const size_t p = base + k;
+      if (p < n) out[p] = run;
+      run += v[k];
+    }
+    if (threadIdx.x == 0) partials[b] = total;
   }
-  if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
 // Second level: one block scans the partials (any count) and writes the
 // 64-bit grand total.
-__global__ __launch_bounds__(kThreads) void k_scan_partials(uint32_t *__restrict__ partials, uint32_t count,
+__global__ __launch_bounds__(kThreads) void k_scan_partials(uint32_t *__restrict__ partials, size_t n_cap,
+                                                            const uint32_t *__restrict__ n_dev,
                                                             uint64_t *__restrict__ total64) {
   __shared__ uint32_t lds4[kWaves];
+  const uint32_t count = (uint32_t)((scan_length(n_dev, n_cap) + kScanTile - 1) / kScanTile);
   uint32_t carry = 0;
   uint64_t grand = 0;
   for (uint32_t base = 0; base < count; base += kThreads) {
@@ -361,13 +374,17 @@ __global__ __launch_bounds__(kThreads) void k_scan_partials(uint32_t *__restrict
   if (threadIdx.x == 0 && total64) *total64 = grand;
 }
 
-__global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t *__restrict__ out, size_t n,
+__global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t *__restrict__ out, size_t n_cap,
+                                                       const uint32_t *__restrict__ n_dev,
                                                        const uint32_t *__restrict__ partials) {
-  const uint32_t add = partials[blockIdx.x];
-  const size_t base = (size_t)blockIdx.x * kScanTile;
-  for (uint32_t k = threadIdx.x; k < kScanTile; k += kThreads) {
-    const size_t p = base + k;
-    if (p < n) out[p] += add;
+  const size_t n = scan_length(n_dev, n_cap);
+  for (size_t b = blockIdx.x + 1; b * kScanTile < n; b += gridDim.x) {
+    const uint32_t add = partials[b];
+    const size_t base = b * kScanTile;
+    for (uint32_t k = threadIdx.x; k < kScanTile; k += kThreads) {
+      const size_t p = base + k;
+      if (p < n) out[p] += add;
+    }
   }
 }
 
@@ -462,20 +479,25 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
     }
     const uint32_t k = p - s_first[lo];
     const uint32_t w = s_w[lo];
-    pair_keys[p] = vw.tile_off + (s_y0[lo] + k / w) * tiles_x + s_x0[lo] + k % w;
+    pair_keys[p] = (v << kPairViewShift) | ((s_y0[lo] + k / w) * tiles_x + s_x0[lo] + k % w);
     pair_vals[p] = s_slot[lo];
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_tile_ranges(const uint32_t *__restrict__ keys,
+// The pairs are sorted by local tile only; each view's pairs were emitted
+// as one contiguous run, so every (view, local tile) key is one contiguous
+// run of the sorted pairs.
+__global__ __launch_bounds__(kThreads) void k_tile_ranges(const ViewDev *__restrict__ views,
+                                                          const uint32_t *__restrict__ keys,
                                                           const uint32_t *__restrict__ m_dev,
                                                           uint2 *__restrict__ ranges) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   const uint32_t m = *m_dev;
-  if (i >= m) return;
-  const uint32_t t = keys[i];
-  if (i == 0 || keys[i - 1] != t) ranges[t].x = i;
-  if (i == m - 1 || keys[i + 1] != t) ranges[t].y = i + 1;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < m; i += gridDim.x * kThreads) {
+    const uint32_t k = keys[i];
+    const uint32_t t = views[k >> kPairViewShift].tile_off + (k & kPairTileMask);
+    if (i == 0 || keys[i - 1] != k) ranges[t].x = i;
+    if (i == m - 1 || keys[i + 1] != k) ranges[t].y = i + 1;
+  }
 }
 
 __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
@@ -491,13 +513,17 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
 // the reference's back-to-front premultiplied "over", draw_pass.rs:75, up to
 // that bound). Coverage is r^2 <= cut (the fragment test of wgsl:358-367
 // precomputed per splat); g = exp(-4.5 r^2) (wgsl:363).
-template <bool XCD>
+// COUNT: also adds to evals[0] the number of (pixel, splat) evaluations that
+// pass coverage before the pixel's early stop (the raster's algorithmic work)
+// and to evals[1] the number of coverage tests made.
+template <bool XCD, bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__ views,
                                                      const uint32_t *__restrict__ tile_start, uint32_t nviews,
                                                      uint32_t total_tiles, const float4 *__restrict__ rec,
                                                      const uint2 *__restrict__ ranges,
                                                      const uint32_t *__restrict__ list, float4 *__restrict__ img32,
-                                                     uchar4 *__restrict__ img8) {
+                                                     uchar4 *__restrict__ img8,
+                                                     unsigned long long *__restrict__ evals) {
   __shared__ float4 s0[kThreads], s1[kThreads], s2[kThreads];
   uint32_t tile = blockIdx.x;
   if (XCD) {
@@ -518,6 +544,7 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
   const uint2 rg = ranges[tile];
   float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
   bool done = !inside;
+  uint32_t passed = 0, tested = 0;
   for (uint32_t start = rg.x; start < rg.y; start += kThreads) {
     if (__syncthreads_count(!done) == 0) break;
     const uint32_t k = start + t;
@@ -538,7 +565,9 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
         const float lx = fmaf(a.w, dy, a.z * dx);
         const float ly = fmaf(b.y, dy, b.x * dx);
         const float r2 = fmaf(ly, ly, lx * lx);
+        if (COUNT) ++tested;
         if (r2 <= b.z) {
+          if (COUNT) ++passed;
           const float g = __expf(-4.5f * r2);
           const float4 c = s2[j];
           const float wgt = T * g;
@@ -560,6 +589,16 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
     img32[o] = make_float4(cr, cg, cb, A);
     auto q = [](float val) { return (unsigned char)(fminf(fmaxf(val, 0.0f), 1.0f) * 255.0f + 0.5f); };
     img8[o] = make_uchar4(q(cr), q(cg), q(cb), q(A));
+  }
+  if (COUNT) {
+    uint32_t w = passed, u = tested;
+#pragma unroll
+    for (uint32_t off = kWave / 2; off > 0; off >>= 1) {
+      w += __shfl_down(w, off, kWave);
+      u += __shfl_down(u, off, kWave);
+    }
+    if (__lane_id() == 0 && w) atomicAdd(evals, (unsigned long long)w);
+    if (__lane_id() == 0 && u) atomicAdd(evals + 1, (unsigned long long)u);
   }
 }
 
@@ -626,16 +665,18 @@ void launch_project_export(uint32_t deg, const ViewDev *views, uint32_t view, ui
 
 size_t scan_partial_entries(size_t n) { return std::max<size_t>(div_up(n, kScanTile), 1); }
 
-void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s, hipStream_t st) {
+void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s, hipStream_t st,
+                    const uint32_t *n_dev) {
   if (n == 0) {
     launch_fill_u32(reinterpret_cast<uint32_t *>(s.total), 0u, 2, st);
     return;
   }
-  const uint32_t blocks = div_up(n, kScanTile);
-  if (blocks > s.partial_cap) throw Error(Code::Internal, "scan scratch too small");
-  k_scan_blocks<<<blocks, kThreads, 0, st>>>(in, out, n, s.partials);
-  k_scan_partials<<<1, kThreads, 0, st>>>(s.partials, blocks, s.total);
-  if (blocks > 1) k_scan_add<<<blocks, kThreads, 0, st>>>(out, n, s.partials);
+  const uint32_t tiles = div_up(n, kScanTile);
+  if (tiles > s.partial_cap) throw Error(Code::Internal, "scan scratch too small");
+  const uint32_t grid = std::min<uint32_t>(tiles, kMaxGrid);
+  k_scan_blocks<<<grid, kThreads, 0, st>>>(in, out, n, n_dev, s.partials);
+  k_scan_partials<<<1, kThreads, 0, st>>>(s.partials, n, n_dev, s.total);
+  if (tiles > 1) k_scan_add<<<std::min<uint32_t>(tiles - 1, kMaxGrid), kThreads, 0, st>>>(out, n, n_dev, s.partials);
   GS_HIP(hipGetLastError());
 }
 
@@ -670,24 +711,34 @@ void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t st) {
   GS_HIP(hipGetLastError());
 }
 
-void launch_tile_ranges(const uint32_t *pair_keys, const uint32_t *m_dev, uint32_t capacity, uint32_t tiles,
+void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const uint32_t *m_dev, uint32_t capacity,
+                        uint32_t tiles,
                         uint2 *ranges, hipStream_t st) {
   launch_fill_u32(reinterpret_cast<uint32_t *>(ranges), 0u, 2 * (size_t)tiles, st);
   if (capacity == 0) return;
-  k_tile_ranges<<<div_up(capacity, kThreads), kThreads, 0, st>>>(pair_keys, m_dev, ranges);
+  k_tile_ranges<<<std::min<uint32_t>(div_up(capacity, kThreads), kMaxGrid), kThreads, 0, st>>>(views, pair_keys,
+                                                                                              m_dev, ranges);
   GS_HIP(hipGetLastError());
 }
 
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
                    uint32_t total_tiles, const float4 *rec, const uint2 *ranges, const uint32_t *list,
-                   float4 *img32, uchar4 *img8, hipStream_t st) {
+                   float4 *img32, uchar4 *img8, hipStream_t st, unsigned long long *evals) {
   if (total_tiles == 0 || nviews == 0) return;
-  if (variant == RasterVariant::XcdBand)
-    k_raster<true><<<8 * div_up(total_tiles, 8), kThreads, 0, st>>>(views, tile_start, nviews, total_tiles, rec,
-                                                                     ranges, list, img32, img8);
+  const bool xcd = variant == RasterVariant::XcdBand;
+  const dim3 grid(xcd ? 8 * div_up(total_tiles, 8) : total_tiles), block(kThreads);
+  if (xcd && evals)
+    k_raster<true, true><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list, img32,
+                                                 img8, evals);
+  else if (xcd)
+    k_raster<true, false><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list, img32,
+                                                  img8, evals);
+  else if (evals)
+    k_raster<false, true><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list, img32,
+                                                  img8, evals);
   else
-    k_raster<false><<<total_tiles, kThreads, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list,
-                                                       img32, img8);
+    k_raster<false, false><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list,
+                                                   img32, img8, evals);
   GS_HIP(hipGetLastError());
 }
 

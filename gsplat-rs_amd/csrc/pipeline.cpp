@@ -2,6 +2,7 @@
 
 #include "pipeline.h"
 
+#include <cstdlib>
 #include <cstring>
 
 #include "sort.h"
@@ -11,7 +12,8 @@ namespace gs {
 namespace {
 
 constexpr size_t kTableBytes = 2 * kMaxViews * sizeof(uint32_t) + kMaxViews * sizeof(ViewDev);
-constexpr size_t kSummaryWords = 4 + kMaxViews;
+constexpr size_t kEvalsWord = 4 + kMaxViews;  // u64 evaluations passing, u64 tests made (counting mode)
+constexpr size_t kSummaryWords = kEvalsWord + 4;
 
 uint32_t bits_for(uint64_t count) {
   uint32_t b = 1;
@@ -54,6 +56,7 @@ uint64_t frame_working_bytes(uint64_t slots, uint64_t pixels, uint64_t tiles, ui
 
 Pipeline::Pipeline() {
   if (!device_present()) throw Error(Code::Unsupported, "gpu rasterizer unavailable");
+  if (const char *e = std::getenv("GSPLAT_COUNT_EVALS")) count_evals_ = std::atoi(e) != 0;
   GS_HIP(hipSetDevice(0));
   GS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   for (auto &e : ev_) GS_HIP(hipEventCreate(&e));
@@ -139,7 +142,9 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
 void Pipeline::launch_frame() {
   const uint32_t nviews = (uint32_t)views_.size();
   const uint32_t slots = total_slots_, tiles = total_tiles_;
-  const uint32_t tile_bits = bits_for(tiles);
+  uint32_t view_tiles = 1;
+  for (const ViewDev &d : host_views_) view_tiles = std::max(view_tiles, d.tiles);
+  const uint32_t tile_bits = bits_for(view_tiles);  // the pair sort orders tiles within a view
   const size_t m = std::max<uint32_t>(slots, 1);
   keys_.ensure(m);
   vals_.ensure(m);
@@ -202,13 +207,20 @@ void Pipeline::launch_frame() {
   launch_tile_emit(views_d, seg_start_d, nviews, slots, v, rect_.get(), counts_.get(), offsets_.get(), pair_cap_,
                    pair_keys_.get(), pair_vals_.get(), st);
   uint32_t *pk = pair_keys_.get(), *pv = pair_vals_.get(), *pka = pair_keys_alt_.get(), *pva = pair_vals_alt_.get();
+  // The last frame's pair count (+25%) sizes the pair sort's grids.
+  const uint32_t hint = pairs_ ? (uint32_t)std::min<uint64_t>(pairs_ + pairs_ / 4, pair_cap_) : pair_cap_;
   onesweep_sort(&pk, &pv, &pka, &pva, pair_cap_, m_dev_.get(), tile_bits, /*descending=*/false,
-                sort_scratch_.get(), sort_scratch_.capacity(), st);
-  launch_tile_ranges(pk, m_dev_.get(), pair_cap_, tiles, ranges_.get(), st);
+                sort_scratch_.get(), sort_scratch_.capacity(), st, -1, hint);
+  launch_tile_ranges(views_d, pk, m_dev_.get(), pair_cap_, tiles, ranges_.get(), st);
   GS_HIP(hipEventRecord(ev_[3], st));
 
+  unsigned long long *evals = nullptr;
+  if (count_evals_) {
+    evals = reinterpret_cast<unsigned long long *>(info + kEvalsWord);
+    launch_fill_u32(info + kEvalsWord, 0u, 4, st);
+  }
   launch_raster(raster_variant_, views_d, tile_start_d, nviews, tiles, rec_.get(), ranges_.get(), pv, img32_.get(),
-                img8_.get(), st);
+                img8_.get(), st, evals);
   GS_HIP(hipEventRecord(ev_[4], st));
   for (size_t i = 0; i < host_rgba8_.size(); ++i) {
     if (!host_rgba8_[i]) continue;
@@ -217,6 +229,9 @@ void Pipeline::launch_frame() {
                           hipMemcpyDeviceToHost, st));
   }
   GS_HIP(hipMemcpyAsync(summary_pinned_, info, (4 + nviews) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  if (count_evals_)
+    GS_HIP(hipMemcpyAsync(summary_pinned_ + kEvalsWord, info + kEvalsWord, 4 * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, st));
   GS_HIP(hipEventRecord(ev_[5], st));
   pending_ = true;
   summary_valid_ = false;
@@ -248,7 +263,9 @@ void Pipeline::wait() {
     float ms[4] = {0, 0, 0, 0}, total = 0;
     for (int s = 0; s < 4; ++s) GS_HIP(hipEventElapsedTime(&ms[s], ev_[s], ev_[s + 1]));
     GS_HIP(hipEventElapsedTime(&total, ev_[0], ev_[4]));
-    times_ = FrameTimes{ms[0], ms[1], ms[2], ms[3], total};
+    uint64_t evals[2] = {0, 0};
+    if (count_evals_) std::memcpy(evals, summary_pinned_ + kEvalsWord, sizeof(evals));
+    times_ = FrameTimes{ms[0], ms[1], ms[2], ms[3], total, evals[0], evals[1]};
     pending_ = false;
     summary_valid_ = true;
   }

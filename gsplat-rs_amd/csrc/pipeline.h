@@ -91,6 +91,8 @@ struct ViewSpec {
 // Stage timings of the last frame (device events), in milliseconds.
 struct FrameTimes {
   float project_ms = 0, sort_ms = 0, bin_ms = 0, raster_ms = 0, total_ms = 0;
+  uint64_t blend_evals = 0;  // counting mode only: evaluations passing coverage
+  uint64_t blend_tests = 0;  // counting mode only: coverage tests made
 };
 
 // Renders batches of views on its own stream. Per frame (Renderer::
@@ -144,6 +146,10 @@ class Pipeline {
   void export_projection(const DeviceScene &scene, const FrameParams &fp, uint32_t *keys_out, float *proj_out);
 
   void set_raster_variant(RasterVariant v) { raster_variant_ = v; }
+  // Counting mode (also GSPLAT_COUNT_EVALS=1): the raster counts the
+  // (pixel, splat) evaluations that pass coverage before each pixel's early
+  // stop; the count is reported with the frame times.
+  void set_count_evals(bool on) { count_evals_ = on; }
   hipStream_t stream() const { return stream_; }
 
  private:
@@ -186,6 +192,7 @@ class Pipeline {
   uint64_t overflow_rerenders_ = 0;
   FrameTimes times_{};
   RasterVariant raster_variant_ = RasterVariant::Linear;
+  bool count_evals_ = false;
 };
 
 }  // namespace gs

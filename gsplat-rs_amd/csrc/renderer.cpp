@@ -86,6 +86,8 @@ GsplatStageTimes Renderer::stage_times() {
   s.tiles_x = d.fp.tiles_x;
   s.tiles_y = d.fp.tiles_y;
   s.tile_pairs = pipe_.pairs();
+  s.blend_evals = t.blend_evals;
+  s.blend_tests = t.blend_tests;
   return s;
 }
 

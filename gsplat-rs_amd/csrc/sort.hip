@@ -1,17 +1,25 @@
-// sort.hip -- onesweep LSD radix sort of (u32 key, u32 value) pairs for gfx950.
+// sort.hip -- LSD radix sort of (u32 key, u32 value) pairs for gfx950, 8-bit
+// digits, 4096-item tiles of 256 threads x 16 items.
 //
+// Every pass ranks a tile the same way: its four waves work independently (a
+// wave owns 1024 consecutive items and a private LDS histogram; ballot
+// match-any over the 8 digit bits + popcount gives the rank among equal
+// digits), then the tile is staged in LDS in digit order and written out
+// coalesced. The passes differ in how a tile learns where its digits go:
+//
+// Look-back mode (onesweep), for sorts of fewer tiles:
 //   k_os_clear   clears the histograms and the look-back state
 //   k_os_hist    one read of the keys (and values, for a value pass): LDS
 //                digit histograms of every pass, added into striped global
 //                copies
-//   k_os_pass    one launch per 8-bit digit: each 4096-key tile is ranked by
-//                its four waves independently (a wave owns 1024 consecutive
-//                keys and a private LDS histogram; ballot match-any over the
-//                8 digit bits + popcount gives the rank among equal digits),
-//                the tile publishes its digit counts, resolves its global
-//                offsets by decoupled look-back over the preceding tiles
-//                (32 predecessors polled per step), stages the tile in LDS
-//                in digit order and writes it out coalesced
+//   k_os_pass    one launch per digit: the tile publishes its digit counts
+//                and resolves its prefix over the preceding tiles by
+//                decoupled look-back, 8 predecessors polled per step
+// Reduce-then-scan mode, for long sorts (the look-back front advances a
+// bounded number of tiles per coherent round trip, which caps its rate):
+//   k_rts_up     per-tile digit counts, digit-major
+//   scan         one exclusive scan of all counts = every (digit, tile) start
+//   k_rts_down   rank and scatter
 //
 // The sort is stable, so with keys complemented (descending) and values in
 // ascending index order it yields the reference's back-to-front order: key
@@ -22,14 +30,17 @@
 //
 // Forward progress of the look-back: tiles take their index from a per-pass
 // atomic counter when they start, so every tile a workgroup waits on has
-// already started and finishes without waiting on later tiles.
+// already started and finishes without waiting on later tiles. The
+// reduce-then-scan passes have no inter-workgroup waits.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <string>
 
 #include "common.h"
+#include "kernels.h"
 #include "sort.h"
 
 namespace gs {
@@ -52,6 +63,8 @@ constexpr uint32_t MAX_PASSES = 5;
 constexpr uint32_t HIST_BLOCKS = 512;
 constexpr uint32_t STRIPES = 8;
 constexpr uint32_t LOOKBACK = 8;  // predecessors polled per look-back step
+constexpr uint32_t kPassGrid = 1024;  // look-back passes: resident 38 KB-LDS workgroups, 4 per CU
+constexpr uint32_t kMinGrid = 1024;   // floor of a hinted grid
 constexpr uint32_t FLAG_AGG = 1u << 30;  // tile count published
 constexpr uint32_t FLAG_INC = 2u << 30;  // inclusive prefix published
 constexpr uint32_t VALUE_MASK = (1u << 30) - 1;
@@ -111,7 +124,11 @@ __device__ __forceinline__ uint32_t load_count(const uint32_t *n_dev, uint32_t n
   return n_dev ? min(*n_dev, n_cap) : n_cap;
 }
 
-__global__ __launch_bounds__(T) void k_os_clear(uint32_t *__restrict__ p, size_t n) {
+// Clears fixed + per_tile * (live tiles) words.
+__global__ __launch_bounds__(T) void k_os_clear(uint32_t *__restrict__ p, size_t fixed, size_t per_tile,
+                                                const uint32_t *__restrict__ n_dev, uint32_t n_cap) {
+  const size_t tiles_n = (load_count(n_dev, n_cap) + TILE - 1) / TILE;
+  const size_t n = fixed + per_tile * tiles_n;
   for (size_t i = (size_t)blockIdx.x * T + threadIdx.x; i < n; i += (size_t)gridDim.x * T) p[i] = 0u;
 }
 
@@ -151,53 +168,17 @@ __global__ __launch_bounds__(T) void k_os_hist(const uint32_t *__restrict__ keys
   }
 }
 
-__global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
-                                               uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                               const uint32_t *__restrict__ n_dev, uint32_t n_cap, uint32_t pass,
-                                               uint32_t shift, uint32_t flip, uint32_t from_value,
-                                               const uint32_t *__restrict__ hist, uint32_t *__restrict__ counter,
-                                               uint32_t *__restrict__ status) {
-  __shared__ uint32_t s_tile, s_trivial;
-  __shared__ uint32_t lds4[WAVES];
-  __shared__ uint32_t gofs[256];    // global start of each digit
-  __shared__ uint32_t lstart[256];  // start of each digit inside this tile
-  __shared__ uint32_t prefix[256];  // count of each digit in earlier tiles
-  __shared__ uint32_t whist[WAVES][256];
-  __shared__ uint32_t skeys[TILE];
-  __shared__ uint32_t svals[TILE];
-
+// Loads tile items [base, base+count) and ranks each among the items of
+// equal digit in its wave: wave w owns items [w*1024, (w+1)*1024) in order, 64
+// per round; within a round the lanes of equal digit are ranked by lane and
+// the running per-digit count lives in whist[w]. On return whist[q][d] is the
+// start of wave q's digit-d items among the tile's digit-d items, and the
+// result is the tile's count of digit threadIdx.x. whist must be zero.
+__device__ __forceinline__ uint32_t rank_tile(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                              size_t base, uint32_t count, uint32_t shift, uint32_t flip,
+                                              uint32_t from_value, uint32_t (&key)[IPT], uint32_t (&val)[IPT],
+                                              uint32_t (&rank)[IPT], uint32_t (*whist)[256]) {
   const uint32_t t = threadIdx.x, w = t / 64, lane = __lane_id();
-  const uint32_t n = load_count(n_dev, n_cap);
-  if (t == 0) {
-    s_tile = atomicAdd(counter, 1u);
-    s_trivial = 0;
-  }
-#pragma unroll
-  for (uint32_t q = 0; q < WAVES; ++q) whist[q][t] = 0;
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const size_t base = (size_t)tile * TILE;
-  if (base >= n) return;  // uniform: no later tile has work either
-  const uint32_t count = (uint32_t)min((size_t)TILE, (size_t)n - base);
-
-  uint32_t hd = 0;
-#pragma unroll
-  for (uint32_t c = 0; c < STRIPES; ++c) hd += hist[(c * MAX_PASSES + pass) * 256 + t];
-  if (hd == n) s_trivial = 1;  // every key has this digit: the pass is the identity
-  gofs[t] = block_exclusive_scan(hd, lds4);
-  __syncthreads();
-  if (s_trivial) {
-    for (uint32_t j = t; j < count; j += T) {
-      kout[base + j] = kin[base + j];
-      vout[base + j] = vin[base + j];
-    }
-    return;
-  }
-
-  // Wave-private ranking: wave w owns tile items [w*1024, (w+1)*1024) in
-  // order, 64 per round; within a round the lanes of equal digit are ranked
-  // by lane, and the running per-digit count lives in whist[w].
-  uint32_t key[IPT], val[IPT], rank[IPT];
   uint32_t *wh = whist[w];
 #pragma unroll
   for (uint32_t r = 0; r < IPT; ++r) {
@@ -221,46 +202,19 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
     whist[q][t] = run;
     run += c;
   }
+  return run;
+}
 
-  // Publish this tile's count of digit t, then look back for the prefix.
-  uint32_t *st = status + (size_t)tile * 256 + t;
-  if (tile == 0) {
-    __hip_atomic_store(st, FLAG_INC | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    prefix[t] = 0;
-  } else {
-    __hip_atomic_store(st, FLAG_AGG | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t excl = 0;
-    int64_t j = (int64_t)tile - 1;  // next predecessor not yet consumed
-    while (j >= 0) {
-      uint32_t sv[LOOKBACK];
-#pragma unroll
-      for (uint32_t k = 0; k < LOOKBACK; ++k)
-        sv[k] = (j - (int64_t)k >= 0)
-                    ? __hip_atomic_load(status + (size_t)(j - k) * 256 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                    : FLAG_INC;
-      // Consume predecessors nearest first until an inclusive prefix
-      // (done) or an unpublished tile (poll again from there).
-      int64_t consumed = 0;
-      bool finished = false, stop = false;
-#pragma unroll
-      for (uint32_t k = 0; k < LOOKBACK; ++k) {
-        const uint32_t flag = sv[k] & ~VALUE_MASK;
-        const bool take = !stop && flag != 0;
-        excl += take ? (sv[k] & VALUE_MASK) : 0u;
-        consumed += take ? 1 : 0;
-        finished = finished || (take && flag == FLAG_INC);
-        stop = stop || !take || flag == FLAG_INC;
-      }
-      if (finished) break;
-      j -= consumed;
-    }
-    __hip_atomic_store(st, FLAG_INC | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    prefix[t] = excl;
-  }
-  lstart[t] = block_exclusive_scan(run, lds4);
-  __syncthreads();
-
-  // Stage in digit order, then write coalesced.
+// Stages the ranked tile in LDS in digit order (lstart[d] = start of digit d
+// in the tile) and writes it out coalesced; digit d's items go to
+// dst[d] + (position among the tile's digit-d items).
+__device__ __forceinline__ void scatter_tile(const uint32_t (&key)[IPT], const uint32_t (&val)[IPT],
+                                             const uint32_t (&rank)[IPT], uint32_t (*whist)[256],
+                                             const uint32_t *lstart, const uint32_t *dst, uint32_t count,
+                                             uint32_t shift, uint32_t flip, uint32_t from_value, uint32_t *skeys,
+                                             uint32_t *svals, uint32_t *__restrict__ kout,
+                                             uint32_t *__restrict__ vout) {
+  const uint32_t t = threadIdx.x, w = t / 64;
 #pragma unroll
   for (uint32_t r = 0; r < IPT; ++r) {
     if (rank[r] != 0xFFFFFFFFu) {
@@ -274,9 +228,166 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
   for (uint32_t j = t; j < count; j += T) {
     const uint32_t k = skeys[j], v = svals[j];
     const uint32_t d = digit(k, v, shift, flip, from_value);
-    const size_t pos = (size_t)gofs[d] + prefix[d] + (j - lstart[d]);
+    const size_t pos = (size_t)dst[d] + (j - lstart[d]);
     kout[pos] = k;
     vout[pos] = v;
+  }
+}
+
+// Onesweep pass: one launch per digit, tile offsets by decoupled look-back.
+// Persistent: each workgroup takes tiles from the pass counter until they run
+// past the live element count, so the grid is bounded whatever the capacity.
+// Pass p's status words start at p * (live tiles) * 256.
+__global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                               uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                               const uint32_t *__restrict__ n_dev, uint32_t n_cap, uint32_t pass,
+                                               uint32_t shift, uint32_t flip, uint32_t from_value,
+                                               const uint32_t *__restrict__ hist, uint32_t *__restrict__ counter,
+                                               uint32_t *__restrict__ status_all) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t lds4[WAVES];
+  __shared__ uint32_t gofs[256];    // global start of this tile's items of each digit
+  __shared__ uint32_t lstart[256];  // start of each digit inside this tile
+  __shared__ uint32_t whist[WAVES][256];
+  __shared__ uint32_t skeys[TILE];
+  __shared__ uint32_t svals[TILE];
+
+  const uint32_t t = threadIdx.x;
+  const uint32_t n = load_count(n_dev, n_cap);
+  const uint32_t tiles_n = (n + TILE - 1) / TILE;
+  uint32_t *status = status_all + (size_t)pass * tiles_n * 256;
+  uint32_t hd = 0;
+#pragma unroll
+  for (uint32_t c = 0; c < STRIPES; ++c) hd += hist[(c * MAX_PASSES + pass) * 256 + t];
+  const uint32_t g = block_exclusive_scan(hd, lds4);
+  // every key has the same digit: the pass is the identity
+  const bool trivial = __syncthreads_or(n > 0 && hd == n) != 0;
+
+  for (;;) {
+    if (t == 0) s_tile = atomicAdd(counter, 1u);
+#pragma unroll
+    for (uint32_t q = 0; q < WAVES; ++q) whist[q][t] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const size_t base = (size_t)tile * TILE;
+    if (base >= n) break;  // uniform: no later tile has work either
+    const uint32_t count = (uint32_t)min((size_t)TILE, (size_t)n - base);
+    if (trivial) {
+      for (uint32_t j = t; j < count; j += T) {
+        kout[base + j] = kin[base + j];
+        vout[base + j] = vin[base + j];
+      }
+      __syncthreads();
+      continue;
+    }
+
+    uint32_t key[IPT], val[IPT], rank[IPT];
+    const uint32_t run = rank_tile(kin, vin, base, count, shift, flip, from_value, key, val, rank, whist);
+
+    // Publish this tile's count of digit t, then look back for the prefix.
+    uint32_t *st = status + (size_t)tile * 256 + t;
+    uint32_t excl = 0;
+    if (tile == 0) {
+      __hip_atomic_store(st, FLAG_INC | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(st, FLAG_AGG | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int64_t j = (int64_t)tile - 1;  // next predecessor not yet consumed
+      while (j >= 0) {
+        uint32_t sv[LOOKBACK];
+#pragma unroll
+        for (uint32_t k = 0; k < LOOKBACK; ++k)
+          sv[k] = (j - (int64_t)k >= 0) ? __hip_atomic_load(status + (size_t)(j - k) * 256 + t, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT)
+                                        : FLAG_INC;
+        // Consume predecessors nearest first until an inclusive prefix
+        // (done) or an unpublished tile (poll again from there).
+        int64_t consumed = 0;
+        bool finished = false, stop = false;
+#pragma unroll
+        for (uint32_t k = 0; k < LOOKBACK; ++k) {
+          const uint32_t flag = sv[k] & ~VALUE_MASK;
+          const bool take = !stop && flag != 0;
+          excl += take ? (sv[k] & VALUE_MASK) : 0u;
+          consumed += take ? 1 : 0;
+          finished = finished || (take && flag == FLAG_INC);
+          stop = stop || !take || flag == FLAG_INC;
+        }
+        if (finished) break;
+        j -= consumed;
+      }
+      __hip_atomic_store(st, FLAG_INC | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    gofs[t] = g + excl;
+    lstart[t] = block_exclusive_scan(run, lds4);
+    __syncthreads();
+    scatter_tile(key, val, rank, whist, lstart, gofs, count, shift, flip, from_value, skeys, svals, kout, vout);
+    __syncthreads();
+  }
+}
+
+// Reduce-then-scan pass, step 1: digit counts of every live tile,
+// digit-major (counts[d * tiles + tile], tiles = live tile count), so one
+// exclusive scan of the 256 * tiles counts (length published in *entries)
+// gives every (digit, tile) its global output start.
+__global__ __launch_bounds__(T) void k_rts_up(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                              const uint32_t *__restrict__ n_dev, uint32_t n_cap, uint32_t shift,
+                                              uint32_t flip, uint32_t from_value, uint32_t *__restrict__ counts,
+                                              uint32_t *__restrict__ entries) {
+  __shared__ uint32_t h[WAVES][256];
+  const uint32_t t = threadIdx.x, w = t / 64;
+  const uint32_t n = load_count(n_dev, n_cap);
+  const uint32_t tiles = (n + TILE - 1) / TILE;
+  if (blockIdx.x == 0 && t == 0) *entries = 256 * tiles;
+  const uint32_t *src = from_value ? vin : kin;
+  for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+#pragma unroll
+    for (uint32_t q = 0; q < WAVES; ++q) h[q][t] = 0;
+    __syncthreads();
+    const size_t base = (size_t)tile * TILE;
+    const uint32_t count = (uint32_t)min((size_t)TILE, (size_t)n - base);
+    uint32_t x[IPT];
+#pragma unroll
+    for (uint32_t r = 0; r < IPT; ++r) {
+      const uint32_t j = r * T + t;
+      x[r] = j < count ? src[base + j] : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < IPT; ++r)
+      if (r * T + t < count) atomicAdd(&h[w][((x[r] ^ flip) >> shift) & 0xFFu], 1u);
+    __syncthreads();
+    counts[(size_t)t * tiles + tile] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
+    __syncthreads();
+  }
+}
+
+// Reduce-then-scan pass, step 3: rank and scatter with the scanned offsets.
+__global__ __launch_bounds__(T) void k_rts_down(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                const uint32_t *__restrict__ n_dev, uint32_t n_cap, uint32_t shift,
+                                                uint32_t flip, uint32_t from_value,
+                                                const uint32_t *__restrict__ offsets) {
+  __shared__ uint32_t lds4[WAVES];
+  __shared__ uint32_t dst[256];
+  __shared__ uint32_t lstart[256];
+  __shared__ uint32_t whist[WAVES][256];
+  __shared__ uint32_t skeys[TILE];
+  __shared__ uint32_t svals[TILE];
+  const uint32_t t = threadIdx.x;
+  const uint32_t n = load_count(n_dev, n_cap);
+  const uint32_t tiles = (n + TILE - 1) / TILE;
+  for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const size_t base = (size_t)tile * TILE;
+    const uint32_t count = (uint32_t)min((size_t)TILE, (size_t)n - base);
+#pragma unroll
+    for (uint32_t q = 0; q < WAVES; ++q) whist[q][t] = 0;
+    dst[t] = offsets[(size_t)t * tiles + tile];
+    __syncthreads();
+    uint32_t key[IPT], val[IPT], rank[IPT];
+    const uint32_t run = rank_tile(kin, vin, base, count, shift, flip, from_value, key, val, rank, whist);
+    lstart[t] = block_exclusive_scan(run, lds4);
+    __syncthreads();
+    scatter_tile(key, val, rank, whist, lstart, dst, count, shift, flip, from_value, skeys, svals, kout, vout);
+    __syncthreads();
   }
 }
 
@@ -286,17 +397,38 @@ uint32_t hist_blocks(uint32_t n_cap) { return std::max<uint32_t>(std::min<uint32
 
 }  // namespace
 
-// Layout: hist [STRIPES][MAX_PASSES][256] | counters [MAX_PASSES] |
-// status [passes][tiles][256], all cleared at the start of every sort.
+// Tile count from which the reduce-then-scan passes replace the look-back
+// passes: the look-back front advances a bounded number of tiles per
+// coherent round trip, so long sorts are faster with three launches per pass
+// and no chain (GSPLAT_SORT_RTS_MIN_TILES overrides).
+// Bound of the pass grids (GSPLAT_SORT_MAX_GRID overrides, default none):
+// the passes loop over tiles, so any grid covers any count; one workgroup
+// per tile measured fastest for long sorts (no per-workgroup serial tiles).
+static uint32_t pass_grid_bound() {
+  const char *e = std::getenv("GSPLAT_SORT_MAX_GRID");
+  return e ? std::max<uint32_t>((uint32_t)std::strtoul(e, nullptr, 10), 1u) : 0xFFFFFFFFu;
+}
+
+static uint32_t rts_min_tiles() {
+  const char *e = std::getenv("GSPLAT_SORT_RTS_MIN_TILES");
+  return e ? (uint32_t)std::strtoul(e, nullptr, 10) : 512u;
+}
+
+// Layout, look-back mode: hist [STRIPES][MAX_PASSES][256] | counters
+// [MAX_PASSES] | status [passes][tiles][256], all cleared at the start of
+// every sort. Reduce-then-scan mode: counts [256][tiles] | scan partials |
+// 64-bit scan total; nothing needs clearing.
 size_t onesweep_status_words(uint32_t n_cap, uint32_t bits) {
   const uint32_t passes = std::min<uint32_t>(std::max<uint32_t>((bits + 7) / 8, 1), 4) + 1;
   const size_t tiles = std::max<uint32_t>(div_up(n_cap, TILE), 1);
-  return (size_t)STRIPES * MAX_PASSES * 256 + MAX_PASSES + (size_t)passes * tiles * 256;
+  const size_t lookback = (size_t)STRIPES * MAX_PASSES * 256 + MAX_PASSES + (size_t)passes * tiles * 256;
+  const size_t rts = 256 * tiles + scan_partial_entries(256 * tiles) + 4;
+  return std::max(lookback, rts);
 }
 
 void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n_cap,
                    const uint32_t *n_dev, uint32_t bits, bool descending, uint32_t *scratch, size_t scratch_words,
-                   hipStream_t st, int value_byte) {
+                   hipStream_t st, int value_byte, uint32_t n_hint) {
   if (n_cap <= 1 || (bits == 0 && value_byte < 0)) return;
   if (n_cap > (1u << 30)) throw Error(Code::Unsupported, "sort length exceeds 2^30");
   PassSet ps{};
@@ -314,16 +446,46 @@ void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32
     ++ps.count;
   }
   const uint32_t tiles = div_up(n_cap, TILE);
-  const size_t words = (size_t)STRIPES * MAX_PASSES * 256 + MAX_PASSES + (size_t)ps.count * tiles * 256;
+  // Grid: one workgroup per expected tile (the hint, for a device-side count,
+  // with the capacity as the cap), bounded by GSPLAT_SORT_MAX_GRID.
+  const uint32_t expect = n_dev && n_hint ? std::min(div_up(n_hint, TILE), tiles) : tiles;
+  const uint32_t grid_tiles = std::min(std::max(expect, std::min(tiles, kMinGrid)), pass_grid_bound());
+  if (tiles >= rts_min_tiles()) {
+    const size_t entries = (size_t)256 * tiles;
+    const size_t partials = scan_partial_entries(entries);
+    if (entries + partials + 4 > scratch_words) throw Error(Code::Internal, "radix sort scratch too small");
+    uint32_t *counts = scratch;
+    ScanScratch sc;
+    sc.partials = counts + entries;
+    sc.partial_cap = partials;
+    uint32_t *live_entries = counts + entries + partials;
+    sc.total = reinterpret_cast<uint64_t *>(scratch + ((entries + partials + 2) & ~size_t(1)));
+    const uint32_t up_grid = grid_tiles, down_grid = grid_tiles;
+    for (uint32_t p = 0; p < ps.count; ++p) {
+      k_rts_up<<<up_grid, T, 0, st>>>(*keys, *vals, n_dev, n_cap, ps.shift[p], ps.flip[p], ps.from_value[p], counts,
+                                      live_entries);
+      exclusive_scan(counts, counts, entries, sc, st, live_entries);
+      k_rts_down<<<down_grid, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, ps.shift[p], ps.flip[p],
+                                          ps.from_value[p], counts);
+      std::swap(*keys, *keys_alt);
+      std::swap(*vals, *vals_alt);
+    }
+    GS_HIP(hipGetLastError());
+    return;
+  }
+  const size_t fixed = (size_t)STRIPES * MAX_PASSES * 256 + MAX_PASSES;
+  const size_t words = fixed + (size_t)ps.count * tiles * 256;
   if (words > scratch_words) throw Error(Code::Internal, "onesweep scratch too small");
   uint32_t *hist = scratch;
   uint32_t *counters = hist + (size_t)STRIPES * MAX_PASSES * 256;
   uint32_t *status = counters + MAX_PASSES;
-  k_os_clear<<<std::min<uint32_t>(div_up(words, T), 1024), T, 0, st>>>(scratch, words);
+  k_os_clear<<<std::min<uint32_t>(div_up(words, T), 1024), T, 0, st>>>(scratch, fixed, (size_t)ps.count * 256, n_dev,
+                                                                        n_cap);
   k_os_hist<<<hist_blocks(n_cap), T, 0, st>>>(*keys, *vals, n_dev, n_cap, ps, hist);
+  const uint32_t grid = std::min(grid_tiles, kPassGrid);
   for (uint32_t p = 0; p < ps.count; ++p) {
-    k_os_pass<<<tiles, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, p, ps.shift[p], ps.flip[p],
-                                   ps.from_value[p], hist, counters + p, status + (size_t)p * tiles * 256);
+    k_os_pass<<<grid, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, p, ps.shift[p], ps.flip[p],
+                                  ps.from_value[p], hist, counters + p, status);
     std::swap(*keys, *keys_alt);
     std::swap(*vals, *vals_alt);
   }

@@ -95,6 +95,8 @@ class GsplatStageTimes(ctypes.Structure):
         ("tiles_x", ctypes.c_uint32),
         ("tiles_y", ctypes.c_uint32),
         ("tile_pairs", ctypes.c_uint64),
+        ("blend_evals", ctypes.c_uint64),
+        ("blend_tests", ctypes.c_uint64),
     ]
 
 

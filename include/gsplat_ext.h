@@ -85,6 +85,12 @@ typedef struct GsplatStageTimes {
   uint32_t tiles_x;
   uint32_t tiles_y;
   uint64_t tile_pairs;     /* M: (tile, splat) pairs                   */
+  uint64_t blend_evals;    /* E: (pixel, splat) evaluations passing the
+                            * coverage test before the pixel's early stop;
+                            * counted only when GSPLAT_COUNT_EVALS=1 (a
+                            * counting raster variant), else 0           */
+  uint64_t blend_tests;    /* coverage tests the raster made (counting
+                            * variant only, else 0)                      */
 } GsplatStageTimes;
 int32_t gsplat_context_get_stage_times(GsplatContext *ctx, GsplatStageTimes *out);
 

@@ -606,6 +606,62 @@ uint32_t oracle_render(const OracleScene *s, const OracleCamera *cam, uint32_t w
   return vis;
 }
 
+// Front-to-back statement of the blend's work, for the raster roofline: per
+// pixel, the visible drawn splats nearest first; an evaluation passes when
+// r2 <= cut (the splat's coverage bound, oracle_coverage_cut capped at 1);
+// each passing one updates T *= 1 - alpha*expf(-4.5*r2) and the pixel stops
+// once T < 1e-4. Returns the number of passing evaluations.
+uint64_t oracle_blend_evals(const OracleScene *s, const OracleCamera *cam, uint32_t width, uint32_t height) {
+  if (!oracle_camera_valid(cam) || width == 0 || height == 0) return 0;
+  std::vector<uint32_t> keys(s->count), order(s->count);
+  uint32_t vis = oracle_preprocess(s, cam, keys.data(), order.data());
+  oracle_sort_values_by_keys(keys.data(), order.data(), vis);
+  View v = make_view(cam);
+  FrameConsts k = frame_consts(v, width, height);
+  std::vector<float> alpha(s->count), cov(6 * (size_t)s->count);
+  oracle_precompute(s, alpha.data(), cov.data());
+  struct Inst {
+    Proj p;
+    InvA ia;
+    float cut;
+    int x0, y0, x1, y1;
+  };
+  std::vector<Inst> inst;
+  for (uint32_t j = vis; j-- > 0;) {  // front to back
+    uint32_t i = order[j];
+    Proj p = project_splat(v, k, s->positions + 3 * (size_t)i, cov.data() + 6 * (size_t)i, alpha[i]);
+    if (!p.valid) continue;
+    Inst in;
+    in.p = p;
+    in.ia = inverse_axes(p);
+    in.cut = std::min(oracle_coverage_cut(p.alpha), 1.0f);
+    if (in.cut < 0.0f) continue;
+    int r[4];
+    if (!tile_rect(p, in.cut, width, height, 1, r)) continue;  // pixel bounds
+    in.x0 = r[0];
+    in.y0 = r[1];
+    in.x1 = r[2];
+    in.y1 = r[3];
+    inst.push_back(in);
+  }
+  std::vector<float> T((size_t)width * height, 1.0f);
+  uint64_t evals = 0;
+  for (const Inst &in : inst) {
+    for (int y = in.y0; y <= in.y1; ++y) {
+      float pny = pixel_ndc_y((uint32_t)y, height);
+      for (int x = in.x0; x <= in.x1; ++x) {
+        float &t = T[(size_t)y * width + x];
+        if (t < 1e-4f) continue;
+        float r2 = local_r2(in.ia, in.p.cx, in.p.cy, pixel_ndc_x((uint32_t)x, width), pny);
+        if (!(r2 <= in.cut)) continue;
+        ++evals;
+        t = t * (1.0f - in.p.alpha * std::exp(-4.5f * r2));
+      }
+    }
+  }
+  return evals;
+}
+
 void oracle_last_timings(double out_ms[4]) {
   for (int i = 0; i < 4; ++i) out_ms[i] = g_timings[i];
 }

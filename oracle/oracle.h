@@ -31,6 +31,8 @@
  *                            analytic single-splat image, brute force
  *   oracle_tile_counts       brute force over the oracle's own coverage
  *                            (every covered pixel lies in a counted tile)
+ *   oracle_blend_evals       brute force per pixel over all splats on tiny
+ *                            scenes; single opaque splat closed form
  */
 #ifndef GSPLAT_ORACLE_H
 #define GSPLAT_ORACLE_H
@@ -97,6 +99,12 @@ void oracle_tile_counts(uint32_t count, const float *proj12, uint32_t width, uin
  * `threads` = OpenMP threads for the raster (0 = default). */
 uint32_t oracle_render(const OracleScene *scene, const OracleCamera *camera, uint32_t width,
                        uint32_t height, int threads, float *rgba_out);
+
+/* Number of (pixel, splat) evaluations a front-to-back blend makes that pass
+ * the coverage test before each pixel's T < 1e-4 stop (the raster's
+ * algorithmic work); 0 on an invalid camera/config. */
+uint64_t oracle_blend_evals(const OracleScene *scene, const OracleCamera *camera, uint32_t width,
+                            uint32_t height);
 
 /* Stage timers of the last oracle_render in ms: preprocess, sort, project, raster. */
 void oracle_last_timings(double out_ms[4]);

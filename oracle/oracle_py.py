@@ -54,6 +54,7 @@ def lib() -> ctypes.CDLL:
         _lib.oracle_coverage_cut.argtypes = [ctypes.c_float]
         _lib.oracle_camera_valid.restype = ctypes.c_int
         _lib.oracle_sort_values_by_keys.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+        _lib.oracle_blend_evals.restype = ctypes.c_uint64
     return _lib
 
 
@@ -167,6 +168,12 @@ def render(scene: Scene, cam, width: int, height: int, threads: int = 0):
     if vis == 0xFFFFFFFF:
         raise ValueError("invalid camera or size")
     return out, int(vis)
+
+
+def blend_evals(scene: Scene, cam, width: int, height: int) -> int:
+    sc, c = scene.c(), make_camera(cam)
+    return int(lib().oracle_blend_evals(ctypes.byref(sc), ctypes.byref(c), ctypes.c_uint32(width),
+                                        ctypes.c_uint32(height)))
 
 
 def last_timings() -> list[float]:

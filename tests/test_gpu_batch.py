@@ -176,3 +176,16 @@ def test_batch_errors(abi):
     scene.close()
     b.wait()
     b.close()
+
+
+def test_batch_reduce_then_scan_sorts_equal_single(abi, monkeypatch):
+    # Both sorts of the batch in reduce-then-scan mode; each view is then
+    # compared with a context frame under the same mode.
+    monkeypatch.setenv("GSPLAT_SORT_RTS_MIN_TILES", "1")
+    scenes, views = mixed_batch(abi)
+    dev = [abi.Scene(s) for s in scenes]
+    b = abi.Batch()
+    say("[batch rts]")
+    b.render([(dev[s], cam, w, h) for (s, cam, w, h) in views])
+    for v, (s, cam, w, h) in enumerate(views):
+        assert_same(batch_view(b, v, scenes[s].count), single(abi, scenes[s], cam, w, h), f"view {v}")

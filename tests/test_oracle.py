@@ -314,3 +314,70 @@ def test_tile_counts_cover_every_covered_pixel(oracle):
         ys, xs = np.nonzero(covered)
         tiles_hit[ys // 16, xs // 16] = True
         assert np.all(counts[tiles_hit] == 1), i
+
+
+def test_blend_evals_single_splat_counts_covered_pixels(oracle):
+    # One splat: every covered pixel is evaluated once and none stops early,
+    # so E is the number of pixels the oracle's own raster covers.
+    s = oracle.Scene([[0.1, -0.05, 2.0]], [1.0], [[-2.4, -2.9, -2.6]], [[0.2, 0.1, 0.0, 1.0]], [[0.3, -0.2, 0.1]])
+    w, h = 120, 90
+    img, _ = oracle.render(s, default_cam(), w, h)
+    assert oracle.blend_evals(s, default_cam(), w, h) == int((img[..., 3] > 0).sum()) > 100
+
+
+def test_blend_evals_opaque_stack_stops_early(oracle):
+    # Eight near-opaque splats (alpha 0.9995) stacked in depth: near the centre
+    # (alpha*g ~ 0.95) T falls below 1e-4 after 3-4 of them, towards the rim
+    # all eight are evaluated; brute force in float64 per pixel.
+    k = 8
+    pos = [[0.0, 0.0, 2.0 + 0.1 * i] for i in range(k)]
+    logit = math.log(0.9995 / 0.0005)
+    s = oracle.Scene(pos, [logit] * k, [[-2.5] * 3] * k, [[0, 0, 0, 1]] * k, [[0.2, 0.2, 0.2]] * k)
+    w = 64
+    e = oracle.blend_evals(s, default_cam(), w, w)
+    proj = oracle.project(s, default_cam(), w, w)
+    yy, xx = np.mgrid[0:w, 0:w]
+    pnx = 2 * (xx + 0.5) / w - 1
+    pny = 1 - 2 * (yy + 0.5) / w
+    T = np.ones((w, w))
+    count = np.zeros((w, w), np.int64)
+    for i in range(k):  # nearest first: index 0 is nearest
+        p = proj[i]
+        inv = np.linalg.inv(np.array([[p[3], p[5]], [p[4], p[6]]], np.float64))
+        dx, dy = pnx - p[1], pny - p[2]
+        r2 = (inv[0, 0] * dx + inv[0, 1] * dy) ** 2 + (inv[1, 0] * dx + inv[1, 1] * dy) ** 2
+        m = (r2 <= oracle.coverage_cut(float(p[10]))) & (T >= 1e-4)
+        count += m
+        T = np.where(m, T * (1 - p[10] * np.exp(-4.5 * r2)), T)
+    assert abs(e - int(count.sum())) <= 4, (e, int(count.sum()))
+    assert count.max() == k and 2 <= count[w // 2, w // 2] < 5
+
+
+def test_blend_evals_brute_force_random(oracle):
+    rng = np.random.default_rng(19)
+    n = 60
+    pos = np.c_[rng.normal(0, 0.25, (n, 2)), rng.uniform(1.5, 3.0, n)]
+    s = oracle.Scene(pos, rng.normal(1.0, 1.5, n), rng.normal(-2.6, 0.4, (n, 3)),
+                     rng.standard_normal((n, 4)), rng.normal(0, 0.3, (n, 3)))
+    w, h = 72, 56
+    e = oracle.blend_evals(s, default_cam(), w, h)
+    proj = oracle.project(s, default_cam(), w, h)
+    keys, idx = oracle.preprocess(s, default_cam())
+    order = oracle.sort_values_by_keys(keys, idx)[::-1]
+    yy, xx = np.mgrid[0:h, 0:w]
+    pnx = 2 * (xx + 0.5) / w - 1
+    pny = 1 - 2 * (yy + 0.5) / h
+    T = np.ones((h, w))
+    total = 0
+    for i in order:
+        p = proj[i]
+        if p[0] == 0:
+            continue
+        inv = np.linalg.inv(np.array([[p[3], p[5]], [p[4], p[6]]], np.float64))
+        dx, dy = pnx - p[1], pny - p[2]
+        r2 = (inv[0, 0] * dx + inv[0, 1] * dy) ** 2 + (inv[1, 0] * dx + inv[1, 1] * dy) ** 2
+        m = (r2 <= min(oracle.coverage_cut(float(p[10])), 1.0)) & (T >= 1e-4)
+        total += int(m.sum())
+        T = np.where(m, T * (1 - p[10] * np.exp(-4.5 * r2)), T)
+    assert total > 1000
+    assert abs(e - total) <= max(3, total // 2000), (e, total)
