@@ -120,14 +120,26 @@ void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const u
 
 // Front-to-back blend of every view's 16x16 tiles: one workgroup per tile,
 // batches of 256 records staged in LDS.
-//   Linear   workgroup b renders global tile b
-//   XcdBand  workgroup b runs on XCD b mod 8 (round-robin dispatch); the
-//            tiles are banded so each XCD renders one contiguous range and
-//            neighbouring tiles share that XCD's L2
-enum class RasterVariant : uint32_t { Linear = 0, XcdBand = 1 };
-// With `evals`, the counting variant also adds to evals[0] the number of
-// (pixel, splat) evaluations that pass coverage before each pixel's early
-// stop (the blend's algorithmic work) and to evals[1] the coverage tests made.
+//   Linear       workgroup b renders global tile b; per-lane early exit
+//   XcdBand      workgroup b runs on XCD b mod 8 (round-robin dispatch); the
+//                tiles are banded so each XCD renders one contiguous range
+//   Predicated   Linear order; wave-uniform splat loop with a predicated
+//                blend (no divergent branch)
+//   UniformLoop  Linear order; wave-uniform splat loop, branched blend
+//   Unrolled     as UniformLoop, four splats per wave liveness check (default)
+//   Unrolled8    eight splats per check
+// All variants give bit-identical images. With `evals`, the counting variant
+// also adds to evals[0] the number of (pixel, splat) evaluations that pass
+// coverage before each pixel's early stop (the blend's algorithmic work) and
+// to evals[1] the coverage tests made.
+enum class RasterVariant : uint32_t {
+  Linear = 0,
+  XcdBand = 1,
+  Predicated = 2,
+  UniformLoop = 3,
+  Unrolled = 4,
+  Unrolled8 = 5
+};
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
                    uint32_t total_tiles, const float4 *rec, const uint2 *ranges, const uint32_t *list,
                    float4 *img32, uchar4 *img8, hipStream_t stream, unsigned long long *evals = nullptr);

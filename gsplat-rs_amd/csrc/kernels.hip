@@ -516,7 +516,19 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
 // COUNT: also adds to evals[0] the number of (pixel, splat) evaluations that
 // pass coverage before the pixel's early stop (the raster's algorithmic work)
 // and to evals[1] the number of coverage tests made.
-template <bool XCD, bool COUNT>
+// LOOP selects the inner loop over the staged splats; all three give every
+// pixel the same operations in the same order (bit-identical images):
+//   0  per-lane break at the early stop, blend under a divergent branch
+//   1  wave-uniform loop, blend predicated (a lane that misses or has
+//      stopped adds fmaf(0, c, C) = C and keeps T), leaves when no lane of
+//      the wave is active
+//   2  wave-uniform loop, blend under a divergent branch
+//   3  as 2, unrolled by four with one liveness check per four splats
+//   4  as 3, by eight
+// COUNT: also adds to evals[0] the number of (pixel, splat) evaluations that
+// pass coverage before the pixel's early stop (the raster's algorithmic work)
+// and to evals[1] the number of coverage tests made.
+template <bool XCD, bool COUNT, int LOOP>
 __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__ views,
                                                      const uint32_t *__restrict__ tile_start, uint32_t nviews,
                                                      uint32_t total_tiles, const float4 *__restrict__ rec,
@@ -556,8 +568,75 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
     }
     __syncthreads();
     const uint32_t cnt = min(kThreads, rg.y - start);
-    if (!done) {
+    if (LOOP == 0) {
+      if (!done) {
+        for (uint32_t j = 0; j < cnt; ++j) {
+          const float4 a = s0[j];
+          const float4 b = s1[j];
+          const float dx = pnx - a.x;
+          const float dy = pny - a.y;
+          const float lx = fmaf(a.w, dy, a.z * dx);
+          const float ly = fmaf(b.y, dy, b.x * dx);
+          const float r2 = fmaf(ly, ly, lx * lx);
+          if (COUNT) ++tested;
+          if (r2 <= b.z) {
+            if (COUNT) ++passed;
+            const float g = __expf(-4.5f * r2);
+            const float4 c = s2[j];
+            const float wgt = T * g;
+            cr = fmaf(wgt, c.x, cr);
+            cg = fmaf(wgt, c.y, cg);
+            cb = fmaf(wgt, c.z, cb);
+            T = T * (1.0f - b.w * g);
+            if (T < 1e-4f) {
+              done = true;
+              break;
+            }
+          }
+        }
+      }
+    } else if (LOOP >= 3) {
+      // As LOOP 2, U splats per liveness check (the tail of the batch is
+      // padded with splats no pixel can hit: cut = -1).
+      constexpr uint32_t U = LOOP == 3 ? 4 : 8;
+      bool active = !done;
+      const uint32_t cntu = (cnt + U - 1) & ~(U - 1);
+      if (t < cntu - cnt) s1[cnt + t] = make_float4(0.0f, 0.0f, -1.0f, 0.0f);
+      __syncthreads();
+      for (uint32_t j0 = 0; j0 < cntu; j0 += U) {
+        if (__ballot(active) == 0) break;  // wave-uniform
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+          const uint32_t j = j0 + u;
+          const float4 a = s0[j];
+          const float4 b = s1[j];
+          const float dx = pnx - a.x;
+          const float dy = pny - a.y;
+          const float lx = fmaf(a.w, dy, a.z * dx);
+          const float ly = fmaf(b.y, dy, b.x * dx);
+          const float r2 = fmaf(ly, ly, lx * lx);
+          const bool hit = active && r2 <= b.z;
+          if (COUNT) {
+            tested += (active && j < cnt) ? 1u : 0u;
+            passed += hit ? 1u : 0u;
+          }
+          if (hit) {
+            const float g = __expf(-4.5f * r2);
+            const float4 c = s2[j];
+            const float wgt = T * g;
+            cr = fmaf(wgt, c.x, cr);
+            cg = fmaf(wgt, c.y, cg);
+            cb = fmaf(wgt, c.z, cb);
+            T = T * (1.0f - b.w * g);
+            active = !(T < 1e-4f);
+          }
+        }
+      }
+      done = !active;
+    } else {
+      bool active = !done;
       for (uint32_t j = 0; j < cnt; ++j) {
+        if (__ballot(active) == 0) break;  // wave-uniform
         const float4 a = s0[j];
         const float4 b = s1[j];
         const float dx = pnx - a.x;
@@ -565,9 +644,22 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
         const float lx = fmaf(a.w, dy, a.z * dx);
         const float ly = fmaf(b.y, dy, b.x * dx);
         const float r2 = fmaf(ly, ly, lx * lx);
-        if (COUNT) ++tested;
-        if (r2 <= b.z) {
-          if (COUNT) ++passed;
+        const bool hit = active && r2 <= b.z;
+        if (COUNT) {
+          tested += active ? 1u : 0u;
+          passed += hit ? 1u : 0u;
+        }
+        if (LOOP == 1) {
+          const float g = __expf(-4.5f * r2);
+          const float4 c = s2[j];
+          const float wgt = hit ? T * g : 0.0f;
+          cr = fmaf(wgt, c.x, cr);
+          cg = fmaf(wgt, c.y, cg);
+          cb = fmaf(wgt, c.z, cb);
+          const float tn = T * (1.0f - b.w * g);
+          T = hit ? tn : T;
+          active = active && !(T < 1e-4f);
+        } else if (hit) {
           const float g = __expf(-4.5f * r2);
           const float4 c = s2[j];
           const float wgt = T * g;
@@ -575,12 +667,10 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
           cg = fmaf(wgt, c.y, cg);
           cb = fmaf(wgt, c.z, cb);
           T = T * (1.0f - b.w * g);
-          if (T < 1e-4f) {
-            done = true;
-            break;
-          }
+          active = !(T < 1e-4f);
         }
       }
+      done = !active;
     }
   }
   if (inside) {
@@ -721,24 +811,48 @@ void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const u
   GS_HIP(hipGetLastError());
 }
 
+template <bool XCD, int LOOP>
+void raster_launch(dim3 grid, hipStream_t st, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
+                   uint32_t total_tiles, const float4 *rec, const uint2 *ranges, const uint32_t *list, float4 *img32,
+                   uchar4 *img8, unsigned long long *evals) {
+  if (evals)
+    k_raster<XCD, true, LOOP><<<grid, kThreads, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list,
+                                                         img32, img8, evals);
+  else
+    k_raster<XCD, false, LOOP><<<grid, kThreads, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list,
+                                                          img32, img8, evals);
+}
+
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
                    uint32_t total_tiles, const float4 *rec, const uint2 *ranges, const uint32_t *list,
                    float4 *img32, uchar4 *img8, hipStream_t st, unsigned long long *evals) {
   if (total_tiles == 0 || nviews == 0) return;
   const bool xcd = variant == RasterVariant::XcdBand;
-  const dim3 grid(xcd ? 8 * div_up(total_tiles, 8) : total_tiles), block(kThreads);
-  if (xcd && evals)
-    k_raster<true, true><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list, img32,
-                                                 img8, evals);
-  else if (xcd)
-    k_raster<true, false><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list, img32,
-                                                  img8, evals);
-  else if (evals)
-    k_raster<false, true><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list, img32,
-                                                  img8, evals);
-  else
-    k_raster<false, false><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list,
-                                                   img32, img8, evals);
+  const dim3 grid(xcd ? 8 * div_up(total_tiles, 8) : total_tiles);
+  switch (variant) {
+    case RasterVariant::XcdBand:
+      raster_launch<true, 0>(grid, st, views, tile_start, nviews, total_tiles, rec, ranges, list, img32, img8, evals);
+      break;
+    case RasterVariant::Predicated:
+      raster_launch<false, 1>(grid, st, views, tile_start, nviews, total_tiles, rec, ranges, list, img32, img8,
+                              evals);
+      break;
+    case RasterVariant::UniformLoop:
+      raster_launch<false, 2>(grid, st, views, tile_start, nviews, total_tiles, rec, ranges, list, img32, img8,
+                              evals);
+      break;
+    case RasterVariant::Unrolled:
+      raster_launch<false, 3>(grid, st, views, tile_start, nviews, total_tiles, rec, ranges, list, img32, img8,
+                              evals);
+      break;
+    case RasterVariant::Unrolled8:
+      raster_launch<false, 4>(grid, st, views, tile_start, nviews, total_tiles, rec, ranges, list, img32, img8,
+                              evals);
+      break;
+    default:
+      raster_launch<false, 0>(grid, st, views, tile_start, nviews, total_tiles, rec, ranges, list, img32, img8,
+                              evals);
+  }
   GS_HIP(hipGetLastError());
 }
 

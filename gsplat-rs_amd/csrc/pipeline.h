@@ -191,7 +191,7 @@ class Pipeline {
   uint64_t pairs_ = 0;
   uint64_t overflow_rerenders_ = 0;
   FrameTimes times_{};
-  RasterVariant raster_variant_ = RasterVariant::Linear;
+  RasterVariant raster_variant_ = RasterVariant::Unrolled;
   bool count_evals_ = false;
 };
 

@@ -11,9 +11,11 @@ namespace gs {
 constexpr uint32_t kMaxDimension = 16384;
 
 RasterVariant raster_variant_from_env() {
-  if (const char *v = std::getenv("GSPLAT_RASTER_VARIANT"))
-    return std::atoi(v) == 1 ? RasterVariant::XcdBand : RasterVariant::Linear;
-  return RasterVariant::Linear;
+  if (const char *v = std::getenv("GSPLAT_RASTER_VARIANT")) {
+    const int k = std::atoi(v);
+    if (k >= 0 && k <= 5) return (RasterVariant)k;
+  }
+  return RasterVariant::Unrolled;
 }
 
 Renderer::Renderer(uint32_t width, uint32_t height) : width_(width), height_(height) {

@@ -228,10 +228,11 @@ def test_gpu_auto_camera_frames_scene(gpu, oracle):
     assert ctx.stats().visible_count == len(idx) > 0
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 def test_gpu_raster_variants_match_oracle(gpu, oracle, monkeypatch, variant):
-    # Both raster tile orders (linear, banded by XCD) blend the same lists to
-    # the same tolerance.
+    # Every raster variant (tile order linear or banded by XCD; per-lane
+    # exit, predicated or uniform-loop blend) blends the same lists to the
+    # same tolerance.
     monkeypatch.setenv("GSPLAT_RASTER_VARIANT", str(variant))
     s = synth.tiny_scene(6000, 77, sh_degree=3, spread=0.6, log_scale=-3.2)
     cam = gpu.look_at((0.2, -0.3, -2.2), (0.0, 0.0, 0.0))
@@ -293,3 +294,19 @@ def test_gpu_blend_evals_match_oracle(gpu, oracle, monkeypatch):
         ref = oracle.blend_evals(oracle_scene(oracle, scene), cam_dict(cam), view.width, view.height)
         assert ref > 0 and abs(int(e) - ref) <= max(10, ref // 1000), (int(e), ref)
         ctx.close()
+
+
+def test_gpu_raster_variants_bit_identical(gpu, monkeypatch):
+    # Tile order and loop shape change no pixel's arithmetic.
+    scene, view = synth.plain_1080p(n=80_000, seed_offset=5)
+    cam = gpu.look_at(view.position, view.target)
+    imgs = []
+    for variant in range(6):
+        monkeypatch.setenv("GSPLAT_RASTER_VARIANT", str(variant))
+        say(f"[raster bit-identical] variant {variant}")
+        ctx = render(gpu, scene, cam, view.width, view.height)
+        imgs.append((ctx.rgba32f().view(np.uint32), ctx.rgba8()))
+        ctx.close()
+    for variant in range(1, 6):
+        assert np.array_equal(imgs[variant][0], imgs[0][0]), f"variant {variant} fp32"
+        assert np.array_equal(imgs[variant][1], imgs[0][1]), f"variant {variant} rgba8"
