@@ -176,7 +176,7 @@ STAGE_KERNELS = {
     "k_os_pass": ["k_os_clear", "k_os_hist", "k_os_pass"],
     "bin(pair sort+emit)": ["k_tile_counts", "k_scan_blocks", "k_scan_partials", "k_scan_add", "k_pairs_finalize",
                             "k_tile_emit", "k_rts_up", "k_rts_down", "k_tile_ranges"],
-    "k_raster": ["k_raster<false, false>"],
+    "k_raster": ["k_raster<false, 1>"],
 }
 
 
@@ -228,7 +228,7 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, copy_fetch_dir, copy_wri
              "    rocprofv3 --kernel-trace --stats --output-format csv -d DIR -o plain -- \\",
              "        python3 bench.py --plain-only --steps 30 --warmup 5 --no-cpu-baseline", "",
              "Summarised with `python3 bench.py --summarize-profiles ...`. Calls cover warmup, timed, latency",
-             "and one counting frame (k_raster<false, true>).", "",
+             "and one counting frame (k_raster<true, 1>).", "",
              "| kernel | calls | avg us | total % |", "|---|---:|---:|---:|"]
     for r in sorted(stats, key=lambda r: -float(r["Percentage"])):
         lines.append(f"| {_kernel_short(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "

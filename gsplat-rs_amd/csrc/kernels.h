@@ -22,13 +22,17 @@ struct FrameParams {
   float lim_x, lim_y;
   float blur_x, blur_y;
   uint32_t width, height;
-  uint32_t tiles_x, tiles_y;
+  uint32_t tiles_x, tiles_y;    // 16x16 raster tiles
+  uint32_t ctiles_x, ctiles_y;  // binning tiles of (16 << bin shift) pixels
 };
 
+// ctiles_x/ctiles_y are left for the caller to set for its bin shift
+// (set_bin_tiles).
 FrameParams make_frame_params(const float position[3], const float rotation_xyzw[4], float fov,
                               float near_plane, float far_plane, uint32_t width, uint32_t height);
 
 constexpr uint32_t kTile = 16;            // raster tile edge in pixels
+constexpr uint32_t kMaxBinShift = 3;      // binning tiles of 16, 32, 64 or 128 pixels
 constexpr uint32_t kThreads = 256;        // threads per workgroup (4 wave64)
 constexpr uint32_t kScanTile = kThreads * 16;
 constexpr uint32_t kMaxGrid = 4096;       // bound of the grids that loop over device-sized work
@@ -48,9 +52,13 @@ struct ViewDev {
   const float *soa;  // scene SoA (coefficient-major, see common.h)
   uint32_t n, sh_degree;
   uint32_t seg_off, seg_len;
-  uint32_t tile_off, tiles;
+  uint32_t tile_off, tiles;    // global 16x16 raster tiles
+  uint32_t ctile_off, ctiles;  // global binning tiles
   uint64_t img_off;
 };
+
+// Binning tile grid of a view for bin shift `shift`.
+void set_bin_tiles(FrameParams &fp, uint32_t shift);
 
 // Projected splat, 3 x float4 per slot (48 B):
 //   [cx, cy, i00, i01] [i10, i11, cut, alpha] [r*a, g*a, b*a, 0]
@@ -87,62 +95,68 @@ size_t scan_partial_entries(size_t n);
 void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s, hipStream_t stream,
                     const uint32_t *n_dev = nullptr);
 
+// Two-level binning. Splats are binned into binning tiles of (16 << shift)
+// pixels; each 16x16 raster tile then keeps, in order, the splats of its
+// binning tile whose pixel bounds overlap it. shift 0 bins straight into the
+// raster tiles. Either way a raster tile blends exactly the splats whose
+// bounds overlap it, in depth order.
+
 // After the depth sort (each view's segment back to front, culled last):
-// counts_rev[total-1-g] = number of tiles of the splat in sorted slot g (0
-// for culled and padding slots), so an exclusive scan of counts_rev gives
-// every slot its first pair index with each view's pairs front to back; and
-// visible[v] = V of view v (written by the last visible slot; zero it first).
+// counts_rev[total-1-g] = number of binning tiles of the splat in sorted slot
+// g (0 for culled and padding slots), so an exclusive scan of counts_rev
+// gives every slot its first pair index with each view's pairs front to
+// back; visible[v] = V of view v (written by the last visible slot; zero it
+// first); block_pairs[b] = the 16x16-tile pair count of workgroup b's
+// slots (total_slots / 256 entries).
 void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
-                        const uint32_t *skeys, const uint32_t *svals, const uint2 *rect, uint32_t *counts_rev,
-                        uint32_t *visible, hipStream_t stream);
+                        uint32_t shift, const uint32_t *skeys, const uint32_t *svals, const uint2 *rect,
+                        uint32_t *counts_rev, uint32_t *visible, uint32_t *block_pairs, hipStream_t stream);
 
 // Frame summary written on the device after the tile-count scan:
-// info[1..2] = 64-bit pair total M, info[3] = 1 when M exceeds the pair
-// capacity; *m_out = min(M, capacity) drives the later kernels.
+// info[1..2] = 64-bit binning pair total M, info[3] = 1 when M exceeds the
+// pair capacity; *m_out = min(M, capacity) drives the later kernels;
+// *raster_pairs = the sum of block_pairs.
 void launch_pairs_finalize(const uint64_t *total, uint32_t capacity, uint32_t *m_out, uint32_t *info,
+                           const uint32_t *block_pairs, uint32_t total_slots, unsigned long long *raster_pairs,
                            hipStream_t stream);
 
-// Writes (view << 24 | tile of the view, record slot) pairs at the scanned
-// offsets: each view's pairs form one contiguous run, front to back. Pairs at
-// or beyond `capacity` are dropped (the frame is then flagged and rendered
-// again).
+// Writes (view << 24 | binning tile of the view, record slot) pairs at the
+// scanned offsets: each view's pairs form one contiguous run, front to back.
+// Pairs at or beyond `capacity` are dropped (the frame is then flagged and
+// rendered again).
 void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
-                      const uint32_t *svals, const uint2 *rect, const uint32_t *counts_rev,
+                      uint32_t shift, const uint32_t *svals, const uint2 *rect, const uint32_t *counts_rev,
                       const uint32_t *offsets_rev, uint32_t capacity, uint32_t *pair_keys, uint32_t *pair_vals,
                       hipStream_t stream);
 
 // After a stable sort of the pairs on the tile bits of their keys:
-// ranges[t] = [first, last+1) of global tile t (= tile_off of the key's view
-// + the key's tile); the pair count is read from *m_dev (grid sized for
-// `capacity`).
+// ranges[t] = [first, last+1) of global binning tile t (= ctile_off of the
+// key's view + the key's tile); the pair count is read from *m_dev (grid
+// sized for `capacity`).
 void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const uint32_t *m_dev, uint32_t capacity,
-                        uint32_t tiles, uint2 *ranges, hipStream_t stream);
+                        uint32_t ctiles, uint2 *ranges, hipStream_t stream);
 
-// Front-to-back blend of every view's 16x16 tiles: one workgroup per tile,
-// batches of 256 records staged in LDS.
-//   Linear       workgroup b renders global tile b; per-lane early exit
-//   XcdBand      workgroup b runs on XCD b mod 8 (round-robin dispatch); the
-//                tiles are banded so each XCD renders one contiguous range
-//   Predicated   Linear order; wave-uniform splat loop with a predicated
-//                blend (no divergent branch)
-//   UniformLoop  Linear order; wave-uniform splat loop, branched blend
-//   Unrolled     as UniformLoop, four splats per wave liveness check (default)
-//   Unrolled8    eight splats per check
-// All variants give bit-identical images. With `evals`, the counting variant
-// also adds to evals[0] the number of (pixel, splat) evaluations that pass
+// 16x16-tile counts of view v (row-major, tiles_x * tiles_y, zeroed here):
+// the number of visible splats whose pixel bounds overlap each raster tile.
+void launch_raster_tile_counts(const ViewDev *views, uint32_t v, const ViewDev &host_view, const uint32_t *skeys,
+                               const uint32_t *svals, const uint2 *rect, uint32_t *counts, hipStream_t stream);
+
+// Front-to-back blend of every view's 16x16 tiles: one workgroup per tile.
+// The workgroup walks its binning tile's list 256 entries at a time, keeps
+// (in order) the splats whose bounds overlap its tile, stages their records
+// in LDS and blends them once 256 are staged or the list ends.
+//   Linear    per-lane exit at the early stop, divergent blend branch
+//   Unrolled  wave-uniform splat loop, four splats per wave liveness check
+//             (default)
+// Both give bit-identical images. With `evals`, the counting variant also
+// adds to evals[0] the number of (pixel, splat) evaluations that pass
 // coverage before each pixel's early stop (the blend's algorithmic work) and
 // to evals[1] the coverage tests made.
-enum class RasterVariant : uint32_t {
-  Linear = 0,
-  XcdBand = 1,
-  Predicated = 2,
-  UniformLoop = 3,
-  Unrolled = 4,
-  Unrolled8 = 5
-};
+enum class RasterVariant : uint32_t { Linear = 0, Unrolled = 1 };
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
-                   uint32_t total_tiles, const float4 *rec, const uint2 *ranges, const uint32_t *list,
-                   float4 *img32, uchar4 *img8, hipStream_t stream, unsigned long long *evals = nullptr);
+                   uint32_t total_tiles, uint32_t shift, const float4 *rec, const uint2 *rect, const uint2 *ranges,
+                   const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t stream,
+                   unsigned long long *evals = nullptr);
 
 // Device fill of n u32 words (graph-safe replacement of hipMemsetAsync).
 void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t stream);

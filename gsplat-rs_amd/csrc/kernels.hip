@@ -390,38 +390,66 @@ __global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t *__restrict__ ou
 
 // ------------------------------------------------------------- tile binning
 
-// Number of 16x16 tiles of a pixel-bounds rectangle (x0|y0<<16, x1|y1<<16).
-__device__ __forceinline__ uint32_t rect_tiles(uint2 r) {
+// Number of tiles of (16 << shift) pixels a pixel-bounds rectangle
+// (x0|y0<<16, x1|y1<<16) touches.
+__device__ __forceinline__ uint32_t rect_tiles(uint2 r, uint32_t shift) {
   if (r.x == 0xFFFFFFFFu) return 0u;
-  const uint32_t x0 = (r.x & 0xFFFFu) / kTile, y0 = (r.x >> 16) / kTile;
-  const uint32_t x1 = (r.y & 0xFFFFu) / kTile, y1 = (r.y >> 16) / kTile;
+  const uint32_t sh = 4 + shift;
+  const uint32_t x0 = (r.x & 0xFFFFu) >> sh, y0 = (r.x >> 16) >> sh;
+  const uint32_t x1 = (r.y & 0xFFFFu) >> sh, y1 = (r.y >> 16) >> sh;
   return (x1 - x0 + 1) * (y1 - y0 + 1);
 }
 
 __global__ __launch_bounds__(kThreads) void k_tile_counts(const ViewDev *__restrict__ views,
                                                           const uint32_t *__restrict__ seg_start, uint32_t nviews,
-                                                          uint32_t total, const uint32_t *__restrict__ skeys,
+                                                          uint32_t total, uint32_t shift,
+                                                          const uint32_t *__restrict__ skeys,
                                                           const uint32_t *__restrict__ svals,
                                                           const uint2 *__restrict__ rect,
                                                           uint32_t *__restrict__ counts_rev,
-                                                          uint32_t *__restrict__ visible) {
+                                                          uint32_t *__restrict__ visible,
+                                                          uint32_t *__restrict__ block_pairs) {
+  __shared__ uint32_t wsum[kWaves];
   const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
   const uint32_t v = find_view(seg_start, nviews, blockIdx.x * kThreads);
   const ViewDev &vw = views[v];
   const uint32_t key = skeys[g];
-  uint32_t c = 0;
+  uint32_t c = 0, fine = 0;
   const uint32_t local = svals[g] - (v << kLocalBits);
   if (key != 0u && local < vw.n) {
-    c = rect_tiles(rect[vw.seg_off + local]);
+    const uint2 r = rect[vw.seg_off + local];
+    c = rect_tiles(r, shift);
+    fine = rect_tiles(r, 0);
     const uint32_t j = g - vw.seg_off;
     if (j + 1 == vw.seg_len || skeys[g + 1] == 0u) visible[v] = j + 1;
   }
   counts_rev[total - 1 - g] = c;
+  // This workgroup's raster-tile pair count, summed by k_pairs_finalize (a
+  // single address updated by every wave would serialise).
+#pragma unroll
+  for (uint32_t off = kWave / 2; off > 0; off >>= 1) fine += __shfl_down(fine, off, kWave);
+  if (__lane_id() == 0) wsum[threadIdx.x / kWave] = fine;
+  __syncthreads();
+  if (threadIdx.x == 0) block_pairs[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-__global__ void k_pairs_finalize(const uint64_t *__restrict__ total, uint32_t capacity, uint32_t *__restrict__ m_out,
-                                 uint32_t *__restrict__ info) {
+// One workgroup: the binning pair total and overflow flag, and the sum of
+// the per-workgroup raster-tile pair counts into *raster_pairs.
+__global__ __launch_bounds__(kThreads) void k_pairs_finalize(const uint64_t *__restrict__ total, uint32_t capacity,
+                                                             uint32_t *__restrict__ m_out,
+                                                             uint32_t *__restrict__ info,
+                                                             const uint32_t *__restrict__ block_pairs,
+                                                             uint32_t blocks,
+                                                             unsigned long long *__restrict__ raster_pairs) {
+  __shared__ unsigned long long wsum[kWaves];
+  unsigned long long sum = 0;
+  for (uint32_t b = threadIdx.x; b < blocks; b += kThreads) sum += block_pairs[b];
+#pragma unroll
+  for (uint32_t off = kWave / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, kWave);
+  if (__lane_id() == 0) wsum[threadIdx.x / kWave] = sum;
+  __syncthreads();
   if (threadIdx.x != 0) return;
+  *raster_pairs = wsum[0] + wsum[1] + wsum[2] + wsum[3];
   const uint64_t m = *total;
   *m_out = m > capacity ? capacity : (uint32_t)m;
   info[1] = (uint32_t)m;
@@ -436,7 +464,8 @@ __global__ void k_pairs_finalize(const uint64_t *__restrict__ total, uint32_t ca
 // every thread finding its pair's slot by binary search.
 __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restrict__ views,
                                                         const uint32_t *__restrict__ seg_start, uint32_t nviews,
-                                                        uint32_t total, const uint32_t *__restrict__ svals,
+                                                        uint32_t total, uint32_t shift,
+                                                        const uint32_t *__restrict__ svals,
                                                         const uint2 *__restrict__ rect,
                                                         const uint32_t *__restrict__ counts_rev,
                                                         const uint32_t *__restrict__ offsets_rev, uint32_t capacity,
@@ -457,15 +486,16 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
   if (cnt) {
     const uint32_t slot = vw.seg_off + (svals[g0 + kThreads - 1 - t] - (v << kLocalBits));
     const uint2 b = rect[slot];
-    const uint32_t x0 = (b.x & 0xFFFFu) / kTile, x1 = (b.y & 0xFFFFu) / kTile;
+    const uint32_t sh = 4 + shift;
+    const uint32_t x0 = (b.x & 0xFFFFu) >> sh, x1 = (b.y & 0xFFFFu) >> sh;
     s_slot[t] = slot;
     s_x0[t] = x0;
-    s_y0[t] = (b.x >> 16) / kTile;
+    s_y0[t] = (b.x >> 16) >> sh;
     s_w[t] = x1 - x0 + 1;
   }
   __syncthreads();
   const uint32_t p_begin = s_first[0], p_end = min(s_first[kThreads], capacity);
-  const uint32_t tiles_x = vw.fp.tiles_x;
+  const uint32_t tiles_x = vw.fp.ctiles_x;
   for (uint32_t p = p_begin + t; p < p_end; p += kThreads) {
     // Largest i with s_first[i] <= p: the entry whose range holds p (entries
     // without pairs share the next entry's first index and are passed over).
@@ -494,10 +524,31 @@ __global__ __launch_bounds__(kThreads) void k_tile_ranges(const ViewDev *__restr
   const uint32_t m = *m_dev;
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < m; i += gridDim.x * kThreads) {
     const uint32_t k = keys[i];
-    const uint32_t t = views[k >> kPairViewShift].tile_off + (k & kPairTileMask);
+    const uint32_t t = views[k >> kPairViewShift].ctile_off + (k & kPairTileMask);
     if (i == 0 || keys[i - 1] != k) ranges[t].x = i;
     if (i == m - 1 || keys[i + 1] != k) ranges[t].y = i + 1;
   }
+}
+
+// One thread per slot of view v: every visible splat adds one to each 16x16
+// tile its pixel bounds overlap.
+__global__ __launch_bounds__(kThreads) void k_raster_tile_counts(const ViewDev *__restrict__ views, uint32_t v,
+                                                                 const uint32_t *__restrict__ skeys,
+                                                                 const uint32_t *__restrict__ svals,
+                                                                 const uint2 *__restrict__ rect,
+                                                                 uint32_t *__restrict__ counts) {
+  const ViewDev &vw = views[v];
+  const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= vw.seg_len) return;
+  const uint32_t g = vw.seg_off + j;
+  const uint32_t local = svals[g] - (v << kLocalBits);
+  if (skeys[g] == 0u || local >= vw.n) return;
+  const uint2 r = rect[vw.seg_off + local];
+  if (r.x == 0xFFFFFFFFu) return;
+  const uint32_t x0 = (r.x & 0xFFFFu) / kTile, y0 = (r.x >> 16) / kTile;
+  const uint32_t x1 = (r.y & 0xFFFFu) / kTile, y1 = (r.y >> 16) / kTile;
+  for (uint32_t ty = y0; ty <= y1; ++ty)
+    for (uint32_t tx = x0; tx <= x1; ++tx) atomicAdd(&counts[ty * vw.fp.tiles_x + tx], 1u);
 }
 
 __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
@@ -516,127 +567,53 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
 // COUNT: also adds to evals[0] the number of (pixel, splat) evaluations that
 // pass coverage before the pixel's early stop (the raster's algorithmic work)
 // and to evals[1] the number of coverage tests made.
-// LOOP selects the inner loop over the staged splats; all three give every
-// pixel the same operations in the same order (bit-identical images):
-//   0  per-lane break at the early stop, blend under a divergent branch
-//   1  wave-uniform loop, blend predicated (a lane that misses or has
-//      stopped adds fmaf(0, c, C) = C and keeps T), leaves when no lane of
-//      the wave is active
-//   2  wave-uniform loop, blend under a divergent branch
-//   3  as 2, unrolled by four with one liveness check per four splats
-//   4  as 3, by eight
-// COUNT: also adds to evals[0] the number of (pixel, splat) evaluations that
-// pass coverage before the pixel's early stop (the raster's algorithmic work)
-// and to evals[1] the number of coverage tests made.
-template <bool XCD, bool COUNT, int LOOP>
-__global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__ views,
-                                                     const uint32_t *__restrict__ tile_start, uint32_t nviews,
-                                                     uint32_t total_tiles, const float4 *__restrict__ rec,
-                                                     const uint2 *__restrict__ ranges,
-                                                     const uint32_t *__restrict__ list, float4 *__restrict__ img32,
-                                                     uchar4 *__restrict__ img8,
-                                                     unsigned long long *__restrict__ evals) {
-  __shared__ float4 s0[kThreads], s1[kThreads], s2[kThreads];
-  uint32_t tile = blockIdx.x;
-  if (XCD) {
-    const uint32_t band = (total_tiles + 7) / 8;
-    tile = (blockIdx.x % 8) * band + blockIdx.x / 8;
-  }
-  if (tile >= total_tiles) return;
-  const uint32_t v = find_view(tile_start, nviews, tile);
-  const ViewDev &vw = views[v];
-  const uint32_t width = vw.fp.width, height = vw.fp.height;
-  const uint32_t local = tile - vw.tile_off;
-  const uint32_t tx = local % vw.fp.tiles_x, ty = local / vw.fp.tiles_x;
-  const uint32_t t = threadIdx.x;
-  const uint32_t x = tx * kTile + (t % kTile), y = ty * kTile + (t / kTile);
-  const bool inside = x < width && y < height;
-  const float pnx = (2.0f * ((float)x + 0.5f)) / (float)width - 1.0f;
-  const float pny = 1.0f - (2.0f * ((float)y + 0.5f)) / (float)height;
-  const uint2 rg = ranges[tile];
-  float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
-  bool done = !inside;
-  uint32_t passed = 0, tested = 0;
-  for (uint32_t start = rg.x; start < rg.y; start += kThreads) {
-    if (__syncthreads_count(!done) == 0) break;
-    const uint32_t k = start + t;
-    if (k < rg.y) {
-      const size_t s = list[k];
-      s0[t] = rec[3 * s + 0];
-      s1[t] = rec[3 * s + 1];
-      s2[t] = rec[3 * s + 2];
+// Blends staged splats [0, n) of the LDS stage into the thread's pixel.
+// LOOP 0: per-lane exit at the early stop, divergent blend branch.
+// LOOP 1: wave-uniform loop, four splats per wave liveness check; the stage
+// is padded to a multiple of four with splats no pixel can hit (cut = -1).
+// Both make the same operations in the same order for every pixel.
+template <int LOOP, bool COUNT>
+__device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const float4 *s2, uint32_t n, float pnx,
+                                            float pny, float &T, float &cr, float &cg, float &cb, bool &done,
+                                            uint32_t &passed, uint32_t &tested) {
+  if (LOOP == 0) {
+    if (done) return;
+    for (uint32_t j = 0; j < n; ++j) {
+      const float4 a = s0[j];
+      const float4 b = s1[j];
+      const float dx = pnx - a.x;
+      const float dy = pny - a.y;
+      const float lx = fmaf(a.w, dy, a.z * dx);
+      const float ly = fmaf(b.y, dy, b.x * dx);
+      const float r2 = fmaf(ly, ly, lx * lx);
+      if (COUNT) ++tested;
+      if (r2 <= b.z) {
+        if (COUNT) ++passed;
+        const float g = __expf(-4.5f * r2);
+        const float4 c = s2[j];
+        const float wgt = T * g;
+        cr = fmaf(wgt, c.x, cr);
+        cg = fmaf(wgt, c.y, cg);
+        cb = fmaf(wgt, c.z, cb);
+        T = T * (1.0f - b.w * g);
+        if (T < 1e-4f) {
+          done = true;
+          return;
+        }
+      }
     }
+  } else {
+    constexpr uint32_t U = 4;
+    const uint32_t nu = (n + U - 1) & ~(U - 1);
+    const uint32_t t = threadIdx.x;
+    if (t < nu - n) s1[n + t] = make_float4(0.0f, 0.0f, -1.0f, 0.0f);
     __syncthreads();
-    const uint32_t cnt = min(kThreads, rg.y - start);
-    if (LOOP == 0) {
-      if (!done) {
-        for (uint32_t j = 0; j < cnt; ++j) {
-          const float4 a = s0[j];
-          const float4 b = s1[j];
-          const float dx = pnx - a.x;
-          const float dy = pny - a.y;
-          const float lx = fmaf(a.w, dy, a.z * dx);
-          const float ly = fmaf(b.y, dy, b.x * dx);
-          const float r2 = fmaf(ly, ly, lx * lx);
-          if (COUNT) ++tested;
-          if (r2 <= b.z) {
-            if (COUNT) ++passed;
-            const float g = __expf(-4.5f * r2);
-            const float4 c = s2[j];
-            const float wgt = T * g;
-            cr = fmaf(wgt, c.x, cr);
-            cg = fmaf(wgt, c.y, cg);
-            cb = fmaf(wgt, c.z, cb);
-            T = T * (1.0f - b.w * g);
-            if (T < 1e-4f) {
-              done = true;
-              break;
-            }
-          }
-        }
-      }
-    } else if (LOOP >= 3) {
-      // As LOOP 2, U splats per liveness check (the tail of the batch is
-      // padded with splats no pixel can hit: cut = -1).
-      constexpr uint32_t U = LOOP == 3 ? 4 : 8;
-      bool active = !done;
-      const uint32_t cntu = (cnt + U - 1) & ~(U - 1);
-      if (t < cntu - cnt) s1[cnt + t] = make_float4(0.0f, 0.0f, -1.0f, 0.0f);
-      __syncthreads();
-      for (uint32_t j0 = 0; j0 < cntu; j0 += U) {
-        if (__ballot(active) == 0) break;  // wave-uniform
+    bool active = !done;
+    for (uint32_t j0 = 0; j0 < nu; j0 += U) {
+      if (__ballot(active) == 0) break;  // wave-uniform
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-          const uint32_t j = j0 + u;
-          const float4 a = s0[j];
-          const float4 b = s1[j];
-          const float dx = pnx - a.x;
-          const float dy = pny - a.y;
-          const float lx = fmaf(a.w, dy, a.z * dx);
-          const float ly = fmaf(b.y, dy, b.x * dx);
-          const float r2 = fmaf(ly, ly, lx * lx);
-          const bool hit = active && r2 <= b.z;
-          if (COUNT) {
-            tested += (active && j < cnt) ? 1u : 0u;
-            passed += hit ? 1u : 0u;
-          }
-          if (hit) {
-            const float g = __expf(-4.5f * r2);
-            const float4 c = s2[j];
-            const float wgt = T * g;
-            cr = fmaf(wgt, c.x, cr);
-            cg = fmaf(wgt, c.y, cg);
-            cb = fmaf(wgt, c.z, cb);
-            T = T * (1.0f - b.w * g);
-            active = !(T < 1e-4f);
-          }
-        }
-      }
-      done = !active;
-    } else {
-      bool active = !done;
-      for (uint32_t j = 0; j < cnt; ++j) {
-        if (__ballot(active) == 0) break;  // wave-uniform
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t j = j0 + u;
         const float4 a = s0[j];
         const float4 b = s1[j];
         const float dx = pnx - a.x;
@@ -646,20 +623,10 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
         const float r2 = fmaf(ly, ly, lx * lx);
         const bool hit = active && r2 <= b.z;
         if (COUNT) {
-          tested += active ? 1u : 0u;
+          tested += (active && j < n) ? 1u : 0u;
           passed += hit ? 1u : 0u;
         }
-        if (LOOP == 1) {
-          const float g = __expf(-4.5f * r2);
-          const float4 c = s2[j];
-          const float wgt = hit ? T * g : 0.0f;
-          cr = fmaf(wgt, c.x, cr);
-          cg = fmaf(wgt, c.y, cg);
-          cb = fmaf(wgt, c.z, cb);
-          const float tn = T * (1.0f - b.w * g);
-          T = hit ? tn : T;
-          active = active && !(T < 1e-4f);
-        } else if (hit) {
+        if (hit) {
           const float g = __expf(-4.5f * r2);
           const float4 c = s2[j];
           const float wgt = T * g;
@@ -670,7 +637,94 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
           active = !(T < 1e-4f);
         }
       }
-      done = !active;
+    }
+    done = !active;
+  }
+}
+
+// One workgroup per 16x16 tile, one thread per pixel. The workgroup walks
+// its binning tile's list (depth order) 256 entries at a time; the entries
+// whose pixel bounds overlap the tile are compacted in order into the LDS
+// stage (records gathered from global memory), and the stage is blended
+// front to back once it holds 256 or the list ends: C += T*g*rgb_a,
+// T *= 1 - a*g, stopping when T < 1e-4 (equivalent to the reference's
+// back-to-front premultiplied "over", draw_pass.rs:75, up to that bound).
+// Coverage is r^2 <= cut (the fragment test of wgsl:358-367 precomputed per
+// splat); g = exp(-4.5 r^2) (wgsl:363).
+// COUNT: also adds to evals[0] the number of (pixel, splat) evaluations that
+// pass coverage before the pixel's early stop (the raster's algorithmic work)
+// and to evals[1] the number of coverage tests made.
+template <bool COUNT, int LOOP>
+__global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__ views,
+                                                     const uint32_t *__restrict__ tile_start, uint32_t nviews,
+                                                     uint32_t total_tiles, uint32_t shift,
+                                                     const float4 *__restrict__ rec, const uint2 *__restrict__ rect,
+                                                     const uint2 *__restrict__ ranges,
+                                                     const uint32_t *__restrict__ list, float4 *__restrict__ img32,
+                                                     uchar4 *__restrict__ img8,
+                                                     unsigned long long *__restrict__ evals) {
+  constexpr uint32_t kStage = 2 * kThreads;  // a stage below 256 takes one more batch of up to 256
+  __shared__ float4 s0[kStage], s1[kStage], s2[kStage];
+  __shared__ uint32_t wkeep[kWaves];
+  const uint32_t tile = blockIdx.x;
+  if (tile >= total_tiles) return;
+  const uint32_t v = find_view(tile_start, nviews, tile);
+  const ViewDev &vw = views[v];
+  const uint32_t width = vw.fp.width, height = vw.fp.height;
+  const uint32_t local = tile - vw.tile_off;
+  const uint32_t tx = local % vw.fp.tiles_x, ty = local / vw.fp.tiles_x;
+  const uint32_t t = threadIdx.x, w = t / kWave, lane = __lane_id();
+  const uint32_t x = tx * kTile + (t % kTile), y = ty * kTile + (t / kTile);
+  const bool inside = x < width && y < height;
+  const float pnx = (2.0f * ((float)x + 0.5f)) / (float)width - 1.0f;
+  const float pny = 1.0f - (2.0f * ((float)y + 0.5f)) / (float)height;
+  const uint2 rg = ranges[vw.ctile_off + (ty >> shift) * vw.fp.ctiles_x + (tx >> shift)];
+  const uint32_t tx0 = tx * kTile, tx1 = tx0 + kTile - 1, ty0 = ty * kTile, ty1 = ty0 + kTile - 1;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+  float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+  bool done = !inside;
+  uint32_t passed = 0, tested = 0;
+  uint32_t staged = 0;
+  for (uint32_t start = rg.x;;) {
+    if (__syncthreads_count(!done) == 0) break;
+    const bool more = start < rg.y;
+    if (more) {
+      const uint32_t k = start + t;
+      uint32_t s = 0;
+      bool keep = false;
+      if (k < rg.y) {
+        s = list[k];
+        if (shift == 0) {
+          keep = true;
+        } else {
+          const uint2 r = rect[s];
+          keep = (r.x & 0xFFFFu) <= tx1 && (r.y & 0xFFFFu) >= tx0 && (r.x >> 16) <= ty1 && (r.y >> 16) >= ty0;
+        }
+      }
+      const uint64_t bal = __ballot(keep);
+      if (lane == 0) wkeep[w] = (uint32_t)__popcll(bal);
+      __syncthreads();
+      uint32_t off = staged, kept = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < kWaves; ++q) {
+        off += q < w ? wkeep[q] : 0u;
+        kept += wkeep[q];
+      }
+      if (keep) {
+        const uint32_t pos = off + (uint32_t)__popcll(bal & lt);
+        s0[pos] = rec[3 * (size_t)s + 0];
+        s1[pos] = rec[3 * (size_t)s + 1];
+        s2[pos] = rec[3 * (size_t)s + 2];
+      }
+      staged += kept;
+      start += kThreads;
+      __syncthreads();
+    }
+    if (staged >= kThreads || (!more || start >= rg.y)) {
+      if (staged > 0)
+        blend_stage<LOOP, COUNT>(s0, s1, s2, staged, pnx, pny, T, cr, cg, cb, done, passed, tested);
+      staged = 0;
+      if (start >= rg.y) break;
     }
   }
   if (inside) {
@@ -681,14 +735,14 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
     img8[o] = make_uchar4(q(cr), q(cg), q(cb), q(A));
   }
   if (COUNT) {
-    uint32_t w = passed, u = tested;
+    uint32_t wp = passed, u = tested;
 #pragma unroll
     for (uint32_t off = kWave / 2; off > 0; off >>= 1) {
-      w += __shfl_down(w, off, kWave);
+      wp += __shfl_down(wp, off, kWave);
       u += __shfl_down(u, off, kWave);
     }
-    if (__lane_id() == 0 && w) atomicAdd(evals, (unsigned long long)w);
-    if (__lane_id() == 0 && u) atomicAdd(evals + 1, (unsigned long long)u);
+    if (lane == 0 && wp) atomicAdd(evals, (unsigned long long)wp);
+    if (lane == 0 && u) atomicAdd(evals + 1, (unsigned long long)u);
   }
 }
 
@@ -720,7 +774,14 @@ FrameParams make_frame_params(const float position[3], const float rotation_xyzw
   p.height = height;
   p.tiles_x = (width + kTile - 1) / kTile;
   p.tiles_y = (height + kTile - 1) / kTile;
+  p.ctiles_x = p.tiles_x;
+  p.ctiles_y = p.tiles_y;
   return p;
+}
+
+void set_bin_tiles(FrameParams &fp, uint32_t shift) {
+  fp.ctiles_x = (fp.tiles_x + (1u << shift) - 1) >> shift;
+  fp.ctiles_y = (fp.tiles_y + (1u << shift) - 1) >> shift;
 }
 
 void launch_project(uint32_t deg, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,
@@ -771,27 +832,39 @@ void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s,
 }
 
 void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
-                        const uint32_t *skeys, const uint32_t *svals, const uint2 *rect, uint32_t *counts_rev,
-                        uint32_t *visible, hipStream_t st) {
+                        uint32_t shift, const uint32_t *skeys, const uint32_t *svals, const uint2 *rect,
+                        uint32_t *counts_rev, uint32_t *visible, uint32_t *block_pairs, hipStream_t st) {
   if (total_slots == 0) return;
-  k_tile_counts<<<total_slots / kThreads, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, skeys, svals,
-                                                               rect, counts_rev, visible);
+  k_tile_counts<<<total_slots / kThreads, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, shift, skeys,
+                                                               svals, rect, counts_rev, visible, block_pairs);
   GS_HIP(hipGetLastError());
 }
 
 void launch_pairs_finalize(const uint64_t *total, uint32_t capacity, uint32_t *m_out, uint32_t *info,
+                           const uint32_t *block_pairs, uint32_t total_slots, unsigned long long *raster_pairs,
                            hipStream_t st) {
-  k_pairs_finalize<<<1, 64, 0, st>>>(total, capacity, m_out, info);
+  k_pairs_finalize<<<1, kThreads, 0, st>>>(total, capacity, m_out, info, block_pairs, total_slots / kThreads,
+                                           raster_pairs);
   GS_HIP(hipGetLastError());
 }
 
 void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
-                      const uint32_t *svals, const uint2 *rect, const uint32_t *counts_rev,
+                      uint32_t shift, const uint32_t *svals, const uint2 *rect, const uint32_t *counts_rev,
                       const uint32_t *offsets_rev, uint32_t capacity, uint32_t *pair_keys, uint32_t *pair_vals,
                       hipStream_t st) {
   if (total_slots == 0) return;
-  k_tile_emit<<<total_slots / kThreads, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, svals, rect,
-                                                             counts_rev, offsets_rev, capacity, pair_keys, pair_vals);
+  k_tile_emit<<<total_slots / kThreads, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, shift, svals,
+                                                             rect, counts_rev, offsets_rev, capacity, pair_keys,
+                                                             pair_vals);
+  GS_HIP(hipGetLastError());
+}
+
+void launch_raster_tile_counts(const ViewDev *views, uint32_t v, const ViewDev &host_view, const uint32_t *skeys,
+                               const uint32_t *svals, const uint2 *rect, uint32_t *counts, hipStream_t st) {
+  launch_fill_u32(counts, 0u, host_view.tiles, st);
+  if (host_view.seg_len == 0) return;
+  k_raster_tile_counts<<<div_up(host_view.seg_len, kThreads), kThreads, 0, st>>>(views, v, skeys, svals, rect,
+                                                                                  counts);
   GS_HIP(hipGetLastError());
 }
 
@@ -811,48 +884,25 @@ void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const u
   GS_HIP(hipGetLastError());
 }
 
-template <bool XCD, int LOOP>
-void raster_launch(dim3 grid, hipStream_t st, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
-                   uint32_t total_tiles, const float4 *rec, const uint2 *ranges, const uint32_t *list, float4 *img32,
-                   uchar4 *img8, unsigned long long *evals) {
-  if (evals)
-    k_raster<XCD, true, LOOP><<<grid, kThreads, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list,
-                                                         img32, img8, evals);
-  else
-    k_raster<XCD, false, LOOP><<<grid, kThreads, 0, st>>>(views, tile_start, nviews, total_tiles, rec, ranges, list,
-                                                          img32, img8, evals);
-}
-
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
-                   uint32_t total_tiles, const float4 *rec, const uint2 *ranges, const uint32_t *list,
-                   float4 *img32, uchar4 *img8, hipStream_t st, unsigned long long *evals) {
+                   uint32_t total_tiles, uint32_t shift, const float4 *rec, const uint2 *rect, const uint2 *ranges,
+                   const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t st, unsigned long long *evals) {
   if (total_tiles == 0 || nviews == 0) return;
-  const bool xcd = variant == RasterVariant::XcdBand;
-  const dim3 grid(xcd ? 8 * div_up(total_tiles, 8) : total_tiles);
-  switch (variant) {
-    case RasterVariant::XcdBand:
-      raster_launch<true, 0>(grid, st, views, tile_start, nviews, total_tiles, rec, ranges, list, img32, img8, evals);
-      break;
-    case RasterVariant::Predicated:
-      raster_launch<false, 1>(grid, st, views, tile_start, nviews, total_tiles, rec, ranges, list, img32, img8,
-                              evals);
-      break;
-    case RasterVariant::UniformLoop:
-      raster_launch<false, 2>(grid, st, views, tile_start, nviews, total_tiles, rec, ranges, list, img32, img8,
-                              evals);
-      break;
-    case RasterVariant::Unrolled:
-      raster_launch<false, 3>(grid, st, views, tile_start, nviews, total_tiles, rec, ranges, list, img32, img8,
-                              evals);
-      break;
-    case RasterVariant::Unrolled8:
-      raster_launch<false, 4>(grid, st, views, tile_start, nviews, total_tiles, rec, ranges, list, img32, img8,
-                              evals);
-      break;
-    default:
-      raster_launch<false, 0>(grid, st, views, tile_start, nviews, total_tiles, rec, ranges, list, img32, img8,
-                              evals);
-  }
+  if (shift > kMaxBinShift) throw Error(Code::InvalidArgument, "bin shift above 3");
+  const dim3 grid(total_tiles), block(kThreads);
+  const bool unrolled = variant != RasterVariant::Linear;
+  if (unrolled && evals)
+    k_raster<true, 1><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, rec, rect, ranges, list,
+                                              img32, img8, evals);
+  else if (unrolled)
+    k_raster<false, 1><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, rec, rect, ranges,
+                                               list, img32, img8, evals);
+  else if (evals)
+    k_raster<true, 0><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, rec, rect, ranges, list,
+                                              img32, img8, evals);
+  else
+    k_raster<false, 0><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, rec, rect, ranges,
+                                               list, img32, img8, evals);
   GS_HIP(hipGetLastError());
 }
 

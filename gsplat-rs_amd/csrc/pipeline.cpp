@@ -12,8 +12,14 @@ namespace gs {
 namespace {
 
 constexpr size_t kTableBytes = 2 * kMaxViews * sizeof(uint32_t) + kMaxViews * sizeof(ViewDev);
-constexpr size_t kEvalsWord = 4 + kMaxViews;  // u64 evaluations passing, u64 tests made (counting mode)
-constexpr size_t kSummaryWords = kEvalsWord + 4;
+// Frame summary words (device and pinned host copies):
+//   [1..2] binning pair count M (u64), [3] overflow flag,
+//   [4 .. 4+kMaxViews) visible count per view,
+//   kEvalsWord: u64 evaluations passing, u64 coverage tests (counting mode),
+//   kRasterPairsWord: u64 (16x16 tile, splat) pairs.
+constexpr size_t kEvalsWord = 4 + kMaxViews;
+constexpr size_t kRasterPairsWord = kEvalsWord + 4;
+constexpr size_t kSummaryWords = kRasterPairsWord + 2;
 
 uint32_t bits_for(uint64_t count) {
   uint32_t b = 1;
@@ -57,6 +63,7 @@ uint64_t frame_working_bytes(uint64_t slots, uint64_t pixels, uint64_t tiles, ui
 Pipeline::Pipeline() {
   if (!device_present()) throw Error(Code::Unsupported, "gpu rasterizer unavailable");
   if (const char *e = std::getenv("GSPLAT_COUNT_EVALS")) count_evals_ = std::atoi(e) != 0;
+  if (const char *e = std::getenv("GSPLAT_BIN_SHIFT")) set_bin_shift((uint32_t)std::atoi(e));
   GS_HIP(hipSetDevice(0));
   GS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   for (auto &e : ev_) GS_HIP(hipEventCreate(&e));
@@ -102,7 +109,7 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
   }
   const bool multi = views.size() > 1;
   std::vector<ViewDev> table(views.size());
-  uint64_t slots = 0, tiles = 0, pixels = 0, splats = 0;
+  uint64_t slots = 0, tiles = 0, ctiles = 0, pixels = 0, splats = 0;
   for (size_t v = 0; v < views.size(); ++v) {
     const ViewSpec &s = views[v];
     if (!s.scene) throw Error(Code::SceneNotLoaded, "scene not loaded");
@@ -112,16 +119,20 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
                                          std::to_string(kLocalMask) + " splats");
     ViewDev &d = table[v];
     d.fp = s.fp;
+    set_bin_tiles(d.fp, bin_shift_);
     d.soa = s.scene->soa.get();
     d.n = n;
     d.sh_degree = s.scene->sh_degree;
     d.seg_off = (uint32_t)slots;
     d.seg_len = (uint32_t)(((uint64_t)n + kSegmentAlign - 1) / kSegmentAlign * kSegmentAlign);
     d.tile_off = (uint32_t)tiles;
-    d.tiles = s.fp.tiles_x * s.fp.tiles_y;
+    d.tiles = d.fp.tiles_x * d.fp.tiles_y;
+    d.ctile_off = (uint32_t)ctiles;
+    d.ctiles = d.fp.ctiles_x * d.fp.ctiles_y;
     d.img_off = pixels;
     slots += d.seg_len;
     tiles += d.tiles;
+    ctiles += d.ctiles;
     pixels += (uint64_t)s.fp.width * s.fp.height;
     splats += n;
     if (slots > (1ull << 30)) throw Error(Code::Unsupported, "batch exceeds 2^30 splat slots");
@@ -133,6 +144,7 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
   host_views_ = std::move(table);
   total_slots_ = (uint32_t)slots;
   total_tiles_ = (uint32_t)tiles;
+  total_ctiles_ = (uint32_t)ctiles;
   total_pixels_ = pixels;
   total_splats_ = splats;
   if (pair_cap_ == 0) pair_cap_ = initial_pair_capacity(splats);
@@ -142,9 +154,10 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
 void Pipeline::launch_frame() {
   const uint32_t nviews = (uint32_t)views_.size();
   const uint32_t slots = total_slots_, tiles = total_tiles_;
-  uint32_t view_tiles = 1;
-  for (const ViewDev &d : host_views_) view_tiles = std::max(view_tiles, d.tiles);
-  const uint32_t tile_bits = bits_for(view_tiles);  // the pair sort orders tiles within a view
+  const uint32_t ctiles = total_ctiles_;
+  uint32_t view_ctiles = 1;
+  for (const ViewDev &d : host_views_) view_ctiles = std::max(view_ctiles, d.ctiles);
+  const uint32_t tile_bits = bits_for(view_ctiles);  // the pair sort orders binning tiles within a view
   const size_t m = std::max<uint32_t>(slots, 1);
   keys_.ensure(m);
   vals_.ensure(m);
@@ -155,9 +168,10 @@ void Pipeline::launch_frame() {
   counts_.ensure(m);
   offsets_.ensure(m);
   partials_.ensure(scan_partial_entries(m));
+  block_pairs_.ensure(m / kThreads + 1);
   img32_.ensure(total_pixels_);
   img8_.ensure(total_pixels_);
-  ranges_.ensure(std::max<uint32_t>(tiles, 1));
+  ranges_.ensure(std::max<uint32_t>(ctiles, 1));
   pair_keys_.ensure(pair_cap_);
   pair_vals_.ensure(pair_cap_);
   pair_keys_alt_.ensure(pair_cap_);
@@ -198,20 +212,24 @@ void Pipeline::launch_frame() {
   uint32_t *k = keys_.get(), *v = vals_.get(), *ka = keys_alt_.get(), *va = vals_alt_.get();
   onesweep_sort(&k, &v, &ka, &va, slots, nullptr, 32, /*descending=*/true, sort_scratch_.get(),
                 sort_scratch_.capacity(), st, nviews > 1 ? 3 : -1);
+  sorted_keys_ = k;
   sorted_vals_ = v;
   GS_HIP(hipEventRecord(ev_[2], st));
 
-  launch_tile_counts(views_d, seg_start_d, nviews, slots, k, v, rect_.get(), counts_.get(), visible, st);
+  unsigned long long *raster_pairs = reinterpret_cast<unsigned long long *>(info + kRasterPairsWord);
+  launch_tile_counts(views_d, seg_start_d, nviews, slots, bin_shift_, k, v, rect_.get(), counts_.get(), visible,
+                     block_pairs_.get(), st);
   exclusive_scan(counts_.get(), offsets_.get(), slots, sc, st);
-  launch_pairs_finalize(total_.get(), pair_cap_, m_dev_.get(), info, st);
-  launch_tile_emit(views_d, seg_start_d, nviews, slots, v, rect_.get(), counts_.get(), offsets_.get(), pair_cap_,
+  launch_pairs_finalize(total_.get(), pair_cap_, m_dev_.get(), info, block_pairs_.get(), slots, raster_pairs, st);
+  launch_tile_emit(views_d, seg_start_d, nviews, slots, bin_shift_, v, rect_.get(), counts_.get(), offsets_.get(),
+                   pair_cap_,
                    pair_keys_.get(), pair_vals_.get(), st);
   uint32_t *pk = pair_keys_.get(), *pv = pair_vals_.get(), *pka = pair_keys_alt_.get(), *pva = pair_vals_alt_.get();
   // The last frame's pair count (+25%) sizes the pair sort's grids.
   const uint32_t hint = pairs_ ? (uint32_t)std::min<uint64_t>(pairs_ + pairs_ / 4, pair_cap_) : pair_cap_;
   onesweep_sort(&pk, &pv, &pka, &pva, pair_cap_, m_dev_.get(), tile_bits, /*descending=*/false,
                 sort_scratch_.get(), sort_scratch_.capacity(), st, -1, hint);
-  launch_tile_ranges(views_d, pk, m_dev_.get(), pair_cap_, tiles, ranges_.get(), st);
+  launch_tile_ranges(views_d, pk, m_dev_.get(), pair_cap_, ctiles, ranges_.get(), st);
   GS_HIP(hipEventRecord(ev_[3], st));
 
   unsigned long long *evals = nullptr;
@@ -219,8 +237,8 @@ void Pipeline::launch_frame() {
     evals = reinterpret_cast<unsigned long long *>(info + kEvalsWord);
     launch_fill_u32(info + kEvalsWord, 0u, 4, st);
   }
-  launch_raster(raster_variant_, views_d, tile_start_d, nviews, tiles, rec_.get(), ranges_.get(), pv, img32_.get(),
-                img8_.get(), st, evals);
+  launch_raster(raster_variant_, views_d, tile_start_d, nviews, tiles, bin_shift_, rec_.get(), rect_.get(),
+                ranges_.get(), pv, img32_.get(), img8_.get(), st, evals);
   GS_HIP(hipEventRecord(ev_[4], st));
   for (size_t i = 0; i < host_rgba8_.size(); ++i) {
     if (!host_rgba8_[i]) continue;
@@ -229,6 +247,8 @@ void Pipeline::launch_frame() {
                           hipMemcpyDeviceToHost, st));
   }
   GS_HIP(hipMemcpyAsync(summary_pinned_, info, (4 + nviews) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  GS_HIP(hipMemcpyAsync(summary_pinned_ + kRasterPairsWord, info + kRasterPairsWord, 2 * sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, st));
   if (count_evals_)
     GS_HIP(hipMemcpyAsync(summary_pinned_ + kEvalsWord, info + kEvalsWord, 4 * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, st));
@@ -241,6 +261,7 @@ bool Pipeline::absorb_summary() {
   const uint32_t nviews = (uint32_t)views_.size();
   visible_.assign(summary_pinned_ + 4, summary_pinned_ + 4 + nviews);
   pairs_ = ((uint64_t)summary_pinned_[2] << 32) | summary_pinned_[1];
+  std::memcpy(&raster_pairs_, summary_pinned_ + kRasterPairsWord, sizeof(raster_pairs_));
   const bool overflow = summary_pinned_[3] != 0;
   if (overflow) {
     const uint64_t want = pairs_ + pairs_ / 4 + 4096;
@@ -292,7 +313,17 @@ uint32_t Pipeline::visible(uint32_t v) {
 
 uint64_t Pipeline::pairs() {
   wait();
+  return raster_pairs_;
+}
+
+uint64_t Pipeline::bin_pairs() {
+  wait();
   return pairs_;
+}
+
+void Pipeline::set_bin_shift(uint32_t shift) {
+  if (shift > kMaxBinShift) throw Error(Code::InvalidArgument, "bin shift above " + std::to_string(kMaxBinShift));
+  bin_shift_ = shift;
 }
 
 FrameTimes Pipeline::times() {
@@ -341,13 +372,13 @@ void Pipeline::read_tile_counts(uint32_t v, uint32_t *out, uint64_t cap, uint32_
   wait();
   const ViewDev &d = host_views_[v];
   if (cap < d.tiles) throw Error(Code::InvalidArgument, "output buffer smaller than the tile count");
-  std::vector<uint2> r(d.tiles);
-  if (d.tiles) {
-    GS_HIP(hipMemcpyAsync(r.data(), ranges_.get() + d.tile_off, d.tiles * sizeof(uint2), hipMemcpyDeviceToHost,
-                          stream_));
-    GS_HIP(hipStreamSynchronize(stream_));
-  }
-  for (uint32_t t = 0; t < d.tiles; ++t) out[t] = r[t].y - r[t].x;
+  const ViewDev *views_d = reinterpret_cast<const ViewDev *>(table_.get() + 2 * kMaxViews * sizeof(uint32_t));
+  DevBuf<uint32_t> counts;
+  counts.ensure(std::max<uint32_t>(d.tiles, 1));
+  launch_raster_tile_counts(views_d, v, d, sorted_keys_, sorted_vals_, rect_.get(), counts.get(), stream_);
+  if (d.tiles)
+    GS_HIP(hipMemcpyAsync(out, counts.get(), d.tiles * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+  GS_HIP(hipStreamSynchronize(stream_));
   *tx = d.fp.tiles_x;
   *ty = d.fp.tiles_y;
 }

@@ -129,7 +129,8 @@ class Pipeline {
   uint32_t view_count() const { return (uint32_t)views_.size(); }
   const ViewDev &view(uint32_t v) const;
   uint32_t visible(uint32_t v);  // waits
-  uint64_t pairs();              // waits
+  uint64_t pairs();              // (16x16 tile, splat) pairs; waits
+  uint64_t bin_pairs();          // (binning tile, splat) pairs emitted and sorted; waits
   uint64_t total_splats() const { return total_splats_; }
   uint32_t total_tiles() const { return total_tiles_; }
   FrameTimes times();  // waits
@@ -150,6 +151,10 @@ class Pipeline {
   // (pixel, splat) evaluations that pass coverage before each pixel's early
   // stop; the count is reported with the frame times.
   void set_count_evals(bool on) { count_evals_ = on; }
+  // Binning tiles of (16 << shift) pixels, shift <= 3 (also GSPLAT_BIN_SHIFT);
+  // images and tile counts do not depend on it.
+  void set_bin_shift(uint32_t shift);
+  uint32_t bin_shift() const { return bin_shift_; }
   hipStream_t stream() const { return stream_; }
 
  private:
@@ -174,25 +179,28 @@ class Pipeline {
   DevBuf<uint2> ranges_;
   DevBuf<float4> img32_;
   DevBuf<uchar4> img8_;
-  DevBuf<uint32_t> sort_scratch_, partials_, m_dev_;
+  DevBuf<uint32_t> sort_scratch_, partials_, m_dev_, block_pairs_;
   DevBuf<uint64_t> total_;
   DevBuf<ViewDev> export_view_;
 
   std::vector<ViewSpec> views_;
   std::vector<uint8_t *> host_rgba8_;
   std::vector<ViewDev> host_views_;
-  uint32_t total_slots_ = 0, total_tiles_ = 0;
+  uint32_t total_slots_ = 0, total_tiles_ = 0, total_ctiles_ = 0;
   uint64_t total_pixels_ = 0, total_splats_ = 0;
   uint32_t pair_cap_ = 0;
+  uint32_t *sorted_keys_ = nullptr;  // depth-sorted keys of the last frame
   uint32_t *sorted_vals_ = nullptr;  // depth-sorted values of the last frame
   bool pending_ = false;
   bool summary_valid_ = false;
   std::vector<uint32_t> visible_;
-  uint64_t pairs_ = 0;
+  uint64_t pairs_ = 0;         // binning pairs of the last frame
+  uint64_t raster_pairs_ = 0;  // (16x16 tile, splat) pairs of the last frame
   uint64_t overflow_rerenders_ = 0;
   FrameTimes times_{};
   RasterVariant raster_variant_ = RasterVariant::Unrolled;
   bool count_evals_ = false;
+  uint32_t bin_shift_ = 2;
 };
 
 }  // namespace gs

@@ -11,10 +11,8 @@ namespace gs {
 constexpr uint32_t kMaxDimension = 16384;
 
 RasterVariant raster_variant_from_env() {
-  if (const char *v = std::getenv("GSPLAT_RASTER_VARIANT")) {
-    const int k = std::atoi(v);
-    if (k >= 0 && k <= 5) return (RasterVariant)k;
-  }
+  if (const char *v = std::getenv("GSPLAT_RASTER_VARIANT"))
+    return std::atoi(v) == 0 ? RasterVariant::Linear : RasterVariant::Unrolled;
   return RasterVariant::Unrolled;
 }
 

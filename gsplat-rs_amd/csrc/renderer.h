@@ -49,9 +49,8 @@ class Renderer {
   float host_ms_ = 0.0f;
 };
 
-// Raster variant chosen by GSPLAT_RASTER_VARIANT (0 linear, 1 XCD bands,
-// 2 predicated blend, 3 uniform loop, 4 uniform loop unrolled by four -- the
-// default, 5 unrolled by eight).
+// Raster variant chosen by GSPLAT_RASTER_VARIANT: 0 per-lane exit, else the
+// default wave-uniform loop unrolled by four.
 RasterVariant raster_variant_from_env();
 
 }  // namespace gs

@@ -122,10 +122,11 @@ def test_batch_against_oracle(abi, oracle):
         assert np.abs(b.rgba32f(v) - ref32).max() <= 2e-3
 
 
-def test_batch_pair_overflow_rerender(abi, oracle):
-    # Large splats: ~5.6M tile pairs over the two views, beyond the initial
-    # capacity of 2^20 pairs for this splat count, so the frame overflows and
-    # is rendered again with grown buffers.
+def test_batch_pair_overflow_rerender(abi, oracle, monkeypatch):
+    # Large splats binned straight into 16x16 tiles: ~5.6M pairs over the two
+    # views, beyond the initial capacity of 2^20 pairs for this splat count,
+    # so the frame overflows and is rendered again with grown buffers.
+    monkeypatch.setenv("GSPLAT_BIN_SHIFT", "0")
     scene = synth.tiny_scene(6000, 5151, sh_degree=0, spread=0.4, log_scale=-2.2)
     dev = abi.Scene(scene)
     cam = abi.look_at((0.0, 0.0, -2.5))

@@ -228,19 +228,20 @@ def test_gpu_auto_camera_frames_scene(gpu, oracle):
     assert ctx.stats().visible_count == len(idx) > 0
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
-def test_gpu_raster_variants_match_oracle(gpu, oracle, monkeypatch, variant):
-    # Every raster variant (tile order linear or banded by XCD; per-lane
-    # exit, predicated or uniform-loop blend) blends the same lists to the
-    # same tolerance.
+@pytest.mark.parametrize("variant,shift", [(0, 0), (1, 0), (0, 2), (1, 2), (1, 3)])
+def test_gpu_raster_variants_match_oracle(gpu, oracle, monkeypatch, variant, shift):
+    # Both blend loops, with splats binned straight into 16x16 tiles (shift
+    # 0) or into larger binning tiles culled per raster tile, blend the same
+    # lists to the same tolerance.
     monkeypatch.setenv("GSPLAT_RASTER_VARIANT", str(variant))
+    monkeypatch.setenv("GSPLAT_BIN_SHIFT", str(shift))
     s = synth.tiny_scene(6000, 77, sh_degree=3, spread=0.6, log_scale=-3.2)
     cam = gpu.look_at((0.2, -0.3, -2.2), (0.0, 0.0, 0.0))
     w, h = 333, 217
-    say(f"[raster variant {variant}]")
+    say(f"[raster variant {variant} shift {shift}]")
     ctx = render(gpu, s, cam, w, h)
     ref, _ = oracle.render(oracle_scene(oracle, s), cam_dict(cam), w, h)
-    check_image(ctx.rgba32f(), ctx.rgba8(), ref, f"variant-{variant}")
+    check_image(ctx.rgba32f(), ctx.rgba8(), ref, f"variant-{variant}-shift-{shift}")
     ctx.close()
 
 
@@ -297,16 +298,23 @@ def test_gpu_blend_evals_match_oracle(gpu, oracle, monkeypatch):
 
 
 def test_gpu_raster_variants_bit_identical(gpu, monkeypatch):
-    # Tile order and loop shape change no pixel's arithmetic.
+    # The blend loop and the binning tile size change no pixel's arithmetic,
+    # no tile count and no pair total.
     scene, view = synth.plain_1080p(n=80_000, seed_offset=5)
     cam = gpu.look_at(view.position, view.target)
-    imgs = []
-    for variant in range(6):
-        monkeypatch.setenv("GSPLAT_RASTER_VARIANT", str(variant))
-        say(f"[raster bit-identical] variant {variant}")
-        ctx = render(gpu, scene, cam, view.width, view.height)
-        imgs.append((ctx.rgba32f().view(np.uint32), ctx.rgba8()))
-        ctx.close()
-    for variant in range(1, 6):
-        assert np.array_equal(imgs[variant][0], imgs[0][0]), f"variant {variant} fp32"
-        assert np.array_equal(imgs[variant][1], imgs[0][1]), f"variant {variant} rgba8"
+    out = {}
+    for variant in (0, 1):
+        for shift in (0, 1, 2, 3):
+            monkeypatch.setenv("GSPLAT_RASTER_VARIANT", str(variant))
+            monkeypatch.setenv("GSPLAT_BIN_SHIFT", str(shift))
+            say(f"[raster bit-identical] variant {variant} shift {shift}")
+            ctx = render(gpu, scene, cam, view.width, view.height)
+            out[(variant, shift)] = (ctx.rgba32f().view(np.uint32), ctx.rgba8(), ctx.tile_counts(),
+                                     ctx.stage_times().tile_pairs)
+            ctx.close()
+    base = out[(0, 0)]
+    for key, got in out.items():
+        assert np.array_equal(got[0], base[0]), f"{key} fp32"
+        assert np.array_equal(got[1], base[1]), f"{key} rgba8"
+        assert np.array_equal(got[2], base[2]), f"{key} tile counts"
+        assert got[3] == base[3] == base[2].sum(), f"{key} pairs"
