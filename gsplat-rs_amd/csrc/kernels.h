@@ -35,6 +35,7 @@ constexpr uint32_t kTile = 16;            // raster tile edge in pixels
 constexpr uint32_t kMaxBinShift = 3;      // binning tiles of 16, 32, 64 or 128 pixels
 constexpr uint32_t kThreads = 256;        // threads per workgroup (4 wave64)
 constexpr uint32_t kScanTile = kThreads * 16;
+constexpr uint32_t kCountBlock = kThreads * 4;  // slots per tile-count workgroup
 constexpr uint32_t kMaxGrid = 4096;       // bound of the grids that loop over device-sized work
 constexpr uint32_t kSegmentAlign = 4096;  // view slot segments align to the sort tile
 constexpr uint32_t kLocalBits = 24;       // splat index bits in a sort value
@@ -66,15 +67,33 @@ void set_bin_tiles(FrameParams &fp, uint32_t shift);
 // bound on r^2 and (r,g,b)*a the premultiplied colour.
 constexpr uint32_t kRecordFloat4 = 3;
 
+// Up to three device word ranges a kernel zeroes with its whole grid before
+// its own work (ranges the frame needs zero later on, folded into an earlier
+// launch).
+struct Clears {
+  static constexpr int kRanges = 3;
+  uint32_t *ptr[kRanges] = {nullptr, nullptr, nullptr};
+  size_t words[kRanges] = {0, 0, 0};
+  void add(uint32_t *p, size_t n) {
+    for (int r = 0; r < kRanges; ++r)
+      if (words[r] == 0) {
+        ptr[r] = p;
+        words[r] = n;
+        return;
+      }
+  }
+};
+
 // Depth key, cull, projection, SH colour and pixel bounds for every slot of
 // every view whose scene has SH degree `degree` (other views are skipped):
 // keys[g] = f32 bits of depth when near <= depth <= far else 0;
 // vals[g] = view << 24 | splat index (index 0xFFFFFF for padding slots);
 // rect[g] = pixel bounds x0|y0<<16, x1|y1<<16, or x = ~0u when the splat
 // covers no pixel.
+// The grid first zeroes `clears`.
 void launch_project(uint32_t degree, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,
                     uint32_t total_slots, uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect,
-                    hipStream_t stream);
+                    const Clears &clears, hipStream_t stream);
 
 // The same projection of one view's n splats exporting, per splat, the depth
 // key (keys_out, optional) and the reference quantities of project_splat +
@@ -101,24 +120,27 @@ void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s,
 // raster tiles. Either way a raster tile blends exactly the splats whose
 // bounds overlap it, in depth order.
 
-// After the depth sort (each view's segment back to front, culled last):
-// counts_rev[total-1-g] = number of binning tiles of the splat in sorted slot
-// g (0 for culled and padding slots), so an exclusive scan of counts_rev
-// gives every slot its first pair index with each view's pairs front to
-// back; visible[v] = V of view v (written by the last visible slot; zero it
-// first); block_pairs[b] = the 16x16-tile pair count of workgroup b's
-// slots (total_slots / 256 entries).
+// After the depth sort (each view's segment back to front, culled last),
+// over total_slots / kCountBlock blocks of reversed slot positions
+// (total_slots a multiple of kCountBlock):
+//   counts_rev[total-1-g] = number of binning tiles of the splat in sorted
+//   slot g (0 for culled and padding slots), offsets_rev its exclusive scan
+//   within the block, block_sums[b] the block's total and block_pairs[b] its
+//   16x16-tile pair count;
+//   visible[v] = V of view v (written by the last visible slot; zero first).
+// The grid first zeroes `clears`.
 void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
                         uint32_t shift, const uint32_t *skeys, const uint32_t *svals, const uint2 *rect,
-                        uint32_t *counts_rev, uint32_t *visible, uint32_t *block_pairs, hipStream_t stream);
+                        uint32_t *counts_rev, uint32_t *offsets_rev, uint32_t *visible, uint32_t *block_sums,
+                        uint32_t *block_pairs, const Clears &clears, hipStream_t stream);
 
-// Frame summary written on the device after the tile-count scan:
-// info[1..2] = 64-bit binning pair total M, info[3] = 1 when M exceeds the
-// pair capacity; *m_out = min(M, capacity) drives the later kernels;
-// *raster_pairs = the sum of block_pairs.
-void launch_pairs_finalize(const uint64_t *total, uint32_t capacity, uint32_t *m_out, uint32_t *info,
-                           const uint32_t *block_pairs, uint32_t total_slots, unsigned long long *raster_pairs,
-                           hipStream_t stream);
+// One workgroup: block_sums becomes its exclusive scan (each block's first
+// pair index, so a slot's first pair is block_sums[r / kCountBlock] +
+// offsets_rev[r]); info[1..2] = the 64-bit binning pair total M, info[3] = 1
+// when M exceeds `capacity`, *m_out = min(M, capacity); *raster_pairs = the
+// sum of block_pairs.
+void launch_block_scan(uint32_t *block_sums, uint32_t blocks, uint32_t capacity, uint32_t *m_out, uint32_t *info,
+                       const uint32_t *block_pairs, unsigned long long *raster_pairs, hipStream_t stream);
 
 // Writes (view << 24 | binning tile of the view, record slot) pairs at the
 // scanned offsets: each view's pairs form one contiguous run, front to back.
@@ -126,15 +148,16 @@ void launch_pairs_finalize(const uint64_t *total, uint32_t capacity, uint32_t *m
 // rendered again).
 void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
                       uint32_t shift, const uint32_t *svals, const uint2 *rect, const uint32_t *counts_rev,
-                      const uint32_t *offsets_rev, uint32_t capacity, uint32_t *pair_keys, uint32_t *pair_vals,
-                      hipStream_t stream);
+                      const uint32_t *offsets_rev, const uint32_t *block_sums, uint32_t capacity,
+                      uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t stream);
 
 // After a stable sort of the pairs on the tile bits of their keys:
 // ranges[t] = [first, last+1) of global binning tile t (= ctile_off of the
-// key's view + the key's tile); the pair count is read from *m_dev (grid
-// sized for `capacity`).
+// key's view + the key's tile; the ranges must be zero beforehand); the pair
+// count is read from *m_dev (grid sized for `capacity`). The grid first
+// zeroes `clears`.
 void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const uint32_t *m_dev, uint32_t capacity,
-                        uint32_t ctiles, uint2 *ranges, hipStream_t stream);
+                        uint2 *ranges, const Clears &clears, hipStream_t stream);
 
 // 16x16-tile counts of view v (row-major, tiles_x * tiles_y, zeroed here):
 // the number of visible splats whose pixel bounds overlap each raster tile.

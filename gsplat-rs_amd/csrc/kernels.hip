@@ -239,11 +239,22 @@ __device__ __forceinline__ Projected project_one(const FrameParams &fp, const fl
   return o;
 }
 
+// Zeroes the word ranges of `c` with the whole grid (grid-stride).
+__device__ __forceinline__ void clear_ranges(const Clears &c) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < Clears::kRanges; ++r)
+    for (size_t i = i0; i < c.words[r]; i += stride) c.ptr[r][i] = 0u;
+}
+
 template <int DEG>
 __global__ __launch_bounds__(kThreads) void k_project(const ViewDev *__restrict__ views,
                                                       const uint32_t *__restrict__ seg_start, uint32_t nviews,
                                                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                      float4 *__restrict__ rec, uint2 *__restrict__ rect) {
+                                                      float4 *__restrict__ rec, uint2 *__restrict__ rect,
+                                                      Clears clears) {
+  clear_ranges(clears);
   const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
   const uint32_t v = find_view(seg_start, nviews, blockIdx.x * kThreads);
   const ViewDev &vw = views[v];
@@ -400,6 +411,19 @@ __device__ __forceinline__ uint32_t rect_tiles(uint2 r, uint32_t shift) {
   return (x1 - x0 + 1) * (y1 - y0 + 1);
 }
 
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+#pragma unroll
+  for (uint32_t off = kWave / 2; off > 0; off >>= 1) {
+    const uint32_t lo = __shfl_xor((uint32_t)x, off, kWave), hi = __shfl_xor((uint32_t)(x >> 32), off, kWave);
+    x += ((unsigned long long)hi << 32) | lo;
+  }
+  return x;
+}
+
+// Tile counts in reversed slot order with their in-block exclusive scan.
+// Block b covers reversed positions [1024 b, 1024 b + 1024), i.e. sorted
+// slots total-1-r, four consecutive per thread (a block lies in one view
+// segment, since segments align to 4096 slots).
 __global__ __launch_bounds__(kThreads) void k_tile_counts(const ViewDev *__restrict__ views,
                                                           const uint32_t *__restrict__ seg_start, uint32_t nviews,
                                                           uint32_t total, uint32_t shift,
@@ -407,50 +431,82 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const ViewDev *__restr
                                                           const uint32_t *__restrict__ svals,
                                                           const uint2 *__restrict__ rect,
                                                           uint32_t *__restrict__ counts_rev,
+                                                          uint32_t *__restrict__ offsets_rev,
                                                           uint32_t *__restrict__ visible,
-                                                          uint32_t *__restrict__ block_pairs) {
-  __shared__ uint32_t wsum[kWaves];
-  const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
-  const uint32_t v = find_view(seg_start, nviews, blockIdx.x * kThreads);
+                                                          uint32_t *__restrict__ block_sums,
+                                                          uint32_t *__restrict__ block_pairs, Clears clears) {
+  __shared__ uint32_t lds4[kWaves];
+  __shared__ uint32_t wfine[kWaves];
+  clear_ranges(clears);
+  const uint32_t t = threadIdx.x, w = t / kWave, lane = __lane_id();
+  const uint32_t b = blockIdx.x;
+  const uint32_t slot_lo = total - kCountBlock * (b + 1);  // lowest slot of the block
+  const uint32_t v = find_view(seg_start, nviews, slot_lo);
   const ViewDev &vw = views[v];
-  const uint32_t key = skeys[g];
-  uint32_t c = 0, fine = 0;
-  const uint32_t local = svals[g] - (v << kLocalBits);
-  if (key != 0u && local < vw.n) {
-    const uint2 r = rect[vw.seg_off + local];
-    c = rect_tiles(r, shift);
-    fine = rect_tiles(r, 0);
-    const uint32_t j = g - vw.seg_off;
-    if (j + 1 == vw.seg_len || skeys[g + 1] == 0u) visible[v] = j + 1;
+  const uint32_t r0 = b * kCountBlock + t * 4;
+  uint32_t c[4];
+  uint32_t sum = 0, fine = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint32_t g = total - 1 - (r0 + k);
+    const uint32_t key = skeys[g];
+    const uint32_t local = svals[g] - (v << kLocalBits);
+    uint32_t cc = 0;
+    if (key != 0u && local < vw.n) {
+      const uint2 r = rect[vw.seg_off + local];
+      cc = rect_tiles(r, shift);
+      fine += rect_tiles(r, 0);
+      const uint32_t j = g - vw.seg_off;
+      if (j + 1 == vw.seg_len || skeys[g + 1] == 0u) visible[v] = j + 1;
+    }
+    c[k] = cc;
+    sum += cc;
   }
-  counts_rev[total - 1 - g] = c;
-  // This workgroup's raster-tile pair count, summed by k_pairs_finalize (a
-  // single address updated by every wave would serialise).
+  uint32_t block_total;
+  uint32_t run = block_exclusive_scan(sum, lds4, &block_total);
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    counts_rev[r0 + k] = c[k];
+    offsets_rev[r0 + k] = run;
+    run += c[k];
+  }
 #pragma unroll
   for (uint32_t off = kWave / 2; off > 0; off >>= 1) fine += __shfl_down(fine, off, kWave);
-  if (__lane_id() == 0) wsum[threadIdx.x / kWave] = fine;
+  if (lane == 0) wfine[w] = fine;
   __syncthreads();
-  if (threadIdx.x == 0) block_pairs[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  if (t == 0) {
+    block_sums[b] = block_total;
+    block_pairs[b] = wfine[0] + wfine[1] + wfine[2] + wfine[3];
+  }
 }
 
-// One workgroup: the binning pair total and overflow flag, and the sum of
-// the per-workgroup raster-tile pair counts into *raster_pairs.
-__global__ __launch_bounds__(kThreads) void k_pairs_finalize(const uint64_t *__restrict__ total, uint32_t capacity,
-                                                             uint32_t *__restrict__ m_out,
-                                                             uint32_t *__restrict__ info,
-                                                             const uint32_t *__restrict__ block_pairs,
-                                                             uint32_t blocks,
-                                                             unsigned long long *__restrict__ raster_pairs) {
+// One workgroup: exclusive scan of the block totals in place (each block's
+// first pair index), the binning pair total M (info[1..2]), its overflow
+// flag (info[3]) and *m_out = min(M, capacity), and the sum of block_pairs
+// into *raster_pairs.
+__global__ __launch_bounds__(kThreads) void k_block_scan(uint32_t *__restrict__ block_sums, uint32_t blocks,
+                                                         uint32_t capacity, uint32_t *__restrict__ m_out,
+                                                         uint32_t *__restrict__ info,
+                                                         const uint32_t *__restrict__ block_pairs,
+                                                         unsigned long long *__restrict__ raster_pairs) {
+  __shared__ uint32_t lds4[kWaves];
   __shared__ unsigned long long wsum[kWaves];
-  unsigned long long sum = 0;
-  for (uint32_t b = threadIdx.x; b < blocks; b += kThreads) sum += block_pairs[b];
-#pragma unroll
-  for (uint32_t off = kWave / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, kWave);
-  if (__lane_id() == 0) wsum[threadIdx.x / kWave] = sum;
+  unsigned long long carry = 0, pairs = 0;
+  for (uint32_t base = 0; base < blocks; base += kThreads) {
+    const uint32_t p = base + threadIdx.x;
+    const uint32_t x = p < blocks ? block_sums[p] : 0u;
+    pairs += p < blocks ? block_pairs[p] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan(x, lds4, &tot);
+    if (p < blocks) block_sums[p] = (uint32_t)(carry + ex);
+    carry += tot;
+  }
+  pairs = wave_sum_u64(pairs);
+  if (__lane_id() == 0) wsum[threadIdx.x / kWave] = pairs;
   __syncthreads();
   if (threadIdx.x != 0) return;
   *raster_pairs = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-  const uint64_t m = *total;
+  const uint64_t m = carry;
   *m_out = m > capacity ? capacity : (uint32_t)m;
   info[1] = (uint32_t)m;
   info[2] = (uint32_t)(m >> 32);
@@ -468,7 +524,8 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
                                                         const uint32_t *__restrict__ svals,
                                                         const uint2 *__restrict__ rect,
                                                         const uint32_t *__restrict__ counts_rev,
-                                                        const uint32_t *__restrict__ offsets_rev, uint32_t capacity,
+                                                        const uint32_t *__restrict__ offsets_rev,
+                                                        const uint32_t *__restrict__ block_sums, uint32_t capacity,
                                                         uint32_t *__restrict__ pair_keys,
                                                         uint32_t *__restrict__ pair_vals) {
   __shared__ uint32_t s_first[kThreads + 1];
@@ -481,8 +538,9 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
   // Staged entry t is sorted slot g0 + 255 - t (nearest first).
   const uint32_t r = total - 1 - (g0 + kThreads - 1 - t);
   const uint32_t cnt = counts_rev[r];
-  s_first[t] = offsets_rev[r];
-  if (t == kThreads - 1) s_first[kThreads] = offsets_rev[r] + cnt;
+  const uint32_t first = block_sums[r / kCountBlock] + offsets_rev[r];
+  s_first[t] = first;
+  if (t == kThreads - 1) s_first[kThreads] = first + cnt;
   if (cnt) {
     const uint32_t slot = vw.seg_off + (svals[g0 + kThreads - 1 - t] - (v << kLocalBits));
     const uint2 b = rect[slot];
@@ -517,10 +575,12 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
 // The pairs are sorted by local tile only; each view's pairs were emitted
 // as one contiguous run, so every (view, local tile) key is one contiguous
 // run of the sorted pairs.
+// The grid first clears `clears`.
 __global__ __launch_bounds__(kThreads) void k_tile_ranges(const ViewDev *__restrict__ views,
                                                           const uint32_t *__restrict__ keys,
                                                           const uint32_t *__restrict__ m_dev,
-                                                          uint2 *__restrict__ ranges) {
+                                                          uint2 *__restrict__ ranges, Clears clears) {
+  clear_ranges(clears);
   const uint32_t m = *m_dev;
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < m; i += gridDim.x * kThreads) {
     const uint32_t k = keys[i];
@@ -785,15 +845,16 @@ void set_bin_tiles(FrameParams &fp, uint32_t shift) {
 }
 
 void launch_project(uint32_t deg, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,
-                    uint32_t total_slots, uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect, hipStream_t st) {
+                    uint32_t total_slots, uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect, const Clears &clr,
+                    hipStream_t st) {
   if (total_slots == 0 || nviews == 0) return;
   const dim3 grid(div_up(total_slots, kThreads)), block(kThreads);
   switch (deg) {
-    case 0: k_project<0><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect); break;
-    case 1: k_project<1><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect); break;
-    case 2: k_project<2><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect); break;
-    case 3: k_project<3><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect); break;
-    case 4: k_project<4><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect); break;
+    case 0: k_project<0><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr); break;
+    case 1: k_project<1><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr); break;
+    case 2: k_project<2><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr); break;
+    case 3: k_project<3><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr); break;
+    case 4: k_project<4><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr); break;
     default: throw Error(Code::Unsupported, "sh degree above 4");
   }
   GS_HIP(hipGetLastError());
@@ -833,29 +894,30 @@ void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s,
 
 void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
                         uint32_t shift, const uint32_t *skeys, const uint32_t *svals, const uint2 *rect,
-                        uint32_t *counts_rev, uint32_t *visible, uint32_t *block_pairs, hipStream_t st) {
+                        uint32_t *counts_rev, uint32_t *offsets_rev, uint32_t *visible, uint32_t *block_sums,
+                        uint32_t *block_pairs, const Clears &clr, hipStream_t st) {
+  if (total_slots % kCountBlock) throw Error(Code::Internal, "slot count not a multiple of the count block");
   if (total_slots == 0) return;
-  k_tile_counts<<<total_slots / kThreads, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, shift, skeys,
-                                                               svals, rect, counts_rev, visible, block_pairs);
+  k_tile_counts<<<total_slots / kCountBlock, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, shift, skeys,
+                                                                  svals, rect, counts_rev, offsets_rev, visible,
+                                                                  block_sums, block_pairs, clr);
   GS_HIP(hipGetLastError());
 }
 
-void launch_pairs_finalize(const uint64_t *total, uint32_t capacity, uint32_t *m_out, uint32_t *info,
-                           const uint32_t *block_pairs, uint32_t total_slots, unsigned long long *raster_pairs,
-                           hipStream_t st) {
-  k_pairs_finalize<<<1, kThreads, 0, st>>>(total, capacity, m_out, info, block_pairs, total_slots / kThreads,
-                                           raster_pairs);
+void launch_block_scan(uint32_t *block_sums, uint32_t blocks, uint32_t capacity, uint32_t *m_out, uint32_t *info,
+                       const uint32_t *block_pairs, unsigned long long *raster_pairs, hipStream_t st) {
+  k_block_scan<<<1, kThreads, 0, st>>>(block_sums, blocks, capacity, m_out, info, block_pairs, raster_pairs);
   GS_HIP(hipGetLastError());
 }
 
 void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
                       uint32_t shift, const uint32_t *svals, const uint2 *rect, const uint32_t *counts_rev,
-                      const uint32_t *offsets_rev, uint32_t capacity, uint32_t *pair_keys, uint32_t *pair_vals,
-                      hipStream_t st) {
+                      const uint32_t *offsets_rev, const uint32_t *block_sums, uint32_t capacity,
+                      uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t st) {
   if (total_slots == 0) return;
   k_tile_emit<<<total_slots / kThreads, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, shift, svals,
-                                                             rect, counts_rev, offsets_rev, capacity, pair_keys,
-                                                             pair_vals);
+                                                             rect, counts_rev, offsets_rev, block_sums, capacity,
+                                                             pair_keys, pair_vals);
   GS_HIP(hipGetLastError());
 }
 
@@ -875,12 +937,9 @@ void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t st) {
 }
 
 void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const uint32_t *m_dev, uint32_t capacity,
-                        uint32_t tiles,
-                        uint2 *ranges, hipStream_t st) {
-  launch_fill_u32(reinterpret_cast<uint32_t *>(ranges), 0u, 2 * (size_t)tiles, st);
-  if (capacity == 0) return;
-  k_tile_ranges<<<std::min<uint32_t>(div_up(capacity, kThreads), kMaxGrid), kThreads, 0, st>>>(views, pair_keys,
-                                                                                              m_dev, ranges);
+                        uint2 *ranges, const Clears &clr, hipStream_t st) {
+  const uint32_t grid = std::max<uint32_t>(std::min<uint32_t>(div_up(capacity, kThreads), kMaxGrid), 1);
+  k_tile_ranges<<<grid, kThreads, 0, st>>>(views, pair_keys, m_dev, ranges, clr);
   GS_HIP(hipGetLastError());
 }
 

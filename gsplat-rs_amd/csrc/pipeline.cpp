@@ -13,13 +13,14 @@ namespace {
 
 constexpr size_t kTableBytes = 2 * kMaxViews * sizeof(uint32_t) + kMaxViews * sizeof(ViewDev);
 // Frame summary words (device and pinned host copies):
-//   [1..2] binning pair count M (u64), [3] overflow flag,
-//   [4 .. 4+kMaxViews) visible count per view,
-//   kEvalsWord: u64 evaluations passing, u64 coverage tests (counting mode),
-//   kRasterPairsWord: u64 (16x16 tile, splat) pairs.
-constexpr size_t kEvalsWord = 4 + kMaxViews;
-constexpr size_t kRasterPairsWord = kEvalsWord + 4;
-constexpr size_t kSummaryWords = kRasterPairsWord + 2;
+//   [0] unused, [1..2] binning pair count M (u64),
+//   [3] overflow flag, [4..5] (16x16 tile, splat) pairs (u64),
+//   [6 .. 6+kMaxViews) visible count per view,
+//   kEvalsWord: u64 evaluations passing, u64 coverage tests (counting mode).
+constexpr size_t kRasterPairsWord = 4;
+constexpr size_t kVisibleWord = 6;
+constexpr size_t kEvalsWord = kVisibleWord + kMaxViews;
+constexpr size_t kSummaryWords = kEvalsWord + 4;
 
 uint32_t bits_for(uint64_t count) {
   uint32_t b = 1;
@@ -77,7 +78,6 @@ Pipeline::Pipeline() {
   std::memset(summary_pinned_, 0, kSummaryWords * sizeof(uint32_t));
   table_.ensure(kTableBytes);
   summary_.ensure(kSummaryWords);
-  total_.ensure(1);
   m_dev_.ensure(1);
   export_view_.ensure(1);
 }
@@ -158,7 +158,9 @@ void Pipeline::launch_frame() {
   uint32_t view_ctiles = 1;
   for (const ViewDev &d : host_views_) view_ctiles = std::max(view_ctiles, d.ctiles);
   const uint32_t tile_bits = bits_for(view_ctiles);  // the pair sort orders binning tiles within a view
+  const int value_byte = nviews > 1 ? 3 : -1;
   const size_t m = std::max<uint32_t>(slots, 1);
+  const uint32_t count_blocks = slots / kCountBlock;
   keys_.ensure(m);
   vals_.ensure(m);
   keys_alt_.ensure(m);
@@ -167,8 +169,8 @@ void Pipeline::launch_frame() {
   rect_.ensure(m);
   counts_.ensure(m);
   offsets_.ensure(m);
-  partials_.ensure(scan_partial_entries(m));
-  block_pairs_.ensure(m / kThreads + 1);
+  block_sums_.ensure(count_blocks + 1);
+  block_pairs_.ensure(count_blocks + 1);
   img32_.ensure(total_pixels_);
   img8_.ensure(total_pixels_);
   ranges_.ensure(std::max<uint32_t>(ctiles, 1));
@@ -176,7 +178,16 @@ void Pipeline::launch_frame() {
   pair_vals_.ensure(pair_cap_);
   pair_keys_alt_.ensure(pair_cap_);
   pair_vals_alt_.ensure(pair_cap_);
-  sort_scratch_.ensure(std::max(onesweep_status_words((uint32_t)m, 32), onesweep_status_words(pair_cap_, tile_bits)));
+  // Look-back sorts need zeroed scratch; a later kernel of the frame clears
+  // what each sort dirtied, so the next frame finds it clean.
+  if (depth_scratch_.ensure_zeroed(onesweep_status_words((uint32_t)m, 32))) depth_clean_ = true;
+  if (pair_scratch_.ensure_zeroed(onesweep_status_words(pair_cap_, tile_bits))) pair_clean_ = true;
+  const bool depth_lookback = radix_uses_lookback((uint32_t)m);
+  // The last frame's pair count (+25%) sizes the pair sort's grids and mode.
+  const uint32_t hint = pairs_ ? (uint32_t)std::min<uint64_t>(pairs_ + pairs_ / 4, pair_cap_) : pair_cap_;
+  const bool pair_lookback = radix_uses_lookback(pair_cap_, hint);
+  const size_t depth_dirty = radix_lookback_words((uint32_t)m, radix_passes(32, true, value_byte).count);
+  const size_t pair_dirty = radix_lookback_words(pair_cap_, radix_passes(tile_bits, false, -1).count);
 
   hipStream_t st = stream_;
   // View table through a pinned staging slot whose previous upload is done.
@@ -197,46 +208,59 @@ void Pipeline::launch_frame() {
   const uint32_t *tile_start_d = seg_start_d + kMaxViews;
   const ViewDev *views_d = reinterpret_cast<const ViewDev *>(table_.get() + 2 * kMaxViews * sizeof(uint32_t));
   uint32_t *info = summary_.get();
-  uint32_t *visible = info + 4;
-  ScanScratch sc{partials_.get(), total_.get(), partials_.capacity()};
+  uint32_t *visible = info + kVisibleWord;
+  unsigned long long *raster_pairs = reinterpret_cast<unsigned long long *>(info + kRasterPairsWord);
 
   GS_HIP(hipEventRecord(ev_[0], st));
-  launch_fill_u32(visible, 0u, nviews, st);
+  // The projection zeroes the summary words and the binning tile ranges of
+  // this frame.
+  Clears frame_clears;
+  frame_clears.add(info, kVisibleWord + nviews);
+  frame_clears.add(reinterpret_cast<uint32_t *>(ranges_.get()), 2 * (size_t)ctiles);
+  if (slots == 0) {
+    for (int c = 0; c < Clears::kRanges; ++c) launch_fill_u32(frame_clears.ptr[c], 0u, frame_clears.words[c], st);
+  }
   bool degree_seen[5] = {false, false, false, false, false};
   for (const ViewDev &d : host_views_) degree_seen[std::min<uint32_t>(d.sh_degree, 4)] = true;
   for (uint32_t deg = 0; deg <= 4; ++deg)
-    if (degree_seen[deg])
-      launch_project(deg, views_d, seg_start_d, nviews, slots, keys_.get(), vals_.get(), rec_.get(), rect_.get(), st);
+    if (degree_seen[deg]) {
+      launch_project(deg, views_d, seg_start_d, nviews, slots, keys_.get(), vals_.get(), rec_.get(), rect_.get(),
+                     frame_clears, st);
+      frame_clears = Clears();  // once
+    }
   GS_HIP(hipEventRecord(ev_[1], st));
 
   uint32_t *k = keys_.get(), *v = vals_.get(), *ka = keys_alt_.get(), *va = vals_alt_.get();
-  onesweep_sort(&k, &v, &ka, &va, slots, nullptr, 32, /*descending=*/true, sort_scratch_.get(),
-                sort_scratch_.capacity(), st, nviews > 1 ? 3 : -1);
+  SortPrep depth_prep;
+  depth_prep.scratch_clean = depth_clean_;
+  onesweep_sort(&k, &v, &ka, &va, slots, nullptr, 32, /*descending=*/true, depth_scratch_.get(),
+                depth_scratch_.capacity(), st, value_byte, 0, depth_prep);
   sorted_keys_ = k;
   sorted_vals_ = v;
   GS_HIP(hipEventRecord(ev_[2], st));
 
-  unsigned long long *raster_pairs = reinterpret_cast<unsigned long long *>(info + kRasterPairsWord);
-  launch_tile_counts(views_d, seg_start_d, nviews, slots, bin_shift_, k, v, rect_.get(), counts_.get(), visible,
-                     block_pairs_.get(), st);
-  exclusive_scan(counts_.get(), offsets_.get(), slots, sc, st);
-  launch_pairs_finalize(total_.get(), pair_cap_, m_dev_.get(), info, block_pairs_.get(), slots, raster_pairs, st);
+  Clears scan_clears;  // the depth sort's scratch, for the next frame
+  if (depth_lookback) scan_clears.add(depth_scratch_.get(), depth_dirty);
+  depth_clean_ = depth_lookback;
+  launch_tile_counts(views_d, seg_start_d, nviews, slots, bin_shift_, k, v, rect_.get(), counts_.get(),
+                     offsets_.get(), visible, block_sums_.get(), block_pairs_.get(), scan_clears, st);
+  launch_block_scan(block_sums_.get(), count_blocks, pair_cap_, m_dev_.get(), info, block_pairs_.get(), raster_pairs,
+                    st);
   launch_tile_emit(views_d, seg_start_d, nviews, slots, bin_shift_, v, rect_.get(), counts_.get(), offsets_.get(),
-                   pair_cap_,
-                   pair_keys_.get(), pair_vals_.get(), st);
+                   block_sums_.get(), pair_cap_, pair_keys_.get(), pair_vals_.get(), st);
   uint32_t *pk = pair_keys_.get(), *pv = pair_vals_.get(), *pka = pair_keys_alt_.get(), *pva = pair_vals_alt_.get();
-  // The last frame's pair count (+25%) sizes the pair sort's grids.
-  const uint32_t hint = pairs_ ? (uint32_t)std::min<uint64_t>(pairs_ + pairs_ / 4, pair_cap_) : pair_cap_;
+  SortPrep pair_prep;
+  pair_prep.scratch_clean = pair_clean_;
   onesweep_sort(&pk, &pv, &pka, &pva, pair_cap_, m_dev_.get(), tile_bits, /*descending=*/false,
-                sort_scratch_.get(), sort_scratch_.capacity(), st, -1, hint);
-  launch_tile_ranges(views_d, pk, m_dev_.get(), pair_cap_, ctiles, ranges_.get(), st);
+                pair_scratch_.get(), pair_scratch_.capacity(), st, -1, hint, pair_prep);
+  Clears range_clears;  // the pair sort's scratch (and the counting words), for later
+  if (pair_lookback) range_clears.add(pair_scratch_.get(), pair_dirty);
+  pair_clean_ = pair_lookback;
+  if (count_evals_) range_clears.add(info + kEvalsWord, 4);
+  launch_tile_ranges(views_d, pk, m_dev_.get(), pair_cap_, ranges_.get(), range_clears, st);
   GS_HIP(hipEventRecord(ev_[3], st));
 
-  unsigned long long *evals = nullptr;
-  if (count_evals_) {
-    evals = reinterpret_cast<unsigned long long *>(info + kEvalsWord);
-    launch_fill_u32(info + kEvalsWord, 0u, 4, st);
-  }
+  unsigned long long *evals = count_evals_ ? reinterpret_cast<unsigned long long *>(info + kEvalsWord) : nullptr;
   launch_raster(raster_variant_, views_d, tile_start_d, nviews, tiles, bin_shift_, rec_.get(), rect_.get(),
                 ranges_.get(), pv, img32_.get(), img8_.get(), st, evals);
   GS_HIP(hipEventRecord(ev_[4], st));
@@ -246,12 +270,8 @@ void Pipeline::launch_frame() {
     GS_HIP(hipMemcpyAsync(host_rgba8_[i], img8_.get() + d.img_off, (size_t)d.fp.width * d.fp.height * 4,
                           hipMemcpyDeviceToHost, st));
   }
-  GS_HIP(hipMemcpyAsync(summary_pinned_, info, (4 + nviews) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  GS_HIP(hipMemcpyAsync(summary_pinned_ + kRasterPairsWord, info + kRasterPairsWord, 2 * sizeof(uint32_t),
-                        hipMemcpyDeviceToHost, st));
-  if (count_evals_)
-    GS_HIP(hipMemcpyAsync(summary_pinned_ + kEvalsWord, info + kEvalsWord, 4 * sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, st));
+  const size_t summary_words = count_evals_ ? kSummaryWords : kVisibleWord + nviews;
+  GS_HIP(hipMemcpyAsync(summary_pinned_, info, summary_words * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   GS_HIP(hipEventRecord(ev_[5], st));
   pending_ = true;
   summary_valid_ = false;
@@ -259,7 +279,7 @@ void Pipeline::launch_frame() {
 
 bool Pipeline::absorb_summary() {
   const uint32_t nviews = (uint32_t)views_.size();
-  visible_.assign(summary_pinned_ + 4, summary_pinned_ + 4 + nviews);
+  visible_.assign(summary_pinned_ + kVisibleWord, summary_pinned_ + kVisibleWord + nviews);
   pairs_ = ((uint64_t)summary_pinned_[2] << 32) | summary_pinned_[1];
   std::memcpy(&raster_pairs_, summary_pinned_ + kRasterPairsWord, sizeof(raster_pairs_));
   const bool overflow = summary_pinned_[3] != 0;

@@ -43,11 +43,13 @@ class DevBuf {
     }
     cap_ = want;
   }
-  // As ensure(), clearing the allocation to zero whenever it is (re)made.
-  void ensure_zeroed(size_t count) {
-    if (count <= cap_) return;
+  // As ensure(), clearing the allocation to zero whenever it is (re)made;
+  // true when it was.
+  bool ensure_zeroed(size_t count) {
+    if (count <= cap_) return false;
     ensure(count);
     GS_HIP(hipMemset(ptr_, 0, cap_ * sizeof(T)));
+    return true;
   }
   void release() {
     if (ptr_) {
@@ -179,8 +181,7 @@ class Pipeline {
   DevBuf<uint2> ranges_;
   DevBuf<float4> img32_;
   DevBuf<uchar4> img8_;
-  DevBuf<uint32_t> sort_scratch_, partials_, m_dev_, block_pairs_;
-  DevBuf<uint64_t> total_;
+  DevBuf<uint32_t> depth_scratch_, pair_scratch_, m_dev_, block_sums_, block_pairs_;
   DevBuf<ViewDev> export_view_;
 
   std::vector<ViewSpec> views_;
@@ -200,6 +201,7 @@ class Pipeline {
   FrameTimes times_{};
   RasterVariant raster_variant_ = RasterVariant::Unrolled;
   bool count_evals_ = false;
+  bool depth_clean_ = false, pair_clean_ = false;  // look-back scratch words are zero
   uint32_t bin_shift_ = 2;
 };
 

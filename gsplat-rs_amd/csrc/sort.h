@@ -11,6 +11,42 @@ namespace gs {
 constexpr uint32_t kSortThreads = 256;
 constexpr uint32_t kSortItems = 16;
 constexpr uint32_t kSortTile = kSortThreads * kSortItems;
+constexpr uint32_t kSortMaxPasses = 5;
+constexpr uint32_t kSortStripes = 8;  // striped copies of the digit histograms
+
+// The 8-bit digit passes of one sort: pass p orders by byte shift[p]/8 of
+// (key ^ flip[p]), or of the value when from_value[p].
+struct RadixPasses {
+  uint32_t count;
+  uint32_t shift[kSortMaxPasses];
+  uint32_t flip[kSortMaxPasses];
+  uint32_t from_value[kSortMaxPasses];
+};
+RadixPasses radix_passes(uint32_t bits, bool descending, int value_byte);
+
+// Digit histograms of a look-back sort, as k_os_hist accumulates them: a
+// kernel that produces the keys may add them instead (sort with hist_ready):
+// hist[((stripe * kSortMaxPasses) + p) * 256 + digit], stripe = workgroup
+// mod kSortStripes.
+__device__ __forceinline__ uint32_t radix_digit(uint32_t key, uint32_t val, uint32_t shift, uint32_t flip,
+                                                uint32_t from_value) {
+  return (((from_value ? val : key) ^ flip) >> shift) & 0xFFu;
+}
+
+// True when a sort of up to n_cap elements, n_hint expected (0: n_cap),
+// runs look-back passes (which need clean scratch and digit histograms)
+// rather than reduce-then-scan passes; onesweep_sort decides the same way.
+bool radix_uses_lookback(uint32_t n_cap, uint32_t n_hint = 0);
+
+// u32 words at the start of the scratch that a look-back sort of n_cap
+// elements with `passes` passes needs zeroed (histograms, tile counters,
+// status words); they are left dirty after the sort.
+size_t radix_lookback_words(uint32_t n_cap, uint32_t passes);
+
+struct SortPrep {
+  bool scratch_clean = false;  // the look-back words are zero: skip the clear
+  bool hist_ready = false;     // the digit histograms are accumulated: skip k_os_hist
+};
 
 // u32 words of scratch needed to sort up to n_cap keys on `bits` key bits
 // (plus an optional value pass).
@@ -26,6 +62,6 @@ size_t onesweep_status_words(uint32_t n_cap, uint32_t bits);
 // any count up to n_cap is still sorted.
 void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n_cap,
                    const uint32_t *n_dev, uint32_t bits, bool descending, uint32_t *scratch, size_t scratch_words,
-                   hipStream_t stream, int value_byte = -1, uint32_t n_hint = 0);
+                   hipStream_t stream, int value_byte = -1, uint32_t n_hint = 0, SortPrep prep = SortPrep());
 
 }  // namespace gs

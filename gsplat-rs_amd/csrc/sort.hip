@@ -59,9 +59,9 @@ constexpr uint32_t IPT = kSortItems;
 constexpr uint32_t TILE = kSortTile;
 constexpr uint32_t WAVES = T / 64;
 constexpr uint32_t WAVE_ITEMS = TILE / WAVES;
-constexpr uint32_t MAX_PASSES = 5;
+constexpr uint32_t MAX_PASSES = kSortMaxPasses;
 constexpr uint32_t HIST_BLOCKS = 512;
-constexpr uint32_t STRIPES = 8;
+constexpr uint32_t STRIPES = kSortStripes;
 constexpr uint32_t LOOKBACK = 8;  // predecessors polled per look-back step
 constexpr uint32_t kPassGrid = 1024;  // look-back passes: resident 38 KB-LDS workgroups, 4 per CU
 constexpr uint32_t kMinGrid = 1024;   // floor of a hinted grid
@@ -69,17 +69,11 @@ constexpr uint32_t FLAG_AGG = 1u << 30;  // tile count published
 constexpr uint32_t FLAG_INC = 2u << 30;  // inclusive prefix published
 constexpr uint32_t VALUE_MASK = (1u << 30) - 1;
 
-// Digit source of each pass: byte `shift/8` of (key ^ flip), or of the value.
-struct PassSet {
-  uint32_t count;
-  uint32_t shift[MAX_PASSES];
-  uint32_t flip[MAX_PASSES];
-  uint32_t from_value[MAX_PASSES];
-};
+using PassSet = RadixPasses;
 
 __device__ __forceinline__ uint32_t digit(uint32_t key, uint32_t val, uint32_t shift, uint32_t flip,
                                           uint32_t from_value) {
-  return (((from_value ? val : key) ^ flip) >> shift) & 0xFFu;
+  return radix_digit(key, val, shift, flip, from_value);
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -426,12 +420,8 @@ size_t onesweep_status_words(uint32_t n_cap, uint32_t bits) {
   return std::max(lookback, rts);
 }
 
-void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n_cap,
-                   const uint32_t *n_dev, uint32_t bits, bool descending, uint32_t *scratch, size_t scratch_words,
-                   hipStream_t st, int value_byte, uint32_t n_hint) {
-  if (n_cap <= 1 || (bits == 0 && value_byte < 0)) return;
-  if (n_cap > (1u << 30)) throw Error(Code::Unsupported, "sort length exceeds 2^30");
-  PassSet ps{};
+RadixPasses radix_passes(uint32_t bits, bool descending, int value_byte) {
+  RadixPasses ps{};
   const uint32_t key_passes = std::min<uint32_t>((bits + 7) / 8, 4);
   for (uint32_t p = 0; p < key_passes; ++p) {
     ps.shift[ps.count] = 8 * p;
@@ -445,12 +435,30 @@ void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32
     ps.from_value[ps.count] = 1;
     ++ps.count;
   }
+  return ps;
+}
+
+bool radix_uses_lookback(uint32_t n_cap, uint32_t n_hint) {
+  const uint32_t n = n_hint ? std::min(n_hint, n_cap) : n_cap;
+  return div_up(n, TILE) < rts_min_tiles();
+}
+
+size_t radix_lookback_words(uint32_t n_cap, uint32_t passes) {
+  return (size_t)STRIPES * MAX_PASSES * 256 + MAX_PASSES + (size_t)passes * std::max<uint32_t>(div_up(n_cap, TILE), 1) * 256;
+}
+
+void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n_cap,
+                   const uint32_t *n_dev, uint32_t bits, bool descending, uint32_t *scratch, size_t scratch_words,
+                   hipStream_t st, int value_byte, uint32_t n_hint, SortPrep prep) {
+  if (n_cap <= 1 || (bits == 0 && value_byte < 0)) return;
+  if (n_cap > (1u << 30)) throw Error(Code::Unsupported, "sort length exceeds 2^30");
+  const PassSet ps = radix_passes(bits, descending, value_byte);
   const uint32_t tiles = div_up(n_cap, TILE);
   // Grid: one workgroup per expected tile (the hint, for a device-side count,
   // with the capacity as the cap), bounded by GSPLAT_SORT_MAX_GRID.
   const uint32_t expect = n_dev && n_hint ? std::min(div_up(n_hint, TILE), tiles) : tiles;
   const uint32_t grid_tiles = std::min(std::max(expect, std::min(tiles, kMinGrid)), pass_grid_bound());
-  if (tiles >= rts_min_tiles()) {
+  if (!radix_uses_lookback(n_cap, n_dev ? n_hint : 0)) {
     const size_t entries = (size_t)256 * tiles;
     const size_t partials = scan_partial_entries(entries);
     if (entries + partials + 4 > scratch_words) throw Error(Code::Internal, "radix sort scratch too small");
@@ -479,9 +487,10 @@ void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32
   uint32_t *hist = scratch;
   uint32_t *counters = hist + (size_t)STRIPES * MAX_PASSES * 256;
   uint32_t *status = counters + MAX_PASSES;
-  k_os_clear<<<std::min<uint32_t>(div_up(words, T), 1024), T, 0, st>>>(scratch, fixed, (size_t)ps.count * 256, n_dev,
-                                                                        n_cap);
-  k_os_hist<<<hist_blocks(n_cap), T, 0, st>>>(*keys, *vals, n_dev, n_cap, ps, hist);
+  if (!prep.scratch_clean)
+    k_os_clear<<<std::min<uint32_t>(div_up(words, T), 1024), T, 0, st>>>(scratch, fixed, (size_t)ps.count * 256,
+                                                                          n_dev, n_cap);
+  if (!prep.hist_ready) k_os_hist<<<hist_blocks(n_cap), T, 0, st>>>(*keys, *vals, n_dev, n_cap, ps, hist);
   const uint32_t grid = std::min(grid_tiles, kPassGrid);
   for (uint32_t p = 0; p < ps.count; ++p) {
     k_os_pass<<<grid, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, p, ps.shift[p], ps.flip[p],
