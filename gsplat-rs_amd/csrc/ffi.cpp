@@ -629,6 +629,51 @@ void gsplat_scene_destroy(GsplatScene *scene) {
 
 // ---------------------------------------------------------------- batches
 
+namespace {
+
+gs::Tuning tuning_from(const GsplatTuning &t) {
+  gs::Tuning out;
+  out.bin_shift = t.bin_shift;
+  if (t.raster_variant > 1) throw gs::Error(gs::Code::InvalidArgument, "raster_variant must be 0 or 1");
+  out.raster = t.raster_variant == 0 ? gs::RasterVariant::Linear : gs::RasterVariant::Unrolled;
+  out.rts_min_tiles = t.sort_rts_min_tiles;
+  return out;
+}
+
+GsplatTuning tuning_to(const gs::Tuning &t, float ms) {
+  GsplatTuning out;
+  out.bin_shift = t.bin_shift;
+  out.raster_variant = t.raster == gs::RasterVariant::Linear ? 0u : 1u;
+  out.sort_rts_min_tiles = t.rts_min_tiles;
+  out.frame_ms = ms;
+  return out;
+}
+
+std::vector<gs::ViewSpec> view_specs(const char *op, const GsplatViewDesc *views, uint32_t count) {
+  if (count == 0 || count > gs::kMaxViews)
+    throw gs::Error(gs::Code::InvalidArgument, "view count must be 1.." + std::to_string(gs::kMaxViews));
+  std::vector<gs::ViewSpec> specs;
+  specs.reserve(count);
+  for (uint32_t v = 0; v < count; ++v) {
+    const GsplatViewDesc &d = views[v];
+    const std::string at = "view " + std::to_string(v) + ": ";
+    if (!d.scene) throw gs::Error(gs::Code::InvalidArgument, at + "scene is null");
+    if (!gs::camera_valid(d.camera)) throw gs::Error(gs::Code::InvalidArgument, at + "invalid camera");
+    if (d.width == 0 || d.height == 0) throw gs::Error(gs::Code::InvalidArgument, at + "invalid renderer configuration");
+    if (d.width > 16384 || d.height > 16384)
+      throw gs::Error(gs::Code::Unsupported, at + "render dimensions exceed the device 2D texture limit 16384");
+    specs.push_back(gs::ViewSpec{d.scene->scene,
+                                 gs::make_frame_params(d.camera.position, d.camera.rotation_xyzw,
+                                                       d.camera.vertical_fov_radians, d.camera.near_plane,
+                                                       d.camera.far_plane, d.width, d.height)});
+  }
+  (void)op;
+  return specs;
+}
+
+}  // namespace
+
+
 int32_t gsplat_batch_create(GsplatBatch **out_batch) {
   static const char *op = "gsplat_batch_create";
   return guard(op, [&]() -> int32_t {
@@ -654,25 +699,7 @@ int32_t gsplat_batch_render(GsplatBatch *batch, const GsplatViewDesc *views, uin
     if (!batch) return null_arg(op, "batch");
     if (!views) return null_arg(op, "views");
     const auto t0 = std::chrono::steady_clock::now();
-    if (count == 0 || count > gs::kMaxViews)
-      return fail(gs::Code::InvalidArgument, std::string(op) + ": view count must be 1.." +
-                                                 std::to_string(gs::kMaxViews));
-    std::vector<gs::ViewSpec> specs;
-    specs.reserve(count);
-    for (uint32_t v = 0; v < count; ++v) {
-      const GsplatViewDesc &d = views[v];
-      const std::string at = std::string(op) + ": view " + std::to_string(v) + ": ";
-      if (!d.scene) return fail(gs::Code::InvalidArgument, at + "scene is null");
-      if (!gs::camera_valid(d.camera)) return fail(gs::Code::InvalidArgument, at + "invalid camera");
-      if (d.width == 0 || d.height == 0)
-        return fail(gs::Code::InvalidArgument, at + "invalid renderer configuration");
-      if (d.width > 16384 || d.height > 16384)
-        return fail(gs::Code::Unsupported, at + "render dimensions exceed the device 2D texture limit 16384");
-      specs.push_back(gs::ViewSpec{
-          d.scene->scene, gs::make_frame_params(d.camera.position, d.camera.rotation_xyzw,
-                                                d.camera.vertical_fov_radians, d.camera.near_plane,
-                                                d.camera.far_plane, d.width, d.height)});
-    }
+    std::vector<gs::ViewSpec> specs = view_specs(op, views, count);
     batch->pipe.enqueue(std::move(specs));
     batch->host_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return ok();
@@ -771,6 +798,66 @@ int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out) 
     out->tile_pairs = batch->pipe.pairs();
     out->blend_evals = t.blend_evals;
     out->blend_tests = t.blend_tests;
+    return ok();
+  });
+}
+
+// ----------------------------------------------------------------- tuning
+
+
+int32_t gsplat_batch_get_tuning(GsplatBatch *batch, GsplatTuning *out_tuning) {
+  static const char *op = "gsplat_batch_get_tuning";
+  return guard(op, [&]() -> int32_t {
+    if (!batch) return null_arg(op, "batch");
+    if (!out_tuning) return null_arg(op, "out_tuning");
+    *out_tuning = tuning_to(batch->pipe.tuning(), 0.0f);
+    return ok();
+  });
+}
+
+int32_t gsplat_batch_set_tuning(GsplatBatch *batch, const GsplatTuning *tuning) {
+  static const char *op = "gsplat_batch_set_tuning";
+  return guard(op, [&]() -> int32_t {
+    if (!batch) return null_arg(op, "batch");
+    if (!tuning) return null_arg(op, "tuning");
+    batch->pipe.set_tuning(tuning_from(*tuning));
+    return ok();
+  });
+}
+
+int32_t gsplat_batch_autotune(GsplatBatch *batch, const GsplatViewDesc *views, uint32_t count, uint32_t reps,
+                              GsplatTuning *out_best) {
+  static const char *op = "gsplat_batch_autotune";
+  return guard(op, [&]() -> int32_t {
+    if (!batch) return null_arg(op, "batch");
+    if (!views) return null_arg(op, "views");
+    if (!out_best) return null_arg(op, "out_best");
+    const std::vector<gs::TuneSample> samples = batch->pipe.autotune(view_specs(op, views, count), reps);
+    gs::TuneSample best = samples.front();
+    for (const gs::TuneSample &s : samples)
+      if (s.frame_ms < best.frame_ms) best = s;
+    *out_best = tuning_to(best.tuning, best.frame_ms);
+    return ok();
+  });
+}
+
+int32_t gsplat_context_set_tuning(GsplatContext *ctx, const GsplatTuning *tuning) {
+  static const char *op = "gsplat_context_set_tuning";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!tuning) return null_arg(op, "tuning");
+    ctx->renderer->set_tuning(tuning_from(*tuning));
+    return ok();
+  });
+}
+
+int32_t gsplat_context_autotune(GsplatContext *ctx, uint32_t reps, GsplatTuning *out_best) {
+  static const char *op = "gsplat_context_autotune";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!out_best) return null_arg(op, "out_best");
+    const gs::TuneSample best = ctx->renderer->autotune(ctx->camera, reps);
+    *out_best = tuning_to(best.tuning, best.frame_ms);
     return ok();
   });
 }

@@ -4,6 +4,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 
 #include "sort.h"
 
@@ -65,6 +66,8 @@ Pipeline::Pipeline() {
   if (!device_present()) throw Error(Code::Unsupported, "gpu rasterizer unavailable");
   if (const char *e = std::getenv("GSPLAT_COUNT_EVALS")) count_evals_ = std::atoi(e) != 0;
   if (const char *e = std::getenv("GSPLAT_BIN_SHIFT")) set_bin_shift((uint32_t)std::atoi(e));
+  if (const char *e = std::getenv("GSPLAT_SORT_RTS_MIN_TILES"))
+    tuning_.rts_min_tiles = std::max<uint32_t>((uint32_t)std::strtoul(e, nullptr, 10), 1u);
   GS_HIP(hipSetDevice(0));
   GS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   for (auto &e : ev_) GS_HIP(hipEventCreate(&e));
@@ -119,7 +122,7 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
                                          std::to_string(kLocalMask) + " splats");
     ViewDev &d = table[v];
     d.fp = s.fp;
-    set_bin_tiles(d.fp, bin_shift_);
+    set_bin_tiles(d.fp, tuning_.bin_shift);
     d.soa = s.scene->soa.get();
     d.n = n;
     d.sh_degree = s.scene->sh_degree;
@@ -182,10 +185,10 @@ void Pipeline::launch_frame() {
   // what each sort dirtied, so the next frame finds it clean.
   if (depth_scratch_.ensure_zeroed(onesweep_status_words((uint32_t)m, 32))) depth_clean_ = true;
   if (pair_scratch_.ensure_zeroed(onesweep_status_words(pair_cap_, tile_bits))) pair_clean_ = true;
-  const bool depth_lookback = radix_uses_lookback((uint32_t)m);
+  const bool depth_lookback = radix_uses_lookback((uint32_t)m, 0, tuning_.rts_min_tiles);
   // The last frame's pair count (+25%) sizes the pair sort's grids and mode.
   const uint32_t hint = pairs_ ? (uint32_t)std::min<uint64_t>(pairs_ + pairs_ / 4, pair_cap_) : pair_cap_;
-  const bool pair_lookback = radix_uses_lookback(pair_cap_, hint);
+  const bool pair_lookback = radix_uses_lookback(pair_cap_, hint, tuning_.rts_min_tiles);
   const size_t depth_dirty = radix_lookback_words((uint32_t)m, radix_passes(32, true, value_byte).count);
   const size_t pair_dirty = radix_lookback_words(pair_cap_, radix_passes(tile_bits, false, -1).count);
 
@@ -233,6 +236,7 @@ void Pipeline::launch_frame() {
   uint32_t *k = keys_.get(), *v = vals_.get(), *ka = keys_alt_.get(), *va = vals_alt_.get();
   SortPrep depth_prep;
   depth_prep.scratch_clean = depth_clean_;
+  depth_prep.rts_min_tiles = tuning_.rts_min_tiles;
   onesweep_sort(&k, &v, &ka, &va, slots, nullptr, 32, /*descending=*/true, depth_scratch_.get(),
                 depth_scratch_.capacity(), st, value_byte, 0, depth_prep);
   sorted_keys_ = k;
@@ -242,15 +246,16 @@ void Pipeline::launch_frame() {
   Clears scan_clears;  // the depth sort's scratch, for the next frame
   if (depth_lookback) scan_clears.add(depth_scratch_.get(), depth_dirty);
   depth_clean_ = depth_lookback;
-  launch_tile_counts(views_d, seg_start_d, nviews, slots, bin_shift_, k, v, rect_.get(), counts_.get(),
+  launch_tile_counts(views_d, seg_start_d, nviews, slots, tuning_.bin_shift, k, v, rect_.get(), counts_.get(),
                      offsets_.get(), visible, block_sums_.get(), block_pairs_.get(), scan_clears, st);
   launch_block_scan(block_sums_.get(), count_blocks, pair_cap_, m_dev_.get(), info, block_pairs_.get(), raster_pairs,
                     st);
-  launch_tile_emit(views_d, seg_start_d, nviews, slots, bin_shift_, v, rect_.get(), counts_.get(), offsets_.get(),
+  launch_tile_emit(views_d, seg_start_d, nviews, slots, tuning_.bin_shift, v, rect_.get(), counts_.get(), offsets_.get(),
                    block_sums_.get(), pair_cap_, pair_keys_.get(), pair_vals_.get(), st);
   uint32_t *pk = pair_keys_.get(), *pv = pair_vals_.get(), *pka = pair_keys_alt_.get(), *pva = pair_vals_alt_.get();
   SortPrep pair_prep;
   pair_prep.scratch_clean = pair_clean_;
+  pair_prep.rts_min_tiles = tuning_.rts_min_tiles;
   onesweep_sort(&pk, &pv, &pka, &pva, pair_cap_, m_dev_.get(), tile_bits, /*descending=*/false,
                 pair_scratch_.get(), pair_scratch_.capacity(), st, -1, hint, pair_prep);
   Clears range_clears;  // the pair sort's scratch (and the counting words), for later
@@ -261,7 +266,7 @@ void Pipeline::launch_frame() {
   GS_HIP(hipEventRecord(ev_[3], st));
 
   unsigned long long *evals = count_evals_ ? reinterpret_cast<unsigned long long *>(info + kEvalsWord) : nullptr;
-  launch_raster(raster_variant_, views_d, tile_start_d, nviews, tiles, bin_shift_, rec_.get(), rect_.get(),
+  launch_raster(tuning_.raster, views_d, tile_start_d, nviews, tiles, tuning_.bin_shift, rec_.get(), rect_.get(),
                 ranges_.get(), pv, img32_.get(), img8_.get(), st, evals);
   GS_HIP(hipEventRecord(ev_[4], st));
   for (size_t i = 0; i < host_rgba8_.size(); ++i) {
@@ -343,7 +348,46 @@ uint64_t Pipeline::bin_pairs() {
 
 void Pipeline::set_bin_shift(uint32_t shift) {
   if (shift > kMaxBinShift) throw Error(Code::InvalidArgument, "bin shift above " + std::to_string(kMaxBinShift));
-  bin_shift_ = shift;
+  tuning_.bin_shift = shift;
+}
+
+void Pipeline::set_tuning(const Tuning &t) {
+  if (t.bin_shift > kMaxBinShift) throw Error(Code::InvalidArgument, "bin shift above " + std::to_string(kMaxBinShift));
+  if (t.raster != RasterVariant::Linear && t.raster != RasterVariant::Unrolled)
+    throw Error(Code::InvalidArgument, "unknown raster variant");
+  if (t.rts_min_tiles == 0) throw Error(Code::InvalidArgument, "sort mode threshold must be positive");
+  wait();
+  tuning_ = t;
+}
+
+std::vector<TuneSample> Pipeline::autotune(const std::vector<ViewSpec> &views, uint32_t reps) {
+  if (reps == 0) throw Error(Code::InvalidArgument, "autotune needs at least one repetition");
+  std::vector<TuneSample> out;
+  const Tuning base = tuning_;
+  TuneSample best{base, 1e30f};
+  for (uint32_t shift = 0; shift <= kMaxBinShift; ++shift)
+    for (uint32_t rts : {128u, 256u, 1024u}) {
+      Tuning t = base;
+      t.bin_shift = shift;
+      t.rts_min_tiles = rts;
+      set_tuning(t);
+      enqueue(views);  // warm: buffers grow, the pair count settles
+      wait();
+      std::vector<float> ms;
+      for (uint32_t r = 0; r < reps; ++r) {
+        enqueue(views);
+        wait();
+        ms.push_back(times_.total_ms);
+      }
+      std::sort(ms.begin(), ms.end());
+      const TuneSample sample{t, ms[ms.size() / 2]};
+      out.push_back(sample);
+      if (sample.frame_ms < best.frame_ms) best = sample;
+    }
+  set_tuning(best.tuning);
+  enqueue(views);
+  wait();
+  return out;
 }
 
 FrameTimes Pipeline::times() {

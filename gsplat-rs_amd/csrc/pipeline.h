@@ -97,6 +97,20 @@ struct FrameTimes {
   uint64_t blend_tests = 0;  // counting mode only: coverage tests made
 };
 
+// Launch configuration of a pipeline. Every setting gives bit-identical
+// images; they trade work between binning, sorting and blending.
+struct Tuning {
+  uint32_t bin_shift = 2;                                // binning tiles of 16 << bin_shift pixels
+  RasterVariant raster = RasterVariant::Unrolled;        // blend loop
+  uint32_t rts_min_tiles = 256;                          // radix sorts of >= this many tiles: reduce-then-scan
+};
+
+// One candidate measured by Pipeline::autotune.
+struct TuneSample {
+  Tuning tuning;
+  float frame_ms;  // median device time of the frame
+};
+
 // Renders batches of views on its own stream. Per frame (Renderer::
 // render_frame, render-wgpu/src/lib.rs:723-742, for every view at once):
 //   project   depth key + cull + projection + SH colour, one thread per slot
@@ -148,7 +162,7 @@ class Pipeline {
   // (proj_out, 12n entries) of `scene` under `fp`, into host memory.
   void export_projection(const DeviceScene &scene, const FrameParams &fp, uint32_t *keys_out, float *proj_out);
 
-  void set_raster_variant(RasterVariant v) { raster_variant_ = v; }
+  void set_raster_variant(RasterVariant v) { tuning_.raster = v; }
   // Counting mode (also GSPLAT_COUNT_EVALS=1): the raster counts the
   // (pixel, splat) evaluations that pass coverage before each pixel's early
   // stop; the count is reported with the frame times.
@@ -156,7 +170,14 @@ class Pipeline {
   // Binning tiles of (16 << shift) pixels, shift <= 3 (also GSPLAT_BIN_SHIFT);
   // images and tile counts do not depend on it.
   void set_bin_shift(uint32_t shift);
-  uint32_t bin_shift() const { return bin_shift_; }
+  uint32_t bin_shift() const { return tuning_.bin_shift; }
+  void set_tuning(const Tuning &t);
+  const Tuning &tuning() const { return tuning_; }
+  // Renders `views` `reps` times under each candidate tuning (bin shift 0..3
+  // x sort mode threshold {128, 256, 1024}), keeps the fastest by median
+  // device frame time and returns every measurement. The last frame rendered
+  // is under the kept tuning.
+  std::vector<TuneSample> autotune(const std::vector<ViewSpec> &views, uint32_t reps);
   hipStream_t stream() const { return stream_; }
 
  private:
@@ -199,10 +220,10 @@ class Pipeline {
   uint64_t raster_pairs_ = 0;  // (16x16 tile, splat) pairs of the last frame
   uint64_t overflow_rerenders_ = 0;
   FrameTimes times_{};
-  RasterVariant raster_variant_ = RasterVariant::Unrolled;
+  Tuning tuning_{};
   bool count_evals_ = false;
   bool depth_clean_ = false, pair_clean_ = false;  // look-back scratch words are zero
-  uint32_t bin_shift_ = 2;
+
 };
 
 }  // namespace gs

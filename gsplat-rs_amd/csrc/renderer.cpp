@@ -51,6 +51,19 @@ void Renderer::render(const GsplatCamera &cam) {
 
 void Renderer::wait() { pipe_.wait(); }
 
+TuneSample Renderer::autotune(const GsplatCamera &cam, uint32_t reps) {
+  if (!scene_) throw Error(Code::SceneNotLoaded, "scene not loaded");
+  if (!camera_valid(cam)) throw Error(Code::InvalidArgument, "invalid camera");
+  const FrameParams fp = make_frame_params(cam.position, cam.rotation_xyzw, cam.vertical_fov_radians,
+                                           cam.near_plane, cam.far_plane, width_, height_);
+  const std::vector<TuneSample> samples = pipe_.autotune({ViewSpec{scene_, fp}}, reps);
+  frame_valid_ = true;
+  TuneSample best = samples.front();
+  for (const TuneSample &s : samples)
+    if (s.frame_ms < best.frame_ms) best = s;
+  return best;
+}
+
 void Renderer::require_frame() const {
   if (!frame_valid_) throw Error(Code::SceneNotLoaded, "no rendered frame");
 }

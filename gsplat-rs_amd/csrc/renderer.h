@@ -30,6 +30,10 @@ class Renderer {
   GsplatStats stats();
   GsplatStageTimes stage_times();
   uint64_t overflow_rerenders() const { return pipe_.overflow_rerenders(); }
+  void set_tuning(const Tuning &t) { pipe_.set_tuning(t); }
+  // Autotunes on the scene and `camera` at the current size; the context's
+  // last frame is then that frame.
+  TuneSample autotune(const GsplatCamera &camera, uint32_t reps);
 
   void readback_rgba8(uint8_t *out, uint64_t len);
   void readback_rgba32f(float *out, uint64_t len);

@@ -36,7 +36,9 @@ __device__ __forceinline__ uint32_t radix_digit(uint32_t key, uint32_t val, uint
 // True when a sort of up to n_cap elements, n_hint expected (0: n_cap),
 // runs look-back passes (which need clean scratch and digit histograms)
 // rather than reduce-then-scan passes; onesweep_sort decides the same way.
-bool radix_uses_lookback(uint32_t n_cap, uint32_t n_hint = 0);
+// rts_min_tiles: sorts of at least this many 4096-item tiles use
+// reduce-then-scan (0: GSPLAT_SORT_RTS_MIN_TILES or 256).
+bool radix_uses_lookback(uint32_t n_cap, uint32_t n_hint = 0, uint32_t rts_min_tiles = 0);
 
 // u32 words at the start of the scratch that a look-back sort of n_cap
 // elements with `passes` passes needs zeroed (histograms, tile counters,
@@ -46,6 +48,7 @@ size_t radix_lookback_words(uint32_t n_cap, uint32_t passes);
 struct SortPrep {
   bool scratch_clean = false;  // the look-back words are zero: skip the clear
   bool hist_ready = false;     // the digit histograms are accumulated: skip k_os_hist
+  uint32_t rts_min_tiles = 0;  // mode threshold, as in radix_uses_lookback
 };
 
 // u32 words of scratch needed to sort up to n_cap keys on `bits` key bits

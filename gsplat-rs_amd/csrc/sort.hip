@@ -438,9 +438,9 @@ RadixPasses radix_passes(uint32_t bits, bool descending, int value_byte) {
   return ps;
 }
 
-bool radix_uses_lookback(uint32_t n_cap, uint32_t n_hint) {
+bool radix_uses_lookback(uint32_t n_cap, uint32_t n_hint, uint32_t min_tiles) {
   const uint32_t n = n_hint ? std::min(n_hint, n_cap) : n_cap;
-  return div_up(n, TILE) < rts_min_tiles();
+  return div_up(n, TILE) < (min_tiles ? min_tiles : rts_min_tiles());
 }
 
 size_t radix_lookback_words(uint32_t n_cap, uint32_t passes) {
@@ -458,7 +458,7 @@ void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32
   // with the capacity as the cap), bounded by GSPLAT_SORT_MAX_GRID.
   const uint32_t expect = n_dev && n_hint ? std::min(div_up(n_hint, TILE), tiles) : tiles;
   const uint32_t grid_tiles = std::min(std::max(expect, std::min(tiles, kMinGrid)), pass_grid_bound());
-  if (!radix_uses_lookback(n_cap, n_dev ? n_hint : 0)) {
+  if (!radix_uses_lookback(n_cap, n_dev ? n_hint : 0, prep.rts_min_tiles)) {
     const size_t entries = (size_t)256 * tiles;
     const size_t partials = scan_partial_entries(entries);
     if (entries + partials + 4 > scratch_words) throw Error(Code::Internal, "radix sort scratch too small");

@@ -123,6 +123,11 @@ class GsplatViewDesc(ctypes.Structure):
     ]
 
 
+class GsplatTuning(ctypes.Structure):
+    _fields_ = [("bin_shift", ctypes.c_uint32), ("raster_variant", ctypes.c_uint32),
+                ("sort_rts_min_tiles", ctypes.c_uint32), ("frame_ms", ctypes.c_float)]
+
+
 class GsplatServerConfig(ctypes.Structure):
     _fields_ = [("memory_budget_bytes", ctypes.c_uint64), ("max_batch_views", ctypes.c_uint32),
                 ("max_queue", ctypes.c_uint32)]
@@ -237,6 +242,11 @@ SIGNATURES = {
     "gsplat_batch_read_sorted_indices": (I32, [P, U32, P, U64, PU32]),
     "gsplat_batch_read_tile_counts": (I32, [P, U32, P, U64, PU32, PU32]),
     "gsplat_batch_get_stage_times": (I32, [P, ctypes.POINTER(GsplatStageTimes)]),
+    "gsplat_batch_get_tuning": (I32, [P, ctypes.POINTER(GsplatTuning)]),
+    "gsplat_batch_set_tuning": (I32, [P, ctypes.POINTER(GsplatTuning)]),
+    "gsplat_batch_autotune": (I32, [P, ctypes.POINTER(GsplatViewDesc), U32, U32, ctypes.POINTER(GsplatTuning)]),
+    "gsplat_context_set_tuning": (I32, [P, ctypes.POINTER(GsplatTuning)]),
+    "gsplat_context_autotune": (I32, [P, U32, ctypes.POINTER(GsplatTuning)]),
     "gsplat_server_config_default": (GsplatServerConfig, []),
     "gsplat_server_create": (I32, [ctypes.POINTER(GsplatServerConfig), ctypes.POINTER(P)]),
     "gsplat_server_destroy": (None, [P]),
@@ -410,6 +420,14 @@ class Context:
         check(lib().gsplat_context_read_projection(self.h, out.ctypes.data, out.size))
         return out[:count]
 
+    def autotune(self, reps: int = 3) -> GsplatTuning:
+        t = GsplatTuning()
+        check(lib().gsplat_context_autotune(self.h, reps, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, t: GsplatTuning):
+        check(lib().gsplat_context_set_tuning(self.h, ctypes.byref(t)))
+
     def tile_counts(self) -> np.ndarray:
         tx, ty = (self.width + 15) // 16, (self.height + 15) // 16
         out = np.zeros(tx * ty, np.uint32)
@@ -466,15 +484,33 @@ class Batch:
         except Exception:
             pass
 
-    def render(self, views):
-        """views: sequence of (Scene, GsplatCamera, width, height)."""
+    @staticmethod
+    def _descs(views):
         arr = (GsplatViewDesc * max(len(views), 1))()
         for i, (scene, cam, w, h) in enumerate(views):
             arr[i].scene = scene.h if scene is not None else None
             arr[i].camera = cam
             arr[i].width, arr[i].height = w, h
-        check(lib().gsplat_batch_render(self.h, arr, len(views)))
+        return arr
+
+    def render(self, views):
+        """views: sequence of (Scene, GsplatCamera, width, height)."""
+        check(lib().gsplat_batch_render(self.h, self._descs(views), len(views)))
         self.views = [(s, c, w, h) for (s, c, w, h) in views]
+
+    def autotune(self, views, reps: int = 3) -> GsplatTuning:
+        t = GsplatTuning()
+        check(lib().gsplat_batch_autotune(self.h, self._descs(views), len(views), reps, ctypes.byref(t)))
+        self.views = [(s, c, w, h) for (s, c, w, h) in views]
+        return t
+
+    def tuning(self) -> GsplatTuning:
+        t = GsplatTuning()
+        check(lib().gsplat_batch_get_tuning(self.h, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, t: GsplatTuning):
+        check(lib().gsplat_batch_set_tuning(self.h, ctypes.byref(t)))
 
     def wait(self):
         check(lib().gsplat_batch_wait(self.h))

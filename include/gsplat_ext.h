@@ -222,6 +222,38 @@ int32_t gsplat_batch_read_tile_counts(GsplatBatch *batch, uint32_t view, uint32_
  * views, tiles_x the total tile count over the views and tiles_y 1. */
 int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out);
 
+/* ----------------------------------------------------------------- tuning */
+
+/*
+ * Launch configuration of a context or batch. Every setting renders
+ * bit-identical images; they trade work between binning, sorting and
+ * blending:
+ *   bin_shift           splats are binned into tiles of (16 << bin_shift)
+ *                       pixels, 0..3; each 16x16 raster tile keeps the
+ *                       splats of its binning tile that overlap it
+ *   raster_variant      0 per-lane exit, 1 wave-uniform loop (default)
+ *   sort_rts_min_tiles  radix sorts of at least this many 4096-key tiles use
+ *                       reduce-then-scan passes, shorter ones look-back passes
+ *   frame_ms            (autotune output) median device time of a frame
+ */
+typedef struct GsplatTuning {
+  uint32_t bin_shift;
+  uint32_t raster_variant;
+  uint32_t sort_rts_min_tiles;
+  float frame_ms;
+} GsplatTuning;
+
+int32_t gsplat_batch_get_tuning(GsplatBatch *batch, GsplatTuning *out_tuning);
+int32_t gsplat_batch_set_tuning(GsplatBatch *batch, const GsplatTuning *tuning);
+/* Renders `views` `reps` times under each candidate (bin_shift 0..3 x
+ * sort_rts_min_tiles {128, 256, 1024}), keeps the fastest and returns it;
+ * the batch's last frame is then those views under the kept tuning. */
+int32_t gsplat_batch_autotune(GsplatBatch *batch, const GsplatViewDesc *views, uint32_t count, uint32_t reps,
+                              GsplatTuning *out_best);
+int32_t gsplat_context_set_tuning(GsplatContext *ctx, const GsplatTuning *tuning);
+/* As gsplat_batch_autotune for the context's scene, camera and size. */
+int32_t gsplat_context_autotune(GsplatContext *ctx, uint32_t reps, GsplatTuning *out_best);
+
 /* ----------------------------------------------------------------- server */
 
 /*

@@ -190,3 +190,51 @@ def test_batch_reduce_then_scan_sorts_equal_single(abi, monkeypatch):
     b.render([(dev[s], cam, w, h) for (s, cam, w, h) in views])
     for v, (s, cam, w, h) in enumerate(views):
         assert_same(batch_view(b, v, scenes[s].count), single(abi, scenes[s], cam, w, h), f"view {v}")
+
+
+def test_autotune_keeps_images(abi):
+    scenes, views = mixed_batch(abi)
+    dev = [abi.Scene(s) for s in scenes]
+    vs = [(dev[s], cam, w, h) for (s, cam, w, h) in views[:6]]
+    b = abi.Batch()
+    b.render(vs)
+    before = [batch_view(b, v, scenes[views[v][0]].count) for v in range(len(vs))]
+    say("[autotune batch]")
+    best = b.autotune(vs, reps=2)
+    assert best.bin_shift <= 3 and best.sort_rts_min_tiles in (128, 256, 1024) and best.frame_ms > 0
+    t = b.tuning()
+    assert (t.bin_shift, t.sort_rts_min_tiles, t.raster_variant) == (best.bin_shift, best.sort_rts_min_tiles,
+                                                                     best.raster_variant)
+    for v in range(len(vs)):
+        assert_same(batch_view(b, v, scenes[views[v][0]].count), before[v], f"view {v} after autotune")
+    # every setting renders the same frame
+    for shift in (0, 3):
+        t.bin_shift, t.raster_variant = shift, 0
+        b.set_tuning(t)
+        b.render(vs)
+        for v in range(len(vs)):
+            assert_same(batch_view(b, v, scenes[views[v][0]].count), before[v], f"view {v} shift {shift}")
+    bad = b.tuning()
+    bad.bin_shift = 4
+    with pytest.raises(abi.GsplatError) as e:
+        b.set_tuning(bad)
+    assert e.value.code == abi.INVALID_ARGUMENT
+    bad = b.tuning()
+    bad.raster_variant = 7
+    with pytest.raises(abi.GsplatError) as e:
+        b.set_tuning(bad)
+    assert e.value.code == abi.INVALID_ARGUMENT
+
+
+def test_context_autotune(abi):
+    scene = synth.tiny_scene(20000, 3131, sh_degree=3)
+    ctx = abi.Context(320, 240)
+    ctx.load(scene)
+    ctx.set_camera(abi.look_at((0.4, 0.2, -2.6)))
+    ctx.render()
+    img = ctx.rgba8()
+    say("[autotune context]")
+    best = ctx.autotune(reps=2)
+    assert best.frame_ms > 0
+    assert np.array_equal(ctx.rgba8(), img)
+    ctx.close()
