@@ -29,6 +29,8 @@ uint32_t bits_for(uint64_t count) {
   return b;
 }
 
+uint64_t div_up_u64(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
 uint32_t initial_pair_capacity(uint64_t splats) {
   return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * splats, 1u << 20), 1u << 30);
 }
@@ -65,6 +67,7 @@ uint64_t frame_working_bytes(uint64_t slots, uint64_t pixels, uint64_t tiles, ui
 Pipeline::Pipeline() {
   if (!device_present()) throw Error(Code::Unsupported, "gpu rasterizer unavailable");
   if (const char *e = std::getenv("GSPLAT_COUNT_EVALS")) count_evals_ = std::atoi(e) != 0;
+  if (const char *e = std::getenv("GSPLAT_GRAPH")) use_graph_ = std::atoi(e) != 0;
   if (const char *e = std::getenv("GSPLAT_BIN_SHIFT")) set_bin_shift((uint32_t)std::atoi(e));
   if (const char *e = std::getenv("GSPLAT_SORT_RTS_MIN_TILES"))
     tuning_.rts_min_tiles = std::max<uint32_t>((uint32_t)std::strtoul(e, nullptr, 10), 1u);
@@ -87,6 +90,7 @@ Pipeline::Pipeline() {
 
 Pipeline::~Pipeline() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  for (GraphSlot &g : graphs_) drop_graph(g);
   for (auto &e : ev_)
     if (e) (void)hipEventDestroy(e);
   for (int r = 0; r < 2; ++r) {
@@ -106,9 +110,15 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
                                            std::to_string(kMaxViews));
   // A finished earlier frame that overflowed only grows the pair buffers:
   // its results are replaced by this frame.
-  if (pending_ && hipEventQuery(ev_[5]) == hipSuccess) {
-    absorb_summary();
-    pending_ = false;
+  if (pending_) {
+    const hipError_t q = hipEventQuery(ev_[5]);
+    if (q == hipSuccess) {
+      absorb_summary();
+      pending_ = false;
+    } else {
+      (void)hipGetLastError();  // "not ready" is no error of the next launch
+      if (q != hipErrorNotReady) GS_HIP(q);
+    }
   }
   const bool multi = views.size() > 1;
   std::vector<ViewDev> table(views.size());
@@ -187,12 +197,16 @@ void Pipeline::launch_frame() {
   if (pair_scratch_.ensure_zeroed(onesweep_status_words(pair_cap_, tile_bits))) pair_clean_ = true;
   const bool depth_lookback = radix_uses_lookback((uint32_t)m, 0, tuning_.rts_min_tiles);
   // The last frame's pair count (+25%) sizes the pair sort's grids and mode.
-  const uint32_t hint = pairs_ ? (uint32_t)std::min<uint64_t>(pairs_ + pairs_ / 4, pair_cap_) : pair_cap_;
+  uint32_t hint = pairs_ ? (uint32_t)std::min<uint64_t>(pairs_ + pairs_ / 4, pair_cap_) : pair_cap_;
+  // A graph replays the grids it was captured with: round the hint up to a
+  // power of two sort tiles so that moving cameras re-capture rarely.
+  const bool as_graph = use_graph_ && host_rgba8_.empty();
+  if (as_graph)
+    hint = (uint32_t)std::min<uint64_t>((uint64_t)kSortTile << bits_for(div_up_u64(hint, kSortTile)), pair_cap_);
   const bool pair_lookback = radix_uses_lookback(pair_cap_, hint, tuning_.rts_min_tiles);
   const size_t depth_dirty = radix_lookback_words((uint32_t)m, radix_passes(32, true, value_byte).count);
   const size_t pair_dirty = radix_lookback_words(pair_cap_, radix_passes(tile_bits, false, -1).count);
 
-  hipStream_t st = stream_;
   // View table through a pinned staging slot whose previous upload is done.
   const int r = (int)(frame_counter_++ % 2);
   GS_HIP(hipEventSynchronize(table_ev_[r]));
@@ -204,9 +218,98 @@ void Pipeline::launch_frame() {
     tile_start[v] = host_views_[v].tile_off;
   }
   std::memcpy(host + 2 * kMaxViews * sizeof(uint32_t), host_views_.data(), nviews * sizeof(ViewDev));
-  GS_HIP(hipMemcpyAsync(table_.get(), host, 2 * kMaxViews * sizeof(uint32_t) + nviews * sizeof(ViewDev),
-                        hipMemcpyHostToDevice, st));
-  GS_HIP(hipEventRecord(table_ev_[r], st));
+
+  FramePlan plan;
+  plan.slot = r;
+  plan.nviews = nviews;
+  plan.slots = slots;
+  plan.tiles = tiles;
+  plan.ctiles = ctiles;
+  plan.tile_bits = tile_bits;
+  plan.value_byte = value_byte;
+  plan.count_blocks = count_blocks;
+  plan.hint = hint;
+  plan.depth_lookback = depth_lookback;
+  plan.pair_lookback = pair_lookback;
+  plan.depth_dirty = depth_dirty;
+  plan.pair_dirty = pair_dirty;
+  if (as_graph) {
+    // The frame as one graph per staging slot, captured again whenever
+    // anything but the view table's contents changes.
+    std::vector<uint64_t> key = {(uint64_t)r, nviews, slots, tiles, ctiles, tile_bits, (uint64_t)(value_byte + 1),
+                                 pair_cap_, hint, tuning_.bin_shift,
+                                 (uint64_t)tuning_.raster, tuning_.rts_min_tiles, count_evals_, depth_clean_,
+                                 pair_clean_};
+    for (const ViewDev &d : host_views_) {
+      key.push_back(reinterpret_cast<uintptr_t>(d.soa));
+      key.push_back(((uint64_t)d.n << 8) | d.sh_degree);
+      key.push_back(((uint64_t)d.fp.width << 32) | d.fp.height);
+    }
+    for (const void *ptr : {(const void *)keys_.get(), (const void *)vals_.get(), (const void *)keys_alt_.get(),
+                            (const void *)vals_alt_.get(), (const void *)rec_.get(), (const void *)rect_.get(),
+                            (const void *)counts_.get(), (const void *)offsets_.get(),
+                            (const void *)block_sums_.get(), (const void *)block_pairs_.get(),
+                            (const void *)img32_.get(), (const void *)img8_.get(), (const void *)ranges_.get(),
+                            (const void *)pair_keys_.get(), (const void *)pair_vals_.get(),
+                            (const void *)pair_keys_alt_.get(), (const void *)pair_vals_alt_.get(),
+                            (const void *)depth_scratch_.get(), (const void *)pair_scratch_.get()})
+      key.push_back(reinterpret_cast<uintptr_t>(ptr));
+    GraphSlot &g = graphs_[r];
+    if (!g.exec || g.key != key) {
+      drop_graph(g);
+      GS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      try {
+        record_frame(plan, true);
+      } catch (...) {
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(stream_, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+        throw;
+      }
+      GS_HIP(hipStreamEndCapture(stream_, &g.graph));
+      GS_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+      g.key = std::move(key);
+      g.sorted_keys = sorted_keys_;
+      g.sorted_vals = sorted_vals_;
+      g.depth_clean = depth_clean_;
+      g.pair_clean = pair_clean_;
+    }
+    GS_HIP(hipEventRecord(ev_[0], stream_));
+    GS_HIP(hipGraphLaunch(g.exec, stream_));
+    for (int e = 1; e <= 4; ++e) GS_HIP(hipEventRecord(ev_[e], stream_));
+    GS_HIP(hipEventRecord(table_ev_[r], stream_));
+    GS_HIP(hipEventRecord(ev_[5], stream_));
+    staged_times_ = false;
+    sorted_keys_ = g.sorted_keys;
+    sorted_vals_ = g.sorted_vals;
+    depth_clean_ = g.depth_clean;
+    pair_clean_ = g.pair_clean;
+  } else {
+    record_frame(plan, false);
+    staged_times_ = true;
+  }
+  pending_ = true;
+  summary_valid_ = false;
+}
+
+void Pipeline::drop_graph(GraphSlot &g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g.exec = nullptr;
+  g.graph = nullptr;
+  g.key.clear();
+}
+
+void Pipeline::record_frame(const FramePlan &plan, bool captured) {
+  const uint32_t nviews = plan.nviews, slots = plan.slots, tiles = plan.tiles, ctiles = plan.ctiles;
+  const uint32_t tile_bits = plan.tile_bits, count_blocks = plan.count_blocks, hint = plan.hint;
+  const int value_byte = plan.value_byte, r = plan.slot;
+  const bool depth_lookback = plan.depth_lookback, pair_lookback = plan.pair_lookback;
+  const size_t depth_dirty = plan.depth_dirty, pair_dirty = plan.pair_dirty;
+  hipStream_t st = stream_;
+  GS_HIP(hipMemcpyAsync(table_.get(), table_pinned_[r],
+                        2 * kMaxViews * sizeof(uint32_t) + nviews * sizeof(ViewDev), hipMemcpyHostToDevice, st));
+  if (!captured) GS_HIP(hipEventRecord(table_ev_[r], st));
   const uint32_t *seg_start_d = reinterpret_cast<const uint32_t *>(table_.get());
   const uint32_t *tile_start_d = seg_start_d + kMaxViews;
   const ViewDev *views_d = reinterpret_cast<const ViewDev *>(table_.get() + 2 * kMaxViews * sizeof(uint32_t));
@@ -214,7 +317,7 @@ void Pipeline::launch_frame() {
   uint32_t *visible = info + kVisibleWord;
   unsigned long long *raster_pairs = reinterpret_cast<unsigned long long *>(info + kRasterPairsWord);
 
-  GS_HIP(hipEventRecord(ev_[0], st));
+  if (!captured) GS_HIP(hipEventRecord(ev_[0], st));
   // The projection zeroes the summary words and the binning tile ranges of
   // this frame.
   Clears frame_clears;
@@ -231,7 +334,7 @@ void Pipeline::launch_frame() {
                      frame_clears, st);
       frame_clears = Clears();  // once
     }
-  GS_HIP(hipEventRecord(ev_[1], st));
+  if (!captured) GS_HIP(hipEventRecord(ev_[1], st));
 
   uint32_t *k = keys_.get(), *v = vals_.get(), *ka = keys_alt_.get(), *va = vals_alt_.get();
   SortPrep depth_prep;
@@ -241,7 +344,7 @@ void Pipeline::launch_frame() {
                 depth_scratch_.capacity(), st, value_byte, 0, depth_prep);
   sorted_keys_ = k;
   sorted_vals_ = v;
-  GS_HIP(hipEventRecord(ev_[2], st));
+  if (!captured) GS_HIP(hipEventRecord(ev_[2], st));
 
   Clears scan_clears;  // the depth sort's scratch, for the next frame
   if (depth_lookback) scan_clears.add(depth_scratch_.get(), depth_dirty);
@@ -263,12 +366,12 @@ void Pipeline::launch_frame() {
   pair_clean_ = pair_lookback;
   if (count_evals_) range_clears.add(info + kEvalsWord, 4);
   launch_tile_ranges(views_d, pk, m_dev_.get(), pair_cap_, ranges_.get(), range_clears, st);
-  GS_HIP(hipEventRecord(ev_[3], st));
+  if (!captured) GS_HIP(hipEventRecord(ev_[3], st));
 
   unsigned long long *evals = count_evals_ ? reinterpret_cast<unsigned long long *>(info + kEvalsWord) : nullptr;
   launch_raster(tuning_.raster, views_d, tile_start_d, nviews, tiles, tuning_.bin_shift, rec_.get(), rect_.get(),
                 ranges_.get(), pv, img32_.get(), img8_.get(), st, evals);
-  GS_HIP(hipEventRecord(ev_[4], st));
+  if (!captured) GS_HIP(hipEventRecord(ev_[4], st));
   for (size_t i = 0; i < host_rgba8_.size(); ++i) {
     if (!host_rgba8_[i]) continue;
     const ViewDev &d = host_views_[i];
@@ -277,9 +380,7 @@ void Pipeline::launch_frame() {
   }
   const size_t summary_words = count_evals_ ? kSummaryWords : kVisibleWord + nviews;
   GS_HIP(hipMemcpyAsync(summary_pinned_, info, summary_words * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  GS_HIP(hipEventRecord(ev_[5], st));
-  pending_ = true;
-  summary_valid_ = false;
+  if (!captured) GS_HIP(hipEventRecord(ev_[5], st));
 }
 
 bool Pipeline::absorb_summary() {
@@ -307,7 +408,8 @@ void Pipeline::wait() {
       continue;
     }
     float ms[4] = {0, 0, 0, 0}, total = 0;
-    for (int s = 0; s < 4; ++s) GS_HIP(hipEventElapsedTime(&ms[s], ev_[s], ev_[s + 1]));
+    if (staged_times_)
+      for (int s = 0; s < 4; ++s) GS_HIP(hipEventElapsedTime(&ms[s], ev_[s], ev_[s + 1]));
     GS_HIP(hipEventElapsedTime(&total, ev_[0], ev_[4]));
     uint64_t evals[2] = {0, 0};
     if (count_evals_) std::memcpy(evals, summary_pinned_ + kEvalsWord, sizeof(evals));

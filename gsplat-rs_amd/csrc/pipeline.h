@@ -18,8 +18,10 @@ namespace gs {
 #define GS_HIP(call)                                                                   \
   do {                                                                                 \
     hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess)                                                              \
+    if (e_ != hipSuccess) {                                                            \
+      (void)hipGetLastError(); /* a later launch check must not see this error */     \
       throw Error(Code::Internal, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    }                                                                                  \
   } while (0)
 
 // Grow-only device allocation. Releasing waits for the device, since work
@@ -163,6 +165,12 @@ class Pipeline {
   void export_projection(const DeviceScene &scene, const FrameParams &fp, uint32_t *keys_out, float *proj_out);
 
   void set_raster_variant(RasterVariant v) { tuning_.raster = v; }
+  // Graph mode (also GSPLAT_GRAPH=1): each frame is replayed from a hipGraph
+  // captured for the batch's configuration (views, sizes, scenes, buffers,
+  // tuning); only the view table (cameras) is uploaded per frame. Frames with
+  // host RGBA8 targets always launch directly.
+  void set_use_graph(bool on) { use_graph_ = on; }
+  bool use_graph() const { return use_graph_; }
   // Counting mode (also GSPLAT_COUNT_EVALS=1): the raster counts the
   // (pixel, splat) evaluations that pass coverage before each pixel's early
   // stop; the count is reported with the frame times.
@@ -181,7 +189,25 @@ class Pipeline {
   hipStream_t stream() const { return stream_; }
 
  private:
+  struct FramePlan {
+    int slot = 0;  // pinned view-table staging slot
+    uint32_t nviews = 0, slots = 0, tiles = 0, ctiles = 0, tile_bits = 0, count_blocks = 0, hint = 0;
+    int value_byte = -1;
+    bool depth_lookback = false, pair_lookback = false;
+    size_t depth_dirty = 0, pair_dirty = 0;
+  };
+  struct GraphSlot {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> key;
+    uint32_t *sorted_keys = nullptr, *sorted_vals = nullptr;
+    bool depth_clean = false, pair_clean = false;
+  };
   void launch_frame();
+  // Every stream operation of a frame; `captured` leaves out the event
+  // records (an event recorded under capture cannot be timed or waited on).
+  void record_frame(const FramePlan &plan, bool captured);
+  void drop_graph(GraphSlot &g);
   bool absorb_summary();  // true when the frame's pairs overflowed
   void check_view(uint32_t v) const;
 
@@ -223,6 +249,9 @@ class Pipeline {
   Tuning tuning_{};
   bool count_evals_ = false;
   bool depth_clean_ = false, pair_clean_ = false;  // look-back scratch words are zero
+  bool use_graph_ = false;
+  GraphSlot graphs_[2];
+  bool staged_times_ = true;  // false: the last frame ran as a graph, total time only
 
 };
 

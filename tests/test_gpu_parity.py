@@ -318,3 +318,42 @@ def test_gpu_raster_variants_bit_identical(gpu, monkeypatch):
         assert np.array_equal(got[1], base[1]), f"{key} rgba8"
         assert np.array_equal(got[2], base[2]), f"{key} tile counts"
         assert got[3] == base[3] == base[2].sum(), f"{key} pairs"
+
+def test_gpu_graph_replays_match_eager(gpu, monkeypatch):
+    # Graph mode replays a captured frame with only the cameras uploaded; each
+    # replay (with eager frames, readbacks and waits in between) must equal
+    # the eager frame of the same camera, and a size change must re-capture.
+    scene = synth.tiny_scene(30000, 2024, sh_degree=3)
+    cams = [gpu.look_at((0.3, -0.2, -2.8)), gpu.look_at((2.0, 1.0, 1.5)), gpu.look_at((-1.5, 2.0, -1.0))]
+    sizes = [(320, 240), (256, 144)]
+    monkeypatch.delenv("GSPLAT_GRAPH", raising=False)
+    eager = {}
+    ctx = gpu.Context(320, 240)
+    ctx.load(scene)
+    for w, h in sizes:
+        ctx.set_size(w, h)
+        for i, cam in enumerate(cams):
+            ctx.set_camera(cam)
+            ctx.render()
+            eager[(w, h, i)] = (ctx.rgba32f().view(np.uint32), ctx.stats().visible_count, ctx.tile_counts())
+    ctx.close()
+    monkeypatch.setenv("GSPLAT_GRAPH", "1")
+    g = gpu.Context(320, 240)
+    g.load(scene)
+    for w, h in sizes:
+        g.set_size(w, h)
+        for rep in range(3):
+            for i, cam in enumerate(cams):
+                say(f"[graph] {w}x{h} rep {rep} camera {i}")
+                g.set_camera(cam)
+                g.render()
+                img, vis, tiles = eager[(w, h, i)]
+                assert np.array_equal(g.rgba32f().view(np.uint32), img), (w, h, rep, i)
+                assert g.stats().visible_count == vis
+                assert np.array_equal(g.tile_counts(), tiles)
+        # back-to-back replays without waits, then the last one is checked
+        for i in (0, 1, 2, 0):
+            g.set_camera(cams[i])
+            g.render()
+        assert np.array_equal(g.rgba32f().view(np.uint32), eager[(w, h, 0)][0])
+    g.close()
