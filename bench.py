@@ -170,18 +170,42 @@ def _kernel_short(name):
     return n.split("(")[0].replace("gs::", "")
 
 
-# Kernels of each plain-frame stage (one frame launches each listed kernel one or more times).
-STAGE_KERNELS = {
-    "k_project": ["k_project<3>"],
-    "k_os_pass": ["k_os_clear", "k_os_hist", "k_os_pass"],
-    "bin(pair sort+emit)": ["k_tile_counts", "k_scan_blocks", "k_scan_partials", "k_scan_add", "k_pairs_finalize",
-                            "k_tile_emit", "k_rts_up", "k_rts_down", "k_tile_ranges"],
-    "k_raster": ["k_raster<false, 1>"],
-}
+# Stages of a frame, in launch order. A dispatch belongs to the stage of the
+# latest stage-opening kernel before it: k_project opens "project", the first
+# radix kernel after it the depth sort, k_tile_counts the binning stage (tile
+# counts, block scan, pair emission, pair radix sort, tile ranges), k_raster
+# the raster. Sort kernels are shared by the depth and pair sorts, so launch
+# order (not the name) tells them apart.
+STAGES = ["project", "sort", "bin", "raster"]
+STAGE_KERNEL = {"project": "k_project", "sort": "depth radix sort", "bin": "bin(pair sort+emit)",
+                "raster": "k_raster"}
+
+
+def _stage_sequence(rows):
+    """(stage, row) per dispatch of CSV rows, in dispatch order; the counting raster is its own stage."""
+    out, stage = [], None
+    for r in sorted(rows, key=lambda r: int(r["Dispatch_Id"])):
+        k = _kernel_short(r["Kernel_Name"])
+        if k.startswith("k_project"):
+            stage = "project"
+        elif stage == "project" and (k.startswith("k_os_") or k.startswith("k_rts_") or k.startswith("k_scan_")):
+            stage = "sort"
+        elif k == "k_tile_counts":
+            stage = "bin"
+        elif k.startswith("k_raster<true"):
+            out.append(("counting", r))
+            stage = None
+            continue
+        elif k.startswith("k_raster"):
+            stage = "raster"
+        out.append((stage, r))
+        if k.startswith("k_raster"):
+            stage = None
+    return out
 
 
 def summarize_profiles(trace_dir, fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, frames, out_dir):
-    """rocprofv3 CSVs of the plain workload -> profiles/ summaries (kernel stats, HBM traffic per frame)."""
+    """rocprofv3 CSVs of the plain workload -> profiles/ summaries (kernel stats, stage times, HBM traffic)."""
     import csv
     import collections
 
@@ -189,33 +213,47 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, copy_fetch_dir, copy_wri
         path = next(pathlib.Path(d).glob(f"*{suffix}"))
         return list(csv.DictReader(open(path)))
 
-    def per_kernel(d):
-        acc = collections.defaultdict(list)
-        for r in rows(d, "counter_collection.csv"):
-            acc[_kernel_short(r["Kernel_Name"])].append(float(r["Counter_Value"]) * 1024.0)
-        return acc
+    def copies_of(d):
+        return [float(r["Counter_Value"]) * 1024.0 for r in rows(d, "counter_collection.csv")
+                if "copyBuffer" in r["Kernel_Name"]]
 
-    cf = sum(float(r["Counter_Value"]) for r in rows(copy_fetch_dir, "counter_collection.csv")
-             if "copyBuffer" in r["Kernel_Name"]) * 1024.0
-    cw = sum(float(r["Counter_Value"]) for r in rows(copy_write_dir, "counter_collection.csv")
-             if "copyBuffer" in r["Kernel_Name"]) * 1024.0
-    copies = sum(1 for r in rows(copy_fetch_dir, "counter_collection.csv") if "copyBuffer" in r["Kernel_Name"])
-    known = copies * (1024 << 20)
-    ff, wf = known / cf, known / cw
-    fetch, write = per_kernel(fetch_dir), per_kernel(write_dir)
-    launches = {k: len(v) for k, v in fetch.items()}
-    # Frames profiled: the counting-variant frame is not a plain frame; k_project<3> runs once per frame.
-    nframes = launches.get("k_project<3>", frames)
+    cf, cw = copies_of(copy_fetch_dir), copies_of(copy_write_dir)
+    known = len(cf) * (1024 << 20)
+    ff, wf = known / sum(cf), known / sum(cw)
+
+    def per_stage_counter(d):
+        acc, kern = collections.defaultdict(float), collections.defaultdict(set)
+        nframes = 0
+        for stage, r in _stage_sequence(rows(d, "counter_collection.csv")):
+            if stage in (None, "counting"):
+                continue
+            k = _kernel_short(r["Kernel_Name"])
+            nframes += k.startswith("k_project")
+            acc[stage] += float(r["Counter_Value"]) * 1024.0
+            kern[stage].add(k)
+        return acc, kern, max(nframes, 1)
+
+    fetch, kern, nf = per_stage_counter(fetch_dir)
+    write, _, nw = per_stage_counter(write_dir)
     kernels = {}
-    for stage, ks in STAGE_KERNELS.items():
-        # a one-kernel stage: per launch of that kernel; else per frame
-        div = launches.get(ks[0], nframes) if len(ks) == 1 else nframes
-        fb = sum(sum(fetch.get(k, [])) for k in ks) / max(div, 1) * ff
-        wb = sum(sum(write.get(k, [])) for k in ks) / max(div, 1) * wf
-        kernels[stage] = {"fetch_bytes": int(fb), "write_bytes": int(wb), "bytes_per_launch": int(fb + wb),
-                          "kernels": ks}
-    out = {"workload": "plain-1080p-300k-sh3", "frames": nframes, "fetch_scale": round(ff, 4),
-           "write_scale": round(wf, 4), "kernels": kernels,
+    for stage in STAGES:
+        fb, wb = fetch[stage] / nf * ff, write[stage] / nw * wf
+        kernels[STAGE_KERNEL[stage]] = {"fetch_bytes": int(fb), "write_bytes": int(wb),
+                                        "bytes_per_launch": int(fb + wb), "kernels": sorted(kern[stage])}
+    # Device time per stage and frame from the kernel trace (sum of its dispatches' durations).
+    times, counts = collections.defaultdict(float), collections.Counter()
+    tframes = 0
+    for stage, r in _stage_sequence(rows(trace_dir, "kernel_trace.csv")):
+        if stage is None:
+            continue
+        k = _kernel_short(r["Kernel_Name"])
+        tframes += k.startswith("k_project")
+        times[stage] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+        counts[stage] += 1
+    tframes = max(tframes - counts["counting"], 1)  # the counting frame's raster is reported apart
+    stage_us = {STAGE_KERNEL[s]: round(times[s] / tframes, 2) for s in STAGES}
+    out = {"workload": "plain-1080p-300k-sh3", "frames": nf, "fetch_scale": round(ff, 4),
+           "write_scale": round(wf, 4), "kernels": kernels, "stage_us_per_frame": stage_us,
            "commands": ["rocprofv3 --pmc FETCH_SIZE --output-format csv -d DIR -o plain -- python3 bench.py "
                         "--plain-only --steps 8 --warmup 2 --no-cpu-baseline",
                         "rocprofv3 --pmc WRITE_SIZE (same)",
@@ -233,12 +271,17 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, copy_fetch_dir, copy_wri
     for r in sorted(stats, key=lambda r: -float(r["Percentage"])):
         lines.append(f"| {_kernel_short(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
                      f"{float(r['Percentage']):.2f} |")
+    lines += ["", f"# Device time per stage and frame ({tframes} frames, kernel trace, launch order)", "",
+              "| stage | kernels | us per frame |", "|---|---|---:|"]
+    for stage in STAGES:
+        lines.append(f"| {STAGE_KERNEL[stage]} | {', '.join(sorted(kern[stage]))} | "
+                     f"{stage_us[STAGE_KERNEL[stage]]:.2f} |")
     lines += ["", "# HBM traffic per frame (rocprofv3 --pmc, two runs)", "",
               f"FETCH_SIZE scaled by {ff:.3f} and WRITE_SIZE by {wf:.3f}: the factors that make the counters read",
-              f"{copies} plain device-to-device copies of 1 GiB (`bench.py --copy-calibration 1024`) as 1 GiB read",
+              f"{len(cf)} plain device-to-device copies of 1 GiB (`bench.py --copy-calibration 1024`) as 1 GiB read",
               "and 1 GiB written.", "", "| stage | kernels | fetch MB | write MB | total MB |", "|---|---|---:|---:|---:|"]
-    for stage, v in kernels.items():
-        lines.append(f"| {stage} | {', '.join(v['kernels'])} | {v['fetch_bytes'] / 1e6:.1f} | "
+    for name, v in kernels.items():
+        lines.append(f"| {name} | {', '.join(v['kernels'])} | {v['fetch_bytes'] / 1e6:.1f} | "
                      f"{v['write_bytes'] / 1e6:.1f} | {v['bytes_per_launch'] / 1e6:.1f} |")
     (out_dir / "kernels_plain-1080p-300k-sh3.md").write_text("\n".join(lines) + "\n")
     print(json.dumps(out))
@@ -350,9 +393,10 @@ def main():
                     help="only run device-to-device copies of MIB MiB (PMC counter calibration)")
     ap.add_argument("--summarize-profiles", nargs=5, metavar=("TRACE", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="write profiles/ summaries from rocprofv3 output directories and exit")
+    ap.add_argument("--profiles-out", default=str(ROOT / "profiles"), help="folder for --summarize-profiles")
     args = ap.parse_args()
     if args.summarize_profiles:
-        summarize_profiles(*args.summarize_profiles, frames=0, out_dir=ROOT / "profiles")
+        summarize_profiles(*args.summarize_profiles, frames=0, out_dir=args.profiles_out)
         return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -426,7 +470,7 @@ def main():
     rooflines = {
         "k_project": {"bound": "hbm", "achieved": proj_bytes / (med[0] * 1e-3) / 1e9, "peak": MI355X_HBM_GBPS,
                       "unit": "GB/s", "algorithmic": proj_bytes},
-        "k_os_pass": {"bound": "hbm", "achieved": sort_bytes / (med[1] * 1e-3) / 1e9,
+        "depth radix sort": {"bound": "hbm", "achieved": sort_bytes / (med[1] * 1e-3) / 1e9,
                              "peak": MI355X_HBM_GBPS, "unit": "GB/s", "algorithmic": sort_bytes},
         "bin(pair sort+emit)": {"bound": "hbm", "achieved": pair_bytes / (med[2] * 1e-3) / 1e9,
                                 "peak": MI355X_HBM_GBPS, "unit": "GB/s", "algorithmic": pair_bytes},
@@ -437,9 +481,7 @@ def main():
         r["achieved"] = round(r["achieved"], 3)
         r["frac"] = round(r["achieved"] / r["peak"], 4)
         r["traffic"] = traffic.get(name)
-    dom_kernel = {"project": "k_project", "sort": "k_os_pass", "bin": "bin(pair sort+emit)",
-                  "raster": "k_raster"}[
-        names[dom]]
+    dom_kernel = STAGE_KERNEL[names[dom]]
     roof = dict(rooflines[dom_kernel])
     roof = {"kernel": dom_kernel, "bound": roof["bound"], "achieved": roof["achieved"],
             "peak": roof["peak"], "unit": roof["unit"], "frac": roof["frac"], "traffic": roof["traffic"]}
