@@ -442,6 +442,10 @@ def main():
 
     n, v = st.splat_count, st.visible_count
     pairs = int(st.tile_pairs)
+    bin_pairs = int(st.bin_pairs)
+    bin_px = 16 << ctx.tuning().bin_shift
+    bin_tiles = -(-view.width // bin_px) * -(-view.height // bin_px)
+    pair_passes = -(-max((bin_tiles - 1).bit_length(), 1) // 8)
     ctx.close()
 
     # One counting frame (outside the timed region) for the blend's algorithmic work E.
@@ -463,8 +467,9 @@ def main():
     #  raster   16 FLOP per (pixel, splat) evaluation passing coverage before the pixel's early stop
     proj_bytes = 12 * n + 8 * n + (24 + 4 + 4 + 12 + 4 * k + 48 + 8) * v
     sort_bytes = 4 * 16 * n
-    #  bin      rect read per visible splat, pair writes, then 2 passes x (key+value read and write) over M
-    pair_bytes = 8 * v + 8 * pairs + 2 * 16 * pairs
+    #  bin      value + rect read per visible splat (counting, then emission), (binning tile, splat) pair
+    #           writes, pair radix passes x (key+value read and write), one key read for the tile ranges
+    pair_bytes = 2 * 12 * v + 8 * bin_pairs + pair_passes * 16 * bin_pairs + 4 * bin_pairs
     raster_flop = 16 * evals
     traffic = load_traffic()
     rooflines = {
@@ -499,7 +504,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded 3-component Gaussian mixture, SH3)",
         "config": {"workload": "plain-1080p-300k-sh3", "splats": n, "width": view.width, "height": view.height,
-                   "visible": v, "tile_pairs": pairs, "blend_evals": evals, "blend_tests": tests},
+                   "visible": v, "tile_pairs": pairs, "bin_pairs": bin_pairs, "bin_tile_px": bin_px,
+                   "pair_sort_passes": pair_passes, "blend_evals": evals, "blend_tests": tests},
         "latency_ms": {"p50": round(nearest_rank(lat, 0.5), 4), "p99": round(nearest_rank(lat, 0.99), 4)},
         "stage_ms": {nm: round(float(x), 4) for nm, x in zip(names, med[:4])},
         "dominant_stage": names[dom],

@@ -798,6 +798,7 @@ int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out) 
     out->tile_pairs = batch->pipe.pairs();
     out->blend_evals = t.blend_evals;
     out->blend_tests = t.blend_tests;
+    out->bin_pairs = batch->pipe.bin_pairs();
     return ok();
   });
 }
@@ -837,6 +838,16 @@ int32_t gsplat_batch_autotune(GsplatBatch *batch, const GsplatViewDesc *views, u
     for (const gs::TuneSample &s : samples)
       if (s.frame_ms < best.frame_ms) best = s;
     *out_best = tuning_to(best.tuning, best.frame_ms);
+    return ok();
+  });
+}
+
+int32_t gsplat_context_get_tuning(GsplatContext *ctx, GsplatTuning *out_tuning) {
+  static const char *op = "gsplat_context_get_tuning";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!out_tuning) return null_arg(op, "out_tuning");
+    *out_tuning = tuning_to(ctx->renderer->tuning(), 0.0f);
     return ok();
   });
 }

@@ -444,20 +444,38 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const ViewDev *__restr
   const uint32_t v = find_view(seg_start, nviews, slot_lo);
   const ViewDev &vw = views[v];
   const uint32_t r0 = b * kCountBlock + t * 4;
+  // Loads first (keys, values, then the rects they name), so that each
+  // level's memory latency is paid once rather than once per item.
+  uint32_t key[4], local[4];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint32_t g = total - 1 - (r0 + k);
+    key[k] = skeys[g];
+    local[k] = svals[g] - (v << kLocalBits);
+  }
+  // The key after item k in slot order is item k-1's; after item 0 it is
+  // the previous thread's (or block's) last item.
+  const uint32_t g_first = total - 1 - r0;
+  const uint32_t next0 = g_first + 1 < total ? skeys[g_first + 1] : 0u;
+  const uint32_t n_v = vw.n, off_v = vw.seg_off;
+  bool live[4];
+  uint2 rc[4];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    live[k] = key[k] != 0u && local[k] < n_v;
+    rc[k] = rect[live[k] ? off_v + local[k] : 0u];  // unconditional: the four loads issue together
+  }
   uint32_t c[4];
   uint32_t sum = 0, fine = 0;
 #pragma unroll
   for (uint32_t k = 0; k < 4; ++k) {
-    const uint32_t g = total - 1 - (r0 + k);
-    const uint32_t key = skeys[g];
-    const uint32_t local = svals[g] - (v << kLocalBits);
     uint32_t cc = 0;
-    if (key != 0u && local < vw.n) {
-      const uint2 r = rect[vw.seg_off + local];
-      cc = rect_tiles(r, shift);
-      fine += rect_tiles(r, 0);
-      const uint32_t j = g - vw.seg_off;
-      if (j + 1 == vw.seg_len || skeys[g + 1] == 0u) visible[v] = j + 1;
+    if (live[k]) {
+      cc = rect_tiles(rc[k], shift);
+      fine += rect_tiles(rc[k], 0);
+      const uint32_t j = total - 1 - (r0 + k) - off_v;
+      const uint32_t next = k == 0 ? next0 : key[k - 1];
+      if (j + 1 == vw.seg_len || next == 0u) visible[v] = j + 1;
     }
     c[k] = cc;
     sum += cc;
@@ -539,10 +557,10 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
   const uint32_t r = total - 1 - (g0 + kThreads - 1 - t);
   const uint32_t cnt = counts_rev[r];
   const uint32_t first = block_sums[r / kCountBlock] + offsets_rev[r];
+  const uint32_t slot = vw.seg_off + (svals[g0 + kThreads - 1 - t] - (v << kLocalBits));  // loaded beside cnt
   s_first[t] = first;
   if (t == kThreads - 1) s_first[kThreads] = first + cnt;
   if (cnt) {
-    const uint32_t slot = vw.seg_off + (svals[g0 + kThreads - 1 - t] - (v << kLocalBits));
     const uint2 b = rect[slot];
     const uint32_t sh = 4 + shift;
     const uint32_t x0 = (b.x & 0xFFFFu) >> sh, x1 = (b.y & 0xFFFFu) >> sh;

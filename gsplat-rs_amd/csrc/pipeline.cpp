@@ -23,6 +23,8 @@ constexpr size_t kVisibleWord = 6;
 constexpr size_t kEvalsWord = kVisibleWord + kMaxViews;
 constexpr size_t kSummaryWords = kEvalsWord + 4;
 
+static_assert(kSegmentAlign % kSortTile == 0, "view segments must be whole sort tiles");
+
 uint32_t bits_for(uint64_t count) {
   uint32_t b = 1;
   while (b < 32 && (1ull << b) < count) ++b;
@@ -340,6 +342,8 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   SortPrep depth_prep;
   depth_prep.scratch_clean = depth_clean_;
   depth_prep.rts_min_tiles = tuning_.rts_min_tiles;
+  depth_prep.seg_start = seg_start_d;  // view v's slots: segment v, value byte 3 = v
+  depth_prep.segs = nviews;
   onesweep_sort(&k, &v, &ka, &va, slots, nullptr, 32, /*descending=*/true, depth_scratch_.get(),
                 depth_scratch_.capacity(), st, value_byte, 0, depth_prep);
   sorted_keys_ = k;

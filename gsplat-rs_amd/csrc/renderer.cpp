@@ -101,6 +101,7 @@ GsplatStageTimes Renderer::stage_times() {
   s.tile_pairs = pipe_.pairs();
   s.blend_evals = t.blend_evals;
   s.blend_tests = t.blend_tests;
+  s.bin_pairs = pipe_.bin_pairs();
   return s;
 }
 

@@ -49,6 +49,14 @@ struct SortPrep {
   bool scratch_clean = false;  // the look-back words are zero: skip the clear
   bool hist_ready = false;     // the digit histograms are accumulated: skip k_os_hist
   uint32_t rts_min_tiles = 0;  // mode threshold, as in radix_uses_lookback
+  // Segments (device array of `segs` element offsets, ascending, each a
+  // multiple of kSortTile, the first 0; the last segment ends at n_cap).
+  // When the value pass orders by the segment index (the element's value
+  // byte names its segment, as for a batch of views), reduce-then-scan
+  // passes sort every segment in place instead and leave the value pass
+  // out: the same result in one pass fewer. Look-back passes ignore them.
+  const uint32_t *seg_start = nullptr;
+  uint32_t segs = 0;
 };
 
 // u32 words of scratch needed to sort up to n_cap keys on `bits` key bits

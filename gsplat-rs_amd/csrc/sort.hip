@@ -68,6 +68,7 @@ constexpr uint32_t kMinGrid = 1024;   // floor of a hinted grid
 constexpr uint32_t FLAG_AGG = 1u << 30;  // tile count published
 constexpr uint32_t FLAG_INC = 2u << 30;  // inclusive prefix published
 constexpr uint32_t VALUE_MASK = (1u << 30) - 1;
+constexpr uint32_t XCDS = 8;  // MI355X: workgroup b is dispatched to XCD b % 8
 
 using PassSet = RadixPasses;
 
@@ -174,12 +175,19 @@ __device__ __forceinline__ uint32_t rank_tile(const uint32_t *__restrict__ kin, 
                                               uint32_t (&rank)[IPT], uint32_t (*whist)[256]) {
   const uint32_t t = threadIdx.x, w = t / 64, lane = __lane_id();
   uint32_t *wh = whist[w];
+  // All loads first: a round's LDS update would otherwise hold back the next
+  // round's loads, one memory latency per round.
 #pragma unroll
   for (uint32_t r = 0; r < IPT; ++r) {
     const uint32_t j = w * WAVE_ITEMS + r * 64 + lane;
     const bool active = j < count;
     key[r] = active ? kin[base + j] : 0u;
     val[r] = active ? vin[base + j] : 0u;
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < IPT; ++r) {
+    const uint32_t j = w * WAVE_ITEMS + r * 64 + lane;
+    const bool active = j < count;
     const uint32_t d = digit(key[r], val[r], shift, flip, from_value);
     const uint64_t peers = match_digit(d, active);
     const uint32_t lrank = (uint32_t)__popcll(peers & lanemask_lt());
@@ -319,21 +327,53 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
   }
 }
 
-// Reduce-then-scan pass, step 1: digit counts of every live tile,
-// digit-major (counts[d * tiles + tile], tiles = live tile count), so one
-// exclusive scan of the 256 * tiles counts (length published in *entries)
-// gives every (digit, tile) its global output start.
+// First tile of a reduce-then-scan workgroup (grid a multiple of XCDS):
+// each XCD takes one run of consecutive tiles, so the partial cache lines
+// that neighbouring tiles write into one digit's output (or into one
+// digit's row of counts) meet in the same L2.
+__device__ __forceinline__ uint32_t xcd_first_tile(uint32_t b, uint32_t grid) {
+  return (b % XCDS) * (grid / XCDS) + b / XCDS;
+}
+
+// Index of the (digit, tile) count. Unsegmented: digit-major over all tiles
+// (d * tiles + tile). Segmented: segment by segment, digit-major inside each
+// (first * 256 + d * seg_tiles + tile - first, `first` the segment's first
+// tile), so one flat exclusive scan sorts every segment in its own range.
+struct CountIndex {
+  uint32_t first, span;
+  __device__ __forceinline__ size_t operator()(uint32_t d, uint32_t tile) const {
+    return (size_t)first * 256 + (size_t)d * span + (tile - first);
+  }
+};
+
+__device__ __forceinline__ CountIndex count_index(uint32_t tile, uint32_t tiles, const uint32_t *seg_start,
+                                                  uint32_t segs) {
+  if (segs == 0) return CountIndex{0, tiles};
+  uint32_t lo = 0, hi = segs;  // last segment whose start <= the tile's first element
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (seg_start[mid] / TILE <= tile) lo = mid; else hi = mid;
+  }
+  const uint32_t first = seg_start[lo] / TILE;
+  const uint32_t end = lo + 1 < segs ? seg_start[lo + 1] / TILE : tiles;
+  return CountIndex{first, end - first};
+}
+
+// Reduce-then-scan pass, step 1: digit counts of every live tile laid out
+// by count_index, so one exclusive scan of the 256 * tiles counts (length
+// published in *entries) gives every (digit, tile) its output start.
 __global__ __launch_bounds__(T) void k_rts_up(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                               const uint32_t *__restrict__ n_dev, uint32_t n_cap, uint32_t shift,
                                               uint32_t flip, uint32_t from_value, uint32_t *__restrict__ counts,
-                                              uint32_t *__restrict__ entries) {
+                                              uint32_t *__restrict__ entries, const uint32_t *__restrict__ seg_start,
+                                              uint32_t segs) {
   __shared__ uint32_t h[WAVES][256];
   const uint32_t t = threadIdx.x, w = t / 64;
   const uint32_t n = load_count(n_dev, n_cap);
   const uint32_t tiles = (n + TILE - 1) / TILE;
   if (blockIdx.x == 0 && t == 0) *entries = 256 * tiles;
   const uint32_t *src = from_value ? vin : kin;
-  for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+  for (uint32_t tile = xcd_first_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x) {
 #pragma unroll
     for (uint32_t q = 0; q < WAVES; ++q) h[q][t] = 0;
     __syncthreads();
@@ -349,7 +389,7 @@ __global__ __launch_bounds__(T) void k_rts_up(const uint32_t *__restrict__ kin, 
     for (uint32_t r = 0; r < IPT; ++r)
       if (r * T + t < count) atomicAdd(&h[w][((x[r] ^ flip) >> shift) & 0xFFu], 1u);
     __syncthreads();
-    counts[(size_t)t * tiles + tile] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
+    counts[count_index(tile, tiles, seg_start, segs)(t, tile)] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
     __syncthreads();
   }
 }
@@ -359,7 +399,8 @@ __global__ __launch_bounds__(T) void k_rts_down(const uint32_t *__restrict__ kin
                                                 uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                 const uint32_t *__restrict__ n_dev, uint32_t n_cap, uint32_t shift,
                                                 uint32_t flip, uint32_t from_value,
-                                                const uint32_t *__restrict__ offsets) {
+                                                const uint32_t *__restrict__ offsets,
+                                                const uint32_t *__restrict__ seg_start, uint32_t segs) {
   __shared__ uint32_t lds4[WAVES];
   __shared__ uint32_t dst[256];
   __shared__ uint32_t lstart[256];
@@ -369,12 +410,12 @@ __global__ __launch_bounds__(T) void k_rts_down(const uint32_t *__restrict__ kin
   const uint32_t t = threadIdx.x;
   const uint32_t n = load_count(n_dev, n_cap);
   const uint32_t tiles = (n + TILE - 1) / TILE;
-  for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+  for (uint32_t tile = xcd_first_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x) {
     const size_t base = (size_t)tile * TILE;
     const uint32_t count = (uint32_t)min((size_t)TILE, (size_t)n - base);
 #pragma unroll
     for (uint32_t q = 0; q < WAVES; ++q) whist[q][t] = 0;
-    dst[t] = offsets[(size_t)t * tiles + tile];
+    dst[t] = offsets[count_index(tile, tiles, seg_start, segs)(t, tile)];
     __syncthreads();
     uint32_t key[IPT], val[IPT], rank[IPT];
     const uint32_t run = rank_tile(kin, vin, base, count, shift, flip, from_value, key, val, rank, whist);
@@ -468,13 +509,19 @@ void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32
     sc.partial_cap = partials;
     uint32_t *live_entries = counts + entries + partials;
     sc.total = reinterpret_cast<uint64_t *>(scratch + ((entries + partials + 2) & ~size_t(1)));
-    const uint32_t up_grid = grid_tiles, down_grid = grid_tiles;
-    for (uint32_t p = 0; p < ps.count; ++p) {
-      k_rts_up<<<up_grid, T, 0, st>>>(*keys, *vals, n_dev, n_cap, ps.shift[p], ps.flip[p], ps.from_value[p], counts,
-                                      live_entries);
+    // the tile order of xcd_first_tile needs a grid of whole XCD runs
+    const uint32_t grid = (grid_tiles + XCDS - 1) / XCDS * XCDS;
+    // Segments replace a value pass that orders by segment index.
+    const bool segmented = prep.seg_start && prep.segs > 1 && value_byte >= 0 && !n_dev;
+    const uint32_t passes = segmented ? ps.count - 1 : ps.count;
+    const uint32_t *seg = segmented ? prep.seg_start : nullptr;
+    const uint32_t segs = segmented ? prep.segs : 0;
+    for (uint32_t p = 0; p < passes; ++p) {
+      k_rts_up<<<grid, T, 0, st>>>(*keys, *vals, n_dev, n_cap, ps.shift[p], ps.flip[p], ps.from_value[p], counts,
+                                   live_entries, seg, segs);
       exclusive_scan(counts, counts, entries, sc, st, live_entries);
-      k_rts_down<<<down_grid, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, ps.shift[p], ps.flip[p],
-                                          ps.from_value[p], counts);
+      k_rts_down<<<grid, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, ps.shift[p], ps.flip[p],
+                                     ps.from_value[p], counts, seg, segs);
       std::swap(*keys, *keys_alt);
       std::swap(*vals, *vals_alt);
     }

@@ -97,6 +97,7 @@ class GsplatStageTimes(ctypes.Structure):
         ("tile_pairs", ctypes.c_uint64),
         ("blend_evals", ctypes.c_uint64),
         ("blend_tests", ctypes.c_uint64),
+        ("bin_pairs", ctypes.c_uint64),
     ]
 
 
@@ -245,6 +246,7 @@ SIGNATURES = {
     "gsplat_batch_get_tuning": (I32, [P, ctypes.POINTER(GsplatTuning)]),
     "gsplat_batch_set_tuning": (I32, [P, ctypes.POINTER(GsplatTuning)]),
     "gsplat_batch_autotune": (I32, [P, ctypes.POINTER(GsplatViewDesc), U32, U32, ctypes.POINTER(GsplatTuning)]),
+    "gsplat_context_get_tuning": (I32, [P, ctypes.POINTER(GsplatTuning)]),
     "gsplat_context_set_tuning": (I32, [P, ctypes.POINTER(GsplatTuning)]),
     "gsplat_context_autotune": (I32, [P, U32, ctypes.POINTER(GsplatTuning)]),
     "gsplat_server_config_default": (GsplatServerConfig, []),
@@ -423,6 +425,11 @@ class Context:
     def autotune(self, reps: int = 3) -> GsplatTuning:
         t = GsplatTuning()
         check(lib().gsplat_context_autotune(self.h, reps, ctypes.byref(t)))
+        return t
+
+    def tuning(self) -> GsplatTuning:
+        t = GsplatTuning()
+        check(lib().gsplat_context_get_tuning(self.h, ctypes.byref(t)))
         return t
 
     def set_tuning(self, t: GsplatTuning):

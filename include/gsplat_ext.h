@@ -91,6 +91,9 @@ typedef struct GsplatStageTimes {
                             * counting raster variant), else 0           */
   uint64_t blend_tests;    /* coverage tests the raster made (counting
                             * variant only, else 0)                      */
+  uint64_t bin_pairs;      /* (binning tile, splat) pairs emitted and
+                            * sorted; binning tiles are 16 << bin_shift
+                            * pixels square (GsplatTuning)               */
 } GsplatStageTimes;
 int32_t gsplat_context_get_stage_times(GsplatContext *ctx, GsplatStageTimes *out);
 
@@ -250,6 +253,7 @@ int32_t gsplat_batch_set_tuning(GsplatBatch *batch, const GsplatTuning *tuning);
  * the batch's last frame is then those views under the kept tuning. */
 int32_t gsplat_batch_autotune(GsplatBatch *batch, const GsplatViewDesc *views, uint32_t count, uint32_t reps,
                               GsplatTuning *out_best);
+int32_t gsplat_context_get_tuning(GsplatContext *ctx, GsplatTuning *out_tuning);
 int32_t gsplat_context_set_tuning(GsplatContext *ctx, const GsplatTuning *tuning);
 /* As gsplat_batch_autotune for the context's scene, camera and size. */
 int32_t gsplat_context_autotune(GsplatContext *ctx, uint32_t reps, GsplatTuning *out_best);
