@@ -95,6 +95,25 @@ void launch_project(uint32_t degree, const ViewDev *views, const uint32_t *seg_s
                     uint32_t total_slots, uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect,
                     const Clears &clears, hipStream_t stream);
 
+// Views of one scene projected together (views of a batch that share a
+// scene, at most kGroupViews per group). Group chunks are runs of 256
+// splats: [chunk_start, chunk_start + seg_len / 256), covering the padding
+// slots of the views' segments too; groups are ordered by chunk_start.
+constexpr uint32_t kGroupViews = 8;
+struct ProjGroup {
+  const float *soa;  // the scene (all views of the group)
+  uint32_t n, sh_degree;
+  uint32_t chunk_start, nv;
+  uint32_t views[kGroupViews];
+};
+
+// As launch_project for the groups whose chunks are [chunk_lo, chunk_lo +
+// chunks) (all of SH degree `degree`): each splat's scene fields are read
+// once for all views of its group.
+void launch_project_groups(uint32_t degree, const ViewDev *views, const ProjGroup *groups, uint32_t ngroups,
+                           uint32_t chunk_lo, uint32_t chunks, uint32_t *keys, uint32_t *vals, float4 *rec,
+                           uint2 *rect, const Clears &clears, hipStream_t stream);
+
 // The same projection of one view's n splats exporting, per splat, the depth
 // key (keys_out, optional) and the reference quantities of project_splat +
 // eval_color (12 floats: drawn, centre, axis_u, axis_v, premultiplied rgba).

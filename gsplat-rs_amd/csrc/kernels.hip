@@ -95,45 +95,66 @@ __device__ __forceinline__ float cov_bilinear(const Cov6 &c, float a0, float a1,
 }
 
 // One splat of eval_sh_rgb (wgsl:233-326) for one channel, in the shader's
-// left-to-right accumulation order. `rest` points at coefficient 0 of the
-// channel in the coefficient-major SoA (stride n floats per coefficient).
-template <int DEG>
-__device__ __forceinline__ float sh_channel(float dc, const float *__restrict__ rest, uint32_t n, float x, float y,
-                                            float z) {
+// left-to-right accumulation order. Coefficient k of the channel is field
+// base + k of the splat (`src`, see SoaSource).
+template <int DEG, class Src>
+__device__ __forceinline__ float sh_channel(float dc, const Src &src, uint32_t base, float x, float y, float z) {
   const float C0 = 0.28209479177387814f;
   float r = C0 * dc;
   if constexpr (DEG >= 1) {
     const float C1 = 0.4886025119029199f;
-    r = r - C1 * y * rest[0 * (size_t)n] + C1 * z * rest[1 * (size_t)n] - C1 * x * rest[2 * (size_t)n];
+    r = r - C1 * y * src(base + 0) + C1 * z * src(base + 1) - C1 * x * src(base + 2);
   }
   if constexpr (DEG >= 2) {
     const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
-    r = r + 1.0925484305920792f * xy * rest[3 * (size_t)n] + -1.0925484305920792f * yz * rest[4 * (size_t)n] +
-        0.31539156525252005f * (2.0f * zz - xx - yy) * rest[5 * (size_t)n] +
-        -1.0925484305920792f * xz * rest[6 * (size_t)n] + 0.5462742152960396f * (xx - yy) * rest[7 * (size_t)n];
+    r = r + 1.0925484305920792f * xy * src(base + 3) + -1.0925484305920792f * yz * src(base + 4) +
+        0.31539156525252005f * (2.0f * zz - xx - yy) * src(base + 5) +
+        -1.0925484305920792f * xz * src(base + 6) + 0.5462742152960396f * (xx - yy) * src(base + 7);
     if constexpr (DEG >= 3) {
-      r = r + -0.5900435899266435f * y * (3.0f * xx - yy) * rest[8 * (size_t)n] +
-          2.890611442640554f * xy * z * rest[9 * (size_t)n] +
-          -0.4570457994644658f * y * (4.0f * zz - xx - yy) * rest[10 * (size_t)n] +
-          0.3731763325901154f * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * rest[11 * (size_t)n] +
-          -0.4570457994644658f * x * (4.0f * zz - xx - yy) * rest[12 * (size_t)n] +
-          1.445305721320277f * z * (xx - yy) * rest[13 * (size_t)n] +
-          -0.5900435899266435f * x * (xx - 3.0f * yy) * rest[14 * (size_t)n];
+      r = r + -0.5900435899266435f * y * (3.0f * xx - yy) * src(base + 8) +
+          2.890611442640554f * xy * z * src(base + 9) +
+          -0.4570457994644658f * y * (4.0f * zz - xx - yy) * src(base + 10) +
+          0.3731763325901154f * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * src(base + 11) +
+          -0.4570457994644658f * x * (4.0f * zz - xx - yy) * src(base + 12) +
+          1.445305721320277f * z * (xx - yy) * src(base + 13) +
+          -0.5900435899266435f * x * (xx - 3.0f * yy) * src(base + 14);
     }
     if constexpr (DEG >= 4) {
-      r = r + 2.5033429417967046f * xy * (xx - yy) * rest[15 * (size_t)n] +
-          -1.7701307697799304f * yz * (3.0f * xx - yy) * rest[16 * (size_t)n] +
-          0.9461746957575601f * xy * (7.0f * zz - 1.0f) * rest[17 * (size_t)n] +
-          -0.6690465435572892f * yz * (7.0f * zz - 3.0f) * rest[18 * (size_t)n] +
-          0.10578554691520431f * (zz * (35.0f * zz - 30.0f) + 3.0f) * rest[19 * (size_t)n] +
-          -0.6690465435572892f * xz * (7.0f * zz - 3.0f) * rest[20 * (size_t)n] +
-          0.47308734787878004f * (xx - yy) * (7.0f * zz - 1.0f) * rest[21 * (size_t)n] +
-          -1.7701307697799304f * xz * (xx - 3.0f * yy) * rest[22 * (size_t)n] +
-          0.6258357354491761f * (xx * (xx - 3.0f * yy) - yy * (3.0f * xx - yy)) * rest[23 * (size_t)n];
+      r = r + 2.5033429417967046f * xy * (xx - yy) * src(base + 15) +
+          -1.7701307697799304f * yz * (3.0f * xx - yy) * src(base + 16) +
+          0.9461746957575601f * xy * (7.0f * zz - 1.0f) * src(base + 17) +
+          -0.6690465435572892f * yz * (7.0f * zz - 3.0f) * src(base + 18) +
+          0.10578554691520431f * (zz * (35.0f * zz - 30.0f) + 3.0f) * src(base + 19) +
+          -0.6690465435572892f * xz * (7.0f * zz - 3.0f) * src(base + 20) +
+          0.47308734787878004f * (xx - yy) * (7.0f * zz - 1.0f) * src(base + 21) +
+          -1.7701307697799304f * xz * (xx - 3.0f * yy) * src(base + 22) +
+          0.6258357354491761f * (xx * (xx - 3.0f * yy) - yy * (3.0f * xx - yy)) * src(base + 23);
     }
   }
   return r;
 }
+
+// Scene fields (common.h SoA order) of splat i, read from the SoA where the
+// projection needs them (culled splats skip their covariance and SH loads).
+struct SoaSource {
+  const float *__restrict__ soa;
+  size_t n;
+  uint32_t i;
+  __device__ __forceinline__ float operator()(uint32_t f) const { return soa[f * n + i]; }
+};
+
+// All scene fields of one splat in registers: loaded once, projected for
+// every view of a group.
+template <int DEG>
+struct RegSource {
+  static constexpr uint32_t kFields = kRest0 + 3 * ((DEG + 1) * (DEG + 1) - 1);
+  float v[kFields];
+  __device__ __forceinline__ void load(const float *__restrict__ soa, size_t n, uint32_t i) {
+#pragma unroll
+    for (uint32_t f = 0; f < kFields; ++f) v[f] = soa[f * n + i];
+  }
+  __device__ __forceinline__ float operator()(uint32_t f) const { return v[f]; }
+};
 
 struct Projected {
   uint32_t key;  // depth key, 0 when culled by near/far
@@ -145,15 +166,13 @@ struct Projected {
 // Depth key and cull (preprocess_visible_into, render-wgpu/src/lib.rs:
 // 1263-1296), projection (project_splat, wgsl:131-217) and colour
 // (eval_color, wgsl:328-334) of splat i of a scene.
-template <int DEG>
-__device__ __forceinline__ Projected project_one(const FrameParams &fp, const float *__restrict__ soa, uint32_t n,
-                                                 uint32_t i) {
+template <int DEG, class Src>
+__device__ __forceinline__ Projected project_one(const FrameParams &fp, const Src &src) {
   constexpr uint32_t PC = (DEG + 1) * (DEG + 1) - 1;
-  const size_t N = n;
   Projected o{};
-  const float relx = soa[kPx * N + i] - fp.cam[0];
-  const float rely = soa[kPy * N + i] - fp.cam[1];
-  const float relz = soa[kPz * N + i] - fp.cam[2];
+  const float relx = src(kPx) - fp.cam[0];
+  const float rely = src(kPy) - fp.cam[1];
+  const float relz = src(kPz) - fp.cam[2];
   // world_to_camera_depth_with_view_row (lib.rs:1346-1354), no FMA.
   const float z = fp.rows[6] * relx + fp.rows[7] * rely + fp.rows[8] * relz;
   const bool vis = z >= fp.near_plane && z <= fp.far_plane;
@@ -164,8 +183,8 @@ __device__ __forceinline__ Projected project_one(const FrameParams &fp, const fl
   const float inv_z = 1.0f / z;
   const float x_ndc = ((x * fp.f) * inv_z) / fp.aspect;
   const float y_ndc = (y * fp.f) * inv_z;
-  const Cov6 wc{soa[kCxx * N + i], soa[kCxy * N + i], soa[kCxz * N + i],
-                soa[kCyy * N + i], soa[kCyz * N + i], soa[kCzz * N + i]};
+  const Cov6 wc{src(kCxx), src(kCxy), src(kCxz),
+                src(kCyy), src(kCyz), src(kCzz)};
   const float *r0 = &fp.rows[0], *r1 = &fp.rows[3], *r2 = &fp.rows[6];
   // transform_covariance_terms_to_camera (wgsl:111-120).
   const Cov6 cc{cov_quadratic(wc, r0[0], r0[1], r0[2]),
@@ -220,7 +239,7 @@ __device__ __forceinline__ Projected project_one(const FrameParams &fp, const fl
   o.uy = uy;
   o.vx = vx;
   o.vy = vy;
-  o.alpha = soa[kAlpha * N + i];
+  o.alpha = src(kAlpha);
   // normalize3_or_default(position - camera) (wgsl:87-93, 329).
   float dxw = 0.0f, dyw = 0.0f, dzw = 1.0f;
   const float len2 = relx * relx + rely * rely + relz * relz;
@@ -232,8 +251,8 @@ __device__ __forceinline__ Projected project_one(const FrameParams &fp, const fl
   }
 #pragma unroll
   for (int ch = 0; ch < 3; ++ch) {
-    const float dc = soa[(kDcR + ch) * N + i];
-    const float s = sh_channel<DEG>(dc, soa + (size_t)(kRest0 + ch * PC) * N + i, n, dxw, dyw, dzw);
+    const float dc = src(kDcR + ch);
+    const float s = sh_channel<DEG>(dc, src, kRest0 + ch * PC, dxw, dyw, dzw);
     o.col[ch] = fminf(fmaxf(s + 0.5f, 0.0f), 1.0f) * o.alpha;
   }
   return o;
@@ -248,31 +267,26 @@ __device__ __forceinline__ void clear_ranges(const Clears &c) {
     for (size_t i = i0; i < c.words[r]; i += stride) c.ptr[r][i] = 0u;
 }
 
-template <int DEG>
-__global__ __launch_bounds__(kThreads) void k_project(const ViewDev *__restrict__ views,
-                                                      const uint32_t *__restrict__ seg_start, uint32_t nviews,
-                                                      uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                      float4 *__restrict__ rec, uint2 *__restrict__ rect,
-                                                      Clears clears) {
-  clear_ranges(clears);
-  const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
-  const uint32_t v = find_view(seg_start, nviews, blockIdx.x * kThreads);
-  const ViewDev &vw = views[v];
-  if (vw.sh_degree != (uint32_t)DEG) return;
-  const uint32_t i = g - vw.seg_off;
-  if (i >= vw.n) {  // padding slot: never visible, sorts last in its view
-    keys[g] = 0u;
-    vals[g] = (v << kLocalBits) | kLocalMask;
-    rect[g] = make_uint2(0xFFFFFFFFu, 0u);
-    return;
-  }
-  const FrameParams &fp = vw.fp;
-  const Projected p = project_one<DEG>(fp, vw.soa, vw.n, i);
+// Padding slot g of view v: never visible, sorts last in its view.
+__device__ __forceinline__ void write_padding(uint32_t v, uint32_t g, uint32_t *__restrict__ keys,
+                                              uint32_t *__restrict__ vals, uint2 *__restrict__ rect) {
+  keys[g] = 0u;
+  vals[g] = (v << kLocalBits) | kLocalMask;
+  rect[g] = make_uint2(0xFFFFFFFFu, 0u);
+}
+
+// Slot g of view v (frame fp) for its splat i, projected as p: depth key,
+// value, 48-byte record and pixel bounds.
+template <class Src>
+__device__ __forceinline__ void write_slot(const FrameParams &fp, uint32_t v, uint32_t g, uint32_t i,
+                                           const Projected &p, const Src &src, uint32_t *__restrict__ keys,
+                                           uint32_t *__restrict__ vals, float4 *__restrict__ rec,
+                                           uint2 *__restrict__ rect) {
   keys[g] = p.key;
   vals[g] = (v << kLocalBits) | i;
   uint2 rr = make_uint2(0xFFFFFFFFu, 0u);
   if (p.drawn) {
-    const float cut = fminf(vw.soa[kCut * (size_t)vw.n + i], 1.0f);
+    const float cut = fminf(src(kCut), 1.0f);
     const float det = p.ux * p.vy - p.vx * p.uy;
     rec[3 * (size_t)g + 0] = make_float4(p.cx, p.cy, p.vy / det, -p.vx / det);
     rec[3 * (size_t)g + 1] = make_float4(-p.uy / det, p.ux / det, cut, p.alpha);
@@ -301,13 +315,71 @@ __global__ __launch_bounds__(kThreads) void k_project(const ViewDev *__restrict_
 }
 
 template <int DEG>
+__global__ __launch_bounds__(kThreads) void k_project(const ViewDev *__restrict__ views,
+                                                      const uint32_t *__restrict__ seg_start, uint32_t nviews,
+                                                      uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                      float4 *__restrict__ rec, uint2 *__restrict__ rect,
+                                                      Clears clears) {
+  clear_ranges(clears);
+  const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t v = find_view(seg_start, nviews, blockIdx.x * kThreads);
+  const ViewDev &vw = views[v];
+  if (vw.sh_degree != (uint32_t)DEG) return;
+  const uint32_t i = g - vw.seg_off;
+  if (i >= vw.n) {
+    write_padding(v, g, keys, vals, rect);
+    return;
+  }
+  const SoaSource src{vw.soa, vw.n, i};
+  write_slot(vw.fp, v, g, i, project_one<DEG>(vw.fp, src), src, keys, vals, rec, rect);
+}
+
+// Views of one scene projected together: workgroup b takes chunk
+// chunk_lo + b, i.e. splats [256 c, 256 c + 256) of the group whose chunk
+// range holds it; each thread loads its splat's fields once and writes its
+// slot in every view of the group (the slot layout and values are those of
+// k_project, so the results are the same bit for bit).
+template <int DEG>
+__global__ __launch_bounds__(kThreads) void k_project_groups(const ViewDev *__restrict__ views,
+                                                             const ProjGroup *__restrict__ groups,
+                                                             uint32_t ngroups, uint32_t chunk_lo,
+                                                             uint32_t *__restrict__ keys,
+                                                             uint32_t *__restrict__ vals, float4 *__restrict__ rec,
+                                                             uint2 *__restrict__ rect, Clears clears) {
+  clear_ranges(clears);
+  const uint32_t chunk = chunk_lo + blockIdx.x;
+  uint32_t lo = 0, hi = ngroups;  // last group whose first chunk <= chunk
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (groups[mid].chunk_start <= chunk) lo = mid; else hi = mid;
+  }
+  const ProjGroup &gr = groups[lo];
+  const uint32_t i = (chunk - gr.chunk_start) * kThreads + threadIdx.x;
+  const uint32_t nv = gr.nv;
+  if (i >= gr.n) {
+    for (uint32_t q = 0; q < nv; ++q) {
+      const uint32_t v = gr.views[q];
+      write_padding(v, views[v].seg_off + i, keys, vals, rect);
+    }
+    return;
+  }
+  RegSource<DEG> src;
+  src.load(gr.soa, gr.n, i);
+  for (uint32_t q = 0; q < nv; ++q) {
+    const uint32_t v = gr.views[q];
+    const ViewDev &vw = views[v];
+    write_slot(vw.fp, v, vw.seg_off + i, i, project_one<DEG>(vw.fp, src), src, keys, vals, rec, rect);
+  }
+}
+
+template <int DEG>
 __global__ __launch_bounds__(kThreads) void k_project_export(const ViewDev *__restrict__ views, uint32_t v,
                                                              uint32_t *__restrict__ keys_out,
                                                              float *__restrict__ export12) {
   const ViewDev &vw = views[v];
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= vw.n) return;
-  const Projected p = project_one<DEG>(vw.fp, vw.soa, vw.n, i);
+  const Projected p = project_one<DEG>(vw.fp, SoaSource{vw.soa, vw.n, i});
   if (keys_out) keys_out[i] = p.key;
   float *o = export12 + 12 * (size_t)i;
   // invalid_splat() (wgsl:122-129): zeros for a splat that is not drawn.
@@ -873,6 +945,22 @@ void launch_project(uint32_t deg, const ViewDev *views, const uint32_t *seg_star
     case 2: k_project<2><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr); break;
     case 3: k_project<3><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr); break;
     case 4: k_project<4><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr); break;
+    default: throw Error(Code::Unsupported, "sh degree above 4");
+  }
+  GS_HIP(hipGetLastError());
+}
+
+void launch_project_groups(uint32_t deg, const ViewDev *views, const ProjGroup *groups, uint32_t ngroups,
+                           uint32_t chunk_lo, uint32_t chunks, uint32_t *keys, uint32_t *vals, float4 *rec,
+                           uint2 *rect, const Clears &clr, hipStream_t st) {
+  if (chunks == 0 || ngroups == 0) return;
+  const dim3 grid(chunks), block(kThreads);
+  switch (deg) {
+    case 0: k_project_groups<0><<<grid, block, 0, st>>>(views, groups, ngroups, chunk_lo, keys, vals, rec, rect, clr); break;
+    case 1: k_project_groups<1><<<grid, block, 0, st>>>(views, groups, ngroups, chunk_lo, keys, vals, rec, rect, clr); break;
+    case 2: k_project_groups<2><<<grid, block, 0, st>>>(views, groups, ngroups, chunk_lo, keys, vals, rec, rect, clr); break;
+    case 3: k_project_groups<3><<<grid, block, 0, st>>>(views, groups, ngroups, chunk_lo, keys, vals, rec, rect, clr); break;
+    case 4: k_project_groups<4><<<grid, block, 0, st>>>(views, groups, ngroups, chunk_lo, keys, vals, rec, rect, clr); break;
     default: throw Error(Code::Unsupported, "sh degree above 4");
   }
   GS_HIP(hipGetLastError());

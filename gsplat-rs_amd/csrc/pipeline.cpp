@@ -12,7 +12,19 @@ namespace gs {
 
 namespace {
 
-constexpr size_t kTableBytes = 2 * kMaxViews * sizeof(uint32_t) + kMaxViews * sizeof(ViewDev);
+// Device view table of a frame, packed in 16-byte aligned sections:
+// seg_start[nviews] | tile_start[nviews] | ViewDev[nviews] | ProjGroup[ngroups].
+struct TableLayout {
+  size_t tile_start, views, groups, bytes;
+};
+constexpr size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+constexpr TableLayout table_layout(uint32_t nviews, uint32_t ngroups) {
+  const size_t tile_start = align16((size_t)nviews * sizeof(uint32_t));
+  const size_t views = align16(tile_start + (size_t)nviews * sizeof(uint32_t));
+  const size_t groups = align16(views + (size_t)nviews * sizeof(ViewDev));
+  return TableLayout{tile_start, views, groups, groups + (size_t)ngroups * sizeof(ProjGroup)};
+}
+constexpr size_t kTableBytes = table_layout(kMaxViews, kMaxViews).bytes;
 // Frame summary words (device and pinned host copies):
 //   [0] unused, [1..2] binning pair count M (u64),
 //   [3] overflow flag, [4..5] (16x16 tile, splat) pairs (u64),
@@ -154,6 +166,41 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
   }
   if (tiles >= (1ull << 30)) throw Error(Code::Unsupported, "batch exceeds 2^30 tiles");
 
+  // Projection groups of a batch: views of one scene (at most kGroupViews
+  // each) project together, reading the scene once. Groups are ordered by
+  // SH degree, so each degree's chunks form one range (one launch).
+  host_groups_.clear();
+  for (auto &r : group_chunks_) r = {0, 0};
+  if (multi) {
+    uint32_t chunk = 0;
+    for (uint32_t deg = 0; deg <= 4; ++deg) {
+      const size_t first = host_groups_.size();
+      group_chunks_[deg].first = chunk;
+      for (uint32_t v = 0; v < (uint32_t)table.size(); ++v) {
+        const ViewDev &d = table[v];
+        if (d.sh_degree != deg) continue;
+        ProjGroup *g = nullptr;
+        for (size_t q = first; q < host_groups_.size() && !g; ++q)
+          if (host_groups_[q].soa == d.soa && host_groups_[q].n == d.n && host_groups_[q].nv < kGroupViews)
+            g = &host_groups_[q];
+        if (!g) {
+          ProjGroup ng{};
+          ng.soa = d.soa;
+          ng.n = d.n;
+          ng.sh_degree = deg;
+          host_groups_.push_back(ng);
+          g = &host_groups_.back();
+        }
+        g->views[g->nv++] = v;
+      }
+      for (size_t q = first; q < host_groups_.size(); ++q) {
+        host_groups_[q].chunk_start = chunk;
+        chunk += table[host_groups_[q].views[0]].seg_len / kThreads;
+      }
+      group_chunks_[deg].second = chunk - group_chunks_[deg].first;
+    }
+  }
+
   views_ = std::move(views);
   host_rgba8_ = std::move(host_rgba8);
   host_views_ = std::move(table);
@@ -213,13 +260,16 @@ void Pipeline::launch_frame() {
   const int r = (int)(frame_counter_++ % 2);
   GS_HIP(hipEventSynchronize(table_ev_[r]));
   uint8_t *host = table_pinned_[r];
+  const TableLayout lay = table_layout(nviews, (uint32_t)host_groups_.size());
   uint32_t *seg_start = reinterpret_cast<uint32_t *>(host);
-  uint32_t *tile_start = seg_start + kMaxViews;
+  uint32_t *tile_start = reinterpret_cast<uint32_t *>(host + lay.tile_start);
   for (uint32_t v = 0; v < nviews; ++v) {
     seg_start[v] = host_views_[v].seg_off;
     tile_start[v] = host_views_[v].tile_off;
   }
-  std::memcpy(host + 2 * kMaxViews * sizeof(uint32_t), host_views_.data(), nviews * sizeof(ViewDev));
+  std::memcpy(host + lay.views, host_views_.data(), nviews * sizeof(ViewDev));
+  if (!host_groups_.empty())
+    std::memcpy(host + lay.groups, host_groups_.data(), host_groups_.size() * sizeof(ProjGroup));
 
   FramePlan plan;
   plan.slot = r;
@@ -309,12 +359,14 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   const bool depth_lookback = plan.depth_lookback, pair_lookback = plan.pair_lookback;
   const size_t depth_dirty = plan.depth_dirty, pair_dirty = plan.pair_dirty;
   hipStream_t st = stream_;
-  GS_HIP(hipMemcpyAsync(table_.get(), table_pinned_[r],
-                        2 * kMaxViews * sizeof(uint32_t) + nviews * sizeof(ViewDev), hipMemcpyHostToDevice, st));
+  const uint32_t ngroups = (uint32_t)host_groups_.size();
+  const TableLayout lay = table_layout(nviews, ngroups);
+  GS_HIP(hipMemcpyAsync(table_.get(), table_pinned_[r], lay.bytes, hipMemcpyHostToDevice, st));
   if (!captured) GS_HIP(hipEventRecord(table_ev_[r], st));
   const uint32_t *seg_start_d = reinterpret_cast<const uint32_t *>(table_.get());
-  const uint32_t *tile_start_d = seg_start_d + kMaxViews;
-  const ViewDev *views_d = reinterpret_cast<const ViewDev *>(table_.get() + 2 * kMaxViews * sizeof(uint32_t));
+  const uint32_t *tile_start_d = reinterpret_cast<const uint32_t *>(table_.get() + lay.tile_start);
+  const ViewDev *views_d = reinterpret_cast<const ViewDev *>(table_.get() + lay.views);
+  const ProjGroup *groups_d = reinterpret_cast<const ProjGroup *>(table_.get() + lay.groups);
   uint32_t *info = summary_.get();
   uint32_t *visible = info + kVisibleWord;
   unsigned long long *raster_pairs = reinterpret_cast<unsigned long long *>(info + kRasterPairsWord);
@@ -328,14 +380,17 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   if (slots == 0) {
     for (int c = 0; c < Clears::kRanges; ++c) launch_fill_u32(frame_clears.ptr[c], 0u, frame_clears.words[c], st);
   }
-  bool degree_seen[5] = {false, false, false, false, false};
-  for (const ViewDev &d : host_views_) degree_seen[std::min<uint32_t>(d.sh_degree, 4)] = true;
-  for (uint32_t deg = 0; deg <= 4; ++deg)
-    if (degree_seen[deg]) {
-      launch_project(deg, views_d, seg_start_d, nviews, slots, keys_.get(), vals_.get(), rec_.get(), rect_.get(),
-                     frame_clears, st);
-      frame_clears = Clears();  // once
-    }
+  if (ngroups == 0) {
+    launch_project(host_views_[0].sh_degree, views_d, seg_start_d, nviews, slots, keys_.get(), vals_.get(),
+                   rec_.get(), rect_.get(), frame_clears, st);
+  } else {
+    for (uint32_t deg = 0; deg <= 4; ++deg)
+      if (group_chunks_[deg].second) {
+        launch_project_groups(deg, views_d, groups_d, ngroups, group_chunks_[deg].first, group_chunks_[deg].second,
+                              keys_.get(), vals_.get(), rec_.get(), rect_.get(), frame_clears, st);
+        frame_clears = Clears();  // once
+      }
+  }
   if (!captured) GS_HIP(hipEventRecord(ev_[1], st));
 
   uint32_t *k = keys_.get(), *v = vals_.get(), *ka = keys_alt_.get(), *va = vals_alt_.get();
@@ -542,7 +597,8 @@ void Pipeline::read_tile_counts(uint32_t v, uint32_t *out, uint64_t cap, uint32_
   wait();
   const ViewDev &d = host_views_[v];
   if (cap < d.tiles) throw Error(Code::InvalidArgument, "output buffer smaller than the tile count");
-  const ViewDev *views_d = reinterpret_cast<const ViewDev *>(table_.get() + 2 * kMaxViews * sizeof(uint32_t));
+  const ViewDev *views_d = reinterpret_cast<const ViewDev *>(
+      table_.get() + table_layout((uint32_t)host_views_.size(), (uint32_t)host_groups_.size()).views);
   DevBuf<uint32_t> counts;
   counts.ensure(std::max<uint32_t>(d.tiles, 1));
   launch_raster_tile_counts(views_d, v, d, sorted_keys_, sorted_vals_, rect_.get(), counts.get(), stream_);

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "common.h"
@@ -218,7 +219,7 @@ class Pipeline {
   uint32_t *summary_pinned_ = nullptr;  // [-, M lo, M hi, overflow, visible[kMaxViews]]
   uint64_t frame_counter_ = 0;
 
-  DevBuf<uint8_t> table_;    // seg_start[kMaxViews] | tile_start[kMaxViews] | ViewDev[kMaxViews]
+  DevBuf<uint8_t> table_;    // seg_start | tile_start | ViewDev | ProjGroup, packed per frame
   DevBuf<uint32_t> summary_;  // device side of summary_pinned_
   DevBuf<uint32_t> keys_, vals_, keys_alt_, vals_alt_;
   DevBuf<float4> rec_;
@@ -234,6 +235,8 @@ class Pipeline {
   std::vector<ViewSpec> views_;
   std::vector<uint8_t *> host_rgba8_;
   std::vector<ViewDev> host_views_;
+  std::vector<ProjGroup> host_groups_;                 // batches only (empty for one view)
+  std::pair<uint32_t, uint32_t> group_chunks_[5] = {};  // per SH degree: first chunk, chunk count
   uint32_t total_slots_ = 0, total_tiles_ = 0, total_ctiles_ = 0;
   uint64_t total_pixels_ = 0, total_splats_ = 0;
   uint32_t pair_cap_ = 0;
