@@ -634,18 +634,20 @@ namespace {
 gs::Tuning tuning_from(const GsplatTuning &t) {
   gs::Tuning out;
   out.bin_shift = t.bin_shift;
-  if (t.raster_variant > 1) throw gs::Error(gs::Code::InvalidArgument, "raster_variant must be 0 or 1");
-  out.raster = t.raster_variant == 0 ? gs::RasterVariant::Linear : gs::RasterVariant::Unrolled;
+  if (t.raster_variant > 2) throw gs::Error(gs::Code::InvalidArgument, "raster_variant must be 0, 1 or 2");
+  out.raster = static_cast<gs::RasterVariant>(t.raster_variant);
   out.rts_min_tiles = t.sort_rts_min_tiles;
+  out.blend_min = t.blend_min;
   return out;
 }
 
 GsplatTuning tuning_to(const gs::Tuning &t, float ms) {
   GsplatTuning out;
   out.bin_shift = t.bin_shift;
-  out.raster_variant = t.raster == gs::RasterVariant::Linear ? 0u : 1u;
+  out.raster_variant = static_cast<uint32_t>(t.raster);
   out.sort_rts_min_tiles = t.rts_min_tiles;
   out.frame_ms = ms;
+  out.blend_min = t.blend_min;
   return out;
 }
 

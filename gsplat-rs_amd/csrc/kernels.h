@@ -190,13 +190,16 @@ void launch_raster_tile_counts(const ViewDev *views, uint32_t v, const ViewDev &
 //   Linear    per-lane exit at the early stop, divergent blend branch
 //   Unrolled  wave-uniform splat loop, four splats per wave liveness check
 //             (default)
-// Both give bit-identical images. With `evals`, the counting variant also
+//   Rows2     as Unrolled with two pixels per thread (two tile rows per
+//             workgroup, one LDS read of a splat for both pixels); needs a
+//             bin shift >= 1 and runs as Unrolled at shift 0
+// All give bit-identical images. With `evals`, the counting variant also
 // adds to evals[0] the number of (pixel, splat) evaluations that pass
 // coverage before each pixel's early stop (the blend's algorithmic work) and
 // to evals[1] the coverage tests made.
-enum class RasterVariant : uint32_t { Linear = 0, Unrolled = 1 };
+enum class RasterVariant : uint32_t { Linear = 0, Unrolled = 1, Rows2 = 2 };
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
-                   uint32_t total_tiles, uint32_t shift, const float4 *rec, const uint2 *rect, const uint2 *ranges,
+                   uint32_t total_tiles, uint32_t shift, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
                    const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t stream,
                    unsigned long long *evals = nullptr);
 

@@ -722,6 +722,33 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
 // LOOP 1: wave-uniform loop, four splats per wave liveness check; the stage
 // is padded to a multiple of four with splats no pixel can hit (cut = -1).
 // Both make the same operations in the same order for every pixel.
+// True unless the splat's coverage ellipse {p : |L (p - c)|^2 <= cut}
+// (record halves a = [cx, cy, L00, L01], b = [L10, L11, cut, alpha], NDC)
+// misses the rectangle [xlo, xhi] x [ylo, yhi]. With c outside the
+// rectangle the convex quadratic's minimum lies on an edge, where it is the
+// clamped 1-D minimum. Used with the rectangle of a tile's pixel edges (half
+// a pixel beyond its pixel centres) and a 1% slack on cut, so a splat it
+// rejects passes the coverage test at none of the tile's pixels.
+__device__ __forceinline__ bool ellipse_meets_rect(const float4 &a, const float4 &b, float xlo, float xhi, float ylo,
+                                                   float yhi) {
+  const float cx = a.x, cy = a.y;
+  if (cx >= xlo && cx <= xhi && cy >= ylo && cy <= yhi) return true;
+  const float l00 = a.z, l01 = a.w, l10 = b.x, l11 = b.y;
+  const float A = l00 * l00 + l10 * l10, B = l00 * l01 + l10 * l11, C = l01 * l01 + l11 * l11;
+  const float ra = A > 0.0f ? __builtin_amdgcn_rcpf(A) : 0.0f;
+  const float rc = C > 0.0f ? __builtin_amdgcn_rcpf(C) : 0.0f;
+  const float dxlo = xlo - cx, dxhi = xhi - cx, dylo = ylo - cy, dyhi = yhi - cy;
+  auto q = [&](float dx, float dy) {
+    const float lx = l00 * dx + l01 * dy, ly = l10 * dx + l11 * dy;
+    return lx * lx + ly * ly;
+  };
+  const float m0 = q(dxlo, fminf(fmaxf(-B * dxlo * rc, dylo), dyhi));
+  const float m1 = q(dxhi, fminf(fmaxf(-B * dxhi * rc, dylo), dyhi));
+  const float m2 = q(fminf(fmaxf(-B * dylo * ra, dxlo), dxhi), dylo);
+  const float m3 = q(fminf(fmaxf(-B * dyhi * ra, dxlo), dxhi), dyhi);
+  return fminf(fminf(m0, m1), fminf(m2, m3)) <= b.z * 1.01f + 1e-6f;
+}
+
 template <int LOOP, bool COUNT>
 __device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const float4 *s2, uint32_t n, float pnx,
                                             float pny, float &T, float &cr, float &cg, float &cb, bool &done,
@@ -761,11 +788,19 @@ __device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const 
     bool active = !done;
     for (uint32_t j0 = 0; j0 < nu; j0 += U) {
       if (__ballot(active) == 0) break;  // wave-uniform
+      // the group's coverage terms first: one LDS latency per group
+      float4 ga[U], gb[U], gc[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        ga[u] = s0[j0 + u];
+        gb[u] = s1[j0 + u];
+        gc[u] = s2[j0 + u];
+      }
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u) {
         const uint32_t j = j0 + u;
-        const float4 a = s0[j];
-        const float4 b = s1[j];
+        const float4 a = ga[u];
+        const float4 b = gb[u];
         const float dx = pnx - a.x;
         const float dy = pny - a.y;
         const float lx = fmaf(a.w, dy, a.z * dx);
@@ -778,7 +813,7 @@ __device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const 
         }
         if (hit) {
           const float g = __expf(-4.5f * r2);
-          const float4 c = s2[j];
+          const float4 c = gc[u];
           const float wgt = T * g;
           cr = fmaf(wgt, c.x, cr);
           cg = fmaf(wgt, c.y, cg);
@@ -807,7 +842,7 @@ __device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const 
 template <bool COUNT, int LOOP>
 __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__ views,
                                                      const uint32_t *__restrict__ tile_start, uint32_t nviews,
-                                                     uint32_t total_tiles, uint32_t shift,
+                                                     uint32_t total_tiles, uint32_t shift, uint32_t blend_min,
                                                      const float4 *__restrict__ rec, const uint2 *__restrict__ rect,
                                                      const uint2 *__restrict__ ranges,
                                                      const uint32_t *__restrict__ list, float4 *__restrict__ img32,
@@ -830,6 +865,9 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
   const float pny = 1.0f - (2.0f * ((float)y + 0.5f)) / (float)height;
   const uint2 rg = ranges[vw.ctile_off + (ty >> shift) * vw.fp.ctiles_x + (tx >> shift)];
   const uint32_t tx0 = tx * kTile, tx1 = tx0 + kTile - 1, ty0 = ty * kTile, ty1 = ty0 + kTile - 1;
+  // the tile's pixel edges in NDC (x right, y up)
+  const float exlo = (2.0f * (float)tx0) / (float)width - 1.0f, exhi = (2.0f * (float)(tx1 + 1)) / (float)width - 1.0f;
+  const float eyhi = 1.0f - (2.0f * (float)ty0) / (float)height, eylo = 1.0f - (2.0f * (float)(ty1 + 1)) / (float)height;
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
   float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
   bool done = !inside;
@@ -842,6 +880,7 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
       const uint32_t k = start + t;
       uint32_t s = 0;
       bool keep = false;
+      float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra;
       if (k < rg.y) {
         s = list[k];
         if (shift == 0) {
@@ -849,6 +888,11 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
         } else {
           const uint2 r = rect[s];
           keep = (r.x & 0xFFFFu) <= tx1 && (r.y & 0xFFFFu) >= tx0 && (r.x >> 16) <= ty1 && (r.y >> 16) >= ty0;
+        }
+        if (LOOP != 0 && keep) {  // drop splats whose ellipse misses the tile (the Linear loop keeps them)
+          ra = rec[3 * (size_t)s + 0];
+          rb = rec[3 * (size_t)s + 1];
+          keep = ellipse_meets_rect(ra, rb, exlo, exhi, eylo, eyhi);
         }
       }
       const uint64_t bal = __ballot(keep);
@@ -862,15 +906,15 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
       }
       if (keep) {
         const uint32_t pos = off + (uint32_t)__popcll(bal & lt);
-        s0[pos] = rec[3 * (size_t)s + 0];
-        s1[pos] = rec[3 * (size_t)s + 1];
+        s0[pos] = LOOP != 0 ? ra : rec[3 * (size_t)s + 0];
+        s1[pos] = LOOP != 0 ? rb : rec[3 * (size_t)s + 1];
         s2[pos] = rec[3 * (size_t)s + 2];
       }
       staged += kept;
       start += kThreads;
       __syncthreads();
     }
-    if (staged >= kThreads || (!more || start >= rg.y)) {
+    if (staged >= blend_min || (!more || start >= rg.y)) {
       if (staged > 0)
         blend_stage<LOOP, COUNT>(s0, s1, s2, staged, pnx, pny, T, cr, cg, cb, done, passed, tested);
       staged = 0;
@@ -886,6 +930,175 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
   }
   if (COUNT) {
     uint32_t wp = passed, u = tested;
+#pragma unroll
+    for (uint32_t off = kWave / 2; off > 0; off >>= 1) {
+      wp += __shfl_down(wp, off, kWave);
+      u += __shfl_down(u, off, kWave);
+    }
+    if (lane == 0 && wp) atomicAdd(evals, (unsigned long long)wp);
+    if (lane == 0 && u) atomicAdd(evals + 1, (unsigned long long)u);
+  }
+}
+
+// One pixel's state in the two-pixel blend.
+struct PixelAcc {
+  float T, cr, cg, cb;
+  bool live;
+  uint32_t passed, tested;
+};
+
+// Coverage test and blend of staged splat j for one pixel at (pnx, pny):
+// the same operations, in the same order, as the one-pixel blend_stage.
+template <bool COUNT>
+__device__ __forceinline__ void blend_one(PixelAcc &p, float pnx, float pny, const float4 &a, const float4 &b,
+                                          const float4 *s2, uint32_t j, bool real) {
+  const float dx = pnx - a.x;
+  const float dy = pny - a.y;
+  const float lx = fmaf(a.w, dy, a.z * dx);
+  const float ly = fmaf(b.y, dy, b.x * dx);
+  const float r2 = fmaf(ly, ly, lx * lx);
+  const bool hit = p.live && r2 <= b.z;
+  if (COUNT) {
+    p.tested += (p.live && real) ? 1u : 0u;
+    p.passed += hit ? 1u : 0u;
+  }
+  if (hit) {
+    const float g = __expf(-4.5f * r2);
+    const float4 c = s2[j];
+    const float wgt = p.T * g;
+    p.cr = fmaf(wgt, c.x, p.cr);
+    p.cg = fmaf(wgt, c.y, p.cg);
+    p.cb = fmaf(wgt, c.z, p.cb);
+    p.T = p.T * (1.0f - b.w * g);
+    p.live = !(p.T < 1e-4f);
+  }
+}
+
+// Blend of n staged splats for the two pixels of a lane: one LDS read of
+// each splat's coverage terms serves both pixels. Wave-uniform loop, four
+// splats per liveness check (padding entries cannot pass coverage).
+template <bool COUNT>
+__device__ __forceinline__ void blend_stage2(const float4 *s0, float4 *s1, const float4 *s2, uint32_t n, float pnx,
+                                             float pny0, float pny1, PixelAcc &p0, PixelAcc &p1) {
+  constexpr uint32_t U = 4;
+  const uint32_t nu = (n + U - 1) & ~(U - 1);
+  const uint32_t t = threadIdx.x;
+  if (t < nu - n) s1[n + t] = make_float4(0.0f, 0.0f, -1.0f, 0.0f);
+  __syncthreads();
+  for (uint32_t j0 = 0; j0 < nu; j0 += U) {
+    if (__ballot(p0.live || p1.live) == 0) break;  // wave-uniform
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t j = j0 + u;
+      const float4 a = s0[j];
+      const float4 b = s1[j];
+      blend_one<COUNT>(p0, pnx, pny0, a, b, s2, j, j < n);
+      blend_one<COUNT>(p1, pnx, pny1, a, b, s2, j, j < n);
+    }
+  }
+}
+
+// Two vertically adjacent 16x16 tiles (tile rows ty, ty + 1, ty even) of
+// one binning tile per workgroup, two pixels per thread: wave w covers rows
+// 8w .. 8w+7 of the 16x32 region, lane l the pixels (l % 16, 8w + 2(l/16))
+// and the one below. The entries of the binning tile's list whose pixel
+// bounds overlap the region are staged in order as in k_raster, and each
+// pixel sees the same splats in the same order with the same arithmetic, so
+// the images equal k_raster's bit for bit. Needs bin shift >= 1 (both tile
+// rows in one binning tile); workgroups of odd tile rows exit at once.
+template <bool COUNT>
+__global__ __launch_bounds__(kThreads) void k_raster_rows2(const ViewDev *__restrict__ views,
+                                                           const uint32_t *__restrict__ tile_start, uint32_t nviews,
+                                                           uint32_t total_tiles, uint32_t shift, uint32_t blend_min,
+                                                           const float4 *__restrict__ rec,
+                                                           const uint2 *__restrict__ rect,
+                                                           const uint2 *__restrict__ ranges,
+                                                           const uint32_t *__restrict__ list,
+                                                           float4 *__restrict__ img32, uchar4 *__restrict__ img8,
+                                                           unsigned long long *__restrict__ evals) {
+  constexpr uint32_t kStage = 2 * kThreads;
+  __shared__ float4 s0[kStage], s1[kStage], s2[kStage];
+  __shared__ uint32_t wkeep[kWaves];
+  const uint32_t tile = blockIdx.x;
+  if (tile >= total_tiles) return;
+  const uint32_t v = find_view(tile_start, nviews, tile);
+  const ViewDev &vw = views[v];
+  const uint32_t width = vw.fp.width, height = vw.fp.height;
+  const uint32_t local = tile - vw.tile_off;
+  const uint32_t tx = local % vw.fp.tiles_x, ty = local / vw.fp.tiles_x;
+  if (ty & 1u) return;  // covered by the workgroup of row ty - 1
+  const uint32_t t = threadIdx.x, w = t / kWave, lane = __lane_id();
+  const uint32_t x = tx * kTile + (lane % kTile);
+  const uint32_t y0 = ty * kTile + 8 * w + 2 * (lane / kTile), y1 = y0 + 1;
+  const bool in0 = x < width && y0 < height, in1 = x < width && y1 < height;
+  const float pnx = (2.0f * ((float)x + 0.5f)) / (float)width - 1.0f;
+  const float pny0 = 1.0f - (2.0f * ((float)y0 + 0.5f)) / (float)height;
+  const float pny1 = 1.0f - (2.0f * ((float)y1 + 0.5f)) / (float)height;
+  const uint2 rg = ranges[vw.ctile_off + (ty >> shift) * vw.fp.ctiles_x + (tx >> shift)];
+  const uint32_t tx0 = tx * kTile, tx1 = tx0 + kTile - 1, ty0 = ty * kTile, ty1 = ty0 + 2 * kTile - 1;
+  const float exlo = (2.0f * (float)tx0) / (float)width - 1.0f, exhi = (2.0f * (float)(tx1 + 1)) / (float)width - 1.0f;
+  const float eyhi = 1.0f - (2.0f * (float)ty0) / (float)height, eylo = 1.0f - (2.0f * (float)(ty1 + 1)) / (float)height;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+  PixelAcc p0{1.0f, 0.0f, 0.0f, 0.0f, in0, 0u, 0u}, p1{1.0f, 0.0f, 0.0f, 0.0f, in1, 0u, 0u};
+  uint32_t staged = 0;
+  for (uint32_t start = rg.x;;) {
+    if (__syncthreads_count(p0.live || p1.live) == 0) break;
+    const bool more = start < rg.y;
+    if (more) {
+      const uint32_t k = start + t;
+      uint32_t s = 0;
+      bool keep = false;
+      float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra;
+      if (k < rg.y) {
+        s = list[k];
+        const uint2 r = rect[s];
+        keep = (r.x & 0xFFFFu) <= tx1 && (r.y & 0xFFFFu) >= tx0 && (r.x >> 16) <= ty1 && (r.y >> 16) >= ty0;
+        if (keep) {
+          ra = rec[3 * (size_t)s + 0];
+          rb = rec[3 * (size_t)s + 1];
+          keep = ellipse_meets_rect(ra, rb, exlo, exhi, eylo, eyhi);
+        }
+      }
+      const uint64_t bal = __ballot(keep);
+      if (lane == 0) wkeep[w] = (uint32_t)__popcll(bal);
+      __syncthreads();
+      uint32_t off = staged, kept = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < kWaves; ++q) {
+        off += q < w ? wkeep[q] : 0u;
+        kept += wkeep[q];
+      }
+      if (keep) {
+        const uint32_t pos = off + (uint32_t)__popcll(bal & lt);
+        s0[pos] = ra;
+        s1[pos] = rb;
+        s2[pos] = rec[3 * (size_t)s + 2];
+      }
+      staged += kept;
+      start += kThreads;
+      __syncthreads();
+    }
+    if (staged >= blend_min || (!more || start >= rg.y)) {
+      if (staged > 0) blend_stage2<COUNT>(s0, s1, s2, staged, pnx, pny0, pny1, p0, p1);
+      staged = 0;
+      if (start >= rg.y) break;
+    }
+  }
+  auto q = [](float val) { return (unsigned char)(fminf(fmaxf(val, 0.0f), 1.0f) * 255.0f + 0.5f); };
+  if (in0) {
+    const size_t o = vw.img_off + (size_t)y0 * width + x;
+    const float A = 1.0f - p0.T;
+    img32[o] = make_float4(p0.cr, p0.cg, p0.cb, A);
+    img8[o] = make_uchar4(q(p0.cr), q(p0.cg), q(p0.cb), q(A));
+  }
+  if (in1) {
+    const size_t o = vw.img_off + (size_t)y1 * width + x;
+    const float A = 1.0f - p1.T;
+    img32[o] = make_float4(p1.cr, p1.cg, p1.cb, A);
+    img8[o] = make_uchar4(q(p1.cr), q(p1.cg), q(p1.cb), q(A));
+  }
+  if (COUNT) {
+    uint32_t wp = p0.passed + p1.passed, u = p0.tested + p1.tested;
 #pragma unroll
     for (uint32_t off = kWave / 2; off > 0; off >>= 1) {
       wp += __shfl_down(wp, off, kWave);
@@ -1050,23 +1263,34 @@ void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const u
 }
 
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
-                   uint32_t total_tiles, uint32_t shift, const float4 *rec, const uint2 *rect, const uint2 *ranges,
+                   uint32_t total_tiles, uint32_t shift, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
                    const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t st, unsigned long long *evals) {
   if (total_tiles == 0 || nviews == 0) return;
   if (shift > kMaxBinShift) throw Error(Code::InvalidArgument, "bin shift above 3");
+  if (blend_min == 0 || blend_min > kThreads) throw Error(Code::InvalidArgument, "blend threshold outside 1..256");
   const dim3 grid(total_tiles), block(kThreads);
-  const bool unrolled = variant != RasterVariant::Linear;
+  if (variant == RasterVariant::Rows2 && shift >= 1) {
+    if (evals)
+      k_raster_rows2<true><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, blend_min, rec, rect, ranges,
+                                                   list, img32, img8, evals);
+    else
+      k_raster_rows2<false><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, blend_min, rec, rect, ranges,
+                                                    list, img32, img8, evals);
+    GS_HIP(hipGetLastError());
+    return;
+  }
+  const bool unrolled = variant != RasterVariant::Linear;  // Rows2 at shift 0: Unrolled
   if (unrolled && evals)
-    k_raster<true, 1><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, rec, rect, ranges, list,
+    k_raster<true, 1><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, blend_min, rec, rect, ranges, list,
                                               img32, img8, evals);
   else if (unrolled)
-    k_raster<false, 1><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, rec, rect, ranges,
+    k_raster<false, 1><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, blend_min, rec, rect, ranges,
                                                list, img32, img8, evals);
   else if (evals)
-    k_raster<true, 0><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, rec, rect, ranges, list,
+    k_raster<true, 0><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, blend_min, rec, rect, ranges, list,
                                               img32, img8, evals);
   else
-    k_raster<false, 0><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, rec, rect, ranges,
+    k_raster<false, 0><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, blend_min, rec, rect, ranges,
                                                list, img32, img8, evals);
   GS_HIP(hipGetLastError());
 }

@@ -83,6 +83,8 @@ Pipeline::Pipeline() {
   if (const char *e = std::getenv("GSPLAT_COUNT_EVALS")) count_evals_ = std::atoi(e) != 0;
   if (const char *e = std::getenv("GSPLAT_GRAPH")) use_graph_ = std::atoi(e) != 0;
   if (const char *e = std::getenv("GSPLAT_BIN_SHIFT")) set_bin_shift((uint32_t)std::atoi(e));
+  if (const char *e = std::getenv("GSPLAT_BLEND_MIN"))
+    tuning_.blend_min = std::min<uint32_t>(std::max<uint32_t>((uint32_t)std::strtoul(e, nullptr, 10), 1u), 256u);
   if (const char *e = std::getenv("GSPLAT_SORT_RTS_MIN_TILES"))
     tuning_.rts_min_tiles = std::max<uint32_t>((uint32_t)std::strtoul(e, nullptr, 10), 1u);
   GS_HIP(hipSetDevice(0));
@@ -290,7 +292,7 @@ void Pipeline::launch_frame() {
     // anything but the view table's contents changes.
     std::vector<uint64_t> key = {(uint64_t)r, nviews, slots, tiles, ctiles, tile_bits, (uint64_t)(value_byte + 1),
                                  pair_cap_, hint, tuning_.bin_shift,
-                                 (uint64_t)tuning_.raster, tuning_.rts_min_tiles, count_evals_, depth_clean_,
+                                 (uint64_t)tuning_.raster, tuning_.rts_min_tiles, tuning_.blend_min, count_evals_, depth_clean_,
                                  pair_clean_};
     for (const ViewDev &d : host_views_) {
       key.push_back(reinterpret_cast<uintptr_t>(d.soa));
@@ -428,7 +430,8 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   if (!captured) GS_HIP(hipEventRecord(ev_[3], st));
 
   unsigned long long *evals = count_evals_ ? reinterpret_cast<unsigned long long *>(info + kEvalsWord) : nullptr;
-  launch_raster(tuning_.raster, views_d, tile_start_d, nviews, tiles, tuning_.bin_shift, rec_.get(), rect_.get(),
+  launch_raster(tuning_.raster, views_d, tile_start_d, nviews, tiles, tuning_.bin_shift, tuning_.blend_min,
+                rec_.get(), rect_.get(),
                 ranges_.get(), pv, img32_.get(), img8_.get(), st, evals);
   if (!captured) GS_HIP(hipEventRecord(ev_[4], st));
   for (size_t i = 0; i < host_rgba8_.size(); ++i) {
@@ -514,9 +517,10 @@ void Pipeline::set_bin_shift(uint32_t shift) {
 
 void Pipeline::set_tuning(const Tuning &t) {
   if (t.bin_shift > kMaxBinShift) throw Error(Code::InvalidArgument, "bin shift above " + std::to_string(kMaxBinShift));
-  if (t.raster != RasterVariant::Linear && t.raster != RasterVariant::Unrolled)
+  if (t.raster != RasterVariant::Linear && t.raster != RasterVariant::Unrolled && t.raster != RasterVariant::Rows2)
     throw Error(Code::InvalidArgument, "unknown raster variant");
   if (t.rts_min_tiles == 0) throw Error(Code::InvalidArgument, "sort mode threshold must be positive");
+  if (t.blend_min == 0 || t.blend_min > 256) throw Error(Code::InvalidArgument, "blend threshold outside 1..256");
   wait();
   tuning_ = t;
 }
@@ -527,10 +531,13 @@ std::vector<TuneSample> Pipeline::autotune(const std::vector<ViewSpec> &views, u
   const Tuning base = tuning_;
   TuneSample best{base, 1e30f};
   for (uint32_t shift = 0; shift <= kMaxBinShift; ++shift)
-    for (uint32_t rts : {128u, 256u, 1024u}) {
+    for (uint32_t rts : {128u, 256u, 1024u})
+      for (RasterVariant rv : {RasterVariant::Unrolled, RasterVariant::Rows2}) {
+      if (rv == RasterVariant::Rows2 && shift == 0) continue;  // runs as Unrolled there
       Tuning t = base;
       t.bin_shift = shift;
       t.rts_min_tiles = rts;
+      t.raster = rv;
       set_tuning(t);
       enqueue(views);  // warm: buffers grow, the pair count settles
       wait();

@@ -106,6 +106,7 @@ struct Tuning {
   uint32_t bin_shift = 2;                                // binning tiles of 16 << bin_shift pixels
   RasterVariant raster = RasterVariant::Unrolled;        // blend loop
   uint32_t rts_min_tiles = 256;                          // radix sorts of >= this many tiles: reduce-then-scan
+  uint32_t blend_min = 64;                               // staged splats that start a blend (1..256)
 };
 
 // One candidate measured by Pipeline::autotune.

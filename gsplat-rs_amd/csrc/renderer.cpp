@@ -11,8 +11,10 @@ namespace gs {
 constexpr uint32_t kMaxDimension = 16384;
 
 RasterVariant raster_variant_from_env() {
-  if (const char *v = std::getenv("GSPLAT_RASTER_VARIANT"))
-    return std::atoi(v) == 0 ? RasterVariant::Linear : RasterVariant::Unrolled;
+  if (const char *v = std::getenv("GSPLAT_RASTER_VARIANT")) {
+    const int k = std::atoi(v);
+    return k == 0 ? RasterVariant::Linear : k == 2 ? RasterVariant::Rows2 : RasterVariant::Unrolled;
+  }
   return RasterVariant::Unrolled;
 }
 

@@ -234,22 +234,30 @@ int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out);
  *   bin_shift           splats are binned into tiles of (16 << bin_shift)
  *                       pixels, 0..3; each 16x16 raster tile keeps the
  *                       splats of its binning tile that overlap it
- *   raster_variant      0 per-lane exit, 1 wave-uniform loop (default)
+ *   raster_variant      0 per-lane exit, 1 wave-uniform loop (default),
+ *                       2 wave-uniform loop with two pixels per thread
+ *                       (bin_shift >= 1; runs as 1 at bin_shift 0)
  *   sort_rts_min_tiles  radix sorts of at least this many 4096-key tiles use
  *                       reduce-then-scan passes, shorter ones look-back passes
  *   frame_ms            (autotune output) median device time of a frame
+ *   blend_min           a raster tile blends its staged splats once this
+ *                       many (1..256, default 64) are staged or its list
+ *                       ends; smaller stops a saturated tile's list walk
+ *                       sooner, larger runs longer blend loops
  */
 typedef struct GsplatTuning {
   uint32_t bin_shift;
   uint32_t raster_variant;
   uint32_t sort_rts_min_tiles;
   float frame_ms;
+  uint32_t blend_min;
 } GsplatTuning;
 
 int32_t gsplat_batch_get_tuning(GsplatBatch *batch, GsplatTuning *out_tuning);
 int32_t gsplat_batch_set_tuning(GsplatBatch *batch, const GsplatTuning *tuning);
 /* Renders `views` `reps` times under each candidate (bin_shift 0..3 x
- * sort_rts_min_tiles {128, 256, 1024}), keeps the fastest and returns it;
+ * sort_rts_min_tiles {128, 256, 1024} x raster_variant {1, 2}, variant 2
+ * from bin_shift 1), keeps the fastest and returns it;
  * the batch's last frame is then those views under the kept tuning. */
 int32_t gsplat_batch_autotune(GsplatBatch *batch, const GsplatViewDesc *views, uint32_t count, uint32_t reps,
                               GsplatTuning *out_best);

@@ -208,14 +208,19 @@ def test_autotune_keeps_images(abi):
     for v in range(len(vs)):
         assert_same(batch_view(b, v, scenes[views[v][0]].count), before[v], f"view {v} after autotune")
     # every setting renders the same frame
-    for shift in (0, 3):
-        t.bin_shift, t.raster_variant = shift, 0
+    for shift, variant, blend_min in ((0, 0, 64), (3, 0, 256), (2, 2, 1), (1, 2, 64), (2, 1, 200)):
+        t.bin_shift, t.raster_variant, t.blend_min = shift, variant, blend_min
         b.set_tuning(t)
         b.render(vs)
         for v in range(len(vs)):
             assert_same(batch_view(b, v, scenes[views[v][0]].count), before[v], f"view {v} shift {shift}")
     bad = b.tuning()
     bad.bin_shift = 4
+    with pytest.raises(abi.GsplatError) as e:
+        b.set_tuning(bad)
+    assert e.value.code == abi.INVALID_ARGUMENT
+    bad = b.tuning()
+    bad.blend_min = 0
     with pytest.raises(abi.GsplatError) as e:
         b.set_tuning(bad)
     assert e.value.code == abi.INVALID_ARGUMENT
