@@ -41,6 +41,7 @@ constexpr uint32_t kSegmentAlign = 4096;  // view slot segments align to the sor
 constexpr uint32_t kLocalBits = 24;       // splat index bits in a sort value
 constexpr uint32_t kLocalMask = (1u << kLocalBits) - 1;
 constexpr uint32_t kMaxViews = 256;       // view id fits the value's top byte
+constexpr uint32_t kXcds = 8;             // MI355X: workgroup b is dispatched to XCD b % 8
 constexpr uint32_t kPairViewShift = 24;   // tile-pair key = view << 24 | tile of the view
 constexpr uint32_t kPairTileMask = (1u << kPairViewShift) - 1;
 
@@ -146,12 +147,15 @@ void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s,
 //   slot g (0 for culled and padding slots), offsets_rev its exclusive scan
 //   within the block, block_sums[b] the block's total and block_pairs[b] its
 //   16x16-tile pair count;
-//   visible[v] = V of view v (written by the last visible slot; zero first).
+//   visible[v] = V of view v (written by the last visible slot; zero first);
+//   rect_lo[g], rect_hi[g] = the pixel bounds of sorted slot g's splat (only
+//   meaningful where counts_rev is non-zero), for the emission.
 // The grid first zeroes `clears`.
 void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
                         uint32_t shift, const uint32_t *skeys, const uint32_t *svals, const uint2 *rect,
                         uint32_t *counts_rev, uint32_t *offsets_rev, uint32_t *visible, uint32_t *block_sums,
-                        uint32_t *block_pairs, const Clears &clears, hipStream_t stream);
+                        uint32_t *block_pairs, uint32_t *rect_lo, uint32_t *rect_hi, const Clears &clears,
+                        hipStream_t stream);
 
 // One workgroup: block_sums becomes its exclusive scan (each block's first
 // pair index, so a slot's first pair is block_sums[r / kCountBlock] +
@@ -166,7 +170,8 @@ void launch_block_scan(uint32_t *block_sums, uint32_t blocks, uint32_t capacity,
 // Pairs at or beyond `capacity` are dropped (the frame is then flagged and
 // rendered again).
 void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
-                      uint32_t shift, const uint32_t *svals, const uint2 *rect, const uint32_t *counts_rev,
+                      uint32_t shift, const uint32_t *svals, const uint32_t *rect_lo, const uint32_t *rect_hi,
+                      const uint32_t *counts_rev,
                       const uint32_t *offsets_rev, const uint32_t *block_sums, uint32_t capacity,
                       uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t stream);
 
@@ -183,7 +188,9 @@ void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const u
 void launch_raster_tile_counts(const ViewDev *views, uint32_t v, const ViewDev &host_view, const uint32_t *skeys,
                                const uint32_t *svals, const uint2 *rect, uint32_t *counts, hipStream_t stream);
 
-// Front-to-back blend of every view's 16x16 tiles: one workgroup per tile.
+// Front-to-back blend of every view's 16x16 tiles: one workgroup per tile,
+// the raster tiles of one binning tile consecutive on one XCD (total_ctiles:
+// the batch's binning tiles; ctile_start: each view's first).
 // The workgroup walks its binning tile's list 256 entries at a time, keeps
 // (in order) the splats whose bounds overlap its tile, stages their records
 // in LDS and blends them once 256 are staged or the list ends.
@@ -198,8 +205,8 @@ void launch_raster_tile_counts(const ViewDev *views, uint32_t v, const ViewDev &
 // coverage before each pixel's early stop (the blend's algorithmic work) and
 // to evals[1] the coverage tests made.
 enum class RasterVariant : uint32_t { Linear = 0, Unrolled = 1, Rows2 = 2 };
-void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
-                   uint32_t total_tiles, uint32_t shift, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
+void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *ctile_start, uint32_t nviews,
+                   uint32_t total_ctiles, uint32_t shift, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
                    const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t stream,
                    unsigned long long *evals = nullptr);
 

@@ -506,7 +506,9 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const ViewDev *__restr
                                                           uint32_t *__restrict__ offsets_rev,
                                                           uint32_t *__restrict__ visible,
                                                           uint32_t *__restrict__ block_sums,
-                                                          uint32_t *__restrict__ block_pairs, Clears clears) {
+                                                          uint32_t *__restrict__ block_pairs,
+                                                          uint32_t *__restrict__ rect_lo,
+                                                          uint32_t *__restrict__ rect_hi, Clears clears) {
   __shared__ uint32_t lds4[kWaves];
   __shared__ uint32_t wfine[kWaves];
   clear_ranges(clears);
@@ -536,6 +538,13 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const ViewDev *__restr
   for (uint32_t k = 0; k < 4; ++k) {
     live[k] = key[k] != 0u && local[k] < n_v;
     rc[k] = rect[live[k] ? off_v + local[k] : 0u];  // unconditional: the four loads issue together
+  }
+  // The rects in sorted slot order, for the pair emission to read in order.
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint32_t g = total - 1 - (r0 + k);
+    rect_lo[g] = rc[k].x;
+    rect_hi[g] = rc[k].y;
   }
   uint32_t c[4];
   uint32_t sum = 0, fine = 0;
@@ -612,7 +621,8 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
                                                         const uint32_t *__restrict__ seg_start, uint32_t nviews,
                                                         uint32_t total, uint32_t shift,
                                                         const uint32_t *__restrict__ svals,
-                                                        const uint2 *__restrict__ rect,
+                                                        const uint32_t *__restrict__ rect_lo,
+                                                        const uint32_t *__restrict__ rect_hi,
                                                         const uint32_t *__restrict__ counts_rev,
                                                         const uint32_t *__restrict__ offsets_rev,
                                                         const uint32_t *__restrict__ block_sums, uint32_t capacity,
@@ -629,11 +639,12 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
   const uint32_t r = total - 1 - (g0 + kThreads - 1 - t);
   const uint32_t cnt = counts_rev[r];
   const uint32_t first = block_sums[r / kCountBlock] + offsets_rev[r];
-  const uint32_t slot = vw.seg_off + (svals[g0 + kThreads - 1 - t] - (v << kLocalBits));  // loaded beside cnt
+  const uint32_t g = g0 + kThreads - 1 - t;
+  const uint32_t slot = vw.seg_off + (svals[g] - (v << kLocalBits));  // loaded beside cnt
+  const uint2 b = make_uint2(rect_lo[g], rect_hi[g]);                 // in slot order
   s_first[t] = first;
   if (t == kThreads - 1) s_first[kThreads] = first + cnt;
   if (cnt) {
-    const uint2 b = rect[slot];
     const uint32_t sh = 4 + shift;
     const uint32_t x0 = (b.x & 0xFFFFu) >> sh, x1 = (b.y & 0xFFFFu) >> sh;
     s_slot[t] = slot;
@@ -707,6 +718,29 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
 }
 
 // ------------------------------------------------------------------ raster
+
+// Raster workgroup order: the grid covers every view's binning tiles,
+// padded to whole runs of kXcds, times S = 4^shift raster tiles each.
+// Workgroup b takes global binning tile c = (b / (kXcds S)) kXcds + b % kXcds
+// and its raster tile k = (b / kXcds) % S, so the S raster tiles of one
+// binning tile run back to back on one XCD (workgroups are dealt to the
+// XCDs round robin) and find their shared list's records in its L2. False
+// for padding (no such binning tile, or a raster tile past the view's edge).
+__device__ __forceinline__ bool raster_tile(const ViewDev *__restrict__ views,
+                                            const uint32_t *__restrict__ ctile_start, uint32_t nviews,
+                                            uint32_t total_ctiles, uint32_t shift, uint32_t b, uint32_t &v,
+                                            uint32_t &tx, uint32_t &ty) {
+  const uint32_t S = 1u << (2 * shift);
+  const uint32_t c = (b / (kXcds * S)) * kXcds + b % kXcds;
+  const uint32_t k = (b / kXcds) % S;
+  if (c >= total_ctiles) return false;
+  v = find_view(ctile_start, nviews, c);
+  const ViewDev &vw = views[v];
+  const uint32_t lc = c - vw.ctile_off;
+  tx = ((lc % vw.fp.ctiles_x) << shift) + (k & ((1u << shift) - 1u));
+  ty = ((lc / vw.fp.ctiles_x) << shift) + (k >> shift);
+  return tx < vw.fp.tiles_x && ty < vw.fp.tiles_y;
+}
 
 // One workgroup per 16x16 tile, one thread per pixel. Batches of 256 records
 // are gathered into LDS (coalesced by the 256 threads) and blended front to
@@ -841,8 +875,8 @@ __device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const 
 // and to evals[1] the number of coverage tests made.
 template <bool COUNT, int LOOP>
 __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__ views,
-                                                     const uint32_t *__restrict__ tile_start, uint32_t nviews,
-                                                     uint32_t total_tiles, uint32_t shift, uint32_t blend_min,
+                                                     const uint32_t *__restrict__ ctile_start, uint32_t nviews,
+                                                     uint32_t total_ctiles, uint32_t shift, uint32_t blend_min,
                                                      const float4 *__restrict__ rec, const uint2 *__restrict__ rect,
                                                      const uint2 *__restrict__ ranges,
                                                      const uint32_t *__restrict__ list, float4 *__restrict__ img32,
@@ -851,13 +885,10 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
   constexpr uint32_t kStage = 2 * kThreads;  // a stage below 256 takes one more batch of up to 256
   __shared__ float4 s0[kStage], s1[kStage], s2[kStage];
   __shared__ uint32_t wkeep[kWaves];
-  const uint32_t tile = blockIdx.x;
-  if (tile >= total_tiles) return;
-  const uint32_t v = find_view(tile_start, nviews, tile);
+  uint32_t v, tx, ty;
+  if (!raster_tile(views, ctile_start, nviews, total_ctiles, shift, blockIdx.x, v, tx, ty)) return;
   const ViewDev &vw = views[v];
   const uint32_t width = vw.fp.width, height = vw.fp.height;
-  const uint32_t local = tile - vw.tile_off;
-  const uint32_t tx = local % vw.fp.tiles_x, ty = local / vw.fp.tiles_x;
   const uint32_t t = threadIdx.x, w = t / kWave, lane = __lane_id();
   const uint32_t x = tx * kTile + (t % kTile), y = ty * kTile + (t / kTile);
   const bool inside = x < width && y < height;
@@ -1008,8 +1039,8 @@ __device__ __forceinline__ void blend_stage2(const float4 *s0, float4 *s1, const
 // rows in one binning tile); workgroups of odd tile rows exit at once.
 template <bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_raster_rows2(const ViewDev *__restrict__ views,
-                                                           const uint32_t *__restrict__ tile_start, uint32_t nviews,
-                                                           uint32_t total_tiles, uint32_t shift, uint32_t blend_min,
+                                                           const uint32_t *__restrict__ ctile_start, uint32_t nviews,
+                                                           uint32_t total_ctiles, uint32_t shift, uint32_t blend_min,
                                                            const float4 *__restrict__ rec,
                                                            const uint2 *__restrict__ rect,
                                                            const uint2 *__restrict__ ranges,
@@ -1019,13 +1050,10 @@ __global__ __launch_bounds__(kThreads) void k_raster_rows2(const ViewDev *__rest
   constexpr uint32_t kStage = 2 * kThreads;
   __shared__ float4 s0[kStage], s1[kStage], s2[kStage];
   __shared__ uint32_t wkeep[kWaves];
-  const uint32_t tile = blockIdx.x;
-  if (tile >= total_tiles) return;
-  const uint32_t v = find_view(tile_start, nviews, tile);
+  uint32_t v, tx, ty;
+  if (!raster_tile(views, ctile_start, nviews, total_ctiles, shift, blockIdx.x, v, tx, ty)) return;
   const ViewDev &vw = views[v];
   const uint32_t width = vw.fp.width, height = vw.fp.height;
-  const uint32_t local = tile - vw.tile_off;
-  const uint32_t tx = local % vw.fp.tiles_x, ty = local / vw.fp.tiles_x;
   if (ty & 1u) return;  // covered by the workgroup of row ty - 1
   const uint32_t t = threadIdx.x, w = t / kWave, lane = __lane_id();
   const uint32_t x = tx * kTile + (lane % kTile);
@@ -1214,12 +1242,13 @@ void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s,
 void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
                         uint32_t shift, const uint32_t *skeys, const uint32_t *svals, const uint2 *rect,
                         uint32_t *counts_rev, uint32_t *offsets_rev, uint32_t *visible, uint32_t *block_sums,
-                        uint32_t *block_pairs, const Clears &clr, hipStream_t st) {
+                        uint32_t *block_pairs, uint32_t *rect_lo, uint32_t *rect_hi, const Clears &clr,
+                        hipStream_t st) {
   if (total_slots % kCountBlock) throw Error(Code::Internal, "slot count not a multiple of the count block");
   if (total_slots == 0) return;
   k_tile_counts<<<total_slots / kCountBlock, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, shift, skeys,
                                                                   svals, rect, counts_rev, offsets_rev, visible,
-                                                                  block_sums, block_pairs, clr);
+                                                                  block_sums, block_pairs, rect_lo, rect_hi, clr);
   GS_HIP(hipGetLastError());
 }
 
@@ -1230,12 +1259,14 @@ void launch_block_scan(uint32_t *block_sums, uint32_t blocks, uint32_t capacity,
 }
 
 void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
-                      uint32_t shift, const uint32_t *svals, const uint2 *rect, const uint32_t *counts_rev,
+                      uint32_t shift, const uint32_t *svals, const uint32_t *rect_lo, const uint32_t *rect_hi,
+                      const uint32_t *counts_rev,
                       const uint32_t *offsets_rev, const uint32_t *block_sums, uint32_t capacity,
                       uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t st) {
   if (total_slots == 0) return;
   k_tile_emit<<<total_slots / kThreads, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, shift, svals,
-                                                             rect, counts_rev, offsets_rev, block_sums, capacity,
+                                                             rect_lo, rect_hi, counts_rev, offsets_rev, block_sums,
+                                                             capacity,
                                                              pair_keys, pair_vals);
   GS_HIP(hipGetLastError());
 }
@@ -1262,35 +1293,35 @@ void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const u
   GS_HIP(hipGetLastError());
 }
 
-void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *tile_start, uint32_t nviews,
-                   uint32_t total_tiles, uint32_t shift, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
+void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *ctile_start, uint32_t nviews,
+                   uint32_t total_ctiles, uint32_t shift, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
                    const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t st, unsigned long long *evals) {
-  if (total_tiles == 0 || nviews == 0) return;
+  if (total_ctiles == 0 || nviews == 0) return;
   if (shift > kMaxBinShift) throw Error(Code::InvalidArgument, "bin shift above 3");
   if (blend_min == 0 || blend_min > kThreads) throw Error(Code::InvalidArgument, "blend threshold outside 1..256");
-  const dim3 grid(total_tiles), block(kThreads);
+  const dim3 grid((total_ctiles + kXcds - 1) / kXcds * kXcds << (2 * shift)), block(kThreads);
   if (variant == RasterVariant::Rows2 && shift >= 1) {
     if (evals)
-      k_raster_rows2<true><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, blend_min, rec, rect, ranges,
+      k_raster_rows2<true><<<grid, block, 0, st>>>(views, ctile_start, nviews, total_ctiles, shift, blend_min, rec, rect, ranges,
                                                    list, img32, img8, evals);
     else
-      k_raster_rows2<false><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, blend_min, rec, rect, ranges,
+      k_raster_rows2<false><<<grid, block, 0, st>>>(views, ctile_start, nviews, total_ctiles, shift, blend_min, rec, rect, ranges,
                                                     list, img32, img8, evals);
     GS_HIP(hipGetLastError());
     return;
   }
   const bool unrolled = variant != RasterVariant::Linear;  // Rows2 at shift 0: Unrolled
   if (unrolled && evals)
-    k_raster<true, 1><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, blend_min, rec, rect, ranges, list,
+    k_raster<true, 1><<<grid, block, 0, st>>>(views, ctile_start, nviews, total_ctiles, shift, blend_min, rec, rect, ranges, list,
                                               img32, img8, evals);
   else if (unrolled)
-    k_raster<false, 1><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, blend_min, rec, rect, ranges,
+    k_raster<false, 1><<<grid, block, 0, st>>>(views, ctile_start, nviews, total_ctiles, shift, blend_min, rec, rect, ranges,
                                                list, img32, img8, evals);
   else if (evals)
-    k_raster<true, 0><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, blend_min, rec, rect, ranges, list,
+    k_raster<true, 0><<<grid, block, 0, st>>>(views, ctile_start, nviews, total_ctiles, shift, blend_min, rec, rect, ranges, list,
                                               img32, img8, evals);
   else
-    k_raster<false, 0><<<grid, block, 0, st>>>(views, tile_start, nviews, total_tiles, shift, blend_min, rec, rect, ranges,
+    k_raster<false, 0><<<grid, block, 0, st>>>(views, ctile_start, nviews, total_ctiles, shift, blend_min, rec, rect, ranges,
                                                list, img32, img8, evals);
   GS_HIP(hipGetLastError());
 }

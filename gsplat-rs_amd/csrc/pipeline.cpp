@@ -13,16 +13,17 @@ namespace gs {
 namespace {
 
 // Device view table of a frame, packed in 16-byte aligned sections:
-// seg_start[nviews] | tile_start[nviews] | ViewDev[nviews] | ProjGroup[ngroups].
+// seg_start[nviews] | ctile_start[nviews] | ViewDev[nviews] | ProjGroup[ngroups]
+// (first slot and first global binning tile of each view).
 struct TableLayout {
-  size_t tile_start, views, groups, bytes;
+  size_t ctile_start, views, groups, bytes;
 };
 constexpr size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 constexpr TableLayout table_layout(uint32_t nviews, uint32_t ngroups) {
-  const size_t tile_start = align16((size_t)nviews * sizeof(uint32_t));
-  const size_t views = align16(tile_start + (size_t)nviews * sizeof(uint32_t));
+  const size_t ctile_start = align16((size_t)nviews * sizeof(uint32_t));
+  const size_t views = align16(ctile_start + (size_t)nviews * sizeof(uint32_t));
   const size_t groups = align16(views + (size_t)nviews * sizeof(ViewDev));
-  return TableLayout{tile_start, views, groups, groups + (size_t)ngroups * sizeof(ProjGroup)};
+  return TableLayout{ctile_start, views, groups, groups + (size_t)ngroups * sizeof(ProjGroup)};
 }
 constexpr size_t kTableBytes = table_layout(kMaxViews, kMaxViews).bytes;
 // Frame summary words (device and pinned host copies):
@@ -264,10 +265,10 @@ void Pipeline::launch_frame() {
   uint8_t *host = table_pinned_[r];
   const TableLayout lay = table_layout(nviews, (uint32_t)host_groups_.size());
   uint32_t *seg_start = reinterpret_cast<uint32_t *>(host);
-  uint32_t *tile_start = reinterpret_cast<uint32_t *>(host + lay.tile_start);
+  uint32_t *ctile_start = reinterpret_cast<uint32_t *>(host + lay.ctile_start);
   for (uint32_t v = 0; v < nviews; ++v) {
     seg_start[v] = host_views_[v].seg_off;
-    tile_start[v] = host_views_[v].tile_off;
+    ctile_start[v] = host_views_[v].ctile_off;
   }
   std::memcpy(host + lay.views, host_views_.data(), nviews * sizeof(ViewDev));
   if (!host_groups_.empty())
@@ -355,7 +356,7 @@ void Pipeline::drop_graph(GraphSlot &g) {
 }
 
 void Pipeline::record_frame(const FramePlan &plan, bool captured) {
-  const uint32_t nviews = plan.nviews, slots = plan.slots, tiles = plan.tiles, ctiles = plan.ctiles;
+  const uint32_t nviews = plan.nviews, slots = plan.slots, ctiles = plan.ctiles;
   const uint32_t tile_bits = plan.tile_bits, count_blocks = plan.count_blocks, hint = plan.hint;
   const int value_byte = plan.value_byte, r = plan.slot;
   const bool depth_lookback = plan.depth_lookback, pair_lookback = plan.pair_lookback;
@@ -366,7 +367,7 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   GS_HIP(hipMemcpyAsync(table_.get(), table_pinned_[r], lay.bytes, hipMemcpyHostToDevice, st));
   if (!captured) GS_HIP(hipEventRecord(table_ev_[r], st));
   const uint32_t *seg_start_d = reinterpret_cast<const uint32_t *>(table_.get());
-  const uint32_t *tile_start_d = reinterpret_cast<const uint32_t *>(table_.get() + lay.tile_start);
+  const uint32_t *ctile_start_d = reinterpret_cast<const uint32_t *>(table_.get() + lay.ctile_start);
   const ViewDev *views_d = reinterpret_cast<const ViewDev *>(table_.get() + lay.views);
   const ProjGroup *groups_d = reinterpret_cast<const ProjGroup *>(table_.get() + lay.groups);
   uint32_t *info = summary_.get();
@@ -410,11 +411,13 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   Clears scan_clears;  // the depth sort's scratch, for the next frame
   if (depth_lookback) scan_clears.add(depth_scratch_.get(), depth_dirty);
   depth_clean_ = depth_lookback;
+  // The sort's other buffers (ka, va) are free until the next frame: they
+  // carry the rects in sorted order from the counts to the emission.
   launch_tile_counts(views_d, seg_start_d, nviews, slots, tuning_.bin_shift, k, v, rect_.get(), counts_.get(),
-                     offsets_.get(), visible, block_sums_.get(), block_pairs_.get(), scan_clears, st);
+                     offsets_.get(), visible, block_sums_.get(), block_pairs_.get(), ka, va, scan_clears, st);
   launch_block_scan(block_sums_.get(), count_blocks, pair_cap_, m_dev_.get(), info, block_pairs_.get(), raster_pairs,
                     st);
-  launch_tile_emit(views_d, seg_start_d, nviews, slots, tuning_.bin_shift, v, rect_.get(), counts_.get(), offsets_.get(),
+  launch_tile_emit(views_d, seg_start_d, nviews, slots, tuning_.bin_shift, v, ka, va, counts_.get(), offsets_.get(),
                    block_sums_.get(), pair_cap_, pair_keys_.get(), pair_vals_.get(), st);
   uint32_t *pk = pair_keys_.get(), *pv = pair_vals_.get(), *pka = pair_keys_alt_.get(), *pva = pair_vals_alt_.get();
   SortPrep pair_prep;
@@ -430,7 +433,7 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   if (!captured) GS_HIP(hipEventRecord(ev_[3], st));
 
   unsigned long long *evals = count_evals_ ? reinterpret_cast<unsigned long long *>(info + kEvalsWord) : nullptr;
-  launch_raster(tuning_.raster, views_d, tile_start_d, nviews, tiles, tuning_.bin_shift, tuning_.blend_min,
+  launch_raster(tuning_.raster, views_d, ctile_start_d, nviews, ctiles, tuning_.bin_shift, tuning_.blend_min,
                 rec_.get(), rect_.get(),
                 ranges_.get(), pv, img32_.get(), img8_.get(), st, evals);
   if (!captured) GS_HIP(hipEventRecord(ev_[4], st));
