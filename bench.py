@@ -293,14 +293,14 @@ def pct(vals):
             "max": round(vals[-1], 3)}
 
 
-def bench_served(requests, rate, closed_seconds):
+def bench_served(requests, rate, closed_seconds, pipelines=2, max_batch=64):
     import queue
     import threading
 
     scenes, reqs = synth.served(requests=requests, rate=rate)
     dev = [A.Scene(s) for s in scenes]
     del scenes
-    srv = A.Server(max_batch_views=64)
+    srv = A.Server(max_batch_views=max_batch, pipelines=pipelines)
     ids = [srv.add_scene(d) for d in dev]
     cams = [(ids[v.scene], A.look_at(v.position, v.target), v.width, v.height) for (_, v) in reqs]
     pool = queue.Queue()
@@ -348,7 +348,7 @@ def bench_served(requests, rate, closed_seconds):
                  "mean_batch_views": round(float(np.mean(batch_views)), 2)}
 
     # Closed loop: 64 clients.
-    done, clat = [0], []
+    done, clat, cserver = [0], [], []
     lock = threading.Lock()
     stop_at = time.perf_counter() + closed_seconds
 
@@ -358,9 +358,10 @@ def bench_served(requests, rate, closed_seconds):
         while time.perf_counter() < stop_at:
             sid, cam, w, h = cams[i % len(cams)]
             ts = time.perf_counter()
-            srv.wait(srv.submit(sid, cam, w, h, buf))
+            r = srv.wait(srv.submit(sid, cam, w, h, buf))
             with lock:
                 clat.append((time.perf_counter() - ts) * 1e3)
+                cserver.append(r.total_ms)
                 done[0] += 1
             i += 64
         buf.close()
@@ -372,11 +373,12 @@ def bench_served(requests, rate, closed_seconds):
     for x in th:
         x.join()
     closed = {"clients": 64, "completed_per_s": round(done[0] / (time.perf_counter() - tc), 1),
-              "latency_ms": pct(clat)}
+              "latency_ms": pct(clat), "server_ms": pct(cserver)}
     st = srv.stats()
     srv.close()
     return {"scenes": len(dev), "splats": int(sum(d.count for d in dev)),
             "resident_scene_bytes": int(st.resident_scene_bytes), "memory_budget_bytes": int(st.memory_budget_bytes),
+            "pipelines": pipelines, "max_batch_views": max_batch,
             "poisson": open_loop, "closed64": closed, "batches": int(st.batches),
             "rejected": int(st.rejected)}
 

@@ -882,6 +882,7 @@ GsplatServerConfig gsplat_server_config_default(void) {
   c.memory_budget_bytes = 0;
   c.max_batch_views = 64;
   c.max_queue = 4096;
+  c.pipelines = 2;
   return c;
 }
 
@@ -895,6 +896,7 @@ int32_t gsplat_server_create(const GsplatServerConfig *config, GsplatServer **ou
     c.memory_budget = config->memory_budget_bytes;
     c.max_batch_views = config->max_batch_views;
     c.max_queue = config->max_queue;
+    c.pipelines = config->pipelines;
     auto s = std::make_unique<GsplatServer>();
     s->server = std::make_unique<gs::Server>(c);
     *out_server = s.release();

@@ -65,12 +65,14 @@ Server::Server(const ServerConfig &cfg) : cfg_(cfg) {
     GS_HIP(hipMemGetInfo(&free_b, &total_b));
     cfg_.memory_budget = free_b / 10 * 9;
   }
+  if (cfg_.pipelines == 0 || cfg_.pipelines > 8) throw Error(Code::InvalidArgument, "pipelines must be 1..8");
   stats_.memory_budget = cfg_.memory_budget;
-  for (Slot &s : slots_) {
-    s.pipe = std::make_unique<Pipeline>();
-    s.pipe->set_raster_variant(raster_variant_from_env());
+  for (uint32_t i = 0; i < cfg_.pipelines; ++i) {
+    slots_.push_back(std::make_unique<Slot>());
+    slots_.back()->pipe = std::make_unique<Pipeline>();
+    slots_.back()->pipe->set_raster_variant(raster_variant_from_env());
   }
-  for (Slot &s : slots_) s.completer = std::thread(&Server::complete_loop, this, &s);
+  for (auto &s : slots_) s->completer = std::thread(&Server::complete_loop, this, s.get());
   submitter_ = std::thread(&Server::submit_loop, this);
 }
 
@@ -81,15 +83,15 @@ Server::~Server() {
   }
   submit_cv_.notify_all();
   if (submitter_.joinable()) submitter_.join();
-  for (Slot &s : slots_) {
-    s.cv.notify_all();
-    if (s.completer.joinable()) s.completer.join();
+  for (auto &s : slots_) {
+    s->cv.notify_all();
+    if (s->completer.joinable()) s->completer.join();
   }
 }
 
 uint64_t Server::slot_budget() const {
   const uint64_t free_b = cfg_.memory_budget > resident_bytes_ ? cfg_.memory_budget - resident_bytes_ : 0;
-  return free_b / 2;
+  return free_b / cfg_.pipelines;
 }
 
 uint32_t Server::add_scene(std::shared_ptr<DeviceScene> scene) {
@@ -99,7 +101,7 @@ uint32_t Server::add_scene(std::shared_ptr<DeviceScene> scene) {
   const uint64_t one_view = frame_working_bytes(round_segment(scene->count), 1920ull * 1080, 120ull * 68,
                                                 initial_pairs(scene->count));
   std::lock_guard<std::mutex> g(m_);
-  const uint64_t need = resident_bytes_ + scene->bytes() + 2 * one_view;
+  const uint64_t need = resident_bytes_ + scene->bytes() + cfg_.pipelines * one_view;
   if (need > cfg_.memory_budget)
     throw Error(Code::Unsupported, "scene of " + std::to_string(scene->bytes()) + " bytes does not fit the memory budget (" +
                                        std::to_string(resident_bytes_) + " resident, " +
@@ -176,14 +178,14 @@ void Server::submit_loop() {
     submit_cv_.wait(lk, [&] {
       if (stop_) return true;
       if (queue_.empty()) return false;
-      for (Slot &s : slots_)
-        if (!s.busy) return true;
+      for (auto &s : slots_)
+        if (!s->busy) return true;
       return false;
     });
     if (stop_) break;
-    for (Slot &s : slots_)
-      if (!s.busy) {
-        slot = &s;
+    for (auto &s : slots_)
+      if (!s->busy) {
+        slot = s.get();
         break;
       }
     // Fold queued requests, oldest first, while the batch's working set

@@ -2,9 +2,9 @@
 //
 // Requests (scene, camera, target size) queue on submit(). A submission
 // thread folds every queued request that fits into the next batch as soon as
-// one of the two pipelines is free, so requests arriving while a batch is on
-// the device ride the following one, and the two pipelines' batches overlap
-// on the device (separate streams). A completion thread per pipeline waits for
+// one of the pipelines (ServerConfig::pipelines, 2 by default) is free, so
+// requests arriving while a batch is on the device ride a following one, and
+// the pipelines' batches overlap on the device (separate streams). A completion thread per pipeline waits for
 // its batch, delivers the images and wakes the waiters. Each request's image
 // is the frame a context would render for it alone: batching never changes
 // a view's sort or blend.
@@ -40,6 +40,7 @@ struct ServerConfig {
   uint64_t memory_budget = 0;  // bytes; 0 = 90% of the device memory free at creation
   uint32_t max_batch_views = 64;
   uint32_t max_queue = 4096;
+  uint32_t pipelines = 2;  // batches in flight at once (1..8), one stream each
 };
 
 struct RequestResult {
@@ -117,7 +118,7 @@ class Server {
   uint64_t resident_bytes_ = 0;
   ServerStats stats_{};
   bool stop_ = false;
-  Slot slots_[2];
+  std::vector<std::unique_ptr<Slot>> slots_;
   std::thread submitter_;
 };
 

@@ -132,7 +132,7 @@ class GsplatTuning(ctypes.Structure):
 
 class GsplatServerConfig(ctypes.Structure):
     _fields_ = [("memory_budget_bytes", ctypes.c_uint64), ("max_batch_views", ctypes.c_uint32),
-                ("max_queue", ctypes.c_uint32)]
+                ("max_queue", ctypes.c_uint32), ("pipelines", ctypes.c_uint32)]
 
 
 class GsplatRenderRequest(ctypes.Structure):
@@ -591,9 +591,11 @@ class PinnedImage:
 class Server:
     """Owns one GsplatServer (continuous batching over resident scenes)."""
 
-    def __init__(self, memory_budget_bytes: int = 0, max_batch_views: int = 64, max_queue: int = 4096):
+    def __init__(self, memory_budget_bytes: int = 0, max_batch_views: int = 64, max_queue: int = 4096,
+                 pipelines: int = 2):
         cfg = lib().gsplat_server_config_default()
         cfg.memory_budget_bytes, cfg.max_batch_views, cfg.max_queue = memory_budget_bytes, max_batch_views, max_queue
+        cfg.pipelines = pipelines
         h = P()
         check(lib().gsplat_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h

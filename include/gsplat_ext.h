@@ -285,6 +285,7 @@ typedef struct GsplatServerConfig {
   uint64_t memory_budget_bytes; /* 0: 90% of the device memory free at create */
   uint32_t max_batch_views;     /* views folded into one batch, 1..256 */
   uint32_t max_queue;           /* queued requests before submit refuses */
+  uint32_t pipelines;           /* batches in flight at once, 1..8 (default 2) */
 } GsplatServerConfig;
 
 typedef struct GsplatRenderRequest {

@@ -516,7 +516,17 @@ void oracle_tile_counts(uint32_t count, const float *proj12, uint32_t width, uin
 
 uint32_t oracle_render(const OracleScene *s, const OracleCamera *cam, uint32_t width,
                        uint32_t height, int threads, float *rgba) {
+  return oracle_render_window(s, cam, width, height, 0, 0, width, height, threads, rgba);
+}
+
+// The same frame restricted to pixels [wx0, wx1) x [wy0, wy1): the window is
+// written row-major into rgba ((wx1-wx0) * (wy1-wy0) * 4 floats), every
+// pixel computed exactly as the full render computes it.
+uint32_t oracle_render_window(const OracleScene *s, const OracleCamera *cam, uint32_t width, uint32_t height,
+                              uint32_t wx0, uint32_t wy0, uint32_t wx1, uint32_t wy1, int threads, float *rgba) {
   if (!oracle_camera_valid(cam) || width == 0 || height == 0) return UINT32_MAX;
+  if (wx0 >= wx1 || wy0 >= wy1 || wx1 > width || wy1 > height) return UINT32_MAX;
+  const uint32_t ww = wx1 - wx0, wh = wy1 - wy0;
   auto t0 = std::chrono::steady_clock::now();
   std::vector<uint32_t> keys(s->count), order(s->count);
   uint32_t vis = oracle_preprocess(s, cam, keys.data(), order.data());
@@ -560,6 +570,12 @@ uint32_t oracle_render(const OracleScene *s, const OracleCamera *cam, uint32_t w
     in.x1 = (int)std::min(std::ceil(x1), (float)width - 1.0f);
     in.y0 = (int)std::max(std::floor(y0), 0.0f);
     in.y1 = (int)std::min(std::ceil(y1), (float)height - 1.0f);
+    // clipped to the window (a pixel's result depends only on the splats
+    // that reach it, in the same order)
+    in.x0 = std::max(in.x0, (int)wx0);
+    in.x1 = std::min(in.x1, (int)wx1 - 1);
+    in.y0 = std::max(in.y0, (int)wy0);
+    in.y1 = std::min(in.y1, (int)wy1 - 1);
     if (in.x0 > in.x1 || in.y0 > in.y1) continue;
     inst.push_back(in);
   }
@@ -568,7 +584,7 @@ uint32_t oracle_render(const OracleScene *s, const OracleCamera *cam, uint32_t w
   // Clear to transparent black (lib.rs:2746), then premultiplied "over"
   // back to front: dst = src + dst*(1 - src.a) (draw_pass.rs:75).
   t0 = std::chrono::steady_clock::now();
-  std::fill(rgba, rgba + (size_t)width * height * 4, 0.0f);
+  std::fill(rgba, rgba + (size_t)ww * wh * 4, 0.0f);
   int nthreads = 1;
 #ifdef _OPENMP
   nthreads = threads > 0 ? threads : omp_get_max_threads();
@@ -578,8 +594,8 @@ uint32_t oracle_render(const OracleScene *s, const OracleCamera *cam, uint32_t w
   int bands = nthreads * 4;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
   for (int band = 0; band < bands; ++band) {
-    int by0 = (int)((int64_t)height * band / bands);
-    int by1 = (int)((int64_t)height * (band + 1) / bands) - 1;
+    int by0 = (int)wy0 + (int)((int64_t)wh * band / bands);
+    int by1 = (int)wy0 + (int)((int64_t)wh * (band + 1) / bands) - 1;
     if (by1 < by0) continue;
     for (const Inst &in : inst) {
       int y0 = std::max(in.y0, by0), y1 = std::min(in.y1, by1);
@@ -592,7 +608,7 @@ uint32_t oracle_render(const OracleScene *s, const OracleCamera *cam, uint32_t w
           float g = std::exp(-4.5f * r2);
           float a = in.rgba[3] * g;
           if (a <= 1.0f / 256.0f) continue;
-          float *d = rgba + ((size_t)y * width + x) * 4;
+          float *d = rgba + ((size_t)(y - (int)wy0) * ww + (size_t)(x - (int)wx0)) * 4;
           float one_minus = 1.0f - a;
           d[0] = in.rgba[0] * g + d[0] * one_minus;
           d[1] = in.rgba[1] * g + d[1] * one_minus;

@@ -99,6 +99,13 @@ void oracle_tile_counts(uint32_t count, const float *proj12, uint32_t width, uin
  * `threads` = OpenMP threads for the raster (0 = default). */
 uint32_t oracle_render(const OracleScene *scene, const OracleCamera *camera, uint32_t width,
                        uint32_t height, int threads, float *rgba_out);
+/* oracle_render restricted to the pixel window [wx0, wx1) x [wy0, wy1),
+ * written row-major into a (wx1-wx0) x (wy1-wy0) RGBA float buffer; each
+ * pixel equals the full render's. UINT32_MAX for an empty or out-of-range
+ * window. */
+uint32_t oracle_render_window(const OracleScene *scene, const OracleCamera *camera, uint32_t width,
+                              uint32_t height, uint32_t wx0, uint32_t wy0, uint32_t wx1, uint32_t wy1, int threads,
+                              float *rgba);
 
 /* Number of (pixel, splat) evaluations a front-to-back blend makes that pass
  * the coverage test before each pixel's T < 1e-4 stop (the raster's

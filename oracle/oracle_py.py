@@ -49,6 +49,7 @@ def lib() -> ctypes.CDLL:
             raise RuntimeError(f"oracle not built: {_LIB_PATH} (run `make -C oracle`)")
         _lib = ctypes.CDLL(str(_LIB_PATH))
         _lib.oracle_render.restype = ctypes.c_uint32
+        _lib.oracle_render_window.restype = ctypes.c_uint32
         _lib.oracle_preprocess.restype = ctypes.c_uint32
         _lib.oracle_coverage_cut.restype = ctypes.c_float
         _lib.oracle_coverage_cut.argtypes = [ctypes.c_float]
@@ -167,6 +168,19 @@ def render(scene: Scene, cam, width: int, height: int, threads: int = 0):
                               ctypes.c_int(threads), _ptr(out))
     if vis == 0xFFFFFFFF:
         raise ValueError("invalid camera or size")
+    return out, int(vis)
+
+
+def render_window(scene: Scene, cam, width: int, height: int, x0: int, y0: int, x1: int, y1: int,
+                  threads: int = 0):
+    """Pixels [x0, x1) x [y0, y1) of render(): (rgba float32 [y1-y0, x1-x0, 4], visible_count)."""
+    out = np.zeros((y1 - y0, x1 - x0, 4), np.float32)
+    sc, c = scene.c(), make_camera(cam)
+    vis = lib().oracle_render_window(ctypes.byref(sc), ctypes.byref(c), ctypes.c_uint32(width),
+                                     ctypes.c_uint32(height), ctypes.c_uint32(x0), ctypes.c_uint32(y0),
+                                     ctypes.c_uint32(x1), ctypes.c_uint32(y1), ctypes.c_int(threads), _ptr(out))
+    if vis == 0xFFFFFFFF:
+        raise ValueError("invalid camera, size or window")
     return out, int(vis)
 
 

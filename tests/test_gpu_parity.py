@@ -45,7 +45,7 @@ def check_image(gpu32, gpu8, ref32, label):
     assert frac_within >= 0.999, f"{label}: only {frac_within:.5f} of pixels within 1 LSB"
 
 
-def full_check(abi, oracle, scene, cam, w, h, label, image=True):
+def full_check(abi, oracle, scene, cam, w, h, label, image=True, windows=()):
     say(f"[{label}] N={scene.count} {w}x{h}")
     ctx = render(abi, scene, cam, w, h)
     osc = oracle_scene(oracle, scene)
@@ -77,6 +77,13 @@ def full_check(abi, oracle, scene, cam, w, h, label, image=True):
         ref32, vis = oracle.render(osc, cd, w, h)
         assert vis == len(idx_ref)
         check_image(ctx.rgba32f(), ctx.rgba8(), ref32, label)
+    if windows:
+        img32, img8 = ctx.rgba32f(), ctx.rgba8()
+        for (x0, y0, x1, y1) in windows:
+            say(f"[{label}] oracle window {x0},{y0}..{x1},{y1}")
+            ref32, vis = oracle.render_window(osc, cd, w, h, x0, y0, x1, y1)
+            assert vis == len(idx_ref)
+            check_image(img32[y0:y1, x0:x1], img8[y0:y1, x0:x1], ref32, f"{label} window {x0},{y0}")
     ctx.close()
 
 
@@ -279,6 +286,20 @@ def test_gpu_parity_1080p_sort_modes(gpu, oracle, monkeypatch, mode):
     scene, view = synth.plain_1080p(n=120_000, seed_offset=7)
     cam = gpu.look_at(view.position, view.target)
     full_check(gpu, oracle, scene, cam, view.width, view.height, f"1080p-120k-{mode}")
+
+
+def test_gpu_parity_4k_3m_heavy_tail(gpu, oracle):
+    # 3,000,000 SH3 splats, 10% of them a heavy tail of large splats, at
+    # 3840x2160 (about 244M 16x16 tile pairs): depth keys, permutation,
+    # visible set, projection and per-tile counts exactly over the whole
+    # frame; the image against the oracle in four 256x256 windows (the
+    # oracle's full 4K image takes minutes on the host).
+    scene, view = synth.plain_4k()
+    cam = gpu.look_at(view.position, view.target)
+    w, h = view.width, view.height
+    windows = [(w // 2 - 128, h // 2 - 128, w // 2 + 128, h // 2 + 128), (0, 0, 256, 256),
+               (w - 256, h - 256, w, h), (1000, 1500, 1256, 1756)]
+    full_check(gpu, oracle, scene, cam, w, h, "4k-3m-heavy", image=False, windows=windows)
 
 
 def test_gpu_blend_evals_match_oracle(gpu, oracle, monkeypatch):

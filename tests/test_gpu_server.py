@@ -44,13 +44,14 @@ def workload(abi, count, seed):
     return scenes, reqs
 
 
-def test_server_results_equal_alone(abi):
+@pytest.mark.parametrize("pipelines", [2, 1, 3])
+def test_server_results_equal_alone(abi, pipelines):
     scenes, reqs = workload(abi, 48, 5)
     dev = [abi.Scene(s) for s in scenes]
-    srv = abi.Server(max_batch_views=16)
+    srv = abi.Server(max_batch_views=16, pipelines=pipelines)
     ids = [srv.add_scene(d) for d in dev]
     outs, tickets = [], []
-    say(f"[server burst] {len(reqs)} requests")
+    say(f"[server burst] {len(reqs)} requests, {pipelines} pipelines")
     for i, (s, cam, w, h) in enumerate(reqs):
         out = abi.PinnedImage(w, h) if i % 2 == 0 else np.zeros((h, w, 4), np.uint8)
         outs.append(out)
@@ -138,6 +139,10 @@ def test_server_errors(abi):
     assert e.value.code == abi.INVALID_ARGUMENT
     with pytest.raises(abi.GsplatError):
         abi.Server(max_batch_views=0)
+    for bad in (0, 9):
+        with pytest.raises(abi.GsplatError) as e:
+            abi.Server(pipelines=bad)
+        assert e.value.code == abi.INVALID_ARGUMENT
     # a removed scene no longer takes requests; queued ones still finish
     t = srv.submit(sid, cam, 64, 64)
     srv.remove_scene(sid)

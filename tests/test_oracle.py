@@ -249,6 +249,25 @@ def test_single_splat_analytic_image(oracle):
     assert np.abs(img - exp)[~border].max() < 1e-5
 
 
+def test_render_window_equals_full_crop(oracle):
+    # A window render computes each pixel exactly as the full render does.
+    rng = np.random.default_rng(5)
+    n = 300
+    pos = np.c_[rng.normal(0, 0.4, (n, 2)), rng.uniform(1.5, 3.5, n)]
+    s = oracle.Scene(pos, rng.normal(0, 1.5, n), rng.normal(-2.8, 0.5, (n, 3)),
+                     rng.standard_normal((n, 4)) / 2, rng.normal(0, 0.3, (n, 3)))
+    w, h = 97, 61
+    full, vis = oracle.render(s, default_cam(), w, h)
+    assert (full[..., 3] > 0).mean() > 0.2
+    for x0, y0, x1, y1 in [(0, 0, w, h), (13, 7, 50, 33), (90, 0, 97, 1), (0, 60, 97, 61), (40, 20, 41, 21)]:
+        win, v2 = oracle.render_window(s, default_cam(), w, h, x0, y0, x1, y1)
+        assert v2 == vis
+        assert np.array_equal(win, full[y0:y1, x0:x1]), (x0, y0, x1, y1)
+    for bad in [(5, 5, 5, 9), (0, 0, w + 1, h), (3, 9, 8, 7)]:
+        with pytest.raises(ValueError):
+            oracle.render_window(s, default_cam(), w, h, *bad)
+
+
 def test_front_to_back_equals_back_to_front(oracle):
     # The oracle blends back to front; restate front to back in float64 and compare.
     rng = np.random.default_rng(11)
