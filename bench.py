@@ -445,7 +445,7 @@ def main():
     n, v = st.splat_count, st.visible_count
     pairs = int(st.tile_pairs)
     bin_pairs = int(st.bin_pairs)
-    bin_px = 16 << ctx.tuning().bin_shift
+    bin_px = 16 << st.bin_shift
     bin_tiles = -(-view.width // bin_px) * -(-view.height // bin_px)
     pair_passes = -(-max((bin_tiles - 1).bit_length(), 1) // 8)
     ctx.close()

@@ -638,6 +638,7 @@ gs::Tuning tuning_from(const GsplatTuning &t) {
   out.raster = static_cast<gs::RasterVariant>(t.raster_variant);
   out.rts_min_tiles = t.sort_rts_min_tiles;
   out.blend_min = t.blend_min;
+  out.bin_auto = t.bin_auto != 0 ? 1u : 0u;
   return out;
 }
 
@@ -648,6 +649,8 @@ GsplatTuning tuning_to(const gs::Tuning &t, float ms) {
   out.sort_rts_min_tiles = t.rts_min_tiles;
   out.frame_ms = ms;
   out.blend_min = t.blend_min;
+  out.bin_auto = t.bin_auto;
+  out.bin_auto = t.bin_auto != 0 ? 1u : 0u;
   return out;
 }
 
@@ -801,6 +804,7 @@ int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out) 
     out->blend_evals = t.blend_evals;
     out->blend_tests = t.blend_tests;
     out->bin_pairs = batch->pipe.bin_pairs();
+    out->bin_shift = batch->pipe.view(0).bin_shift;
     return ok();
   });
 }

@@ -56,11 +56,19 @@ struct ViewDev {
   uint32_t seg_off, seg_len;
   uint32_t tile_off, tiles;    // global 16x16 raster tiles
   uint32_t ctile_off, ctiles;  // global binning tiles
+  uint32_t bin_shift;          // binning tiles of (16 << bin_shift) pixels
+  uint32_t rblock_off;         // first raster workgroup (see launch_raster)
   uint64_t img_off;
 };
 
 // Binning tile grid of a view for bin shift `shift`.
 void set_bin_tiles(FrameParams &fp, uint32_t shift);
+
+// Raster workgroups of a view (binning tiles in whole runs of kXcds, 4^shift
+// raster tiles each).
+inline uint32_t raster_blocks(uint32_t ctiles, uint32_t shift) {
+  return (ctiles + kXcds - 1) / kXcds * kXcds << (2 * shift);
+}
 
 // Projected splat, 3 x float4 per slot (48 B):
 //   [cx, cy, i00, i01] [i10, i11, cut, alpha] [r*a, g*a, b*a, 0]
@@ -152,7 +160,7 @@ void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s,
 //   meaningful where counts_rev is non-zero), for the emission.
 // The grid first zeroes `clears`.
 void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
-                        uint32_t shift, const uint32_t *skeys, const uint32_t *svals, const uint2 *rect,
+                        const uint32_t *skeys, const uint32_t *svals, const uint2 *rect,
                         uint32_t *counts_rev, uint32_t *offsets_rev, uint32_t *visible, uint32_t *block_sums,
                         uint32_t *block_pairs, uint32_t *rect_lo, uint32_t *rect_hi, const Clears &clears,
                         hipStream_t stream);
@@ -170,7 +178,7 @@ void launch_block_scan(uint32_t *block_sums, uint32_t blocks, uint32_t capacity,
 // Pairs at or beyond `capacity` are dropped (the frame is then flagged and
 // rendered again).
 void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
-                      uint32_t shift, const uint32_t *svals, const uint32_t *rect_lo, const uint32_t *rect_hi,
+                      const uint32_t *svals, const uint32_t *rect_lo, const uint32_t *rect_hi,
                       const uint32_t *counts_rev,
                       const uint32_t *offsets_rev, const uint32_t *block_sums, uint32_t capacity,
                       uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t stream);
@@ -189,8 +197,10 @@ void launch_raster_tile_counts(const ViewDev *views, uint32_t v, const ViewDev &
                                const uint32_t *svals, const uint2 *rect, uint32_t *counts, hipStream_t stream);
 
 // Front-to-back blend of every view's 16x16 tiles: one workgroup per tile,
-// the raster tiles of one binning tile consecutive on one XCD (total_ctiles:
-// the batch's binning tiles; ctile_start: each view's first).
+// the raster tiles of one binning tile consecutive on one XCD. View v owns
+// the workgroups [rblock_start[v], rblock_start[v] + raster_blocks(view v))
+// (raster_blocks: the view's binning tiles rounded up to whole kXcds runs,
+// times 4^bin_shift); total_rblocks is their sum over the batch.
 // The workgroup walks its binning tile's list 256 entries at a time, keeps
 // (in order) the splats whose bounds overlap its tile, stages their records
 // in LDS and blends them once 256 are staged or the list ends.
@@ -198,15 +208,15 @@ void launch_raster_tile_counts(const ViewDev *views, uint32_t v, const ViewDev &
 //   Unrolled  wave-uniform splat loop, four splats per wave liveness check
 //             (default)
 //   Rows2     as Unrolled with two pixels per thread (two tile rows per
-//             workgroup, one LDS read of a splat for both pixels); needs a
-//             bin shift >= 1 and runs as Unrolled at shift 0
+//             workgroup, one LDS read of a splat for both pixels); every
+//             view's bin shift must be >= 1 (both rows in one binning tile)
 // All give bit-identical images. With `evals`, the counting variant also
 // adds to evals[0] the number of (pixel, splat) evaluations that pass
 // coverage before each pixel's early stop (the blend's algorithmic work) and
 // to evals[1] the coverage tests made.
 enum class RasterVariant : uint32_t { Linear = 0, Unrolled = 1, Rows2 = 2 };
-void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *ctile_start, uint32_t nviews,
-                   uint32_t total_ctiles, uint32_t shift, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
+void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *rblock_start, uint32_t nviews,
+                   uint32_t total_rblocks, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
                    const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t stream,
                    unsigned long long *evals = nullptr);
 

@@ -498,7 +498,7 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x)
 // segment, since segments align to 4096 slots).
 __global__ __launch_bounds__(kThreads) void k_tile_counts(const ViewDev *__restrict__ views,
                                                           const uint32_t *__restrict__ seg_start, uint32_t nviews,
-                                                          uint32_t total, uint32_t shift,
+                                                          uint32_t total,
                                                           const uint32_t *__restrict__ skeys,
                                                           const uint32_t *__restrict__ svals,
                                                           const uint2 *__restrict__ rect,
@@ -517,6 +517,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const ViewDev *__restr
   const uint32_t slot_lo = total - kCountBlock * (b + 1);  // lowest slot of the block
   const uint32_t v = find_view(seg_start, nviews, slot_lo);
   const ViewDev &vw = views[v];
+  const uint32_t shift = vw.bin_shift;
   const uint32_t r0 = b * kCountBlock + t * 4;
   // Loads first (keys, values, then the rects they name), so that each
   // level's memory latency is paid once rather than once per item.
@@ -619,7 +620,7 @@ __global__ __launch_bounds__(kThreads) void k_block_scan(uint32_t *__restrict__ 
 // every thread finding its pair's slot by binary search.
 __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restrict__ views,
                                                         const uint32_t *__restrict__ seg_start, uint32_t nviews,
-                                                        uint32_t total, uint32_t shift,
+                                                        uint32_t total,
                                                         const uint32_t *__restrict__ svals,
                                                         const uint32_t *__restrict__ rect_lo,
                                                         const uint32_t *__restrict__ rect_hi,
@@ -645,7 +646,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
   s_first[t] = first;
   if (t == kThreads - 1) s_first[kThreads] = first + cnt;
   if (cnt) {
-    const uint32_t sh = 4 + shift;
+    const uint32_t sh = 4 + vw.bin_shift;
     const uint32_t x0 = (b.x & 0xFFFFu) >> sh, x1 = (b.y & 0xFFFFu) >> sh;
     s_slot[t] = slot;
     s_x0[t] = x0;
@@ -719,26 +720,29 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
 
 // ------------------------------------------------------------------ raster
 
-// Raster workgroup order: the grid covers every view's binning tiles,
-// padded to whole runs of kXcds, times S = 4^shift raster tiles each.
-// Workgroup b takes global binning tile c = (b / (kXcds S)) kXcds + b % kXcds
-// and its raster tile k = (b / kXcds) % S, so the S raster tiles of one
-// binning tile run back to back on one XCD (workgroups are dealt to the
-// XCDs round robin) and find their shared list's records in its L2. False
-// for padding (no such binning tile, or a raster tile past the view's edge).
+// Raster workgroup order. View v owns the workgroups [rblock_start[v],
+// rblock_start[v] + raster_blocks) (its binning tiles in whole runs of kXcds,
+// times S = 4^shift raster tiles each, shift the view's bin shift). Its local
+// workgroup l takes binning tile c = (l / (kXcds S)) kXcds + l % kXcds and
+// raster tile k = (l / kXcds) % S of it, so the S raster tiles of one binning
+// tile run back to back on one XCD (workgroups are dealt to the XCDs round
+// robin, and view block ranges start on multiples of kXcds) and find their
+// shared list's records in its L2. False for padding (no such binning tile,
+// or a raster tile past the view's edge).
 __device__ __forceinline__ bool raster_tile(const ViewDev *__restrict__ views,
-                                            const uint32_t *__restrict__ ctile_start, uint32_t nviews,
-                                            uint32_t total_ctiles, uint32_t shift, uint32_t b, uint32_t &v,
-                                            uint32_t &tx, uint32_t &ty) {
-  const uint32_t S = 1u << (2 * shift);
-  const uint32_t c = (b / (kXcds * S)) * kXcds + b % kXcds;
-  const uint32_t k = (b / kXcds) % S;
-  if (c >= total_ctiles) return false;
-  v = find_view(ctile_start, nviews, c);
+                                            const uint32_t *__restrict__ rblock_start, uint32_t nviews,
+                                            uint32_t total_rblocks, uint32_t b, uint32_t &v, uint32_t &tx,
+                                            uint32_t &ty) {
+  if (b >= total_rblocks) return false;
+  v = find_view(rblock_start, nviews, b);
   const ViewDev &vw = views[v];
-  const uint32_t lc = c - vw.ctile_off;
-  tx = ((lc % vw.fp.ctiles_x) << shift) + (k & ((1u << shift) - 1u));
-  ty = ((lc / vw.fp.ctiles_x) << shift) + (k >> shift);
+  const uint32_t shift = vw.bin_shift, S = 1u << (2 * shift);
+  const uint32_t l = b - vw.rblock_off;
+  const uint32_t c = (l / (kXcds * S)) * kXcds + l % kXcds;
+  const uint32_t k = (l / kXcds) % S;
+  if (c >= vw.ctiles) return false;
+  tx = ((c % vw.fp.ctiles_x) << shift) + (k & ((1u << shift) - 1u));
+  ty = ((c / vw.fp.ctiles_x) << shift) + (k >> shift);
   return tx < vw.fp.tiles_x && ty < vw.fp.tiles_y;
 }
 
@@ -875,8 +879,8 @@ __device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const 
 // and to evals[1] the number of coverage tests made.
 template <bool COUNT, int LOOP>
 __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__ views,
-                                                     const uint32_t *__restrict__ ctile_start, uint32_t nviews,
-                                                     uint32_t total_ctiles, uint32_t shift, uint32_t blend_min,
+                                                     const uint32_t *__restrict__ rblock_start, uint32_t nviews,
+                                                     uint32_t total_rblocks, uint32_t blend_min,
                                                      const float4 *__restrict__ rec, const uint2 *__restrict__ rect,
                                                      const uint2 *__restrict__ ranges,
                                                      const uint32_t *__restrict__ list, float4 *__restrict__ img32,
@@ -886,8 +890,9 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
   __shared__ float4 s0[kStage], s1[kStage], s2[kStage];
   __shared__ uint32_t wkeep[kWaves];
   uint32_t v, tx, ty;
-  if (!raster_tile(views, ctile_start, nviews, total_ctiles, shift, blockIdx.x, v, tx, ty)) return;
+  if (!raster_tile(views, rblock_start, nviews, total_rblocks, blockIdx.x, v, tx, ty)) return;
   const ViewDev &vw = views[v];
+  const uint32_t shift = vw.bin_shift;
   const uint32_t width = vw.fp.width, height = vw.fp.height;
   const uint32_t t = threadIdx.x, w = t / kWave, lane = __lane_id();
   const uint32_t x = tx * kTile + (t % kTile), y = ty * kTile + (t / kTile);
@@ -1039,8 +1044,8 @@ __device__ __forceinline__ void blend_stage2(const float4 *s0, float4 *s1, const
 // rows in one binning tile); workgroups of odd tile rows exit at once.
 template <bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_raster_rows2(const ViewDev *__restrict__ views,
-                                                           const uint32_t *__restrict__ ctile_start, uint32_t nviews,
-                                                           uint32_t total_ctiles, uint32_t shift, uint32_t blend_min,
+                                                           const uint32_t *__restrict__ rblock_start, uint32_t nviews,
+                                                           uint32_t total_rblocks, uint32_t blend_min,
                                                            const float4 *__restrict__ rec,
                                                            const uint2 *__restrict__ rect,
                                                            const uint2 *__restrict__ ranges,
@@ -1051,8 +1056,9 @@ __global__ __launch_bounds__(kThreads) void k_raster_rows2(const ViewDev *__rest
   __shared__ float4 s0[kStage], s1[kStage], s2[kStage];
   __shared__ uint32_t wkeep[kWaves];
   uint32_t v, tx, ty;
-  if (!raster_tile(views, ctile_start, nviews, total_ctiles, shift, blockIdx.x, v, tx, ty)) return;
+  if (!raster_tile(views, rblock_start, nviews, total_rblocks, blockIdx.x, v, tx, ty)) return;
   const ViewDev &vw = views[v];
+  const uint32_t shift = vw.bin_shift;
   const uint32_t width = vw.fp.width, height = vw.fp.height;
   if (ty & 1u) return;  // covered by the workgroup of row ty - 1
   const uint32_t t = threadIdx.x, w = t / kWave, lane = __lane_id();
@@ -1240,13 +1246,13 @@ void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s,
 }
 
 void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
-                        uint32_t shift, const uint32_t *skeys, const uint32_t *svals, const uint2 *rect,
+                        const uint32_t *skeys, const uint32_t *svals, const uint2 *rect,
                         uint32_t *counts_rev, uint32_t *offsets_rev, uint32_t *visible, uint32_t *block_sums,
                         uint32_t *block_pairs, uint32_t *rect_lo, uint32_t *rect_hi, const Clears &clr,
                         hipStream_t st) {
   if (total_slots % kCountBlock) throw Error(Code::Internal, "slot count not a multiple of the count block");
   if (total_slots == 0) return;
-  k_tile_counts<<<total_slots / kCountBlock, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, shift, skeys,
+  k_tile_counts<<<total_slots / kCountBlock, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, skeys,
                                                                   svals, rect, counts_rev, offsets_rev, visible,
                                                                   block_sums, block_pairs, rect_lo, rect_hi, clr);
   GS_HIP(hipGetLastError());
@@ -1259,12 +1265,12 @@ void launch_block_scan(uint32_t *block_sums, uint32_t blocks, uint32_t capacity,
 }
 
 void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
-                      uint32_t shift, const uint32_t *svals, const uint32_t *rect_lo, const uint32_t *rect_hi,
+                      const uint32_t *svals, const uint32_t *rect_lo, const uint32_t *rect_hi,
                       const uint32_t *counts_rev,
                       const uint32_t *offsets_rev, const uint32_t *block_sums, uint32_t capacity,
                       uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t st) {
   if (total_slots == 0) return;
-  k_tile_emit<<<total_slots / kThreads, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, shift, svals,
+  k_tile_emit<<<total_slots / kThreads, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, svals,
                                                              rect_lo, rect_hi, counts_rev, offsets_rev, block_sums,
                                                              capacity,
                                                              pair_keys, pair_vals);
@@ -1293,35 +1299,34 @@ void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const u
   GS_HIP(hipGetLastError());
 }
 
-void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *ctile_start, uint32_t nviews,
-                   uint32_t total_ctiles, uint32_t shift, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
+void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *rblock_start, uint32_t nviews,
+                   uint32_t total_rblocks, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
                    const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t st, unsigned long long *evals) {
-  if (total_ctiles == 0 || nviews == 0) return;
-  if (shift > kMaxBinShift) throw Error(Code::InvalidArgument, "bin shift above 3");
+  if (total_rblocks == 0 || nviews == 0) return;
   if (blend_min == 0 || blend_min > kThreads) throw Error(Code::InvalidArgument, "blend threshold outside 1..256");
-  const dim3 grid((total_ctiles + kXcds - 1) / kXcds * kXcds << (2 * shift)), block(kThreads);
-  if (variant == RasterVariant::Rows2 && shift >= 1) {
+  const dim3 grid(total_rblocks), block(kThreads);
+  if (variant == RasterVariant::Rows2) {  // every view at bin shift >= 1 (the caller's duty)
     if (evals)
-      k_raster_rows2<true><<<grid, block, 0, st>>>(views, ctile_start, nviews, total_ctiles, shift, blend_min, rec, rect, ranges,
+      k_raster_rows2<true><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, blend_min, rec, rect, ranges,
                                                    list, img32, img8, evals);
     else
-      k_raster_rows2<false><<<grid, block, 0, st>>>(views, ctile_start, nviews, total_ctiles, shift, blend_min, rec, rect, ranges,
+      k_raster_rows2<false><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, blend_min, rec, rect, ranges,
                                                     list, img32, img8, evals);
     GS_HIP(hipGetLastError());
     return;
   }
-  const bool unrolled = variant != RasterVariant::Linear;  // Rows2 at shift 0: Unrolled
+  const bool unrolled = variant != RasterVariant::Linear;
   if (unrolled && evals)
-    k_raster<true, 1><<<grid, block, 0, st>>>(views, ctile_start, nviews, total_ctiles, shift, blend_min, rec, rect, ranges, list,
+    k_raster<true, 1><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, blend_min, rec, rect, ranges, list,
                                               img32, img8, evals);
   else if (unrolled)
-    k_raster<false, 1><<<grid, block, 0, st>>>(views, ctile_start, nviews, total_ctiles, shift, blend_min, rec, rect, ranges,
+    k_raster<false, 1><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, blend_min, rec, rect, ranges,
                                                list, img32, img8, evals);
   else if (evals)
-    k_raster<true, 0><<<grid, block, 0, st>>>(views, ctile_start, nviews, total_ctiles, shift, blend_min, rec, rect, ranges, list,
+    k_raster<true, 0><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, blend_min, rec, rect, ranges, list,
                                               img32, img8, evals);
   else
-    k_raster<false, 0><<<grid, block, 0, st>>>(views, ctile_start, nviews, total_ctiles, shift, blend_min, rec, rect, ranges,
+    k_raster<false, 0><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, blend_min, rec, rect, ranges,
                                                list, img32, img8, evals);
   GS_HIP(hipGetLastError());
 }

@@ -2,6 +2,7 @@
 
 #include "pipeline.h"
 
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
@@ -13,17 +14,17 @@ namespace gs {
 namespace {
 
 // Device view table of a frame, packed in 16-byte aligned sections:
-// seg_start[nviews] | ctile_start[nviews] | ViewDev[nviews] | ProjGroup[ngroups]
-// (first slot and first global binning tile of each view).
+// seg_start[nviews] | rblock_start[nviews] | ViewDev[nviews] | ProjGroup[ngroups]
+// (first slot and first raster workgroup of each view).
 struct TableLayout {
-  size_t ctile_start, views, groups, bytes;
+  size_t rblock_start, views, groups, bytes;
 };
 constexpr size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 constexpr TableLayout table_layout(uint32_t nviews, uint32_t ngroups) {
-  const size_t ctile_start = align16((size_t)nviews * sizeof(uint32_t));
-  const size_t views = align16(ctile_start + (size_t)nviews * sizeof(uint32_t));
+  const size_t rblock_start = align16((size_t)nviews * sizeof(uint32_t));
+  const size_t views = align16(rblock_start + (size_t)nviews * sizeof(uint32_t));
   const size_t groups = align16(views + (size_t)nviews * sizeof(ViewDev));
-  return TableLayout{ctile_start, views, groups, groups + (size_t)ngroups * sizeof(ProjGroup)};
+  return TableLayout{rblock_start, views, groups, groups + (size_t)ngroups * sizeof(ProjGroup)};
 }
 constexpr size_t kTableBytes = table_layout(kMaxViews, kMaxViews).bytes;
 // Frame summary words (device and pinned host copies):
@@ -50,6 +51,34 @@ uint32_t initial_pair_capacity(uint64_t splats) {
   return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * splats, 1u << 20), 1u << 30);
 }
 
+// Binning tile of a view: about `bin_auto_k` splat diameters, no larger
+// than the tuning's. A splat's on-screen radius is estimated as 3 sigma of
+// the scene's median largest scale at the depth of the scene's centroid
+// (at least half the scene's RMS radius away). Splats that are small next to
+// the binning tile make each 16x16 tile scan a list mostly of splats outside
+// it; large ones multiply the pairs of small binning tiles.
+float bin_auto_k() {
+  static const float k = [] {
+    const char *e = std::getenv("GSPLAT_BIN_AUTO_K");
+    const float v = e ? std::strtof(e, nullptr) : 2.0f;
+    return v > 0.0f ? v : 2.0f;
+  }();
+  return k;
+}
+
+uint32_t view_bin_shift(const DeviceScene &sc, const FrameParams &fp, const Tuning &t) {
+  const uint32_t lo = t.raster == RasterVariant::Rows2 ? 1u : 0u;  // two tile rows share a list
+  if (!t.bin_auto || !(sc.median_scale > 0.0f)) return std::max(t.bin_shift, lo);
+  const float rel[3] = {sc.center[0] - fp.cam[0], sc.center[1] - fp.cam[1], sc.center[2] - fp.cam[2]};
+  const float z = std::max(fp.rows[6] * rel[0] + fp.rows[7] * rel[1] + fp.rows[8] * rel[2],
+                           std::max(0.5f * sc.radius, 1e-3f));
+  const float r_px = 3.0f * sc.median_scale * fp.f * 0.5f * (float)fp.height / z;
+  const float tile = bin_auto_k() * 2.0f * r_px;
+  uint32_t s = 0;
+  while (s < t.bin_shift && (float)(kTile << (s + 1)) <= tile) ++s;
+  return std::max(s, lo);
+}
+
 }  // namespace
 
 bool device_present() {
@@ -62,6 +91,24 @@ std::shared_ptr<DeviceScene> upload_scene(const HostScene &s, hipStream_t stream
   auto d = std::make_shared<DeviceScene>();
   d->count = p.count;
   d->sh_degree = p.sh_degree;
+  if (s.count > 0 && s.positions.size() >= 3 * (size_t)s.count && s.scale_xyz.size() >= 3 * (size_t)s.count) {
+    double c[3] = {0.0, 0.0, 0.0};
+    for (uint32_t i = 0; i < s.count; ++i)
+      for (int a = 0; a < 3; ++a) c[a] += s.positions[3 * (size_t)i + a];
+    for (int a = 0; a < 3; ++a) d->center[a] = (float)(c[a] / s.count);
+    double r2 = 0.0;
+    std::vector<float> ls(s.count);
+    for (uint32_t i = 0; i < s.count; ++i) {
+      for (int a = 0; a < 3; ++a) {
+        const double q = s.positions[3 * (size_t)i + a] - d->center[a];
+        r2 += q * q;
+      }
+      ls[i] = std::max(s.scale_xyz[3 * (size_t)i], std::max(s.scale_xyz[3 * (size_t)i + 1], s.scale_xyz[3 * (size_t)i + 2]));
+    }
+    d->radius = (float)std::sqrt(r2 / s.count);
+    std::nth_element(ls.begin(), ls.begin() + ls.size() / 2, ls.end());
+    d->median_scale = std::exp(ls[ls.size() / 2]);
+  }
   d->soa.ensure(p.soa.size());
   if (!p.soa.empty()) {
     GS_HIP(hipMemcpyAsync(d->soa.get(), p.soa.data(), p.soa.size() * sizeof(float), hipMemcpyHostToDevice, stream));
@@ -84,6 +131,7 @@ Pipeline::Pipeline() {
   if (const char *e = std::getenv("GSPLAT_COUNT_EVALS")) count_evals_ = std::atoi(e) != 0;
   if (const char *e = std::getenv("GSPLAT_GRAPH")) use_graph_ = std::atoi(e) != 0;
   if (const char *e = std::getenv("GSPLAT_BIN_SHIFT")) set_bin_shift((uint32_t)std::atoi(e));
+  if (const char *e = std::getenv("GSPLAT_BIN_AUTO")) tuning_.bin_auto = std::atoi(e) != 0;
   if (const char *e = std::getenv("GSPLAT_BLEND_MIN"))
     tuning_.blend_min = std::min<uint32_t>(std::max<uint32_t>((uint32_t)std::strtoul(e, nullptr, 10), 1u), 256u);
   if (const char *e = std::getenv("GSPLAT_SORT_RTS_MIN_TILES"))
@@ -139,7 +187,7 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
   }
   const bool multi = views.size() > 1;
   std::vector<ViewDev> table(views.size());
-  uint64_t slots = 0, tiles = 0, ctiles = 0, pixels = 0, splats = 0;
+  uint64_t slots = 0, tiles = 0, ctiles = 0, rblocks = 0, pixels = 0, splats = 0;
   for (size_t v = 0; v < views.size(); ++v) {
     const ViewSpec &s = views[v];
     if (!s.scene) throw Error(Code::SceneNotLoaded, "scene not loaded");
@@ -149,7 +197,8 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
                                          std::to_string(kLocalMask) + " splats");
     ViewDev &d = table[v];
     d.fp = s.fp;
-    set_bin_tiles(d.fp, tuning_.bin_shift);
+    d.bin_shift = view_bin_shift(*s.scene, s.fp, tuning_);
+    set_bin_tiles(d.fp, d.bin_shift);
     d.soa = s.scene->soa.get();
     d.n = n;
     d.sh_degree = s.scene->sh_degree;
@@ -159,6 +208,8 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
     d.tiles = d.fp.tiles_x * d.fp.tiles_y;
     d.ctile_off = (uint32_t)ctiles;
     d.ctiles = d.fp.ctiles_x * d.fp.ctiles_y;
+    d.rblock_off = (uint32_t)rblocks;
+    rblocks += raster_blocks(d.ctiles, d.bin_shift);
     d.img_off = pixels;
     slots += d.seg_len;
     tiles += d.tiles;
@@ -167,7 +218,7 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
     splats += n;
     if (slots > (1ull << 30)) throw Error(Code::Unsupported, "batch exceeds 2^30 splat slots");
   }
-  if (tiles >= (1ull << 30)) throw Error(Code::Unsupported, "batch exceeds 2^30 tiles");
+  if (tiles >= (1ull << 30) || rblocks >= (1ull << 31)) throw Error(Code::Unsupported, "batch exceeds 2^30 tiles");
 
   // Projection groups of a batch: views of one scene (at most kGroupViews
   // each) project together, reading the scene once. Groups are ordered by
@@ -210,6 +261,7 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
   total_slots_ = (uint32_t)slots;
   total_tiles_ = (uint32_t)tiles;
   total_ctiles_ = (uint32_t)ctiles;
+  total_rblocks_ = (uint32_t)rblocks;
   total_pixels_ = pixels;
   total_splats_ = splats;
   if (pair_cap_ == 0) pair_cap_ = initial_pair_capacity(splats);
@@ -265,10 +317,10 @@ void Pipeline::launch_frame() {
   uint8_t *host = table_pinned_[r];
   const TableLayout lay = table_layout(nviews, (uint32_t)host_groups_.size());
   uint32_t *seg_start = reinterpret_cast<uint32_t *>(host);
-  uint32_t *ctile_start = reinterpret_cast<uint32_t *>(host + lay.ctile_start);
+  uint32_t *rblock_start = reinterpret_cast<uint32_t *>(host + lay.rblock_start);
   for (uint32_t v = 0; v < nviews; ++v) {
     seg_start[v] = host_views_[v].seg_off;
-    ctile_start[v] = host_views_[v].ctile_off;
+    rblock_start[v] = host_views_[v].rblock_off;
   }
   std::memcpy(host + lay.views, host_views_.data(), nviews * sizeof(ViewDev));
   if (!host_groups_.empty())
@@ -293,12 +345,13 @@ void Pipeline::launch_frame() {
     // anything but the view table's contents changes.
     std::vector<uint64_t> key = {(uint64_t)r, nviews, slots, tiles, ctiles, tile_bits, (uint64_t)(value_byte + 1),
                                  pair_cap_, hint, tuning_.bin_shift,
-                                 (uint64_t)tuning_.raster, tuning_.rts_min_tiles, tuning_.blend_min, count_evals_, depth_clean_,
+                                 (uint64_t)tuning_.raster, tuning_.rts_min_tiles, tuning_.blend_min, tuning_.bin_auto, count_evals_, depth_clean_,
                                  pair_clean_};
     for (const ViewDev &d : host_views_) {
       key.push_back(reinterpret_cast<uintptr_t>(d.soa));
       key.push_back(((uint64_t)d.n << 8) | d.sh_degree);
       key.push_back(((uint64_t)d.fp.width << 32) | d.fp.height);
+      key.push_back(d.bin_shift);
     }
     for (const void *ptr : {(const void *)keys_.get(), (const void *)vals_.get(), (const void *)keys_alt_.get(),
                             (const void *)vals_alt_.get(), (const void *)rec_.get(), (const void *)rect_.get(),
@@ -367,7 +420,7 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   GS_HIP(hipMemcpyAsync(table_.get(), table_pinned_[r], lay.bytes, hipMemcpyHostToDevice, st));
   if (!captured) GS_HIP(hipEventRecord(table_ev_[r], st));
   const uint32_t *seg_start_d = reinterpret_cast<const uint32_t *>(table_.get());
-  const uint32_t *ctile_start_d = reinterpret_cast<const uint32_t *>(table_.get() + lay.ctile_start);
+  const uint32_t *rblock_start_d = reinterpret_cast<const uint32_t *>(table_.get() + lay.rblock_start);
   const ViewDev *views_d = reinterpret_cast<const ViewDev *>(table_.get() + lay.views);
   const ProjGroup *groups_d = reinterpret_cast<const ProjGroup *>(table_.get() + lay.groups);
   uint32_t *info = summary_.get();
@@ -413,11 +466,11 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   depth_clean_ = depth_lookback;
   // The sort's other buffers (ka, va) are free until the next frame: they
   // carry the rects in sorted order from the counts to the emission.
-  launch_tile_counts(views_d, seg_start_d, nviews, slots, tuning_.bin_shift, k, v, rect_.get(), counts_.get(),
+  launch_tile_counts(views_d, seg_start_d, nviews, slots, k, v, rect_.get(), counts_.get(),
                      offsets_.get(), visible, block_sums_.get(), block_pairs_.get(), ka, va, scan_clears, st);
   launch_block_scan(block_sums_.get(), count_blocks, pair_cap_, m_dev_.get(), info, block_pairs_.get(), raster_pairs,
                     st);
-  launch_tile_emit(views_d, seg_start_d, nviews, slots, tuning_.bin_shift, v, ka, va, counts_.get(), offsets_.get(),
+  launch_tile_emit(views_d, seg_start_d, nviews, slots, v, ka, va, counts_.get(), offsets_.get(),
                    block_sums_.get(), pair_cap_, pair_keys_.get(), pair_vals_.get(), st);
   uint32_t *pk = pair_keys_.get(), *pv = pair_vals_.get(), *pka = pair_keys_alt_.get(), *pva = pair_vals_alt_.get();
   SortPrep pair_prep;
@@ -433,7 +486,7 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   if (!captured) GS_HIP(hipEventRecord(ev_[3], st));
 
   unsigned long long *evals = count_evals_ ? reinterpret_cast<unsigned long long *>(info + kEvalsWord) : nullptr;
-  launch_raster(tuning_.raster, views_d, ctile_start_d, nviews, ctiles, tuning_.bin_shift, tuning_.blend_min,
+  launch_raster(tuning_.raster, views_d, rblock_start_d, nviews, total_rblocks_, tuning_.blend_min,
                 rec_.get(), rect_.get(),
                 ranges_.get(), pv, img32_.get(), img8_.get(), st, evals);
   if (!captured) GS_HIP(hipEventRecord(ev_[4], st));

@@ -78,6 +78,12 @@ struct DeviceScene {
   uint32_t count = 0;
   uint32_t sh_degree = 0;
   DevBuf<float> soa;
+  // Host statistics for launch choices: centroid of the positions, RMS
+  // distance from it, and the median over splats of the largest axis scale
+  // exp(max log-scale).
+  float center[3] = {0.0f, 0.0f, 0.0f};
+  float radius = 0.0f;
+  float median_scale = 0.0f;
   uint64_t bytes() const { return (uint64_t)soa_fields(sh_degree) * count * sizeof(float); }
 };
 std::shared_ptr<DeviceScene> upload_scene(const HostScene &s, hipStream_t stream);
@@ -103,7 +109,10 @@ struct FrameTimes {
 // Launch configuration of a pipeline. Every setting gives bit-identical
 // images; they trade work between binning, sorting and blending.
 struct Tuning {
-  uint32_t bin_shift = 2;                                // binning tiles of 16 << bin_shift pixels
+  uint32_t bin_shift = 2;                                // binning tiles of 16 << bin_shift pixels (the
+                                                         // largest, with bin_auto)
+  uint32_t bin_auto = 1;                                 // per view: smaller binning tiles for splats
+                                                         // that are small on screen
   RasterVariant raster = RasterVariant::Unrolled;        // blend loop
   uint32_t rts_min_tiles = 256;                          // radix sorts of >= this many tiles: reduce-then-scan
   uint32_t blend_min = 64;                               // staged splats that start a blend (1..256)
@@ -238,7 +247,7 @@ class Pipeline {
   std::vector<ViewDev> host_views_;
   std::vector<ProjGroup> host_groups_;                 // batches only (empty for one view)
   std::pair<uint32_t, uint32_t> group_chunks_[5] = {};  // per SH degree: first chunk, chunk count
-  uint32_t total_slots_ = 0, total_tiles_ = 0, total_ctiles_ = 0;
+  uint32_t total_slots_ = 0, total_tiles_ = 0, total_ctiles_ = 0, total_rblocks_ = 0;
   uint64_t total_pixels_ = 0, total_splats_ = 0;
   uint32_t pair_cap_ = 0;
   uint32_t *sorted_keys_ = nullptr;  // depth-sorted keys of the last frame

@@ -104,6 +104,7 @@ GsplatStageTimes Renderer::stage_times() {
   s.blend_evals = t.blend_evals;
   s.blend_tests = t.blend_tests;
   s.bin_pairs = pipe_.bin_pairs();
+  s.bin_shift = d.bin_shift;
   return s;
 }
 

@@ -93,7 +93,9 @@ typedef struct GsplatStageTimes {
                             * variant only, else 0)                      */
   uint64_t bin_pairs;      /* (binning tile, splat) pairs emitted and
                             * sorted; binning tiles are 16 << bin_shift
-                            * pixels square (GsplatTuning)               */
+                            * pixels square                             */
+  uint32_t bin_shift;      /* binning tile of the (first) view: 16 <<
+                            * bin_shift pixels (GsplatTuning.bin_auto)  */
 } GsplatStageTimes;
 int32_t gsplat_context_get_stage_times(GsplatContext *ctx, GsplatStageTimes *out);
 
@@ -232,11 +234,12 @@ int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out);
  * bit-identical images; they trade work between binning, sorting and
  * blending:
  *   bin_shift           splats are binned into tiles of (16 << bin_shift)
- *                       pixels, 0..3; each 16x16 raster tile keeps the
- *                       splats of its binning tile that overlap it
+ *                       pixels, 0..3 (the largest, with bin_auto); each
+ *                       16x16 raster tile keeps the splats of its binning
+ *                       tile that overlap it
  *   raster_variant      0 per-lane exit, 1 wave-uniform loop (default),
  *                       2 wave-uniform loop with two pixels per thread
- *                       (bin_shift >= 1; runs as 1 at bin_shift 0)
+ *                       (its views bin into tiles of at least 32 pixels)
  *   sort_rts_min_tiles  radix sorts of at least this many 4096-key tiles use
  *                       reduce-then-scan passes, shorter ones look-back passes
  *   frame_ms            (autotune output) median device time of a frame
@@ -244,6 +247,10 @@ int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out);
  *                       many (1..256, default 64) are staged or its list
  *                       ends; smaller stops a saturated tile's list walk
  *                       sooner, larger runs longer blend loops
+ *   bin_auto            1 (default): each view bins into tiles of about two
+ *                       on-screen splat diameters, at most 16 << bin_shift
+ *                       pixels (small views of large scenes get small
+ *                       binning tiles); 0: every view uses bin_shift
  */
 typedef struct GsplatTuning {
   uint32_t bin_shift;
@@ -251,6 +258,7 @@ typedef struct GsplatTuning {
   uint32_t sort_rts_min_tiles;
   float frame_ms;
   uint32_t blend_min;
+  uint32_t bin_auto;
 } GsplatTuning;
 
 int32_t gsplat_batch_get_tuning(GsplatBatch *batch, GsplatTuning *out_tuning);
