@@ -293,7 +293,7 @@ def pct(vals):
             "max": round(vals[-1], 3)}
 
 
-def bench_served(requests, rate, closed_seconds, pipelines=2, max_batch=64):
+def bench_served(requests, rate, closed_seconds, pipelines=3, max_batch=64):
     import queue
     import threading
 
@@ -303,9 +303,14 @@ def bench_served(requests, rate, closed_seconds, pipelines=2, max_batch=64):
     srv = A.Server(max_batch_views=max_batch, pipelines=pipelines)
     ids = [srv.add_scene(d) for d in dev]
     cams = [(ids[v.scene], A.look_at(v.position, v.target), v.width, v.height) for (_, v) in reqs]
-    pool = queue.Queue()
-    for _ in range(128):
-        pool.put(A.PinnedImage(1920, 1080))
+    # Pinned targets per resolution, enough for the requests in flight at the
+    # offered rate (the pool must not throttle the arrivals).
+    pools = {}
+    for (_, _, w, h) in cams:
+        if (w, h) not in pools:
+            pools[(w, h)] = queue.Queue()
+            for _ in range(128):
+                pools[(w, h)].put(A.PinnedImage(w, h))
     # warm the pipelines' buffers on a few batches
     warm = [srv.submit(*cams[i % len(cams)]) for i in range(256)]
     for t in warm:
@@ -316,13 +321,16 @@ def bench_served(requests, rate, closed_seconds, pipelines=2, max_batch=64):
     lat, server_ms, batch_views = [], [], []
     t0 = time.perf_counter() + 0.05
 
+    submitted_at = [0.0]
+
     def submitter():
         for (arrival, _), (sid, cam, w, h) in zip(reqs, cams):
             delay = t0 + arrival - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            buf = pool.get()
-            tickets.put((t0 + arrival, srv.submit(sid, cam, w, h, buf), buf))
+            buf = pools[(w, h)].get()
+            tickets.put((t0 + arrival, srv.submit(sid, cam, w, h, buf), buf, (w, h)))
+        submitted_at[0] = time.perf_counter() - t0
         tickets.put(None)
 
     def waiter():
@@ -330,12 +338,12 @@ def bench_served(requests, rate, closed_seconds, pipelines=2, max_batch=64):
             item = tickets.get()
             if item is None:
                 return
-            due, t, buf = item
+            due, t, buf, size = item
             r = srv.wait(t)
             lat.append((time.perf_counter() - due) * 1e3)
             server_ms.append(r.total_ms)
             batch_views.append(r.batch_views)
-            pool.put(buf)
+            pools[size].put(buf)
 
     th = [threading.Thread(target=submitter), threading.Thread(target=waiter)]
     for x in th:
@@ -343,7 +351,9 @@ def bench_served(requests, rate, closed_seconds, pipelines=2, max_batch=64):
     for x in th:
         x.join()
     span = max(time.perf_counter() - t0, 1e-9)
-    open_loop = {"requests": len(reqs), "offered_per_s": rate, "completed_per_s": round(len(lat) / span, 1),
+    open_loop = {"requests": len(reqs), "offered_per_s": rate,
+                 "submitted_per_s": round(len(reqs) / max(submitted_at[0], 1e-9), 1),
+                 "completed_per_s": round(len(lat) / span, 1),
                  "latency_ms": pct(lat), "server_ms": pct(server_ms),
                  "mean_batch_views": round(float(np.mean(batch_views)), 2)}
 

@@ -631,8 +631,9 @@ void gsplat_scene_destroy(GsplatScene *scene) {
 
 namespace {
 
-gs::Tuning tuning_from(const GsplatTuning &t) {
-  gs::Tuning out;
+// The ABI's fields over `base` (settings the ABI does not carry stay).
+gs::Tuning tuning_from(const GsplatTuning &t, gs::Tuning base) {
+  gs::Tuning out = base;
   out.bin_shift = t.bin_shift;
   if (t.raster_variant > 2) throw gs::Error(gs::Code::InvalidArgument, "raster_variant must be 0, 1 or 2");
   out.raster = static_cast<gs::RasterVariant>(t.raster_variant);
@@ -827,7 +828,7 @@ int32_t gsplat_batch_set_tuning(GsplatBatch *batch, const GsplatTuning *tuning) 
   return guard(op, [&]() -> int32_t {
     if (!batch) return null_arg(op, "batch");
     if (!tuning) return null_arg(op, "tuning");
-    batch->pipe.set_tuning(tuning_from(*tuning));
+    batch->pipe.set_tuning(tuning_from(*tuning, batch->pipe.tuning()));
     return ok();
   });
 }
@@ -863,7 +864,7 @@ int32_t gsplat_context_set_tuning(GsplatContext *ctx, const GsplatTuning *tuning
   return guard(op, [&]() -> int32_t {
     if (!ctx) return null_arg(op, "ctx");
     if (!tuning) return null_arg(op, "tuning");
-    ctx->renderer->set_tuning(tuning_from(*tuning));
+    ctx->renderer->set_tuning(tuning_from(*tuning, ctx->renderer->tuning()));
     return ok();
   });
 }
@@ -886,7 +887,7 @@ GsplatServerConfig gsplat_server_config_default(void) {
   c.memory_budget_bytes = 0;
   c.max_batch_views = 64;
   c.max_queue = 4096;
-  c.pipelines = 2;
+  c.pipelines = 3;
   return c;
 }
 

@@ -59,6 +59,10 @@ struct ViewDev {
   uint32_t bin_shift;          // binning tiles of (16 << bin_shift) pixels
   uint32_t rblock_off;         // first raster workgroup (see launch_raster)
   uint64_t img_off;
+  // Caller's pinned host target (device-visible): the raster writes the
+  // view's RGBA8 pixels there and keeps no device image of it. Null: the
+  // device images at img_off.
+  uchar4 *out8;
 };
 
 // Binning tile grid of a view for bin shift `shift`.

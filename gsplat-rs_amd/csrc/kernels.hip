@@ -746,6 +746,22 @@ __device__ __forceinline__ bool raster_tile(const ViewDev *__restrict__ views,
   return tx < vw.fp.tiles_x && ty < vw.fp.tiles_y;
 }
 
+// Final pixel (x, y) of view vw: RGBA8 (rounded once, clamped) and RGBA32F
+// in the device images, or RGBA8 alone into the view's host target.
+__device__ __forceinline__ void write_pixel(const ViewDev &vw, uint32_t x, uint32_t y, float cr, float cg, float cb,
+                                            float T, float4 *__restrict__ img32, uchar4 *__restrict__ img8) {
+  const float A = 1.0f - T;
+  auto q = [](float val) { return (unsigned char)(fminf(fmaxf(val, 0.0f), 1.0f) * 255.0f + 0.5f); };
+  const uchar4 p8 = make_uchar4(q(cr), q(cg), q(cb), q(A));
+  const size_t pix = (size_t)y * vw.fp.width + x;
+  if (vw.out8) {
+    vw.out8[pix] = p8;
+    return;
+  }
+  img32[vw.img_off + pix] = make_float4(cr, cg, cb, A);
+  img8[vw.img_off + pix] = p8;
+}
+
 // One workgroup per 16x16 tile, one thread per pixel. Batches of 256 records
 // are gathered into LDS (coalesced by the 256 threads) and blended front to
 // back: C += T*g*rgb_a, T *= 1 - a*g, stopping when T < 1e-4 (equivalent to
@@ -957,13 +973,7 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
       if (start >= rg.y) break;
     }
   }
-  if (inside) {
-    const size_t o = vw.img_off + (size_t)y * width + x;
-    const float A = 1.0f - T;
-    img32[o] = make_float4(cr, cg, cb, A);
-    auto q = [](float val) { return (unsigned char)(fminf(fmaxf(val, 0.0f), 1.0f) * 255.0f + 0.5f); };
-    img8[o] = make_uchar4(q(cr), q(cg), q(cb), q(A));
-  }
+  if (inside) write_pixel(vw, x, y, cr, cg, cb, T, img32, img8);
   if (COUNT) {
     uint32_t wp = passed, u = tested;
 #pragma unroll
@@ -1118,19 +1128,8 @@ __global__ __launch_bounds__(kThreads) void k_raster_rows2(const ViewDev *__rest
       if (start >= rg.y) break;
     }
   }
-  auto q = [](float val) { return (unsigned char)(fminf(fmaxf(val, 0.0f), 1.0f) * 255.0f + 0.5f); };
-  if (in0) {
-    const size_t o = vw.img_off + (size_t)y0 * width + x;
-    const float A = 1.0f - p0.T;
-    img32[o] = make_float4(p0.cr, p0.cg, p0.cb, A);
-    img8[o] = make_uchar4(q(p0.cr), q(p0.cg), q(p0.cb), q(A));
-  }
-  if (in1) {
-    const size_t o = vw.img_off + (size_t)y1 * width + x;
-    const float A = 1.0f - p1.T;
-    img32[o] = make_float4(p1.cr, p1.cg, p1.cb, A);
-    img8[o] = make_uchar4(q(p1.cr), q(p1.cg), q(p1.cb), q(A));
-  }
+  if (in0) write_pixel(vw, x, y0, p0.cr, p0.cg, p0.cb, p0.T, img32, img8);
+  if (in1) write_pixel(vw, x, y1, p1.cr, p1.cg, p1.cb, p1.T, img32, img8);
   if (COUNT) {
     uint32_t wp = p0.passed + p1.passed, u = p0.tested + p1.tested;
 #pragma unroll

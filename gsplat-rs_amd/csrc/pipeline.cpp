@@ -57,15 +57,6 @@ uint32_t initial_pair_capacity(uint64_t splats) {
 // (at least half the scene's RMS radius away). Splats that are small next to
 // the binning tile make each 16x16 tile scan a list mostly of splats outside
 // it; large ones multiply the pairs of small binning tiles.
-float bin_auto_k() {
-  static const float k = [] {
-    const char *e = std::getenv("GSPLAT_BIN_AUTO_K");
-    const float v = e ? std::strtof(e, nullptr) : 2.0f;
-    return v > 0.0f ? v : 2.0f;
-  }();
-  return k;
-}
-
 uint32_t view_bin_shift(const DeviceScene &sc, const FrameParams &fp, const Tuning &t) {
   const uint32_t lo = t.raster == RasterVariant::Rows2 ? 1u : 0u;  // two tile rows share a list
   if (!t.bin_auto || !(sc.median_scale > 0.0f)) return std::max(t.bin_shift, lo);
@@ -73,7 +64,7 @@ uint32_t view_bin_shift(const DeviceScene &sc, const FrameParams &fp, const Tuni
   const float z = std::max(fp.rows[6] * rel[0] + fp.rows[7] * rel[1] + fp.rows[8] * rel[2],
                            std::max(0.5f * sc.radius, 1e-3f));
   const float r_px = 3.0f * sc.median_scale * fp.f * 0.5f * (float)fp.height / z;
-  const float tile = bin_auto_k() * 2.0f * r_px;
+  const float tile = t.bin_auto_k * 2.0f * r_px;
   uint32_t s = 0;
   while (s < t.bin_shift && (float)(kTile << (s + 1)) <= tile) ++s;
   return std::max(s, lo);
@@ -132,6 +123,10 @@ Pipeline::Pipeline() {
   if (const char *e = std::getenv("GSPLAT_GRAPH")) use_graph_ = std::atoi(e) != 0;
   if (const char *e = std::getenv("GSPLAT_BIN_SHIFT")) set_bin_shift((uint32_t)std::atoi(e));
   if (const char *e = std::getenv("GSPLAT_BIN_AUTO")) tuning_.bin_auto = std::atoi(e) != 0;
+  if (const char *e = std::getenv("GSPLAT_BIN_AUTO_K")) {
+    const float k = std::strtof(e, nullptr);
+    if (k > 0.0f) tuning_.bin_auto_k = k;
+  }
   if (const char *e = std::getenv("GSPLAT_BLEND_MIN"))
     tuning_.blend_min = std::min<uint32_t>(std::max<uint32_t>((uint32_t)std::strtoul(e, nullptr, 10), 1u), 256u);
   if (const char *e = std::getenv("GSPLAT_SORT_RTS_MIN_TILES"))
@@ -211,6 +206,16 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
     d.rblock_off = (uint32_t)rblocks;
     rblocks += raster_blocks(d.ctiles, d.bin_shift);
     d.img_off = pixels;
+    d.out8 = nullptr;
+    if (!host_rgba8.empty() && host_rgba8[v]) {
+      // A pinned host target the raster can write directly; otherwise the
+      // image is copied there after the frame.
+      void *dev = nullptr;
+      if (hipHostGetDevicePointer(&dev, host_rgba8[v], 0) == hipSuccess && dev)
+        d.out8 = reinterpret_cast<uchar4 *>(dev);
+      else
+        (void)hipGetLastError();
+    }
     slots += d.seg_len;
     tiles += d.tiles;
     ctiles += d.ctiles;
@@ -491,7 +496,7 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
                 ranges_.get(), pv, img32_.get(), img8_.get(), st, evals);
   if (!captured) GS_HIP(hipEventRecord(ev_[4], st));
   for (size_t i = 0; i < host_rgba8_.size(); ++i) {
-    if (!host_rgba8_[i]) continue;
+    if (!host_rgba8_[i] || host_views_[i].out8) continue;  // written by the raster itself
     const ViewDev &d = host_views_[i];
     GS_HIP(hipMemcpyAsync(host_rgba8_[i], img8_.get() + d.img_off, (size_t)d.fp.width * d.fp.height * 4,
                           hipMemcpyDeviceToHost, st));

@@ -113,6 +113,7 @@ struct Tuning {
                                                          // largest, with bin_auto)
   uint32_t bin_auto = 1;                                 // per view: smaller binning tiles for splats
                                                          // that are small on screen
+  float bin_auto_k = 3.0f;                               // binning tile edge in splat diameters (bin_auto)
   RasterVariant raster = RasterVariant::Unrolled;        // blend loop
   uint32_t rts_min_tiles = 256;                          // radix sorts of >= this many tiles: reduce-then-scan
   uint32_t blend_min = 64;                               // staged splats that start a blend (1..256)
@@ -147,9 +148,10 @@ class Pipeline {
   // Enqueues one frame of `views` (1..kMaxViews) and returns without waiting
   // for the device. The scenes stay referenced until the next enqueue. When
   // `host_rgba8` is given, entry v (if not null) receives view v's RGBA8
-  // image by a copy queued behind the raster; it must be pinned host memory
-  // (a pageable target would make the copy synchronous) and stay valid until
-  // wait() returns.
+  // image: the raster writes it there directly when the memory is pinned and
+  // device-visible (hipHostMalloc), else a copy queued behind the raster
+  // does. It must stay valid until wait() returns. Such a view keeps no
+  // device image (readback_* of it reads stale data).
   void enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_rgba8 = {});
   // Waits for the last frame, re-rendering it after an overflow.
   void wait();

@@ -2,7 +2,7 @@
 //
 // Requests (scene, camera, target size) queue on submit(). A submission
 // thread folds every queued request that fits into the next batch as soon as
-// one of the pipelines (ServerConfig::pipelines, 2 by default) is free, so
+// one of the pipelines (ServerConfig::pipelines, 3 by default) is free, so
 // requests arriving while a batch is on the device ride a following one, and
 // the pipelines' batches overlap on the device (separate streams). A completion thread per pipeline waits for
 // its batch, delivers the images and wakes the waiters. Each request's image
@@ -40,7 +40,7 @@ struct ServerConfig {
   uint64_t memory_budget = 0;  // bytes; 0 = 90% of the device memory free at creation
   uint32_t max_batch_views = 64;
   uint32_t max_queue = 4096;
-  uint32_t pipelines = 2;  // batches in flight at once (1..8), one stream each
+  uint32_t pipelines = 3;  // batches in flight at once (1..8), one stream each
 };
 
 struct RequestResult {

@@ -593,7 +593,7 @@ class Server:
     """Owns one GsplatServer (continuous batching over resident scenes)."""
 
     def __init__(self, memory_budget_bytes: int = 0, max_batch_views: int = 64, max_queue: int = 4096,
-                 pipelines: int = 2):
+                 pipelines: int = 3):
         cfg = lib().gsplat_server_config_default()
         cfg.memory_budget_bytes, cfg.max_batch_views, cfg.max_queue = memory_budget_bytes, max_batch_views, max_queue
         cfg.pipelines = pipelines

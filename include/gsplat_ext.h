@@ -247,7 +247,7 @@ int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out);
  *                       many (1..256, default 64) are staged or its list
  *                       ends; smaller stops a saturated tile's list walk
  *                       sooner, larger runs longer blend loops
- *   bin_auto            1 (default): each view bins into tiles of about two
+ *   bin_auto            1 (default): each view bins into tiles of about three
  *                       on-screen splat diameters, at most 16 << bin_shift
  *                       pixels (small views of large scenes get small
  *                       binning tiles); 0: every view uses bin_shift
@@ -293,7 +293,7 @@ typedef struct GsplatServerConfig {
   uint64_t memory_budget_bytes; /* 0: 90% of the device memory free at create */
   uint32_t max_batch_views;     /* views folded into one batch, 1..256 */
   uint32_t max_queue;           /* queued requests before submit refuses */
-  uint32_t pipelines;           /* batches in flight at once, 1..8 (default 2) */
+  uint32_t pipelines;           /* batches in flight at once, 1..8 (default 3) */
 } GsplatServerConfig;
 
 typedef struct GsplatRenderRequest {
