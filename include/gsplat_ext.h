@@ -279,9 +279,10 @@ int32_t gsplat_context_autotune(GsplatContext *ctx, uint32_t reps, GsplatTuning 
 /*
  * Request loop over resident scenes with continuous batching: requests that
  * arrive while batches run on the device are folded into the next batch as
- * soon as one of its two pipelines is free, and consecutive batches overlap
- * on the device. Each result is the frame a context would render for the
- * request alone. Admission is by device memory (the analogue of the
+ * soon as one of its pipelines (config.pipelines) is free, and consecutive
+ * batches overlap on the device. Each result is the frame a context would
+ * render for the request alone. RGBA8 results for targets allocated with
+ * gsplat_host_alloc are written there by the raster itself. Admission is by device memory (the analogue of the
  * reference's direct_scene_preflight, render-wgpu/src/lib.rs:2202-2304, with
  * a byte budget in place of the storage-binding limit): resident scenes plus
  * each pipeline's batch working set stay within memory_budget_bytes. All
