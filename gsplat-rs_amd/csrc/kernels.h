@@ -200,6 +200,22 @@ void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const u
 void launch_raster_tile_counts(const ViewDev *views, uint32_t v, const ViewDev &host_view, const uint32_t *skeys,
                                const uint32_t *svals, const uint2 *rect, uint32_t *counts, hipStream_t stream);
 
+// Heaviest-first order of the batch's binning tiles for the raster: eight
+// classes by list length (class 0 longest), order[class * total_ctiles +
+// place] = global binning tile, counts[class] = class size (zero on entry).
+constexpr uint32_t kOrderClasses = 8;
+void launch_raster_order(const uint2 *ranges, uint32_t total_ctiles, uint32_t *counts, uint32_t *order,
+                         hipStream_t stream);
+
+// Raster workgroup layout: order null for per-view ranges (rblock_start);
+// else the heaviest-first order, every view at bin shift `shift`.
+struct RasterOrder {
+  const uint32_t *order = nullptr;
+  const uint32_t *counts = nullptr;
+  uint32_t total_ctiles = 0;
+  uint32_t shift = 0;
+};
+
 // Front-to-back blend of every view's 16x16 tiles: one workgroup per tile,
 // the raster tiles of one binning tile consecutive on one XCD. View v owns
 // the workgroups [rblock_start[v], rblock_start[v] + raster_blocks(view v))
@@ -220,7 +236,7 @@ void launch_raster_tile_counts(const ViewDev *views, uint32_t v, const ViewDev &
 // to evals[1] the coverage tests made.
 enum class RasterVariant : uint32_t { Linear = 0, Unrolled = 1, Rows2 = 2 };
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *rblock_start, uint32_t nviews,
-                   uint32_t total_rblocks, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
+                   uint32_t total_rblocks, const RasterOrder &order, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
                    const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t stream,
                    unsigned long long *evals = nullptr);
 

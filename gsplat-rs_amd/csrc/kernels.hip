@@ -692,6 +692,33 @@ __global__ __launch_bounds__(kThreads) void k_tile_ranges(const ViewDev *__restr
   }
 }
 
+// Heaviest binning tiles first: binning tile c (global) goes to class
+// clamp(12 - floor(log2 list length), 0, 7) (class 0: 4096 or more pairs,
+// class 7: fewer than 64 or none) and takes the next place of its class,
+// order[class * total_ctiles + place]; counts[class] ends as the class size
+// (zero before). The raster walks the classes in turn.
+__global__ __launch_bounds__(kThreads) void k_raster_order(const uint2 *__restrict__ ranges, uint32_t total_ctiles,
+                                                           uint32_t *__restrict__ counts,
+                                                           uint32_t *__restrict__ order) {
+  __shared__ uint32_t local[kOrderClasses], base[kOrderClasses];
+  const uint32_t t = threadIdx.x;
+  if (t < kOrderClasses) local[t] = 0;
+  __syncthreads();
+  const uint32_t c = blockIdx.x * kThreads + t;
+  uint32_t cls = 0, place = 0;
+  if (c < total_ctiles) {
+    const uint2 r = ranges[c];
+    const uint32_t len = r.y - r.x;
+    const int lg = len ? 31 - __clz(len) : 0;
+    cls = (uint32_t)min(max(12 - lg, 0), (int)kOrderClasses - 1);
+    place = atomicAdd(&local[cls], 1u);
+  }
+  __syncthreads();
+  if (t < kOrderClasses) base[t] = local[t] ? atomicAdd(&counts[t], local[t]) : 0u;
+  __syncthreads();
+  if (c < total_ctiles) order[(size_t)cls * total_ctiles + base[cls] + place] = c;
+}
+
 // One thread per slot of view v: every visible splat adds one to each 16x16
 // tile its pixel bounds overlap.
 __global__ __launch_bounds__(kThreads) void k_raster_tile_counts(const ViewDev *__restrict__ views, uint32_t v,
@@ -720,26 +747,55 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
 
 // ------------------------------------------------------------------ raster
 
-// Raster workgroup order. View v owns the workgroups [rblock_start[v],
+// Raster workgroup order, in one of two layouts.
+// Per view (order null): view v owns the workgroups [rblock_start[v],
 // rblock_start[v] + raster_blocks) (its binning tiles in whole runs of kXcds,
 // times S = 4^shift raster tiles each, shift the view's bin shift). Its local
 // workgroup l takes binning tile c = (l / (kXcds S)) kXcds + l % kXcds and
-// raster tile k = (l / kXcds) % S of it, so the S raster tiles of one binning
-// tile run back to back on one XCD (workgroups are dealt to the XCDs round
-// robin, and view block ranges start on multiples of kXcds) and find their
-// shared list's records in its L2. False for padding (no such binning tile,
-// or a raster tile past the view's edge).
+// raster tile k = (l / kXcds) % S of it.
+// Ordered (every view at one bin shift): workgroup b takes place
+// j = (b / (kXcds S)) kXcds + b % kXcds of the heaviest-first order of all
+// binning tiles (k_raster_order) and raster tile k = (b / kXcds) % S, so the
+// long lists start first and the kernel does not end on them.
+// Either way the S raster tiles of one binning tile run back to back on one
+// XCD (workgroups are dealt to the XCDs round robin) and find their shared
+// list's records in its L2. False for padding (no such binning tile, or a
+// raster tile past the view's edge).
 __device__ __forceinline__ bool raster_tile(const ViewDev *__restrict__ views,
                                             const uint32_t *__restrict__ rblock_start, uint32_t nviews,
-                                            uint32_t total_rblocks, uint32_t b, uint32_t &v, uint32_t &tx,
-                                            uint32_t &ty) {
+                                            uint32_t total_rblocks, const RasterOrder &ro, uint32_t b, uint32_t &v,
+                                            uint32_t &tx, uint32_t &ty) {
   if (b >= total_rblocks) return false;
-  v = find_view(rblock_start, nviews, b);
+  uint32_t c, k, shift;
+  if (ro.order) {
+    shift = ro.shift;
+    const uint32_t S = 1u << (2 * shift);
+    const uint32_t j = (b / (kXcds * S)) * kXcds + b % kXcds;
+    k = (b / kXcds) % S;
+    if (j >= ro.total_ctiles) return false;
+    uint32_t cls = 0, first = 0;
+    for (; cls + 1 < kOrderClasses; ++cls) {
+      const uint32_t n = ro.counts[cls];
+      if (j < first + n) break;
+      first += n;
+    }
+    const uint32_t g = ro.order[(size_t)cls * ro.total_ctiles + (j - first)];
+    uint32_t lo = 0, hi = nviews - 1;  // last view whose first binning tile <= g
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (views[mid].ctile_off <= g) lo = mid; else hi = mid - 1;
+    }
+    v = __builtin_amdgcn_readfirstlane(lo);
+    c = g - views[v].ctile_off;
+  } else {
+    v = find_view(rblock_start, nviews, b);
+    shift = views[v].bin_shift;
+    const uint32_t S = 1u << (2 * shift);
+    const uint32_t l = b - views[v].rblock_off;
+    c = (l / (kXcds * S)) * kXcds + l % kXcds;
+    k = (l / kXcds) % S;
+  }
   const ViewDev &vw = views[v];
-  const uint32_t shift = vw.bin_shift, S = 1u << (2 * shift);
-  const uint32_t l = b - vw.rblock_off;
-  const uint32_t c = (l / (kXcds * S)) * kXcds + l % kXcds;
-  const uint32_t k = (l / kXcds) % S;
   if (c >= vw.ctiles) return false;
   tx = ((c % vw.fp.ctiles_x) << shift) + (k & ((1u << shift) - 1u));
   ty = ((c / vw.fp.ctiles_x) << shift) + (k >> shift);
@@ -896,7 +952,7 @@ __device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const 
 template <bool COUNT, int LOOP>
 __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__ views,
                                                      const uint32_t *__restrict__ rblock_start, uint32_t nviews,
-                                                     uint32_t total_rblocks, uint32_t blend_min,
+                                                     uint32_t total_rblocks, RasterOrder ro, uint32_t blend_min,
                                                      const float4 *__restrict__ rec, const uint2 *__restrict__ rect,
                                                      const uint2 *__restrict__ ranges,
                                                      const uint32_t *__restrict__ list, float4 *__restrict__ img32,
@@ -906,7 +962,7 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
   __shared__ float4 s0[kStage], s1[kStage], s2[kStage];
   __shared__ uint32_t wkeep[kWaves];
   uint32_t v, tx, ty;
-  if (!raster_tile(views, rblock_start, nviews, total_rblocks, blockIdx.x, v, tx, ty)) return;
+  if (!raster_tile(views, rblock_start, nviews, total_rblocks, ro, blockIdx.x, v, tx, ty)) return;
   const ViewDev &vw = views[v];
   const uint32_t shift = vw.bin_shift;
   const uint32_t width = vw.fp.width, height = vw.fp.height;
@@ -1055,7 +1111,7 @@ __device__ __forceinline__ void blend_stage2(const float4 *s0, float4 *s1, const
 template <bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_raster_rows2(const ViewDev *__restrict__ views,
                                                            const uint32_t *__restrict__ rblock_start, uint32_t nviews,
-                                                           uint32_t total_rblocks, uint32_t blend_min,
+                                                           uint32_t total_rblocks, RasterOrder ro, uint32_t blend_min,
                                                            const float4 *__restrict__ rec,
                                                            const uint2 *__restrict__ rect,
                                                            const uint2 *__restrict__ ranges,
@@ -1066,7 +1122,7 @@ __global__ __launch_bounds__(kThreads) void k_raster_rows2(const ViewDev *__rest
   __shared__ float4 s0[kStage], s1[kStage], s2[kStage];
   __shared__ uint32_t wkeep[kWaves];
   uint32_t v, tx, ty;
-  if (!raster_tile(views, rblock_start, nviews, total_rblocks, blockIdx.x, v, tx, ty)) return;
+  if (!raster_tile(views, rblock_start, nviews, total_rblocks, ro, blockIdx.x, v, tx, ty)) return;
   const ViewDev &vw = views[v];
   const uint32_t shift = vw.bin_shift;
   const uint32_t width = vw.fp.width, height = vw.fp.height;
@@ -1298,34 +1354,41 @@ void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const u
   GS_HIP(hipGetLastError());
 }
 
+void launch_raster_order(const uint2 *ranges, uint32_t total_ctiles, uint32_t *counts, uint32_t *order,
+                         hipStream_t st) {
+  if (total_ctiles == 0) return;
+  k_raster_order<<<div_up(total_ctiles, kThreads), kThreads, 0, st>>>(ranges, total_ctiles, counts, order);
+  GS_HIP(hipGetLastError());
+}
+
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *rblock_start, uint32_t nviews,
-                   uint32_t total_rblocks, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
+                   uint32_t total_rblocks, const RasterOrder &ro, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
                    const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t st, unsigned long long *evals) {
   if (total_rblocks == 0 || nviews == 0) return;
   if (blend_min == 0 || blend_min > kThreads) throw Error(Code::InvalidArgument, "blend threshold outside 1..256");
   const dim3 grid(total_rblocks), block(kThreads);
   if (variant == RasterVariant::Rows2) {  // every view at bin shift >= 1 (the caller's duty)
     if (evals)
-      k_raster_rows2<true><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, blend_min, rec, rect, ranges,
+      k_raster_rows2<true><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
                                                    list, img32, img8, evals);
     else
-      k_raster_rows2<false><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, blend_min, rec, rect, ranges,
+      k_raster_rows2<false><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
                                                     list, img32, img8, evals);
     GS_HIP(hipGetLastError());
     return;
   }
   const bool unrolled = variant != RasterVariant::Linear;
   if (unrolled && evals)
-    k_raster<true, 1><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, blend_min, rec, rect, ranges, list,
+    k_raster<true, 1><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges, list,
                                               img32, img8, evals);
   else if (unrolled)
-    k_raster<false, 1><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, blend_min, rec, rect, ranges,
+    k_raster<false, 1><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
                                                list, img32, img8, evals);
   else if (evals)
-    k_raster<true, 0><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, blend_min, rec, rect, ranges, list,
+    k_raster<true, 0><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges, list,
                                               img32, img8, evals);
   else
-    k_raster<false, 0><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, blend_min, rec, rect, ranges,
+    k_raster<false, 0><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
                                                list, img32, img8, evals);
   GS_HIP(hipGetLastError());
 }

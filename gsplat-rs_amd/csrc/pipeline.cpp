@@ -31,11 +31,13 @@ constexpr size_t kTableBytes = table_layout(kMaxViews, kMaxViews).bytes;
 //   [0] unused, [1..2] binning pair count M (u64),
 //   [3] overflow flag, [4..5] (16x16 tile, splat) pairs (u64),
 //   [6 .. 6+kMaxViews) visible count per view,
-//   kEvalsWord: u64 evaluations passing, u64 coverage tests (counting mode).
+//   kEvalsWord: u64 evaluations passing, u64 coverage tests (counting mode),
+//   kOrderWord: class sizes of the raster's heaviest-first order.
 constexpr size_t kRasterPairsWord = 4;
 constexpr size_t kVisibleWord = 6;
 constexpr size_t kEvalsWord = kVisibleWord + kMaxViews;
-constexpr size_t kSummaryWords = kEvalsWord + 4;
+constexpr size_t kOrderWord = kEvalsWord + 4;  // raster order class sizes (kOrderClasses words)
+constexpr size_t kSummaryWords = kOrderWord + kOrderClasses;
 
 static_assert(kSegmentAlign % kSortTile == 0, "view segments must be whole sort tiles");
 
@@ -114,7 +116,8 @@ uint64_t frame_working_bytes(uint64_t slots, uint64_t pixels, uint64_t tiles, ui
   // pair keys/vals and alternates, plus the sort status words of the larger
   // of the two sorts.
   const uint64_t sort_words = onesweep_status_words((uint32_t)std::max<uint64_t>(slots, pair_cap), 32);
-  return per_slot * slots + 16 * pixels + 20 * pixels / 16 + 8 * tiles + 16 * pair_cap + 4 * sort_words;
+  // targets, tile ranges and the raster order (bounded by the 16x16 tiles).
+  return per_slot * slots + 20 * pixels + 8 * tiles + 4 * kOrderClasses * tiles + 16 * pair_cap + 4 * sort_words;
 }
 
 Pipeline::Pipeline() {
@@ -296,6 +299,7 @@ void Pipeline::launch_frame() {
   img32_.ensure(total_pixels_);
   img8_.ensure(total_pixels_);
   ranges_.ensure(std::max<uint32_t>(ctiles, 1));
+  order_.ensure((size_t)kOrderClasses * std::max<uint32_t>(ctiles, 1));
   pair_keys_.ensure(pair_cap_);
   pair_vals_.ensure(pair_cap_);
   pair_keys_alt_.ensure(pair_cap_);
@@ -345,6 +349,8 @@ void Pipeline::launch_frame() {
   plan.pair_lookback = pair_lookback;
   plan.depth_dirty = depth_dirty;
   plan.pair_dirty = pair_dirty;
+  plan.uniform_shift = true;
+  for (const ViewDev &d : host_views_) plan.uniform_shift = plan.uniform_shift && d.bin_shift == host_views_[0].bin_shift;
   if (as_graph) {
     // The frame as one graph per staging slot, captured again whenever
     // anything but the view table's contents changes.
@@ -365,7 +371,8 @@ void Pipeline::launch_frame() {
                             (const void *)img32_.get(), (const void *)img8_.get(), (const void *)ranges_.get(),
                             (const void *)pair_keys_.get(), (const void *)pair_vals_.get(),
                             (const void *)pair_keys_alt_.get(), (const void *)pair_vals_alt_.get(),
-                            (const void *)depth_scratch_.get(), (const void *)pair_scratch_.get()})
+                            (const void *)depth_scratch_.get(), (const void *)pair_scratch_.get(),
+                            (const void *)order_.get()})
       key.push_back(reinterpret_cast<uintptr_t>(ptr));
     GraphSlot &g = graphs_[r];
     if (!g.exec || g.key != key) {
@@ -438,6 +445,7 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   Clears frame_clears;
   frame_clears.add(info, kVisibleWord + nviews);
   frame_clears.add(reinterpret_cast<uint32_t *>(ranges_.get()), 2 * (size_t)ctiles);
+  frame_clears.add(info + kOrderWord, kOrderClasses);
   if (slots == 0) {
     for (int c = 0; c < Clears::kRanges; ++c) launch_fill_u32(frame_clears.ptr[c], 0u, frame_clears.words[c], st);
   }
@@ -488,10 +496,21 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   pair_clean_ = pair_lookback;
   if (count_evals_) range_clears.add(info + kEvalsWord, 4);
   launch_tile_ranges(views_d, pk, m_dev_.get(), pair_cap_, ranges_.get(), range_clears, st);
+  // Heaviest binning tiles first when every view bins alike (one layout).
+  RasterOrder ro;
+  uint32_t rblocks = total_rblocks_;
+  if (plan.uniform_shift) {
+    launch_raster_order(ranges_.get(), ctiles, info + kOrderWord, order_.get(), st);
+    ro.order = order_.get();
+    ro.counts = info + kOrderWord;
+    ro.total_ctiles = ctiles;
+    ro.shift = host_views_[0].bin_shift;
+    rblocks = raster_blocks(ctiles, ro.shift);
+  }
   if (!captured) GS_HIP(hipEventRecord(ev_[3], st));
 
   unsigned long long *evals = count_evals_ ? reinterpret_cast<unsigned long long *>(info + kEvalsWord) : nullptr;
-  launch_raster(tuning_.raster, views_d, rblock_start_d, nviews, total_rblocks_, tuning_.blend_min,
+  launch_raster(tuning_.raster, views_d, rblock_start_d, nviews, rblocks, ro, tuning_.blend_min,
                 rec_.get(), rect_.get(),
                 ranges_.get(), pv, img32_.get(), img8_.get(), st, evals);
   if (!captured) GS_HIP(hipEventRecord(ev_[4], st));

@@ -205,6 +205,7 @@ class Pipeline {
   struct FramePlan {
     int slot = 0;  // pinned view-table staging slot
     uint32_t nviews = 0, slots = 0, tiles = 0, ctiles = 0, tile_bits = 0, count_blocks = 0, hint = 0;
+    bool uniform_shift = false;  // every view at one bin shift: heaviest-first raster order
     int value_byte = -1;
     bool depth_lookback = false, pair_lookback = false;
     size_t depth_dirty = 0, pair_dirty = 0;
@@ -231,6 +232,7 @@ class Pipeline {
   uint32_t *summary_pinned_ = nullptr;  // [-, M lo, M hi, overflow, visible[kMaxViews]]
   uint64_t frame_counter_ = 0;
 
+  DevBuf<uint32_t> order_;   // heaviest-first binning tile order (kOrderClasses x binning tiles)
   DevBuf<uint8_t> table_;    // seg_start | tile_start | ViewDev | ProjGroup, packed per frame
   DevBuf<uint32_t> summary_;  // device side of summary_pinned_
   DevBuf<uint32_t> keys_, vals_, keys_alt_, vals_alt_;

@@ -101,8 +101,9 @@ def test_server_concurrent_submitters(abi):
 def test_server_memory_admission(abi):
     small = abi.Scene(synth.tiny_scene(1000, 11))
     big = abi.Scene(synth.tiny_scene(200000, 12))
-    # a budget that holds the small scene and its 1080p working set but not the big one
-    srv = abi.Server(memory_budget_bytes=160 << 20)
+    # a budget that holds the small scene and a 1080p working set per pipeline
+    # (about 60 MB each, three pipelines) but not the big one (about 85 MB each)
+    srv = abi.Server(memory_budget_bytes=220 << 20)
     sid = srv.add_scene(small)
     with pytest.raises(abi.GsplatError) as e:
         srv.add_scene(big)
