@@ -58,8 +58,13 @@ uint32_t initial_pair_capacity(uint64_t splats) {
 // the scene's median largest scale at the depth of the scene's centroid
 // (at least half the scene's RMS radius away). Splats that are small next to
 // the binning tile make each 16x16 tile scan a list mostly of splats outside
-// it; large ones multiply the pairs of small binning tiles.
-uint32_t view_bin_shift(const DeviceScene &sc, const FrameParams &fp, const Tuning &t) {
+// it; large ones multiply the pairs of small binning tiles. In a batch, with
+// bin_max_ctiles set, a view is then binned coarser (up to 128 px) until it
+// has at most that many binning tiles: with 256 or fewer the pair sort is one
+// 8-bit pass. That pays for batches of many small views (served requests)
+// and costs batches of large views with large splats (their raster lists
+// grow more than one pass saves), so it is off by default.
+uint32_t view_bin_shift(const DeviceScene &sc, const FrameParams &fp, const Tuning &t, bool batch) {
   const uint32_t lo = t.raster == RasterVariant::Rows2 ? 1u : 0u;  // two tile rows share a list
   if (!t.bin_auto || !(sc.median_scale > 0.0f)) return std::max(t.bin_shift, lo);
   const float rel[3] = {sc.center[0] - fp.cam[0], sc.center[1] - fp.cam[1], sc.center[2] - fp.cam[2]};
@@ -69,7 +74,15 @@ uint32_t view_bin_shift(const DeviceScene &sc, const FrameParams &fp, const Tuni
   const float tile = t.bin_auto_k * 2.0f * r_px;
   uint32_t s = 0;
   while (s < t.bin_shift && (float)(kTile << (s + 1)) <= tile) ++s;
-  return std::max(s, lo);
+  s = std::max(s, lo);
+  if (batch && t.bin_max_ctiles) {  // coarser until the view has at most bin_max_ctiles binning tiles
+    auto ctiles = [&](uint32_t sh) {
+      const uint32_t e = kTile << sh;
+      return ((fp.width + e - 1) / e) * ((fp.height + e - 1) / e);
+    };
+    while (s < kMaxBinShift && ctiles(s) > t.bin_max_ctiles) ++s;
+  }
+  return s;
 }
 
 }  // namespace
@@ -126,6 +139,7 @@ Pipeline::Pipeline() {
   if (const char *e = std::getenv("GSPLAT_GRAPH")) use_graph_ = std::atoi(e) != 0;
   if (const char *e = std::getenv("GSPLAT_BIN_SHIFT")) set_bin_shift((uint32_t)std::atoi(e));
   if (const char *e = std::getenv("GSPLAT_BIN_AUTO")) tuning_.bin_auto = std::atoi(e) != 0;
+  if (const char *e = std::getenv("GSPLAT_BIN_MAX_CTILES")) tuning_.bin_max_ctiles = (uint32_t)std::strtoul(e, nullptr, 10);
   if (const char *e = std::getenv("GSPLAT_BIN_AUTO_K")) {
     const float k = std::strtof(e, nullptr);
     if (k > 0.0f) tuning_.bin_auto_k = k;
@@ -195,7 +209,7 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
                                          std::to_string(kLocalMask) + " splats");
     ViewDev &d = table[v];
     d.fp = s.fp;
-    d.bin_shift = view_bin_shift(*s.scene, s.fp, tuning_);
+    d.bin_shift = view_bin_shift(*s.scene, s.fp, tuning_, multi);
     set_bin_tiles(d.fp, d.bin_shift);
     d.soa = s.scene->soa.get();
     d.n = n;

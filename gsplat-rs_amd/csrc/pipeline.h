@@ -114,6 +114,9 @@ struct Tuning {
   uint32_t bin_auto = 1;                                 // per view: smaller binning tiles for splats
                                                          // that are small on screen
   float bin_auto_k = 3.0f;                               // binning tile edge in splat diameters (bin_auto)
+  uint32_t bin_max_ctiles = 0;                           // bin_auto, batches: at most this many binning
+                                                         // tiles per view where a shift up to 3 allows
+                                                         // (0: no bound; the server uses 256)
   RasterVariant raster = RasterVariant::Unrolled;        // blend loop
   uint32_t rts_min_tiles = 256;                          // radix sorts of >= this many tiles: reduce-then-scan
   uint32_t blend_min = 64;                               // staged splats that start a blend (1..256)
