@@ -818,15 +818,6 @@ __device__ __forceinline__ void write_pixel(const ViewDev &vw, uint32_t x, uint3
   img8[vw.img_off + pix] = p8;
 }
 
-// One workgroup per 16x16 tile, one thread per pixel. Batches of 256 records
-// are gathered into LDS (coalesced by the 256 threads) and blended front to
-// back: C += T*g*rgb_a, T *= 1 - a*g, stopping when T < 1e-4 (equivalent to
-// the reference's back-to-front premultiplied "over", draw_pass.rs:75, up to
-// that bound). Coverage is r^2 <= cut (the fragment test of wgsl:358-367
-// precomputed per splat); g = exp(-4.5 r^2) (wgsl:363).
-// COUNT: also adds to evals[0] the number of (pixel, splat) evaluations that
-// pass coverage before the pixel's early stop (the raster's algorithmic work)
-// and to evals[1] the number of coverage tests made.
 // Blends staged splats [0, n) of the LDS stage into the thread's pixel.
 // LOOP 0: per-lane exit at the early stop, divergent blend branch.
 // LOOP 1: wave-uniform loop, four splats per wave liveness check; the stage
@@ -859,6 +850,23 @@ __device__ __forceinline__ bool ellipse_meets_rect(const float4 &a, const float4
   return fminf(fminf(m0, m1), fminf(m2, m3)) <= b.z * 1.01f + 1e-6f;
 }
 
+// One passing evaluation of a staged splat at a pixel, the same operations
+// in every raster loop: g = exp(-4.5 r2) evaluated as 2^(r2 * -4.5 log2 e)
+// (one multiply and the hardware exp2), weight w = T g, colour += w times
+// the premultiplied rgb, T -= w alpha (T (1 - alpha g) up to rounding). The
+// reference's back-to-front "over" equals this front-to-back form up to the
+// T < 1e-4 early stop (draw_pass.rs:75).
+__device__ __forceinline__ void blend_hit(float r2, float alpha, const float4 &c, float &T, float &cr, float &cg,
+                                          float &cb) {
+  constexpr float kNeg45Log2e = -6.4921276840f;  // -4.5 / ln 2
+  const float g = __builtin_amdgcn_exp2f(r2 * kNeg45Log2e);
+  const float w = T * g;
+  cr = fmaf(w, c.x, cr);
+  cg = fmaf(w, c.y, cg);
+  cb = fmaf(w, c.z, cb);
+  T = fmaf(-w, alpha, T);
+}
+
 template <int LOOP, bool COUNT>
 __device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const float4 *s2, uint32_t n, float pnx,
                                             float pny, float &T, float &cr, float &cg, float &cb, bool &done,
@@ -876,13 +884,7 @@ __device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const 
       if (COUNT) ++tested;
       if (r2 <= b.z) {
         if (COUNT) ++passed;
-        const float g = __expf(-4.5f * r2);
-        const float4 c = s2[j];
-        const float wgt = T * g;
-        cr = fmaf(wgt, c.x, cr);
-        cg = fmaf(wgt, c.y, cg);
-        cb = fmaf(wgt, c.z, cb);
-        T = T * (1.0f - b.w * g);
+        blend_hit(r2, b.w, s2[j], T, cr, cg, cb);
         if (T < 1e-4f) {
           done = true;
           return;
@@ -922,13 +924,7 @@ __device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const 
           passed += hit ? 1u : 0u;
         }
         if (hit) {
-          const float g = __expf(-4.5f * r2);
-          const float4 c = gc[u];
-          const float wgt = T * g;
-          cr = fmaf(wgt, c.x, cr);
-          cg = fmaf(wgt, c.y, cg);
-          cb = fmaf(wgt, c.z, cb);
-          T = T * (1.0f - b.w * g);
+          blend_hit(r2, b.w, gc[u], T, cr, cg, cb);
           active = !(T < 1e-4f);
         }
       }
@@ -941,9 +937,10 @@ __device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const 
 // its binning tile's list (depth order) 256 entries at a time; the entries
 // whose pixel bounds overlap the tile are compacted in order into the LDS
 // stage (records gathered from global memory), and the stage is blended
-// front to back once it holds 256 or the list ends: C += T*g*rgb_a,
-// T *= 1 - a*g, stopping when T < 1e-4 (equivalent to the reference's
-// back-to-front premultiplied "over", draw_pass.rs:75, up to that bound).
+// front to back once it holds blend_min splats or the list ends (blend_hit:
+// C += T*g*rgb_a, T -= T*g*a), stopping when T < 1e-4 (equivalent to the
+// reference's back-to-front premultiplied "over", draw_pass.rs:75, up to that
+// bound).
 // Coverage is r^2 <= cut (the fragment test of wgsl:358-367 precomputed per
 // splat); g = exp(-4.5 r^2) (wgsl:363).
 // COUNT: also adds to evals[0] the number of (pixel, splat) evaluations that
@@ -1065,13 +1062,7 @@ __device__ __forceinline__ void blend_one(PixelAcc &p, float pnx, float pny, con
     p.passed += hit ? 1u : 0u;
   }
   if (hit) {
-    const float g = __expf(-4.5f * r2);
-    const float4 c = s2[j];
-    const float wgt = p.T * g;
-    p.cr = fmaf(wgt, c.x, p.cr);
-    p.cg = fmaf(wgt, c.y, p.cg);
-    p.cb = fmaf(wgt, c.z, p.cb);
-    p.T = p.T * (1.0f - b.w * g);
+    blend_hit(r2, b.w, s2[j], p.T, p.cr, p.cg, p.cb);
     p.live = !(p.T < 1e-4f);
   }
 }
