@@ -173,8 +173,8 @@ def _kernel_short(name):
 # Stages of a frame, in launch order. A dispatch belongs to the stage of the
 # latest stage-opening kernel before it: k_project opens "project", the first
 # radix kernel after it the depth sort, k_tile_counts the binning stage (tile
-# counts, block scan, pair emission, pair radix sort, tile ranges), k_raster
-# the raster. Sort kernels are shared by the depth and pair sorts, so launch
+# counts, block scan, pair emission, pair radix sort, tile ranges, raster
+# order), k_raster<...> the raster. Sort kernels are shared by the depth and pair sorts, so launch
 # order (not the name) tells them apart.
 STAGES = ["project", "sort", "bin", "raster"]
 STAGE_KERNEL = {"project": "k_project", "sort": "depth radix sort", "bin": "bin(pair sort+emit)",
@@ -196,10 +196,10 @@ def _stage_sequence(rows):
             out.append(("counting", r))
             stage = None
             continue
-        elif k.startswith("k_raster"):
+        elif k.startswith("k_raster<"):
             stage = "raster"
         out.append((stage, r))
-        if k.startswith("k_raster"):
+        if k.startswith("k_raster<"):
             stage = None
     return out
 
