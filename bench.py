@@ -98,13 +98,22 @@ def time_one_per_launch(batch, views, passes, warmup):
     return len(views) * passes / (time.perf_counter() - t0)
 
 
+def stage_frame(b, views):
+    """Stage device times of one more batch frame, rendered with stage events."""
+    b.set_stage_timing(True)
+    b.render(views)
+    st = b.stage_times()
+    b.set_stage_timing(False)
+    return st
+
+
 def bench_batched_1080p(scene, steps, warmup):
     rng = np.random.default_rng(synth.SEED_BASE + 1 + 500)
     dev = A.Scene(scene)
     views = [(dev, A.look_at(synth.sphere_point(rng, 4.0)), 1920, 1080) for _ in range(8)]
     b = A.Batch()
     batched = time_batches(b, views, steps, warmup)
-    st = b.stage_times()
+    st = stage_frame(b, views)
     plain = time_one_per_launch(A.Batch(), views, max(steps // 2, 1), warmup)
     return {"views": len(views), "frames_per_s": round(batched, 3), "plain_frames_per_s": round(plain, 3),
             "speedup_vs_plain": round(batched / plain, 3), "ms_per_batch": round(1e3 * len(views) / batched, 4),
@@ -120,7 +129,7 @@ def bench_batched_64v(steps, warmup):
     views = [(dev[v.scene], A.look_at(v.position, v.target), v.width, v.height) for v in vs]
     b = A.Batch()
     batched = time_batches(b, views, steps, warmup)
-    st = b.stage_times()
+    st = stage_frame(b, views)
     plain = time_one_per_launch(A.Batch(), views, max(steps // 4, 1), max(warmup // 4, 1))
     pixels = sum(w * h for (_, _, w, h) in views)
     return {"views": len(views), "scenes": len(dev), "frames_per_s": round(batched, 3),
@@ -437,16 +446,24 @@ def main():
     elapsed = time.perf_counter() - t0
     fps = args.steps / elapsed
 
-    # Per-frame latency and stage device times (separate pass, one frame at a time).
-    lat, stages = [], []
+    # Per-frame latency (render + wait, one frame at a time).
+    lat = []
     for _ in range(min(args.steps, 60)):
         t = time.perf_counter()
         ctx.render()
         ctx.wait()
         lat.append((time.perf_counter() - t) * 1e3)
+    lat.sort()
+    # Stage device times, from frames with an event at each stage boundary
+    # (each costs the device a drain, so the timed frames above have none).
+    ctx.set_stage_timing(True)
+    stages = []
+    for _ in range(min(args.steps, 60)):
+        ctx.render()
+        ctx.wait()
         s = ctx.stage_times()
         stages.append((s.project_ms, s.sort_ms, s.bin_ms, s.raster_ms, s.total_ms))
-    lat.sort()
+    ctx.set_stage_timing(False)
     st = ctx.stage_times()
     med = np.median(np.array(stages), axis=0)
     names = ["project", "sort", "bin", "raster"]

@@ -384,6 +384,15 @@ int32_t gsplat_context_readback_rgba32f(GsplatContext *ctx, float *out, uint64_t
   });
 }
 
+int32_t gsplat_context_set_stage_timing(GsplatContext *ctx, uint32_t on) {
+  static const char *op = "gsplat_context_set_stage_timing";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    ctx->renderer->set_stage_timing(on != 0);
+    return ok();
+  });
+}
+
 int32_t gsplat_context_get_stage_times(GsplatContext *ctx, GsplatStageTimes *out) {
   static const char *op = "gsplat_context_get_stage_times";
   return guard(op, [&]() -> int32_t {
@@ -778,6 +787,15 @@ int32_t gsplat_batch_read_tile_counts(GsplatBatch *batch, uint32_t view, uint32_
     if (!out) return null_arg(op, "out");
     if (!out_tiles_x || !out_tiles_y) return null_arg(op, "out_tiles");
     batch->pipe.read_tile_counts(view, out, cap, out_tiles_x, out_tiles_y);
+    return ok();
+  });
+}
+
+int32_t gsplat_batch_set_stage_timing(GsplatBatch *batch, uint32_t on) {
+  static const char *op = "gsplat_batch_set_stage_timing";
+  return guard(op, [&]() -> int32_t {
+    if (!batch) return null_arg(op, "batch");
+    batch->pipe.set_stage_timing(on != 0);
     return ok();
   });
 }

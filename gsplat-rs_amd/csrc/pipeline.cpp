@@ -137,6 +137,7 @@ Pipeline::Pipeline() {
   if (!device_present()) throw Error(Code::Unsupported, "gpu rasterizer unavailable");
   if (const char *e = std::getenv("GSPLAT_COUNT_EVALS")) count_evals_ = std::atoi(e) != 0;
   if (const char *e = std::getenv("GSPLAT_GRAPH")) use_graph_ = std::atoi(e) != 0;
+  if (const char *e = std::getenv("GSPLAT_STAGE_TIMES")) stage_timing_ = std::atoi(e) != 0;
   if (const char *e = std::getenv("GSPLAT_BIN_SHIFT")) set_bin_shift((uint32_t)std::atoi(e));
   if (const char *e = std::getenv("GSPLAT_BIN_AUTO")) tuning_.bin_auto = std::atoi(e) != 0;
   if (const char *e = std::getenv("GSPLAT_BIN_MAX_CTILES")) tuning_.bin_max_ctiles = (uint32_t)std::strtoul(e, nullptr, 10);
@@ -420,7 +421,7 @@ void Pipeline::launch_frame() {
     pair_clean_ = g.pair_clean;
   } else {
     record_frame(plan, false);
-    staged_times_ = true;
+    staged_times_ = stage_timing_;
   }
   pending_ = true;
   summary_valid_ = false;
@@ -443,8 +444,12 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   hipStream_t st = stream_;
   const uint32_t ngroups = (uint32_t)host_groups_.size();
   const TableLayout lay = table_layout(nviews, ngroups);
+  // Events break the back-to-back run of kernels (the device drains at each,
+  // about 6 us measured): a frame records its start and end, and the stage
+  // boundaries only with stage timing on.
+  const bool stage_events = !captured && stage_timing_;
+  if (!captured) GS_HIP(hipEventRecord(ev_[0], st));
   GS_HIP(hipMemcpyAsync(table_.get(), table_pinned_[r], lay.bytes, hipMemcpyHostToDevice, st));
-  if (!captured) GS_HIP(hipEventRecord(table_ev_[r], st));
   const uint32_t *seg_start_d = reinterpret_cast<const uint32_t *>(table_.get());
   const uint32_t *rblock_start_d = reinterpret_cast<const uint32_t *>(table_.get() + lay.rblock_start);
   const ViewDev *views_d = reinterpret_cast<const ViewDev *>(table_.get() + lay.views);
@@ -453,7 +458,6 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   uint32_t *visible = info + kVisibleWord;
   unsigned long long *raster_pairs = reinterpret_cast<unsigned long long *>(info + kRasterPairsWord);
 
-  if (!captured) GS_HIP(hipEventRecord(ev_[0], st));
   // The projection zeroes the summary words and the binning tile ranges of
   // this frame.
   Clears frame_clears;
@@ -474,7 +478,7 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
         frame_clears = Clears();  // once
       }
   }
-  if (!captured) GS_HIP(hipEventRecord(ev_[1], st));
+  if (stage_events) GS_HIP(hipEventRecord(ev_[1], st));
 
   uint32_t *k = keys_.get(), *v = vals_.get(), *ka = keys_alt_.get(), *va = vals_alt_.get();
   SortPrep depth_prep;
@@ -486,7 +490,7 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
                 depth_scratch_.capacity(), st, value_byte, 0, depth_prep);
   sorted_keys_ = k;
   sorted_vals_ = v;
-  if (!captured) GS_HIP(hipEventRecord(ev_[2], st));
+  if (stage_events) GS_HIP(hipEventRecord(ev_[2], st));
 
   Clears scan_clears;  // the depth sort's scratch, for the next frame
   if (depth_lookback) scan_clears.add(depth_scratch_.get(), depth_dirty);
@@ -521,13 +525,13 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
     ro.shift = host_views_[0].bin_shift;
     rblocks = raster_blocks(ctiles, ro.shift);
   }
-  if (!captured) GS_HIP(hipEventRecord(ev_[3], st));
+  if (stage_events) GS_HIP(hipEventRecord(ev_[3], st));
 
   unsigned long long *evals = count_evals_ ? reinterpret_cast<unsigned long long *>(info + kEvalsWord) : nullptr;
   launch_raster(tuning_.raster, views_d, rblock_start_d, nviews, rblocks, ro, tuning_.blend_min,
                 rec_.get(), rect_.get(),
                 ranges_.get(), pv, img32_.get(), img8_.get(), st, evals);
-  if (!captured) GS_HIP(hipEventRecord(ev_[4], st));
+  if (stage_events) GS_HIP(hipEventRecord(ev_[4], st));
   for (size_t i = 0; i < host_rgba8_.size(); ++i) {
     if (!host_rgba8_[i] || host_views_[i].out8) continue;  // written by the raster itself
     const ViewDev &d = host_views_[i];
@@ -536,7 +540,10 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   }
   const size_t summary_words = count_evals_ ? kSummaryWords : kVisibleWord + nviews;
   GS_HIP(hipMemcpyAsync(summary_pinned_, info, summary_words * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  if (!captured) GS_HIP(hipEventRecord(ev_[5], st));
+  if (!captured) {
+    GS_HIP(hipEventRecord(ev_[5], st));
+    GS_HIP(hipEventRecord(table_ev_[r], st));  // the staging slot may be rewritten
+  }
 }
 
 bool Pipeline::absorb_summary() {
@@ -564,9 +571,12 @@ void Pipeline::wait() {
       continue;
     }
     float ms[4] = {0, 0, 0, 0}, total = 0;
-    if (staged_times_)
+    if (staged_times_) {
       for (int s = 0; s < 4; ++s) GS_HIP(hipEventElapsedTime(&ms[s], ev_[s], ev_[s + 1]));
-    GS_HIP(hipEventElapsedTime(&total, ev_[0], ev_[4]));
+      GS_HIP(hipEventElapsedTime(&total, ev_[0], ev_[4]));
+    } else {
+      GS_HIP(hipEventElapsedTime(&total, ev_[0], ev_[5]));
+    }
     uint64_t evals[2] = {0, 0};
     if (count_evals_) std::memcpy(evals, summary_pinned_ + kEvalsWord, sizeof(evals));
     times_ = FrameTimes{ms[0], ms[1], ms[2], ms[3], total, evals[0], evals[1]};

@@ -191,6 +191,10 @@ class Pipeline {
   // (pixel, splat) evaluations that pass coverage before each pixel's early
   // stop; the count is reported with the frame times.
   void set_count_evals(bool on) { count_evals_ = on; }
+  // Per-stage device times for the frames that follow (events at each stage
+  // boundary, which cost the device a drain each); otherwise only the total.
+  void set_stage_timing(bool on) { stage_timing_ = on; }
+  bool stage_timing() const { return stage_timing_; }
   // Binning tiles of (16 << shift) pixels, shift <= 3 (also GSPLAT_BIN_SHIFT);
   // images and tile counts do not depend on it.
   void set_bin_shift(uint32_t shift);
@@ -271,7 +275,8 @@ class Pipeline {
   bool depth_clean_ = false, pair_clean_ = false;  // look-back scratch words are zero
   bool use_graph_ = false;
   GraphSlot graphs_[2];
-  bool staged_times_ = true;  // false: the last frame ran as a graph, total time only
+  bool staged_times_ = false;  // the last frame recorded its stage events
+  bool stage_timing_ = false;  // record stage events (set_stage_timing, GSPLAT_STAGE_TIMES)
 
 };
 

@@ -31,6 +31,7 @@ class Renderer {
   GsplatStageTimes stage_times();
   uint64_t overflow_rerenders() const { return pipe_.overflow_rerenders(); }
   void set_tuning(const Tuning &t) { pipe_.set_tuning(t); }
+  void set_stage_timing(bool on) { pipe_.set_stage_timing(on); }
   const Tuning &tuning() const { return pipe_.tuning(); }
   // Autotunes on the scene and `camera` at the current size; the context's
   // last frame is then that frame.

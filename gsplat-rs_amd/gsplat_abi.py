@@ -249,6 +249,8 @@ SIGNATURES = {
     "gsplat_batch_set_tuning": (I32, [P, ctypes.POINTER(GsplatTuning)]),
     "gsplat_batch_autotune": (I32, [P, ctypes.POINTER(GsplatViewDesc), U32, U32, ctypes.POINTER(GsplatTuning)]),
     "gsplat_context_get_tuning": (I32, [P, ctypes.POINTER(GsplatTuning)]),
+    "gsplat_context_set_stage_timing": (I32, [P, U32]),
+    "gsplat_batch_set_stage_timing": (I32, [P, U32]),
     "gsplat_context_set_tuning": (I32, [P, ctypes.POINTER(GsplatTuning)]),
     "gsplat_context_autotune": (I32, [P, U32, ctypes.POINTER(GsplatTuning)]),
     "gsplat_server_config_default": (GsplatServerConfig, []),
@@ -397,6 +399,10 @@ class Context:
         s = GsplatStageTimes()
         check(lib().gsplat_context_get_stage_times(self.h, ctypes.byref(s)))
         return s
+
+    def set_stage_timing(self, on: bool = True):
+        """Per-stage device times for the following frames (an event per stage boundary)."""
+        check(lib().gsplat_context_set_stage_timing(self.h, 1 if on else 0))
 
     def rgba8(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 4), np.uint8)
@@ -549,6 +555,10 @@ class Batch:
         s = GsplatStageTimes()
         check(lib().gsplat_batch_get_stage_times(self.h, ctypes.byref(s)))
         return s
+
+    def set_stage_timing(self, on: bool = True):
+        """Per-stage device times for the following frames (an event per stage boundary)."""
+        check(lib().gsplat_batch_set_stage_timing(self.h, 1 if on else 0))
 
     def sorted_indices(self, v: int, count: int) -> np.ndarray:
         out = np.zeros(max(count, 1), np.uint32)

@@ -73,7 +73,11 @@ int32_t gsplat_context_readback_rgba8(GsplatContext *ctx, uint8_t *out, uint64_t
 int32_t gsplat_context_readback_rgba32f(GsplatContext *ctx, float *out, uint64_t len);
 
 /* Device time per stage of the last frame (hipEvent pairs on the context's
- * stream) and the sizes that drive each stage. */
+ * stream) and the sizes that drive each stage. The four stage times are
+ * filled for frames rendered with stage timing on
+ * (gsplat_context_set_stage_timing / gsplat_batch_set_stage_timing, or
+ * GSPLAT_STAGE_TIMES=1) and are 0 otherwise: an event at each stage boundary
+ * costs the device a drain (about 6 us each). total_ms is always filled. */
 typedef struct GsplatStageTimes {
   float project_ms;  /* depth key + cull + projection + SH colour      */
   float sort_ms;     /* stable descending depth sort                   */
@@ -98,6 +102,7 @@ typedef struct GsplatStageTimes {
                             * bin_shift pixels (GsplatTuning.bin_auto)  */
 } GsplatStageTimes;
 int32_t gsplat_context_get_stage_times(GsplatContext *ctx, GsplatStageTimes *out);
+int32_t gsplat_context_set_stage_timing(GsplatContext *ctx, uint32_t on);
 
 /* ------------------------------------------------ parity/inspection hooks */
 
@@ -226,6 +231,7 @@ int32_t gsplat_batch_read_tile_counts(GsplatBatch *batch, uint32_t view, uint32_
 /* Whole-frame stage times: splat_count and visible_count are sums over the
  * views, tiles_x the total tile count over the views and tiles_y 1. */
 int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out);
+int32_t gsplat_batch_set_stage_timing(GsplatBatch *batch, uint32_t on);
 
 /* ----------------------------------------------------------------- tuning */
 

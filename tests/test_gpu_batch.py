@@ -243,3 +243,26 @@ def test_context_autotune(abi):
     assert best.frame_ms > 0
     assert np.array_equal(ctx.rgba8(), img)
     ctx.close()
+
+
+def test_stage_timing_switch(abi):
+    # Stage times come from frames with stage events; without them only the
+    # total is measured. The frame itself is the same either way.
+    scene = synth.tiny_scene(20000, 4242, sh_degree=3)
+    dev = abi.Scene(scene)
+    views = [(dev, abi.look_at((0.3, -0.2, -2.8)), 320, 240), (dev, abi.look_at((2.0, 1.0, 1.5)), 256, 144)]
+    b = abi.Batch()
+    b.render(views)
+    st = b.stage_times()
+    base = batch_view(b, 1, scene.count)
+    assert st.total_ms > 0 and (st.project_ms, st.sort_ms, st.bin_ms, st.raster_ms) == (0, 0, 0, 0)
+    b.set_stage_timing(True)
+    b.render(views)
+    st = b.stage_times()
+    assert min(st.project_ms, st.sort_ms, st.bin_ms, st.raster_ms) > 0
+    assert st.project_ms + st.sort_ms + st.bin_ms + st.raster_ms <= st.total_ms * 1.0001
+    assert_same(batch_view(b, 1, scene.count), base, "stage-timed frame")
+    b.set_stage_timing(False)
+    b.render(views)
+    assert b.stage_times().raster_ms == 0
+    b.close()
