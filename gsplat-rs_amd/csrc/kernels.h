@@ -34,7 +34,6 @@ FrameParams make_frame_params(const float position[3], const float rotation_xyzw
 constexpr uint32_t kTile = 16;            // raster tile edge in pixels
 constexpr uint32_t kMaxBinShift = 3;      // binning tiles of 16, 32, 64 or 128 pixels
 constexpr uint32_t kThreads = 256;        // threads per workgroup (4 wave64)
-constexpr uint32_t kScanTile = kThreads * 16;
 constexpr uint32_t kCountBlock = kThreads * 4;  // slots per tile-count workgroup
 constexpr uint32_t kMaxGrid = 4096;       // bound of the grids that loop over device-sized work
 constexpr uint32_t kSegmentAlign = 4096;  // view slot segments align to the sort tile
@@ -134,18 +133,6 @@ void launch_project_export(uint32_t degree, const ViewDev *views, uint32_t view,
                            float *export12, hipStream_t stream);
 
 // Scratch of the two-level exclusive scan.
-struct ScanScratch {
-  uint32_t *partials = nullptr;  // first-level partials
-  uint64_t *total = nullptr;     // 64-bit grand total (device)
-  size_t partial_cap = 0;
-};
-size_t scan_partial_entries(size_t n);
-
-// Exclusive scan of n u32 values (in place allowed); the 64-bit sum goes to
-// *scratch.total. With n_dev the length is min(*n_dev, n), read on the device.
-void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s, hipStream_t stream,
-                    const uint32_t *n_dev = nullptr);
-
 // Two-level binning. Splats are binned into binning tiles of (16 << shift)
 // pixels; each 16x16 raster tile then keeps, in order, the splats of its
 // binning tile whose pixel bounds overlap it. shift 0 bins straight into the

@@ -397,80 +397,6 @@ __global__ __launch_bounds__(kThreads) void k_project_export(const ViewDev *__re
   o[11] = 0.0f;
 }
 
-// ------------------------------------------------------------------- scans
-
-// The scan's length is n_cap, or min(*n_dev, n_cap) when n_dev is given (a
-// count produced on the device); grids are bounded and loop over scan tiles,
-// so the cost follows the live length, not the capacity.
-__device__ __forceinline__ size_t scan_length(const uint32_t *n_dev, size_t n_cap) {
-  return n_dev ? min((size_t)*n_dev, n_cap) : n_cap;
-}
-
-// First level: each block scans kScanTile items (16 consecutive per thread)
-// into `out` and writes their sum to partials[tile].
-__global__ __launch_bounds__(kThreads) void k_scan_blocks(const uint32_t *__restrict__ in,
-                                                          uint32_t *__restrict__ out, size_t n_cap,
-                                                          const uint32_t *__restrict__ n_dev,
-                                                          uint32_t *__restrict__ partials) {
-  __shared__ uint32_t lds4[kWaves];
-  const size_t n = scan_length(n_dev, n_cap);
-  for (size_t b = blockIdx.x; b * kScanTile < n; b += gridDim.x) {
-    const size_t base = b * kScanTile + (size_t)threadIdx.x * 16;
-    uint32_t v[16];
-    uint32_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const size_t p = base + k;
-      v[k] = p < n ? in[p] : 0u;
-      sum += v[k];
-    }
-    uint32_t total;
-    uint32_t run = block_exclusive_scan(sum, lds4, &total);
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const size_t p = base + k;
-      if (p < n) out[p] = run;
-      run += v[k];
-    }
-    if (threadIdx.x == 0) partials[b] = total;
-  }
-}
-
-// Second level: one block scans the partials (any count) and writes the
-// 64-bit grand total.
-__global__ __launch_bounds__(kThreads) void k_scan_partials(uint32_t *__restrict__ partials, size_t n_cap,
-                                                            const uint32_t *__restrict__ n_dev,
-                                                            uint64_t *__restrict__ total64) {
-  __shared__ uint32_t lds4[kWaves];
-  const uint32_t count = (uint32_t)((scan_length(n_dev, n_cap) + kScanTile - 1) / kScanTile);
-  uint32_t carry = 0;
-  uint64_t grand = 0;
-  for (uint32_t base = 0; base < count; base += kThreads) {
-    const uint32_t p = base + threadIdx.x;
-    const uint32_t v = p < count ? partials[p] : 0u;
-    uint32_t tot;
-    const uint32_t ex = block_exclusive_scan(v, lds4, &tot);
-    if (p < count) partials[p] = carry + ex;
-    carry += tot;
-    grand += tot;
-  }
-  if (threadIdx.x == 0 && total64) *total64 = grand;
-}
-
-__global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t *__restrict__ out, size_t n_cap,
-                                                       const uint32_t *__restrict__ n_dev,
-                                                       const uint32_t *__restrict__ partials) {
-  const size_t n = scan_length(n_dev, n_cap);
-  for (size_t b = blockIdx.x + 1; b * kScanTile < n; b += gridDim.x) {
-    const uint32_t add = partials[b];
-    const size_t base = b * kScanTile;
-    for (uint32_t k = threadIdx.x; k < kScanTile; k += kThreads) {
-      const size_t p = base + k;
-      if (p < n) out[p] += add;
-    }
-  }
-}
-
 // ------------------------------------------------------------- tile binning
 
 // Number of tiles of (16 << shift) pixels a pixel-bounds rectangle
@@ -1271,23 +1197,6 @@ void launch_project_export(uint32_t deg, const ViewDev *views, uint32_t view, ui
     case 4: k_project_export<4><<<grid, block, 0, st>>>(views, view, keys_out, export12); break;
     default: throw Error(Code::Unsupported, "sh degree above 4");
   }
-  GS_HIP(hipGetLastError());
-}
-
-size_t scan_partial_entries(size_t n) { return std::max<size_t>(div_up(n, kScanTile), 1); }
-
-void exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, ScanScratch &s, hipStream_t st,
-                    const uint32_t *n_dev) {
-  if (n == 0) {
-    launch_fill_u32(reinterpret_cast<uint32_t *>(s.total), 0u, 2, st);
-    return;
-  }
-  const uint32_t tiles = div_up(n, kScanTile);
-  if (tiles > s.partial_cap) throw Error(Code::Internal, "scan scratch too small");
-  const uint32_t grid = std::min<uint32_t>(tiles, kMaxGrid);
-  k_scan_blocks<<<grid, kThreads, 0, st>>>(in, out, n, n_dev, s.partials);
-  k_scan_partials<<<1, kThreads, 0, st>>>(s.partials, n, n_dev, s.total);
-  if (tiles > 1) k_scan_add<<<std::min<uint32_t>(tiles - 1, kMaxGrid), kThreads, 0, st>>>(out, n, n_dev, s.partials);
   GS_HIP(hipGetLastError());
 }
 

@@ -18,7 +18,8 @@
 // Reduce-then-scan mode, for long sorts (the look-back front advances a
 // bounded number of tiles per coherent round trip, which caps its rate):
 //   k_rts_up     per-tile digit counts, digit-major
-//   scan         one exclusive scan of all counts = every (digit, tile) start
+//   k_rts_scan   one exclusive scan of all counts = every (digit, tile) start
+//                (chained over 4096-count chunks by decoupled look-back)
 //   k_rts_down   rank and scatter
 //
 // The sort is stable, so with keys complemented (descending) and values in
@@ -31,7 +32,8 @@
 // Forward progress of the look-back: tiles take their index from a per-pass
 // atomic counter when they start, so every tile a workgroup waits on has
 // already started and finishes without waiting on later tiles. The
-// reduce-then-scan passes have no inter-workgroup waits.
+// reduce-then-scan scan takes its chunks by ticket the same way; its up- and
+// down-sweeps have no inter-workgroup waits.
 
 #include <hip/hip_runtime.h>
 
@@ -69,6 +71,9 @@ constexpr uint32_t FLAG_AGG = 1u << 30;  // tile count published
 constexpr uint32_t FLAG_INC = 2u << 30;  // inclusive prefix published
 constexpr uint32_t VALUE_MASK = (1u << 30) - 1;
 constexpr uint32_t XCDS = 8;  // MI355X: workgroup b is dispatched to XCD b % 8
+constexpr uint32_t SCAN_IPT = 16;               // counts per thread of the reduce-then-scan scan
+constexpr uint32_t SCAN_CHUNK = T * SCAN_IPT;   // counts per scan chunk (one workgroup step)
+constexpr uint32_t kScanGrid = 1024;            // bound of the scan grid (workgroups loop over chunks)
 
 using PassSet = RadixPasses;
 
@@ -115,6 +120,19 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *l
   return off + inc - v;
 }
 
+// Adds one count of digit d for every active lane to the LDS histogram h.
+// The lanes sharing the first active lane's digit add once, together: keys
+// that share a digit across a wave (a near-constant high byte, a view id)
+// would otherwise serialise on one LDS address.
+__device__ __forceinline__ void wave_count(uint32_t *h, uint32_t d) {
+  const uint32_t lead = __builtin_amdgcn_readfirstlane(d);
+  const uint64_t same = __ballot(d == lead);
+  if (d != lead)
+    atomicAdd(&h[d], 1u);
+  else if (__lane_id() == (uint32_t)__builtin_ctzll(same))
+    atomicAdd(&h[d], (uint32_t)__popcll(same));
+}
+
 __device__ __forceinline__ uint32_t load_count(const uint32_t *n_dev, uint32_t n_cap) {
   return n_dev ? min(*n_dev, n_cap) : n_cap;
 }
@@ -151,8 +169,7 @@ __global__ __launch_bounds__(T) void k_os_hist(const uint32_t *__restrict__ keys
 #pragma unroll
     for (uint32_t r = 0; r < IPT; ++r) {
       if (base + (size_t)r * T + threadIdx.x < n)
-        for (uint32_t p = 0; p < ps.count; ++p)
-          atomicAdd(&h[p][digit(k[r], v[r], ps.shift[p], ps.flip[p], ps.from_value[p])], 1u);
+        for (uint32_t p = 0; p < ps.count; ++p) wave_count(h[p], digit(k[r], v[r], ps.shift[p], ps.flip[p], ps.from_value[p]));
     }
   }
   __syncthreads();
@@ -361,17 +378,23 @@ __device__ __forceinline__ CountIndex count_index(uint32_t tile, uint32_t tiles,
 
 // Reduce-then-scan pass, step 1: digit counts of every live tile laid out
 // by count_index, so one exclusive scan of the 256 * tiles counts (length
-// published in *entries) gives every (digit, tile) its output start.
+// published in *entries) gives every (digit, tile) its output start. Block 0
+// also clears the scan state of the pass (ticket + one word per chunk).
 __global__ __launch_bounds__(T) void k_rts_up(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                               const uint32_t *__restrict__ n_dev, uint32_t n_cap, uint32_t shift,
                                               uint32_t flip, uint32_t from_value, uint32_t *__restrict__ counts,
-                                              uint32_t *__restrict__ entries, const uint32_t *__restrict__ seg_start,
-                                              uint32_t segs) {
+                                              uint32_t *__restrict__ entries, uint32_t *__restrict__ scan_state,
+                                              const uint32_t *__restrict__ seg_start, uint32_t segs) {
   __shared__ uint32_t h[WAVES][256];
   const uint32_t t = threadIdx.x, w = t / 64;
   const uint32_t n = load_count(n_dev, n_cap);
   const uint32_t tiles = (n + TILE - 1) / TILE;
-  if (blockIdx.x == 0 && t == 0) *entries = 256 * tiles;
+  if (blockIdx.x == 0) {
+    if (t == 0) *entries = 256 * tiles;
+    // the scan's ticket and chunk status words start every pass at zero
+    const uint32_t chunks = (256 * tiles + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    for (uint32_t i = t; i <= chunks; i += T) scan_state[i] = 0;
+  }
   const uint32_t *src = from_value ? vin : kin;
   for (uint32_t tile = xcd_first_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x) {
 #pragma unroll
@@ -387,10 +410,86 @@ __global__ __launch_bounds__(T) void k_rts_up(const uint32_t *__restrict__ kin, 
     }
 #pragma unroll
     for (uint32_t r = 0; r < IPT; ++r)
-      if (r * T + t < count) atomicAdd(&h[w][((x[r] ^ flip) >> shift) & 0xFFu], 1u);
+      if (r * T + t < count) wave_count(h[w], ((x[r] ^ flip) >> shift) & 0xFFu);
     __syncthreads();
     counts[count_index(tile, tiles, seg_start, segs)(t, tile)] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
     __syncthreads();
+  }
+}
+
+// Reduce-then-scan pass, step 2: exclusive scan of the live counts in
+// place, one launch. Workgroups take chunks of SCAN_CHUNK counts by ticket
+// (so every chunk a workgroup waits on has started), publish the chunk total
+// and resolve the prefix over the preceding chunks by decoupled look-back:
+// wave 0 polls 64 predecessors per step and adds published totals up to the
+// nearest inclusive prefix. Count totals are at most the element count, so
+// they fit the 30 value bits of a status word.
+__global__ __launch_bounds__(T) void k_rts_scan(uint32_t *__restrict__ counts, const uint32_t *__restrict__ entries,
+                                                uint32_t *__restrict__ scan_state) {
+  __shared__ uint32_t s_chunk, s_prefix;
+  __shared__ uint32_t lds4[WAVES];
+  const uint32_t t = threadIdx.x, lane = __lane_id();
+  const uint32_t n = *entries;  // a multiple of 256: a thread's 16 counts are all live or all past the end
+  uint32_t *status = scan_state + 1;
+  for (;;) {
+    if (t == 0) s_chunk = atomicAdd(scan_state, 1u);
+    __syncthreads();
+    const uint32_t c = s_chunk;
+    const size_t base = (size_t)c * SCAN_CHUNK;
+    if (base >= n) break;  // uniform: later tickets are past the end too
+    const size_t mine = base + (size_t)t * SCAN_IPT;
+    const bool live = mine < n;
+    uint4 q[SCAN_IPT / 4];
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_IPT / 4; ++k)
+      q[k] = live ? reinterpret_cast<const uint4 *>(counts + mine)[k] : make_uint4(0u, 0u, 0u, 0u);
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_IPT / 4; ++k) sum += q[k].x + q[k].y + q[k].z + q[k].w;
+    uint32_t run = block_exclusive_scan(sum, lds4);
+    if (t < 64) {
+      const uint32_t total = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+      uint32_t prefix = 0;
+      if (c == 0) {
+        if (t == 0) __hip_atomic_store(status, FLAG_INC | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        if (t == 0) __hip_atomic_store(status + c, FLAG_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int64_t j = (int64_t)c - 1;  // nearest predecessor not yet added
+        for (;;) {
+          const int64_t pred = j - (int64_t)lane;
+          const uint32_t sv =
+              pred >= 0 ? __hip_atomic_load(status + pred, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : FLAG_INC;
+          const uint32_t flag = sv & ~VALUE_MASK;
+          const uint64_t unpublished = __ballot(flag == 0);
+          const uint64_t inclusive = __ballot(flag == FLAG_INC);
+          const uint32_t first_gap = unpublished ? (uint32_t)__builtin_ctzll(unpublished) : 64u;
+          const uint32_t first_inc = inclusive ? (uint32_t)__builtin_ctzll(inclusive) : 64u;
+          const bool done = first_inc < first_gap;
+          const uint32_t take = done ? first_inc + 1 : first_gap;
+          const uint32_t part = wave_inclusive_scan(lane < take ? (sv & VALUE_MASK) : 0u);
+          prefix += __shfl(part, 63, 64);
+          if (done) break;
+          j -= take;
+        }
+        if (t == 0)
+          __hip_atomic_store(status + c, FLAG_INC | (prefix + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (t == 0) s_prefix = prefix;
+    }
+    __syncthreads();
+    run += s_prefix;
+    if (live) {
+#pragma unroll
+      for (uint32_t k = 0; k < SCAN_IPT / 4; ++k) {
+        uint4 o;
+        o.x = run; run += q[k].x;
+        o.y = run; run += q[k].y;
+        o.z = run; run += q[k].z;
+        o.w = run; run += q[k].w;
+        reinterpret_cast<uint4 *>(counts + mine)[k] = o;
+      }
+    }
+    __syncthreads();  // s_chunk, s_prefix and lds4 are reused by the next chunk
   }
 }
 
@@ -451,13 +550,13 @@ static uint32_t rts_min_tiles() {
 
 // Layout, look-back mode: hist [STRIPES][MAX_PASSES][256] | counters
 // [MAX_PASSES] | status [passes][tiles][256], all cleared at the start of
-// every sort. Reduce-then-scan mode: counts [256][tiles] | scan partials |
-// 64-bit scan total; nothing needs clearing.
+// every sort. Reduce-then-scan mode: counts [256][tiles] | live count
+// length | scan ticket | one status word per scan chunk (cleared by k_rts_up).
 size_t onesweep_status_words(uint32_t n_cap, uint32_t bits) {
   const uint32_t passes = std::min<uint32_t>(std::max<uint32_t>((bits + 7) / 8, 1), 4) + 1;
   const size_t tiles = std::max<uint32_t>(div_up(n_cap, TILE), 1);
   const size_t lookback = (size_t)STRIPES * MAX_PASSES * 256 + MAX_PASSES + (size_t)passes * tiles * 256;
-  const size_t rts = 256 * tiles + scan_partial_entries(256 * tiles) + 4;
+  const size_t rts = 256 * tiles + 2 + div_up(256 * tiles, SCAN_CHUNK);
   return std::max(lookback, rts);
 }
 
@@ -501,14 +600,12 @@ void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32
   const uint32_t grid_tiles = std::min(std::max(expect, std::min(tiles, kMinGrid)), pass_grid_bound());
   if (!radix_uses_lookback(n_cap, n_dev ? n_hint : 0, prep.rts_min_tiles)) {
     const size_t entries = (size_t)256 * tiles;
-    const size_t partials = scan_partial_entries(entries);
-    if (entries + partials + 4 > scratch_words) throw Error(Code::Internal, "radix sort scratch too small");
+    const uint32_t chunks = div_up(entries, SCAN_CHUNK);
+    if (entries + 2 + chunks > scratch_words) throw Error(Code::Internal, "radix sort scratch too small");
     uint32_t *counts = scratch;
-    ScanScratch sc;
-    sc.partials = counts + entries;
-    sc.partial_cap = partials;
-    uint32_t *live_entries = counts + entries + partials;
-    sc.total = reinterpret_cast<uint64_t *>(scratch + ((entries + partials + 2) & ~size_t(1)));
+    uint32_t *live_entries = counts + entries;
+    uint32_t *scan_state = live_entries + 1;
+    const uint32_t scan_grid = std::min(chunks, kScanGrid);
     // the tile order of xcd_first_tile needs a grid of whole XCD runs
     const uint32_t grid = (grid_tiles + XCDS - 1) / XCDS * XCDS;
     // Segments replace a value pass that orders by segment index.
@@ -518,8 +615,8 @@ void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32
     const uint32_t segs = segmented ? prep.segs : 0;
     for (uint32_t p = 0; p < passes; ++p) {
       k_rts_up<<<grid, T, 0, st>>>(*keys, *vals, n_dev, n_cap, ps.shift[p], ps.flip[p], ps.from_value[p], counts,
-                                   live_entries, seg, segs);
-      exclusive_scan(counts, counts, entries, sc, st, live_entries);
+                                   live_entries, scan_state, seg, segs);
+      k_rts_scan<<<scan_grid, T, 0, st>>>(counts, live_entries, scan_state);
       k_rts_down<<<grid, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, ps.shift[p], ps.flip[p],
                                      ps.from_value[p], counts, seg, segs);
       std::swap(*keys, *keys_alt);
