@@ -8,8 +8,8 @@
 
 namespace gs {
 
-constexpr uint32_t kSortThreads = 256;
-constexpr uint32_t kSortItems = 16;
+constexpr uint32_t kSortThreads = 512;
+constexpr uint32_t kSortItems = 8;
 constexpr uint32_t kSortTile = kSortThreads * kSortItems;
 constexpr uint32_t kSortMaxPasses = 5;
 constexpr uint32_t kSortStripes = 8;  // striped copies of the digit histograms

@@ -1,8 +1,10 @@
 // sort.hip -- LSD radix sort of (u32 key, u32 value) pairs for gfx950, 8-bit
-// digits, 4096-item tiles of 256 threads x 16 items.
+// digits, 4096-item tiles of 512 threads x 8 items (eight waves per tile:
+// ranking is a chain of LDS rounds per wave, so more waves per tile shorten
+// it; threads 0-255 keep the books of one digit each).
 //
-// Every pass ranks a tile the same way: its four waves work independently (a
-// wave owns 1024 consecutive items and a private LDS histogram; ballot
+// Every pass ranks a tile the same way: its eight waves work independently (a
+// wave owns 512 consecutive items and a private LDS histogram; ballot
 // match-any over the 8 digit bits + popcount gives the rank among equal
 // digits), then the tile is staged in LDS in digit order and written out
 // coalesced. The passes differ in how a tile learns where its digits go:
@@ -60,12 +62,14 @@ constexpr uint32_t T = kSortThreads;
 constexpr uint32_t IPT = kSortItems;
 constexpr uint32_t TILE = kSortTile;
 constexpr uint32_t WAVES = T / 64;
+constexpr uint32_t DIGITS = 256;  // thread t < DIGITS keeps the books of digit t
+static_assert(T >= DIGITS && T % 64 == 0 && TILE % T == 0, "sort workgroup shape");
 constexpr uint32_t WAVE_ITEMS = TILE / WAVES;
 constexpr uint32_t MAX_PASSES = kSortMaxPasses;
 constexpr uint32_t HIST_BLOCKS = 512;
 constexpr uint32_t STRIPES = kSortStripes;
 constexpr uint32_t LOOKBACK = 8;  // predecessors polled per look-back step
-constexpr uint32_t kPassGrid = 1024;  // look-back passes: resident 38 KB-LDS workgroups, 4 per CU
+constexpr uint32_t kPassGrid = 768;   // look-back passes: resident 42 KB-LDS workgroups, 3 per CU
 constexpr uint32_t kMinGrid = 1024;   // floor of a hinted grid
 constexpr uint32_t FLAG_AGG = 1u << 30;  // tile count published
 constexpr uint32_t FLAG_INC = 2u << 30;  // inclusive prefix published
@@ -145,7 +149,7 @@ __global__ __launch_bounds__(T) void k_os_clear(uint32_t *__restrict__ p, size_t
   for (size_t i = (size_t)blockIdx.x * T + threadIdx.x; i < n; i += (size_t)gridDim.x * T) p[i] = 0u;
 }
 
-// Workgroups take tiles of 4096 keys (16 per thread, all loaded before any
+// Workgroups take tiles of 4096 keys (8 per thread, all loaded before any
 // is counted) and count every pass's digits with LDS atomics; each
 // workgroup then adds its non-zero bins into one of STRIPES global copies
 // (copy = workgroup mod STRIPES) to spread the global atomics.
@@ -153,7 +157,7 @@ __global__ __launch_bounds__(T) void k_os_hist(const uint32_t *__restrict__ keys
                                                const uint32_t *__restrict__ n_dev, uint32_t n_cap, PassSet ps,
                                                uint32_t *__restrict__ hist) {
   __shared__ uint32_t h[MAX_PASSES][256];
-  for (uint32_t p = 0; p < MAX_PASSES; ++p) h[p][threadIdx.x] = 0;
+  for (uint32_t i = threadIdx.x; i < MAX_PASSES * 256; i += T) h[i / 256][i % 256] = 0;
   __syncthreads();
   bool need_vals = false;
   for (uint32_t p = 0; p < ps.count; ++p) need_vals = need_vals || ps.from_value[p];
@@ -174,18 +178,24 @@ __global__ __launch_bounds__(T) void k_os_hist(const uint32_t *__restrict__ keys
   }
   __syncthreads();
   uint32_t *dst = hist + (size_t)(blockIdx.x % STRIPES) * MAX_PASSES * 256;
-  for (uint32_t p = 0; p < ps.count; ++p) {
-    const uint32_t c = h[p][threadIdx.x];
-    if (c) atomicAdd(&dst[p * 256 + threadIdx.x], c);
+  for (uint32_t i = threadIdx.x; i < ps.count * 256; i += T) {
+    const uint32_t c = h[i / 256][i % 256];
+    if (c) atomicAdd(&dst[i], c);
   }
 }
 
+__device__ __forceinline__ void clear_whist(uint32_t (*whist)[256]) {
+#pragma unroll
+  for (uint32_t i = threadIdx.x; i < WAVES * 256; i += T) whist[i / 256][i % 256] = 0;
+}
+
 // Loads tile items [base, base+count) and ranks each among the items of
-// equal digit in its wave: wave w owns items [w*1024, (w+1)*1024) in order, 64
+// equal digit in its wave: wave w owns items [w*512, (w+1)*512) in order, 64
 // per round; within a round the lanes of equal digit are ranked by lane and
 // the running per-digit count lives in whist[w]. On return whist[q][d] is the
 // start of wave q's digit-d items among the tile's digit-d items, and the
-// result is the tile's count of digit threadIdx.x. whist must be zero.
+// result is the tile's count of digit threadIdx.x (0 for threads past the
+// digits). whist must be zero.
 __device__ __forceinline__ uint32_t rank_tile(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                               size_t base, uint32_t count, uint32_t shift, uint32_t flip,
                                               uint32_t from_value, uint32_t (&key)[IPT], uint32_t (&val)[IPT],
@@ -213,13 +223,15 @@ __device__ __forceinline__ uint32_t rank_tile(const uint32_t *__restrict__ kin, 
     if (active && lrank == 0) wh[d] = before + (uint32_t)__popcll(peers);
   }
   __syncthreads();
-  // Thread t owns digit t: turn the four wave counts into wave offsets.
+  // Thread t < DIGITS owns digit t: turn the wave counts into wave offsets.
   uint32_t run = 0;
+  if (t < DIGITS) {
 #pragma unroll
-  for (uint32_t q = 0; q < WAVES; ++q) {
-    const uint32_t c = whist[q][t];
-    whist[q][t] = run;
-    run += c;
+    for (uint32_t q = 0; q < WAVES; ++q) {
+      const uint32_t c = whist[q][t];
+      whist[q][t] = run;
+      run += c;
+    }
   }
   return run;
 }
@@ -276,16 +288,17 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
   const uint32_t tiles_n = (n + TILE - 1) / TILE;
   uint32_t *status = status_all + (size_t)pass * tiles_n * 256;
   uint32_t hd = 0;
+  if (t < DIGITS) {
 #pragma unroll
-  for (uint32_t c = 0; c < STRIPES; ++c) hd += hist[(c * MAX_PASSES + pass) * 256 + t];
+    for (uint32_t c = 0; c < STRIPES; ++c) hd += hist[(c * MAX_PASSES + pass) * 256 + t];
+  }
   const uint32_t g = block_exclusive_scan(hd, lds4);
   // every key has the same digit: the pass is the identity
   const bool trivial = __syncthreads_or(n > 0 && hd == n) != 0;
 
   for (;;) {
     if (t == 0) s_tile = atomicAdd(counter, 1u);
-#pragma unroll
-    for (uint32_t q = 0; q < WAVES; ++q) whist[q][t] = 0;
+    clear_whist(whist);
     __syncthreads();
     const uint32_t tile = s_tile;
     const size_t base = (size_t)tile * TILE;
@@ -304,9 +317,10 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
     const uint32_t run = rank_tile(kin, vin, base, count, shift, flip, from_value, key, val, rank, whist);
 
     // Publish this tile's count of digit t, then look back for the prefix.
-    uint32_t *st = status + (size_t)tile * 256 + t;
     uint32_t excl = 0;
-    if (tile == 0) {
+    uint32_t *st = status + (size_t)tile * 256 + t;
+    if (t >= DIGITS) {
+    } else if (tile == 0) {
       __hip_atomic_store(st, FLAG_INC | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       __hip_atomic_store(st, FLAG_AGG | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -336,8 +350,11 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
       }
       __hip_atomic_store(st, FLAG_INC | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    gofs[t] = g + excl;
-    lstart[t] = block_exclusive_scan(run, lds4);
+    const uint32_t ls = block_exclusive_scan(run, lds4);
+    if (t < DIGITS) {
+      gofs[t] = g + excl;
+      lstart[t] = ls;
+    }
     __syncthreads();
     scatter_tile(key, val, rank, whist, lstart, gofs, count, shift, flip, from_value, skeys, svals, kout, vout);
     __syncthreads();
@@ -397,8 +414,7 @@ __global__ __launch_bounds__(T) void k_rts_up(const uint32_t *__restrict__ kin, 
   }
   const uint32_t *src = from_value ? vin : kin;
   for (uint32_t tile = xcd_first_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x) {
-#pragma unroll
-    for (uint32_t q = 0; q < WAVES; ++q) h[q][t] = 0;
+    clear_whist(h);
     __syncthreads();
     const size_t base = (size_t)tile * TILE;
     const uint32_t count = (uint32_t)min((size_t)TILE, (size_t)n - base);
@@ -412,7 +428,12 @@ __global__ __launch_bounds__(T) void k_rts_up(const uint32_t *__restrict__ kin, 
     for (uint32_t r = 0; r < IPT; ++r)
       if (r * T + t < count) wave_count(h[w], ((x[r] ^ flip) >> shift) & 0xFFu);
     __syncthreads();
-    counts[count_index(tile, tiles, seg_start, segs)(t, tile)] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
+    if (t < DIGITS) {
+      uint32_t c = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < WAVES; ++q) c += h[q][t];
+      counts[count_index(tile, tiles, seg_start, segs)(t, tile)] = c;
+    }
     __syncthreads();
   }
 }
@@ -448,7 +469,9 @@ __global__ __launch_bounds__(T) void k_rts_scan(uint32_t *__restrict__ counts, c
     for (uint32_t k = 0; k < SCAN_IPT / 4; ++k) sum += q[k].x + q[k].y + q[k].z + q[k].w;
     uint32_t run = block_exclusive_scan(sum, lds4);
     if (t < 64) {
-      const uint32_t total = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+      uint32_t total = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < WAVES; ++q) total += lds4[q];
       uint32_t prefix = 0;
       if (c == 0) {
         if (t == 0) __hip_atomic_store(status, FLAG_INC | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -512,13 +535,13 @@ __global__ __launch_bounds__(T) void k_rts_down(const uint32_t *__restrict__ kin
   for (uint32_t tile = xcd_first_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x) {
     const size_t base = (size_t)tile * TILE;
     const uint32_t count = (uint32_t)min((size_t)TILE, (size_t)n - base);
-#pragma unroll
-    for (uint32_t q = 0; q < WAVES; ++q) whist[q][t] = 0;
-    dst[t] = offsets[count_index(tile, tiles, seg_start, segs)(t, tile)];
+    clear_whist(whist);
+    if (t < DIGITS) dst[t] = offsets[count_index(tile, tiles, seg_start, segs)(t, tile)];
     __syncthreads();
     uint32_t key[IPT], val[IPT], rank[IPT];
     const uint32_t run = rank_tile(kin, vin, base, count, shift, flip, from_value, key, val, rank, whist);
-    lstart[t] = block_exclusive_scan(run, lds4);
+    const uint32_t ls = block_exclusive_scan(run, lds4);
+    if (t < DIGITS) lstart[t] = ls;
     __syncthreads();
     scatter_tile(key, val, rank, whist, lstart, dst, count, shift, flip, from_value, skeys, svals, kout, vout);
     __syncthreads();
