@@ -649,6 +649,7 @@ gs::Tuning tuning_from(const GsplatTuning &t, gs::Tuning base) {
   out.rts_min_tiles = t.sort_rts_min_tiles;
   out.blend_min = t.blend_min;
   out.bin_auto = t.bin_auto != 0 ? 1u : 0u;
+  out.rows2_min_tiles = t.rows2_min_tiles;
   return out;
 }
 
@@ -659,8 +660,8 @@ GsplatTuning tuning_to(const gs::Tuning &t, float ms) {
   out.sort_rts_min_tiles = t.rts_min_tiles;
   out.frame_ms = ms;
   out.blend_min = t.blend_min;
-  out.bin_auto = t.bin_auto;
   out.bin_auto = t.bin_auto != 0 ? 1u : 0u;
+  out.rows2_min_tiles = t.rows2_min_tiles;
   return out;
 }
 

@@ -147,6 +147,8 @@ Pipeline::Pipeline() {
   }
   if (const char *e = std::getenv("GSPLAT_BLEND_MIN"))
     tuning_.blend_min = std::min<uint32_t>(std::max<uint32_t>((uint32_t)std::strtoul(e, nullptr, 10), 1u), 256u);
+  if (const char *e = std::getenv("GSPLAT_RASTER_ROWS2_MIN_TILES"))
+    tuning_.rows2_min_tiles = (uint32_t)std::strtoul(e, nullptr, 10);
   if (const char *e = std::getenv("GSPLAT_SORT_RTS_MIN_TILES"))
     tuning_.rts_min_tiles = std::max<uint32_t>((uint32_t)std::strtoul(e, nullptr, 10), 1u);
   GS_HIP(hipSetDevice(0));
@@ -371,8 +373,8 @@ void Pipeline::launch_frame() {
     // anything but the view table's contents changes.
     std::vector<uint64_t> key = {(uint64_t)r, nviews, slots, tiles, ctiles, tile_bits, (uint64_t)(value_byte + 1),
                                  pair_cap_, hint, tuning_.bin_shift,
-                                 (uint64_t)tuning_.raster, tuning_.rts_min_tiles, tuning_.blend_min, tuning_.bin_auto, count_evals_, depth_clean_,
-                                 pair_clean_};
+                                 (uint64_t)tuning_.raster, tuning_.rows2_min_tiles, tuning_.rts_min_tiles, tuning_.blend_min,
+                                 tuning_.bin_auto, count_evals_, depth_clean_, pair_clean_};
     for (const ViewDev &d : host_views_) {
       key.push_back(reinterpret_cast<uintptr_t>(d.soa));
       key.push_back(((uint64_t)d.n << 8) | d.sh_degree);
@@ -528,7 +530,13 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   if (stage_events) GS_HIP(hipEventRecord(ev_[3], st));
 
   unsigned long long *evals = count_evals_ ? reinterpret_cast<unsigned long long *>(info + kEvalsWord) : nullptr;
-  launch_raster(tuning_.raster, views_d, rblock_start_d, nviews, rblocks, ro, tuning_.blend_min,
+  // Large frames of one binning layout blend two rows per thread: with the
+  // heaviest tiles first the longer workgroups do not stretch the tail.
+  RasterVariant variant = tuning_.raster;
+  if (variant == RasterVariant::Unrolled && plan.uniform_shift && host_views_[0].bin_shift >= 1 &&
+      tuning_.rows2_min_tiles && plan.tiles >= tuning_.rows2_min_tiles)
+    variant = RasterVariant::Rows2;
+  launch_raster(variant, views_d, rblock_start_d, nviews, rblocks, ro, tuning_.blend_min,
                 rec_.get(), rect_.get(),
                 ranges_.get(), pv, img32_.get(), img8_.get(), st, evals);
   if (stage_events) GS_HIP(hipEventRecord(ev_[4], st));
@@ -642,6 +650,7 @@ std::vector<TuneSample> Pipeline::autotune(const std::vector<ViewSpec> &views, u
       t.bin_shift = shift;
       t.rts_min_tiles = rts;
       t.raster = rv;
+      t.rows2_min_tiles = 0;  // the variant is chosen here
       set_tuning(t);
       enqueue(views);  // warm: buffers grow, the pair count settles
       wait();

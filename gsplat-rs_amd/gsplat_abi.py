@@ -128,7 +128,8 @@ class GsplatViewDesc(ctypes.Structure):
 class GsplatTuning(ctypes.Structure):
     _fields_ = [("bin_shift", ctypes.c_uint32), ("raster_variant", ctypes.c_uint32),
                 ("sort_rts_min_tiles", ctypes.c_uint32), ("frame_ms", ctypes.c_float),
-                ("blend_min", ctypes.c_uint32), ("bin_auto", ctypes.c_uint32)]
+                ("blend_min", ctypes.c_uint32), ("bin_auto", ctypes.c_uint32),
+                ("rows2_min_tiles", ctypes.c_uint32)]
 
 
 class GsplatServerConfig(ctypes.Structure):

@@ -257,6 +257,10 @@ int32_t gsplat_batch_set_stage_timing(GsplatBatch *batch, uint32_t on);
  *                       on-screen splat diameters, at most 16 << bin_shift
  *                       pixels (small views of large scenes get small
  *                       binning tiles); 0: every view uses bin_shift
+ *   rows2_min_tiles     with raster_variant 1, frames of at least this many
+ *                       16x16 tiles whose views share one binning tile of at
+ *                       least 32 pixels blend as variant 2 (default 24576,
+ *                       three 1080p views; 0: never)
  */
 typedef struct GsplatTuning {
   uint32_t bin_shift;
@@ -265,6 +269,7 @@ typedef struct GsplatTuning {
   float frame_ms;
   uint32_t blend_min;
   uint32_t bin_auto;
+  uint32_t rows2_min_tiles;
 } GsplatTuning;
 
 int32_t gsplat_batch_get_tuning(GsplatBatch *batch, GsplatTuning *out_tuning);

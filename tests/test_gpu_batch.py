@@ -207,10 +207,15 @@ def test_autotune_keeps_images(abi):
                                                                      best.raster_variant)
     for v in range(len(vs)):
         assert_same(batch_view(b, v, scenes[views[v][0]].count), before[v], f"view {v} after autotune")
-    # every setting renders the same frame
-    for shift, variant, blend_min in ((0, 0, 64), (3, 0, 256), (2, 2, 1), (1, 2, 64), (2, 1, 200)):
-        t.bin_shift, t.raster_variant, t.blend_min = shift, variant, blend_min
+    # every setting renders the same frame; rows2_min_tiles 1 with one
+    # binning layout (bin_auto off) switches variant 1 to two rows per thread
+    for shift, variant, blend_min, bin_auto, rows2 in ((0, 0, 64, 1, 0), (3, 0, 256, 1, 0), (2, 2, 1, 1, 0),
+                                                       (1, 2, 64, 1, 0), (2, 1, 200, 1, 0), (2, 1, 64, 0, 1),
+                                                       (1, 1, 64, 0, 1), (0, 1, 64, 0, 1)):
+        t.bin_shift, t.raster_variant, t.blend_min, t.bin_auto, t.rows2_min_tiles = shift, variant, blend_min, \
+            bin_auto, rows2
         b.set_tuning(t)
+        assert b.tuning().rows2_min_tiles == rows2
         b.render(vs)
         for v in range(len(vs)):
             assert_same(batch_view(b, v, scenes[views[v][0]].count), before[v], f"view {v} shift {shift}")
