@@ -153,7 +153,11 @@ Pipeline::Pipeline() {
     tuning_.rts_min_tiles = std::max<uint32_t>((uint32_t)std::strtoul(e, nullptr, 10), 1u);
   GS_HIP(hipSetDevice(0));
   GS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  for (auto &e : ev_) GS_HIP(hipEventCreate(&e));
+  // The frame-start and stage-boundary events only time the device: no
+  // system-scope fence (a cache writeback and invalidation per event). The
+  // frame's last event keeps it: the host then reads what the frame wrote.
+  for (int e = 0; e < 6; ++e)
+    GS_HIP(hipEventCreateWithFlags(&ev_[e], e < 5 ? hipEventDisableSystemFence : hipEventDefault));
   for (int r = 0; r < 2; ++r) {
     GS_HIP(hipEventCreateWithFlags(&table_ev_[r], hipEventDisableTiming));
     GS_HIP(hipEventRecord(table_ev_[r], stream_));

@@ -20,6 +20,10 @@ RasterVariant raster_variant_from_env() {
 
 Renderer::Renderer(uint32_t width, uint32_t height) : width_(width), height_(height) {
   pipe_.set_raster_variant(raster_variant_from_env());
+  // A context reports its phase times with every frame, as FrameStats does
+  // (render-wgpu/src/lib.rs:565-583): stage events on unless the
+  // environment says otherwise (gsplat_context_set_stage_timing turns them off).
+  if (!std::getenv("GSPLAT_STAGE_TIMES")) pipe_.set_stage_timing(true);
   set_size(width, height);
 }
 

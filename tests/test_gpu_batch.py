@@ -271,3 +271,18 @@ def test_stage_timing_switch(abi):
     b.render(views)
     assert b.stage_times().raster_ms == 0
     b.close()
+    # a context times its stages by default (its stats carry the phase times)
+    ctx = abi.Context(320, 240)
+    ctx.load(scene)
+    ctx.set_camera(abi.look_at((0.3, -0.2, -2.8)))
+    ctx.render()
+    ctx.wait()
+    s = ctx.stage_times()
+    assert min(s.project_ms, s.sort_ms, s.bin_ms, s.raster_ms) > 0
+    img = ctx.rgba8()
+    ctx.set_stage_timing(False)
+    ctx.render()
+    ctx.wait()
+    assert ctx.stage_times().sort_ms == 0 and ctx.stage_times().total_ms > 0
+    assert np.array_equal(ctx.rgba8(), img)
+    ctx.close()
