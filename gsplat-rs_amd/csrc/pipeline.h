@@ -119,9 +119,10 @@ struct Tuning {
                                                          // (0: no bound; the server uses 256)
   RasterVariant raster = RasterVariant::Unrolled;        // blend loop
   uint32_t rows2_min_tiles = 24576;                      // Unrolled frames of >= this many 16x16 tiles, all
-                                                         // views at one bin shift >= 1, blend two rows per
-                                                         // thread (Rows2; 0: never): fewer LDS reads and
-                                                         // shared column terms pay once the chip is full
+                                                         // views at one bin shift >= 1, at most one splat
+                                                         // slot per two pixels, blend two rows per thread
+                                                         // (Rows2; 0: never): fewer LDS reads and shared
+                                                         // column terms pay once the chip is full
   uint32_t rts_min_tiles = 256;                          // radix sorts of >= this many tiles: reduce-then-scan
   uint32_t blend_min = 64;                               // staged splats that start a blend (1..256)
 };

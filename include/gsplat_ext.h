@@ -259,8 +259,9 @@ int32_t gsplat_batch_set_stage_timing(GsplatBatch *batch, uint32_t on);
  *                       binning tiles); 0: every view uses bin_shift
  *   rows2_min_tiles     with raster_variant 1, frames of at least this many
  *                       16x16 tiles whose views share one binning tile of at
- *                       least 32 pixels blend as variant 2 (default 24576,
- *                       three 1080p views; 0: never)
+ *                       least 32 pixels, with at most one splat per two
+ *                       pixels, blend as variant 2 (default 24576, three
+ *                       1080p views; 0: never)
  */
 typedef struct GsplatTuning {
   uint32_t bin_shift;
