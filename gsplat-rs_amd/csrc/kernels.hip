@@ -673,6 +673,13 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
 
 // ------------------------------------------------------------------ raster
 
+// Records a raster stage holds: a blend starts once blend_min are staged, so
+// at most blend_min - 1 are left when up to kThreads more are added; the
+// blend pads the stage to a multiple of four.
+__host__ __device__ __forceinline__ uint32_t stage_capacity(uint32_t blend_min) {
+  return (blend_min + kThreads - 1 + 3) & ~3u;
+}
+
 // Raster workgroup order, in one of two layouts.
 // Per view (order null): view v owns the workgroups [rblock_start[v],
 // rblock_start[v] + raster_blocks) (its binning tiles in whole runs of kXcds,
@@ -881,8 +888,11 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
                                                      const uint32_t *__restrict__ list, float4 *__restrict__ img32,
                                                      uchar4 *__restrict__ img8,
                                                      unsigned long long *__restrict__ evals) {
-  constexpr uint32_t kStage = 2 * kThreads;  // a stage below 256 takes one more batch of up to 256
-  __shared__ float4 s0[kStage], s1[kStage], s2[kStage];
+  // the LDS stage, three arrays of stage_capacity(blend_min) records (sized
+  // at launch: a smaller stage leaves room for more workgroups per CU)
+  extern __shared__ float4 stage_lds[];
+  const uint32_t cap = stage_capacity(blend_min);
+  float4 *s0 = stage_lds, *s1 = stage_lds + cap, *s2 = stage_lds + 2 * cap;
   __shared__ uint32_t wkeep[kWaves];
   uint32_t v, tx, ty;
   if (!raster_tile(views, rblock_start, nviews, total_rblocks, ro, blockIdx.x, v, tx, ty)) return;
@@ -1035,8 +1045,9 @@ __global__ __launch_bounds__(kThreads) void k_raster_rows2(const ViewDev *__rest
                                                            const uint32_t *__restrict__ list,
                                                            float4 *__restrict__ img32, uchar4 *__restrict__ img8,
                                                            unsigned long long *__restrict__ evals) {
-  constexpr uint32_t kStage = 2 * kThreads;
-  __shared__ float4 s0[kStage], s1[kStage], s2[kStage];
+  extern __shared__ float4 stage_lds[];
+  const uint32_t cap = stage_capacity(blend_min);
+  float4 *s0 = stage_lds, *s1 = stage_lds + cap, *s2 = stage_lds + 2 * cap;
   __shared__ uint32_t wkeep[kWaves];
   uint32_t v, tx, ty;
   if (!raster_tile(views, rblock_start, nviews, total_rblocks, ro, blockIdx.x, v, tx, ty)) return;
@@ -1267,28 +1278,29 @@ void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *
   if (total_rblocks == 0 || nviews == 0) return;
   if (blend_min == 0 || blend_min > kThreads) throw Error(Code::InvalidArgument, "blend threshold outside 1..256");
   const dim3 grid(total_rblocks), block(kThreads);
+  const size_t lds = 3 * (size_t)stage_capacity(blend_min) * sizeof(float4);
   if (variant == RasterVariant::Rows2) {  // every view at bin shift >= 1 (the caller's duty)
     if (evals)
-      k_raster_rows2<true><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
+      k_raster_rows2<true><<<grid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
                                                    list, img32, img8, evals);
     else
-      k_raster_rows2<false><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
+      k_raster_rows2<false><<<grid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
                                                     list, img32, img8, evals);
     GS_HIP(hipGetLastError());
     return;
   }
   const bool unrolled = variant != RasterVariant::Linear;
   if (unrolled && evals)
-    k_raster<true, 1><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges, list,
+    k_raster<true, 1><<<grid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges, list,
                                               img32, img8, evals);
   else if (unrolled)
-    k_raster<false, 1><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
+    k_raster<false, 1><<<grid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
                                                list, img32, img8, evals);
   else if (evals)
-    k_raster<true, 0><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges, list,
+    k_raster<true, 0><<<grid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges, list,
                                               img32, img8, evals);
   else
-    k_raster<false, 0><<<grid, block, 0, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
+    k_raster<false, 0><<<grid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
                                                list, img32, img8, evals);
   GS_HIP(hipGetLastError());
 }
