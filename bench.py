@@ -72,6 +72,51 @@ def cpu_baseline(scene, view, cam):
                       f"project {st[2]:.1f}, raster {st[3]:.1f}"}
 
 
+def write_plain_artifact(ctx, scene, view, directory, iterations=120, warmup=10):
+    """The plain workload as a gsplat-benchmark/v1 run: the reference runner's
+    protocol (warmup 10, 120 measured frames, one render call + wait each) on
+    the context, with its stage times as the frame stats."""
+    import bench_artifact as BA
+    started = BA.utc_now()
+    run_id = "plain-1080p-300k-sh3-" + started.replace(":", "").replace("-", "").replace(".", "")
+    ctx.set_stage_timing(True)
+    frames, m0, m1 = BA.measure_context(ctx, iterations, warmup, run_id)
+    arrays = [scene.positions, scene.opacity, scene.scale_xyz, scene.rotation_xyzw, scene.color_dc]
+    if scene.sh_rest is not None:
+        arrays.append(scene.sh_rest)
+    blob = b"".join(np.ascontiguousarray(a, dtype=np.float32).tobytes() for a in arrays)
+    trace = (f"gsplat-camera-trace/v1\nmode=static\ncamera=look_at position={list(view.position)} "
+             f"target={list(view.target)}\n").encode()
+    build, build_un = BA.build_info(ROOT)
+    env, env_un = BA.environment_info(adapter="AMD Instinct MI355X (gfx950)")
+    timed = frames[0]["preprocess_ms"] is not None
+    unavailable = build_un + env_un + ["frames[*].sort_refreshed"]
+    if not timed:
+        unavailable += [f"frames[*].{k}" for k in ("preprocess_ms", "sort_ms", "geometry_submit_ms")]
+    manifest = {
+        "schema": BA.SCHEMA, "record_type": "manifest", "run_id": run_id,
+        "identity": {"series_id": "mi355x-plain-1080p-300k-sh3", "started_at_utc": started,
+                     "ended_at_utc": BA.utc_now(), "measurement_started_at_utc": m0,
+                     "measurement_ended_at_utc": m1},
+        "build": build,
+        "dataset": {"id": "synthetic-plain-1080p-300k-sh3", "sha256": BA.sha256_bytes(blob), "bytes": len(blob),
+                    "splat_count": scene.count, "sh_degree": scene.sh_degree},
+        "trace": {"id": "static-look-at-v1", "sha256": BA.sha256_bytes(trace)},
+        "renderer": {"implementation": "gsplat-rs_amd", "path": "sorted_alpha_tiles", "backend": "hip-gfx950",
+                     "sort_policy": "gpu_every_frame"},
+        "display": {"width": view.width, "height": view.height, "dpr": 1.0, "refresh_hz": 60.0,
+                    "frame_budget_ms": 1000.0 / 60.0, "refresh_hz_source": "configured",
+                    "frame_budget_source": "configured"},
+        "environment": env,
+        "timing": {"frame_wall_source": "host wall clock from the render call to gsplat_context_wait returning "
+                                        "(device completion); stage times are device events"},
+        "unavailable_fields": unavailable,
+    }
+    BA.write_run(directory, manifest, frames, warmup)
+    BA.check_run(directory)
+    print(f"artifact written: {directory}", file=sys.stderr, flush=True)
+
+
 def time_batches(batch, views, steps, warmup):
     """Frames (views) per second of `steps` back-to-back batch renders of `views`."""
     for _ in range(warmup):
@@ -418,6 +463,8 @@ def main():
     ap.add_argument("--summarize-profiles", nargs=5, metavar=("TRACE", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="write profiles/ summaries from rocprofv3 output directories and exit")
     ap.add_argument("--profiles-out", default=str(ROOT / "profiles"), help="folder for --summarize-profiles")
+    ap.add_argument("--artifact-dir", default=None,
+                    help="also write the plain workload as a gsplat-benchmark/v1 run directory (120 frames)")
     args = ap.parse_args()
     if args.summarize_profiles:
         summarize_profiles(*args.summarize_profiles, frames=0, out_dir=args.profiles_out)
@@ -487,6 +534,8 @@ def main():
     bin_px = 16 << st.bin_shift
     bin_tiles = -(-view.width // bin_px) * -(-view.height // bin_px)
     pair_passes = -(-max((bin_tiles - 1).bit_length(), 1) // 8)
+    if args.artifact_dir:
+        write_plain_artifact(ctx, scene, view, args.artifact_dir)
     ctx.close()
 
     # One counting frame (outside the timed region) for the blend's algorithmic work E.
