@@ -72,6 +72,39 @@ def cpu_baseline(scene, view, cam):
                       f"project {st[2]:.1f}, raster {st[3]:.1f}"}
 
 
+def bench_plain_config(name, scene, view, frames, warmup):
+    """Back-to-back frames of one more plain configuration on a context (its
+    own scene and camera), with the stage device times of stage-timed frames
+    and the frame's sizes."""
+    ctx = A.Context(view.width, view.height)
+    ctx.load(scene)
+    ctx.set_camera(A.look_at(view.position, view.target))
+    ctx.set_stage_timing(False)
+    for _ in range(warmup):
+        ctx.render()
+    ctx.wait()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        ctx.render()
+    ctx.wait()
+    fps = frames / (time.perf_counter() - t0)
+    ctx.set_stage_timing(True)
+    stages = []
+    for _ in range(5):
+        ctx.render()
+        ctx.wait()
+        s = ctx.stage_times()
+        stages.append((s.project_ms, s.sort_ms, s.bin_ms, s.raster_ms))
+    st = ctx.stage_times()
+    ctx.close()
+    med = np.median(np.array(stages), axis=0)
+    return {"workload": name, "splats": scene.count, "sh_degree": scene.sh_degree, "width": view.width,
+            "height": view.height, "frames_per_s": round(fps, 3), "ms_per_frame": round(1e3 / fps, 4),
+            "mpix_per_s": round(fps * view.width * view.height / 1e6, 1), "visible": int(st.visible_count),
+            "tile_pairs": int(st.tile_pairs),
+            "stage_ms": {nm: round(float(x), 4) for nm, x in zip(["project", "sort", "bin", "raster"], med)}}
+
+
 def write_plain_artifact(ctx, scene, view, directory, iterations=120, warmup=10):
     """The plain workload as a gsplat-benchmark/v1 run: the reference runner's
     protocol (warmup 10, 120 measured frames, one render call + wait each) on
@@ -606,6 +639,15 @@ def main():
         "rooflines": rooflines,
     }
     if not args.plain_only:
+        s0, v0 = synth.plain_small()
+        line["plain_640x480_16k"] = bench_plain_config("plain-640x480-16k-sh0", s0, v0, args.steps, args.warmup)
+        print(f"plain_640x480_16k {line['plain_640x480_16k']}", file=sys.stderr, flush=True)
+        del s0
+        s2, v2 = synth.plain_4k()
+        line["plain_4k_3m"] = bench_plain_config("plain-4k-3m-sh3-heavy-tail", s2, v2, max(args.steps // 10, 10),
+                                                 max(args.warmup // 2, 3))
+        print(f"plain_4k_3m {line['plain_4k_3m']}", file=sys.stderr, flush=True)
+        del s2
         line["batched_1080p"] = bench_batched_1080p(scene, max(args.steps // 8, 5), max(args.warmup // 4, 2))
         print(f"batched_1080p {line['batched_1080p']}", file=sys.stderr, flush=True)
         line["batched_64v"] = bench_batched_64v(max(args.steps // 20, 5), max(args.warmup // 4, 2))
