@@ -338,8 +338,8 @@ void Pipeline::launch_frame() {
   if (as_graph)
     hint = (uint32_t)std::min<uint64_t>((uint64_t)kSortTile << bits_for(div_up_u64(hint, kSortTile)), pair_cap_);
   const bool pair_lookback = radix_uses_lookback(pair_cap_, hint, tuning_.rts_min_tiles);
-  const size_t depth_dirty = radix_lookback_words((uint32_t)m, radix_passes(32, true, value_byte).count);
-  const size_t pair_dirty = radix_lookback_words(pair_cap_, radix_passes(tile_bits, false, -1).count);
+  const size_t depth_dirty = radix_lookback_words((uint32_t)m, radix_passes(32, true, value_byte));
+  const size_t pair_dirty = radix_lookback_words(pair_cap_, radix_passes(tile_bits, false, -1));
 
   // View table through a pinned staging slot whose previous upload is done.
   const int r = (int)(frame_counter_++ % 2);

@@ -14,10 +14,13 @@ constexpr uint32_t kSortTile = kSortThreads * kSortItems;
 constexpr uint32_t kSortMaxPasses = 5;
 constexpr uint32_t kSortStripes = 8;  // striped copies of the digit histograms
 
-// The 8-bit digit passes of one sort: pass p orders by byte shift[p]/8 of
-// (key ^ flip[p]), or of the value when from_value[p].
+// The digit passes of one sort: pass p orders by the digit_bits-bit digit
+// at bit shift[p] of (key ^ flip[p]), or of the value when from_value[p].
+// Digits are 8 bits, except that 9-bit keys without a value pass sort in
+// one 9-bit pass.
 struct RadixPasses {
   uint32_t count;
+  uint32_t digit_bits;
   uint32_t shift[kSortMaxPasses];
   uint32_t flip[kSortMaxPasses];
   uint32_t from_value[kSortMaxPasses];
@@ -26,12 +29,8 @@ RadixPasses radix_passes(uint32_t bits, bool descending, int value_byte);
 
 // Digit histograms of a look-back sort, as k_os_hist accumulates them: a
 // kernel that produces the keys may add them instead (sort with hist_ready):
-// hist[((stripe * kSortMaxPasses) + p) * 256 + digit], stripe = workgroup
-// mod kSortStripes.
-__device__ __forceinline__ uint32_t radix_digit(uint32_t key, uint32_t val, uint32_t shift, uint32_t flip,
-                                                uint32_t from_value) {
-  return (((from_value ? val : key) ^ flip) >> shift) & 0xFFu;
-}
+// hist[((stripe * kSortMaxPasses) + p) * 2^digit_bits + digit], stripe =
+// workgroup mod kSortStripes.
 
 // True when a sort of up to n_cap elements, n_hint expected (0: n_cap),
 // runs look-back passes (which need clean scratch and digit histograms)
@@ -41,9 +40,9 @@ __device__ __forceinline__ uint32_t radix_digit(uint32_t key, uint32_t val, uint
 bool radix_uses_lookback(uint32_t n_cap, uint32_t n_hint = 0, uint32_t rts_min_tiles = 0);
 
 // u32 words at the start of the scratch that a look-back sort of n_cap
-// elements with `passes` passes needs zeroed (histograms, tile counters,
-// status words); they are left dirty after the sort.
-size_t radix_lookback_words(uint32_t n_cap, uint32_t passes);
+// elements with passes `ps` needs zeroed (histograms, tile counters, status
+// words); they are left dirty after the sort.
+size_t radix_lookback_words(uint32_t n_cap, const RadixPasses &ps);
 
 struct SortPrep {
   bool scratch_clean = false;  // the look-back words are zero: skip the clear

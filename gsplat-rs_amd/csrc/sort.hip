@@ -62,8 +62,8 @@ constexpr uint32_t T = kSortThreads;
 constexpr uint32_t IPT = kSortItems;
 constexpr uint32_t TILE = kSortTile;
 constexpr uint32_t WAVES = T / 64;
-constexpr uint32_t DIGITS = 256;  // thread t < DIGITS keeps the books of digit t
-static_assert(T >= DIGITS && T % 64 == 0 && TILE % T == 0, "sort workgroup shape");
+constexpr uint32_t MAX_DIGIT_BITS = 9;  // 8-bit digits, or one 9-bit pass for 9-bit keys
+static_assert(T >= (1u << MAX_DIGIT_BITS) && T % 64 == 0 && TILE % T == 0, "sort workgroup shape");
 constexpr uint32_t WAVE_ITEMS = TILE / WAVES;
 constexpr uint32_t MAX_PASSES = kSortMaxPasses;
 constexpr uint32_t HIST_BLOCKS = 512;
@@ -81,9 +81,11 @@ constexpr uint32_t kScanGrid = 1024;            // bound of the scan grid (workg
 
 using PassSet = RadixPasses;
 
+// Digits of DB bits: thread t < 2^DB keeps the books of digit t.
+template <uint32_t DB>
 __device__ __forceinline__ uint32_t digit(uint32_t key, uint32_t val, uint32_t shift, uint32_t flip,
                                           uint32_t from_value) {
-  return radix_digit(key, val, shift, flip, from_value);
+  return (((from_value ? val : key) ^ flip) >> shift) & ((1u << DB) - 1u);
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -91,10 +93,11 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+template <uint32_t DB>
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool active) {
   uint64_t m = __ballot(active);
 #pragma unroll
-  for (int b = 0; b < 8; ++b) {
+  for (uint32_t b = 0; b < DB; ++b) {
     const bool bit = (d >> b) & 1u;
     const uint64_t bb = __ballot(active && bit);
     m &= bit ? bb : ~bb;
@@ -153,11 +156,13 @@ __global__ __launch_bounds__(T) void k_os_clear(uint32_t *__restrict__ p, size_t
 // is counted) and count every pass's digits with LDS atomics; each
 // workgroup then adds its non-zero bins into one of STRIPES global copies
 // (copy = workgroup mod STRIPES) to spread the global atomics.
+template <uint32_t DB>
 __global__ __launch_bounds__(T) void k_os_hist(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
                                                const uint32_t *__restrict__ n_dev, uint32_t n_cap, PassSet ps,
                                                uint32_t *__restrict__ hist) {
-  __shared__ uint32_t h[MAX_PASSES][256];
-  for (uint32_t i = threadIdx.x; i < MAX_PASSES * 256; i += T) h[i / 256][i % 256] = 0;
+  constexpr uint32_t D = 1u << DB;
+  __shared__ uint32_t h[MAX_PASSES][D];
+  for (uint32_t i = threadIdx.x; i < MAX_PASSES * D; i += T) h[i / D][i % D] = 0;
   __syncthreads();
   bool need_vals = false;
   for (uint32_t p = 0; p < ps.count; ++p) need_vals = need_vals || ps.from_value[p];
@@ -173,20 +178,23 @@ __global__ __launch_bounds__(T) void k_os_hist(const uint32_t *__restrict__ keys
 #pragma unroll
     for (uint32_t r = 0; r < IPT; ++r) {
       if (base + (size_t)r * T + threadIdx.x < n)
-        for (uint32_t p = 0; p < ps.count; ++p) wave_count(h[p], digit(k[r], v[r], ps.shift[p], ps.flip[p], ps.from_value[p]));
+        for (uint32_t p = 0; p < ps.count; ++p)
+          wave_count(h[p], digit<DB>(k[r], v[r], ps.shift[p], ps.flip[p], ps.from_value[p]));
     }
   }
   __syncthreads();
-  uint32_t *dst = hist + (size_t)(blockIdx.x % STRIPES) * MAX_PASSES * 256;
-  for (uint32_t i = threadIdx.x; i < ps.count * 256; i += T) {
-    const uint32_t c = h[i / 256][i % 256];
+  uint32_t *dst = hist + (size_t)(blockIdx.x % STRIPES) * MAX_PASSES * D;
+  for (uint32_t i = threadIdx.x; i < ps.count * D; i += T) {
+    const uint32_t c = h[i / D][i % D];
     if (c) atomicAdd(&dst[i], c);
   }
 }
 
-__device__ __forceinline__ void clear_whist(uint32_t (*whist)[256]) {
+template <uint32_t DB>
+__device__ __forceinline__ void clear_whist(uint32_t (*whist)[1u << DB]) {
+  constexpr uint32_t D = 1u << DB;
 #pragma unroll
-  for (uint32_t i = threadIdx.x; i < WAVES * 256; i += T) whist[i / 256][i % 256] = 0;
+  for (uint32_t i = threadIdx.x; i < WAVES * D; i += T) whist[i / D][i % D] = 0;
 }
 
 // Loads tile items [base, base+count) and ranks each among the items of
@@ -196,10 +204,11 @@ __device__ __forceinline__ void clear_whist(uint32_t (*whist)[256]) {
 // start of wave q's digit-d items among the tile's digit-d items, and the
 // result is the tile's count of digit threadIdx.x (0 for threads past the
 // digits). whist must be zero.
+template <uint32_t DB>
 __device__ __forceinline__ uint32_t rank_tile(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                               size_t base, uint32_t count, uint32_t shift, uint32_t flip,
                                               uint32_t from_value, uint32_t (&key)[IPT], uint32_t (&val)[IPT],
-                                              uint32_t (&rank)[IPT], uint32_t (*whist)[256]) {
+                                              uint32_t (&rank)[IPT], uint32_t (*whist)[1u << DB]) {
   const uint32_t t = threadIdx.x, w = t / 64, lane = __lane_id();
   uint32_t *wh = whist[w];
   // All loads first: a round's LDS update would otherwise hold back the next
@@ -215,17 +224,17 @@ __device__ __forceinline__ uint32_t rank_tile(const uint32_t *__restrict__ kin, 
   for (uint32_t r = 0; r < IPT; ++r) {
     const uint32_t j = w * WAVE_ITEMS + r * 64 + lane;
     const bool active = j < count;
-    const uint32_t d = digit(key[r], val[r], shift, flip, from_value);
-    const uint64_t peers = match_digit(d, active);
+    const uint32_t d = digit<DB>(key[r], val[r], shift, flip, from_value);
+    const uint64_t peers = match_digit<DB>(d, active);
     const uint32_t lrank = (uint32_t)__popcll(peers & lanemask_lt());
     const uint32_t before = active ? wh[d] : 0u;
     rank[r] = active ? before + lrank : 0xFFFFFFFFu;
     if (active && lrank == 0) wh[d] = before + (uint32_t)__popcll(peers);
   }
   __syncthreads();
-  // Thread t < DIGITS owns digit t: turn the wave counts into wave offsets.
+  // Thread t < 2^DB owns digit t: turn the wave counts into wave offsets.
   uint32_t run = 0;
-  if (t < DIGITS) {
+  if (t < (1u << DB)) {
 #pragma unroll
     for (uint32_t q = 0; q < WAVES; ++q) {
       const uint32_t c = whist[q][t];
@@ -239,8 +248,9 @@ __device__ __forceinline__ uint32_t rank_tile(const uint32_t *__restrict__ kin, 
 // Stages the ranked tile in LDS in digit order (lstart[d] = start of digit d
 // in the tile) and writes it out coalesced; digit d's items go to
 // dst[d] + (position among the tile's digit-d items).
+template <uint32_t DB>
 __device__ __forceinline__ void scatter_tile(const uint32_t (&key)[IPT], const uint32_t (&val)[IPT],
-                                             const uint32_t (&rank)[IPT], uint32_t (*whist)[256],
+                                             const uint32_t (&rank)[IPT], uint32_t (*whist)[1u << DB],
                                              const uint32_t *lstart, const uint32_t *dst, uint32_t count,
                                              uint32_t shift, uint32_t flip, uint32_t from_value, uint32_t *skeys,
                                              uint32_t *svals, uint32_t *__restrict__ kout,
@@ -249,7 +259,7 @@ __device__ __forceinline__ void scatter_tile(const uint32_t (&key)[IPT], const u
 #pragma unroll
   for (uint32_t r = 0; r < IPT; ++r) {
     if (rank[r] != 0xFFFFFFFFu) {
-      const uint32_t d = digit(key[r], val[r], shift, flip, from_value);
+      const uint32_t d = digit<DB>(key[r], val[r], shift, flip, from_value);
       const uint32_t pos = lstart[d] + whist[w][d] + rank[r];
       skeys[pos] = key[r];
       svals[pos] = val[r];
@@ -258,7 +268,7 @@ __device__ __forceinline__ void scatter_tile(const uint32_t (&key)[IPT], const u
   __syncthreads();
   for (uint32_t j = t; j < count; j += T) {
     const uint32_t k = skeys[j], v = svals[j];
-    const uint32_t d = digit(k, v, shift, flip, from_value);
+    const uint32_t d = digit<DB>(k, v, shift, flip, from_value);
     const size_t pos = (size_t)dst[d] + (j - lstart[d]);
     kout[pos] = k;
     vout[pos] = v;
@@ -268,7 +278,8 @@ __device__ __forceinline__ void scatter_tile(const uint32_t (&key)[IPT], const u
 // Onesweep pass: one launch per digit, tile offsets by decoupled look-back.
 // Persistent: each workgroup takes tiles from the pass counter until they run
 // past the live element count, so the grid is bounded whatever the capacity.
-// Pass p's status words start at p * (live tiles) * 256.
+// Pass p's status words start at p * (live tiles) * 2^DB.
+template <uint32_t DB>
 __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                const uint32_t *__restrict__ n_dev, uint32_t n_cap, uint32_t pass,
@@ -277,20 +288,21 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
                                                uint32_t *__restrict__ status_all) {
   __shared__ uint32_t s_tile;
   __shared__ uint32_t lds4[WAVES];
-  __shared__ uint32_t gofs[256];    // global start of this tile's items of each digit
-  __shared__ uint32_t lstart[256];  // start of each digit inside this tile
-  __shared__ uint32_t whist[WAVES][256];
+  constexpr uint32_t D = 1u << DB;
+  __shared__ uint32_t gofs[D];    // global start of this tile's items of each digit
+  __shared__ uint32_t lstart[D];  // start of each digit inside this tile
+  __shared__ uint32_t whist[WAVES][D];
   __shared__ uint32_t skeys[TILE];
   __shared__ uint32_t svals[TILE];
 
   const uint32_t t = threadIdx.x;
   const uint32_t n = load_count(n_dev, n_cap);
   const uint32_t tiles_n = (n + TILE - 1) / TILE;
-  uint32_t *status = status_all + (size_t)pass * tiles_n * 256;
+  uint32_t *status = status_all + (size_t)pass * tiles_n * D;
   uint32_t hd = 0;
-  if (t < DIGITS) {
+  if (t < D) {
 #pragma unroll
-    for (uint32_t c = 0; c < STRIPES; ++c) hd += hist[(c * MAX_PASSES + pass) * 256 + t];
+    for (uint32_t c = 0; c < STRIPES; ++c) hd += hist[(c * MAX_PASSES + pass) * D + t];
   }
   const uint32_t g = block_exclusive_scan(hd, lds4);
   // every key has the same digit: the pass is the identity
@@ -298,7 +310,7 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
 
   for (;;) {
     if (t == 0) s_tile = atomicAdd(counter, 1u);
-    clear_whist(whist);
+    clear_whist<DB>(whist);
     __syncthreads();
     const uint32_t tile = s_tile;
     const size_t base = (size_t)tile * TILE;
@@ -314,12 +326,12 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
     }
 
     uint32_t key[IPT], val[IPT], rank[IPT];
-    const uint32_t run = rank_tile(kin, vin, base, count, shift, flip, from_value, key, val, rank, whist);
+    const uint32_t run = rank_tile<DB>(kin, vin, base, count, shift, flip, from_value, key, val, rank, whist);
 
     // Publish this tile's count of digit t, then look back for the prefix.
     uint32_t excl = 0;
-    uint32_t *st = status + (size_t)tile * 256 + t;
-    if (t >= DIGITS) {
+    uint32_t *st = status + (size_t)tile * D + t;
+    if (t >= D) {
     } else if (tile == 0) {
       __hip_atomic_store(st, FLAG_INC | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
@@ -329,7 +341,7 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
         uint32_t sv[LOOKBACK];
 #pragma unroll
         for (uint32_t k = 0; k < LOOKBACK; ++k)
-          sv[k] = (j - (int64_t)k >= 0) ? __hip_atomic_load(status + (size_t)(j - k) * 256 + t, __ATOMIC_RELAXED,
+          sv[k] = (j - (int64_t)k >= 0) ? __hip_atomic_load(status + (size_t)(j - k) * D + t, __ATOMIC_RELAXED,
                                                             __HIP_MEMORY_SCOPE_AGENT)
                                         : FLAG_INC;
         // Consume predecessors nearest first until an inclusive prefix
@@ -351,12 +363,12 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
       __hip_atomic_store(st, FLAG_INC | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     const uint32_t ls = block_exclusive_scan(run, lds4);
-    if (t < DIGITS) {
+    if (t < D) {
       gofs[t] = g + excl;
       lstart[t] = ls;
     }
     __syncthreads();
-    scatter_tile(key, val, rank, whist, lstart, gofs, count, shift, flip, from_value, skeys, svals, kout, vout);
+    scatter_tile<DB>(key, val, rank, whist, lstart, gofs, count, shift, flip, from_value, skeys, svals, kout, vout);
     __syncthreads();
   }
 }
@@ -369,20 +381,22 @@ __device__ __forceinline__ uint32_t xcd_first_tile(uint32_t b, uint32_t grid) {
   return (b % XCDS) * (grid / XCDS) + b / XCDS;
 }
 
-// Index of the (digit, tile) count. Unsegmented: digit-major over all tiles
-// (d * tiles + tile). Segmented: segment by segment, digit-major inside each
-// (first * 256 + d * seg_tiles + tile - first, `first` the segment's first
-// tile), so one flat exclusive scan sorts every segment in its own range.
+// Index of the (digit, tile) count for D digits. Unsegmented: digit-major
+// over all tiles (d * tiles + tile). Segmented: segment by segment,
+// digit-major inside each (first * D + d * seg_tiles + tile - first, `first`
+// the segment's first tile), so one flat exclusive scan sorts every segment
+// in its own range.
 struct CountIndex {
-  uint32_t first, span;
+  uint32_t first, span, digits;
   __device__ __forceinline__ size_t operator()(uint32_t d, uint32_t tile) const {
-    return (size_t)first * 256 + (size_t)d * span + (tile - first);
+    return (size_t)first * digits + (size_t)d * span + (tile - first);
   }
 };
 
+template <uint32_t DB>
 __device__ __forceinline__ CountIndex count_index(uint32_t tile, uint32_t tiles, const uint32_t *seg_start,
                                                   uint32_t segs) {
-  if (segs == 0) return CountIndex{0, tiles};
+  if (segs == 0) return CountIndex{0, tiles, 1u << DB};
   uint32_t lo = 0, hi = segs;  // last segment whose start <= the tile's first element
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) / 2;
@@ -390,31 +404,33 @@ __device__ __forceinline__ CountIndex count_index(uint32_t tile, uint32_t tiles,
   }
   const uint32_t first = seg_start[lo] / TILE;
   const uint32_t end = lo + 1 < segs ? seg_start[lo + 1] / TILE : tiles;
-  return CountIndex{first, end - first};
+  return CountIndex{first, end - first, 1u << DB};
 }
 
 // Reduce-then-scan pass, step 1: digit counts of every live tile laid out
-// by count_index, so one exclusive scan of the 256 * tiles counts (length
+// by count_index, so one exclusive scan of the 2^DB * tiles counts (length
 // published in *entries) gives every (digit, tile) its output start. Block 0
 // also clears the scan state of the pass (ticket + one word per chunk).
+template <uint32_t DB>
 __global__ __launch_bounds__(T) void k_rts_up(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                               const uint32_t *__restrict__ n_dev, uint32_t n_cap, uint32_t shift,
                                               uint32_t flip, uint32_t from_value, uint32_t *__restrict__ counts,
                                               uint32_t *__restrict__ entries, uint32_t *__restrict__ scan_state,
                                               const uint32_t *__restrict__ seg_start, uint32_t segs) {
-  __shared__ uint32_t h[WAVES][256];
+  constexpr uint32_t D = 1u << DB;
+  __shared__ uint32_t h[WAVES][D];
   const uint32_t t = threadIdx.x, w = t / 64;
   const uint32_t n = load_count(n_dev, n_cap);
   const uint32_t tiles = (n + TILE - 1) / TILE;
   if (blockIdx.x == 0) {
-    if (t == 0) *entries = 256 * tiles;
+    if (t == 0) *entries = D * tiles;
     // the scan's ticket and chunk status words start every pass at zero
-    const uint32_t chunks = (256 * tiles + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    const uint32_t chunks = (D * tiles + SCAN_CHUNK - 1) / SCAN_CHUNK;
     for (uint32_t i = t; i <= chunks; i += T) scan_state[i] = 0;
   }
   const uint32_t *src = from_value ? vin : kin;
   for (uint32_t tile = xcd_first_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x) {
-    clear_whist(h);
+    clear_whist<DB>(h);
     __syncthreads();
     const size_t base = (size_t)tile * TILE;
     const uint32_t count = (uint32_t)min((size_t)TILE, (size_t)n - base);
@@ -426,13 +442,13 @@ __global__ __launch_bounds__(T) void k_rts_up(const uint32_t *__restrict__ kin, 
     }
 #pragma unroll
     for (uint32_t r = 0; r < IPT; ++r)
-      if (r * T + t < count) wave_count(h[w], ((x[r] ^ flip) >> shift) & 0xFFu);
+      if (r * T + t < count) wave_count(h[w], ((x[r] ^ flip) >> shift) & (D - 1u));
     __syncthreads();
-    if (t < DIGITS) {
+    if (t < D) {
       uint32_t c = 0;
 #pragma unroll
       for (uint32_t q = 0; q < WAVES; ++q) c += h[q][t];
-      counts[count_index(tile, tiles, seg_start, segs)(t, tile)] = c;
+      counts[count_index<DB>(tile, tiles, seg_start, segs)(t, tile)] = c;
     }
     __syncthreads();
   }
@@ -517,6 +533,7 @@ __global__ __launch_bounds__(T) void k_rts_scan(uint32_t *__restrict__ counts, c
 }
 
 // Reduce-then-scan pass, step 3: rank and scatter with the scanned offsets.
+template <uint32_t DB>
 __global__ __launch_bounds__(T) void k_rts_down(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                 uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                 const uint32_t *__restrict__ n_dev, uint32_t n_cap, uint32_t shift,
@@ -524,9 +541,10 @@ __global__ __launch_bounds__(T) void k_rts_down(const uint32_t *__restrict__ kin
                                                 const uint32_t *__restrict__ offsets,
                                                 const uint32_t *__restrict__ seg_start, uint32_t segs) {
   __shared__ uint32_t lds4[WAVES];
-  __shared__ uint32_t dst[256];
-  __shared__ uint32_t lstart[256];
-  __shared__ uint32_t whist[WAVES][256];
+  constexpr uint32_t D = 1u << DB;
+  __shared__ uint32_t dst[D];
+  __shared__ uint32_t lstart[D];
+  __shared__ uint32_t whist[WAVES][D];
   __shared__ uint32_t skeys[TILE];
   __shared__ uint32_t svals[TILE];
   const uint32_t t = threadIdx.x;
@@ -535,15 +553,15 @@ __global__ __launch_bounds__(T) void k_rts_down(const uint32_t *__restrict__ kin
   for (uint32_t tile = xcd_first_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x) {
     const size_t base = (size_t)tile * TILE;
     const uint32_t count = (uint32_t)min((size_t)TILE, (size_t)n - base);
-    clear_whist(whist);
-    if (t < DIGITS) dst[t] = offsets[count_index(tile, tiles, seg_start, segs)(t, tile)];
+    clear_whist<DB>(whist);
+    if (t < D) dst[t] = offsets[count_index<DB>(tile, tiles, seg_start, segs)(t, tile)];
     __syncthreads();
     uint32_t key[IPT], val[IPT], rank[IPT];
-    const uint32_t run = rank_tile(kin, vin, base, count, shift, flip, from_value, key, val, rank, whist);
+    const uint32_t run = rank_tile<DB>(kin, vin, base, count, shift, flip, from_value, key, val, rank, whist);
     const uint32_t ls = block_exclusive_scan(run, lds4);
-    if (t < DIGITS) lstart[t] = ls;
+    if (t < D) lstart[t] = ls;
     __syncthreads();
-    scatter_tile(key, val, rank, whist, lstart, dst, count, shift, flip, from_value, skeys, svals, kout, vout);
+    scatter_tile<DB>(key, val, rank, whist, lstart, dst, count, shift, flip, from_value, skeys, svals, kout, vout);
     __syncthreads();
   }
 }
@@ -571,23 +589,33 @@ static uint32_t rts_min_tiles() {
   return e ? (uint32_t)std::strtoul(e, nullptr, 10) : 512u;
 }
 
-// Layout, look-back mode: hist [STRIPES][MAX_PASSES][256] | counters
-// [MAX_PASSES] | status [passes][tiles][256], all cleared at the start of
-// every sort. Reduce-then-scan mode: counts [256][tiles] | live count
-// length | scan ticket | one status word per scan chunk (cleared by k_rts_up).
+// Layout for D = 2^digit_bits digits, look-back mode: hist
+// [STRIPES][MAX_PASSES][D] | counters [MAX_PASSES] | status
+// [passes][tiles][D], all cleared at the start of every sort.
+// Reduce-then-scan mode: counts [D][tiles] | live count length | scan
+// ticket | one status word per scan chunk (cleared by k_rts_up).
+static size_t lookback_words(uint32_t n_cap, uint32_t passes, uint32_t digits) {
+  return (size_t)STRIPES * MAX_PASSES * digits + MAX_PASSES +
+         (size_t)passes * std::max<uint32_t>(div_up(n_cap, TILE), 1) * digits;
+}
+
 size_t onesweep_status_words(uint32_t n_cap, uint32_t bits) {
+  // the larger of the two layouts, with a value pass and the widest digit
+  const uint32_t digits = bits == MAX_DIGIT_BITS ? 1u << MAX_DIGIT_BITS : 256u;
   const uint32_t passes = std::min<uint32_t>(std::max<uint32_t>((bits + 7) / 8, 1), 4) + 1;
   const size_t tiles = std::max<uint32_t>(div_up(n_cap, TILE), 1);
-  const size_t lookback = (size_t)STRIPES * MAX_PASSES * 256 + MAX_PASSES + (size_t)passes * tiles * 256;
-  const size_t rts = 256 * tiles + 2 + div_up(256 * tiles, SCAN_CHUNK);
-  return std::max(lookback, rts);
+  const size_t rts = digits * tiles + 2 + div_up(digits * tiles, SCAN_CHUNK);
+  return std::max(lookback_words(n_cap, passes, digits), rts);
 }
 
 RadixPasses radix_passes(uint32_t bits, bool descending, int value_byte) {
   RadixPasses ps{};
-  const uint32_t key_passes = std::min<uint32_t>((bits + 7) / 8, 4);
+  // 9-bit keys (a view of up to 512 binning tiles) sort in one 9-bit pass
+  // instead of two 8-bit ones; a value pass keeps 8-bit digits (its byte).
+  ps.digit_bits = (bits == MAX_DIGIT_BITS && value_byte < 0) ? MAX_DIGIT_BITS : 8u;
+  const uint32_t key_passes = std::min<uint32_t>((bits + ps.digit_bits - 1) / ps.digit_bits, 4);
   for (uint32_t p = 0; p < key_passes; ++p) {
-    ps.shift[ps.count] = 8 * p;
+    ps.shift[ps.count] = ps.digit_bits * p;
     ps.flip[ps.count] = descending ? 0xFFFFFFFFu : 0u;
     ps.from_value[ps.count] = 0;
     ++ps.count;
@@ -606,23 +634,24 @@ bool radix_uses_lookback(uint32_t n_cap, uint32_t n_hint, uint32_t min_tiles) {
   return div_up(n, TILE) < (min_tiles ? min_tiles : rts_min_tiles());
 }
 
-size_t radix_lookback_words(uint32_t n_cap, uint32_t passes) {
-  return (size_t)STRIPES * MAX_PASSES * 256 + MAX_PASSES + (size_t)passes * std::max<uint32_t>(div_up(n_cap, TILE), 1) * 256;
+size_t radix_lookback_words(uint32_t n_cap, const RadixPasses &ps) {
+  return lookback_words(n_cap, ps.count, 1u << ps.digit_bits);
 }
 
-void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n_cap,
-                   const uint32_t *n_dev, uint32_t bits, bool descending, uint32_t *scratch, size_t scratch_words,
-                   hipStream_t st, int value_byte, uint32_t n_hint, SortPrep prep) {
-  if (n_cap <= 1 || (bits == 0 && value_byte < 0)) return;
-  if (n_cap > (1u << 30)) throw Error(Code::Unsupported, "sort length exceeds 2^30");
-  const PassSet ps = radix_passes(bits, descending, value_byte);
+namespace {
+
+template <uint32_t DB>
+void sort_passes(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n_cap,
+                 const uint32_t *n_dev, const PassSet &ps, uint32_t *scratch, size_t scratch_words, hipStream_t st,
+                 int value_byte, uint32_t n_hint, const SortPrep &prep) {
+  constexpr uint32_t D = 1u << DB;
   const uint32_t tiles = div_up(n_cap, TILE);
   // Grid: one workgroup per expected tile (the hint, for a device-side count,
   // with the capacity as the cap), bounded by GSPLAT_SORT_MAX_GRID.
   const uint32_t expect = n_dev && n_hint ? std::min(div_up(n_hint, TILE), tiles) : tiles;
   const uint32_t grid_tiles = std::min(std::max(expect, std::min(tiles, kMinGrid)), pass_grid_bound());
   if (!radix_uses_lookback(n_cap, n_dev ? n_hint : 0, prep.rts_min_tiles)) {
-    const size_t entries = (size_t)256 * tiles;
+    const size_t entries = (size_t)D * tiles;
     const uint32_t chunks = div_up(entries, SCAN_CHUNK);
     if (entries + 2 + chunks > scratch_words) throw Error(Code::Internal, "radix sort scratch too small");
     uint32_t *counts = scratch;
@@ -637,35 +666,51 @@ void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32
     const uint32_t *seg = segmented ? prep.seg_start : nullptr;
     const uint32_t segs = segmented ? prep.segs : 0;
     for (uint32_t p = 0; p < passes; ++p) {
-      k_rts_up<<<grid, T, 0, st>>>(*keys, *vals, n_dev, n_cap, ps.shift[p], ps.flip[p], ps.from_value[p], counts,
-                                   live_entries, scan_state, seg, segs);
+      k_rts_up<DB><<<grid, T, 0, st>>>(*keys, *vals, n_dev, n_cap, ps.shift[p], ps.flip[p], ps.from_value[p],
+                                       counts, live_entries, scan_state, seg, segs);
       k_rts_scan<<<scan_grid, T, 0, st>>>(counts, live_entries, scan_state);
-      k_rts_down<<<grid, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, ps.shift[p], ps.flip[p],
-                                     ps.from_value[p], counts, seg, segs);
+      k_rts_down<DB><<<grid, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, ps.shift[p], ps.flip[p],
+                                         ps.from_value[p], counts, seg, segs);
       std::swap(*keys, *keys_alt);
       std::swap(*vals, *vals_alt);
     }
     GS_HIP(hipGetLastError());
     return;
   }
-  const size_t fixed = (size_t)STRIPES * MAX_PASSES * 256 + MAX_PASSES;
-  const size_t words = fixed + (size_t)ps.count * tiles * 256;
+  const size_t fixed = (size_t)STRIPES * MAX_PASSES * D + MAX_PASSES;
+  const size_t words = fixed + (size_t)ps.count * tiles * D;
   if (words > scratch_words) throw Error(Code::Internal, "onesweep scratch too small");
   uint32_t *hist = scratch;
-  uint32_t *counters = hist + (size_t)STRIPES * MAX_PASSES * 256;
+  uint32_t *counters = hist + (size_t)STRIPES * MAX_PASSES * D;
   uint32_t *status = counters + MAX_PASSES;
   if (!prep.scratch_clean)
-    k_os_clear<<<std::min<uint32_t>(div_up(words, T), 1024), T, 0, st>>>(scratch, fixed, (size_t)ps.count * 256,
+    k_os_clear<<<std::min<uint32_t>(div_up(words, T), 1024), T, 0, st>>>(scratch, fixed, (size_t)ps.count * D,
                                                                           n_dev, n_cap);
-  if (!prep.hist_ready) k_os_hist<<<hist_blocks(n_cap), T, 0, st>>>(*keys, *vals, n_dev, n_cap, ps, hist);
+  if (!prep.hist_ready) k_os_hist<DB><<<hist_blocks(n_cap), T, 0, st>>>(*keys, *vals, n_dev, n_cap, ps, hist);
   const uint32_t grid = std::min(grid_tiles, kPassGrid);
   for (uint32_t p = 0; p < ps.count; ++p) {
-    k_os_pass<<<grid, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, p, ps.shift[p], ps.flip[p],
-                                  ps.from_value[p], hist, counters + p, status);
+    k_os_pass<DB><<<grid, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, p, ps.shift[p], ps.flip[p],
+                                      ps.from_value[p], hist, counters + p, status);
     std::swap(*keys, *keys_alt);
     std::swap(*vals, *vals_alt);
   }
   GS_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n_cap,
+                   const uint32_t *n_dev, uint32_t bits, bool descending, uint32_t *scratch, size_t scratch_words,
+                   hipStream_t st, int value_byte, uint32_t n_hint, SortPrep prep) {
+  if (n_cap <= 1 || (bits == 0 && value_byte < 0)) return;
+  if (n_cap > (1u << 30)) throw Error(Code::Unsupported, "sort length exceeds 2^30");
+  const PassSet ps = radix_passes(bits, descending, value_byte);
+  if (ps.digit_bits == MAX_DIGIT_BITS)
+    sort_passes<MAX_DIGIT_BITS>(keys, vals, keys_alt, vals_alt, n_cap, n_dev, ps, scratch, scratch_words, st,
+                                value_byte, n_hint, prep);
+  else
+    sort_passes<8>(keys, vals, keys_alt, vals_alt, n_cap, n_dev, ps, scratch, scratch_words, st, value_byte, n_hint,
+                   prep);
 }
 
 }  // namespace gs
