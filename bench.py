@@ -566,7 +566,8 @@ def main():
     bin_pairs = int(st.bin_pairs)
     bin_px = 16 << st.bin_shift
     bin_tiles = -(-view.width // bin_px) * -(-view.height // bin_px)
-    pair_passes = -(-max((bin_tiles - 1).bit_length(), 1) // 8)
+    pair_bits = max((bin_tiles - 1).bit_length(), 1)
+    pair_passes = 1 if pair_bits == 9 else -(-pair_bits // 8)  # 9-bit keys: one 9-bit pass
     if args.artifact_dir:
         write_plain_artifact(ctx, scene, view, args.artifact_dir)
     ctx.close()

@@ -60,10 +60,10 @@ uint32_t initial_pair_capacity(uint64_t splats) {
 // the binning tile make each 16x16 tile scan a list mostly of splats outside
 // it; large ones multiply the pairs of small binning tiles. In a batch, with
 // bin_max_ctiles set, a view is then binned coarser (up to 128 px) until it
-// has at most that many binning tiles: with 256 or fewer the pair sort is one
-// 8-bit pass. That pays for batches of many small views (served requests)
-// and costs batches of large views with large splats (their raster lists
-// grow more than one pass saves), so it is off by default.
+// has at most that many binning tiles: with 512 or fewer the pair sort is one
+// pass (9-bit digits). That pays for batches of many small views (served
+// requests) and costs batches of large views with large splats (their raster
+// lists grow more than one pass saves), so it is off by default.
 uint32_t view_bin_shift(const DeviceScene &sc, const FrameParams &fp, const Tuning &t, bool batch) {
   const uint32_t lo = t.raster == RasterVariant::Rows2 ? 1u : 0u;  // two tile rows share a list
   if (!t.bin_auto || !(sc.median_scale > 0.0f)) return std::max(t.bin_shift, lo);

@@ -116,7 +116,7 @@ struct Tuning {
   float bin_auto_k = 3.0f;                               // binning tile edge in splat diameters (bin_auto)
   uint32_t bin_max_ctiles = 0;                           // bin_auto, batches: at most this many binning
                                                          // tiles per view where a shift up to 3 allows
-                                                         // (0: no bound; the server uses 256)
+                                                         // (0: no bound; the server uses 512)
   RasterVariant raster = RasterVariant::Unrolled;        // blend loop
   uint32_t rows2_min_tiles = 24576;                      // Unrolled frames of >= this many 16x16 tiles, all
                                                          // views at one bin shift >= 1, at most one splat

@@ -73,10 +73,11 @@ Server::Server(const ServerConfig &cfg) : cfg_(cfg) {
     slots_.push_back(std::make_unique<Slot>());
     slots_.back()->pipe = std::make_unique<Pipeline>();
     slots_.back()->pipe->set_raster_variant(raster_variant_from_env());
-    // Served batches mix many small requests: binning them coarse enough for
-    // a one-pass pair sort pays (measured 6.3k -> 6.9k requests/s closed loop).
+    // Served batches mix many small requests: binning each at most 512
+    // binning tiles keeps the pair sort one (9-bit) pass (a 24-request batch:
+    // 3.26 ms, against 3.40 ms unbounded and 3.82 ms at 256).
     Tuning t = slots_.back()->pipe->tuning();
-    if (!std::getenv("GSPLAT_BIN_MAX_CTILES")) t.bin_max_ctiles = 256;
+    if (!std::getenv("GSPLAT_BIN_MAX_CTILES")) t.bin_max_ctiles = 512;
     slots_.back()->pipe->set_tuning(t);
   }
   for (auto &s : slots_) s->completer = std::thread(&Server::complete_loop, this, s.get());
