@@ -1,11 +1,12 @@
 // sort.hip -- LSD radix sort of (u32 key, u32 value) pairs for gfx950, 8-bit
-// digits, 4096-item tiles of 512 threads x 8 items (eight waves per tile:
-// ranking is a chain of LDS rounds per wave, so more waves per tile shorten
-// it; threads 0-255 keep the books of one digit each).
+// digits (or one 9-bit pass for 9-bit keys), 4096-item tiles of 512 threads x
+// 8 items (eight waves per tile: ranking is a chain of LDS rounds per wave, so
+// more waves per tile shorten it; thread t keeps the books of digit t, for
+// the 256 or 512 digits of a pass).
 //
 // Every pass ranks a tile the same way: its eight waves work independently (a
 // wave owns 512 consecutive items and a private LDS histogram; ballot
-// match-any over the 8 digit bits + popcount gives the rank among equal
+// match-any over the digit bits + popcount gives the rank among equal
 // digits), then the tile is staged in LDS in digit order and written out
 // coalesced. The passes differ in how a tile learns where its digits go:
 //
