@@ -576,6 +576,8 @@ uint32_t gsplat_device_available(void) {
 
 // ------------------------------------------------------- resident scenes
 
+}  // extern "C"
+
 namespace {
 
 int32_t make_scene(gs::HostScene &&s, GsplatScene **out) {
@@ -588,6 +590,8 @@ int32_t make_scene(gs::HostScene &&s, GsplatScene **out) {
 }
 
 }  // namespace
+
+extern "C" {
 
 int32_t gsplat_scene_create(const GsplatSceneView *view, GsplatScene **out_scene) {
   static const char *op = "gsplat_scene_create";
@@ -637,6 +641,8 @@ void gsplat_scene_destroy(GsplatScene *scene) {
 }
 
 // ---------------------------------------------------------------- batches
+
+}  // extern "C"
 
 namespace {
 
@@ -688,6 +694,8 @@ std::vector<gs::ViewSpec> view_specs(const char *op, const GsplatViewDesc *views
 }
 
 }  // namespace
+
+extern "C" {
 
 
 int32_t gsplat_batch_create(GsplatBatch **out_batch) {
