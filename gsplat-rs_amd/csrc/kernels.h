@@ -174,13 +174,22 @@ void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t 
                       const uint32_t *offsets_rev, const uint32_t *block_sums, uint32_t capacity,
                       uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t stream);
 
+// Words copied by one workgroup of a kernel, dst[i] = src[i] for i < words
+// (dst may be pinned host memory: the frame's summary, read by the host
+// after the frame's last event, without a copy launch of its own).
+struct WordCopy {
+  const uint32_t *src = nullptr;
+  uint32_t *dst = nullptr;
+  uint32_t words = 0;
+};
+
 // After a stable sort of the pairs on the tile bits of their keys:
 // ranges[t] = [first, last+1) of global binning tile t (= ctile_off of the
 // key's view + the key's tile; the ranges must be zero beforehand); the pair
 // count is read from *m_dev (grid sized for `capacity`). The grid first
-// zeroes `clears`.
+// zeroes `clears`; workgroup 0 also performs `copy`.
 void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const uint32_t *m_dev, uint32_t capacity,
-                        uint2 *ranges, const Clears &clears, hipStream_t stream);
+                        uint2 *ranges, const Clears &clears, const WordCopy &copy, hipStream_t stream);
 
 // 16x16-tile counts of view v (row-major, tiles_x * tiles_y, zeroed here):
 // the number of visible splats whose pixel bounds overlap each raster tile.

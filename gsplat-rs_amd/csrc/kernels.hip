@@ -607,8 +607,11 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
 __global__ __launch_bounds__(kThreads) void k_tile_ranges(const ViewDev *__restrict__ views,
                                                           const uint32_t *__restrict__ keys,
                                                           const uint32_t *__restrict__ m_dev,
-                                                          uint2 *__restrict__ ranges, Clears clears) {
+                                                          uint2 *__restrict__ ranges, Clears clears,
+                                                          WordCopy copy) {
   clear_ranges(clears);
+  if (blockIdx.x == 0)
+    for (uint32_t i = threadIdx.x; i < copy.words; i += kThreads) copy.dst[i] = copy.src[i];
   const uint32_t m = *m_dev;
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < m; i += gridDim.x * kThreads) {
     const uint32_t k = keys[i];
@@ -1259,9 +1262,9 @@ void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t st) {
 }
 
 void launch_tile_ranges(const ViewDev *views, const uint32_t *pair_keys, const uint32_t *m_dev, uint32_t capacity,
-                        uint2 *ranges, const Clears &clr, hipStream_t st) {
+                        uint2 *ranges, const Clears &clr, const WordCopy &copy, hipStream_t st) {
   const uint32_t grid = std::max<uint32_t>(std::min<uint32_t>(div_up(capacity, kThreads), kMaxGrid), 1);
-  k_tile_ranges<<<grid, kThreads, 0, st>>>(views, pair_keys, m_dev, ranges, clr);
+  k_tile_ranges<<<grid, kThreads, 0, st>>>(views, pair_keys, m_dev, ranges, clr, copy);
   GS_HIP(hipGetLastError());
 }
 

@@ -519,7 +519,11 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   if (pair_lookback) range_clears.add(pair_scratch_.get(), pair_dirty);
   pair_clean_ = pair_lookback;
   if (count_evals_) range_clears.add(info + kEvalsWord, 4);
-  launch_tile_ranges(views_d, pk, m_dev_.get(), pair_cap_, ranges_.get(), range_clears, st);
+  // Every summary word but the evaluation counts is final here: the ranges
+  // kernel hands them to the pinned summary itself.
+  WordCopy summary_copy;
+  if (!count_evals_) summary_copy = WordCopy{info, summary_pinned_, (uint32_t)(kVisibleWord + nviews)};
+  launch_tile_ranges(views_d, pk, m_dev_.get(), pair_cap_, ranges_.get(), range_clears, summary_copy, st);
   // Heaviest binning tiles first when every view bins alike (one layout).
   RasterOrder ro;
   uint32_t rblocks = total_rblocks_;
@@ -553,8 +557,8 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
     GS_HIP(hipMemcpyAsync(host_rgba8_[i], img8_.get() + d.img_off, (size_t)d.fp.width * d.fp.height * 4,
                           hipMemcpyDeviceToHost, st));
   }
-  const size_t summary_words = count_evals_ ? kSummaryWords : kVisibleWord + nviews;
-  GS_HIP(hipMemcpyAsync(summary_pinned_, info, summary_words * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  if (count_evals_)
+    GS_HIP(hipMemcpyAsync(summary_pinned_, info, kSummaryWords * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   if (!captured) {
     GS_HIP(hipEventRecord(ev_[5], st));
     GS_HIP(hipEventRecord(table_ev_[r], st));  // the staging slot may be rewritten
