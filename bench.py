@@ -219,6 +219,101 @@ def bench_batched_64v(steps, warmup):
             "splat_slots": int(st.splat_count), "visible": int(st.visible_count), "tile_pairs": int(st.tile_pairs)}
 
 
+BATCH_WORKLOADS = {
+    "batched-64v": "64 views over 8 scenes (50k-1M splats SH3) at 320x240-1920x1080",
+    "batched-8x1080p": "8 cameras over the 300k SH3 scene at 1920x1080",
+}
+
+
+def batch_views(name):
+    """Views of a batch workload of BATCH_WORKLOADS, scenes uploaded (the objects stay alive with the views)."""
+    if name == "batched-64v":
+        scenes, vs = synth.batched()
+        dev = [A.Scene(s) for s in scenes]
+        return [(dev[v.scene], A.look_at(v.position, v.target), v.width, v.height) for v in vs]
+    if name == "batched-8x1080p":
+        scene, _ = synth.plain_1080p()
+        rng = np.random.default_rng(synth.SEED_BASE + 1 + 500)
+        dev = A.Scene(scene)
+        return [(dev, A.look_at(synth.sphere_point(rng, 4.0)), 1920, 1080) for _ in range(8)]
+    raise SystemExit(f"unknown batch workload {name!r}; one of {sorted(BATCH_WORKLOADS)}")
+
+
+def profile_batch(name, steps, warmup):
+    """`steps` back-to-back renders of one batch workload (for rocprofv3), and one summary line."""
+    views = batch_views(name)
+    b = A.Batch()
+    fps = time_batches(b, views, steps, warmup)
+    print(json.dumps({"workload": name, "views": len(views), "batches": warmup + steps,
+                      "ms_per_batch": round(1e3 * len(views) / fps, 4)}), flush=True)
+
+
+def summarize_batch_profiles(name, trace_dir, fetch_dir, write_dir, out_dir, steps, warmup):
+    """rocprofv3 CSVs of `bench.py --profile-batch NAME` -> profiles/kernels_NAME.md: per kernel and
+    per stage, device time and HBM bytes per batch (FETCH_SIZE and WRITE_SIZE scaled by the copy
+    calibration factors of the plain summary)."""
+    import csv
+    import collections
+
+    out_dir = pathlib.Path(out_dir)
+    out_dir.mkdir(parents=True, exist_ok=True)
+    scale = json.loads(TRAFFIC_FILE.read_text())
+    ff, wf = float(scale["fetch_scale"]), float(scale["write_scale"])
+
+    def rows(d, suffix):
+        return list(csv.DictReader(open(next(pathlib.Path(d).glob(f"*{suffix}")))))
+
+    def batches(rs):
+        return max(sum(_kernel_short(r["Kernel_Name"]).startswith("k_project") for r in rs), 1)
+
+    trace = rows(trace_dir, "kernel_trace.csv")
+    nb = batches(trace)
+    k_us, k_calls, s_us = collections.defaultdict(float), collections.Counter(), collections.defaultdict(float)
+    for stage, r in _stage_sequence(trace):
+        k = _kernel_short(r["Kernel_Name"])
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+        k_us[k] += d
+        k_calls[k] += 1
+        if stage in STAGES:
+            s_us[stage] += d
+
+    def counter(d):
+        rs = rows(d, "counter_collection.csv")
+        per_k, per_s = collections.defaultdict(float), collections.defaultdict(float)
+        for stage, r in _stage_sequence(rs):
+            v = float(r["Counter_Value"]) * 1024.0
+            per_k[_kernel_short(r["Kernel_Name"])] += v
+            if stage in STAGES:
+                per_s[stage] += v
+        return per_k, per_s, batches(rs)
+
+    fk, fs, nf = counter(fetch_dir)
+    wk, ws, nw = counter(write_dir)
+    total = sum(k_us.values())
+    lines = [f"# Kernel trace and HBM traffic: {name} ({BATCH_WORKLOADS[name]})", "",
+             "Commands (from /tmp, TMPDIR=/tmp):", "",
+             f"    rocprofv3 --kernel-trace --stats --output-format csv -d TRACE -o b -- \\",
+             f"        python3 bench.py --profile-batch {name} --steps {steps} --warmup {warmup}",
+             f"    rocprofv3 --pmc FETCH_SIZE --output-format csv -d FETCH -o b -- python3 bench.py --profile-batch {name} "
+             f"--steps {steps} --warmup {warmup}",
+             "    rocprofv3 --pmc WRITE_SIZE (same, into WRITE)", "",
+             f"Summarised with `python3 bench.py --summarize-batch-profiles {name} TRACE FETCH WRITE --steps {steps} "
+             f"--warmup {warmup}`. {nb} batches traced; figures are per batch. FETCH_SIZE is scaled by {ff:.3f} and "
+             f"WRITE_SIZE by {wf:.3f} (the copy calibration of pmc_traffic_plain-1080p-300k-sh3.json).", "",
+             "| kernel | calls per batch | avg us | us per batch | % | fetch MB | write MB |",
+             "|---|---:|---:|---:|---:|---:|---:|"]
+    for k in sorted(k_us, key=lambda k: -k_us[k]):
+        lines.append(f"| {k} | {k_calls[k] / nb:.2f} | {k_us[k] / k_calls[k]:.2f} | {k_us[k] / nb:.1f} | "
+                     f"{100 * k_us[k] / total:.1f} | {fk[k] / nf * ff / 1e6:.1f} | {wk[k] / nw * wf / 1e6:.1f} |")
+    lines += ["", "| stage | us per batch | fetch MB | write MB |", "|---|---:|---:|---:|"]
+    for stage in STAGES:
+        lines.append(f"| {STAGE_KERNEL[stage]} | {s_us[stage] / nb:.1f} | {fs[stage] / nf * ff / 1e6:.1f} | "
+                     f"{ws[stage] / nw * wf / 1e6:.1f} |")
+    path = out_dir / f"kernels_{name}.md"
+    path.write_text("\n".join(lines) + "\n")
+    print(f"wrote {path}", file=sys.stderr)
+
+
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_plain-1080p-300k-sh3.json"
 
 
@@ -261,7 +356,7 @@ def _kernel_short(name):
 # latest stage-opening kernel before it: k_project opens "project", the first
 # radix kernel after it the depth sort, k_tile_counts the binning stage (tile
 # counts, block scan, pair emission, pair radix sort, tile ranges, raster
-# order), k_raster<...> the raster. Sort kernels are shared by the depth and pair sorts, so launch
+# order), k_raster<...> or k_raster_rows2<...> the raster. Sort kernels are shared by the depth and pair sorts, so launch
 # order (not the name) tells them apart.
 STAGES = ["project", "sort", "bin", "raster"]
 STAGE_KERNEL = {"project": "k_project", "sort": "depth radix sort", "bin": "bin(pair sort+emit)",
@@ -273,20 +368,21 @@ def _stage_sequence(rows):
     out, stage = [], None
     for r in sorted(rows, key=lambda r: int(r["Dispatch_Id"])):
         k = _kernel_short(r["Kernel_Name"])
+        raster = k.startswith("k_raster<") or k.startswith("k_raster_rows2<")
         if k.startswith("k_project"):
             stage = "project"
         elif stage == "project" and (k.startswith("k_os_") or k.startswith("k_rts_") or k.startswith("k_scan_")):
             stage = "sort"
         elif k == "k_tile_counts":
             stage = "bin"
-        elif k.startswith("k_raster<true"):
+        elif raster and "<true" in k:
             out.append(("counting", r))
             stage = None
             continue
-        elif k.startswith("k_raster<"):
+        elif raster:
             stage = "raster"
         out.append((stage, r))
-        if k.startswith("k_raster<"):
+        if raster:
             stage = None
     return out
 
@@ -496,11 +592,20 @@ def main():
     ap.add_argument("--summarize-profiles", nargs=5, metavar=("TRACE", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="write profiles/ summaries from rocprofv3 output directories and exit")
     ap.add_argument("--profiles-out", default=str(ROOT / "profiles"), help="folder for --summarize-profiles")
+    ap.add_argument("--profile-batch", default=None, metavar="WORKLOAD",
+                    help=f"only render --steps batches of one workload ({', '.join(sorted(BATCH_WORKLOADS))}), "
+                         "for rocprofv3")
+    ap.add_argument("--summarize-batch-profiles", nargs=4, metavar=("WORKLOAD", "TRACE", "FETCH", "WRITE"),
+                    help="write profiles/kernels_WORKLOAD.md from rocprofv3 runs of --profile-batch and exit")
     ap.add_argument("--artifact-dir", default=None,
                     help="also write the plain workload as a gsplat-benchmark/v1 run directory (120 frames)")
     args = ap.parse_args()
     if args.summarize_profiles:
         summarize_profiles(*args.summarize_profiles, frames=0, out_dir=args.profiles_out)
+        return
+    if args.summarize_batch_profiles:
+        summarize_batch_profiles(*args.summarize_batch_profiles, out_dir=args.profiles_out, steps=args.steps,
+                                 warmup=args.warmup)
         return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -511,6 +616,9 @@ def main():
     if not A.device_available():
         print("no gfx950 device visible", file=sys.stderr)
         sys.exit(1)
+    if args.profile_batch:
+        profile_batch(args.profile_batch, args.steps, args.warmup)
+        return
 
     scene, view = synth.plain_1080p()
     cam = A.look_at(view.position, view.target)
