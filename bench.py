@@ -222,6 +222,7 @@ def bench_batched_64v(steps, warmup):
 BATCH_WORKLOADS = {
     "batched-64v": "64 views over 8 scenes (50k-1M splats SH3) at 320x240-1920x1080",
     "batched-8x1080p": "8 cameras over the 300k SH3 scene at 1920x1080",
+    "served-24": "the first 24 served requests (16 scenes, 10M splats SH3) as one batch, binned as the server bins",
 }
 
 
@@ -236,12 +237,18 @@ def batch_views(name):
         rng = np.random.default_rng(synth.SEED_BASE + 1 + 500)
         dev = A.Scene(scene)
         return [(dev, A.look_at(synth.sphere_point(rng, 4.0)), 1920, 1080) for _ in range(8)]
+    if name == "served-24":
+        scenes, reqs = synth.served(requests=24)
+        dev = [A.Scene(s) for s in scenes]
+        return [(dev[v.scene], A.look_at(v.position, v.target), v.width, v.height) for (_, v) in reqs]
     raise SystemExit(f"unknown batch workload {name!r}; one of {sorted(BATCH_WORKLOADS)}")
 
 
 def profile_batch(name, steps, warmup):
     """`steps` back-to-back renders of one batch workload (for rocprofv3), and one summary line."""
     views = batch_views(name)
+    if name.startswith("served"):
+        os.environ["GSPLAT_BIN_MAX_CTILES"] = "512"  # the server's binning cap (read when a pipeline is made)
     b = A.Batch()
     fps = time_batches(b, views, steps, warmup)
     print(json.dumps({"workload": name, "views": len(views), "batches": warmup + steps,
