@@ -613,11 +613,35 @@ __global__ __launch_bounds__(kThreads) void k_tile_ranges(const ViewDev *__restr
   if (blockIdx.x == 0)
     for (uint32_t i = threadIdx.x; i < copy.words; i += kThreads) copy.dst[i] = copy.src[i];
   const uint32_t m = *m_dev;
-  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < m; i += gridDim.x * kThreads) {
+  // Four keys per thread and step (one 16-byte load); the view table is read
+  // only at a run boundary, once per binning tile.
+  const uint32_t quads = m / 4;
+  for (uint32_t q = blockIdx.x * kThreads + threadIdx.x; q < quads; q += gridDim.x * kThreads) {
+    const uint4 k4 = reinterpret_cast<const uint4 *>(keys)[q];
+    const uint32_t i = 4 * q;
+    const uint32_t prev = i == 0 ? ~k4.x : keys[i - 1];
+    const uint32_t next = i + 4 == m ? ~k4.w : keys[i + 4];
+    const uint32_t k[4] = {k4.x, k4.y, k4.z, k4.w};
+    const uint32_t before[4] = {prev, k4.x, k4.y, k4.z};
+    const uint32_t after[4] = {k4.y, k4.z, k4.w, next};
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const bool first = before[u] != k[u], last = after[u] != k[u];
+      if (first || last) {
+        const uint32_t t = views[k[u] >> kPairViewShift].ctile_off + (k[u] & kPairTileMask);
+        if (first) ranges[t].x = i + u;
+        if (last) ranges[t].y = i + u + 1;
+      }
+    }
+  }
+  for (uint32_t i = 4 * quads + blockIdx.x * kThreads + threadIdx.x; i < m; i += gridDim.x * kThreads) {
     const uint32_t k = keys[i];
-    const uint32_t t = views[k >> kPairViewShift].ctile_off + (k & kPairTileMask);
-    if (i == 0 || keys[i - 1] != k) ranges[t].x = i;
-    if (i == m - 1 || keys[i + 1] != k) ranges[t].y = i + 1;
+    const bool first = i == 0 || keys[i - 1] != k, last = i == m - 1 || keys[i + 1] != k;
+    if (first || last) {
+      const uint32_t t = views[k >> kPairViewShift].ctile_off + (k & kPairTileMask);
+      if (first) ranges[t].x = i;
+      if (last) ranges[t].y = i + 1;
+    }
   }
 }
 
