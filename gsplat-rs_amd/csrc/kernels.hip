@@ -509,30 +509,53 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const ViewDev *__restr
 // One workgroup: exclusive scan of the block totals in place (each block's
 // first pair index), the binning pair total M (info[1..2]), its overflow
 // flag (info[3]) and *m_out = min(M, capacity), and the sum of block_pairs
-// into *raster_pairs.
-__global__ __launch_bounds__(kThreads) void k_block_scan(uint32_t *__restrict__ block_sums, uint32_t blocks,
-                                                         uint32_t capacity, uint32_t *__restrict__ m_out,
-                                                         uint32_t *__restrict__ info,
-                                                         const uint32_t *__restrict__ block_pairs,
-                                                         unsigned long long *__restrict__ raster_pairs) {
-  __shared__ uint32_t lds4[kWaves];
-  __shared__ unsigned long long wsum[kWaves];
-  unsigned long long carry = 0, pairs = 0;
-  for (uint32_t base = 0; base < blocks; base += kThreads) {
-    const uint32_t p = base + threadIdx.x;
-    const uint32_t x = p < blocks ? block_sums[p] : 0u;
-    pairs += p < blocks ? block_pairs[p] : 0u;
-    uint32_t tot;
-    const uint32_t ex = block_exclusive_scan(x, lds4, &tot);
-    if (p < blocks) block_sums[p] = (uint32_t)(carry + ex);
-    carry += tot;
+// into *raster_pairs. Thread t of the 1024 owns the contiguous block range
+// [t * per, (t + 1) * per): it sums its blocks, the workgroup scans the
+// thread sums (64-bit), and each thread writes its blocks' exclusive prefix.
+constexpr uint32_t kBlockScanThreads = 1024;
+
+__global__ __launch_bounds__(kBlockScanThreads) void k_block_scan(uint32_t *__restrict__ block_sums, uint32_t blocks,
+                                                                  uint32_t capacity, uint32_t *__restrict__ m_out,
+                                                                  uint32_t *__restrict__ info,
+                                                                  const uint32_t *__restrict__ block_pairs,
+                                                                  unsigned long long *__restrict__ raster_pairs) {
+  constexpr uint32_t W = kBlockScanThreads / kWave;
+  __shared__ unsigned long long wsum[W], wpairs[W];
+  const uint32_t t = threadIdx.x, lane = __lane_id(), w = t / kWave;
+  const uint32_t per = (blocks + kBlockScanThreads - 1) / kBlockScanThreads;
+  const uint32_t b0 = min(t * per, blocks), b1 = min(b0 + per, blocks);
+  unsigned long long sum = 0, pairs = 0;
+#pragma unroll 8
+  for (uint32_t i = b0; i < b1; ++i) {
+    sum += block_sums[i];
+    pairs += block_pairs[i];
+  }
+  unsigned long long inc = sum;
+#pragma unroll
+  for (uint32_t d = 1; d < kWave; d <<= 1) {
+    const unsigned long long y = __shfl_up(inc, d, kWave);
+    if (lane >= d) inc += y;
   }
   pairs = wave_sum_u64(pairs);
-  if (__lane_id() == 0) wsum[threadIdx.x / kWave] = pairs;
+  if (lane == kWave - 1) wsum[w] = inc;
+  if (lane == 0) wpairs[w] = pairs;
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  *raster_pairs = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-  const uint64_t m = carry;
+  unsigned long long run = inc - sum, total = 0, total_pairs = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < W; ++q) {
+    run += q < w ? wsum[q] : 0ull;
+    total += wsum[q];
+    total_pairs += wpairs[q];
+  }
+#pragma unroll 8
+  for (uint32_t i = b0; i < b1; ++i) {
+    const uint32_t x = block_sums[i];
+    block_sums[i] = (uint32_t)run;
+    run += x;
+  }
+  if (t != 0) return;
+  *raster_pairs = total_pairs;
+  const uint64_t m = total;
   *m_out = m > capacity ? capacity : (uint32_t)m;
   info[1] = (uint32_t)m;
   info[2] = (uint32_t)(m >> 32);
@@ -1253,7 +1276,7 @@ void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_
 
 void launch_block_scan(uint32_t *block_sums, uint32_t blocks, uint32_t capacity, uint32_t *m_out, uint32_t *info,
                        const uint32_t *block_pairs, unsigned long long *raster_pairs, hipStream_t st) {
-  k_block_scan<<<1, kThreads, 0, st>>>(block_sums, blocks, capacity, m_out, info, block_pairs, raster_pairs);
+  k_block_scan<<<1, kBlockScanThreads, 0, st>>>(block_sums, blocks, capacity, m_out, info, block_pairs, raster_pairs);
   GS_HIP(hipGetLastError());
 }
 
