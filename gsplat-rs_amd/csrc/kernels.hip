@@ -439,7 +439,11 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const ViewDev *__restr
   __shared__ uint32_t wfine[kWaves];
   clear_ranges(clears);
   const uint32_t t = threadIdx.x, w = t / kWave, lane = __lane_id();
-  const uint32_t b = blockIdx.x;
+  // Workgroups reach the XCDs round robin (grid a multiple of kXcds): each
+  // XCD takes one contiguous run of blocks in order, so the rect gathers of
+  // a view's blocks meet in one L2 instead of in all eight.
+  const uint32_t b = (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
+  if (b >= total / kCountBlock) return;
   const uint32_t slot_lo = total - kCountBlock * (b + 1);  // lowest slot of the block
   const uint32_t v = find_view(seg_start, nviews, slot_lo);
   const ViewDev &vw = views[v];
@@ -1268,7 +1272,8 @@ void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_
                         hipStream_t st) {
   if (total_slots % kCountBlock) throw Error(Code::Internal, "slot count not a multiple of the count block");
   if (total_slots == 0) return;
-  k_tile_counts<<<total_slots / kCountBlock, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, skeys,
+  const uint32_t grid = (total_slots / kCountBlock + kXcds - 1) / kXcds * kXcds;
+  k_tile_counts<<<grid, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, skeys,
                                                                   svals, rect, counts_rev, offsets_rev, visible,
                                                                   block_sums, block_pairs, rect_lo, rect_hi, clr);
   GS_HIP(hipGetLastError());
