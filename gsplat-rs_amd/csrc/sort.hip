@@ -419,8 +419,8 @@ __global__ __launch_bounds__(T) void k_rts_up(const uint32_t *__restrict__ kin, 
                                               uint32_t *__restrict__ entries, uint32_t *__restrict__ scan_state,
                                               const uint32_t *__restrict__ seg_start, uint32_t segs) {
   constexpr uint32_t D = 1u << DB;
-  __shared__ uint32_t h[WAVES][D];
-  const uint32_t t = threadIdx.x, w = t / 64;
+  __shared__ uint32_t h[D];  // one table for the workgroup (wave_count folds equal digits)
+  const uint32_t t = threadIdx.x;
   const uint32_t n = load_count(n_dev, n_cap);
   const uint32_t tiles = (n + TILE - 1) / TILE;
   if (blockIdx.x == 0) {
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(T) void k_rts_up(const uint32_t *__restrict__ kin, 
   }
   const uint32_t *src = from_value ? vin : kin;
   for (uint32_t tile = xcd_first_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x) {
-    clear_whist<DB>(h);
+    for (uint32_t i = t; i < D; i += T) h[i] = 0;
     __syncthreads();
     const size_t base = (size_t)tile * TILE;
     const uint32_t count = (uint32_t)min((size_t)TILE, (size_t)n - base);
@@ -443,14 +443,9 @@ __global__ __launch_bounds__(T) void k_rts_up(const uint32_t *__restrict__ kin, 
     }
 #pragma unroll
     for (uint32_t r = 0; r < IPT; ++r)
-      if (r * T + t < count) wave_count(h[w], ((x[r] ^ flip) >> shift) & (D - 1u));
+      if (r * T + t < count) wave_count(h, ((x[r] ^ flip) >> shift) & (D - 1u));
     __syncthreads();
-    if (t < D) {
-      uint32_t c = 0;
-#pragma unroll
-      for (uint32_t q = 0; q < WAVES; ++q) c += h[q][t];
-      counts[count_index<DB>(tile, tiles, seg_start, segs)(t, tile)] = c;
-    }
+    if (t < D) counts[count_index<DB>(tile, tiles, seg_start, segs)(t, tile)] = h[t];
     __syncthreads();
   }
 }
