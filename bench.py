@@ -321,6 +321,36 @@ def summarize_batch_profiles(name, trace_dir, fetch_dir, write_dir, out_dir, ste
     print(f"wrote {path}", file=sys.stderr)
 
 
+def summarize_counters(pmc_dir, out_dir, counters):
+    """rocprofv3 --pmc CSV of the plain workload -> profiles/sq_counters_plain-1080p-300k-sh3.md: the
+    per-dispatch average of each counter for every kernel of the frame."""
+    import csv
+    import collections
+
+    path = next(pathlib.Path(pmc_dir).glob("*counter_collection.csv"))
+    acc = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(path)):
+        acc[_kernel_short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    names = sorted({c for d in acc.values() for c in d})
+    lines = ["# SQ counters: plain-1080p-300k-sh3 (300,000 splats SH3, 1920x1080)", "",
+             "Command (from /tmp, TMPDIR=/tmp):", "",
+             f"    rocprofv3 --pmc {counters} --output-format csv -d DIR -o plain -- \\",
+             "        python3 bench.py --plain-only --steps 8 --warmup 2 --no-cpu-baseline",
+             "", f"Summarised with `python3 bench.py --summarize-counters DIR`. Per-dispatch averages.", "",
+             "| kernel | dispatches | " + " | ".join(names) + " |", "|---|---:|" + "---:|" * len(names)]
+    for k in sorted(acc, key=lambda k: -sum(acc[k].get("SQ_WAVE_CYCLES", [0]))):
+        d = acc[k]
+        n = max(len(v) for v in d.values())
+        lines.append(f"| {k} | {n} | " + " | ".join(f"{sum(d[c]) / len(d[c]):.0f}" if d.get(c) else "" for c in names)
+                     + " |")
+    out = pathlib.Path(out_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    (out / "sq_counters_plain-1080p-300k-sh3.md").write_text("\n".join(lines) + "\n")
+
+
+SQ_COUNTERS = ("SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_LDS SQ_ACTIVE_INST_VALU SQ_WAIT_INST_LDS "
+               "SQ_WAVE_CYCLES SQ_BUSY_CYCLES")
+
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_plain-1080p-300k-sh3.json"
 
 
@@ -599,6 +629,9 @@ def main():
     ap.add_argument("--summarize-profiles", nargs=5, metavar=("TRACE", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="write profiles/ summaries from rocprofv3 output directories and exit")
     ap.add_argument("--profiles-out", default=str(ROOT / "profiles"), help="folder for --summarize-profiles")
+    ap.add_argument("--summarize-counters", default=None, metavar="DIR",
+                    help=f"write profiles/sq_counters_plain-1080p-300k-sh3.md from a rocprofv3 --pmc run of the "
+                         f"plain workload with the counters {SQ_COUNTERS!r} and exit")
     ap.add_argument("--profile-batch", default=None, metavar="WORKLOAD",
                     help=f"only render --steps batches of one workload ({', '.join(sorted(BATCH_WORKLOADS))}), "
                          "for rocprofv3")
@@ -609,6 +642,9 @@ def main():
     args = ap.parse_args()
     if args.summarize_profiles:
         summarize_profiles(*args.summarize_profiles, frames=0, out_dir=args.profiles_out)
+        return
+    if args.summarize_counters:
+        summarize_counters(args.summarize_counters, args.profiles_out, SQ_COUNTERS)
         return
     if args.summarize_batch_profiles:
         summarize_batch_profiles(*args.summarize_batch_profiles, out_dir=args.profiles_out, steps=args.steps,
