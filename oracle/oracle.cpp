@@ -19,6 +19,18 @@
 #include <omp.h>
 #endif
 
+// Planted mistakes for the pinning tests' self-check: tests/
+// test_oracle_mutations.py builds this file with -DORACLE_MUTATION=k into a
+// temporary directory and expects the pins of tests/test_oracle_projection.py
+// to fail. Every normal build has ORACLE_MUTATION 0 (no mistake).
+//   1 sign of j02 flipped            5 NDC-box cull on |axis_u.x| only
+//   2 aspect dropped from lim_x      6 blur term left out
+//   3 64x anisotropy cap removed     7 tile rect from the quad extent
+//   4 upper radius clamp removed     8 lower radius clamp removed
+#ifndef ORACLE_MUTATION
+#define ORACLE_MUTATION 0
+#endif
+
 namespace {
 
 struct Q {
@@ -144,12 +156,18 @@ FrameConsts frame_consts(const View &v, uint32_t width, uint32_t height) {
   c.fx = c.f / c.aspect;
   c.fy = c.f;
   c.lim_x = 1.3f * (tan_half * c.aspect);
+#if ORACLE_MUTATION == 2
+  c.lim_x = 1.3f * tan_half;
+#endif
   c.lim_y = 1.3f * tan_half;
   float px_x = 2.0f / std::max((float)width, 1.0f);
   float px_y = 2.0f / std::max((float)height, 1.0f);
   float bx = 0.3f * px_x, by = 0.3f * px_y;
   c.blur_x = bx * bx;
   c.blur_y = by * by;
+#if ORACLE_MUTATION == 6
+  c.blur_x = c.blur_y = 0.0f;
+#endif
   return c;
 }
 
@@ -189,6 +207,9 @@ Proj project_splat(const View &v, const FrameConsts &k, const float *p, const fl
   float inv_z2 = inv_z * inv_z;
   float j00 = k.fx * inv_z;
   float j02 = -k.fx * x_clamped * inv_z2;
+#if ORACLE_MUTATION == 1
+  j02 = -j02;
+#endif
   float j11 = k.fy * inv_z;
   float j12 = -k.fy * y_clamped * inv_z2;
 
@@ -223,12 +244,24 @@ Proj project_splat(const View &v, const FrameConsts &k, const float *p, const fl
 
   float major_r = std::min(std::max(std::sqrt(major) * 3.0f, 1e-4f), 2.0f);
   float minor_r = std::min(std::max(std::sqrt(minor) * 3.0f, 1e-4f), 2.0f);
+#if ORACLE_MUTATION == 4
+  major_r = std::max(std::sqrt(major) * 3.0f, 1e-4f);
+  minor_r = std::max(std::sqrt(minor) * 3.0f, 1e-4f);
+#elif ORACLE_MUTATION == 8
+  major_r = std::min(std::sqrt(major) * 3.0f, 2.0f);
+  minor_r = std::min(std::sqrt(minor) * 3.0f, 2.0f);
+#endif
+#if ORACLE_MUTATION != 3
   major_r = std::min(major_r, minor_r * 64.0f);
+#endif
   float ux = ud_x * major_r, uy = ud_y * major_r;
   float vx = vd_x * minor_r, vy = vd_y * minor_r;
 
   float extent_x = std::fabs(ux) + std::fabs(vx);
   float extent_y = std::fabs(uy) + std::fabs(vy);
+#if ORACLE_MUTATION == 5
+  extent_x = std::fabs(ux);
+#endif
   if (x_ndc + extent_x < -1.0f || x_ndc - extent_x > 1.0f || y_ndc + extent_y < -1.0f ||
       y_ndc - extent_y > 1.0f)
     return out;
@@ -346,6 +379,10 @@ bool tile_rect(const Proj &p, float cut, uint32_t w, uint32_t h, uint32_t tile, 
   float sc = std::sqrt(cut);
   float ex = sc * std::sqrt(p.ux * p.ux + p.vx * p.vx);
   float ey = sc * std::sqrt(p.uy * p.uy + p.vy * p.vy);
+#if ORACLE_MUTATION == 7
+  ex = sc * (std::fabs(p.ux) + std::fabs(p.vx));
+  ey = sc * (std::fabs(p.uy) + std::fabs(p.vy));
+#endif
   float hw = 0.5f * (float)w, hh = 0.5f * (float)h;
   float cxp = (p.cx + 1.0f) * hw;
   float cyp = (1.0f - p.cy) * hh;

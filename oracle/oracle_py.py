@@ -13,7 +13,9 @@ import pathlib
 import numpy as np
 
 _HERE = pathlib.Path(__file__).resolve().parent
-_LIB_PATH = _HERE / "liboracle.so"
+# GSPLAT_ORACLE_LIB names another build of oracle.cpp (the mutation self-check
+# of the pinning tests loads builds with a planted mistake this way).
+_LIB_PATH = pathlib.Path(os.environ.get("GSPLAT_ORACLE_LIB") or (_HERE / "liboracle.so"))
 
 
 class OracleCamera(ctypes.Structure):
