@@ -1023,13 +1023,12 @@ void *gsplat_host_alloc(uint64_t bytes) {
   }
 }
 
-void gsplat_host_free(void *ptr) {
-  try {
+int32_t gsplat_host_free(void *ptr) {
+  static const char *op = "gsplat_host_free";
+  return guard(op, [&]() -> int32_t {
     gs::host_free(ptr);
-  } catch (const gs::Error &e) {
-    set_last(std::string("gsplat_host_free: ") + e.what());
-  } catch (...) {
-  }
+    return ok();
+  });
 }
 
 }  // extern "C"

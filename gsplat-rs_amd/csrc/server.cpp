@@ -12,8 +12,23 @@ namespace gs {
 
 namespace {
 
+// Every host_alloc block: its size and the number of queued or running
+// requests whose target lies in it (host_free refuses a block in use).
+struct PinnedBlock {
+  size_t bytes = 0;
+  uint32_t users = 0;
+};
 std::mutex g_pinned_m;
-std::map<uintptr_t, size_t> g_pinned;  // base -> bytes of every host_alloc block
+std::map<uintptr_t, PinnedBlock> g_pinned;  // base -> block
+
+// Block holding [p, p + len), or end() (caller holds g_pinned_m).
+std::map<uintptr_t, PinnedBlock>::iterator find_block(const void *p, size_t len) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  auto it = g_pinned.upper_bound(a);
+  if (it == g_pinned.begin()) return g_pinned.end();
+  --it;
+  return (a >= it->first && a + len <= it->first + it->second.bytes) ? it : g_pinned.end();
+}
 
 constexpr uint32_t kMaxDimension = 16384;
 
@@ -33,7 +48,7 @@ void *host_alloc(size_t bytes) {
   void *p = nullptr;
   GS_HIP(hipHostMalloc(&p, std::max<size_t>(bytes, 1), hipHostMallocDefault));
   std::lock_guard<std::mutex> g(g_pinned_m);
-  g_pinned[reinterpret_cast<uintptr_t>(p)] = std::max<size_t>(bytes, 1);
+  g_pinned[reinterpret_cast<uintptr_t>(p)] = PinnedBlock{std::max<size_t>(bytes, 1), 0};
   return p;
 }
 
@@ -43,18 +58,31 @@ void host_free(void *p) {
     std::lock_guard<std::mutex> g(g_pinned_m);
     auto it = g_pinned.find(reinterpret_cast<uintptr_t>(p));
     if (it == g_pinned.end()) throw Error(Code::InvalidArgument, "pointer was not returned by gsplat_host_alloc");
+    if (it->second.users)
+      throw Error(Code::InvalidArgument, "block is the target of " + std::to_string(it->second.users) +
+                                             " queued or running request(s); wait for them first");
     g_pinned.erase(it);
   }
   (void)hipHostFree(p);
 }
 
 bool host_is_pinned(const void *p, size_t len) {
-  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
   std::lock_guard<std::mutex> g(g_pinned_m);
-  auto it = g_pinned.upper_bound(a);
-  if (it == g_pinned.begin()) return false;
-  --it;
-  return a >= it->first && a + len <= it->first + it->second;
+  return find_block(p, len) != g_pinned.end();
+}
+
+bool host_hold(const void *p, size_t len) {
+  std::lock_guard<std::mutex> g(g_pinned_m);
+  auto it = find_block(p, len);
+  if (it == g_pinned.end()) return false;
+  ++it->second.users;
+  return true;
+}
+
+void host_release(const void *p) {
+  std::lock_guard<std::mutex> g(g_pinned_m);
+  auto it = find_block(p, 1);
+  if (it != g_pinned.end() && it->second.users) --it->second.users;
 }
 
 Server::Server(const ServerConfig &cfg) : cfg_(cfg) {
@@ -97,9 +125,29 @@ Server::~Server() {
   }
 }
 
-uint64_t Server::slot_budget() const {
-  const uint64_t free_b = cfg_.memory_budget > resident_bytes_ ? cfg_.memory_budget - resident_bytes_ : 0;
+uint64_t Server::held_bytes() {
+  uint64_t b = resident_bytes_;
+  for (auto it = retired_.begin(); it != retired_.end();) {
+    if (it->first.expired()) {
+      it = retired_.erase(it);
+    } else {
+      b += it->second;
+      ++it;
+    }
+  }
+  return b;
+}
+
+uint64_t Server::slot_budget() {
+  const uint64_t held = held_bytes();
+  const uint64_t free_b = cfg_.memory_budget > held ? cfg_.memory_budget - held : 0;
   return free_b / cfg_.pipelines;
+}
+
+void Server::finish(Request &r) {
+  if (r.out_pinned) host_release(r.out);
+  r.out_pinned = false;
+  r.done = true;
 }
 
 uint32_t Server::add_scene(std::shared_ptr<DeviceScene> scene) {
@@ -109,10 +157,11 @@ uint32_t Server::add_scene(std::shared_ptr<DeviceScene> scene) {
   const uint64_t one_view = frame_working_bytes(round_segment(scene->count), 1920ull * 1080, 120ull * 68,
                                                 initial_pairs(scene->count));
   std::lock_guard<std::mutex> g(m_);
-  const uint64_t need = resident_bytes_ + scene->bytes() + cfg_.pipelines * one_view;
+  const uint64_t held = held_bytes();
+  const uint64_t need = held + scene->bytes() + cfg_.pipelines * one_view;
   if (need > cfg_.memory_budget)
     throw Error(Code::Unsupported, "scene of " + std::to_string(scene->bytes()) + " bytes does not fit the memory budget (" +
-                                       std::to_string(resident_bytes_) + " resident, " +
+                                       std::to_string(held) + " resident, " +
                                        std::to_string(cfg_.memory_budget) + " budget)");
   const uint32_t id = next_scene_++;
   resident_bytes_ += scene->bytes();
@@ -124,8 +173,12 @@ void Server::remove_scene(uint32_t id) {
   std::lock_guard<std::mutex> g(m_);
   auto it = scenes_.find(id);
   if (it == scenes_.end()) throw Error(Code::InvalidArgument, "unknown scene id " + std::to_string(id));
+  // Queued and running requests (and the pipelines' last frames) keep their
+  // own reference: the scene's HBM counts against the budget until the last
+  // of them lets go.
   resident_bytes_ -= it->second->bytes();
-  scenes_.erase(it);  // queued requests keep their own reference
+  retired_.emplace_back(std::weak_ptr<DeviceScene>(it->second), it->second->bytes());
+  scenes_.erase(it);
 }
 
 uint64_t Server::submit(uint32_t scene_id, const GsplatCamera &camera, uint32_t width, uint32_t height,
@@ -140,7 +193,6 @@ uint64_t Server::submit(uint32_t scene_id, const GsplatCamera &camera, uint32_t 
   r->fp = make_frame_params(camera.position, camera.rotation_xyzw, camera.vertical_fov_radians, camera.near_plane,
                             camera.far_plane, width, height);
   r->out = out;
-  r->out_pinned = out && host_is_pinned(out, need);
   std::lock_guard<std::mutex> g(m_);
   if (stop_) throw Error(Code::Internal, "server is shutting down");
   auto it = scenes_.find(scene_id);
@@ -150,6 +202,8 @@ uint64_t Server::submit(uint32_t scene_id, const GsplatCamera &camera, uint32_t 
     throw Error(Code::Unsupported, "request queue full (" + std::to_string(cfg_.max_queue) + ")");
   }
   r->scene = it->second;
+  // A pinned target is held (host_free refuses it) until the request ends.
+  r->out_pinned = out && host_hold(out, need);
   r->ticket = next_ticket_++;
   r->arrival = std::chrono::steady_clock::now();
   tickets_[r->ticket] = r;
@@ -172,7 +226,7 @@ RequestResult Server::wait(uint64_t ticket) {
 ServerStats Server::stats() {
   std::lock_guard<std::mutex> g(m_);
   ServerStats s = stats_;
-  s.resident_scene_bytes = resident_bytes_;
+  s.resident_scene_bytes = held_bytes();
   s.memory_budget = cfg_.memory_budget;
   s.scenes = (uint32_t)scenes_.size();
   s.queued = (uint32_t)queue_.size();
@@ -214,7 +268,7 @@ void Server::submit_loop() {
         queue_.pop_front();
         r->result.status = (int32_t)Code::Unsupported;
         r->result.error = "request exceeds the memory budget";
-        r->done = true;
+        finish(*r);
         ++stats_.rejected;
         done_cv_.notify_all();
         continue;
@@ -256,7 +310,7 @@ void Server::submit_loop() {
       for (auto &r : batch) {
         r->result.status = code;
         r->result.error = error;
-        r->done = true;
+        finish(*r);
       }
       slot->batch.clear();
       slot->busy = false;
@@ -271,7 +325,7 @@ void Server::submit_loop() {
   for (auto &r : queue_) {
     r->result.status = (int32_t)Code::Internal;
     r->result.error = "server shut down before the request ran";
-    r->done = true;
+    finish(*r);
   }
   queue_.clear();
   done_cv_.notify_all();
@@ -314,7 +368,7 @@ void Server::complete_loop(Slot *slot) {
       r.result.queue_ms = ms_between(r.arrival, r.start);
       r.result.device_ms = t.total_ms;
       r.result.total_ms = ms_between(r.arrival, now);
-      r.done = true;
+      finish(r);
     }
     stats_.completed += batch.size();
     slot->batch.clear();

@@ -59,11 +59,16 @@ struct ServerStats {
 };
 
 // Pinned host allocations handed out to callers for server outputs; the
-// server queues the device-to-host copy of a request inside its batch when
-// the request's target lies in one of them.
+// raster writes a request's RGBA8 pixels straight into its target when the
+// target lies in one of them. A block stays held while a queued or running
+// request targets it: host_free refuses it (InvalidArgument) until then.
 void *host_alloc(size_t bytes);
 void host_free(void *p);
 bool host_is_pinned(const void *p, size_t len);
+// Holds the block containing [p, p + len) for one request (false, holding
+// nothing, when no block contains it); host_release lets one hold go.
+bool host_hold(const void *p, size_t len);
+void host_release(const void *p);
 
 class Server {
  public:
@@ -104,8 +109,11 @@ class Server {
 
   void submit_loop();
   void complete_loop(Slot *slot);
-  uint64_t view_bytes(const Request &r) const;
-  uint64_t slot_budget() const;  // working-set share of one pipeline (under m_)
+  uint64_t slot_budget();  // working-set share of one pipeline (under m_)
+  // Resident scene bytes plus removed scenes still referenced (under m_).
+  uint64_t held_bytes();
+  // Marks the request done and lets its pinned target go (under m_).
+  void finish(Request &r);
 
   ServerConfig cfg_;
   std::mutex m_;
@@ -116,6 +124,9 @@ class Server {
   uint32_t next_scene_ = 1;
   uint64_t next_ticket_ = 1;
   uint64_t resident_bytes_ = 0;
+  // Removed scenes and their bytes: counted against the budget while any
+  // request or pipeline still holds them.
+  std::vector<std::pair<std::weak_ptr<DeviceScene>, uint64_t>> retired_;
   ServerStats stats_{};
   bool stop_ = false;
   std::vector<std::unique_ptr<Slot>> slots_;

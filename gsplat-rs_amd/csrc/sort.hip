@@ -248,13 +248,17 @@ __device__ __forceinline__ uint32_t rank_tile(const uint32_t *__restrict__ kin, 
 
 // Stages the ranked tile in LDS in digit order (lstart[d] = start of digit d
 // in the tile) and writes it out coalesced; digit d's items go to
-// dst[d] + (position among the tile's digit-d items).
+// dst[d] + (position among the tile's digit-d items). Positions at or past
+// the element count n are not written: with correct offsets every position
+// is below n (the offsets are a permutation), and offsets broken by a fault
+// elsewhere then lose items (a parity failure) instead of writing past the
+// buffers.
 template <uint32_t DB>
 __device__ __forceinline__ void scatter_tile(const uint32_t (&key)[IPT], const uint32_t (&val)[IPT],
                                              const uint32_t (&rank)[IPT], uint32_t (*whist)[1u << DB],
                                              const uint32_t *lstart, const uint32_t *dst, uint32_t count,
                                              uint32_t shift, uint32_t flip, uint32_t from_value, uint32_t *skeys,
-                                             uint32_t *svals, uint32_t *__restrict__ kout,
+                                             uint32_t *svals, uint32_t n, uint32_t *__restrict__ kout,
                                              uint32_t *__restrict__ vout) {
   const uint32_t t = threadIdx.x, w = t / 64;
 #pragma unroll
@@ -262,8 +266,10 @@ __device__ __forceinline__ void scatter_tile(const uint32_t (&key)[IPT], const u
     if (rank[r] != 0xFFFFFFFFu) {
       const uint32_t d = digit<DB>(key[r], val[r], shift, flip, from_value);
       const uint32_t pos = lstart[d] + whist[w][d] + rank[r];
-      skeys[pos] = key[r];
-      svals[pos] = val[r];
+      if (pos < TILE) {  // always, for ranks of one tile; bounds the LDS stage regardless
+        skeys[pos] = key[r];
+        svals[pos] = val[r];
+      }
     }
   }
   __syncthreads();
@@ -271,8 +277,10 @@ __device__ __forceinline__ void scatter_tile(const uint32_t (&key)[IPT], const u
     const uint32_t k = skeys[j], v = svals[j];
     const uint32_t d = digit<DB>(k, v, shift, flip, from_value);
     const size_t pos = (size_t)dst[d] + (j - lstart[d]);
-    kout[pos] = k;
-    vout[pos] = v;
+    if (pos < n) {
+      kout[pos] = k;
+      vout[pos] = v;
+    }
   }
 }
 
@@ -369,7 +377,8 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
       lstart[t] = ls;
     }
     __syncthreads();
-    scatter_tile<DB>(key, val, rank, whist, lstart, gofs, count, shift, flip, from_value, skeys, svals, kout, vout);
+    scatter_tile<DB>(key, val, rank, whist, lstart, gofs, count, shift, flip, from_value, skeys, svals, n, kout,
+                     vout);
     __syncthreads();
   }
 }
@@ -557,7 +566,8 @@ __global__ __launch_bounds__(T) void k_rts_down(const uint32_t *__restrict__ kin
     const uint32_t ls = block_exclusive_scan(run, lds4);
     if (t < D) lstart[t] = ls;
     __syncthreads();
-    scatter_tile<DB>(key, val, rank, whist, lstart, dst, count, shift, flip, from_value, skeys, svals, kout, vout);
+    scatter_tile<DB>(key, val, rank, whist, lstart, dst, count, shift, flip, from_value, skeys, svals, n, kout,
+                     vout);
     __syncthreads();
   }
 }

@@ -263,7 +263,7 @@ SIGNATURES = {
     "gsplat_server_wait": (I32, [P, U64, ctypes.POINTER(GsplatRequestResult)]),
     "gsplat_server_get_stats": (I32, [P, ctypes.POINTER(GsplatServerStats)]),
     "gsplat_host_alloc": (P, [U64]),
-    "gsplat_host_free": (None, [P]),
+    "gsplat_host_free": (I32, [P]),
 }
 
 _lib = None
@@ -588,9 +588,10 @@ class PinnedImage:
         self.array = np.ctypeslib.as_array(buf).reshape(height, width, 4)
 
     def close(self):
+        """Frees the block; raises GsplatError (and keeps it) while a request still targets it."""
         if self.ptr:
+            check(lib().gsplat_host_free(self.ptr))
             self.array = None
-            lib().gsplat_host_free(self.ptr)
             self.ptr = None
 
     def __del__(self):
@@ -611,11 +612,15 @@ class Server:
         h = P()
         check(lib().gsplat_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
+        # ticket -> target buffer: a target stays referenced (not freed by the
+        # garbage collector) until wait() returns for its ticket
+        self._targets = {}
 
     def close(self):
         if self.h:
             lib().gsplat_server_destroy(self.h)
             self.h = P()
+            self._targets.clear()
 
     def __del__(self):
         try:
@@ -641,11 +646,14 @@ class Server:
             req.out_rgba8, req.out_len = out.ctypes.data, out.nbytes
         t = ctypes.c_uint64()
         check(lib().gsplat_server_submit(self.h, ctypes.byref(req), ctypes.byref(t)))
+        if out is not None:
+            self._targets[t.value] = out
         return t.value
 
     def wait(self, ticket: int, raise_on_error: bool = True) -> GsplatRequestResult:
         r = GsplatRequestResult()
         rc = lib().gsplat_server_wait(self.h, ticket, ctypes.byref(r))
+        self._targets.pop(ticket, None)  # the ticket is released (or was never known)
         if raise_on_error:
             check(rc)
         return r

@@ -314,7 +314,8 @@ typedef struct GsplatRenderRequest {
   GsplatCamera camera;
   uint32_t width;
   uint32_t height;
-  uint8_t *out_rgba8; /* width*height*4 bytes or NULL; valid until wait returns */
+  uint8_t *out_rgba8; /* width*height*4 bytes or NULL; kept allocated and unused
+                       * by the caller until gsplat_server_wait returns */
   uint64_t out_len;
 } GsplatRenderRequest;
 
@@ -348,17 +349,24 @@ void gsplat_server_destroy(GsplatServer *server);
  * budget cannot hold it plus one 1080p view of it per pipeline. */
 int32_t gsplat_server_add_scene(GsplatServer *server, const GsplatScene *scene, uint32_t *out_scene_id);
 int32_t gsplat_server_remove_scene(GsplatServer *server, uint32_t scene_id);
-/* Queues a render. out_rgba8 inside a gsplat_host_alloc block is written by
- * a copy queued in the batch; other memory is written after the batch. */
+/* Queues a render. The RGBA8 result goes to out_rgba8 (may be NULL). When
+ * out_rgba8 lies in a gsplat_host_alloc block, the raster writes the pixels
+ * there directly from the device; other memory receives a copy after the
+ * batch. Either way the buffer must stay allocated, and must not be reused,
+ * until gsplat_server_wait returns for the ticket (gsplat_host_free refuses a
+ * block that a queued or running request still targets). */
 int32_t gsplat_server_submit(GsplatServer *server, const GsplatRenderRequest *request, uint64_t *out_ticket);
 /* Blocks until the request is done and releases the ticket; returns the
  * request's own status (its message in gsplat_last_error_message). */
 int32_t gsplat_server_wait(GsplatServer *server, uint64_t ticket, GsplatRequestResult *out_result);
 int32_t gsplat_server_get_stats(GsplatServer *server, GsplatServerStats *out_stats);
 
-/* Pinned host memory for server outputs (NULL on failure). */
+/* Pinned host memory for server outputs (NULL on failure). gsplat_host_free
+ * returns INVALID_ARGUMENT, freeing nothing, for a pointer that
+ * gsplat_host_alloc did not return or a block that a queued or running
+ * request still targets; NULL is a no-op. */
 void *gsplat_host_alloc(uint64_t bytes);
-void gsplat_host_free(void *ptr);
+int32_t gsplat_host_free(void *ptr);
 
 /* Camera at `position` looking at `target` with the reference's look-at
  * basis and quaternion (camera_rotation_looking_at / quat_from_camera_basis,

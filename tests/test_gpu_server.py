@@ -151,3 +151,59 @@ def test_server_errors(abi):
     with pytest.raises(abi.GsplatError):
         srv.submit(sid, cam, 64, 64)
     srv.close()
+
+
+def test_pinned_target_held_until_wait(abi):
+    # gsplat_host_free refuses a block that a queued or running request targets,
+    # and frees it once the request has ended; the binding keeps a target it was
+    # handed alive until wait() even when the caller dropped it.
+    import gc
+
+    scene = synth.tiny_scene(60000, 21, sh_degree=3)
+    dev = abi.Scene(scene)
+    srv = abi.Server(max_batch_views=8)
+    sid = srv.add_scene(dev)
+    rng = np.random.default_rng(3)
+    targets, tickets = [], []
+    say("[server pinned hold] 120 requests at 1280x720")
+    for _ in range(120):
+        out = abi.PinnedImage(1280, 720)
+        targets.append(out)
+        tickets.append(srv.submit(sid, abi.look_at(synth.sphere_point(rng, 3.0)), 1280, 720, out))
+    last = targets[-1]
+    rc = abi.lib().gsplat_host_free(last.ptr)
+    assert rc == abi.INVALID_ARGUMENT and "queued or running" in abi.last_error()
+    with pytest.raises(abi.GsplatError):
+        last.close()
+    assert last.ptr  # still allocated
+    dropped = srv.submit(sid, abi.look_at((0.2, 0.1, -3.0)), 320, 240, abi.PinnedImage(320, 240))
+    gc.collect()
+    for t in tickets:
+        assert srv.wait(t).status == 0
+    assert srv.wait(dropped).status == 0
+    assert not srv._targets
+    ref, _ = alone(abi, dev, abi.look_at(synth.sphere_point(np.random.default_rng(3), 3.0)), 1280, 720)
+    assert np.array_equal(targets[0].array, ref)
+    last.close()  # the request has ended
+    assert last.ptr is None
+    bogus = np.zeros(16, np.uint8)
+    assert abi.lib().gsplat_host_free(bogus.ctypes.data) == abi.INVALID_ARGUMENT
+    srv.close()
+
+
+def test_removed_scene_counts_until_released(abi):
+    # A removed scene's HBM stays counted against the budget while a pipeline's
+    # last frame (or a queued request) still holds it, and is released after.
+    a, b = abi.Scene(synth.tiny_scene(30000, 31)), abi.Scene(synth.tiny_scene(20000, 32))
+    srv = abi.Server(pipelines=1)
+    ia, ib = srv.add_scene(a), srv.add_scene(b)
+    both = srv.stats().resident_scene_bytes
+    assert both == a.device_bytes + b.device_bytes
+    assert srv.wait(srv.submit(ia, abi.look_at((0, 0, -3)), 128, 96)).status == 0
+    srv.remove_scene(ia)
+    bytes_b = b.device_bytes
+    a.close()  # the caller's handle may go too
+    assert srv.stats().resident_scene_bytes == both and srv.stats().scenes == 1
+    assert srv.wait(srv.submit(ib, abi.look_at((0, 0, -3)), 128, 96)).status == 0
+    assert srv.stats().resident_scene_bytes == bytes_b
+    srv.close()

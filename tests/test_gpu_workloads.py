@@ -1,0 +1,159 @@
+"""The batched and served workloads at their benchmark sizes, on the GPU.
+
+batched-64v-8s: 64 views over 8 resident scenes of 50k..1M SH3 splats at
+320x240..1920x1080 in one batch (29.6M splat slots). Every view must equal,
+bit for bit, the frame a context (gsplat_context_*) renders for the same
+scene, camera and size: visible set, back-to-front permutation, 16x16 tile
+counts, fp32 and RGBA8 images. Three views are also checked against the
+oracle to the plain-path bars, among them a 320x240 view of the 1M scene.
+
+served-16s-10m: Poisson arrivals (5000 requests/s) of single-view renders over
+16 resident scenes totalling 10M SH3 splats, into the continuous-batching
+server. Every request's image and visible count must equal the same view
+rendered alone, and two requests are checked against the oracle.
+"""
+
+import sys
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import synth
+from helpers import cam_dict, oracle_scene
+
+pytestmark = pytest.mark.gpu
+IMG_TOL = 2e-3
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def check_image8(gpu8, ref32, label):
+    ref8 = np.floor(np.clip(ref32, 0, 1) * np.float32(255) + np.float32(0.5)).astype(np.int16)
+    off = np.abs(gpu8.astype(np.int16) - ref8).max(axis=-1)
+    frac = (off <= 1).mean()
+    assert frac >= 0.999, f"{label}: only {frac:.5f} of pixels within 1 LSB"
+
+
+def oracle_check(oracle, scene, cam, w, h, order, tiles, img32, img8, visible, label):
+    say(f"[{label}] oracle N={scene.count} {w}x{h}")
+    osc = oracle_scene(oracle, scene)
+    cd = cam_dict(cam)
+    keys_ref, idx_ref = oracle.preprocess(osc, cd)
+    assert visible == len(idx_ref), f"{label}: visible {visible} vs {len(idx_ref)}"
+    assert np.array_equal(order, oracle.sort_values_by_keys(keys_ref, idx_ref)), f"{label}: permutation differs"
+    assert np.array_equal(tiles, oracle.tile_counts(oracle.project(osc, cd, w, h), w, h)), f"{label}: tile counts"
+    ref32, vis = oracle.render(osc, cd, w, h)
+    assert vis == len(idx_ref)
+    if img32 is not None:
+        d = np.abs(img32 - ref32).max()
+        assert d <= IMG_TOL, f"{label}: fp32 max-abs {d:.3g}"
+    check_image8(img8, ref32, label)
+
+
+@pytest.fixture(scope="module")
+def batched():
+    say("[batched-64v-8s] generating 8 scenes (3.7M splats)")
+    return synth.batched()
+
+
+def test_batched_64v_views_equal_context_frames(abi, oracle, batched):
+    scenes, views = batched
+    assert len(views) == 64 and [s.count for s in scenes] == synth.BATCH_SCENE_SIZES
+    dev = [abi.Scene(s) for s in scenes]
+    cams = [abi.look_at(v.position, v.target) for v in views]
+    b = abi.Batch()
+    say("[batched-64v-8s] one batch of 64 views")
+    b.render([(dev[v.scene], cams[i], v.width, v.height) for i, v in enumerate(views)])
+    b.wait()
+    st = b.stage_times()
+    assert st.splat_count == sum(scenes[v.scene].count for v in views)
+    ctxs = {}
+    total_pairs = 0
+    for i, v in enumerate(views):
+        n = scenes[v.scene].count
+        if v.scene not in ctxs:
+            ctx = abi.Context(v.width, v.height)
+            ctx.load(scenes[v.scene])
+            ctxs[v.scene] = ctx
+        ctx = ctxs[v.scene]
+        ctx.set_size(v.width, v.height)
+        ctx.set_camera(cams[i])
+        ctx.render()
+        label = f"view {i} (scene {v.scene}, N={n}, {v.width}x{v.height})"
+        say(f"[batched-64v-8s] {label}")
+        vis = b.view_stats(i).visible_count
+        assert vis == ctx.stats().visible_count, f"{label}: visible"
+        assert np.array_equal(b.sorted_indices(i, n), ctx.sorted_indices(n)), f"{label}: permutation"
+        tiles = b.tile_counts(i)
+        assert np.array_equal(tiles, ctx.tile_counts()), f"{label}: tile counts"
+        assert np.array_equal(b.rgba32f(i).view(np.uint32), ctx.rgba32f().view(np.uint32)), f"{label}: fp32 image"
+        assert np.array_equal(b.rgba8(i), ctx.rgba8()), f"{label}: rgba8 image"
+        total_pairs += int(tiles.sum())
+    assert st.tile_pairs == total_pairs
+    for c in ctxs.values():
+        c.close()
+    for i in (39, 23, 19):  # 1M scene at 320x240 and 640x480; 300k scene at 1920x1080
+        v = views[i]
+        n = scenes[v.scene].count
+        oracle_check(oracle, scenes[v.scene], cams[i], v.width, v.height, b.sorted_indices(i, n), b.tile_counts(i),
+                     b.rgba32f(i), b.rgba8(i), b.view_stats(i).visible_count, f"batched view {i}")
+    b.close()
+
+
+def test_served_16s_10m_poisson_results_equal_alone(abi, oracle):
+    say("[served-16s-10m] generating 16 scenes (10M splats)")
+    scenes, reqs = synth.served(requests=240, rate=5000.0)
+    assert sum(s.count for s in scenes) == 10_000_000
+    dev = [abi.Scene(s) for s in scenes]
+    srv = abi.Server()
+    ids = [srv.add_scene(d) for d in dev]
+    cams = [abi.look_at(v.position, v.target) for (_, v) in reqs]
+    outs = [abi.PinnedImage(v.width, v.height) if i % 2 == 0 else np.zeros((v.height, v.width, 4), np.uint8)
+            for i, (_, v) in enumerate(reqs)]
+    tickets = [None] * len(reqs)
+
+    def submitter():
+        t0 = time.perf_counter()
+        for i, (arrival, v) in enumerate(reqs):
+            delay = t0 + arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[i] = srv.submit(ids[v.scene], cams[i], v.width, v.height, outs[i])
+
+    say(f"[served-16s-10m] {len(reqs)} Poisson requests at 5000/s")
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    results = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    assert st.completed == len(reqs) and st.rejected == 0
+    assert all(r.status == 0 for r in results)
+    assert st.batches < len(reqs) and max(r.batch_views for r in results) > 1, "arrivals must share batches"
+    alone = abi.Batch()
+    for i, ((_, v), r, out) in enumerate(zip(reqs, results, outs)):
+        alone.render([(dev[v.scene], cams[i], v.width, v.height)])
+        got = out.array if isinstance(out, abi.PinnedImage) else out
+        label = f"request {i} (scene {v.scene}, N={scenes[v.scene].count}, {v.width}x{v.height}, " \
+                f"batch of {r.batch_views})"
+        if i % 40 == 0:
+            say(f"[served-16s-10m] {label}")
+        assert r.visible_count == alone.view_stats(0).visible_count, f"{label}: visible"
+        assert np.array_equal(got, alone.rgba8(0)), f"{label}: image differs from the view rendered alone"
+    # two requests against the oracle: the smallest view of the largest scene and a 640x480 view
+    big = max(range(len(scenes)), key=lambda s: scenes[s].count)
+    picks = [min((i for i, (_, v) in enumerate(reqs) if v.scene == big), key=lambda i: reqs[i][1].width),
+             next(i for i, (_, v) in enumerate(reqs) if v.width == 640 and v.scene != big)]
+    for i in picks:
+        v = reqs[i][1]
+        n = scenes[v.scene].count
+        alone.render([(dev[v.scene], cams[i], v.width, v.height)])
+        got = outs[i].array if isinstance(outs[i], abi.PinnedImage) else outs[i]
+        oracle_check(oracle, scenes[v.scene], cams[i], v.width, v.height, alone.sorted_indices(0, n),
+                     alone.tile_counts(0), None, got, results[i].visible_count, f"served request {i}")
+    alone.close()
+    srv.close()
