@@ -473,7 +473,7 @@ int32_t gsplat_sort_values_by_keys(const uint32_t *keys, uint32_t *values, uint6
     v.ensure(n);
     ka.ensure(n);
     va.ensure(n);
-    scratch.ensure_zeroed(gs::onesweep_status_words(n, 32));
+    scratch.ensure(gs::onesweep_status_words(n, 32));  // the sort clears what it needs on `st`
     if (hipMemcpyAsync(k.get(), keys, len * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(v.get(), values, len * 4, hipMemcpyHostToDevice, st) != hipSuccess)
       return fail(gs::Code::Internal, std::string(op) + ": upload failed");
@@ -1007,6 +1007,7 @@ int32_t gsplat_server_get_stats(GsplatServer *server, GsplatServerStats *out_sta
     out_stats->memory_budget_bytes = s.memory_budget;
     out_stats->scenes = s.scenes;
     out_stats->queued = s.queued;
+    out_stats->overflow_rerenders = s.overflow_rerenders;
     return ok();
   });
 }

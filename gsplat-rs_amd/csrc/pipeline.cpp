@@ -327,8 +327,8 @@ void Pipeline::launch_frame() {
   pair_vals_alt_.ensure(pair_cap_);
   // Look-back sorts need zeroed scratch; a later kernel of the frame clears
   // what each sort dirtied, so the next frame finds it clean.
-  if (depth_scratch_.ensure_zeroed(onesweep_status_words((uint32_t)m, 32))) depth_clean_ = true;
-  if (pair_scratch_.ensure_zeroed(onesweep_status_words(pair_cap_, tile_bits))) pair_clean_ = true;
+  if (depth_scratch_.ensure_zeroed(onesweep_status_words((uint32_t)m, 32), stream_)) depth_state_.reset();
+  if (pair_scratch_.ensure_zeroed(onesweep_status_words(pair_cap_, tile_bits), stream_)) pair_state_.reset();
   const bool depth_lookback = radix_uses_lookback((uint32_t)m, 0, tuning_.rts_min_tiles);
   // The last frame's pair count (+25%) sizes the pair sort's grids and mode.
   uint32_t hint = pairs_ ? (uint32_t)std::min<uint64_t>(pairs_ + pairs_ / 4, pair_cap_) : pair_cap_;
@@ -338,8 +338,9 @@ void Pipeline::launch_frame() {
   if (as_graph)
     hint = (uint32_t)std::min<uint64_t>((uint64_t)kSortTile << bits_for(div_up_u64(hint, kSortTile)), pair_cap_);
   const bool pair_lookback = radix_uses_lookback(pair_cap_, hint, tuning_.rts_min_tiles);
-  const size_t depth_dirty = radix_lookback_words((uint32_t)m, radix_passes(32, true, value_byte));
-  const size_t pair_dirty = radix_lookback_words(pair_cap_, radix_passes(tile_bits, false, -1));
+  const RadixPasses depth_ps = radix_passes(32, true, value_byte), pair_ps = radix_passes(tile_bits, false, -1);
+  const size_t depth_dirty = radix_lookback_words((uint32_t)m, depth_ps);
+  const size_t pair_dirty = radix_lookback_words(pair_cap_, pair_ps);
 
   // View table through a pinned staging slot whose previous upload is done.
   const int r = (int)(frame_counter_++ % 2);
@@ -370,6 +371,10 @@ void Pipeline::launch_frame() {
   plan.pair_lookback = pair_lookback;
   plan.depth_dirty = depth_dirty;
   plan.pair_dirty = pair_dirty;
+  plan.depth_scratch = depth_state_.before(depth_lookback, depth_dirty);
+  plan.pair_scratch = pair_state_.before(pair_lookback, pair_dirty);
+  plan.depth_rts_words = radix_rts_words((uint32_t)m, depth_ps);
+  plan.pair_rts_words = radix_rts_words(pair_cap_, pair_ps);
   plan.uniform_shift = true;
   for (const ViewDev &d : host_views_) plan.uniform_shift = plan.uniform_shift && d.bin_shift == host_views_[0].bin_shift;
   if (as_graph) {
@@ -378,7 +383,7 @@ void Pipeline::launch_frame() {
     std::vector<uint64_t> key = {(uint64_t)r, nviews, slots, tiles, ctiles, tile_bits, (uint64_t)(value_byte + 1),
                                  pair_cap_, hint, tuning_.bin_shift,
                                  (uint64_t)tuning_.raster, tuning_.rows2_min_tiles, tuning_.rts_min_tiles, tuning_.blend_min,
-                                 tuning_.bin_auto, count_evals_, depth_clean_, pair_clean_};
+                                 tuning_.bin_auto, count_evals_, depth_state_.rts_dirty(), pair_state_.rts_dirty()};
     for (const ViewDev &d : host_views_) {
       key.push_back(reinterpret_cast<uintptr_t>(d.soa));
       key.push_back(((uint64_t)d.n << 8) | d.sh_degree);
@@ -412,8 +417,8 @@ void Pipeline::launch_frame() {
       g.key = std::move(key);
       g.sorted_keys = sorted_keys_;
       g.sorted_vals = sorted_vals_;
-      g.depth_clean = depth_clean_;
-      g.pair_clean = pair_clean_;
+      g.depth_state = depth_state_;
+      g.pair_state = pair_state_;
     }
     GS_HIP(hipEventRecord(ev_[0], stream_));
     GS_HIP(hipGraphLaunch(g.exec, stream_));
@@ -423,8 +428,8 @@ void Pipeline::launch_frame() {
     staged_times_ = false;
     sorted_keys_ = g.sorted_keys;
     sorted_vals_ = g.sorted_vals;
-    depth_clean_ = g.depth_clean;
-    pair_clean_ = g.pair_clean;
+    depth_state_ = g.depth_state;
+    pair_state_ = g.pair_state;
   } else {
     record_frame(plan, false);
     staged_times_ = stage_timing_;
@@ -488,7 +493,8 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
 
   uint32_t *k = keys_.get(), *v = vals_.get(), *ka = keys_alt_.get(), *va = vals_alt_.get();
   SortPrep depth_prep;
-  depth_prep.scratch_clean = depth_clean_;
+  if (plan.depth_scratch.fill) launch_fill_u32(depth_scratch_.get(), 0u, plan.depth_scratch.fill, st);
+  depth_prep.scratch_clean = plan.depth_scratch.skip_clear;
   depth_prep.rts_min_tiles = tuning_.rts_min_tiles;
   depth_prep.seg_start = seg_start_d;  // view v's slots: segment v, value byte 3 = v
   depth_prep.segs = nviews;
@@ -500,7 +506,7 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
 
   Clears scan_clears;  // the depth sort's scratch, for the next frame
   if (depth_lookback) scan_clears.add(depth_scratch_.get(), depth_dirty);
-  depth_clean_ = depth_lookback;
+  depth_state_.after(depth_lookback, plan.depth_rts_words);
   // The sort's other buffers (ka, va) are free until the next frame: they
   // carry the rects in sorted order from the counts to the emission.
   launch_tile_counts(views_d, seg_start_d, nviews, slots, k, v, rect_.get(), counts_.get(),
@@ -511,13 +517,14 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
                    block_sums_.get(), pair_cap_, pair_keys_.get(), pair_vals_.get(), st);
   uint32_t *pk = pair_keys_.get(), *pv = pair_vals_.get(), *pka = pair_keys_alt_.get(), *pva = pair_vals_alt_.get();
   SortPrep pair_prep;
-  pair_prep.scratch_clean = pair_clean_;
+  if (plan.pair_scratch.fill) launch_fill_u32(pair_scratch_.get(), 0u, plan.pair_scratch.fill, st);
+  pair_prep.scratch_clean = plan.pair_scratch.skip_clear;
   pair_prep.rts_min_tiles = tuning_.rts_min_tiles;
   onesweep_sort(&pk, &pv, &pka, &pva, pair_cap_, m_dev_.get(), tile_bits, /*descending=*/false,
                 pair_scratch_.get(), pair_scratch_.capacity(), st, -1, hint, pair_prep);
   Clears range_clears;  // the pair sort's scratch (and the counting words), for later
   if (pair_lookback) range_clears.add(pair_scratch_.get(), pair_dirty);
-  pair_clean_ = pair_lookback;
+  pair_state_.after(pair_lookback, plan.pair_rts_words);
   if (count_evals_) range_clears.add(info + kEvalsWord, 4);
   // Every summary word but the evaluation counts is final here: the ranges
   // kernel hands them to the pinned summary itself.

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -13,6 +14,7 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "scratch_state.h"
 
 namespace gs {
 
@@ -47,11 +49,13 @@ class DevBuf {
     cap_ = want;
   }
   // As ensure(), clearing the allocation to zero whenever it is (re)made;
-  // true when it was.
-  bool ensure_zeroed(size_t count) {
+  // true when it was. The clear is queued on `stream`, the stream whose work
+  // reads the buffer next: a null-stream hipMemset is not ordered before work
+  // on a non-blocking stream. Never called under graph capture.
+  bool ensure_zeroed(size_t count, hipStream_t stream) {
     if (count <= cap_) return false;
     ensure(count);
-    GS_HIP(hipMemset(ptr_, 0, cap_ * sizeof(T)));
+    GS_HIP(hipMemsetAsync(ptr_, 0, cap_ * sizeof(T), stream));
     return true;
   }
   void release() {
@@ -173,7 +177,7 @@ class Pipeline {
   uint64_t total_splats() const { return total_splats_; }
   uint32_t total_tiles() const { return total_tiles_; }
   FrameTimes times();  // waits
-  uint64_t overflow_rerenders() const { return overflow_rerenders_; }
+  uint64_t overflow_rerenders() const { return overflow_rerenders_.load(std::memory_order_relaxed); }
   uint32_t pair_capacity() const { return pair_cap_; }
 
   void readback_rgba8(uint32_t v, uint8_t *out, uint64_t len);
@@ -220,14 +224,16 @@ class Pipeline {
     bool uniform_shift = false;  // every view at one bin shift: heaviest-first raster order
     int value_byte = -1;
     bool depth_lookback = false, pair_lookback = false;
-    size_t depth_dirty = 0, pair_dirty = 0;
+    size_t depth_dirty = 0, pair_dirty = 0;                // look-back words the frame clears after use
+    size_t depth_rts_words = 0, pair_rts_words = 0;        // words a reduce-then-scan sort may write
+    ScratchState::Plan depth_scratch, pair_scratch;
   };
   struct GraphSlot {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<uint64_t> key;
     uint32_t *sorted_keys = nullptr, *sorted_vals = nullptr;
-    bool depth_clean = false, pair_clean = false;
+    ScratchState depth_state, pair_state;  // the scratch state a replay leaves
   };
   void launch_frame();
   // Every stream operation of a frame; `captured` leaves out the event
@@ -273,11 +279,11 @@ class Pipeline {
   std::vector<uint32_t> visible_;
   uint64_t pairs_ = 0;         // binning pairs of the last frame
   uint64_t raster_pairs_ = 0;  // (16x16 tile, splat) pairs of the last frame
-  uint64_t overflow_rerenders_ = 0;
+  std::atomic<uint64_t> overflow_rerenders_{0};  // read by other threads (server statistics)
   FrameTimes times_{};
   Tuning tuning_{};
   bool count_evals_ = false;
-  bool depth_clean_ = false, pair_clean_ = false;  // look-back scratch words are zero
+  ScratchState depth_state_, pair_state_;  // what the depth and pair sorts' scratch holds
   bool use_graph_ = false;
   GraphSlot graphs_[2];
   bool staged_times_ = false;  // the last frame recorded its stage events

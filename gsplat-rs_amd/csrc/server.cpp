@@ -230,6 +230,7 @@ ServerStats Server::stats() {
   s.memory_budget = cfg_.memory_budget;
   s.scenes = (uint32_t)scenes_.size();
   s.queued = (uint32_t)queue_.size();
+  for (const auto &sl : slots_) s.overflow_rerenders += sl->pipe->overflow_rerenders();
   return s;
 }
 

@@ -56,6 +56,7 @@ struct ServerStats {
   uint64_t views_in_batches = 0;
   uint64_t resident_scene_bytes = 0, memory_budget = 0;
   uint32_t scenes = 0, queued = 0;
+  uint64_t overflow_rerenders = 0;  // batches rendered again after a tile-pair overflow
 };
 
 // Pinned host allocations handed out to callers for server outputs; the

@@ -44,6 +44,11 @@ bool radix_uses_lookback(uint32_t n_cap, uint32_t n_hint = 0, uint32_t rts_min_t
 // words); they are left dirty after the sort.
 size_t radix_lookback_words(uint32_t n_cap, const RadixPasses &ps);
 
+// u32 words at the start of the scratch that a reduce-then-scan sort of up
+// to n_cap elements with passes `ps` may write (digit counts, the live count,
+// the scan ticket and its chunk status words, which carry flag bits).
+size_t radix_rts_words(uint32_t n_cap, const RadixPasses &ps);
+
 struct SortPrep {
   bool scratch_clean = false;  // the look-back words are zero: skip the clear
   bool hist_ready = false;     // the digit histograms are accumulated: skip k_os_hist

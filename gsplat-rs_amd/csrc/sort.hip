@@ -644,6 +644,11 @@ size_t radix_lookback_words(uint32_t n_cap, const RadixPasses &ps) {
   return lookback_words(n_cap, ps.count, 1u << ps.digit_bits);
 }
 
+size_t radix_rts_words(uint32_t n_cap, const RadixPasses &ps) {
+  const size_t entries = (size_t)(1u << ps.digit_bits) * std::max<uint32_t>(div_up(n_cap, TILE), 1);
+  return entries + 2 + div_up(entries, SCAN_CHUNK);  // counts | live count | ticket | chunk status words
+}
+
 namespace {
 
 template <uint32_t DB>

@@ -338,6 +338,8 @@ typedef struct GsplatServerStats {
   uint64_t memory_budget_bytes;
   uint32_t scenes;
   uint32_t queued;
+  uint64_t overflow_rerenders; /* batches rendered again after their tile-pair
+                                * buffers overflowed (the buffers then grow) */
 } GsplatServerStats;
 
 /* {0, 64, 4096} */

@@ -286,3 +286,29 @@ def test_stage_timing_switch(abi):
     assert ctx.stage_times().sort_ms == 0 and ctx.stage_times().total_ms > 0
     assert np.array_equal(ctx.rgba8(), img)
     ctx.close()
+
+
+def test_sort_mode_switches_keep_frames(abi):
+    # One batch object alternating between a view whose depth sort runs in
+    # reduce-then-scan mode (1.1M splats, 269 sort tiles) and views sorted by
+    # look-back (50k, then 700k splats): every frame must equal a fresh
+    # batch's frame of the same view (stale reduce-then-scan scan state must
+    # never reach a later look-back sort).
+    scenes = [synth.tiny_scene(n, 6100 + i, sh_degree=0, spread=0.8, log_scale=-4.0)
+              for i, n in enumerate((1_100_000, 50_000, 700_000))]
+    dev = [abi.Scene(s) for s in scenes]
+    cams = [abi.look_at((0.3, 0.2, -3.0)), abi.look_at((2.0, 0.5, 2.0)), abi.look_at((-1.0, 1.5, -2.5))]
+    sizes = [(1280, 720), (640, 480), (320, 240)]
+    fresh = {}
+    for s in range(3):
+        f = abi.Batch()
+        f.render([(dev[s], cams[s], *sizes[s])])
+        fresh[s] = batch_view(f, 0, scenes[s].count)
+        f.close()
+    b = abi.Batch()
+    for cycle in range(4):
+        for s in (0, 1, 2):
+            say(f"[sort mode switches] cycle {cycle} scene {s}")
+            b.render([(dev[s], cams[s], *sizes[s])])
+            assert_same(batch_view(b, 0, scenes[s].count), fresh[s], f"cycle {cycle} scene {s}")
+    b.close()
