@@ -17,6 +17,7 @@
 #include "renderer.h"
 #include "server.h"
 #include "sort.h"
+#include "sort_noalloc.h"
 
 struct GsplatContext {
   std::unique_ptr<gs::Renderer> renderer;
@@ -482,6 +483,24 @@ int32_t gsplat_sort_values_by_keys(const uint32_t *keys, uint32_t *values, uint6
     if (hipMemcpyAsync(values, pv, len * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
       return fail(gs::Code::Internal, std::string(op) + ": download failed");
+    return ok();
+  });
+}
+
+// The same sort with every device buffer in a fixed device array of the
+// library (no allocation); lengths above 2^20 are refused.
+int32_t gsplat_sort_values_by_keys_noalloc(const uint32_t *keys, uint32_t *values, uint64_t len) {
+  static const char *op = "gsplat_sort_values_by_keys_noalloc";
+  return guard(op, [&]() -> int32_t {
+    if (len <= 1) return ok();
+    if (!keys) return null_arg(op, "keys");
+    if (!values) return null_arg(op, "values");
+    if (len > gs::kNoAllocSortMaxLen)
+      return fail(gs::Code::Unsupported, std::string(op) + ": length " + std::to_string(len) +
+                                             " exceeds the no-allocation limit of " +
+                                             std::to_string(gs::kNoAllocSortMaxLen));
+    if (!gs::device_present()) return fail(gs::Code::Unsupported, std::string(op) + ": gpu backend unavailable");
+    gs::sort_values_by_keys_noalloc(keys, values, (uint32_t)len);
     return ok();
   });
 }

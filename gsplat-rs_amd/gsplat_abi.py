@@ -224,6 +224,7 @@ SIGNATURES = {
     "gsplat_context_read_projection": (I32, [P, P, U64]),
     "gsplat_context_read_tile_counts": (I32, [P, P, U64, PU32, PU32]),
     "gsplat_sort_values_by_keys": (I32, [P, P, U64]),
+    "gsplat_sort_values_by_keys_noalloc": (I32, [P, P, U64]),
     "gsplat_ply_limits_default": (GsplatPlyLimits, []),
     "gsplat_ply_load": (I32, [ctypes.c_char_p, ctypes.POINTER(P)]),
     "gsplat_ply_parse_bytes": (I32, [P, U64, ctypes.POINTER(GsplatPlyLimits), ctypes.POINTER(P)]),

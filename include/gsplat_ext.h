@@ -137,6 +137,18 @@ int32_t gsplat_context_read_tile_counts(GsplatContext *ctx, uint32_t *out, uint6
  * equal keys keep their input order. Host buffers; runs on the device. */
 int32_t gsplat_sort_values_by_keys(const uint32_t *keys, uint32_t *values, uint64_t len);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc). The same
+ * sort as gsplat_sort_values_by_keys, same arguments and result, with every
+ * device buffer (keys, values, their radix alternates, the sort's status
+ * words) in one fixed 18 MiB device array of the library that each GPU it
+ * runs on holds until the process ends: no hipMalloc, hipFree or stream
+ * creation. len > 1048576 (2^20) -> UNSUPPORTED. Calls share their GPU's
+ * array: the calls of one process are serialised by a lock and have finished
+ * on the device when they return; overlapping use of the array from outside
+ * these calls would race. */
+int32_t gsplat_sort_values_by_keys_noalloc(const uint32_t *keys, uint32_t *values, uint64_t len);
+
 /* ------------------------------------------------------------- PLY loader */
 
 /* Resource budgets of gsplat_io_ply::PlyLoadLimits (io-ply/src/lib.rs:28-47);
