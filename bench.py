@@ -510,6 +510,57 @@ def summarize_profiles(trace_dir, fetch_dir, write_dir, copy_fetch_dir, copy_wri
     print(json.dumps(out))
 
 
+def bench_session(scene, view, steps):
+    """A context following a slow orbit (0.003 rad per frame) of the 1080p scene: frames/s when every
+    frame sorts (the plain path) and with session order reuse at sort interval 2, and the stale-order
+    gate (mean |delta RGB|, share of pixels over 3/255) of the reused frames against exact ones."""
+    import math
+
+    from image_metrics import rgba_parity_metrics
+
+    def cam(k):
+        a = 0.7 + 0.003 * k
+        return A.look_at((4.0 * math.cos(a), 0.8, 4.0 * math.sin(a)))
+
+    cams = [cam(k) for k in range(steps)]
+    out = {"orbit_rad_per_frame": 0.003, "frames": steps}
+    for label, interval in (("sort_every_frame", None), ("sort_interval_2", 2)):
+        ctx = A.Context(view.width, view.height)
+        ctx.load(scene)
+        ctx.set_stage_timing(False)
+        if interval:
+            ctx.set_sort_interval(interval)
+        for c in cams[:10]:
+            ctx.set_camera(c)
+            ctx.render()
+        ctx.wait()
+        t0 = time.perf_counter()
+        for c in cams:
+            ctx.set_camera(c)
+            ctx.render()
+        ctx.wait()
+        out[label] = {"frames_per_s": round(steps / (time.perf_counter() - t0), 1)}
+        ctx.close()
+    exact, sess = A.Context(view.width, view.height), A.Context(view.width, view.height)
+    for c in (exact, sess):
+        c.load(scene)
+    sess.set_sort_interval(2)
+    worst = [0.0, 0.0]
+    for c in cams[:16]:
+        for x in (exact, sess):
+            x.set_camera(c)
+            x.render()
+        if not sess.sort_stats().flags & 1:
+            mean, frac, _ = rgba_parity_metrics(sess.rgba8(), exact.rgba8())
+            worst = [max(worst[0], mean), max(worst[1], frac)]
+    exact.close()
+    sess.close()
+    out["sort_interval_2"]["worst_reused_mean_abs_rgb"] = round(worst[0], 6)
+    out["sort_interval_2"]["worst_reused_frac_over_3_255"] = round(worst[1], 6)
+    out["speedup"] = round(out["sort_interval_2"]["frames_per_s"] / out["sort_every_frame"]["frames_per_s"], 3)
+    return out
+
+
 def pct(vals):
     vals = sorted(vals)
     return {"p50": round(nearest_rank(vals, 0.5), 3), "p99": round(nearest_rank(vals, 0.99), 3),
@@ -806,6 +857,8 @@ def main():
         print(f"batched_64v {line['batched_64v']}", file=sys.stderr, flush=True)
         line["served"] = bench_served(args.served_requests, 5000.0, 2.0)
         print(f"served {line['served']}", file=sys.stderr, flush=True)
+        line["session"] = bench_session(scene, view, max(args.steps, 60))
+        print(f"session {line['session']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, view, cam)
     print(json.dumps(line), flush=True)

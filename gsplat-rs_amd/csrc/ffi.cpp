@@ -181,6 +181,7 @@ int32_t gsplat_context_set_camera(GsplatContext *ctx, GsplatCamera camera) {
     if (!ctx) return null_arg(op, "ctx");
     if (!gs::camera_valid(camera)) return fail(gs::Code::InvalidArgument, std::string(op) + ": invalid camera");
     ctx->camera = camera;
+    ctx->renderer->camera_set(camera);
     return ok();
   });
 }
@@ -194,6 +195,7 @@ int32_t gsplat_context_set_auto_camera(GsplatContext *ctx) {
     } catch (const gs::Error &e) {
       return fail(e.code, std::string(op) + ": scene is not loaded");
     }
+    ctx->renderer->camera_set(ctx->camera);
     return ok();
   });
 }
@@ -215,6 +217,38 @@ int32_t gsplat_context_render_frame(GsplatContext *ctx) {
   return guard(op, [&]() -> int32_t {
     if (!ctx) return null_arg(op, "ctx");
     ctx->renderer->render(ctx->camera);
+    return ok();
+  });
+}
+
+// SurfaceRenderSession::set_sort_interval (render-wgpu/src/surface_session.rs:
+// 851-860) and the FFI's check (ffi-c/src/lib.rs:1005-1033), for a context.
+int32_t gsplat_context_set_sort_interval(GsplatContext *ctx, uint32_t interval) {
+  static const char *op = "gsplat_context_set_sort_interval";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (interval == 0) return fail(gs::Code::InvalidArgument, std::string(op) + ": interval must be positive");
+    ctx->renderer->set_sort_interval(interval);
+    return ok();
+  });
+}
+
+int32_t gsplat_context_force_sort_refresh(GsplatContext *ctx) {
+  static const char *op = "gsplat_context_force_sort_refresh";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    ctx->renderer->force_sort_refresh();
+    return ok();
+  });
+}
+
+// gsplat_surface_renderer_get_sort_stats (ffi-c/src/lib.rs:1566-1591) for a context.
+int32_t gsplat_context_get_sort_stats(const GsplatContext *ctx, GsplatSurfaceSortStats *out_stats) {
+  static const char *op = "gsplat_context_get_sort_stats";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!out_stats) return null_arg(op, "out_stats");
+    *out_stats = ctx->renderer->sort_stats();
     return ok();
   });
 }

@@ -185,8 +185,14 @@ Pipeline::~Pipeline() {
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_rgba8) {
+bool Pipeline::has_order(const DeviceScene *scene) const {
+  return scene && order_scene_ == scene && order_count_ == scene->count && order_keys_.get();
+}
+
+void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_rgba8, bool reuse_order) {
   if (views.empty()) throw Error(Code::InvalidArgument, "batch has no views");
+  if (reuse_order && (views.size() != 1 || !has_order(views[0].scene.get())))
+    throw Error(Code::Internal, "no kept depth order for this view");
   if (!host_rgba8.empty() && host_rgba8.size() != views.size())
     throw Error(Code::InvalidArgument, "one host target per view expected");
   if (views.size() > kMaxViews)
@@ -284,6 +290,7 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
     }
   }
 
+  reuse_order_ = reuse_order;
   views_ = std::move(views);
   host_rgba8_ = std::move(host_rgba8);
   host_views_ = std::move(table);
@@ -375,6 +382,14 @@ void Pipeline::launch_frame() {
   plan.pair_scratch = pair_state_.before(pair_lookback, pair_dirty);
   plan.depth_rts_words = radix_rts_words((uint32_t)m, depth_ps);
   plan.pair_rts_words = radix_rts_words(pair_cap_, pair_ps);
+  plan.reuse_order = reuse_order_;
+  plan.keep_order = keep_order_ && nviews == 1 && !reuse_order_;
+  if (plan.keep_order) {
+    order_keys_.ensure(m);
+    order_vals_.ensure(m);
+    order_scene_ = views_[0].scene.get();
+    order_count_ = views_[0].scene->count;
+  }
   plan.uniform_shift = true;
   for (const ViewDev &d : host_views_) plan.uniform_shift = plan.uniform_shift && d.bin_shift == host_views_[0].bin_shift;
   if (as_graph) {
@@ -383,7 +398,8 @@ void Pipeline::launch_frame() {
     std::vector<uint64_t> key = {(uint64_t)r, nviews, slots, tiles, ctiles, tile_bits, (uint64_t)(value_byte + 1),
                                  pair_cap_, hint, tuning_.bin_shift,
                                  (uint64_t)tuning_.raster, tuning_.rows2_min_tiles, tuning_.rts_min_tiles, tuning_.blend_min,
-                                 tuning_.bin_auto, count_evals_, depth_state_.rts_dirty(), pair_state_.rts_dirty()};
+                                 tuning_.bin_auto, count_evals_, depth_state_.rts_dirty(), pair_state_.rts_dirty(),
+                                 (uint64_t)reuse_order_, (uint64_t)keep_order_};
     for (const ViewDev &d : host_views_) {
       key.push_back(reinterpret_cast<uintptr_t>(d.soa));
       key.push_back(((uint64_t)d.n << 8) | d.sh_degree);
@@ -492,21 +508,34 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   if (stage_events) GS_HIP(hipEventRecord(ev_[1], st));
 
   uint32_t *k = keys_.get(), *v = vals_.get(), *ka = keys_alt_.get(), *va = vals_alt_.get();
-  SortPrep depth_prep;
-  if (plan.depth_scratch.fill) launch_fill_u32(depth_scratch_.get(), 0u, plan.depth_scratch.fill, st);
-  depth_prep.scratch_clean = plan.depth_scratch.skip_clear;
-  depth_prep.rts_min_tiles = tuning_.rts_min_tiles;
-  depth_prep.seg_start = seg_start_d;  // view v's slots: segment v, value byte 3 = v
-  depth_prep.segs = nviews;
-  onesweep_sort(&k, &v, &ka, &va, slots, nullptr, 32, /*descending=*/true, depth_scratch_.get(),
-                depth_scratch_.capacity(), st, value_byte, 0, depth_prep);
+  const bool sorted = !plan.reuse_order;
+  if (plan.reuse_order) {
+    // The last refreshed order, presented with this frame's projection: the
+    // splats it holds are binned (those now culled have no pixel bounds),
+    // and no depth sort runs. keys_alt_ / vals_alt_ stay free for the rects.
+    k = order_keys_.get();
+    v = order_vals_.get();
+  } else {
+    SortPrep depth_prep;
+    if (plan.depth_scratch.fill) launch_fill_u32(depth_scratch_.get(), 0u, plan.depth_scratch.fill, st);
+    depth_prep.scratch_clean = plan.depth_scratch.skip_clear;
+    depth_prep.rts_min_tiles = tuning_.rts_min_tiles;
+    depth_prep.seg_start = seg_start_d;  // view v's slots: segment v, value byte 3 = v
+    depth_prep.segs = nviews;
+    onesweep_sort(&k, &v, &ka, &va, slots, nullptr, 32, /*descending=*/true, depth_scratch_.get(),
+                  depth_scratch_.capacity(), st, value_byte, 0, depth_prep);
+    if (plan.keep_order) {  // kept for the frames that reuse it
+      GS_HIP(hipMemcpyAsync(order_keys_.get(), k, (size_t)slots * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+      GS_HIP(hipMemcpyAsync(order_vals_.get(), v, (size_t)slots * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    }
+  }
   sorted_keys_ = k;
   sorted_vals_ = v;
   if (stage_events) GS_HIP(hipEventRecord(ev_[2], st));
 
   Clears scan_clears;  // the depth sort's scratch, for the next frame
-  if (depth_lookback) scan_clears.add(depth_scratch_.get(), depth_dirty);
-  depth_state_.after(depth_lookback, plan.depth_rts_words);
+  if (sorted && depth_lookback) scan_clears.add(depth_scratch_.get(), depth_dirty);
+  if (sorted) depth_state_.after(depth_lookback, plan.depth_rts_words);
   // The sort's other buffers (ka, va) are free until the next frame: they
   // carry the rects in sorted order from the counts to the emission.
   launch_tile_counts(views_d, seg_start_d, nviews, slots, k, v, rect_.get(), counts_.get(),

@@ -164,7 +164,14 @@ class Pipeline {
   // device-visible (hipHostMalloc), else a copy queued behind the raster
   // does. It must stay valid until wait() returns. Such a view keeps no
   // device image (readback_* of it reads stale data).
-  void enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_rgba8 = {});
+  // With `reuse_order` (one view only, has_order() true) the frame skips the
+  // depth sort and bins the view in the last kept order (see set_keep_order).
+  void enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_rgba8 = {}, bool reuse_order = false);
+  // Keep each one-view frame's depth order (a device copy of the sorted keys
+  // and values) for later frames of the same scene that reuse it.
+  void set_keep_order(bool on) { keep_order_ = on; }
+  // A kept order of `scene` exists.
+  bool has_order(const DeviceScene *scene) const;
   // Waits for the last frame, re-rendering it after an overflow.
   void wait();
   bool has_frame() const { return !views_.empty(); }
@@ -223,6 +230,7 @@ class Pipeline {
     uint32_t nviews = 0, slots = 0, tiles = 0, ctiles = 0, tile_bits = 0, count_blocks = 0, hint = 0;
     bool uniform_shift = false;  // every view at one bin shift: heaviest-first raster order
     int value_byte = -1;
+    bool reuse_order = false, keep_order = false;
     bool depth_lookback = false, pair_lookback = false;
     size_t depth_dirty = 0, pair_dirty = 0;                // look-back words the frame clears after use
     size_t depth_rts_words = 0, pair_rts_words = 0;        // words a reduce-then-scan sort may write
@@ -263,6 +271,10 @@ class Pipeline {
   DevBuf<uchar4> img8_;
   DevBuf<uint32_t> depth_scratch_, pair_scratch_, m_dev_, block_sums_, block_pairs_;
   DevBuf<ViewDev> export_view_;
+  DevBuf<uint32_t> order_keys_, order_vals_;  // kept depth order (set_keep_order)
+  const DeviceScene *order_scene_ = nullptr;  // the scene whose order is kept
+  uint32_t order_count_ = 0;
+  bool keep_order_ = false, reuse_order_ = false;
 
   std::vector<ViewSpec> views_;
   std::vector<uint8_t *> host_rgba8_;

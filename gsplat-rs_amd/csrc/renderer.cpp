@@ -2,7 +2,9 @@
 
 #include "renderer.h"
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <string>
 
@@ -40,8 +42,27 @@ void Renderer::load_scene(HostScene &&scene) {
   validate_scene(scene);
   pipe_.wait();
   scene_ = upload_scene(scene, pipe_.stream());
+  // The pose envelope's translation bound: 2% of the bounding-box diagonal
+  // (SurfaceRenderSession::new, surface_session.rs:659-674).
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = 0; i < scene.count; ++i)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], scene.positions[3 * (size_t)i + a]);
+      hi[a] = std::max(hi[a], scene.positions[3 * (size_t)i + a]);
+    }
+  float d2 = 0.0f;
+  for (int a = 0; a < 3; ++a) d2 += scene.count ? (hi[a] - lo[a]) * (hi[a] - lo[a]) : 0.0f;
+  policy_.set_scene_diagonal(std::sqrt(d2));
   host_scene_ = std::make_unique<HostScene>(std::move(scene));
   frame_valid_ = false;
+}
+
+void Renderer::set_sort_interval(uint32_t interval) {
+  if (interval == 0) throw Error(Code::InvalidArgument, "interval must be positive");
+  policy_.set_interval(interval);
+  if (!reuse_enabled_) policy_.force_refresh();
+  reuse_enabled_ = true;
+  pipe_.set_keep_order(true);
 }
 
 void Renderer::render(const GsplatCamera &cam) {
@@ -50,7 +71,14 @@ void Renderer::render(const GsplatCamera &cam) {
   if (!camera_valid(cam)) throw Error(Code::InvalidArgument, "invalid camera");
   const FrameParams fp = make_frame_params(cam.position, cam.rotation_xyzw, cam.vertical_fov_radians,
                                            cam.near_plane, cam.far_plane, width_, height_);
-  pipe_.enqueue({ViewSpec{scene_, fp}});
+  if (reuse_enabled_) {
+    policy_.camera_set(cam);
+    const OrderReusePolicy::Plan p = policy_.plan(pipe_.has_order(scene_.get()));
+    pipe_.enqueue({ViewSpec{scene_, fp}}, {}, !p.refresh);
+    policy_.finish(p);
+  } else {
+    pipe_.enqueue({ViewSpec{scene_, fp}});
+  }
   frame_valid_ = true;
   host_ms_ = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }

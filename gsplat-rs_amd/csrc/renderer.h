@@ -8,6 +8,7 @@
 #include <memory>
 
 #include "common.h"
+#include "order_reuse.h"
 #include "pipeline.h"
 
 namespace gs {
@@ -44,6 +45,16 @@ class Renderer {
   void read_projection(const GsplatCamera &camera, float *out, uint64_t cap);
   void read_tile_counts(uint32_t *out, uint64_t cap, uint32_t *tx, uint32_t *ty);
 
+  // Session order reuse (off until set_sort_interval is called; the plain
+  // path sorts every frame). With it, the context refreshes its depth order
+  // every `interval` camera changes, within the lag and pose bounds of
+  // OrderReusePolicy, and presents the other frames in the kept order.
+  void set_sort_interval(uint32_t interval);
+  void force_sort_refresh() { policy_.force_refresh(); }
+  // The context's camera was set (counts camera revisions).
+  void camera_set(const GsplatCamera &c) { policy_.camera_set(c); }
+  GsplatSurfaceSortStats sort_stats() const { return policy_.stats(); }
+
  private:
   void require_frame() const;
 
@@ -53,6 +64,8 @@ class Renderer {
   std::shared_ptr<DeviceScene> scene_;
   bool frame_valid_ = false;
   float host_ms_ = 0.0f;
+  OrderReusePolicy policy_;
+  bool reuse_enabled_ = false;
 };
 
 // Raster variant chosen by GSPLAT_RASTER_VARIANT: 0 per-lane exit, else the

@@ -188,6 +188,9 @@ SIGNATURES = {
     "gsplat_context_load_scene_path": (I32, [P, ctypes.c_char_p]),
     "gsplat_context_render_frame": (I32, [P]),
     "gsplat_context_get_stats": (I32, [P, ctypes.POINTER(GsplatStats)]),
+    "gsplat_context_set_sort_interval": (I32, [P, U32]),
+    "gsplat_context_force_sort_refresh": (I32, [P]),
+    "gsplat_context_get_sort_stats": (I32, [P, ctypes.POINTER(GsplatSurfaceSortStats)]),
     "gsplat_surface_renderer_create_android": (I32, [P, ctypes.c_char_p, U32, U32, ctypes.POINTER(P)]),
     "gsplat_surface_renderer_create_android_with_geometry_path": (
         I32, [P, ctypes.c_char_p, U32, U32, U32, ctypes.POINTER(P)]),
@@ -380,6 +383,18 @@ class Context:
     def set_size(self, width: int, height: int):
         check(lib().gsplat_context_set_size(self.h, width, height))
         self.width, self.height = width, height
+
+    def set_sort_interval(self, interval: int):
+        """Session order reuse: refresh the depth order every `interval` camera changes."""
+        check(lib().gsplat_context_set_sort_interval(self.h, interval))
+
+    def force_sort_refresh(self):
+        check(lib().gsplat_context_force_sort_refresh(self.h))
+
+    def sort_stats(self) -> GsplatSurfaceSortStats:
+        s = GsplatSurfaceSortStats()
+        check(lib().gsplat_context_get_sort_stats(self.h, ctypes.byref(s)))
+        return s
 
     def get_camera(self) -> GsplatCamera:
         c = GsplatCamera()

@@ -104,6 +104,33 @@ typedef struct GsplatStageTimes {
 int32_t gsplat_context_get_stage_times(GsplatContext *ctx, GsplatStageTimes *out);
 int32_t gsplat_context_set_stage_timing(GsplatContext *ctx, uint32_t on);
 
+/* ------------------------------------------------- session order reuse */
+
+/*
+ * Opt-in session semantics for a context, after the reference's
+ * SurfaceRenderSession (render-wgpu/src/surface_session.rs:17-32, 556-596,
+ * 734-744, 851-860, 903-1273). Until gsplat_context_set_sort_interval is
+ * called, every frame sorts (the plain SortedAlpha path). After it, each
+ * changed camera (gsplat_context_set_camera / _set_auto_camera) is a new
+ * camera revision; the depth order is refreshed every `interval` camera
+ * changes, and other frames draw the current camera's projection in the last
+ * refreshed order, as long as that order lags at most 2 revisions and its
+ * camera is within the pose envelope of the current one (translation at most
+ * 2% of the scene's bounding-box diagonal, rotation at most 0.01 rad);
+ * otherwise the frame refreshes (flag bit 7). Such frames are judged by the
+ * reference's stale-order image gate (mean |delta RGB| <= 1/255, at most
+ * 0.1% of pixels off by more than 3/255; render-wgpu/src/lib.rs:3491-3545).
+ * interval 0 -> INVALID_ARGUMENT.
+ */
+int32_t gsplat_context_set_sort_interval(GsplatContext *ctx, uint32_t interval);
+/* The next frame refreshes its order (SurfaceRenderSession::force_sort_refresh). */
+int32_t gsplat_context_force_sort_refresh(GsplatContext *ctx);
+/* Telemetry of the last frame in GsplatSurfaceSortStats (gsplat.h:69-84):
+ * camera_revision, applied_order_revision, presented_order_revision_lag;
+ * flags bit 0 order refreshed, bit 1 order uploaded, bit 7 refresh forced by
+ * the lag or pose bounds; the async fields stay 0. */
+int32_t gsplat_context_get_sort_stats(const GsplatContext *ctx, GsplatSurfaceSortStats *out_stats);
+
 /* ------------------------------------------------ parity/inspection hooks */
 
 /* Depth keys of the last frame for every splat, in splat order:

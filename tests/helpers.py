@@ -16,13 +16,7 @@ def cam_dict(c):
                 near=c.near_plane, far=c.far_plane)
 
 
-def rgba_parity_metrics(a8, b8):
-    """rgba_image_parity_metrics (render-wgpu/src/lib.rs:3491-3524): mean |d| over RGB,
-    fraction of pixels with any RGB channel above 3/255, max |d|."""
-    a = a8.reshape(-1, 4)[:, :3].astype(np.float64) / 255.0
-    b = b8.reshape(-1, 4)[:, :3].astype(np.float64) / 255.0
-    d = np.abs(a - b)
-    return d.mean() if d.size else 0.0, (d > 3.0 / 255.0).any(axis=1).mean() if d.size else 0.0, d.max(initial=0.0)
+from image_metrics import rgba_parity_metrics  # noqa: E402,F401  (the reference's image-parity metric)
 
 
 def minimal_ascii_rows():

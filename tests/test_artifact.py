@@ -146,3 +146,16 @@ def test_written_run_checks_and_is_written_once(tmp_path):
     with pytest.raises(BA.ArtifactError):
         BA.write_run(tmp_path / "out2", m2, frames, 3)
     assert not (tmp_path / "out2").exists()
+
+
+def test_image_parity_metric_counts_pixels_not_channels():
+    # render-wgpu/src/lib.rs:3539-3545: one of two pixels has one channel 4/255 off
+    import numpy as np
+    from image_metrics import passes_stale_order_gate, rgba_parity_metrics
+
+    direct = np.array([0, 0, 0, 255, 0, 0, 0, 255], np.uint8)
+    packed = np.array([4, 0, 0, 255, 0, 0, 0, 255], np.uint8)
+    mean, frac, mx = rgba_parity_metrics(direct, packed)
+    assert frac == 0.5 and abs(mean - 4 / 255 / 6) < 1e-12 and abs(mx - 4 / 255) < 1e-12
+    assert not passes_stale_order_gate(direct, packed)
+    assert passes_stale_order_gate(direct, direct)
