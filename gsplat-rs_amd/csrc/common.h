@@ -84,6 +84,30 @@ inline uint32_t soa_fields(uint32_t degree) { return kRest0 + 3 * sh_per_channel
 
 PreparedScene prepare_scene(const HostScene &s);
 
+// Packed scene codec (the reference's packed atlas, render-wgpu/src/
+// packed_atlas.rs:1-600, shaders/splat_surface_packed.wgsl:125-241): per splat
+// a 20-byte hot record -- u16 bounds-relative position, u8 opacity (alpha),
+// three u10 log-scales over the scene's range, a smallest-three quaternion
+// (10+10+10+2 bits), the DC colour baked to RGB10 -- and, for SH degrees 1..3,
+// a 48-byte sidecar of 45 i8 rest coefficients over per-channel scene scales.
+// Stored coefficient-major: word w of splat i at words[w * N + i].
+constexpr uint32_t kPackedHotWords = 5;
+constexpr uint32_t kPackedShWords = 12;
+constexpr uint32_t packed_words(uint32_t degree) { return kPackedHotWords + (degree ? kPackedShWords : 0); }
+// Scene-wide decode constants (device array of kPackedConsts floats):
+// bounds min xyz, bounds span xyz, log-scale min, log-scale span, SH scales
+// rgb, then the coverage cut of each of the 256 opacity codes.
+enum PackedConst : uint32_t { kPkMin = 0, kPkSpan = 3, kPkLogMin = 6, kPkLogSpan = 7, kPkShScale = 8, kPkCut = 16 };
+constexpr uint32_t kPackedConsts = kPkCut + 256;
+struct PackedScene {
+  uint32_t count = 0;
+  uint32_t sh_degree = 0;
+  std::vector<uint32_t> words;
+  std::vector<float> consts;  // kPackedConsts
+};
+// Throws Unsupported for SH degree 4 (the sidecar holds degrees up to 3).
+PackedScene pack_scene(const HostScene &s);
+
 // Largest r2 in [0,1] for which a splat of this alpha passes the fragment
 // test r2 <= 1 && alpha*exp(-4.5 r2) > 1/256 (splat_surface_direct.wgsl:358-367),
 // or -1 when none does.

@@ -233,6 +233,29 @@ int32_t gsplat_context_set_sort_interval(GsplatContext *ctx, uint32_t interval) 
   });
 }
 
+// gsplat_surface_renderer_set_geometry_path (ffi-c/src/lib.rs:1040-1087) for a
+// context: 0 direct, 1 packed atlas, 2 (paged active atlas) unsupported, any
+// other value INVALID_ARGUMENT "unsupported path".
+int32_t gsplat_context_set_geometry_path(GsplatContext *ctx, uint32_t path) {
+  static const char *op = "gsplat_context_set_geometry_path";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (path > 2) return fail(gs::Code::InvalidArgument, std::string(op) + ": unsupported path");
+    ctx->renderer->set_geometry_path(path);
+    return ok();
+  });
+}
+
+int32_t gsplat_context_get_geometry_path(const GsplatContext *ctx, uint32_t *out_path) {
+  static const char *op = "gsplat_context_get_geometry_path";
+  return guard(op, [&]() -> int32_t {
+    if (!ctx) return null_arg(op, "ctx");
+    if (!out_path) return null_arg(op, "out_path");
+    *out_path = ctx->renderer->geometry_path();
+    return ok();
+  });
+}
+
 int32_t gsplat_context_force_sort_refresh(GsplatContext *ctx) {
   static const char *op = "gsplat_context_force_sort_refresh";
   return guard(op, [&]() -> int32_t {
@@ -633,25 +656,21 @@ uint32_t gsplat_device_available(void) {
 
 namespace {
 
-int32_t make_scene(gs::HostScene &&s, GsplatScene **out) {
+int32_t make_scene(gs::HostScene &&s, GsplatScene **out, uint32_t geometry_path = 0) {
   if (!gs::device_present()) throw gs::Error(gs::Code::Unsupported, "gpu rasterizer unavailable");
   gs::validate_scene(s);
   auto h = std::make_unique<GsplatScene>();
-  h->scene = gs::upload_scene(s, nullptr);
+  h->scene = gs::upload_scene(s, nullptr, geometry_path);
   *out = h.release();
   return ok();
 }
 
-}  // namespace
-
-extern "C" {
-
-int32_t gsplat_scene_create(const GsplatSceneView *view, GsplatScene **out_scene) {
-  static const char *op = "gsplat_scene_create";
+int32_t create_scene(const char *op, const GsplatSceneView *view, uint32_t geometry_path, GsplatScene **out_scene) {
   return guard(op, [&]() -> int32_t {
     if (!out_scene) return null_arg(op, "out_scene");
     *out_scene = nullptr;
     if (!view) return null_arg(op, "scene");
+    if (geometry_path > 2) return fail(gs::Code::InvalidArgument, std::string(op) + ": unsupported path");
     gs::HostScene s;
     try {
       s = gs::scene_from_view(*view);
@@ -659,19 +678,69 @@ int32_t gsplat_scene_create(const GsplatSceneView *view, GsplatScene **out_scene
       if (e.code == gs::Code::Unsupported) throw;
       return fail(gs::Code::InvalidArgument, std::string(op) + ": invalid scene buffers");
     }
-    return make_scene(std::move(s), out_scene);
+    return make_scene(std::move(s), out_scene, geometry_path);
   });
 }
 
-int32_t gsplat_scene_create_from_path(const char *path, GsplatScene **out_scene) {
-  static const char *op = "gsplat_scene_create_from_path";
+int32_t create_scene_from_path(const char *op, const char *path, uint32_t geometry_path, GsplatScene **out_scene) {
   return guard(op, [&]() -> int32_t {
     if (!out_scene) return null_arg(op, "out_scene");
     *out_scene = nullptr;
     if (!path) return null_arg(op, "path");
+    if (geometry_path > 2) return fail(gs::Code::InvalidArgument, std::string(op) + ": unsupported path");
     if (!path_is_utf8(path)) return fail(gs::Code::InvalidArgument, std::string(op) + ": path is not valid UTF-8");
-    return make_scene(gs::ply_load(path, gs::PlyLimits()), out_scene);
+    return make_scene(gs::ply_load(path, gs::PlyLimits()), out_scene, geometry_path);
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gsplat_scene_create(const GsplatSceneView *view, GsplatScene **out_scene) {
+  return create_scene("gsplat_scene_create", view, 0, out_scene);
+}
+
+int32_t gsplat_scene_create_from_path(const char *path, GsplatScene **out_scene) {
+  return create_scene_from_path("gsplat_scene_create_from_path", path, 0, out_scene);
+}
+
+int32_t gsplat_pack_scene(const GsplatSceneView *view, uint32_t *out_words, uint64_t words_cap, float *out_consts,
+                          uint64_t consts_cap, uint32_t *out_words_per_splat) {
+  static const char *op = "gsplat_pack_scene";
+  static_assert(GSPLAT_PACKED_CONSTS == gs::kPackedConsts, "packed constant count");
+  return guard(op, [&]() -> int32_t {
+    if (!view) return null_arg(op, "scene");
+    if (!out_words_per_splat) return null_arg(op, "out_words_per_splat");
+    gs::HostScene s;
+    try {
+      s = gs::scene_from_view(*view);
+    } catch (const gs::Error &e) {
+      if (e.code == gs::Code::Unsupported) throw;
+      return fail(gs::Code::InvalidArgument, std::string(op) + ": invalid scene buffers");
+    }
+    const uint32_t degree = s.has_sh_rest ? s.sh_degree : 0;
+    if (degree > 3) return fail(gs::Code::Unsupported, std::string(op) + ": the packed scene codec holds SH degrees 0..3");
+    *out_words_per_splat = gs::packed_words(degree);
+    if (!out_words) return ok();
+    if (!out_consts) return null_arg(op, "out_consts");
+    if (words_cap < (uint64_t)gs::packed_words(degree) * s.count || consts_cap < gs::kPackedConsts)
+      return fail(gs::Code::InvalidArgument, std::string(op) + ": output buffers too small");
+    const gs::PackedScene p = gs::pack_scene(s);
+    std::memcpy(out_words, p.words.data(), p.words.size() * sizeof(uint32_t));
+    std::memcpy(out_consts, p.consts.data(), p.consts.size() * sizeof(float));
+    return ok();
+  });
+}
+
+int32_t gsplat_scene_create_with_geometry_path(const GsplatSceneView *view, uint32_t geometry_path,
+                                               GsplatScene **out_scene) {
+  return create_scene("gsplat_scene_create_with_geometry_path", view, geometry_path, out_scene);
+}
+
+int32_t gsplat_scene_create_from_path_with_geometry_path(const char *path, uint32_t geometry_path,
+                                                         GsplatScene **out_scene) {
+  return create_scene_from_path("gsplat_scene_create_from_path_with_geometry_path", path, geometry_path, out_scene);
 }
 
 int32_t gsplat_scene_get_info(const GsplatScene *scene, GsplatSceneInfo *out_info) {
@@ -682,6 +751,8 @@ int32_t gsplat_scene_get_info(const GsplatScene *scene, GsplatSceneInfo *out_inf
     out_info->count = scene->scene->count;
     out_info->sh_degree = scene->scene->sh_degree;
     out_info->device_bytes = scene->scene->bytes();
+    out_info->geometry_path = scene->scene->geometry_path;
+    out_info->reserved = 0;
     return ok();
   });
 }

@@ -44,14 +44,22 @@ constexpr uint32_t kXcds = 8;             // MI355X: workgroup b is dispatched t
 constexpr uint32_t kPairViewShift = 24;   // tile-pair key = view << 24 | tile of the view
 constexpr uint32_t kPairTileMask = (1u << kPairViewShift) - 1;
 
+// Scene format of a view: an fp32 SoA scene of SH degree d is format d
+// (0..4); a packed scene (common.h pack_scene) of SH degree d is
+// kPackedFormat + d (d = 0..3). Its device image at `soa` is then the word
+// array followed by the kPackedConsts decode constants.
+constexpr uint32_t kPackedFormat = 5;
+constexpr uint32_t kFormats = kPackedFormat + 4;
+inline uint32_t scene_format(uint32_t degree, bool packed) { return packed ? kPackedFormat + degree : degree; }
+
 // One view of a batch. Its splats own the slot segment [seg_off,
 // seg_off + seg_len) of every per-splat array (seg_len = n rounded up to
 // kSegmentAlign), its tiles own the global tile ids [tile_off, tile_off +
 // tiles), and its target starts at pixel img_off.
 struct ViewDev {
   FrameParams fp;
-  const float *soa;  // scene SoA (coefficient-major, see common.h)
-  uint32_t n, sh_degree;
+  const float *soa;  // scene image (coefficient-major, see common.h)
+  uint32_t n, format;
   uint32_t seg_off, seg_len;
   uint32_t tile_off, tiles;    // global 16x16 raster tiles
   uint32_t ctile_off, ctiles;  // global binning tiles
@@ -97,13 +105,13 @@ struct Clears {
 };
 
 // Depth key, cull, projection, SH colour and pixel bounds for every slot of
-// every view whose scene has SH degree `degree` (other views are skipped):
+// every view whose scene has scene format `format` (other views are skipped):
 // keys[g] = f32 bits of depth when near <= depth <= far else 0;
 // vals[g] = view << 24 | splat index (index 0xFFFFFF for padding slots);
 // rect[g] = pixel bounds x0|y0<<16, x1|y1<<16, or x = ~0u when the splat
 // covers no pixel.
 // The grid first zeroes `clears`.
-void launch_project(uint32_t degree, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,
+void launch_project(uint32_t format, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,
                     uint32_t total_slots, uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect,
                     const Clears &clears, hipStream_t stream);
 
@@ -114,22 +122,22 @@ void launch_project(uint32_t degree, const ViewDev *views, const uint32_t *seg_s
 constexpr uint32_t kGroupViews = 8;
 struct ProjGroup {
   const float *soa;  // the scene (all views of the group)
-  uint32_t n, sh_degree;
+  uint32_t n, format;
   uint32_t chunk_start, nv;
   uint32_t views[kGroupViews];
 };
 
 // As launch_project for the groups whose chunks are [chunk_lo, chunk_lo +
-// chunks) (all of SH degree `degree`): each splat's scene fields are read
-// once for all views of its group.
-void launch_project_groups(uint32_t degree, const ViewDev *views, const ProjGroup *groups, uint32_t ngroups,
+// chunks) (all of scene format `format`): each splat's scene fields are read
+// (and a packed record decoded) once for all views of its group.
+void launch_project_groups(uint32_t format, const ViewDev *views, const ProjGroup *groups, uint32_t ngroups,
                            uint32_t chunk_lo, uint32_t chunks, uint32_t *keys, uint32_t *vals, float4 *rec,
                            uint2 *rect, const Clears &clears, hipStream_t stream);
 
 // The same projection of one view's n splats exporting, per splat, the depth
 // key (keys_out, optional) and the reference quantities of project_splat +
 // eval_color (12 floats: drawn, centre, axis_u, axis_v, premultiplied rgba).
-void launch_project_export(uint32_t degree, const ViewDev *views, uint32_t view, uint32_t n, uint32_t *keys_out,
+void launch_project_export(uint32_t format, const ViewDev *views, uint32_t view, uint32_t n, uint32_t *keys_out,
                            float *export12, hipStream_t stream);
 
 // Scratch of the two-level exclusive scan.

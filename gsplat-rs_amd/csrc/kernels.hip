@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <type_traits>
 
 #include "common.h"
 #include "kernels.h"
@@ -94,66 +95,206 @@ __device__ __forceinline__ float cov_bilinear(const Cov6 &c, float a0, float a1,
   return a0 * bx + a1 * by + a2 * bz;
 }
 
-// One splat of eval_sh_rgb (wgsl:233-326) for one channel, in the shader's
-// left-to-right accumulation order. Coefficient k of the channel is field
-// base + k of the splat (`src`, see SoaSource).
+constexpr float kShC0 = 0.28209479177387814f;
+
+// One splat of eval_sh_rgb (wgsl:233-326) for channel ch, in the shader's
+// left-to-right accumulation order, from the degree-0 term C0 * dc
+// (src.dc_term) and the channel's coefficient k = src.rest(base + k).
 template <int DEG, class Src>
-__device__ __forceinline__ float sh_channel(float dc, const Src &src, uint32_t base, float x, float y, float z) {
-  const float C0 = 0.28209479177387814f;
-  float r = C0 * dc;
+__device__ __forceinline__ float sh_channel(const Src &src, uint32_t ch, float x, float y, float z) {
+  constexpr uint32_t PC = (DEG + 1) * (DEG + 1) - 1;
+  const uint32_t base = ch * PC;
+  float r = src.dc_term(ch);
   if constexpr (DEG >= 1) {
     const float C1 = 0.4886025119029199f;
-    r = r - C1 * y * src(base + 0) + C1 * z * src(base + 1) - C1 * x * src(base + 2);
+    r = r - C1 * y * src.rest(base + 0) + C1 * z * src.rest(base + 1) - C1 * x * src.rest(base + 2);
   }
   if constexpr (DEG >= 2) {
     const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
-    r = r + 1.0925484305920792f * xy * src(base + 3) + -1.0925484305920792f * yz * src(base + 4) +
-        0.31539156525252005f * (2.0f * zz - xx - yy) * src(base + 5) +
-        -1.0925484305920792f * xz * src(base + 6) + 0.5462742152960396f * (xx - yy) * src(base + 7);
+    r = r + 1.0925484305920792f * xy * src.rest(base + 3) + -1.0925484305920792f * yz * src.rest(base + 4) +
+        0.31539156525252005f * (2.0f * zz - xx - yy) * src.rest(base + 5) +
+        -1.0925484305920792f * xz * src.rest(base + 6) + 0.5462742152960396f * (xx - yy) * src.rest(base + 7);
     if constexpr (DEG >= 3) {
-      r = r + -0.5900435899266435f * y * (3.0f * xx - yy) * src(base + 8) +
-          2.890611442640554f * xy * z * src(base + 9) +
-          -0.4570457994644658f * y * (4.0f * zz - xx - yy) * src(base + 10) +
-          0.3731763325901154f * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * src(base + 11) +
-          -0.4570457994644658f * x * (4.0f * zz - xx - yy) * src(base + 12) +
-          1.445305721320277f * z * (xx - yy) * src(base + 13) +
-          -0.5900435899266435f * x * (xx - 3.0f * yy) * src(base + 14);
+      r = r + -0.5900435899266435f * y * (3.0f * xx - yy) * src.rest(base + 8) +
+          2.890611442640554f * xy * z * src.rest(base + 9) +
+          -0.4570457994644658f * y * (4.0f * zz - xx - yy) * src.rest(base + 10) +
+          0.3731763325901154f * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * src.rest(base + 11) +
+          -0.4570457994644658f * x * (4.0f * zz - xx - yy) * src.rest(base + 12) +
+          1.445305721320277f * z * (xx - yy) * src.rest(base + 13) +
+          -0.5900435899266435f * x * (xx - 3.0f * yy) * src.rest(base + 14);
     }
     if constexpr (DEG >= 4) {
-      r = r + 2.5033429417967046f * xy * (xx - yy) * src(base + 15) +
-          -1.7701307697799304f * yz * (3.0f * xx - yy) * src(base + 16) +
-          0.9461746957575601f * xy * (7.0f * zz - 1.0f) * src(base + 17) +
-          -0.6690465435572892f * yz * (7.0f * zz - 3.0f) * src(base + 18) +
-          0.10578554691520431f * (zz * (35.0f * zz - 30.0f) + 3.0f) * src(base + 19) +
-          -0.6690465435572892f * xz * (7.0f * zz - 3.0f) * src(base + 20) +
-          0.47308734787878004f * (xx - yy) * (7.0f * zz - 1.0f) * src(base + 21) +
-          -1.7701307697799304f * xz * (xx - 3.0f * yy) * src(base + 22) +
-          0.6258357354491761f * (xx * (xx - 3.0f * yy) - yy * (3.0f * xx - yy)) * src(base + 23);
+      r = r + 2.5033429417967046f * xy * (xx - yy) * src.rest(base + 15) +
+          -1.7701307697799304f * yz * (3.0f * xx - yy) * src.rest(base + 16) +
+          0.9461746957575601f * xy * (7.0f * zz - 1.0f) * src.rest(base + 17) +
+          -0.6690465435572892f * yz * (7.0f * zz - 3.0f) * src.rest(base + 18) +
+          0.10578554691520431f * (zz * (35.0f * zz - 30.0f) + 3.0f) * src.rest(base + 19) +
+          -0.6690465435572892f * xz * (7.0f * zz - 3.0f) * src.rest(base + 20) +
+          0.47308734787878004f * (xx - yy) * (7.0f * zz - 1.0f) * src.rest(base + 21) +
+          -1.7701307697799304f * xz * (xx - 3.0f * yy) * src.rest(base + 22) +
+          0.6258357354491761f * (xx * (xx - 3.0f * yy) - yy * (3.0f * xx - yy)) * src.rest(base + 23);
     }
   }
   return r;
 }
 
-// Scene fields (common.h SoA order) of splat i, read from the SoA where the
+// A splat's scene quantities as the projection reads them: position,
+// world covariance, alpha, coverage cut, the degree-0 colour term and the
+// SH rest coefficients (channel-major). Three sources provide them.
+
+// Fields of splat i of an fp32 SoA scene (common.h order), read where the
 // projection needs them (culled splats skip their covariance and SH loads).
+template <int DEG>
 struct SoaSource {
   const float *__restrict__ soa;
   size_t n;
   uint32_t i;
-  __device__ __forceinline__ float operator()(uint32_t f) const { return soa[f * n + i]; }
+  __device__ __forceinline__ float f(uint32_t k) const { return soa[k * n + i]; }
+  __device__ __forceinline__ float pos(uint32_t a) const { return f(kPx + a); }
+  __device__ __forceinline__ Cov6 cov() const { return {f(kCxx), f(kCxy), f(kCxz), f(kCyy), f(kCyz), f(kCzz)}; }
+  __device__ __forceinline__ float alpha() const { return f(kAlpha); }
+  __device__ __forceinline__ float cut() const { return f(kCut); }
+  __device__ __forceinline__ float dc_term(uint32_t ch) const { return kShC0 * f(kDcR + ch); }
+  __device__ __forceinline__ float rest(uint32_t k) const { return f(kRest0 + k); }
 };
 
-// All scene fields of one splat in registers: loaded once, projected for
-// every view of a group.
+// Decode of the packed scene codec (common.h pack_scene), following the
+// packed shader (splat_surface_packed.wgsl:125-241) and the codec's CPU
+// decoders (packed_atlas.rs:86-97, 184-189, 311-345, 409-419). Word q of splat
+// i is w[q * n + i]; pk holds the scene's decode constants. The codec is
+// lossy, and its divisions by 65535, 1023, 255 and 127 are multiplications by
+// the reciprocal here (within an ulp of the quotient, as GPU shading
+// languages allow for `/`).
+struct PackedWords {
+  const uint32_t *__restrict__ w;
+  const float *__restrict__ pk;
+  size_t n;
+  uint32_t i;
+  __device__ __forceinline__ uint32_t word(uint32_t q) const { return w[q * n + i]; }
+  // decode_position: min + (code / 65535) * extent.
+  __device__ __forceinline__ float pos(uint32_t a) const {
+    const uint32_t code = a == 0 ? (word(0) & 0xFFFFu) : a == 1 ? (word(0) >> 16) : (word(1) & 0xFFFFu);
+    return pk[kPkMin + a] + ((float)code * (1.0f / 65535.0f)) * pk[kPkSpan + a];
+  }
+  __device__ __forceinline__ uint32_t opacity_code() const { return (word(1) >> 16) & 0xFFu; }
+  // decode_opacity: clamp(code / 255, 1e-4, 1 - 1e-4).
+  __device__ __forceinline__ float alpha() const {
+    return fminf(fmaxf((float)opacity_code() * (1.0f / 255.0f), 1e-4f), 1.0f - 1e-4f);
+  }
+  __device__ __forceinline__ float cut() const { return pk[kPkCut + opacity_code()]; }
+  // world_covariance_from_scale_rotation of decode_log_scale and
+  // unpack_quat_smallest_three, with quat_to_mat3's rows.
+  __device__ __forceinline__ Cov6 cov() const {
+    const uint32_t sw = word(2), qw = word(3);
+    float s2[3];
+#pragma unroll
+    for (uint32_t a = 0; a < 3; ++a) {
+      const float ls = pk[kPkLogMin] + ((float)((sw >> (10 * a)) & 0x3FFu) * (1.0f / 1023.0f)) * pk[kPkLogSpan];
+      s2[a] = expf(2.0f * ls);
+    }
+    const float inv_sqrt2 = 0.70710678118654752440f;
+    float c[3];
+#pragma unroll
+    for (uint32_t l = 0; l < 3; ++l)
+      c[l] = ((float)((qw >> (2 + 10 * l)) & 0x3FFu) * (1.0f / 1023.0f) * 2.0f - 1.0f) * inv_sqrt2;
+    const float dropped = sqrtf(fmaxf(1.0f - (c[0] * c[0] + c[1] * c[1] + c[2] * c[2]), 0.0f));
+    const uint32_t big = qw & 3u;
+    float q[4];
+    q[0] = big == 0 ? dropped : c[0];
+    q[1] = big == 0 ? c[0] : big == 1 ? dropped : c[1];
+    q[2] = big <= 1 ? c[1] : big == 2 ? dropped : c[2];
+    q[3] = big == 3 ? dropped : c[2];
+    const float len2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    float x = 0.0f, y = 0.0f, z = 0.0f, ww = 1.0f;
+    if (len2 > 1e-20f) {
+      const float inv = 1.0f / sqrtf(len2);
+      x = q[0] * inv;
+      y = q[1] * inv;
+      z = q[2] * inv;
+      ww = q[3] * inv;
+    }
+    const float x2 = x + x, y2 = y + y, z2 = z + z;
+    const float xx = x * x2, yy = y * y2, zz = z * z2, xy = x * y2, xz = x * z2, yz = y * z2;
+    const float wx = ww * x2, wy = ww * y2, wz = ww * z2;
+    const float r0[3] = {1.0f - (yy + zz), xy - wz, xz + wy};
+    const float r1[3] = {xy + wz, 1.0f - (xx + zz), yz - wx};
+    const float r2[3] = {xz - wy, yz + wx, 1.0f - (xx + yy)};
+    auto term = [&](const float *a, const float *b) {
+      return s2[0] * a[0] * b[0] + s2[1] * a[1] * b[1] + s2[2] * a[2] * b[2];
+    };
+    return {term(r0, r0), term(r0, r1), term(r0, r2), term(r1, r1), term(r1, r2), term(r2, r2)};
+  }
+  // The RGB10 degree-0 bake, clamp(C0 * dc + 0.5, 0, 1), less the 0.5 the
+  // colour adds back.
+  __device__ __forceinline__ float dc_term(uint32_t ch) const {
+    return (float)((word(4) >> (10 * ch)) & 0x3FFu) * (1.0f / 1023.0f) - 0.5f;
+  }
+  // dequantize_sh_rest: byte ch * 15 + k of the 48-byte sidecar, q / 127 * scale.
+  template <int DEG>
+  __device__ __forceinline__ float rest_of(uint32_t k) const {
+    constexpr uint32_t PC = (DEG + 1) * (DEG + 1) - 1;
+    if constexpr (PC == 0) {
+      return 0.0f;
+    } else {
+      const uint32_t ch = k / PC, b = ch * 15 + k % PC;
+      const int8_t q = (int8_t)((word(kPackedHotWords + b / 4) >> (8 * (b % 4))) & 0xFFu);
+      return (float)q * (pk[kPkShScale + ch] * (1.0f / 127.0f));
+    }
+  }
+};
+
+template <int DEG>
+struct PackedSource : PackedWords {
+  __device__ __forceinline__ float rest(uint32_t k) const { return this->template rest_of<DEG>(k); }
+};
+
+// All quantities of one splat in registers: read (and decoded) once,
+// projected for every view of a group.
 template <int DEG>
 struct RegSource {
-  static constexpr uint32_t kFields = kRest0 + 3 * ((DEG + 1) * (DEG + 1) - 1);
-  float v[kFields];
-  __device__ __forceinline__ void load(const float *__restrict__ soa, size_t n, uint32_t i) {
+  static constexpr uint32_t kRest = 3 * ((DEG + 1) * (DEG + 1) - 1);
+  float p[3], a, ct, dc[3], r[kRest > 0 ? kRest : 1];
+  Cov6 c;
+  template <class Src>
+  __device__ __forceinline__ void load(const Src &s) {
 #pragma unroll
-    for (uint32_t f = 0; f < kFields; ++f) v[f] = soa[f * n + i];
+    for (uint32_t k = 0; k < 3; ++k) p[k] = s.pos(k);
+    c = s.cov();
+    a = s.alpha();
+    ct = s.cut();
+#pragma unroll
+    for (uint32_t k = 0; k < 3; ++k) dc[k] = s.dc_term(k);
+#pragma unroll
+    for (uint32_t k = 0; k < kRest; ++k) r[k] = s.rest(k);
   }
-  __device__ __forceinline__ float operator()(uint32_t f) const { return v[f]; }
+  __device__ __forceinline__ float pos(uint32_t k) const { return p[k]; }
+  __device__ __forceinline__ Cov6 cov() const { return c; }
+  __device__ __forceinline__ float alpha() const { return a; }
+  __device__ __forceinline__ float cut() const { return ct; }
+  __device__ __forceinline__ float dc_term(uint32_t ch) const { return dc[ch]; }
+  __device__ __forceinline__ float rest(uint32_t k) const { return r[k]; }
+};
+
+// Scene format FMT (kernels.h scene_format) as a source of splat i of view
+// or group `d` (anything with soa and n).
+template <uint32_t FMT>
+struct FormatOf {
+  static constexpr bool kPacked = FMT >= kPackedFormat;
+  static constexpr int kDeg = kPacked ? (int)(FMT - kPackedFormat) : (int)FMT;
+  using Source = typename std::conditional<kPacked, PackedSource<kDeg>, SoaSource<kDeg>>::type;
+  __device__ __forceinline__ static Source at(const float *soa, size_t n, uint32_t i) {
+    if constexpr (kPacked) {
+      const uint32_t *w = reinterpret_cast<const uint32_t *>(soa);
+      Source s;
+      s.w = w;
+      s.pk = reinterpret_cast<const float *>(w + (size_t)packed_words(kDeg) * n);
+      s.n = n;
+      s.i = i;
+      return s;
+    } else {
+      return Source{soa, n, i};
+    }
+  }
 };
 
 struct Projected {
@@ -168,11 +309,10 @@ struct Projected {
 // (eval_color, wgsl:328-334) of splat i of a scene.
 template <int DEG, class Src>
 __device__ __forceinline__ Projected project_one(const FrameParams &fp, const Src &src) {
-  constexpr uint32_t PC = (DEG + 1) * (DEG + 1) - 1;
   Projected o{};
-  const float relx = src(kPx) - fp.cam[0];
-  const float rely = src(kPy) - fp.cam[1];
-  const float relz = src(kPz) - fp.cam[2];
+  const float relx = src.pos(0) - fp.cam[0];
+  const float rely = src.pos(1) - fp.cam[1];
+  const float relz = src.pos(2) - fp.cam[2];
   // world_to_camera_depth_with_view_row (lib.rs:1346-1354), no FMA.
   const float z = fp.rows[6] * relx + fp.rows[7] * rely + fp.rows[8] * relz;
   const bool vis = z >= fp.near_plane && z <= fp.far_plane;
@@ -183,8 +323,7 @@ __device__ __forceinline__ Projected project_one(const FrameParams &fp, const Sr
   const float inv_z = 1.0f / z;
   const float x_ndc = ((x * fp.f) * inv_z) / fp.aspect;
   const float y_ndc = (y * fp.f) * inv_z;
-  const Cov6 wc{src(kCxx), src(kCxy), src(kCxz),
-                src(kCyy), src(kCyz), src(kCzz)};
+  const Cov6 wc = src.cov();
   const float *r0 = &fp.rows[0], *r1 = &fp.rows[3], *r2 = &fp.rows[6];
   // transform_covariance_terms_to_camera (wgsl:111-120).
   const Cov6 cc{cov_quadratic(wc, r0[0], r0[1], r0[2]),
@@ -239,7 +378,7 @@ __device__ __forceinline__ Projected project_one(const FrameParams &fp, const Sr
   o.uy = uy;
   o.vx = vx;
   o.vy = vy;
-  o.alpha = src(kAlpha);
+  o.alpha = src.alpha();
   // normalize3_or_default(position - camera) (wgsl:87-93, 329).
   float dxw = 0.0f, dyw = 0.0f, dzw = 1.0f;
   const float len2 = relx * relx + rely * rely + relz * relz;
@@ -250,9 +389,8 @@ __device__ __forceinline__ Projected project_one(const FrameParams &fp, const Sr
     dzw = relz * inv;
   }
 #pragma unroll
-  for (int ch = 0; ch < 3; ++ch) {
-    const float dc = src(kDcR + ch);
-    const float s = sh_channel<DEG>(dc, src, kRest0 + ch * PC, dxw, dyw, dzw);
+  for (uint32_t ch = 0; ch < 3; ++ch) {
+    const float s = sh_channel<DEG>(src, ch, dxw, dyw, dzw);
     o.col[ch] = fminf(fmaxf(s + 0.5f, 0.0f), 1.0f) * o.alpha;
   }
   return o;
@@ -286,7 +424,7 @@ __device__ __forceinline__ void write_slot(const FrameParams &fp, uint32_t v, ui
   vals[g] = (v << kLocalBits) | i;
   uint2 rr = make_uint2(0xFFFFFFFFu, 0u);
   if (p.drawn) {
-    const float cut = fminf(src(kCut), 1.0f);
+    const float cut = fminf(src.cut(), 1.0f);
     const float det = p.ux * p.vy - p.vx * p.uy;
     rec[3 * (size_t)g + 0] = make_float4(p.cx, p.cy, p.vy / det, -p.vx / det);
     rec[3 * (size_t)g + 1] = make_float4(-p.uy / det, p.ux / det, cut, p.alpha);
@@ -314,7 +452,7 @@ __device__ __forceinline__ void write_slot(const FrameParams &fp, uint32_t v, ui
   rect[g] = rr;
 }
 
-template <int DEG>
+template <uint32_t FMT>
 __global__ __launch_bounds__(kThreads) void k_project(const ViewDev *__restrict__ views,
                                                       const uint32_t *__restrict__ seg_start, uint32_t nviews,
                                                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
@@ -324,14 +462,14 @@ __global__ __launch_bounds__(kThreads) void k_project(const ViewDev *__restrict_
   const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
   const uint32_t v = find_view(seg_start, nviews, blockIdx.x * kThreads);
   const ViewDev &vw = views[v];
-  if (vw.sh_degree != (uint32_t)DEG) return;
+  if (vw.format != FMT) return;
   const uint32_t i = g - vw.seg_off;
   if (i >= vw.n) {
     write_padding(v, g, keys, vals, rect);
     return;
   }
-  const SoaSource src{vw.soa, vw.n, i};
-  write_slot(vw.fp, v, g, i, project_one<DEG>(vw.fp, src), src, keys, vals, rec, rect);
+  const auto src = FormatOf<FMT>::at(vw.soa, vw.n, i);
+  write_slot(vw.fp, v, g, i, project_one<FormatOf<FMT>::kDeg>(vw.fp, src), src, keys, vals, rec, rect);
 }
 
 // Views of one scene projected together: workgroup b takes chunk
@@ -339,7 +477,7 @@ __global__ __launch_bounds__(kThreads) void k_project(const ViewDev *__restrict_
 // range holds it; each thread loads its splat's fields once and writes its
 // slot in every view of the group (the slot layout and values are those of
 // k_project, so the results are the same bit for bit).
-template <int DEG>
+template <uint32_t FMT>
 __global__ __launch_bounds__(kThreads) void k_project_groups(const ViewDev *__restrict__ views,
                                                              const ProjGroup *__restrict__ groups,
                                                              uint32_t ngroups, uint32_t chunk_lo,
@@ -363,8 +501,9 @@ __global__ __launch_bounds__(kThreads) void k_project_groups(const ViewDev *__re
     }
     return;
   }
+  constexpr int DEG = FormatOf<FMT>::kDeg;
   RegSource<DEG> src;
-  src.load(gr.soa, gr.n, i);
+  src.load(FormatOf<FMT>::at(gr.soa, gr.n, i));
   for (uint32_t q = 0; q < nv; ++q) {
     const uint32_t v = gr.views[q];
     const ViewDev &vw = views[v];
@@ -372,14 +511,14 @@ __global__ __launch_bounds__(kThreads) void k_project_groups(const ViewDev *__re
   }
 }
 
-template <int DEG>
+template <uint32_t FMT>
 __global__ __launch_bounds__(kThreads) void k_project_export(const ViewDev *__restrict__ views, uint32_t v,
                                                              uint32_t *__restrict__ keys_out,
                                                              float *__restrict__ export12) {
   const ViewDev &vw = views[v];
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= vw.n) return;
-  const Projected p = project_one<DEG>(vw.fp, SoaSource{vw.soa, vw.n, i});
+  const Projected p = project_one<FormatOf<FMT>::kDeg>(vw.fp, FormatOf<FMT>::at(vw.soa, vw.n, i));
   if (keys_out) keys_out[i] = p.key;
   float *o = export12 + 12 * (size_t)i;
   // invalid_splat() (wgsl:122-129): zeros for a splat that is not drawn.
@@ -1218,50 +1357,57 @@ void set_bin_tiles(FrameParams &fp, uint32_t shift) {
   fp.ctiles_y = (fp.tiles_y + (1u << shift) - 1) >> shift;
 }
 
-void launch_project(uint32_t deg, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,
+namespace {
+
+// Calls f(std::integral_constant<uint32_t, F>) for the scene format F = fmt.
+template <class F>
+void with_format(uint32_t fmt, F &&f) {
+  switch (fmt) {
+    case 0: f(std::integral_constant<uint32_t, 0>()); break;
+    case 1: f(std::integral_constant<uint32_t, 1>()); break;
+    case 2: f(std::integral_constant<uint32_t, 2>()); break;
+    case 3: f(std::integral_constant<uint32_t, 3>()); break;
+    case 4: f(std::integral_constant<uint32_t, 4>()); break;
+    case kPackedFormat + 0: f(std::integral_constant<uint32_t, kPackedFormat + 0>()); break;
+    case kPackedFormat + 1: f(std::integral_constant<uint32_t, kPackedFormat + 1>()); break;
+    case kPackedFormat + 2: f(std::integral_constant<uint32_t, kPackedFormat + 2>()); break;
+    case kPackedFormat + 3: f(std::integral_constant<uint32_t, kPackedFormat + 3>()); break;
+    default: throw Error(Code::Unsupported, "unknown scene format " + std::to_string(fmt));
+  }
+}
+
+}  // namespace
+
+void launch_project(uint32_t fmt, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,
                     uint32_t total_slots, uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect, const Clears &clr,
                     hipStream_t st) {
   if (total_slots == 0 || nviews == 0) return;
   const dim3 grid(div_up(total_slots, kThreads)), block(kThreads);
-  switch (deg) {
-    case 0: k_project<0><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr); break;
-    case 1: k_project<1><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr); break;
-    case 2: k_project<2><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr); break;
-    case 3: k_project<3><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr); break;
-    case 4: k_project<4><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr); break;
-    default: throw Error(Code::Unsupported, "sh degree above 4");
-  }
+  with_format(fmt, [&](auto F) {
+    k_project<decltype(F)::value><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr);
+  });
   GS_HIP(hipGetLastError());
 }
 
-void launch_project_groups(uint32_t deg, const ViewDev *views, const ProjGroup *groups, uint32_t ngroups,
+void launch_project_groups(uint32_t fmt, const ViewDev *views, const ProjGroup *groups, uint32_t ngroups,
                            uint32_t chunk_lo, uint32_t chunks, uint32_t *keys, uint32_t *vals, float4 *rec,
                            uint2 *rect, const Clears &clr, hipStream_t st) {
   if (chunks == 0 || ngroups == 0) return;
   const dim3 grid(chunks), block(kThreads);
-  switch (deg) {
-    case 0: k_project_groups<0><<<grid, block, 0, st>>>(views, groups, ngroups, chunk_lo, keys, vals, rec, rect, clr); break;
-    case 1: k_project_groups<1><<<grid, block, 0, st>>>(views, groups, ngroups, chunk_lo, keys, vals, rec, rect, clr); break;
-    case 2: k_project_groups<2><<<grid, block, 0, st>>>(views, groups, ngroups, chunk_lo, keys, vals, rec, rect, clr); break;
-    case 3: k_project_groups<3><<<grid, block, 0, st>>>(views, groups, ngroups, chunk_lo, keys, vals, rec, rect, clr); break;
-    case 4: k_project_groups<4><<<grid, block, 0, st>>>(views, groups, ngroups, chunk_lo, keys, vals, rec, rect, clr); break;
-    default: throw Error(Code::Unsupported, "sh degree above 4");
-  }
+  with_format(fmt, [&](auto F) {
+    k_project_groups<decltype(F)::value>
+        <<<grid, block, 0, st>>>(views, groups, ngroups, chunk_lo, keys, vals, rec, rect, clr);
+  });
   GS_HIP(hipGetLastError());
 }
 
-void launch_project_export(uint32_t deg, const ViewDev *views, uint32_t view, uint32_t n, uint32_t *keys_out,
+void launch_project_export(uint32_t fmt, const ViewDev *views, uint32_t view, uint32_t n, uint32_t *keys_out,
                            float *export12, hipStream_t st) {
   if (n == 0) return;
   const dim3 grid(div_up(n, kThreads)), block(kThreads);
-  switch (deg) {
-    case 0: k_project_export<0><<<grid, block, 0, st>>>(views, view, keys_out, export12); break;
-    case 1: k_project_export<1><<<grid, block, 0, st>>>(views, view, keys_out, export12); break;
-    case 2: k_project_export<2><<<grid, block, 0, st>>>(views, view, keys_out, export12); break;
-    case 3: k_project_export<3><<<grid, block, 0, st>>>(views, view, keys_out, export12); break;
-    case 4: k_project_export<4><<<grid, block, 0, st>>>(views, view, keys_out, export12); break;
-    default: throw Error(Code::Unsupported, "sh degree above 4");
-  }
+  with_format(fmt, [&](auto F) {
+    k_project_export<decltype(F)::value><<<grid, block, 0, st>>>(views, view, keys_out, export12);
+  });
   GS_HIP(hipGetLastError());
 }
 

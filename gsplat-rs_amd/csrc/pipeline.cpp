@@ -92,11 +92,24 @@ bool device_present() {
   return hipGetDeviceCount(&n) == hipSuccess && n > 0;
 }
 
-std::shared_ptr<DeviceScene> upload_scene(const HostScene &s, hipStream_t stream) {
-  PreparedScene p = prepare_scene(s);
+std::shared_ptr<DeviceScene> upload_scene(const HostScene &s, hipStream_t stream, uint32_t geometry_path) {
+  if (geometry_path > 1) throw Error(Code::Unsupported, "geometry path " + std::to_string(geometry_path) + " is not built");
+  std::vector<float> image;
   auto d = std::make_shared<DeviceScene>();
-  d->count = p.count;
-  d->sh_degree = p.sh_degree;
+  d->geometry_path = geometry_path;
+  if (geometry_path == 1) {
+    PackedScene p = pack_scene(s);
+    d->count = p.count;
+    d->sh_degree = p.sh_degree;
+    image.resize(p.words.size() + p.consts.size());
+    std::memcpy(image.data(), p.words.data(), p.words.size() * sizeof(uint32_t));
+    std::memcpy(image.data() + p.words.size(), p.consts.data(), p.consts.size() * sizeof(float));
+  } else {
+    PreparedScene p = prepare_scene(s);
+    d->count = p.count;
+    d->sh_degree = p.sh_degree;
+    image = std::move(p.soa);
+  }
   if (s.count > 0 && s.positions.size() >= 3 * (size_t)s.count && s.scale_xyz.size() >= 3 * (size_t)s.count) {
     double c[3] = {0.0, 0.0, 0.0};
     for (uint32_t i = 0; i < s.count; ++i)
@@ -115,9 +128,9 @@ std::shared_ptr<DeviceScene> upload_scene(const HostScene &s, hipStream_t stream
     std::nth_element(ls.begin(), ls.begin() + ls.size() / 2, ls.end());
     d->median_scale = std::exp(ls[ls.size() / 2]);
   }
-  d->soa.ensure(p.soa.size());
-  if (!p.soa.empty()) {
-    GS_HIP(hipMemcpyAsync(d->soa.get(), p.soa.data(), p.soa.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+  d->soa.ensure(image.size());
+  if (d->count > 0) {
+    GS_HIP(hipMemcpyAsync(d->soa.get(), image.data(), image.size() * sizeof(float), hipMemcpyHostToDevice, stream));
     GS_HIP(hipStreamSynchronize(stream));
   }
   return d;
@@ -226,7 +239,7 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
     set_bin_tiles(d.fp, d.bin_shift);
     d.soa = s.scene->soa.get();
     d.n = n;
-    d.sh_degree = s.scene->sh_degree;
+    d.format = s.scene->format();
     d.seg_off = (uint32_t)slots;
     d.seg_len = (uint32_t)(((uint64_t)n + kSegmentAlign - 1) / kSegmentAlign * kSegmentAlign);
     d.tile_off = (uint32_t)tiles;
@@ -257,17 +270,17 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
 
   // Projection groups of a batch: views of one scene (at most kGroupViews
   // each) project together, reading the scene once. Groups are ordered by
-  // SH degree, so each degree's chunks form one range (one launch).
+  // scene format, so each format's chunks form one range (one launch).
   host_groups_.clear();
   for (auto &r : group_chunks_) r = {0, 0};
   if (multi) {
     uint32_t chunk = 0;
-    for (uint32_t deg = 0; deg <= 4; ++deg) {
+    for (uint32_t fmt = 0; fmt < kFormats; ++fmt) {
       const size_t first = host_groups_.size();
-      group_chunks_[deg].first = chunk;
+      group_chunks_[fmt].first = chunk;
       for (uint32_t v = 0; v < (uint32_t)table.size(); ++v) {
         const ViewDev &d = table[v];
-        if (d.sh_degree != deg) continue;
+        if (d.format != fmt) continue;
         ProjGroup *g = nullptr;
         for (size_t q = first; q < host_groups_.size() && !g; ++q)
           if (host_groups_[q].soa == d.soa && host_groups_[q].n == d.n && host_groups_[q].nv < kGroupViews)
@@ -276,7 +289,7 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
           ProjGroup ng{};
           ng.soa = d.soa;
           ng.n = d.n;
-          ng.sh_degree = deg;
+          ng.format = fmt;
           host_groups_.push_back(ng);
           g = &host_groups_.back();
         }
@@ -286,7 +299,7 @@ void Pipeline::enqueue(std::vector<ViewSpec> views, std::vector<uint8_t *> host_
         host_groups_[q].chunk_start = chunk;
         chunk += table[host_groups_[q].views[0]].seg_len / kThreads;
       }
-      group_chunks_[deg].second = chunk - group_chunks_[deg].first;
+      group_chunks_[fmt].second = chunk - group_chunks_[fmt].first;
     }
   }
 
@@ -402,7 +415,7 @@ void Pipeline::launch_frame() {
                                  (uint64_t)reuse_order_, (uint64_t)keep_order_};
     for (const ViewDev &d : host_views_) {
       key.push_back(reinterpret_cast<uintptr_t>(d.soa));
-      key.push_back(((uint64_t)d.n << 8) | d.sh_degree);
+      key.push_back(((uint64_t)d.n << 8) | d.format);
       key.push_back(((uint64_t)d.fp.width << 32) | d.fp.height);
       key.push_back(d.bin_shift);
     }
@@ -495,12 +508,12 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
     for (int c = 0; c < Clears::kRanges; ++c) launch_fill_u32(frame_clears.ptr[c], 0u, frame_clears.words[c], st);
   }
   if (ngroups == 0) {
-    launch_project(host_views_[0].sh_degree, views_d, seg_start_d, nviews, slots, keys_.get(), vals_.get(),
+    launch_project(host_views_[0].format, views_d, seg_start_d, nviews, slots, keys_.get(), vals_.get(),
                    rec_.get(), rect_.get(), frame_clears, st);
   } else {
-    for (uint32_t deg = 0; deg <= 4; ++deg)
-      if (group_chunks_[deg].second) {
-        launch_project_groups(deg, views_d, groups_d, ngroups, group_chunks_[deg].first, group_chunks_[deg].second,
+    for (uint32_t fmt = 0; fmt < kFormats; ++fmt)
+      if (group_chunks_[fmt].second) {
+        launch_project_groups(fmt, views_d, groups_d, ngroups, group_chunks_[fmt].first, group_chunks_[fmt].second,
                               keys_.get(), vals_.get(), rec_.get(), rect_.get(), frame_clears, st);
         frame_clears = Clears();  // once
       }
@@ -784,14 +797,14 @@ void Pipeline::export_projection(const DeviceScene &scene, const FrameParams &fp
   d.fp = fp;
   d.soa = scene.soa.get();
   d.n = n;
-  d.sh_degree = scene.sh_degree;
+  d.format = scene.format();
   GS_HIP(hipStreamSynchronize(stream_));
   GS_HIP(hipMemcpy(export_view_.get(), &d, sizeof(d), hipMemcpyHostToDevice));
   DevBuf<uint32_t> keys;
   DevBuf<float> proj;
   keys.ensure(n);
   proj.ensure(12ull * n);
-  launch_project_export(scene.sh_degree, export_view_.get(), 0, n, keys.get(), proj.get(), stream_);
+  launch_project_export(scene.format(), export_view_.get(), 0, n, keys.get(), proj.get(), stream_);
   if (keys_out)
     GS_HIP(hipMemcpyAsync(keys_out, keys.get(), (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
   if (proj_out)

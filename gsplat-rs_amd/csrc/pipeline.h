@@ -81,6 +81,10 @@ bool device_present();
 struct DeviceScene {
   uint32_t count = 0;
   uint32_t sh_degree = 0;
+  // Geometry path (gsplat.h GsplatGeometryPath): 0 the fp32 SoA of
+  // prepare_scene, 1 the packed records of pack_scene followed by their
+  // decode constants.
+  uint32_t geometry_path = 0;
   DevBuf<float> soa;
   // Host statistics for launch choices: centroid of the positions, RMS
   // distance from it, and the median over splats of the largest axis scale
@@ -88,9 +92,16 @@ struct DeviceScene {
   float center[3] = {0.0f, 0.0f, 0.0f};
   float radius = 0.0f;
   float median_scale = 0.0f;
-  uint64_t bytes() const { return (uint64_t)soa_fields(sh_degree) * count * sizeof(float); }
+  bool packed() const { return geometry_path == 1; }
+  uint32_t format() const { return scene_format(sh_degree, packed()); }
+  uint64_t bytes() const {
+    return packed() ? ((uint64_t)packed_words(sh_degree) * count + kPackedConsts) * sizeof(uint32_t)
+                    : (uint64_t)soa_fields(sh_degree) * count * sizeof(float);
+  }
 };
-std::shared_ptr<DeviceScene> upload_scene(const HostScene &s, hipStream_t stream);
+// Uploads `s` in geometry path `geometry_path` (0 direct, 1 packed; packed
+// refuses SH degree 4 with Unsupported).
+std::shared_ptr<DeviceScene> upload_scene(const HostScene &s, hipStream_t stream, uint32_t geometry_path = 0);
 
 // Device bytes a batch frame needs beyond its scenes: per-slot buffers,
 // targets, tile ranges and the initial tile-pair buffers.
@@ -280,7 +291,7 @@ class Pipeline {
   std::vector<uint8_t *> host_rgba8_;
   std::vector<ViewDev> host_views_;
   std::vector<ProjGroup> host_groups_;                 // batches only (empty for one view)
-  std::pair<uint32_t, uint32_t> group_chunks_[5] = {};  // per SH degree: first chunk, chunk count
+  std::pair<uint32_t, uint32_t> group_chunks_[kFormats] = {};  // per scene format: first chunk, chunk count
   uint32_t total_slots_ = 0, total_tiles_ = 0, total_ctiles_ = 0, total_rblocks_ = 0;
   uint64_t total_pixels_ = 0, total_splats_ = 0;
   uint32_t pair_cap_ = 0;

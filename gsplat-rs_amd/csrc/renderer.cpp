@@ -41,7 +41,7 @@ void Renderer::set_size(uint32_t width, uint32_t height) {
 void Renderer::load_scene(HostScene &&scene) {
   validate_scene(scene);
   pipe_.wait();
-  scene_ = upload_scene(scene, pipe_.stream());
+  scene_ = upload_scene(scene, pipe_.stream(), geometry_path_);
   // The pose envelope's translation bound: 2% of the bounding-box diagonal
   // (SurfaceRenderSession::new, surface_session.rs:659-674).
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -55,6 +55,20 @@ void Renderer::load_scene(HostScene &&scene) {
   policy_.set_scene_diagonal(std::sqrt(d2));
   host_scene_ = std::make_unique<HostScene>(std::move(scene));
   frame_valid_ = false;
+}
+
+void Renderer::set_geometry_path(uint32_t path) {
+  if (path > 2) throw Error(Code::InvalidArgument, "unsupported path");
+  if (path == 2) throw Error(Code::Unsupported, "the paged active atlas is not built");
+  if (path == geometry_path_) return;
+  if (host_scene_) {
+    pipe_.wait();
+    auto next = upload_scene(*host_scene_, pipe_.stream(), path);
+    scene_ = std::move(next);
+    policy_.force_refresh();
+    frame_valid_ = false;
+  }
+  geometry_path_ = path;
 }
 
 void Renderer::set_sort_interval(uint32_t interval) {

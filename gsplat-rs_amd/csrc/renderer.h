@@ -22,6 +22,11 @@ class Renderer {
   uint32_t height() const { return height_; }
   void load_scene(HostScene &&scene);
   const HostScene *scene() const { return host_scene_.get(); }
+  // Geometry path of the scene on the device (GsplatGeometryPath: 0 direct,
+  // 1 packed atlas). A loaded scene is uploaded again in the new path before
+  // the old one is released; a failure leaves path and scene unchanged.
+  void set_geometry_path(uint32_t path);
+  uint32_t geometry_path() const { return geometry_path_; }
 
   // Enqueues a full frame for `camera` and returns without waiting for the
   // device (as the reference's queue.submit, render-wgpu/src/lib.rs:2751).
@@ -66,6 +71,7 @@ class Renderer {
   float host_ms_ = 0.0f;
   OrderReusePolicy policy_;
   bool reuse_enabled_ = false;
+  uint32_t geometry_path_ = 0;
 };
 
 // Raster variant chosen by GSPLAT_RASTER_VARIANT: 0 per-lane exit, else the

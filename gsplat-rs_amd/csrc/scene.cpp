@@ -251,6 +251,134 @@ float coverage_cut(float alpha) {
   return out;
 }
 
+namespace {
+
+// Round half away from zero (Rust's f32::round) of a non-negative value.
+uint32_t round_u(float x) { return (uint32_t)std::round(x); }
+
+// SceneBounds::encode_u16 and LogScaleRange::encode_u10 (packed_atlas.rs:52-190).
+uint32_t quantize(float v, float lo, float span, float levels) {
+  const float t = std::min(std::max((v - lo) / span, 0.0f), 1.0f);
+  return round_u(t * levels);
+}
+
+// pack_quat_smallest_three (packed_atlas.rs:273-309): the largest component
+// is dropped (made positive), the other three map [-1/sqrt 2, 1/sqrt 2] to
+// 10 bits each above a 2-bit index.
+uint32_t pack_quat(const float *xyzw) {
+  float q[4] = {xyzw[0], xyzw[1], xyzw[2], xyzw[3]};
+  const float len2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+  if (len2 <= 1e-20f) {
+    q[0] = q[1] = q[2] = 0.0f;
+    q[3] = 1.0f;
+  } else {
+    const float inv = 1.0f / std::sqrt(len2);
+    for (float &c : q) c *= inv;
+  }
+  uint32_t big = 0;
+  for (uint32_t k = 1; k < 4; ++k)
+    if (std::fabs(q[k]) > std::fabs(q[big])) big = k;
+  if (q[big] < 0.0f)
+    for (float &c : q) c = -c;
+  const float scale = 0.70710678118654752440f;
+  uint32_t out = big, lane = 0;
+  for (uint32_t k = 0; k < 4; ++k) {
+    if (k == big) continue;
+    const float t = std::min(std::max((q[k] / scale) * 0.5f + 0.5f, 0.0f), 1.0f);
+    out |= (round_u(t * 1023.0f) & 0x3FFu) << (2 + 10 * lane++);
+  }
+  return out;
+}
+
+}  // namespace
+
+PackedScene pack_scene(const HostScene &s) {
+  PackedScene p;
+  p.count = s.count;
+  p.sh_degree = s.has_sh_rest ? s.sh_degree : 0;
+  if (p.sh_degree > 3) throw Error(Code::Unsupported, "the packed scene codec holds SH degrees 0..3");
+  const size_t n = s.count;
+  const uint32_t pc = p.sh_degree ? sh_per_channel(p.sh_degree) : 0;
+  p.consts.assign(kPackedConsts, 0.0f);
+  float *k = p.consts.data();
+  // SceneBounds::from_positions, LogScaleRange::from_scales, scene_sh_scales
+  // (packed_atlas.rs:52-78, 160-190, 439-462).
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float lmin = INFINITY, lmax = -INFINITY;
+  float shs[3] = {1e-6f, 1e-6f, 1e-6f};
+  for (size_t i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], s.positions[3 * i + a]);
+      hi[a] = std::max(hi[a], s.positions[3 * i + a]);
+      lmin = std::min(lmin, s.scale_xyz[3 * i + a]);
+      lmax = std::max(lmax, s.scale_xyz[3 * i + a]);
+    }
+    for (uint32_t ch = 0; ch < 3; ++ch)
+      for (uint32_t c = 0; c < pc; ++c) shs[ch] = std::max(shs[ch], std::fabs(s.sh_rest[i * 3 * pc + ch * pc + c]));
+  }
+  for (int a = 0; a < 3; ++a) {
+    if (!std::isfinite(lo[a]) || !std::isfinite(hi[a]) || std::fabs(hi[a] - lo[a]) < 1e-6f) {
+      lo[a] = 0.0f;
+      hi[a] = 1.0f;
+    }
+    k[kPkMin + a] = lo[a];
+    k[kPkSpan + a] = std::max(hi[a] - lo[a], 1e-6f);
+  }
+  if (!std::isfinite(lmin) || !std::isfinite(lmax)) {
+    lmin = -8.0f;
+    lmax = 8.0f;
+  } else if (std::fabs(lmax - lmin) < 1e-6f) {
+    lmax = lmin + 1e-3f;
+  }
+  k[kPkLogMin] = lmin;
+  k[kPkLogSpan] = std::max(lmax - lmin, 1e-6f);
+  for (int ch = 0; ch < 3; ++ch) k[kPkShScale + ch] = shs[ch];
+  // decode_opacity of the packed shader (splat_surface_packed.wgsl:130-133):
+  // alpha = clamp(code / 255, 1e-4, 1 - 1e-4); its coverage cut per code.
+  for (uint32_t c = 0; c < 256; ++c)
+    k[kPkCut + c] = coverage_cut(std::min(std::max((float)c / 255.0f, 1e-4f), 1.0f - 1e-4f));
+
+  const uint32_t words = packed_words(p.sh_degree);
+  p.words.assign((size_t)words * n, 0u);
+  uint32_t *w = p.words.data();
+  const float C0 = 0.28209479177387814f;
+  for (size_t i = 0; i < n; ++i) {
+    uint32_t pos[3];
+    for (int a = 0; a < 3; ++a) pos[a] = quantize(s.positions[3 * i + a], k[kPkMin + a], k[kPkSpan + a], 65535.0f);
+    // encode_opacity_u8 (packed_atlas.rs:268-271)
+    const float alpha = std::min(std::max(1.0f / (1.0f + std::exp(-s.opacity[i])), 0.0f), 1.0f);
+    const uint32_t op = round_u(alpha * 255.0f);
+    uint32_t scales = 0;
+    for (int a = 0; a < 3; ++a)
+      scales |= quantize(s.scale_xyz[3 * i + a], lmin, k[kPkLogSpan], 1023.0f) << (10 * a);
+    uint32_t rgb = 0;  // pack_color_rgb10 of the degree-0 bake (packed_atlas.rs:311-318, 517-524)
+    for (int a = 0; a < 3; ++a) {
+      const float v = std::min(std::max(C0 * s.color_dc[3 * i + a] + 0.5f, 0.0f), 1.0f);
+      rgb |= (round_u(v * 1023.0f) & 0x3FFu) << (10 * a);
+    }
+    w[0 * n + i] = pos[0] | (pos[1] << 16);
+    w[1 * n + i] = pos[2] | (op << 16);
+    w[2 * n + i] = scales;
+    w[3 * n + i] = pack_quat(&s.rotation_xyzw[4 * i]);
+    w[4 * n + i] = rgb;
+    if (pc) {  // quantize_sh_rest_with_scales (packed_atlas.rs:380-400): 15 lanes per channel
+      uint8_t b[48] = {};
+      for (uint32_t ch = 0; ch < 3; ++ch)
+        for (uint32_t c = 0; c < pc; ++c) {
+          const float v = s.sh_rest[i * 3 * pc + ch * pc + c] / shs[ch] * 127.0f;
+          const float r = std::min(std::max(std::round(v), -127.0f), 127.0f);
+          b[ch * 15 + c] = (uint8_t)(int8_t)r;
+        }
+      for (uint32_t q = 0; q < kPackedShWords; ++q) {
+        uint32_t word;
+        std::memcpy(&word, b + 4 * q, 4);
+        w[(kPackedHotWords + q) * n + i] = word;
+      }
+    }
+  }
+  return p;
+}
+
 // precompute_alpha_values (lib.rs:1251-1261), precompute_world_covariances
 // (lib.rs:1654-1674), and the coverage cut, into the device SoA image.
 PreparedScene prepare_scene(const HostScene &s) {

@@ -113,7 +113,8 @@ class GsplatPlyLimits(ctypes.Structure):
 
 
 class GsplatSceneInfo(ctypes.Structure):
-    _fields_ = [("count", ctypes.c_uint32), ("sh_degree", ctypes.c_uint32), ("device_bytes", ctypes.c_uint64)]
+    _fields_ = [("count", ctypes.c_uint32), ("sh_degree", ctypes.c_uint32), ("device_bytes", ctypes.c_uint64),
+                ("geometry_path", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class GsplatViewDesc(ctypes.Structure):
@@ -238,6 +239,11 @@ SIGNATURES = {
     "gsplat_camera_look_at": (GsplatCamera, [ctypes.POINTER(F32), ctypes.POINTER(F32), F32, F32, F32]),
     "gsplat_scene_create": (I32, [ctypes.POINTER(GsplatSceneView), ctypes.POINTER(P)]),
     "gsplat_scene_create_from_path": (I32, [ctypes.c_char_p, ctypes.POINTER(P)]),
+    "gsplat_scene_create_with_geometry_path": (I32, [ctypes.POINTER(GsplatSceneView), U32, ctypes.POINTER(P)]),
+    "gsplat_scene_create_from_path_with_geometry_path": (I32, [ctypes.c_char_p, U32, ctypes.POINTER(P)]),
+    "gsplat_pack_scene": (I32, [ctypes.POINTER(GsplatSceneView), P, U64, P, U64, PU32]),
+    "gsplat_context_set_geometry_path": (I32, [P, U32]),
+    "gsplat_context_get_geometry_path": (I32, [P, PU32]),
     "gsplat_scene_get_info": (I32, [P, ctypes.POINTER(GsplatSceneInfo)]),
     "gsplat_scene_destroy": (None, [P]),
     "gsplat_batch_create": (I32, [ctypes.POINTER(P)]),
@@ -391,6 +397,15 @@ class Context:
     def force_sort_refresh(self):
         check(lib().gsplat_context_force_sort_refresh(self.h))
 
+    def set_geometry_path(self, path: int):
+        """Device form of the context's scene (GEOMETRY_PATH_DIRECT or GEOMETRY_PATH_PACKED_ATLAS)."""
+        check(lib().gsplat_context_set_geometry_path(self.h, path))
+
+    def geometry_path(self) -> int:
+        out = ctypes.c_uint32()
+        check(lib().gsplat_context_get_geometry_path(self.h, ctypes.byref(out)))
+        return out.value
+
     def sort_stats(self) -> GsplatSurfaceSortStats:
         s = GsplatSurfaceSortStats()
         check(lib().gsplat_context_get_sort_stats(self.h, ctypes.byref(s)))
@@ -469,20 +484,29 @@ class Context:
         return out.reshape(oy.value, ox.value)
 
 
-class Scene:
-    """A scene resident in device memory (GsplatScene), shared by batches."""
+GEOMETRY_PATH_DIRECT, GEOMETRY_PATH_PACKED_ATLAS, GEOMETRY_PATH_PAGED_ACTIVE_ATLAS = 0, 1, 2
 
-    def __init__(self, scene=None, path=None):
+
+class Scene:
+    """A scene resident in device memory (GsplatScene), shared by batches.
+
+    geometry_path: GEOMETRY_PATH_DIRECT (fp32) or GEOMETRY_PATH_PACKED_ATLAS
+    (the packed 20-byte records plus the i8 SH sidecar).
+    """
+
+    def __init__(self, scene=None, path=None, geometry_path=GEOMETRY_PATH_DIRECT):
         h = P()
         if path is not None:
-            check(lib().gsplat_scene_create_from_path(str(path).encode(), ctypes.byref(h)))
+            check(lib().gsplat_scene_create_from_path_with_geometry_path(str(path).encode(), geometry_path,
+                                                                         ctypes.byref(h)))
         else:
             v = scene_view(scene)
-            check(lib().gsplat_scene_create(ctypes.byref(v), ctypes.byref(h)))
+            check(lib().gsplat_scene_create_with_geometry_path(ctypes.byref(v), geometry_path, ctypes.byref(h)))
         self.h = h
         info = GsplatSceneInfo()
         check(lib().gsplat_scene_get_info(self.h, ctypes.byref(info)))
         self.count, self.sh_degree, self.device_bytes = info.count, info.sh_degree, info.device_bytes
+        self.geometry_path = info.geometry_path
 
     def close(self):
         if self.h:
@@ -688,6 +712,25 @@ def sort_values_by_keys(keys, values) -> np.ndarray:
     check(lib().gsplat_sort_values_by_keys(k.ctypes.data if k.size else None, v.ctypes.data if v.size else None,
                                            k.size))
     return v
+
+
+PACKED_CONSTS = 272
+
+
+def pack_scene(scene):
+    """The packed records of a scene (gsplat_pack_scene): (words, consts).
+
+    words has shape (words_per_splat, count): word w of splat i at words[w, i].
+    """
+    v = scene_view(scene)
+    per = ctypes.c_uint32()
+    check(lib().gsplat_pack_scene(ctypes.byref(v), None, 0, None, 0, ctypes.byref(per)))
+    words = np.zeros((per.value, v.count), np.uint32)
+    consts = np.zeros(PACKED_CONSTS, np.float32)
+    if v.count:
+        check(lib().gsplat_pack_scene(ctypes.byref(v), words.ctypes.data, words.size, consts.ctypes.data, consts.size,
+                                      ctypes.byref(per)))
+    return words, consts
 
 
 class PlyScene:

@@ -104,6 +104,18 @@ typedef struct GsplatStageTimes {
 int32_t gsplat_context_get_stage_times(GsplatContext *ctx, GsplatStageTimes *out);
 int32_t gsplat_context_set_stage_timing(GsplatContext *ctx, uint32_t on);
 
+/* ------------------------------------------------------- geometry path */
+
+/*
+ * gsplat_surface_renderer_set_geometry_path (gsplat.h:149-157,
+ * ffi-c/src/lib.rs:1040-1087) for a context: the geometry path of its scene
+ * (as gsplat_scene_create_with_geometry_path). A loaded scene is uploaded
+ * again in the new path before the old one is released; on an error the
+ * context keeps its path and scene. Default GSPLAT_GEOMETRY_PATH_DIRECT.
+ */
+int32_t gsplat_context_set_geometry_path(GsplatContext *ctx, uint32_t path);
+int32_t gsplat_context_get_geometry_path(const GsplatContext *ctx, uint32_t *out_path);
+
 /* ------------------------------------------------- session order reuse */
 
 /*
@@ -222,11 +234,51 @@ typedef struct GsplatScene GsplatScene;
 typedef struct GsplatSceneInfo {
   uint32_t count;
   uint32_t sh_degree;
-  uint64_t device_bytes; /* HBM held by the scene's SoA */
+  uint64_t device_bytes;  /* HBM held by the scene's device image */
+  uint32_t geometry_path; /* GsplatGeometryPath the scene was created with */
+  uint32_t reserved;
 } GsplatSceneInfo;
 
 int32_t gsplat_scene_create(const GsplatSceneView *scene, GsplatScene **out_scene);
 int32_t gsplat_scene_create_from_path(const char *path, GsplatScene **out_scene);
+/*
+ * A resident scene in a chosen geometry path (the reference's
+ * gsplat_surface_renderer_create_*_with_geometry_path, gsplat.h:119-140, for
+ * a scene). GSPLAT_GEOMETRY_PATH_DIRECT (0) is the path above;
+ * GSPLAT_GEOMETRY_PATH_PACKED_ATLAS (1) holds the scene as the reference's
+ * packed records (render-wgpu/src/packed_atlas.rs): 20 bytes per splat
+ * (u16 bounds-relative position, u8 opacity, three u10 log-scales, a
+ * smallest-three quaternion, the RGB10 degree-0 colour) plus, for SH degrees
+ * 1..3, 48 bytes of i8 SH coefficients over per-channel scene scales, decoded
+ * by the projection. Lossy: its frames match the direct path's within the
+ * reference's packed-vs-direct gate (mean |dRGB| <= 1/255, at most 0.1% of
+ * pixels off by more than 3/255; render-wgpu/src/lib.rs:3520-3537,
+ * 3733-3771). SH degree 4 is GSPLAT_ERROR_UNSUPPORTED in the packed path,
+ * and so is GSPLAT_GEOMETRY_PATH_PAGED_ACTIVE_ATLAS (2); other values are
+ * GSPLAT_ERROR_INVALID_ARGUMENT.
+ */
+int32_t gsplat_scene_create_with_geometry_path(const GsplatSceneView *scene, uint32_t geometry_path,
+                                               GsplatScene **out_scene);
+int32_t gsplat_scene_create_from_path_with_geometry_path(const char *path, uint32_t geometry_path,
+                                                         GsplatScene **out_scene);
+/*
+ * The packed records of GSPLAT_GEOMETRY_PATH_PACKED_ATLAS for a scene, on
+ * the host (pack_scene, render-wgpu/src/packed_atlas.rs:420-426, 479-566):
+ * word w of splat i at out_words[w * count + i]; w 0..4 the hot record
+ * (x | y << 16, z | opacity << 16, three u10 log-scales, smallest-three
+ * quaternion, RGB10 colour), w 5..16 the 48-byte SH sidecar (byte
+ * 15 * channel + k is coefficient k of the channel as i8; SH degrees 1..3
+ * only). out_consts receives GSPLAT_PACKED_CONSTS floats: bounds min xyz,
+ * bounds span xyz, log-scale min, log-scale span, SH scales rgb, five zeros,
+ * then the coverage cut of each of the 256 opacity codes. *out_words_per_splat
+ * = 5 or 17. With out_words NULL only *out_words_per_splat is set; otherwise
+ * words_cap must hold words_per_splat * count words and consts_cap
+ * GSPLAT_PACKED_CONSTS floats (GSPLAT_ERROR_INVALID_ARGUMENT otherwise).
+ * SH degree 4: GSPLAT_ERROR_UNSUPPORTED. Needs no device.
+ */
+#define GSPLAT_PACKED_CONSTS 272
+int32_t gsplat_pack_scene(const GsplatSceneView *scene, uint32_t *out_words, uint64_t words_cap, float *out_consts,
+                          uint64_t consts_cap, uint32_t *out_words_per_splat);
 int32_t gsplat_scene_get_info(const GsplatScene *scene, GsplatSceneInfo *out_info);
 void gsplat_scene_destroy(GsplatScene *scene);
 

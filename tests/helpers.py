@@ -73,3 +73,49 @@ def lcg_keys_fast(n, seed):
         out[start:start + cnt] = vals
         s = int(vals[-1])
     return out.astype(np.uint32)
+
+
+def decode_packed(words, consts, sh_degree):
+    """The packed scene codec's CPU decoders (render-wgpu/src/packed_atlas.rs: SceneBounds::decode_u16
+    :86-97, LogScaleRange::decode_u10 :184-189, unpack_quat_smallest_three :311-337, unpack_color_rgb10
+    :350-357, dequantize_sh_rest :409-419) over coefficient-major words (gsplat_pack_scene)."""
+    f = np.float32
+    lo, span = consts[0:3], consts[3:6]
+    lmin, lspan = consts[6], consts[7]
+    pos = np.stack([words[0] & 0xFFFF, words[0] >> 16, words[1] & 0xFFFF], 1).astype(f)
+    pos = lo + (pos / f(65535.0)) * span
+    op = (words[1] >> 16) & 0xFF
+    log = np.stack([(words[2] >> (10 * a)) & 0x3FF for a in range(3)], 1).astype(f)
+    log = lmin + (log / f(1023.0)) * lspan
+    q = words[3]
+    comps = np.stack([((q >> (2 + 10 * l)) & 0x3FF).astype(f) / f(1023.0) * f(2.0) - f(1.0) for l in range(3)], 1)
+    comps = comps * np.float32(0.70710678118654752440)
+    dropped = np.sqrt(np.maximum(f(1.0) - (comps ** 2).sum(1), f(0.0)))
+    big = q & 3
+    quat = np.zeros((len(q), 4), f)
+    for k in range(len(q)):
+        others = [c for c in range(4) if c != big[k]]
+        quat[k, others] = comps[k]
+        quat[k, big[k]] = dropped[k]
+    quat /= np.sqrt((quat ** 2).sum(1, keepdims=True))
+    rgb = np.stack([(words[4] >> (10 * a)) & 0x3FF for a in range(3)], 1).astype(f) / f(1023.0)
+    rest = np.zeros((len(q), 45), f)
+    if sh_degree:
+        side = words[5:17].T.copy().view(np.int8).reshape(len(q), 48)[:, :45].astype(f)
+        rest = side / f(127.0) * np.repeat(consts[8:11], 15)[None, :]
+    return dict(pos=pos, op=op, log=log, quat=quat, rgb=rgb, rest=rest, cut=consts[16:16 + 256])
+
+
+def packed_decoded_scene(oracle, words, consts, sh_degree):
+    """An oracle scene holding what a packed scene decodes to: decoded positions, log-scales and
+    unit quaternions, the logit of the decoded alpha, the DC coefficient whose degree-0 term is the
+    decoded RGB10 colour less 0.5, and the dequantized SH coefficients."""
+    d = decode_packed(words, consts, sh_degree)
+    alpha = np.clip(d["op"].astype(np.float64) / 255.0, 1e-4, 1.0 - 1e-4)
+    opacity = np.log(alpha / (1.0 - alpha)).astype(np.float32)
+    dc = ((d["rgb"].astype(np.float64) - 0.5) / 0.28209479177387814).astype(np.float32)
+    rest = None
+    if sh_degree:
+        pc = (sh_degree + 1) ** 2 - 1
+        rest = np.ascontiguousarray(d["rest"].reshape(-1, 3, 15)[:, :, :pc].reshape(-1, 3 * pc))
+    return oracle.Scene(d["pos"], opacity, d["log"], d["quat"], dc, sh_degree, rest)
