@@ -19,6 +19,13 @@ and the served form (16 resident scenes totalling 10M splats, SH3):
                   pinned host memory
   served_closed64 64 clients each submitting its next request when the last
                   one returns
+and two forms of the 1080p scene:
+  session         a context on a slow orbit, sorting every frame and with
+                  session order reuse (sort interval 2), with the stale-order
+                  image gate of the reused frames
+  packed          the scene in the packed geometry path (20-byte records and
+                  the i8 SH sidecar): frames/s, stage times, device bytes and
+                  the packed-vs-direct image gate
 """
 
 from __future__ import annotations
@@ -50,26 +57,40 @@ def nearest_rank(sorted_vals, p):
     return sorted_vals[min(rank - 1, len(sorted_vals) - 1)]
 
 
+def host_cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(scene, view, cam):
-    """Oracle (CPU restatement of the reference path) on the host cores."""
+    """Oracle (CPU restatement of the reference path) on the host cores: per frame the preprocess and
+    the sort on one thread (as the reference's), the projection of the sorted instances and the blend
+    on `threads`; the per-splat alpha and covariance are computed once, as the reference does at load."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle_py as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     osc = O.Scene(scene.positions, scene.opacity, scene.scale_xyz, scene.rotation_xyzw, scene.color_dc,
                   scene.sh_degree, scene.sh_rest)
+    prepared = O.precompute(osc)
     frames, t0 = 0, time.perf_counter()
     while True:
-        O.render(osc, cam.as_dict(), view.width, view.height, threads=threads)
+        O.render_prepared(osc, prepared, cam.as_dict(), view.width, view.height, threads=threads)
         frames += 1
         if time.perf_counter() - t0 > 10.0 or frames >= 8:
             break
     el = time.perf_counter() - t0
     st = O.last_timings()
     return {"value": round(frames / el, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"{frames} full frames of the same workload (preprocess 1 thread, sort 1 thread, raster "
-                      f"{threads} threads); last frame ms: preprocess {st[0]:.1f}, sort {st[1]:.1f}, "
-                      f"project {st[2]:.1f}, raster {st[3]:.1f}"}
+            "cpu": host_cpu_model(),
+            "sample": f"{frames} full frames of the same workload (load-time alpha/covariance once; per frame "
+                      f"preprocess 1 thread, sort 1 thread, projection and blend {threads} threads); last frame "
+                      f"ms: preprocess {st[0]:.1f}, sort {st[1]:.1f}, project {st[2]:.1f}, raster {st[3]:.1f}"}
 
 
 def bench_plain_config(name, scene, view, frames, warmup):
@@ -561,6 +582,53 @@ def bench_session(scene, view, steps):
     return out
 
 
+def bench_packed(scene, view, cam, steps, warmup, direct_stage_ms):
+    """The 1080p workload with the scene in the packed geometry path: frames/s, the projection's device
+    time beside the fp32 path's, device bytes of the scene in both paths, and the packed-vs-direct
+    image gate (mean |delta RGB|, share of pixels over 3/255) of the frame."""
+    from image_metrics import rgba_parity_metrics
+
+    out = {}
+    frames = {}
+    for label, path in (("direct", A.GEOMETRY_PATH_DIRECT), ("packed", A.GEOMETRY_PATH_PACKED_ATLAS)):
+        ctx = A.Context(view.width, view.height)
+        ctx.set_geometry_path(path)
+        ctx.load(scene)
+        ctx.set_camera(cam)
+        ctx.render()
+        frames[label] = ctx.rgba8()
+        if path == A.GEOMETRY_PATH_PACKED_ATLAS:
+            ctx.set_stage_timing(False)
+            for _ in range(warmup):
+                ctx.render()
+            ctx.wait()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ctx.render()
+            ctx.wait()
+            out["frames_per_s"] = round(steps / (time.perf_counter() - t0), 3)
+            ctx.set_stage_timing(True)
+            st = []
+            for _ in range(min(steps, 60)):
+                ctx.render()
+                ctx.wait()
+                s = ctx.stage_times()
+                st.append((s.project_ms, s.sort_ms, s.bin_ms, s.raster_ms))
+            out["stage_ms"] = {nm: round(float(x), 4)
+                               for nm, x in zip(("project", "sort", "bin", "raster"), np.median(np.array(st), 0))}
+        ctx.close()
+    out["direct_project_ms"] = round(float(direct_stage_ms), 4)
+    for label, path in (("direct", A.GEOMETRY_PATH_DIRECT), ("packed", A.GEOMETRY_PATH_PACKED_ATLAS)):
+        d = A.Scene(scene, geometry_path=path)
+        out[f"{label}_scene_bytes"] = int(d.device_bytes)
+        d.close()
+    out["bytes_ratio"] = round(out["direct_scene_bytes"] / out["packed_scene_bytes"], 3)
+    mean, frac, worst = rgba_parity_metrics(frames["packed"], frames["direct"])
+    out["gate"] = {"mean_abs_rgb": round(mean, 6), "frac_over_3_255": round(frac, 6), "max_abs_rgb": round(worst, 6),
+                   "passes": bool(mean <= 1 / 255 and frac <= 0.001)}
+    return out
+
+
 def pct(vals):
     vals = sorted(vals)
     return {"p50": round(nearest_rank(vals, 0.5), 3), "p99": round(nearest_rank(vals, 0.99), 3),
@@ -674,6 +742,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="skip the batched and served sections")
+    ap.add_argument("--geometry-path", type=int, default=0, choices=(0, 1),
+                    help="device form of the plain section's scene: 0 fp32 SoA (the headline), 1 packed records")
     ap.add_argument("--served-requests", type=int, default=2000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (PMC counter calibration)")
@@ -717,6 +787,7 @@ def main():
     scene, view = synth.plain_1080p()
     cam = A.look_at(view.position, view.target)
     ctx = A.Context(view.width, view.height)
+    ctx.set_geometry_path(args.geometry_path)
     ctx.load(scene)
     ctx.set_camera(cam)
     # The timed frames carry no per-stage timer events (a context records
@@ -829,7 +900,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded 3-component Gaussian mixture, SH3)",
-        "config": {"workload": "plain-1080p-300k-sh3", "splats": n, "width": view.width, "height": view.height,
+        "config": {"workload": "plain-1080p-300k-sh3", "geometry_path": args.geometry_path, "splats": n,
+                   "width": view.width, "height": view.height,
                    "visible": v, "tile_pairs": pairs, "bin_pairs": bin_pairs, "bin_tile_px": bin_px,
                    "pair_sort_passes": pair_passes, "blend_evals": evals, "blend_tests": tests},
         "latency_ms": {"p50": round(nearest_rank(lat, 0.5), 4), "p99": round(nearest_rank(lat, 0.99), 4)},
@@ -859,6 +931,8 @@ def main():
         print(f"served {line['served']}", file=sys.stderr, flush=True)
         line["session"] = bench_session(scene, view, max(args.steps, 60))
         print(f"session {line['session']}", file=sys.stderr, flush=True)
+        line["packed"] = bench_packed(scene, view, cam, args.steps, args.warmup, med[0])
+        print(f"packed {line['packed']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, view, cam)
     print(json.dumps(line), flush=True)

@@ -556,14 +556,27 @@ uint32_t oracle_render(const OracleScene *s, const OracleCamera *cam, uint32_t w
   return oracle_render_window(s, cam, width, height, 0, 0, width, height, threads, rgba);
 }
 
-// The same frame restricted to pixels [wx0, wx1) x [wy0, wy1): the window is
-// written row-major into rgba ((wx1-wx0) * (wy1-wy0) * 4 floats), every
-// pixel computed exactly as the full render computes it.
-uint32_t oracle_render_window(const OracleScene *s, const OracleCamera *cam, uint32_t width, uint32_t height,
-                              uint32_t wx0, uint32_t wy0, uint32_t wx1, uint32_t wy1, int threads, float *rgba) {
+}  // extern "C"
+
+namespace {
+
+// A frame (window [wx0, wx1) x [wy0, wy1)) from the load-time values alpha
+// and cov6 of oracle_precompute (Renderer::load_scene computes them once,
+// lib.rs:499-527). The preprocess and the sort run on one thread, as the
+// reference's do; the projection of the sorted instances runs on `threads`
+// (the reference's rayon par_iter, lib.rs:1066-1101) and so does the blend.
+uint32_t render_window_prepared(const OracleScene *s, const float *alpha, const float *cov6,
+                                const OracleCamera *cam, uint32_t width, uint32_t height, uint32_t wx0,
+                                uint32_t wy0, uint32_t wx1, uint32_t wy1, int threads, float *rgba) {
   if (!oracle_camera_valid(cam) || width == 0 || height == 0) return UINT32_MAX;
   if (wx0 >= wx1 || wy0 >= wy1 || wx1 > width || wy1 > height) return UINT32_MAX;
   const uint32_t ww = wx1 - wx0, wh = wy1 - wy0;
+  int nthreads = 1;
+#ifdef _OPENMP
+  nthreads = threads > 0 ? threads : omp_get_max_threads();
+#else
+  (void)threads;
+#endif
   auto t0 = std::chrono::steady_clock::now();
   std::vector<uint32_t> keys(s->count), order(s->count);
   uint32_t vis = oracle_preprocess(s, cam, keys.data(), order.data());
@@ -576,21 +589,23 @@ uint32_t oracle_render_window(const OracleScene *s, const OracleCamera *cam, uin
   t0 = std::chrono::steady_clock::now();
   View v = make_view(cam);
   FrameConsts k = frame_consts(v, width, height);
-  std::vector<float> alpha(s->count), cov(6 * (size_t)s->count);
-  oracle_precompute(s, alpha.data(), cov.data());
   struct Inst {
     Proj p;
     InvA ia;
     float rgba[4];
     int x0, y0, x1, y1;
   };
-  std::vector<Inst> inst;
-  inst.reserve(vis);
+  // Instances in sorted order; one that is not drawn or reaches no pixel of
+  // the window keeps x0 > x1 and is dropped afterwards.
+  std::vector<Inst> all(vis);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
   for (uint32_t j = 0; j < vis; ++j) {
     uint32_t i = order[j];
-    Proj p = project_splat(v, k, s->positions + 3 * (size_t)i, cov.data() + 6 * (size_t)i, alpha[i]);
+    Inst &in = all[j];
+    in.x0 = 1;
+    in.x1 = 0;
+    Proj p = project_splat(v, k, s->positions + 3 * (size_t)i, cov6 + 6 * (size_t)i, alpha[i]);
     if (!p.valid) continue;
-    Inst in;
     in.p = p;
     in.ia = inverse_axes(p);
     eval_color(s, i, v, p.alpha, in.rgba);
@@ -613,21 +628,18 @@ uint32_t oracle_render_window(const OracleScene *s, const OracleCamera *cam, uin
     in.x1 = std::min(in.x1, (int)wx1 - 1);
     in.y0 = std::max(in.y0, (int)wy0);
     in.y1 = std::min(in.y1, (int)wy1 - 1);
-    if (in.x0 > in.x1 || in.y0 > in.y1) continue;
-    inst.push_back(in);
+    if (in.y0 > in.y1) in.x1 = in.x0 - 1;
   }
+  std::vector<Inst> inst;
+  inst.reserve(vis);
+  for (const Inst &in : all)
+    if (in.x0 <= in.x1) inst.push_back(in);
   g_timings[2] = ms_since(t0);
 
   // Clear to transparent black (lib.rs:2746), then premultiplied "over"
   // back to front: dst = src + dst*(1 - src.a) (draw_pass.rs:75).
   t0 = std::chrono::steady_clock::now();
   std::fill(rgba, rgba + (size_t)ww * wh * 4, 0.0f);
-  int nthreads = 1;
-#ifdef _OPENMP
-  nthreads = threads > 0 ? threads : omp_get_max_threads();
-#else
-  (void)threads;
-#endif
   int bands = nthreads * 4;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
   for (int band = 0; band < bands; ++band) {
@@ -657,6 +669,28 @@ uint32_t oracle_render_window(const OracleScene *s, const OracleCamera *cam, uin
   }
   g_timings[3] = ms_since(t0);
   return vis;
+}
+
+}  // namespace
+
+extern "C" {
+
+// The same frame restricted to pixels [wx0, wx1) x [wy0, wy1): the window is
+// written row-major into rgba ((wx1-wx0) * (wy1-wy0) * 4 floats), every
+// pixel computed exactly as the full render computes it.
+uint32_t oracle_render_window(const OracleScene *s, const OracleCamera *cam, uint32_t width, uint32_t height,
+                              uint32_t wx0, uint32_t wy0, uint32_t wx1, uint32_t wy1, int threads, float *rgba) {
+  std::vector<float> alpha(s->count), cov(6 * (size_t)s->count);
+  oracle_precompute(s, alpha.data(), cov.data());
+  return render_window_prepared(s, alpha.data(), cov.data(), cam, width, height, wx0, wy0, wx1, wy1, threads,
+                                rgba);
+}
+
+// oracle_render with the load-time values of oracle_precompute passed in.
+uint32_t oracle_render_prepared(const OracleScene *s, const float *alpha, const float *cov6,
+                                const OracleCamera *cam, uint32_t width, uint32_t height, int threads,
+                                float *rgba) {
+  return render_window_prepared(s, alpha, cov6, cam, width, height, 0, 0, width, height, threads, rgba);
 }
 
 // Front-to-back statement of the blend's work, for the raster roofline: per

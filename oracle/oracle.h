@@ -96,9 +96,17 @@ void oracle_tile_counts(uint32_t count, const float *proj12, uint32_t width, uin
 /* Full frame: preprocess, sort, project, back-to-front premultiplied "over"
  * into a cleared fp32 RGBA target (width*height*4 floats, rows top first).
  * Returns V (visible count) or UINT32_MAX on an invalid camera/config.
- * `threads` = OpenMP threads for the raster (0 = default). */
+ * `threads` = OpenMP threads for the projection of the sorted instances and
+ * the raster (0 = default); the preprocess and the sort take one thread, as
+ * the reference's do. */
 uint32_t oracle_render(const OracleScene *scene, const OracleCamera *camera, uint32_t width,
                        uint32_t height, int threads, float *rgba_out);
+/* oracle_render from the load-time values of oracle_precompute (alpha N,
+ * cov6 6N), which the reference computes once per scene load rather than per
+ * frame (render-wgpu/src/lib.rs:499-527). */
+uint32_t oracle_render_prepared(const OracleScene *scene, const float *alpha, const float *cov6,
+                                const OracleCamera *camera, uint32_t width, uint32_t height, int threads,
+                                float *rgba_out);
 /* oracle_render restricted to the pixel window [wx0, wx1) x [wy0, wy1),
  * written row-major into a (wx1-wx0) x (wy1-wy0) RGBA float buffer; each
  * pixel equals the full render's. UINT32_MAX for an empty or out-of-range

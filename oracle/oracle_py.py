@@ -52,6 +52,7 @@ def lib() -> ctypes.CDLL:
         _lib = ctypes.CDLL(str(_LIB_PATH))
         _lib.oracle_render.restype = ctypes.c_uint32
         _lib.oracle_render_window.restype = ctypes.c_uint32
+        _lib.oracle_render_prepared.restype = ctypes.c_uint32
         _lib.oracle_preprocess.restype = ctypes.c_uint32
         _lib.oracle_coverage_cut.restype = ctypes.c_float
         _lib.oracle_coverage_cut.argtypes = [ctypes.c_float]
@@ -168,6 +169,19 @@ def render(scene: Scene, cam, width: int, height: int, threads: int = 0):
     sc, c = scene.c(), make_camera(cam)
     vis = lib().oracle_render(ctypes.byref(sc), ctypes.byref(c), ctypes.c_uint32(width), ctypes.c_uint32(height),
                               ctypes.c_int(threads), _ptr(out))
+    if vis == 0xFFFFFFFF:
+        raise ValueError("invalid camera or size")
+    return out, int(vis)
+
+
+def render_prepared(scene: Scene, prepared, cam, width: int, height: int, threads: int = 0):
+    """render() from precompute(scene)'s load-time values (computed once per scene, as the reference does)."""
+    alpha, cov = prepared
+    out = np.zeros((height, width, 4), np.float32)
+    sc, c = scene.c(), make_camera(cam)
+    vis = lib().oracle_render_prepared(ctypes.byref(sc), _ptr(alpha), _ptr(cov), ctypes.byref(c),
+                                       ctypes.c_uint32(width), ctypes.c_uint32(height), ctypes.c_int(threads),
+                                       _ptr(out))
     if vis == 0xFFFFFFFF:
         raise ValueError("invalid camera or size")
     return out, int(vis)

@@ -266,6 +266,11 @@ def test_render_window_equals_full_crop(oracle):
     for bad in [(5, 5, 5, 9), (0, 0, w + 1, h), (3, 9, 8, 7)]:
         with pytest.raises(ValueError):
             oracle.render_window(s, default_cam(), w, h, *bad)
+    # Load-time values computed once give the same frame, on any thread count.
+    prepared = oracle.precompute(s)
+    for threads in (1, 3, 8):
+        img, v3 = oracle.render_prepared(s, prepared, default_cam(), w, h, threads=threads)
+        assert v3 == vis and np.array_equal(img.view(np.uint32), full.view(np.uint32)), threads
 
 
 def test_front_to_back_equals_back_to_front(oracle):
