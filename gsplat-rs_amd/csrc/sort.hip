@@ -1,14 +1,16 @@
 // sort.hip -- LSD radix sort of (u32 key, u32 value) pairs for gfx950, 8-bit
 // digits (or one 9-bit pass for 9-bit keys), 4096-item tiles of 512 threads x
-// 8 items (eight waves per tile: ranking is a chain of LDS rounds per wave, so
-// more waves per tile shorten it; thread t keeps the books of digit t, for
-// the 256 or 512 digits of a pass).
+// 8 items (eight waves per tile share a CU's four SIMDs; thread t keeps the
+// books of digit t, for the 256 or 512 digits of a pass).
 //
 // Every pass ranks a tile the same way: its eight waves work independently (a
 // wave owns 512 consecutive items and a private LDS histogram; ballot
-// match-any over the digit bits + popcount gives the rank among equal
-// digits), then the tile is staged in LDS in digit order and written out
-// coalesced. The passes differ in how a tile learns where its digits go:
+// match-any over the digit bits + popcount gives the rank among equal digits
+// of a 64-item round, and one returning LDS atomic per digit group the count
+// of earlier rounds), then the tile is staged in LDS in digit order and
+// written out coalesced. Ranking is VALU work (about five instructions per
+// digit bit and round), the rest memory latency. The passes differ in how a
+// tile learns where its digits go:
 //
 // Look-back mode (onesweep), for sorts of fewer tiles:
 //   k_os_clear   clears the histograms and the look-back state
@@ -67,7 +69,9 @@ constexpr uint32_t MAX_DIGIT_BITS = 9;  // 8-bit digits, or one 9-bit pass for 9
 static_assert(T >= (1u << MAX_DIGIT_BITS) && T % 64 == 0 && TILE % T == 0, "sort workgroup shape");
 constexpr uint32_t WAVE_ITEMS = TILE / WAVES;
 constexpr uint32_t MAX_PASSES = kSortMaxPasses;
-constexpr uint32_t HIST_BLOCKS = 512;
+constexpr uint32_t HIST_BLOCKS = 1024;
+constexpr uint32_t HIST_IPT = 2;                   // keys per thread of a histogram chunk
+constexpr uint32_t HIST_CHUNK = T * HIST_IPT;      // keys per histogram workgroup step
 constexpr uint32_t STRIPES = kSortStripes;
 constexpr uint32_t LOOKBACK = 8;  // predecessors polled per look-back step
 constexpr uint32_t kPassGrid = 768;   // look-back passes: resident 42 KB-LDS workgroups, 3 per CU
@@ -94,16 +98,21 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Lanes of the wave's active lanes whose digit equals this lane's: per bit,
+// the ballot of the bit, kept where it agrees with this lane's bit (x is the
+// bit spread to 0 or ~0, so agreement is ~(ballot ^ x)); the active mask is
+// applied once at the end.
 template <uint32_t DB>
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool active) {
-  uint64_t m = __ballot(active);
+  uint32_t lo = 0xFFFFFFFFu, hi = 0xFFFFFFFFu;
 #pragma unroll
   for (uint32_t b = 0; b < DB; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t bb = __ballot(active && bit);
-    m &= bit ? bb : ~bb;
+    const uint32_t x = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+    const uint64_t bb = __builtin_amdgcn_ballot_w64(x != 0u);
+    lo &= ~((uint32_t)bb ^ x);
+    hi &= ~((uint32_t)(bb >> 32) ^ x);
   }
-  return m;
+  return ((((uint64_t)hi) << 32) | lo) & __builtin_amdgcn_ballot_w64(active);
 }
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
@@ -153,10 +162,11 @@ __global__ __launch_bounds__(T) void k_os_clear(uint32_t *__restrict__ p, size_t
   for (size_t i = (size_t)blockIdx.x * T + threadIdx.x; i < n; i += (size_t)gridDim.x * T) p[i] = 0u;
 }
 
-// Workgroups take tiles of 4096 keys (8 per thread, all loaded before any
-// is counted) and count every pass's digits with LDS atomics; each
-// workgroup then adds its non-zero bins into one of STRIPES global copies
-// (copy = workgroup mod STRIPES) to spread the global atomics.
+// Workgroups take chunks of 1024 keys (2 per thread, loaded before any is
+// counted: small chunks spread a short sort over the whole chip) and count
+// every pass's digits with LDS atomics; each workgroup then adds its
+// non-zero bins into one of STRIPES global copies (copy = workgroup mod
+// STRIPES) to spread the global atomics.
 template <uint32_t DB>
 __global__ __launch_bounds__(T) void k_os_hist(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
                                                const uint32_t *__restrict__ n_dev, uint32_t n_cap, PassSet ps,
@@ -168,16 +178,16 @@ __global__ __launch_bounds__(T) void k_os_hist(const uint32_t *__restrict__ keys
   bool need_vals = false;
   for (uint32_t p = 0; p < ps.count; ++p) need_vals = need_vals || ps.from_value[p];
   const uint32_t n = load_count(n_dev, n_cap);
-  for (size_t base = (size_t)blockIdx.x * TILE; base < n; base += (size_t)gridDim.x * TILE) {
-    uint32_t k[IPT], v[IPT];
+  for (size_t base = (size_t)blockIdx.x * HIST_CHUNK; base < n; base += (size_t)gridDim.x * HIST_CHUNK) {
+    uint32_t k[HIST_IPT], v[HIST_IPT];
 #pragma unroll
-    for (uint32_t r = 0; r < IPT; ++r) {
+    for (uint32_t r = 0; r < HIST_IPT; ++r) {
       const size_t i = base + (size_t)r * T + threadIdx.x;
       k[r] = i < n ? keys[i] : 0u;
       v[r] = (need_vals && i < n) ? vals[i] : 0u;
     }
 #pragma unroll
-    for (uint32_t r = 0; r < IPT; ++r) {
+    for (uint32_t r = 0; r < HIST_IPT; ++r) {
       if (base + (size_t)r * T + threadIdx.x < n)
         for (uint32_t p = 0; p < ps.count; ++p)
           wave_count(h[p], digit<DB>(k[r], v[r], ps.shift[p], ps.flip[p], ps.from_value[p]));
@@ -204,16 +214,14 @@ __device__ __forceinline__ void clear_whist(uint32_t (*whist)[1u << DB]) {
 // the running per-digit count lives in whist[w]. On return whist[q][d] is the
 // start of wave q's digit-d items among the tile's digit-d items, and the
 // result is the tile's count of digit threadIdx.x (0 for threads past the
-// digits). whist must be zero.
-template <uint32_t DB>
-__device__ __forceinline__ uint32_t rank_tile(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
-                                              size_t base, uint32_t count, uint32_t shift, uint32_t flip,
-                                              uint32_t from_value, uint32_t (&key)[IPT], uint32_t (&val)[IPT],
-                                              uint32_t (&rank)[IPT], uint32_t (*whist)[1u << DB]) {
-  const uint32_t t = threadIdx.x, w = t / 64, lane = __lane_id();
-  uint32_t *wh = whist[w];
-  // All loads first: a round's LDS update would otherwise hold back the next
-  // round's loads, one memory latency per round.
+// digits). whist has WAVES + 1 rows; rows 0..WAVES-1 must be zero.
+// Tile items [base, base+count) in registers: item r of a thread is tile
+// item w*512 + r*64 + lane (w its wave). All loads are issued before any is
+// used: a round's LDS update would otherwise hold back the next round's
+// loads, one memory latency per round.
+__device__ __forceinline__ void load_tile(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                          size_t base, uint32_t count, uint32_t (&key)[IPT], uint32_t (&val)[IPT]) {
+  const uint32_t w = threadIdx.x / 64, lane = __lane_id();
 #pragma unroll
   for (uint32_t r = 0; r < IPT; ++r) {
     const uint32_t j = w * WAVE_ITEMS + r * 64 + lane;
@@ -221,6 +229,23 @@ __device__ __forceinline__ uint32_t rank_tile(const uint32_t *__restrict__ kin, 
     key[r] = active ? kin[base + j] : 0u;
     val[r] = active ? vin[base + j] : 0u;
   }
+}
+
+// rank_tile of items already loaded by load_tile.
+template <uint32_t DB>
+__device__ __forceinline__ uint32_t rank_loaded(uint32_t count, uint32_t shift, uint32_t flip, uint32_t from_value,
+                                                const uint32_t (&key)[IPT], const uint32_t (&val)[IPT],
+                                                uint32_t (&rank)[IPT], uint32_t (*whist)[1u << DB]) {
+  const uint32_t t = threadIdx.x, w = t / 64, lane = __lane_id();
+  uint32_t *wh = whist[w];
+  // Each round's leader (lowest lane of a digit group) adds the group's size
+  // to the wave's running count with an LDS atomic that returns the count
+  // before it; the group then takes that value from its leader. A wave's LDS
+  // operations complete in order, so the rounds' atomics issue back to back
+  // instead of each waiting on the last round's read and write. The other
+  // lanes add 0 to a word of their own in row WAVES of whist (whatever it
+  // holds), so the atomic needs no branch.
+  uint32_t old[IPT], leader[IPT];
 #pragma unroll
   for (uint32_t r = 0; r < IPT; ++r) {
     const uint32_t j = w * WAVE_ITEMS + r * 64 + lane;
@@ -228,9 +253,15 @@ __device__ __forceinline__ uint32_t rank_tile(const uint32_t *__restrict__ kin, 
     const uint32_t d = digit<DB>(key[r], val[r], shift, flip, from_value);
     const uint64_t peers = match_digit<DB>(d, active);
     const uint32_t lrank = (uint32_t)__popcll(peers & lanemask_lt());
-    const uint32_t before = active ? wh[d] : 0u;
-    rank[r] = active ? before + lrank : 0xFFFFFFFFu;
-    if (active && lrank == 0) wh[d] = before + (uint32_t)__popcll(peers);
+    const bool lead = active && lrank == 0;
+    old[r] = atomicAdd(lead ? &wh[d] : &whist[WAVES][lane], lead ? (uint32_t)__popcll(peers) : 0u);
+    leader[r] = active ? (uint32_t)__builtin_ctzll(peers) : lane;
+    rank[r] = active ? lrank : 0xFFFFFFFFu;
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < IPT; ++r) {
+    const uint32_t before = (uint32_t)__shfl((int)old[r], (int)leader[r], 64);
+    if (rank[r] != 0xFFFFFFFFu) rank[r] += before;
   }
   __syncthreads();
   // Thread t < 2^DB owns digit t: turn the wave counts into wave offsets.
@@ -244,6 +275,15 @@ __device__ __forceinline__ uint32_t rank_tile(const uint32_t *__restrict__ kin, 
     }
   }
   return run;
+}
+
+template <uint32_t DB>
+__device__ __forceinline__ uint32_t rank_tile(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                              size_t base, uint32_t count, uint32_t shift, uint32_t flip,
+                                              uint32_t from_value, uint32_t (&key)[IPT], uint32_t (&val)[IPT],
+                                              uint32_t (&rank)[IPT], uint32_t (*whist)[1u << DB]) {
+  load_tile(kin, vin, base, count, key, val);
+  return rank_loaded<DB>(count, shift, flip, from_value, key, val, rank, whist);
 }
 
 // Stages the ranked tile in LDS in digit order (lstart[d] = start of digit d
@@ -300,7 +340,7 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
   constexpr uint32_t D = 1u << DB;
   __shared__ uint32_t gofs[D];    // global start of this tile's items of each digit
   __shared__ uint32_t lstart[D];  // start of each digit inside this tile
-  __shared__ uint32_t whist[WAVES][D];
+  __shared__ uint32_t whist[WAVES + 1][D];  // row WAVES: rank_loaded's spare words
   __shared__ uint32_t skeys[TILE];
   __shared__ uint32_t svals[TILE];
 
@@ -308,6 +348,17 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
   const uint32_t n = load_count(n_dev, n_cap);
   const uint32_t tiles_n = (n + TILE - 1) / TILE;
   uint32_t *status = status_all + (size_t)pass * tiles_n * D;
+  // The first tile is claimed before anything else and its items are loaded
+  // together with the digit histogram: the claim, the histogram and the tile
+  // cost one memory round trip after another instead of three.
+  if (t == 0) s_tile = atomicAdd(counter, 1u);
+  clear_whist<DB>(whist);
+  __syncthreads();
+  uint32_t tile = s_tile;
+  size_t base = (size_t)tile * TILE;
+  uint32_t count = base < n ? (uint32_t)min((size_t)TILE, (size_t)n - base) : 0u;
+  uint32_t key[IPT], val[IPT], rank[IPT];
+  load_tile(kin, vin, base, count, key, val);
   uint32_t hd = 0;
   if (t < D) {
 #pragma unroll
@@ -317,69 +368,71 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
   // every key has the same digit: the pass is the identity
   const bool trivial = __syncthreads_or(n > 0 && hd == n) != 0;
 
-  for (;;) {
+  while (base < n) {  // uniform: no later tile has work either
+    if (trivial) {
+      const uint32_t w = t / 64, lane = __lane_id();
+#pragma unroll
+      for (uint32_t r = 0; r < IPT; ++r) {
+        const uint32_t j = w * WAVE_ITEMS + r * 64 + lane;
+        if (j < count) {
+          kout[base + j] = key[r];
+          vout[base + j] = val[r];
+        }
+      }
+    } else {
+      const uint32_t run = rank_loaded<DB>(count, shift, flip, from_value, key, val, rank, whist);
+
+      // Publish this tile's count of digit t, then look back for the prefix.
+      uint32_t excl = 0;
+      uint32_t *st = status + (size_t)tile * D + t;
+      if (t >= D) {
+      } else if (tile == 0) {
+        __hip_atomic_store(st, FLAG_INC | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __hip_atomic_store(st, FLAG_AGG | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int64_t j = (int64_t)tile - 1;  // next predecessor not yet consumed
+        while (j >= 0) {
+          uint32_t sv[LOOKBACK];
+#pragma unroll
+          for (uint32_t k = 0; k < LOOKBACK; ++k)
+            sv[k] = (j - (int64_t)k >= 0) ? __hip_atomic_load(status + (size_t)(j - k) * D + t, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT)
+                                          : FLAG_INC;
+          // Consume predecessors nearest first until an inclusive prefix
+          // (done) or an unpublished tile (poll again from there).
+          int64_t consumed = 0;
+          bool finished = false, stop = false;
+#pragma unroll
+          for (uint32_t k = 0; k < LOOKBACK; ++k) {
+            const uint32_t flag = sv[k] & ~VALUE_MASK;
+            const bool take = !stop && flag != 0;
+            excl += take ? (sv[k] & VALUE_MASK) : 0u;
+            consumed += take ? 1 : 0;
+            finished = finished || (take && flag == FLAG_INC);
+            stop = stop || !take || flag == FLAG_INC;
+          }
+          if (finished) break;
+          j -= consumed;
+        }
+        __hip_atomic_store(st, FLAG_INC | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      const uint32_t ls = block_exclusive_scan(run, lds4);
+      if (t < D) {
+        gofs[t] = g + excl;
+        lstart[t] = ls;
+      }
+      __syncthreads();
+      scatter_tile<DB>(key, val, rank, whist, lstart, gofs, count, shift, flip, from_value, skeys, svals, n, kout,
+                       vout);
+    }
+    __syncthreads();
     if (t == 0) s_tile = atomicAdd(counter, 1u);
     clear_whist<DB>(whist);
     __syncthreads();
-    const uint32_t tile = s_tile;
-    const size_t base = (size_t)tile * TILE;
-    if (base >= n) break;  // uniform: no later tile has work either
-    const uint32_t count = (uint32_t)min((size_t)TILE, (size_t)n - base);
-    if (trivial) {
-      for (uint32_t j = t; j < count; j += T) {
-        kout[base + j] = kin[base + j];
-        vout[base + j] = vin[base + j];
-      }
-      __syncthreads();
-      continue;
-    }
-
-    uint32_t key[IPT], val[IPT], rank[IPT];
-    const uint32_t run = rank_tile<DB>(kin, vin, base, count, shift, flip, from_value, key, val, rank, whist);
-
-    // Publish this tile's count of digit t, then look back for the prefix.
-    uint32_t excl = 0;
-    uint32_t *st = status + (size_t)tile * D + t;
-    if (t >= D) {
-    } else if (tile == 0) {
-      __hip_atomic_store(st, FLAG_INC | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      __hip_atomic_store(st, FLAG_AGG | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      int64_t j = (int64_t)tile - 1;  // next predecessor not yet consumed
-      while (j >= 0) {
-        uint32_t sv[LOOKBACK];
-#pragma unroll
-        for (uint32_t k = 0; k < LOOKBACK; ++k)
-          sv[k] = (j - (int64_t)k >= 0) ? __hip_atomic_load(status + (size_t)(j - k) * D + t, __ATOMIC_RELAXED,
-                                                            __HIP_MEMORY_SCOPE_AGENT)
-                                        : FLAG_INC;
-        // Consume predecessors nearest first until an inclusive prefix
-        // (done) or an unpublished tile (poll again from there).
-        int64_t consumed = 0;
-        bool finished = false, stop = false;
-#pragma unroll
-        for (uint32_t k = 0; k < LOOKBACK; ++k) {
-          const uint32_t flag = sv[k] & ~VALUE_MASK;
-          const bool take = !stop && flag != 0;
-          excl += take ? (sv[k] & VALUE_MASK) : 0u;
-          consumed += take ? 1 : 0;
-          finished = finished || (take && flag == FLAG_INC);
-          stop = stop || !take || flag == FLAG_INC;
-        }
-        if (finished) break;
-        j -= consumed;
-      }
-      __hip_atomic_store(st, FLAG_INC | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    const uint32_t ls = block_exclusive_scan(run, lds4);
-    if (t < D) {
-      gofs[t] = g + excl;
-      lstart[t] = ls;
-    }
-    __syncthreads();
-    scatter_tile<DB>(key, val, rank, whist, lstart, gofs, count, shift, flip, from_value, skeys, svals, n, kout,
-                     vout);
-    __syncthreads();
+    tile = s_tile;
+    base = (size_t)tile * TILE;
+    count = base < n ? (uint32_t)min((size_t)TILE, (size_t)n - base) : 0u;
+    if (base < n) load_tile(kin, vin, base, count, key, val);
   }
 }
 
@@ -549,7 +602,7 @@ __global__ __launch_bounds__(T) void k_rts_down(const uint32_t *__restrict__ kin
   constexpr uint32_t D = 1u << DB;
   __shared__ uint32_t dst[D];
   __shared__ uint32_t lstart[D];
-  __shared__ uint32_t whist[WAVES][D];
+  __shared__ uint32_t whist[WAVES + 1][D];  // row WAVES: rank_loaded's spare words
   __shared__ uint32_t skeys[TILE];
   __shared__ uint32_t svals[TILE];
   const uint32_t t = threadIdx.x;
@@ -574,7 +627,9 @@ __global__ __launch_bounds__(T) void k_rts_down(const uint32_t *__restrict__ kin
 
 uint32_t div_up(size_t a, size_t b) { return (uint32_t)((a + b - 1) / b); }
 
-uint32_t hist_blocks(uint32_t n_cap) { return std::max<uint32_t>(std::min<uint32_t>(div_up(n_cap, TILE), HIST_BLOCKS), 1); }
+uint32_t hist_blocks(uint32_t n_cap) {
+  return std::max<uint32_t>(std::min<uint32_t>(div_up(n_cap, HIST_CHUNK), HIST_BLOCKS), 1);
+}
 
 }  // namespace
 
