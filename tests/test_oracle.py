@@ -249,6 +249,36 @@ def test_single_splat_analytic_image(oracle):
     assert np.abs(img - exp)[~border].max() < 1e-5
 
 
+@pytest.mark.parametrize("cx,cy", [(0.0, 0.0), (-0.148, 0.31), (0.5, -0.62), (0.853, 0.853)])
+def test_per_pixel_quadratic_form_matches_float64(oracle, cx, cy):
+    # The oracle's per-pixel r^2 (evaluated relative to 128-pixel block
+    # origins, centre terms folded first) against |A^-1 (p - c)|^2 in
+    # float64 from the same projected axes: small anisotropic splats placed
+    # on and across block boundaries of a 300x300 frame (blocks at 0, 128, 256).
+    w = 300
+    z = 2.0
+    f = 1.0 / math.tan(float(DEFAULT_FOV) / 2)
+    s = oracle.Scene([[cx * z / f, cy * z / f, z]], [2.0], [[-4.2, -5.0, -4.6]], [[0.2, -0.4, 0.3, 0.85]],
+                     [[0.3, -0.2, 0.1]])
+    img, vis = oracle.render(s, default_cam(), w, w)
+    p = oracle.project(s, default_cam(), w, w)[0].astype(np.float64)
+    assert vis == 1 and p[0] == 1
+    A = np.array([[p[3], p[5]], [p[4], p[6]]])  # columns axis_u, axis_v
+    Ai = np.linalg.inv(A)
+    yy, xx = np.mgrid[0:w, 0:w]
+    d = np.stack([2 * (xx + 0.5) / w - 1 - p[1], 1 - 2 * (yy + 0.5) / w - p[2]], -1)
+    l = d @ Ai.T
+    r2 = (l ** 2).sum(-1)
+    a = 1 / (1 + math.exp(-2.0))
+    g = np.exp(-4.5 * r2)
+    cover = (r2 <= 1) & (a * g > 1 / 256)
+    border = np.abs(np.sqrt(r2) - math.sqrt(min(1.0, math.log(256 * a) / 4.5))) < 2e-3
+    assert cover.sum() > 20
+    got_a = img[..., 3]
+    exp_a = np.where(cover, a * g, 0)
+    assert np.abs(got_a - exp_a)[~border].max() < 2e-5
+
+
 def test_render_window_equals_full_crop(oracle):
     # A window render computes each pixel exactly as the full render does.
     rng = np.random.default_rng(5)
