@@ -342,9 +342,13 @@ def summarize_batch_profiles(name, trace_dir, fetch_dir, write_dir, out_dir, ste
     print(f"wrote {path}", file=sys.stderr)
 
 
-def summarize_counters(pmc_dir, out_dir, counters):
-    """rocprofv3 --pmc CSV of the plain workload -> profiles/sq_counters_plain-1080p-300k-sh3.md: the
-    per-dispatch average of each counter for every kernel of the frame."""
+def summarize_counters(pmc_dir, out_dir, counters, lds=False):
+    """rocprofv3 --pmc CSV of the plain workload -> profiles/sq_counters_plain-1080p-300k-sh3.md (or,
+    with `lds`, lds_valu_plain-1080p-300k-sh3.md plus its .json): the per-dispatch average of each
+    counter for every kernel of the frame. The LDS pass also derives, per kernel, LDS utilisation
+    (SQ_LDS_IDX_ACTIVE over CU cycles: the share of cycles the CU's LDS pipe serves indexed
+    operations) and VALU busy (SQ_ACTIVE_INST_VALU over CU cycles), with the kernel's cycles taken as
+    GRBM_GUI_ACTIVE per XCD (rocprofv3 sums the 8 XCDs' GRBM counters) and 256 CUs."""
     import csv
     import collections
 
@@ -353,24 +357,54 @@ def summarize_counters(pmc_dir, out_dir, counters):
     for r in csv.DictReader(open(path)):
         acc[_kernel_short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
     names = sorted({c for d in acc.values() for c in d})
-    lines = ["# SQ counters: plain-1080p-300k-sh3 (300,000 splats SH3, 1920x1080)", "",
+    stem = "lds_valu_plain-1080p-300k-sh3" if lds else "sq_counters_plain-1080p-300k-sh3"
+    flag = "--summarize-lds-counters" if lds else "--summarize-counters"
+    title = "LDS and VALU utilisation" if lds else "SQ counters"
+    derived = ["LdsUtil", "VALUBusy"] if lds else []
+    lines = [f"# {title}: plain-1080p-300k-sh3 (300,000 splats SH3, 1920x1080)", "",
              "Command (from /tmp, TMPDIR=/tmp):", "",
              f"    rocprofv3 --pmc {counters} --output-format csv -d DIR -o plain -- \\",
              "        python3 bench.py --plain-only --steps 8 --warmup 2 --no-cpu-baseline",
-             "", f"Summarised with `python3 bench.py --summarize-counters DIR`. Per-dispatch averages.", "",
-             "| kernel | dispatches | " + " | ".join(names) + " |", "|---|---:|" + "---:|" * len(names)]
-    for k in sorted(acc, key=lambda k: -sum(acc[k].get("SQ_WAVE_CYCLES", [0]))):
+             "", f"Summarised with `python3 bench.py {flag} DIR`. Per-dispatch averages."
+             + (" LdsUtil = SQ_LDS_IDX_ACTIVE / (256 CUs x GRBM_GUI_ACTIVE / 8 XCDs); VALUBusy = "
+                "SQ_ACTIVE_INST_VALU / (256 x GRBM_GUI_ACTIVE / 8)." if lds else ""), "",
+             "| kernel | dispatches | " + " | ".join(names + derived) + " |",
+             "|---|---:|" + "---:|" * (len(names) + len(derived))]
+    summary = {}
+    for k in sorted(acc, key=lambda k: -sum(acc[k].get("SQ_WAVE_CYCLES", acc[k].get("GRBM_GUI_ACTIVE", [0])))):
         d = acc[k]
         n = max(len(v) for v in d.values())
-        lines.append(f"| {k} | {n} | " + " | ".join(f"{sum(d[c]) / len(d[c]):.0f}" if d.get(c) else "" for c in names)
-                     + " |")
+        m = {c: sum(d[c]) / len(d[c]) for c in names if d.get(c)}
+        cells = [f"{m[c]:.0f}" if c in m else "" for c in names]
+        if lds and m.get("GRBM_GUI_ACTIVE"):
+            cyc = 256.0 * m["GRBM_GUI_ACTIVE"] / 8.0
+            util = {"lds_util": m.get("SQ_LDS_IDX_ACTIVE", 0.0) / cyc,
+                    "valu_busy": m.get("SQ_ACTIVE_INST_VALU", 0.0) / cyc}
+            summary[k] = {key: round(v, 4) for key, v in util.items()}
+            cells += [f"{100 * util['lds_util']:.1f}%", f"{100 * util['valu_busy']:.1f}%"]
+        elif lds:
+            cells += ["", ""]
+        lines.append(f"| {k} | {n} | " + " | ".join(cells) + " |")
     out = pathlib.Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
-    (out / "sq_counters_plain-1080p-300k-sh3.md").write_text("\n".join(lines) + "\n")
+    (out / f"{stem}.md").write_text("\n".join(lines) + "\n")
+    if lds:
+        (out / f"{stem}.json").write_text(json.dumps({"counters": counters, "kernels": summary}, indent=1) + "\n")
 
 
 SQ_COUNTERS = ("SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_LDS SQ_ACTIVE_INST_VALU SQ_WAIT_INST_LDS "
                "SQ_WAVE_CYCLES SQ_BUSY_CYCLES")
+LDS_COUNTERS = ("SQ_LDS_IDX_ACTIVE SQ_LDS_BANK_CONFLICT SQ_INSTS_LDS SQ_WAIT_INST_LDS SQ_ACTIVE_INST_LDS "
+                "SQ_ACTIVE_INST_VALU SQ_INSTS_VALU SQ_BUSY_CYCLES GRBM_GUI_ACTIVE")
+UTIL_FILE = ROOT / "profiles" / "lds_valu_plain-1080p-300k-sh3.json"
+
+
+def load_utilisation():
+    """LDS utilisation and VALU busy per kernel from the committed rocprofv3 summary; {} when absent."""
+    try:
+        return json.loads(UTIL_FILE.read_text()).get("kernels", {})
+    except (OSError, ValueError):
+        return {}
 
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_plain-1080p-300k-sh3.json"
 
@@ -753,6 +787,9 @@ def main():
     ap.add_argument("--summarize-counters", default=None, metavar="DIR",
                     help=f"write profiles/sq_counters_plain-1080p-300k-sh3.md from a rocprofv3 --pmc run of the "
                          f"plain workload with the counters {SQ_COUNTERS!r} and exit")
+    ap.add_argument("--summarize-lds-counters", default=None, metavar="DIR",
+                    help="summarise a rocprofv3 --pmc directory of the plain workload with the counters "
+                         f"{LDS_COUNTERS!r} (LDS utilisation and VALU busy per kernel) and exit")
     ap.add_argument("--profile-batch", default=None, metavar="WORKLOAD",
                     help=f"only render --steps batches of one workload ({', '.join(sorted(BATCH_WORKLOADS))}), "
                          "for rocprofv3")
@@ -766,6 +803,9 @@ def main():
         return
     if args.summarize_counters:
         summarize_counters(args.summarize_counters, args.profiles_out, SQ_COUNTERS)
+        return
+    if args.summarize_lds_counters:
+        summarize_counters(args.summarize_lds_counters, args.profiles_out, LDS_COUNTERS, lds=True)
         return
     if args.summarize_batch_profiles:
         summarize_batch_profiles(*args.summarize_batch_profiles, out_dir=args.profiles_out, steps=args.steps,
@@ -879,14 +919,19 @@ def main():
         "k_raster": {"bound": "valu", "achieved": raster_flop / (med[3] * 1e-3) / 1e12,
                      "peak": MI355X_FP32_TFLOPS, "unit": "TFLOP/s", "algorithmic": raster_flop},
     }
+    util = load_utilisation()
     for name, r in rooflines.items():
         r["achieved"] = round(r["achieved"], 3)
         r["frac"] = round(r["achieved"] / r["peak"], 4)
         r["traffic"] = traffic.get(name)
+    # the raster's issue limits, from the committed LDS/VALU counter pass of this workload
+    if util.get("k_raster<false, 1>"):
+        rooflines["k_raster"].update(util["k_raster<false, 1>"])
     dom_kernel = STAGE_KERNEL[names[dom]]
     roof = dict(rooflines[dom_kernel])
     roof = {"kernel": dom_kernel, "bound": roof["bound"], "achieved": roof["achieved"],
-            "peak": roof["peak"], "unit": roof["unit"], "frac": roof["frac"], "traffic": roof["traffic"]}
+            "peak": roof["peak"], "unit": roof["unit"], "frac": roof["frac"], "traffic": roof["traffic"],
+            **{k: roof[k] for k in ("lds_util", "valu_busy") if k in roof}}
     line = {
         "metric": "frames/s @300k splats SH3 1080p (plain SortedAlpha)",
         "value": round(fps, 3),
