@@ -26,6 +26,8 @@ and two forms of the 1080p scene:
   packed          the scene in the packed geometry path (20-byte records and
                   the i8 SH sidecar): frames/s, stage times, device bytes and
                   the packed-vs-direct image gate
+and the served form again with its 16 scenes in the packed geometry path
+(served_packed).
 """
 
 from __future__ import annotations
@@ -669,12 +671,14 @@ def pct(vals):
             "max": round(vals[-1], 3)}
 
 
-def bench_served(requests, rate, closed_seconds, pipelines=3, max_batch=64):
+def bench_served(requests, rate, closed_seconds, pipelines=3, max_batch=64, geometry_path=0):
+    """The served workload through the server: Poisson arrivals, then a closed loop of 64 clients.
+    geometry_path 1 holds the 16 scenes as packed records (68 bytes per splat instead of 236)."""
     import queue
     import threading
 
     scenes, reqs = synth.served(requests=requests, rate=rate)
-    dev = [A.Scene(s) for s in scenes]
+    dev = [A.Scene(s, geometry_path=geometry_path) for s in scenes]
     del scenes
     srv = A.Server(max_batch_views=max_batch, pipelines=pipelines)
     ids = [srv.add_scene(d) for d in dev]
@@ -762,7 +766,7 @@ def bench_served(requests, rate, closed_seconds, pipelines=3, max_batch=64):
               "latency_ms": pct(clat), "server_ms": pct(cserver)}
     st = srv.stats()
     srv.close()
-    return {"scenes": len(dev), "splats": int(sum(d.count for d in dev)),
+    return {"scenes": len(dev), "splats": int(sum(d.count for d in dev)), "geometry_path": geometry_path,
             "resident_scene_bytes": int(st.resident_scene_bytes), "memory_budget_bytes": int(st.memory_budget_bytes),
             "pipelines": pipelines, "max_batch_views": max_batch,
             "poisson": open_loop, "closed64": closed, "batches": int(st.batches),
@@ -978,6 +982,9 @@ def main():
         print(f"session {line['session']}", file=sys.stderr, flush=True)
         line["packed"] = bench_packed(scene, view, cam, args.steps, args.warmup, med[0])
         print(f"packed {line['packed']}", file=sys.stderr, flush=True)
+        line["served_packed"] = bench_served(args.served_requests, 5000.0, 2.0,
+                                             geometry_path=A.GEOMETRY_PATH_PACKED_ATLAS)
+        print(f"served_packed {line['served_packed']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, view, cam)
     print(json.dumps(line), flush=True)
