@@ -215,7 +215,11 @@ def bench_batched_1080p(scene, steps, warmup):
     b = A.Batch()
     batched = time_batches(b, views, steps, warmup)
     st = stage_frame(b, views)
-    plain = time_one_per_launch(A.Batch(), views, max(steps // 2, 1), warmup)
+    pb = A.Batch()
+    plain = time_one_per_launch(pb, views, max(steps // 2, 1), warmup)
+    b.close()
+    pb.close()
+    dev.close()
     return {"views": len(views), "frames_per_s": round(batched, 3), "plain_frames_per_s": round(plain, 3),
             "speedup_vs_plain": round(batched / plain, 3), "ms_per_batch": round(1e3 * len(views) / batched, 4),
             "stage_ms": {"project": round(st.project_ms, 4), "sort": round(st.sort_ms, 4),
@@ -231,7 +235,12 @@ def bench_batched_64v(steps, warmup):
     b = A.Batch()
     batched = time_batches(b, views, steps, warmup)
     st = stage_frame(b, views)
-    plain = time_one_per_launch(A.Batch(), views, max(steps // 4, 1), max(warmup // 4, 1))
+    pb = A.Batch()
+    plain = time_one_per_launch(pb, views, max(steps // 4, 1), max(warmup // 4, 1))
+    b.close()
+    pb.close()
+    for d in dev:
+        d.close()
     pixels = sum(w * h for (_, _, w, h) in views)
     return {"views": len(views), "scenes": len(dev), "frames_per_s": round(batched, 3),
             "plain_frames_per_s": round(plain, 3), "speedup_vs_plain": round(batched / plain, 3),
@@ -674,9 +683,11 @@ def pct(vals):
 def bench_served(requests, rate, closed_seconds, pipelines=3, max_batch=64, geometry_path=0):
     """The served workload through the server: Poisson arrivals, then a closed loop of 64 clients.
     geometry_path 1 holds the 16 scenes as packed records (68 bytes per splat instead of 236)."""
+    import gc
     import queue
     import threading
 
+    gc.collect()  # objects of earlier sections are released now, not by a collection during the timing
     scenes, reqs = synth.served(requests=requests, rate=rate)
     dev = [A.Scene(s, geometry_path=geometry_path) for s in scenes]
     del scenes
@@ -695,6 +706,14 @@ def bench_served(requests, rate, closed_seconds, pipelines=3, max_batch=64, geom
     warm = [srv.submit(*cams[i % len(cams)]) for i in range(256)]
     for t in warm:
         srv.wait(t)
+
+    def regrowth(before):
+        # batches that had to reallocate (a stall of their own pipeline) or render again (pair overflow)
+        st = srv.stats()
+        return {"buffer_grows": int(st.buffer_grows - before.buffer_grows),
+                "overflow_rerenders": int(st.overflow_rerenders - before.overflow_rerenders)}, st
+
+    st0 = srv.stats()
 
     # Open loop: one thread submits at the scheduled arrival times, one waits in order.
     tickets = queue.Queue()
@@ -736,9 +755,10 @@ def bench_served(requests, rate, closed_seconds, pipelines=3, max_batch=64, geom
                  "completed_per_s": round(len(lat) / span, 1),
                  "latency_ms": pct(lat), "server_ms": pct(server_ms),
                  "mean_batch_views": round(float(np.mean(batch_views)), 2)}
+    open_loop["regrowth"], st1 = regrowth(st0)
 
     # Closed loop: 64 clients.
-    done, clat, cserver = [0], [], []
+    done, clat, cserver, cviews = [0], [], [], []
     lock = threading.Lock()
     stop_at = time.perf_counter() + closed_seconds
 
@@ -752,6 +772,7 @@ def bench_served(requests, rate, closed_seconds, pipelines=3, max_batch=64, geom
             with lock:
                 clat.append((time.perf_counter() - ts) * 1e3)
                 cserver.append(r.total_ms)
+                cviews.append(r.batch_views)
                 done[0] += 1
             i += 64
         buf.close()
@@ -763,8 +784,9 @@ def bench_served(requests, rate, closed_seconds, pipelines=3, max_batch=64, geom
     for x in th:
         x.join()
     closed = {"clients": 64, "completed_per_s": round(done[0] / (time.perf_counter() - tc), 1),
-              "latency_ms": pct(clat), "server_ms": pct(cserver)}
-    st = srv.stats()
+              "latency_ms": pct(clat), "server_ms": pct(cserver),
+              "mean_batch_views": round(float(np.mean(cviews)), 2) if cviews else 0.0}
+    closed["regrowth"], st = regrowth(st1)
     srv.close()
     return {"scenes": len(dev), "splats": int(sum(d.count for d in dev)), "geometry_path": geometry_path,
             "resident_scene_bytes": int(st.resident_scene_bytes), "memory_budget_bytes": int(st.memory_budget_bytes),

@@ -1132,6 +1132,7 @@ int32_t gsplat_server_get_stats(GsplatServer *server, GsplatServerStats *out_sta
     out_stats->scenes = s.scenes;
     out_stats->queued = s.queued;
     out_stats->overflow_rerenders = s.overflow_rerenders;
+    out_stats->buffer_grows = s.buffer_grows;
     return ok();
   });
 }

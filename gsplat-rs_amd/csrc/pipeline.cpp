@@ -188,6 +188,30 @@ Pipeline::Pipeline() {
 Pipeline::~Pipeline() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (GraphSlot &g : graphs_) drop_graph(g);
+  // Stream-ordered buffers go back while their stream exists.
+  keys_.release();
+  vals_.release();
+  keys_alt_.release();
+  vals_alt_.release();
+  rec_.release();
+  rect_.release();
+  counts_.release();
+  offsets_.release();
+  pair_keys_.release();
+  pair_vals_.release();
+  pair_keys_alt_.release();
+  pair_vals_alt_.release();
+  ranges_.release();
+  img32_.release();
+  img8_.release();
+  depth_scratch_.release();
+  pair_scratch_.release();
+  block_sums_.release();
+  block_pairs_.release();
+  order_.release();
+  order_keys_.release();
+  order_vals_.release();
+  if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto &e : ev_)
     if (e) (void)hipEventDestroy(e);
   for (int r = 0; r < 2; ++r) {
@@ -327,28 +351,40 @@ void Pipeline::launch_frame() {
   const int value_byte = nviews > 1 ? 3 : -1;
   const size_t m = std::max<uint32_t>(slots, 1);
   const uint32_t count_blocks = slots / kCountBlock;
-  keys_.ensure(m);
-  vals_.ensure(m);
-  keys_alt_.ensure(m);
-  vals_alt_.ensure(m);
-  rec_.ensure(kRecordFloat4 * m);
-  rect_.ensure(m);
-  counts_.ensure(m);
-  offsets_.ensure(m);
-  block_sums_.ensure(count_blocks + 1);
-  block_pairs_.ensure(count_blocks + 1);
-  img32_.ensure(total_pixels_);
-  img8_.ensure(total_pixels_);
-  ranges_.ensure(std::max<uint32_t>(ctiles, 1));
-  order_.ensure((size_t)kOrderClasses * std::max<uint32_t>(ctiles, 1));
-  pair_keys_.ensure(pair_cap_);
-  pair_vals_.ensure(pair_cap_);
-  pair_keys_alt_.ensure(pair_cap_);
-  pair_vals_alt_.ensure(pair_cap_);
+  // Buffers grow in the stream's order with 25% headroom (no other stream
+  // waits; a batch mix that varies reallocates rarely).
+  const hipStream_t st = stream_;
+  constexpr double kHead = 1.25;
+  bool grew = false;
+  grew |= keys_.ensure(m, st, kHead);
+  grew |= vals_.ensure(m, st, kHead);
+  grew |= keys_alt_.ensure(m, st, kHead);
+  grew |= vals_alt_.ensure(m, st, kHead);
+  grew |= rec_.ensure(kRecordFloat4 * m, st, kHead);
+  grew |= rect_.ensure(m, st, kHead);
+  grew |= counts_.ensure(m, st, kHead);
+  grew |= offsets_.ensure(m, st, kHead);
+  grew |= block_sums_.ensure(count_blocks + 1, st, kHead);
+  grew |= block_pairs_.ensure(count_blocks + 1, st, kHead);
+  grew |= img32_.ensure(total_pixels_, st, kHead);
+  grew |= img8_.ensure(total_pixels_, st, kHead);
+  grew |= ranges_.ensure(std::max<uint32_t>(ctiles, 1), st, kHead);
+  grew |= order_.ensure((size_t)kOrderClasses * std::max<uint32_t>(ctiles, 1), st, kHead);
+  grew |= pair_keys_.ensure(pair_cap_, st);
+  grew |= pair_vals_.ensure(pair_cap_, st);
+  grew |= pair_keys_alt_.ensure(pair_cap_, st);
+  grew |= pair_vals_alt_.ensure(pair_cap_, st);
   // Look-back sorts need zeroed scratch; a later kernel of the frame clears
   // what each sort dirtied, so the next frame finds it clean.
-  if (depth_scratch_.ensure_zeroed(onesweep_status_words((uint32_t)m, 32), stream_)) depth_state_.reset();
-  if (pair_scratch_.ensure_zeroed(onesweep_status_words(pair_cap_, tile_bits), stream_)) pair_state_.reset();
+  if (depth_scratch_.ensure_zeroed(onesweep_status_words((uint32_t)m, 32), st, kHead)) {
+    depth_state_.reset();
+    grew = true;
+  }
+  if (pair_scratch_.ensure_zeroed(onesweep_status_words(pair_cap_, tile_bits), st)) {
+    pair_state_.reset();
+    grew = true;
+  }
+  if (grew) buffer_grows_.fetch_add(1, std::memory_order_relaxed);
   const bool depth_lookback = radix_uses_lookback((uint32_t)m, 0, tuning_.rts_min_tiles);
   // The last frame's pair count (+25%) sizes the pair sort's grids and mode.
   uint32_t hint = pairs_ ? (uint32_t)std::min<uint64_t>(pairs_ + pairs_ / 4, pair_cap_) : pair_cap_;
@@ -398,8 +434,8 @@ void Pipeline::launch_frame() {
   plan.reuse_order = reuse_order_;
   plan.keep_order = keep_order_ && nviews == 1 && !reuse_order_;
   if (plan.keep_order) {
-    order_keys_.ensure(m);
-    order_vals_.ensure(m);
+    order_keys_.ensure(m, st, kHead);
+    order_vals_.ensure(m, st, kHead);
     order_scene_ = views_[0].scene.get();
     order_count_ = views_[0].scene->count;
   }

@@ -27,8 +27,14 @@ namespace gs {
     }                                                                                  \
   } while (0)
 
-// Grow-only device allocation. Releasing waits for the device, since work
-// queued on any stream may still use the old allocation.
+// Grow-only device allocation; ensure() returns true when it (re)allocated.
+// Without a stream, releasing waits for the device, since work queued on any
+// stream may still use the old allocation. A buffer grown on a stream (the
+// only stream whose work uses it) is freed and made again in that stream's
+// order (hipFreeAsync / hipMallocAsync): work queued before still sees the
+// old block, and no other stream waits, so one server pipeline growing its
+// buffers does not stall the batches of the others. Such a buffer must be
+// released (release()) while its stream exists.
 template <typename T>
 class DevBuf {
  public:
@@ -36,35 +42,46 @@ class DevBuf {
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
-  void ensure(size_t count) {
-    if (count <= cap_) return;
+  // At least `count` elements; a stream-ordered growth takes `headroom`
+  // times the count (>= 1), so a slowly growing need reallocates rarely.
+  bool ensure(size_t count, hipStream_t stream = nullptr, double headroom = 1.0) {
+    if (count <= cap_) return false;
     release();
-    const size_t want = std::max<size_t>(count, 1);
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(&ptr_), want * sizeof(T));
+    size_t want = std::max<size_t>(count, 1);
+    if (stream && headroom > 1.0) want = std::max(want, (size_t)((double)want * headroom));
+    const hipError_t e = stream ? hipMallocAsync(reinterpret_cast<void **>(&ptr_), want * sizeof(T), stream)
+                                : hipMalloc(reinterpret_cast<void **>(&ptr_), want * sizeof(T));
     if (e != hipSuccess) {
       ptr_ = nullptr;
+      (void)hipGetLastError();
       throw Error(Code::Internal, "device allocation of " + std::to_string(want * sizeof(T)) +
                                       " bytes failed: " + hipGetErrorString(e));
     }
     cap_ = want;
+    owner_ = stream;
+    return true;
   }
   // As ensure(), clearing the allocation to zero whenever it is (re)made;
   // true when it was. The clear is queued on `stream`, the stream whose work
   // reads the buffer next: a null-stream hipMemset is not ordered before work
   // on a non-blocking stream. Never called under graph capture.
-  bool ensure_zeroed(size_t count, hipStream_t stream) {
-    if (count <= cap_) return false;
-    ensure(count);
+  bool ensure_zeroed(size_t count, hipStream_t stream, double headroom = 1.0) {
+    if (!ensure(count, stream, headroom)) return false;
     GS_HIP(hipMemsetAsync(ptr_, 0, cap_ * sizeof(T), stream));
     return true;
   }
   void release() {
     if (ptr_) {
-      (void)hipDeviceSynchronize();
-      (void)hipFree(ptr_);
+      if (owner_) {
+        (void)hipFreeAsync(ptr_, owner_);
+      } else {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(ptr_);
+      }
     }
     ptr_ = nullptr;
     cap_ = 0;
+    owner_ = nullptr;
   }
   T *get() const { return ptr_; }
   size_t capacity() const { return cap_; }
@@ -72,6 +89,7 @@ class DevBuf {
  private:
   T *ptr_ = nullptr;
   size_t cap_ = 0;
+  hipStream_t owner_ = nullptr;  // the stream a stream-ordered allocation belongs to
 };
 
 bool device_present();
@@ -196,6 +214,8 @@ class Pipeline {
   uint32_t total_tiles() const { return total_tiles_; }
   FrameTimes times();  // waits
   uint64_t overflow_rerenders() const { return overflow_rerenders_.load(std::memory_order_relaxed); }
+  // Frames that had to grow (reallocate) any of the frame buffers.
+  uint64_t buffer_grows() const { return buffer_grows_.load(std::memory_order_relaxed); }
   uint32_t pair_capacity() const { return pair_cap_; }
 
   void readback_rgba8(uint32_t v, uint8_t *out, uint64_t len);
@@ -303,6 +323,7 @@ class Pipeline {
   uint64_t pairs_ = 0;         // binning pairs of the last frame
   uint64_t raster_pairs_ = 0;  // (16x16 tile, splat) pairs of the last frame
   std::atomic<uint64_t> overflow_rerenders_{0};  // read by other threads (server statistics)
+  std::atomic<uint64_t> buffer_grows_{0};        // likewise
   FrameTimes times_{};
   Tuning tuning_{};
   bool count_evals_ = false;

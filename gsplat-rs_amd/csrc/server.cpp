@@ -42,6 +42,13 @@ uint64_t initial_pairs(uint64_t splats) {
   return std::min<uint64_t>(std::max<uint64_t>(8 * splats, 1u << 20), 1u << 30);
 }
 
+// Relative device cost of a request: its splat slots (projection, depth sort,
+// tile counts) plus its pixels (raster, pair sort). On batches of served
+// requests both cost about 0.1 us per thousand, so they weigh alike.
+double request_cost(uint32_t splats, const FrameParams &fp) {
+  return (double)round_segment(splats) + (double)fp.width * fp.height;
+}
+
 }  // namespace
 
 void *host_alloc(size_t bytes) {
@@ -96,6 +103,8 @@ Server::Server(const ServerConfig &cfg) : cfg_(cfg) {
     cfg_.memory_budget = free_b / 10 * 9;
   }
   if (cfg_.pipelines == 0 || cfg_.pipelines > 8) throw Error(Code::InvalidArgument, "pipelines must be 1..8");
+  if (const char *e = std::getenv("GSPLAT_SERVER_BALANCE")) cfg_.balance = std::atoi(e) != 0;
+
   stats_.memory_budget = cfg_.memory_budget;
   for (uint32_t i = 0; i < cfg_.pipelines; ++i) {
     slots_.push_back(std::make_unique<Slot>());
@@ -202,12 +211,14 @@ uint64_t Server::submit(uint32_t scene_id, const GsplatCamera &camera, uint32_t 
     throw Error(Code::Unsupported, "request queue full (" + std::to_string(cfg_.max_queue) + ")");
   }
   r->scene = it->second;
+  r->cost = request_cost(r->scene->count, r->fp);
   // A pinned target is held (host_free refuses it) until the request ends.
   r->out_pinned = out && host_hold(out, need);
   r->ticket = next_ticket_++;
   r->arrival = std::chrono::steady_clock::now();
   tickets_[r->ticket] = r;
   queue_.push_back(r);
+  queued_cost_ += r->cost;
   ++stats_.submitted;
   submit_cv_.notify_one();
   return r->ticket;
@@ -230,7 +241,10 @@ ServerStats Server::stats() {
   s.memory_budget = cfg_.memory_budget;
   s.scenes = (uint32_t)scenes_.size();
   s.queued = (uint32_t)queue_.size();
-  for (const auto &sl : slots_) s.overflow_rerenders += sl->pipe->overflow_rerenders();
+  for (const auto &sl : slots_) {
+    s.overflow_rerenders += sl->pipe->overflow_rerenders();
+    s.buffer_grows += sl->pipe->buffer_grows();
+  }
   return s;
 }
 
@@ -252,13 +266,24 @@ void Server::submit_loop() {
         break;
       }
     // Fold queued requests, oldest first, while the batch's working set
-    // fits this pipeline's share of the budget.
+    // fits this pipeline's share of the budget. Balanced (cfg_.balance): also
+    // while the batch's cost stays within the pipeline's share of all work
+    // outstanding (queued plus running on the other pipelines), so batches
+    // of one size, in cost, run side by side: every request then waits
+    // about one batch, where a batch of double cost beside a small one
+    // would set the latency tail.
+    double share = queued_cost_;
+    for (auto &s : slots_)
+      if (s->busy) share += s->cost;
+    share /= (double)cfg_.pipelines;
     const uint64_t budget = slot_budget();
     const uint64_t pair_floor = slot->pipe->pair_capacity();
     std::vector<std::shared_ptr<Request>> batch;
     uint64_t slots = 0, pixels = 0, tiles = 0, splats = 0;
+    double cost = 0;
     while (!queue_.empty() && batch.size() < cfg_.max_batch_views) {
       const std::shared_ptr<Request> r = queue_.front();
+      if (cfg_.balance && !batch.empty() && cost + 0.5 * r->cost > share) break;
       const uint64_t s2 = slots + round_segment(r->scene->count);
       const uint64_t p2 = pixels + (uint64_t)r->fp.width * r->fp.height;
       const uint64_t t2 = tiles + (uint64_t)r->fp.tiles_x * r->fp.tiles_y;
@@ -267,6 +292,7 @@ void Server::submit_loop() {
       if (est > budget || s2 > (1ull << 30)) {
         if (!batch.empty()) break;
         queue_.pop_front();
+        queued_cost_ -= r->cost;
         r->result.status = (int32_t)Code::Unsupported;
         r->result.error = "request exceeds the memory budget";
         finish(*r);
@@ -275,16 +301,20 @@ void Server::submit_loop() {
         continue;
       }
       queue_.pop_front();
+      queued_cost_ -= r->cost;
       batch.push_back(r);
+      cost += r->cost;
       slots = s2;
       pixels = p2;
       tiles = t2;
       splats = n2;
     }
+    if (queue_.empty()) queued_cost_ = 0;  // no drift of the running sum
     if (batch.empty()) continue;
     const auto now = std::chrono::steady_clock::now();
     for (auto &r : batch) r->start = now;
     slot->batch = batch;
+    slot->cost = cost;
     slot->busy = true;
     lk.unlock();
     std::vector<ViewSpec> specs;
@@ -314,6 +344,7 @@ void Server::submit_loop() {
         finish(*r);
       }
       slot->batch.clear();
+      slot->cost = 0;
       slot->busy = false;
       done_cv_.notify_all();
       continue;
@@ -323,6 +354,7 @@ void Server::submit_loop() {
     slot->ready = true;
     slot->cv.notify_all();
   }
+  queued_cost_ = 0;
   for (auto &r : queue_) {
     r->result.status = (int32_t)Code::Internal;
     r->result.error = "server shut down before the request ran";
@@ -373,6 +405,7 @@ void Server::complete_loop(Slot *slot) {
     }
     stats_.completed += batch.size();
     slot->batch.clear();
+    slot->cost = 0;
     slot->ready = false;
     slot->busy = false;
     done_cv_.notify_all();

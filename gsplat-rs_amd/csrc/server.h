@@ -41,6 +41,10 @@ struct ServerConfig {
   uint32_t max_batch_views = 64;
   uint32_t max_queue = 4096;
   uint32_t pipelines = 3;  // batches in flight at once (1..8), one stream each
+  // Balance batches by cost: a free pipeline takes queued requests up to its
+  // share of the outstanding work (queued + running elsewhere) instead of all
+  // of them (also GSPLAT_SERVER_BALANCE=0/1).
+  bool balance = true;
 };
 
 struct RequestResult {
@@ -57,6 +61,7 @@ struct ServerStats {
   uint64_t resident_scene_bytes = 0, memory_budget = 0;
   uint32_t scenes = 0, queued = 0;
   uint64_t overflow_rerenders = 0;  // batches rendered again after a tile-pair overflow
+  uint64_t buffer_grows = 0;        // batches that reallocated their pipeline's frame buffers
 };
 
 // Pinned host allocations handed out to callers for server outputs; the
@@ -93,6 +98,7 @@ class Server {
     uint64_t ticket = 0;
     std::shared_ptr<DeviceScene> scene;
     FrameParams fp{};
+    double cost = 0;  // request_cost
     uint8_t *out = nullptr;
     bool out_pinned = false;
     std::chrono::steady_clock::time_point arrival, start;
@@ -102,6 +108,7 @@ class Server {
   struct Slot {
     std::unique_ptr<Pipeline> pipe;
     std::vector<std::shared_ptr<Request>> batch;
+    double cost = 0;     // request_cost summed over the batch
     bool busy = false;   // a batch is assigned (being enqueued or in flight)
     bool ready = false;  // the batch is enqueued; the completer may wait on it
     std::condition_variable cv;
@@ -120,6 +127,7 @@ class Server {
   std::mutex m_;
   std::condition_variable submit_cv_, done_cv_;
   std::deque<std::shared_ptr<Request>> queue_;
+  double queued_cost_ = 0;  // request_cost summed over queue_
   std::unordered_map<uint64_t, std::shared_ptr<Request>> tickets_;
   std::map<uint32_t, std::shared_ptr<DeviceScene>> scenes_;
   uint32_t next_scene_ = 1;

@@ -163,7 +163,8 @@ class GsplatRequestResult(ctypes.Structure):
 class GsplatServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "rejected", "batches", "views_in_batches",
                                               "resident_scene_bytes", "memory_budget_bytes")] + \
-               [("scenes", ctypes.c_uint32), ("queued", ctypes.c_uint32), ("overflow_rerenders", ctypes.c_uint64)]
+               [("scenes", ctypes.c_uint32), ("queued", ctypes.c_uint32), ("overflow_rerenders", ctypes.c_uint64),
+                ("buffer_grows", ctypes.c_uint64)]
 
 
 P = ctypes.c_void_p

@@ -431,6 +431,8 @@ typedef struct GsplatServerStats {
   uint32_t queued;
   uint64_t overflow_rerenders; /* batches rendered again after their tile-pair
                                 * buffers overflowed (the buffers then grow) */
+  uint64_t buffer_grows;       /* batches whose pipeline had to reallocate its
+                                * frame buffers first (in its stream's order) */
 } GsplatServerStats;
 
 /* {0, 64, 4096} */
