@@ -957,6 +957,7 @@ int32_t gsplat_batch_get_stage_times(GsplatBatch *batch, GsplatStageTimes *out) 
     out->blend_tests = t.blend_tests;
     out->bin_pairs = batch->pipe.bin_pairs();
     out->bin_shift = batch->pipe.view(0).bin_shift;
+    out->depth_sort = batch->pipe.depth_sort_mode();
     return ok();
   });
 }

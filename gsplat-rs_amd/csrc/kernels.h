@@ -110,10 +110,13 @@ struct Clears {
 // vals[g] = view << 24 | splat index (index 0xFFFFFF for padding slots);
 // rect[g] = pixel bounds x0|y0<<16, x1|y1<<16, or x = ~0u when the splat
 // covers no pixel.
-// The grid first zeroes `clears`.
+// The grid first zeroes `clears`. With `key_range` (total_slots / 256
+// entries), workgroup b (slots [256 b, 256 b + 256)) also stores its smallest
+// non-zero key (~0u when none) and its largest key at key_range[b], for a
+// bucket sort of the keys (sort.h).
 void launch_project(uint32_t format, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,
                     uint32_t total_slots, uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect,
-                    const Clears &clears, hipStream_t stream);
+                    const Clears &clears, hipStream_t stream, uint2 *key_range = nullptr);
 
 // Views of one scene projected together (views of a batch that share a
 // scene, at most kGroupViews per group). Group chunks are runs of 256

@@ -452,24 +452,54 @@ __device__ __forceinline__ void write_slot(const FrameParams &fp, uint32_t v, ui
   rect[g] = rr;
 }
 
+// The workgroup's smallest non-zero key (~0u when none) and largest key,
+// stored at range[blockIdx.x].
+__device__ __forceinline__ void store_key_range(uint32_t key, uint2 *__restrict__ range) {
+  __shared__ uint32_t part[2][kWaves];
+  uint32_t mn = key ? key : 0xFFFFFFFFu, mx = key;
+#pragma unroll
+  for (uint32_t d = 1; d < kWave; d <<= 1) {
+    mn = min(mn, (uint32_t)__shfl_xor((int)mn, (int)d, kWave));
+    mx = max(mx, (uint32_t)__shfl_xor((int)mx, (int)d, kWave));
+  }
+  const uint32_t w = threadIdx.x / kWave;
+  if (__lane_id() == 0) {
+    part[0][w] = mn;
+    part[1][w] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (uint32_t q = 1; q < kWaves; ++q) {
+      mn = min(mn, part[0][q]);
+      mx = max(mx, part[1][q]);
+    }
+    range[blockIdx.x] = make_uint2(min(mn, part[0][0]), max(mx, part[1][0]));
+  }
+}
+
 template <uint32_t FMT>
 __global__ __launch_bounds__(kThreads) void k_project(const ViewDev *__restrict__ views,
                                                       const uint32_t *__restrict__ seg_start, uint32_t nviews,
                                                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                       float4 *__restrict__ rec, uint2 *__restrict__ rect,
-                                                      Clears clears) {
+                                                      Clears clears, uint2 *__restrict__ key_range) {
   clear_ranges(clears);
   const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
   const uint32_t v = find_view(seg_start, nviews, blockIdx.x * kThreads);
   const ViewDev &vw = views[v];
-  if (vw.format != FMT) return;
+  if (vw.format != FMT) return;  // uniform: one view per workgroup
   const uint32_t i = g - vw.seg_off;
+  uint32_t key = 0;
   if (i >= vw.n) {
     write_padding(v, g, keys, vals, rect);
-    return;
+  } else {
+    const auto src = FormatOf<FMT>::at(vw.soa, vw.n, i);
+    const Projected p = project_one<FormatOf<FMT>::kDeg>(vw.fp, src);
+    write_slot(vw.fp, v, g, i, p, src, keys, vals, rec, rect);
+    key = p.key;
   }
-  const auto src = FormatOf<FMT>::at(vw.soa, vw.n, i);
-  write_slot(vw.fp, v, g, i, project_one<FormatOf<FMT>::kDeg>(vw.fp, src), src, keys, vals, rec, rect);
+  if (key_range) store_key_range(key, key_range);
 }
 
 // Views of one scene projected together: workgroup b takes chunk
@@ -1380,11 +1410,12 @@ void with_format(uint32_t fmt, F &&f) {
 
 void launch_project(uint32_t fmt, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,
                     uint32_t total_slots, uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect, const Clears &clr,
-                    hipStream_t st) {
+                    hipStream_t st, uint2 *key_range) {
   if (total_slots == 0 || nviews == 0) return;
   const dim3 grid(div_up(total_slots, kThreads)), block(kThreads);
   with_format(fmt, [&](auto F) {
-    k_project<decltype(F)::value><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr);
+    k_project<decltype(F)::value><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr,
+                                                          key_range);
   });
   GS_HIP(hipGetLastError());
 }

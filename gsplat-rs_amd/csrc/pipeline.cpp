@@ -28,11 +28,12 @@ constexpr TableLayout table_layout(uint32_t nviews, uint32_t ngroups) {
 }
 constexpr size_t kTableBytes = table_layout(kMaxViews, kMaxViews).bytes;
 // Frame summary words (device and pinned host copies):
-//   [0] unused, [1..2] binning pair count M (u64),
+//   [0] bucket depth sort overflow, [1..2] binning pair count M (u64),
 //   [3] overflow flag, [4..5] (16x16 tile, splat) pairs (u64),
 //   [6 .. 6+kMaxViews) visible count per view,
 //   kEvalsWord: u64 evaluations passing, u64 coverage tests (counting mode),
 //   kOrderWord: class sizes of the raster's heaviest-first order.
+constexpr size_t kSortOverflowWord = 0;
 constexpr size_t kRasterPairsWord = 4;
 constexpr size_t kVisibleWord = 6;
 constexpr size_t kEvalsWord = kVisibleWord + kMaxViews;
@@ -162,10 +163,12 @@ Pipeline::Pipeline() {
     tuning_.blend_min = std::min<uint32_t>(std::max<uint32_t>((uint32_t)std::strtoul(e, nullptr, 10), 1u), 256u);
   if (const char *e = std::getenv("GSPLAT_RASTER_ROWS2_MIN_TILES"))
     tuning_.rows2_min_tiles = (uint32_t)std::strtoul(e, nullptr, 10);
+  if (const char *e = std::getenv("GSPLAT_SORT_BUCKET")) bucket_sort_ = std::atoi(e) != 0;
   if (const char *e = std::getenv("GSPLAT_SORT_RTS_MIN_TILES"))
     tuning_.rts_min_tiles = std::max<uint32_t>((uint32_t)std::strtoul(e, nullptr, 10), 1u);
   GS_HIP(hipSetDevice(0));
   GS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  stream_owner_.s = stream_;
   // The frame-start and stage-boundary events only time the device: no
   // system-scope fence (a cache writeback and invalidation per event). The
   // frame's last event keeps it: the host then reads what the frame wrote.
@@ -188,30 +191,6 @@ Pipeline::Pipeline() {
 Pipeline::~Pipeline() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (GraphSlot &g : graphs_) drop_graph(g);
-  // Stream-ordered buffers go back while their stream exists.
-  keys_.release();
-  vals_.release();
-  keys_alt_.release();
-  vals_alt_.release();
-  rec_.release();
-  rect_.release();
-  counts_.release();
-  offsets_.release();
-  pair_keys_.release();
-  pair_vals_.release();
-  pair_keys_alt_.release();
-  pair_vals_alt_.release();
-  ranges_.release();
-  img32_.release();
-  img8_.release();
-  depth_scratch_.release();
-  pair_scratch_.release();
-  block_sums_.release();
-  block_pairs_.release();
-  order_.release();
-  order_keys_.release();
-  order_vals_.release();
-  if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto &e : ev_)
     if (e) (void)hipEventDestroy(e);
   for (int r = 0; r < 2; ++r) {
@@ -219,7 +198,7 @@ Pipeline::~Pipeline() {
     if (table_pinned_[r]) (void)hipHostFree(table_pinned_[r]);
   }
   if (summary_pinned_) (void)hipHostFree(summary_pinned_);
-  if (stream_) (void)hipStreamDestroy(stream_);
+  // the buffers free on stream_, which stream_owner_ destroys after them
 }
 
 bool Pipeline::has_order(const DeviceScene *scene) const {
@@ -376,7 +355,8 @@ void Pipeline::launch_frame() {
   grew |= pair_vals_alt_.ensure(pair_cap_, st);
   // Look-back sorts need zeroed scratch; a later kernel of the frame clears
   // what each sort dirtied, so the next frame finds it clean.
-  if (depth_scratch_.ensure_zeroed(onesweep_status_words((uint32_t)m, 32), st, kHead)) {
+  if (depth_scratch_.ensure_zeroed(std::max(onesweep_status_words((uint32_t)m, 32), bucket_sort_words((uint32_t)m)),
+                                   st, kHead)) {
     depth_state_.reset();
     grew = true;
   }
@@ -386,6 +366,12 @@ void Pipeline::launch_frame() {
   }
   if (grew) buffer_grows_.fetch_add(1, std::memory_order_relaxed);
   const bool depth_lookback = radix_uses_lookback((uint32_t)m, 0, tuning_.rts_min_tiles);
+  // One short view sorts its depth keys by bucket (three launches instead of
+  // five); after a bucket overflow the next 64 frames use the radix passes.
+  const bool depth_bucket =
+      depth_lookback && nviews == 1 && !reuse_order_ && bucket_sort_ && bucket_off_ == 0 && m <= kBucketMaxKeys;
+  if (bucket_off_) --bucket_off_;
+  if (depth_bucket && key_range_.ensure(m / kThreads + 1, st, kHead)) buffer_grows_.fetch_add(1, std::memory_order_relaxed);
   // The last frame's pair count (+25%) sizes the pair sort's grids and mode.
   uint32_t hint = pairs_ ? (uint32_t)std::min<uint64_t>(pairs_ + pairs_ / 4, pair_cap_) : pair_cap_;
   // A graph replays the grids it was captured with: round the hint up to a
@@ -395,7 +381,8 @@ void Pipeline::launch_frame() {
     hint = (uint32_t)std::min<uint64_t>((uint64_t)kSortTile << bits_for(div_up_u64(hint, kSortTile)), pair_cap_);
   const bool pair_lookback = radix_uses_lookback(pair_cap_, hint, tuning_.rts_min_tiles);
   const RadixPasses depth_ps = radix_passes(32, true, value_byte), pair_ps = radix_passes(tile_bits, false, -1);
-  const size_t depth_dirty = radix_lookback_words((uint32_t)m, depth_ps);
+  // A bucket sort dirties its look-back words and the range words after them.
+  const size_t depth_dirty = depth_bucket ? bucket_sort_words((uint32_t)m) : radix_lookback_words((uint32_t)m, depth_ps);
   const size_t pair_dirty = radix_lookback_words(pair_cap_, pair_ps);
 
   // View table through a pinned staging slot whose previous upload is done.
@@ -424,6 +411,8 @@ void Pipeline::launch_frame() {
   plan.count_blocks = count_blocks;
   plan.hint = hint;
   plan.depth_lookback = depth_lookback;
+  plan.depth_bucket = depth_bucket;
+  depth_sort_mode_ = reuse_order_ ? 3u : depth_bucket ? 2u : depth_lookback ? 0u : 1u;
   plan.pair_lookback = pair_lookback;
   plan.depth_dirty = depth_dirty;
   plan.pair_dirty = pair_dirty;
@@ -448,7 +437,7 @@ void Pipeline::launch_frame() {
                                  pair_cap_, hint, tuning_.bin_shift,
                                  (uint64_t)tuning_.raster, tuning_.rows2_min_tiles, tuning_.rts_min_tiles, tuning_.blend_min,
                                  tuning_.bin_auto, count_evals_, depth_state_.rts_dirty(), pair_state_.rts_dirty(),
-                                 (uint64_t)reuse_order_, (uint64_t)keep_order_};
+                                 (uint64_t)reuse_order_, (uint64_t)keep_order_, (uint64_t)depth_bucket};
     for (const ViewDev &d : host_views_) {
       key.push_back(reinterpret_cast<uintptr_t>(d.soa));
       key.push_back(((uint64_t)d.n << 8) | d.format);
@@ -545,7 +534,7 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   }
   if (ngroups == 0) {
     launch_project(host_views_[0].format, views_d, seg_start_d, nviews, slots, keys_.get(), vals_.get(),
-                   rec_.get(), rect_.get(), frame_clears, st);
+                   rec_.get(), rect_.get(), frame_clears, st, plan.depth_bucket ? key_range_.get() : nullptr);
   } else {
     for (uint32_t fmt = 0; fmt < kFormats; ++fmt)
       if (group_chunks_[fmt].second) {
@@ -565,14 +554,20 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
     k = order_keys_.get();
     v = order_vals_.get();
   } else {
-    SortPrep depth_prep;
     if (plan.depth_scratch.fill) launch_fill_u32(depth_scratch_.get(), 0u, plan.depth_scratch.fill, st);
-    depth_prep.scratch_clean = plan.depth_scratch.skip_clear;
-    depth_prep.rts_min_tiles = tuning_.rts_min_tiles;
-    depth_prep.seg_start = seg_start_d;  // view v's slots: segment v, value byte 3 = v
-    depth_prep.segs = nviews;
-    onesweep_sort(&k, &v, &ka, &va, slots, nullptr, 32, /*descending=*/true, depth_scratch_.get(),
-                  depth_scratch_.capacity(), st, value_byte, 0, depth_prep);
+    if (plan.depth_bucket) {
+      // an overflow flags summary word 0; wait() then renders the frame again
+      bucket_sort(&k, &v, &ka, &va, slots, key_range_.get(), slots / kThreads, depth_scratch_.get(),
+                  depth_scratch_.capacity(), plan.depth_scratch.skip_clear, info + kSortOverflowWord, st);
+    } else {
+      SortPrep depth_prep;
+      depth_prep.scratch_clean = plan.depth_scratch.skip_clear;
+      depth_prep.rts_min_tiles = tuning_.rts_min_tiles;
+      depth_prep.seg_start = seg_start_d;  // view v's slots: segment v, value byte 3 = v
+      depth_prep.segs = nviews;
+      onesweep_sort(&k, &v, &ka, &va, slots, nullptr, 32, /*descending=*/true, depth_scratch_.get(),
+                    depth_scratch_.capacity(), st, value_byte, 0, depth_prep);
+    }
     if (plan.keep_order) {  // kept for the frames that reuse it
       GS_HIP(hipMemcpyAsync(order_keys_.get(), k, (size_t)slots * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
       GS_HIP(hipMemcpyAsync(order_vals_.get(), v, (size_t)slots * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
@@ -661,7 +656,12 @@ bool Pipeline::absorb_summary() {
     if (want > (1ull << 30)) throw Error(Code::Unsupported, "tile pair count exceeds 2^30");
     pair_cap_ = (uint32_t)want;
   }
-  return overflow;
+  const bool sort_overflow = summary_pinned_[kSortOverflowWord] != 0;
+  if (sort_overflow) {
+    bucket_off_ = 64;
+    ++sort_overflows_;
+  }
+  return overflow || sort_overflow;
 }
 
 void Pipeline::wait() {

@@ -216,6 +216,16 @@ class Pipeline {
   uint64_t overflow_rerenders() const { return overflow_rerenders_.load(std::memory_order_relaxed); }
   // Frames that had to grow (reallocate) any of the frame buffers.
   uint64_t buffer_grows() const { return buffer_grows_.load(std::memory_order_relaxed); }
+  // Frames whose bucket depth sort overflowed (rendered again with the radix
+  // passes; the next 64 frames use them too).
+  uint64_t sort_overflows() const { return sort_overflows_; }
+  // Depth sorts of one-view frames of at most kBucketMaxKeys slots in
+  // look-back mode as a bucket sort (default on; also GSPLAT_SORT_BUCKET=0/1).
+  // The order is the same either way.
+  void set_bucket_sort(bool on) { bucket_sort_ = on; }
+  // How the last frame's depth sort ran: 0 look-back radix passes, 1
+  // reduce-then-scan radix passes, 2 bucket sort, 3 none (order reused).
+  uint32_t depth_sort_mode() const { return depth_sort_mode_; }
   uint32_t pair_capacity() const { return pair_cap_; }
 
   void readback_rgba8(uint32_t v, uint8_t *out, uint64_t len);
@@ -263,6 +273,7 @@ class Pipeline {
     int value_byte = -1;
     bool reuse_order = false, keep_order = false;
     bool depth_lookback = false, pair_lookback = false;
+    bool depth_bucket = false;  // the depth sort is a bucket sort (bucket_sort, sort.h)
     size_t depth_dirty = 0, pair_dirty = 0;                // look-back words the frame clears after use
     size_t depth_rts_words = 0, pair_rts_words = 0;        // words a reduce-then-scan sort may write
     ScratchState::Plan depth_scratch, pair_scratch;
@@ -282,6 +293,17 @@ class Pipeline {
   bool absorb_summary();  // true when the frame's pairs overflowed
   void check_view(uint32_t v) const;
 
+  // Declared first, so destroyed last: the buffers below that grew in the
+  // stream's order free themselves on it when they are destroyed.
+  struct StreamOwner {
+    hipStream_t s = nullptr;
+    ~StreamOwner() {
+      if (s) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+      }
+    }
+  } stream_owner_;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev_[6] = {};        // stage boundaries 0..4, 5 = summary copied
   hipEvent_t table_ev_[2] = {};  // view-table uploads out of each pinned slot
@@ -303,6 +325,7 @@ class Pipeline {
   DevBuf<uint32_t> depth_scratch_, pair_scratch_, m_dev_, block_sums_, block_pairs_;
   DevBuf<ViewDev> export_view_;
   DevBuf<uint32_t> order_keys_, order_vals_;  // kept depth order (set_keep_order)
+  DevBuf<uint2> key_range_;                   // per projection workgroup key ranges (bucket sort)
   const DeviceScene *order_scene_ = nullptr;  // the scene whose order is kept
   uint32_t order_count_ = 0;
   bool keep_order_ = false, reuse_order_ = false;
@@ -324,6 +347,10 @@ class Pipeline {
   uint64_t raster_pairs_ = 0;  // (16x16 tile, splat) pairs of the last frame
   std::atomic<uint64_t> overflow_rerenders_{0};  // read by other threads (server statistics)
   std::atomic<uint64_t> buffer_grows_{0};        // likewise
+  bool bucket_sort_ = true;
+  uint32_t bucket_off_ = 0;      // frames left that sort by radix passes after a bucket overflow
+  uint64_t sort_overflows_ = 0;
+  uint32_t depth_sort_mode_ = 0;
   FrameTimes times_{};
   Tuning tuning_{};
   bool count_evals_ = false;

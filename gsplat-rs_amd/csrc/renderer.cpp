@@ -151,6 +151,7 @@ GsplatStageTimes Renderer::stage_times() {
   s.blend_tests = t.blend_tests;
   s.bin_pairs = pipe_.bin_pairs();
   s.bin_shift = d.bin_shift;
+  s.depth_sort = pipe_.depth_sort_mode();
   return s;
 }
 

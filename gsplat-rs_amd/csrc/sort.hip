@@ -86,10 +86,16 @@ constexpr uint32_t kScanGrid = 1024;            // bound of the scan grid (workg
 
 using PassSet = RadixPasses;
 
+// from_value of a bucket pass: the digit is the key's bucket (kmax - key) >>
+// shift for non-zero keys, with kmax passed as `flip`, and the last digit for
+// key 0 (see bucket_sort).
+constexpr uint32_t kRangeDigit = 2;
+
 // Digits of DB bits: thread t < 2^DB keeps the books of digit t.
 template <uint32_t DB>
 __device__ __forceinline__ uint32_t digit(uint32_t key, uint32_t val, uint32_t shift, uint32_t flip,
                                           uint32_t from_value) {
+  if (from_value == kRangeDigit) return key ? min((flip - key) >> shift, (1u << DB) - 2u) : (1u << DB) - 1u;
   return (((from_value ? val : key) ^ flip) >> shift) & ((1u << DB) - 1u);
 }
 
@@ -328,13 +334,20 @@ __device__ __forceinline__ void scatter_tile(const uint32_t (&key)[IPT], const u
 // Persistent: each workgroup takes tiles from the pass counter until they run
 // past the live element count, so the grid is bounded whatever the capacity.
 // Pass p's status words start at p * (live tiles) * 2^DB.
+// A bucket pass (from_value kRangeDigit) reads kmax and the shift from
+// range[0..1], and the workgroup of tile 0 stores every digit's start at
+// range[2 + d] (range[2 + 2^DB] = n).
 template <uint32_t DB>
 __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                const uint32_t *__restrict__ n_dev, uint32_t n_cap, uint32_t pass,
                                                uint32_t shift, uint32_t flip, uint32_t from_value,
                                                const uint32_t *__restrict__ hist, uint32_t *__restrict__ counter,
-                                               uint32_t *__restrict__ status_all) {
+                                               uint32_t *__restrict__ status_all, uint32_t *__restrict__ range) {
+  if (from_value == kRangeDigit) {
+    flip = range[0];
+    shift = range[1];
+  }
   __shared__ uint32_t s_tile;
   __shared__ uint32_t lds4[WAVES];
   constexpr uint32_t D = 1u << DB;
@@ -365,6 +378,10 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
     for (uint32_t c = 0; c < STRIPES; ++c) hd += hist[(c * MAX_PASSES + pass) * D + t];
   }
   const uint32_t g = block_exclusive_scan(hd, lds4);
+  if (from_value == kRangeDigit && tile == 0) {
+    if (t < D) range[2 + t] = g;
+    if (t == 0) range[2 + D] = n;
+  }
   // every key has the same digit: the pass is the identity
   const bool trivial = __syncthreads_or(n > 0 && hd == n) != 0;
 
@@ -625,6 +642,135 @@ __global__ __launch_bounds__(T) void k_rts_down(const uint32_t *__restrict__ kin
   }
 }
 
+// Bucket sort, step 1 (one launch, grid-stride over 1024-key chunks like
+// k_os_hist): every workgroup reduces the producer's per-workgroup key
+// ranges to kmin (smallest non-zero key) and kmax, takes the smallest shift
+// s with (kmax - kmin) >> s <= 510, and counts the buckets of its keys into
+// the look-back histogram of pass 0 (stripe = workgroup mod STRIPES);
+// workgroup 0 stores range[0] = kmax, range[1] = s.
+constexpr uint32_t BUCKET_BITS = MAX_DIGIT_BITS;
+constexpr uint32_t BUCKETS = 1u << BUCKET_BITS;  // 511 for the key range, the last for key 0
+
+__global__ __launch_bounds__(T) void k_bucket_hist(const uint32_t *__restrict__ keys, uint32_t n,
+                                                   const uint2 *__restrict__ parts, uint32_t nparts,
+                                                   uint32_t *__restrict__ hist, uint32_t *__restrict__ range) {
+  __shared__ uint32_t h[BUCKETS];
+  __shared__ uint32_t red[2][WAVES];
+  const uint32_t t = threadIdx.x, lane = __lane_id(), w = t / 64;
+  for (uint32_t i = t; i < BUCKETS; i += T) h[i] = 0;
+  uint32_t mn = 0xFFFFFFFFu, mx = 0;
+  for (uint32_t i = t; i < nparts; i += T) {
+    const uint2 p = parts[i];
+    mn = min(mn, p.x);
+    mx = max(mx, p.y);
+  }
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    mn = min(mn, (uint32_t)__shfl_xor((int)mn, (int)d, 64));
+    mx = max(mx, (uint32_t)__shfl_xor((int)mx, (int)d, 64));
+  }
+  if (lane == 0) {
+    red[0][w] = mn;
+    red[1][w] = mx;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t q = 0; q < WAVES; ++q) {
+    mn = min(mn, red[0][q]);
+    mx = max(mx, red[1][q]);
+  }
+  const uint32_t span = mn == 0xFFFFFFFFu ? 0u : mx - mn;  // no non-zero key: every key is 0
+  uint32_t s = 0;
+  while ((span >> s) > BUCKETS - 2u) ++s;
+  if (blockIdx.x == 0 && t == 0) {
+    range[0] = mx;
+    range[1] = s;
+  }
+  for (size_t base = (size_t)blockIdx.x * HIST_CHUNK; base < n; base += (size_t)gridDim.x * HIST_CHUNK) {
+    uint32_t k[HIST_IPT];
+#pragma unroll
+    for (uint32_t r = 0; r < HIST_IPT; ++r) {
+      const size_t i = base + (size_t)r * T + t;
+      k[r] = i < n ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < HIST_IPT; ++r)
+      if (base + (size_t)r * T + t < n) wave_count(h, digit<BUCKET_BITS>(k[r], 0u, s, mx, kRangeDigit));
+  }
+  __syncthreads();
+  uint32_t *dst = hist + (size_t)(blockIdx.x % STRIPES) * MAX_PASSES * BUCKETS;  // pass 0
+  for (uint32_t i = t; i < BUCKETS; i += T) {
+    const uint32_t c = h[i];
+    if (c) atomicAdd(&dst[i], c);
+  }
+}
+
+// Bucket sort, step 3: workgroup b sorts bucket b's run [range[2 + b],
+// range[3 + b]) in place: keys become kmax - key (the low s bits; the bucket
+// holds the high bits), are ranked by stable 8-bit LDS passes, ascending,
+// and are restored on the way out. Equal keys keep the order the stable
+// bucket pass gave them. A run longer than one tile sets *overflow.
+__global__ __launch_bounds__(T) void k_bucket_local(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                    const uint32_t *__restrict__ range, uint32_t *__restrict__ overflow) {
+  __shared__ uint32_t lds4[WAVES];
+  __shared__ uint32_t lstart[256];
+  __shared__ uint32_t whist[WAVES + 1][256];  // row WAVES: rank_loaded's spare words
+  __shared__ uint32_t skeys[TILE];
+  __shared__ uint32_t svals[TILE];
+  const uint32_t b = blockIdx.x, t = threadIdx.x, w = t / 64, lane = __lane_id();
+  const uint32_t kmax = range[0], s = range[1];
+  const uint32_t off = range[2 + b], cnt = range[3 + b] - off;
+  // s == 0: a bucket holds one key value, already in order
+  if (b >= BUCKETS - 1 || cnt <= 1 || s == 0) return;
+  if (cnt > TILE) {
+    if (t == 0) atomicOr(overflow, 1u);
+    return;
+  }
+  uint32_t key[IPT], val[IPT], rank[IPT];
+  load_tile(keys, vals, off, cnt, key, val);
+  const uint32_t low = s >= 32 ? 0xFFFFFFFFu : (1u << s) - 1u;
+#pragma unroll
+  for (uint32_t r = 0; r < IPT; ++r) key[r] = (kmax - key[r]) & low;
+  const uint32_t passes = (s + 7) / 8;
+  for (uint32_t p = 0; p < passes; ++p) {
+    clear_whist<8>(whist);
+    __syncthreads();
+    const uint32_t run = rank_loaded<8>(cnt, 8 * p, 0u, 0u, key, val, rank, whist);
+    const uint32_t ls = block_exclusive_scan(run, lds4);
+    if (t < 256) lstart[t] = ls;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < IPT; ++r) {
+      if (rank[r] != 0xFFFFFFFFu) {
+        const uint32_t d = (key[r] >> (8 * p)) & 255u;
+        const uint32_t pos = lstart[d] + whist[w][d] + rank[r];
+        if (pos < TILE) {  // always (ranks of one tile); bounds the LDS stage regardless
+          skeys[pos] = key[r];
+          svals[pos] = val[r];
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < IPT; ++r) {
+      const uint32_t j = w * WAVE_ITEMS + r * 64 + lane;
+      if (j < cnt) {
+        key[r] = skeys[j];
+        val[r] = svals[j];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < IPT; ++r) {
+    const uint32_t j = w * WAVE_ITEMS + r * 64 + lane;
+    if (j < cnt) {
+      keys[off + j] = kmax - ((b << s) | key[r]);
+      vals[off + j] = val[r];
+    }
+  }
+}
+
 uint32_t div_up(size_t a, size_t b) { return (uint32_t)((a + b - 1) / b); }
 
 uint32_t hist_blocks(uint32_t n_cap) {
@@ -756,7 +902,7 @@ void sort_passes(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t
   const uint32_t grid = std::min(grid_tiles, kPassGrid);
   for (uint32_t p = 0; p < ps.count; ++p) {
     k_os_pass<DB><<<grid, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, p, ps.shift[p], ps.flip[p],
-                                      ps.from_value[p], hist, counters + p, status);
+                                      ps.from_value[p], hist, counters + p, status, nullptr);
     std::swap(*keys, *keys_alt);
     std::swap(*vals, *vals_alt);
   }
@@ -764,6 +910,35 @@ void sort_passes(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t
 }
 
 }  // namespace
+
+size_t bucket_sort_lookback_words(uint32_t n) { return lookback_words(n, 1, BUCKETS); }
+
+size_t bucket_sort_words(uint32_t n) { return bucket_sort_lookback_words(n) + 3 + BUCKETS; }
+
+void bucket_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n,
+                 const uint2 *key_range, uint32_t parts, uint32_t *scratch, size_t scratch_words, bool scratch_clean,
+                 uint32_t *overflow, hipStream_t st) {
+  if (n <= 1) return;
+  if (n > kBucketMaxKeys) throw Error(Code::Internal, "bucket sort of more than 2^19 keys");
+  if (bucket_sort_words(n) > scratch_words) throw Error(Code::Internal, "bucket sort scratch too small");
+  const uint32_t tiles = div_up(n, TILE);
+  const size_t fixed = (size_t)STRIPES * MAX_PASSES * BUCKETS + MAX_PASSES;
+  uint32_t *hist = scratch;
+  uint32_t *counter = hist + (size_t)STRIPES * MAX_PASSES * BUCKETS;
+  uint32_t *status = counter + MAX_PASSES;
+  uint32_t *range = scratch + bucket_sort_lookback_words(n);  // kmax, s, bucket starts [BUCKETS + 1]
+  if (!scratch_clean)
+    k_os_clear<<<std::min<uint32_t>(div_up(bucket_sort_lookback_words(n), T), 1024), T, 0, st>>>(
+        scratch, fixed, BUCKETS, nullptr, n);
+  k_bucket_hist<<<hist_blocks(n), T, 0, st>>>(*keys, n, key_range, parts, hist, range);
+  k_os_pass<BUCKET_BITS><<<std::min(tiles, kPassGrid), T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, nullptr, n,
+                                                                   0, 0, 0, kRangeDigit, hist, counter, status,
+                                                                   range);
+  std::swap(*keys, *keys_alt);
+  std::swap(*vals, *vals_alt);
+  k_bucket_local<<<BUCKETS - 1, T, 0, st>>>(*keys, *vals, range, overflow);
+  GS_HIP(hipGetLastError());
+}
 
 void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n_cap,
                    const uint32_t *n_dev, uint32_t bits, bool descending, uint32_t *scratch, size_t scratch_words,

@@ -99,6 +99,7 @@ class GsplatStageTimes(ctypes.Structure):
         ("blend_tests", ctypes.c_uint64),
         ("bin_pairs", ctypes.c_uint64),
         ("bin_shift", ctypes.c_uint32),
+        ("depth_sort", ctypes.c_uint32),
     ]
 
 

@@ -100,6 +100,11 @@ typedef struct GsplatStageTimes {
                             * pixels square                             */
   uint32_t bin_shift;      /* binning tile of the (first) view: 16 <<
                             * bin_shift pixels (GsplatTuning.bin_auto)  */
+  uint32_t depth_sort;     /* how the frame's depth sort ran (the order is
+                            * the same): 0 look-back radix passes, 1
+                            * reduce-then-scan radix passes, 2 bucket sort
+                            * (one view of at most 2^19 slots), 3 none (a
+                            * kept order reused)                         */
 } GsplatStageTimes;
 int32_t gsplat_context_get_stage_times(GsplatContext *ctx, GsplatStageTimes *out);
 int32_t gsplat_context_set_stage_timing(GsplatContext *ctx, uint32_t on);
