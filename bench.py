@@ -474,7 +474,7 @@ def _stage_sequence(rows):
         raster = k.startswith("k_raster<") or k.startswith("k_raster_rows2<")
         if k.startswith("k_project"):
             stage = "project"
-        elif stage == "project" and (k.startswith("k_os_") or k.startswith("k_rts_") or k.startswith("k_scan_")):
+        elif stage == "project" and k.startswith(("k_os_", "k_rts_", "k_scan_", "k_bucket_")):
             stage = "sort"
         elif k == "k_tile_counts":
             stage = "bin"
@@ -926,10 +926,12 @@ def main():
     # Algorithmic work per stage (units as stated; per frame):
     #  project  positions for all splats, covariance + alpha + cut + SH for visible, key/value writes for
     #           every slot, 48-byte record + 8-byte rect for visible
-    #  sort     8-bit LSD: 4 passes x (key+value read and write) over N slots
+    #  sort     bucket sort (one view of <= 2^19 slots): key read for the bucket counts, one stable pass and
+    #           the per-bucket LDS sort (key+value read and write each); else 8-bit LSD: 4 passes x (key+value
+    #           read and write) over N slots
     #  raster   16 FLOP per (pixel, splat) evaluation passing coverage before the pixel's early stop
     proj_bytes = 12 * n + 8 * n + (24 + 4 + 4 + 12 + 4 * k + 48 + 8) * v
-    sort_bytes = 4 * 16 * n
+    sort_bytes = (4 + 16 + 16) * n if st.depth_sort == 2 else 4 * 16 * n
     #  bin      value + rect read per visible splat (counting, then emission), (binning tile, splat) pair
     #           writes, pair radix passes x (key+value read and write), one key read for the tile ranges
     pair_bytes = 2 * 12 * v + 8 * bin_pairs + pair_passes * 16 * bin_pairs + 4 * bin_pairs
@@ -972,6 +974,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded 3-component Gaussian mixture, SH3)",
         "config": {"workload": "plain-1080p-300k-sh3", "geometry_path": args.geometry_path, "splats": n,
+                   "depth_sort": ["lookback radix", "reduce-then-scan radix", "bucket", "reused order"][st.depth_sort],
                    "width": view.width, "height": view.height,
                    "visible": v, "tile_pairs": pairs, "bin_pairs": bin_pairs, "bin_tile_px": bin_px,
                    "pair_sort_passes": pair_passes, "blend_evals": evals, "blend_tests": tests},

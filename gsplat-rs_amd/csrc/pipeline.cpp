@@ -366,8 +366,9 @@ void Pipeline::launch_frame() {
   }
   if (grew) buffer_grows_.fetch_add(1, std::memory_order_relaxed);
   const bool depth_lookback = radix_uses_lookback((uint32_t)m, 0, tuning_.rts_min_tiles);
-  // One short view sorts its depth keys by bucket (three launches instead of
-  // five); after a bucket overflow the next 64 frames use the radix passes.
+  // One short view sorts its depth keys by bucket (four launches instead of
+  // five, one pass over the keys instead of four); after a bucket overflow
+  // the next 64 frames use the radix passes.
   const bool depth_bucket =
       depth_lookback && nviews == 1 && !reuse_order_ && bucket_sort_ && bucket_off_ == 0 && m <= kBucketMaxKeys;
   if (bucket_off_) --bucket_off_;

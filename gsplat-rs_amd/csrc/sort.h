@@ -81,22 +81,26 @@ void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32
 
 // Bucket sort of short key arrays (n <= kBucketMaxKeys): the same result as
 // onesweep_sort(bits 32, descending) -- keys descending, equal keys in their
-// input order -- in three launches instead of five:
+// input order -- in three launches instead of five, one pass over the keys
+// instead of four:
 //   k_bucket_hist   reduces the key producer's per-workgroup ranges (smallest
 //                   non-zero key, largest key) to [kmin, kmax], picks the
-//                   shift s with (kmax - kmin) >> s <= 510 and counts the
-//                   buckets (kmax - key) >> s (key 0: bucket 511, last);
-//   k_os_pass<9>    one stable look-back pass by bucket;
-//   k_bucket_local  one workgroup per bucket sorts its run in LDS by the low
-//                   s bits of kmax - key (stable 8-bit radix passes).
-// A bucket of more than kSortTile keys (more than 4096 keys within one
-// 511th of the key range) is left unsorted and *overflow gets bit 0: the
+//                   shift s with (kmax - kmin) >> s < 8192 and counts the
+//                   fine bins (kmax - key) >> s of the non-zero keys;
+//   k_os_pass<9>    each workgroup first plans the buckets (consecutive fine
+//                   bins, about max(1024, visible / 510) keys each; key 0 in
+//                   the last), then one stable look-back pass by bucket;
+//   k_bucket_local  one workgroup per bucket sorts its run in LDS by
+//                   kmax - key (stable 8-bit radix passes over the bits its
+//                   fine bins span).
+// A bucket of more than kSortTile keys (a fine bin holding thousands of
+// equal or nearly equal keys) is left unsorted and *overflow gets bit 0: the
 // caller must then sort again with onesweep_sort.
 // The result ends in keys[0]/vals[0] (swapped with the alternates).
 // key_range[0..parts) are the producer's ranges (launch_project). The
 // scratch's first bucket_sort_lookback_words(n) words must be zero unless
-// scratch_clean is false (then they are cleared first); the sort leaves them
-// dirty. It uses bucket_sort_words(n) words in all.
+// scratch_clean is false (then they are cleared first); the sort leaves
+// bucket_sort_words(n) words dirty.
 constexpr uint32_t kBucketMaxKeys = 1u << 19;
 size_t bucket_sort_lookback_words(uint32_t n);
 size_t bucket_sort_words(uint32_t n);

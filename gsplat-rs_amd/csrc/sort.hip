@@ -83,19 +83,20 @@ constexpr uint32_t XCDS = 8;  // MI355X: workgroup b is dispatched to XCD b % 8
 constexpr uint32_t SCAN_IPT = 16;               // counts per thread of the reduce-then-scan scan
 constexpr uint32_t SCAN_CHUNK = T * SCAN_IPT;   // counts per scan chunk (one workgroup step)
 constexpr uint32_t kScanGrid = 1024;            // bound of the scan grid (workgroups loop over chunks)
+constexpr uint32_t kFineBins = 8192;            // bucket sort: fine bins of the key range
 
 using PassSet = RadixPasses;
 
-// from_value of a bucket pass: the digit is the key's bucket (kmax - key) >>
-// shift for non-zero keys, with kmax passed as `flip`, and the last digit for
-// key 0 (see bucket_sort).
+// from_value of a bucket pass: the digit of a non-zero key is the bucket
+// tab[(kmax - key) >> shift] of its fine bin, with kmax passed as `flip`, and
+// the last digit for key 0 (see bucket_sort).
 constexpr uint32_t kRangeDigit = 2;
 
 // Digits of DB bits: thread t < 2^DB keeps the books of digit t.
 template <uint32_t DB>
 __device__ __forceinline__ uint32_t digit(uint32_t key, uint32_t val, uint32_t shift, uint32_t flip,
-                                          uint32_t from_value) {
-  if (from_value == kRangeDigit) return key ? min((flip - key) >> shift, (1u << DB) - 2u) : (1u << DB) - 1u;
+                                          uint32_t from_value, const uint16_t *tab = nullptr) {
+  if (from_value == kRangeDigit) return key ? tab[min((flip - key) >> shift, kFineBins - 1)] : (1u << DB) - 1u;
   return (((from_value ? val : key) ^ flip) >> shift) & ((1u << DB) - 1u);
 }
 
@@ -225,12 +226,15 @@ __device__ __forceinline__ void clear_whist(uint32_t (*whist)[1u << DB]) {
 // item w*512 + r*64 + lane (w its wave). All loads are issued before any is
 // used: a round's LDS update would otherwise hold back the next round's
 // loads, one memory latency per round.
+// `per` (a multiple of 64, at most WAVE_ITEMS) spreads a short run over all
+// waves: wave w owns items [w * per, (w + 1) * per).
 __device__ __forceinline__ void load_tile(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
-                                          size_t base, uint32_t count, uint32_t (&key)[IPT], uint32_t (&val)[IPT]) {
+                                          size_t base, uint32_t count, uint32_t (&key)[IPT], uint32_t (&val)[IPT],
+                                          uint32_t per = WAVE_ITEMS) {
   const uint32_t w = threadIdx.x / 64, lane = __lane_id();
 #pragma unroll
   for (uint32_t r = 0; r < IPT; ++r) {
-    const uint32_t j = w * WAVE_ITEMS + r * 64 + lane;
+    const uint32_t j = w * per + r * 64 + lane;
     const bool active = j < count;
     key[r] = active ? kin[base + j] : 0u;
     val[r] = active ? vin[base + j] : 0u;
@@ -241,7 +245,8 @@ __device__ __forceinline__ void load_tile(const uint32_t *__restrict__ kin, cons
 template <uint32_t DB>
 __device__ __forceinline__ uint32_t rank_loaded(uint32_t count, uint32_t shift, uint32_t flip, uint32_t from_value,
                                                 const uint32_t (&key)[IPT], const uint32_t (&val)[IPT],
-                                                uint32_t (&rank)[IPT], uint32_t (*whist)[1u << DB]) {
+                                                uint32_t (&rank)[IPT], uint32_t (*whist)[1u << DB],
+                                                const uint16_t *tab = nullptr, uint32_t per = WAVE_ITEMS) {
   const uint32_t t = threadIdx.x, w = t / 64, lane = __lane_id();
   uint32_t *wh = whist[w];
   // Each round's leader (lowest lane of a digit group) adds the group's size
@@ -254,9 +259,15 @@ __device__ __forceinline__ uint32_t rank_loaded(uint32_t count, uint32_t shift, 
   uint32_t old[IPT], leader[IPT];
 #pragma unroll
   for (uint32_t r = 0; r < IPT; ++r) {
-    const uint32_t j = w * WAVE_ITEMS + r * 64 + lane;
+    if (r * 64 >= per) {  // a round past the wave's items (short runs): nothing to rank
+      old[r] = 0;
+      leader[r] = lane;
+      rank[r] = 0xFFFFFFFFu;
+      continue;
+    }
+    const uint32_t j = w * per + r * 64 + lane;
     const bool active = j < count;
-    const uint32_t d = digit<DB>(key[r], val[r], shift, flip, from_value);
+    const uint32_t d = digit<DB>(key[r], val[r], shift, flip, from_value, tab);
     const uint64_t peers = match_digit<DB>(d, active);
     const uint32_t lrank = (uint32_t)__popcll(peers & lanemask_lt());
     const bool lead = active && lrank == 0;
@@ -305,12 +316,12 @@ __device__ __forceinline__ void scatter_tile(const uint32_t (&key)[IPT], const u
                                              const uint32_t *lstart, const uint32_t *dst, uint32_t count,
                                              uint32_t shift, uint32_t flip, uint32_t from_value, uint32_t *skeys,
                                              uint32_t *svals, uint32_t n, uint32_t *__restrict__ kout,
-                                             uint32_t *__restrict__ vout) {
+                                             uint32_t *__restrict__ vout, const uint16_t *tab = nullptr) {
   const uint32_t t = threadIdx.x, w = t / 64;
 #pragma unroll
   for (uint32_t r = 0; r < IPT; ++r) {
     if (rank[r] != 0xFFFFFFFFu) {
-      const uint32_t d = digit<DB>(key[r], val[r], shift, flip, from_value);
+      const uint32_t d = digit<DB>(key[r], val[r], shift, flip, from_value, tab);
       const uint32_t pos = lstart[d] + whist[w][d] + rank[r];
       if (pos < TILE) {  // always, for ranks of one tile; bounds the LDS stage regardless
         skeys[pos] = key[r];
@@ -321,7 +332,7 @@ __device__ __forceinline__ void scatter_tile(const uint32_t (&key)[IPT], const u
   __syncthreads();
   for (uint32_t j = t; j < count; j += T) {
     const uint32_t k = skeys[j], v = svals[j];
-    const uint32_t d = digit<DB>(k, v, shift, flip, from_value);
+    const uint32_t d = digit<DB>(k, v, shift, flip, from_value, tab);
     const size_t pos = (size_t)dst[d] + (j - lstart[d]);
     if (pos < n) {
       kout[pos] = k;
@@ -330,20 +341,102 @@ __device__ __forceinline__ void scatter_tile(const uint32_t (&key)[IPT], const u
   }
 }
 
+// Bucket sort. The visible keys' range [kmin, kmax] is cut into kFineBins
+// fine bins, fine bin f = (kmax - key) >> s (largest keys first); the plan
+// groups consecutive fine bins into buckets of about `fill` keys (a new bucket
+// opens every fill keys of the running count, fill = max(1024, visible /
+// 510)), so one stable 9-bit pass by bucket leaves runs of at most fill + (one
+// fine bin) keys, and each run is sorted in LDS. Float depth keys are
+// logarithmic in depth, so a few far or near splats stretch the range only
+// mildly.
+constexpr uint32_t BUCKET_BITS = MAX_DIGIT_BITS;
+constexpr uint32_t BUCKETS = 1u << BUCKET_BITS;  // buckets of the visible keys, the last for key 0
+constexpr uint32_t FINE_PER = kFineBins / T;     // fine bins per thread of the plan
+static_assert(FINE_PER % 4 == 0, "fine bins are read four at a time");
+
+// Scratch words of a bucket sort of n keys, in order: the look-back region of
+// one 9-bit pass (digit histograms, unused here, tile counter, status words)
+// and the fine histogram, all zero on entry; then the range words: kmax, s,
+// the bucket starts [BUCKETS + 1], and the first and last fine bin of each
+// bucket (at kBucketFirst and kBucketLast from the range words).
+struct BucketLayout {
+  size_t fine, range, zero_words, words;
+};
+constexpr uint32_t kBucketFirst = 3 + BUCKETS;
+constexpr uint32_t kBucketLast = kBucketFirst + BUCKETS;
+constexpr uint32_t kBucketRangeWords = kBucketLast + BUCKETS;
+
+// Bucket sort, step 2 (the prologue of the bucket pass, in every workgroup):
+// from the fine histogram, fine bin f (running count P of the bins before it)
+// goes to bucket P / fill: tab[f] (LDS). Returns, for thread t < BUCKETS, the
+// count of bucket t (t = BUCKETS - 1: the n - visible zero keys). With
+// first/last, also stores each bucket's first and last non-empty fine bin.
+// plan: LDS words [3 * BUCKETS] (counts, first, last).
+__device__ __forceinline__ uint32_t bucket_plan(const uint32_t *__restrict__ fine, uint32_t n, uint16_t *tab,
+                                                uint32_t *plan, uint32_t *lds4, uint32_t *__restrict__ first,
+                                                uint32_t *__restrict__ last) {
+  const uint32_t t = threadIdx.x;
+  uint32_t *cnt = plan, *lo = plan + BUCKETS, *hi = plan + 2 * BUCKETS;
+  for (uint32_t j = t; j < BUCKETS; j += T) {
+    cnt[j] = 0;
+    lo[j] = 0xFFFFFFFFu;
+    hi[j] = 0;
+  }
+  uint32_t c[FINE_PER], sum = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < FINE_PER; q += 4) {
+    const uint4 x = reinterpret_cast<const uint4 *>(fine + t * FINE_PER)[q / 4];
+    c[q] = x.x;
+    c[q + 1] = x.y;
+    c[q + 2] = x.z;
+    c[q + 3] = x.w;
+    sum += x.x + x.y + x.z + x.w;
+  }
+  uint32_t p = block_exclusive_scan(sum, lds4);  // its barrier orders the clears above
+  uint32_t visible = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < WAVES; ++q) visible += lds4[q];
+  const uint32_t fill = max(1024u, (visible + BUCKETS - 3) / (BUCKETS - 2));
+#pragma unroll
+  for (uint32_t q = 0; q < FINE_PER; ++q) {
+    const uint32_t b = min(p / fill, BUCKETS - 2), f = t * FINE_PER + q;
+    tab[f] = (uint16_t)b;
+    p += c[q];
+    if (c[q]) {
+      atomicAdd(&cnt[b], c[q]);
+      if (first) {
+        atomicMin(&lo[b], f);
+        atomicMax(&hi[b], f);
+      }
+    }
+  }
+  __syncthreads();
+  if (first && t < BUCKETS) {
+    first[t] = lo[t];
+    last[t] = hi[t];
+  }
+  return t < BUCKETS - 1 ? cnt[t] : t == BUCKETS - 1 ? n - visible : 0u;
+}
+
 // Onesweep pass: one launch per digit, tile offsets by decoupled look-back.
 // Persistent: each workgroup takes tiles from the pass counter until they run
 // past the live element count, so the grid is bounded whatever the capacity.
 // Pass p's status words start at p * (live tiles) * 2^DB.
-// A bucket pass (from_value kRangeDigit) reads kmax and the shift from
-// range[0..1], and the workgroup of tile 0 stores every digit's start at
-// range[2 + d] (range[2 + 2^DB] = n).
+// A bucket pass (from_value kRangeDigit, 9-bit digits) reads kmax and the
+// fine shift from range[0..1] and the fine-bin -> bucket table from
+// `bucket_tab` (u16) into LDS, and the workgroup of tile 0 stores every
+// digit's start at range[2 + d] (range[2 + 2^DB] = n).
 template <uint32_t DB>
 __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                const uint32_t *__restrict__ n_dev, uint32_t n_cap, uint32_t pass,
                                                uint32_t shift, uint32_t flip, uint32_t from_value,
                                                const uint32_t *__restrict__ hist, uint32_t *__restrict__ counter,
-                                               uint32_t *__restrict__ status_all, uint32_t *__restrict__ range) {
+                                               uint32_t *__restrict__ status_all, uint32_t *__restrict__ range,
+                                               const uint32_t *__restrict__ fine) {
+  constexpr bool kBucket = DB == BUCKET_BITS;  // only 9-bit passes carry the bucket plan's LDS
+  __shared__ uint16_t tab[kBucket ? kFineBins : 1];
+  __shared__ uint32_t plan[kBucket ? 3 * BUCKETS : 1];
   if (from_value == kRangeDigit) {
     flip = range[0];
     shift = range[1];
@@ -373,7 +466,12 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
   uint32_t key[IPT], val[IPT], rank[IPT];
   load_tile(kin, vin, base, count, key, val);
   uint32_t hd = 0;
-  if (t < D) {
+  if (kBucket && from_value == kRangeDigit) {
+    // every workgroup plans the buckets (fine bin -> bucket table in LDS);
+    // tile 0's also stores each bucket's fine bin span for the local sort
+    hd = bucket_plan(fine, n, tab, plan, lds4, tile == 0 ? range + kBucketFirst : nullptr,
+                     tile == 0 ? range + kBucketLast : nullptr);
+  } else if (t < D) {
 #pragma unroll
     for (uint32_t c = 0; c < STRIPES; ++c) hd += hist[(c * MAX_PASSES + pass) * D + t];
   }
@@ -397,7 +495,7 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
         }
       }
     } else {
-      const uint32_t run = rank_loaded<DB>(count, shift, flip, from_value, key, val, rank, whist);
+      const uint32_t run = rank_loaded<DB>(count, shift, flip, from_value, key, val, rank, whist, tab);
 
       // Publish this tile's count of digit t, then look back for the prefix.
       uint32_t excl = 0;
@@ -440,7 +538,7 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
       }
       __syncthreads();
       scatter_tile<DB>(key, val, rank, whist, lstart, gofs, count, shift, flip, from_value, skeys, svals, n, kout,
-                       vout);
+                       vout, tab);
     }
     __syncthreads();
     if (t == 0) s_tile = atomicAdd(counter, 1u);
@@ -642,22 +740,18 @@ __global__ __launch_bounds__(T) void k_rts_down(const uint32_t *__restrict__ kin
   }
 }
 
-// Bucket sort, step 1 (one launch, grid-stride over 1024-key chunks like
-// k_os_hist): every workgroup reduces the producer's per-workgroup key
-// ranges to kmin (smallest non-zero key) and kmax, takes the smallest shift
-// s with (kmax - kmin) >> s <= 510, and counts the buckets of its keys into
-// the look-back histogram of pass 0 (stripe = workgroup mod STRIPES);
+// Bucket sort, step 1 (grid-stride over 4096-key chunks): every workgroup
+// reduces the producer's per-workgroup key ranges to kmin (smallest non-zero
+// key) and kmax, takes the smallest shift s with (kmax - kmin) >> s <
+// kFineBins, and adds its non-zero keys' fine bins into the fine histogram;
 // workgroup 0 stores range[0] = kmax, range[1] = s.
-constexpr uint32_t BUCKET_BITS = MAX_DIGIT_BITS;
-constexpr uint32_t BUCKETS = 1u << BUCKET_BITS;  // 511 for the key range, the last for key 0
-
 __global__ __launch_bounds__(T) void k_bucket_hist(const uint32_t *__restrict__ keys, uint32_t n,
                                                    const uint2 *__restrict__ parts, uint32_t nparts,
-                                                   uint32_t *__restrict__ hist, uint32_t *__restrict__ range) {
-  __shared__ uint32_t h[BUCKETS];
+                                                   uint32_t *__restrict__ fine, uint32_t *__restrict__ range) {
+  __shared__ uint32_t h[kFineBins];
   __shared__ uint32_t red[2][WAVES];
   const uint32_t t = threadIdx.x, lane = __lane_id(), w = t / 64;
-  for (uint32_t i = t; i < BUCKETS; i += T) h[i] = 0;
+  for (uint32_t i = t; i < kFineBins; i += T) h[i] = 0;
   uint32_t mn = 0xFFFFFFFFu, mx = 0;
   for (uint32_t i = t; i < nparts; i += T) {
     const uint2 p = parts[i];
@@ -681,68 +775,78 @@ __global__ __launch_bounds__(T) void k_bucket_hist(const uint32_t *__restrict__ 
   }
   const uint32_t span = mn == 0xFFFFFFFFu ? 0u : mx - mn;  // no non-zero key: every key is 0
   uint32_t s = 0;
-  while ((span >> s) > BUCKETS - 2u) ++s;
+  while ((span >> s) >= kFineBins) ++s;
   if (blockIdx.x == 0 && t == 0) {
     range[0] = mx;
     range[1] = s;
   }
-  for (size_t base = (size_t)blockIdx.x * HIST_CHUNK; base < n; base += (size_t)gridDim.x * HIST_CHUNK) {
-    uint32_t k[HIST_IPT];
+  for (size_t base = (size_t)blockIdx.x * TILE; base < n; base += (size_t)gridDim.x * TILE) {
+    uint32_t k[IPT];
 #pragma unroll
-    for (uint32_t r = 0; r < HIST_IPT; ++r) {
+    for (uint32_t r = 0; r < IPT; ++r) {
       const size_t i = base + (size_t)r * T + t;
       k[r] = i < n ? keys[i] : 0u;
     }
 #pragma unroll
-    for (uint32_t r = 0; r < HIST_IPT; ++r)
-      if (base + (size_t)r * T + t < n) wave_count(h, digit<BUCKET_BITS>(k[r], 0u, s, mx, kRangeDigit));
+    for (uint32_t r = 0; r < IPT; ++r)
+      if (k[r] != 0u) atomicAdd(&h[min((mx - k[r]) >> s, kFineBins - 1)], 1u);
   }
   __syncthreads();
-  uint32_t *dst = hist + (size_t)(blockIdx.x % STRIPES) * MAX_PASSES * BUCKETS;  // pass 0
-  for (uint32_t i = t; i < BUCKETS; i += T) {
+  for (uint32_t i = t; i < kFineBins; i += T) {
     const uint32_t c = h[i];
-    if (c) atomicAdd(&dst[i], c);
+    if (c) atomicAdd(&fine[i], c);
   }
 }
 
-// Bucket sort, step 3: workgroup b sorts bucket b's run [range[2 + b],
-// range[3 + b]) in place: keys become kmax - key (the low s bits; the bucket
-// holds the high bits), are ranked by stable 8-bit LDS passes, ascending,
-// and are restored on the way out. Equal keys keep the order the stable
-// bucket pass gave them. A run longer than one tile sets *overflow.
+// Bucket sort, step 4: workgroup b sorts bucket b's run [range[2 + b],
+// range[3 + b]) in place. Its keys lie in fine bins first[b]..last[b], so
+// w = kmax - key - (first[b] << s) spans fewer than (last - first + 1) << s
+// values; stable 8-bit LDS passes over those bits order the run by w
+// ascending (key descending), equal keys keeping the order the stable
+// bucket pass gave them, and keys are restored on the way out. The run is
+// spread over all waves (per items each). A run longer than one tile sets
+// *overflow (a fine bin of more than about `fill` keys).
 __global__ __launch_bounds__(T) void k_bucket_local(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                    const uint32_t *__restrict__ range, uint32_t *__restrict__ overflow) {
+                                                    const uint32_t *__restrict__ range,
+                                                    const uint32_t *__restrict__ first,
+                                                    const uint32_t *__restrict__ last,
+                                                    uint32_t *__restrict__ overflow) {
+  constexpr uint32_t LB = 8, LD = 1u << LB;  // digits of a local pass (9-bit passes measured slower)
   __shared__ uint32_t lds4[WAVES];
-  __shared__ uint32_t lstart[256];
-  __shared__ uint32_t whist[WAVES + 1][256];  // row WAVES: rank_loaded's spare words
+  __shared__ uint32_t lstart[LD];
+  __shared__ uint32_t whist[WAVES + 1][LD];  // row WAVES: rank_loaded's spare words
   __shared__ uint32_t skeys[TILE];
   __shared__ uint32_t svals[TILE];
   const uint32_t b = blockIdx.x, t = threadIdx.x, w = t / 64, lane = __lane_id();
   const uint32_t kmax = range[0], s = range[1];
   const uint32_t off = range[2 + b], cnt = range[3 + b] - off;
-  // s == 0: a bucket holds one key value, already in order
-  if (b >= BUCKETS - 1 || cnt <= 1 || s == 0) return;
+  if (b >= BUCKETS - 1 || cnt <= 1) return;
   if (cnt > TILE) {
     if (t == 0) atomicOr(overflow, 1u);
     return;
   }
+  const uint32_t f0 = first[b], f1 = last[b];
+  const uint64_t wspan = (((uint64_t)(f1 - f0) + 1) << s) - 1;  // largest w
+  const uint32_t bits = wspan ? 64 - __builtin_clzll(wspan) : 0;
+  if (bits == 0) return;  // one key value: already in order
+  const uint32_t base = kmax - (f0 << s);
+  const uint32_t per = min(WAVE_ITEMS, (cnt + WAVES * 64 - 1) / (WAVES * 64) * 64);
   uint32_t key[IPT], val[IPT], rank[IPT];
-  load_tile(keys, vals, off, cnt, key, val);
-  const uint32_t low = s >= 32 ? 0xFFFFFFFFu : (1u << s) - 1u;
+  load_tile(keys, vals, off, cnt, key, val, per);
 #pragma unroll
-  for (uint32_t r = 0; r < IPT; ++r) key[r] = (kmax - key[r]) & low;
-  const uint32_t passes = (s + 7) / 8;
+  for (uint32_t r = 0; r < IPT; ++r) key[r] = base - key[r];
+  const uint32_t passes = (bits + LB - 1) / LB;
   for (uint32_t p = 0; p < passes; ++p) {
-    clear_whist<8>(whist);
+    clear_whist<LB>(whist);
     __syncthreads();
-    const uint32_t run = rank_loaded<8>(cnt, 8 * p, 0u, 0u, key, val, rank, whist);
+    const uint32_t run = rank_loaded<LB>(cnt, LB * p, 0u, 0u, key, val, rank, whist, nullptr, per);
     const uint32_t ls = block_exclusive_scan(run, lds4);
-    if (t < 256) lstart[t] = ls;
+    if (t < LD) lstart[t] = ls;
     __syncthreads();
 #pragma unroll
     for (uint32_t r = 0; r < IPT; ++r) {
       if (rank[r] != 0xFFFFFFFFu) {
-        const uint32_t d = (key[r] >> (8 * p)) & 255u;
+        const uint32_t d = (key[r] >> (LB * p)) & (LD - 1u);
         const uint32_t pos = lstart[d] + whist[w][d] + rank[r];
         if (pos < TILE) {  // always (ranks of one tile); bounds the LDS stage regardless
           skeys[pos] = key[r];
@@ -753,8 +857,8 @@ __global__ __launch_bounds__(T) void k_bucket_local(uint32_t *__restrict__ keys,
     __syncthreads();
 #pragma unroll
     for (uint32_t r = 0; r < IPT; ++r) {
-      const uint32_t j = w * WAVE_ITEMS + r * 64 + lane;
-      if (j < cnt) {
+      const uint32_t j = w * per + r * 64 + lane;
+      if (r * 64 < per && j < cnt) {
         key[r] = skeys[j];
         val[r] = svals[j];
       }
@@ -763,9 +867,9 @@ __global__ __launch_bounds__(T) void k_bucket_local(uint32_t *__restrict__ keys,
   }
 #pragma unroll
   for (uint32_t r = 0; r < IPT; ++r) {
-    const uint32_t j = w * WAVE_ITEMS + r * 64 + lane;
-    if (j < cnt) {
-      keys[off + j] = kmax - ((b << s) | key[r]);
+    const uint32_t j = w * per + r * 64 + lane;
+    if (r * 64 < per && j < cnt) {
+      keys[off + j] = base - key[r];
       vals[off + j] = val[r];
     }
   }
@@ -902,7 +1006,7 @@ void sort_passes(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t
   const uint32_t grid = std::min(grid_tiles, kPassGrid);
   for (uint32_t p = 0; p < ps.count; ++p) {
     k_os_pass<DB><<<grid, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, p, ps.shift[p], ps.flip[p],
-                                      ps.from_value[p], hist, counters + p, status, nullptr);
+                                      ps.from_value[p], hist, counters + p, status, nullptr, nullptr);
     std::swap(*keys, *keys_alt);
     std::swap(*vals, *vals_alt);
   }
@@ -911,32 +1015,42 @@ void sort_passes(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t
 
 }  // namespace
 
-size_t bucket_sort_lookback_words(uint32_t n) { return lookback_words(n, 1, BUCKETS); }
+static BucketLayout bucket_layout(uint32_t n) {
+  BucketLayout l{};
+  l.fine = lookback_words(n, 1, BUCKETS);
+  l.zero_words = l.fine + kFineBins;
+  l.range = l.zero_words;
+  l.words = l.range + kBucketRangeWords;
+  return l;
+}
 
-size_t bucket_sort_words(uint32_t n) { return bucket_sort_lookback_words(n) + 3 + BUCKETS; }
+size_t bucket_sort_lookback_words(uint32_t n) { return bucket_layout(n).zero_words; }
+
+size_t bucket_sort_words(uint32_t n) { return bucket_layout(n).words; }
 
 void bucket_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t **vals_alt, uint32_t n,
                  const uint2 *key_range, uint32_t parts, uint32_t *scratch, size_t scratch_words, bool scratch_clean,
                  uint32_t *overflow, hipStream_t st) {
   if (n <= 1) return;
   if (n > kBucketMaxKeys) throw Error(Code::Internal, "bucket sort of more than 2^19 keys");
-  if (bucket_sort_words(n) > scratch_words) throw Error(Code::Internal, "bucket sort scratch too small");
+  const BucketLayout l = bucket_layout(n);
+  if (l.words > scratch_words) throw Error(Code::Internal, "bucket sort scratch too small");
   const uint32_t tiles = div_up(n, TILE);
-  const size_t fixed = (size_t)STRIPES * MAX_PASSES * BUCKETS + MAX_PASSES;
-  uint32_t *hist = scratch;
+  uint32_t *hist = scratch;  // unused by a bucket pass (it plans its digit counts itself)
   uint32_t *counter = hist + (size_t)STRIPES * MAX_PASSES * BUCKETS;
   uint32_t *status = counter + MAX_PASSES;
-  uint32_t *range = scratch + bucket_sort_lookback_words(n);  // kmax, s, bucket starts [BUCKETS + 1]
+  uint32_t *range = scratch + l.range;
   if (!scratch_clean)
-    k_os_clear<<<std::min<uint32_t>(div_up(bucket_sort_lookback_words(n), T), 1024), T, 0, st>>>(
-        scratch, fixed, BUCKETS, nullptr, n);
-  k_bucket_hist<<<hist_blocks(n), T, 0, st>>>(*keys, n, key_range, parts, hist, range);
+    k_os_clear<<<std::min<uint32_t>(div_up(l.zero_words, T), 1024), T, 0, st>>>(scratch, l.zero_words, 0, nullptr,
+                                                                                  n);
+  k_bucket_hist<<<tiles, T, 0, st>>>(*keys, n, key_range, parts, scratch + l.fine, range);
   k_os_pass<BUCKET_BITS><<<std::min(tiles, kPassGrid), T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, nullptr, n,
                                                                    0, 0, 0, kRangeDigit, hist, counter, status,
-                                                                   range);
+                                                                   range, scratch + l.fine);
   std::swap(*keys, *keys_alt);
   std::swap(*vals, *vals_alt);
-  k_bucket_local<<<BUCKETS - 1, T, 0, st>>>(*keys, *vals, range, overflow);
+  k_bucket_local<<<BUCKETS - 1, T, 0, st>>>(*keys, *vals, range, range + kBucketFirst, range + kBucketLast,
+                                            overflow);
   GS_HIP(hipGetLastError());
 }
 

@@ -1,11 +1,13 @@
 """Bucket depth sort of one-view frames against the radix passes and the oracle.
 
-One view of at most 2^19 slots sorts its depth keys by bucket: one stable
-9-bit pass by (kmax - key) >> s over the frame's key range, then every bucket
-sorted in LDS (sort.h bucket_sort). The order must be the radix passes' order
-bit for bit (key descending, ties by splat index). A bucket of more than 4096
-keys overflows: the frame is rendered again with the radix passes and the next
-frames keep them for a while; the result must still be exact.
+One view of at most 2^19 slots sorts its depth keys by bucket: the key range
+cut into 8192 fine bins, consecutive bins grouped into buckets of about 1024
+keys, one stable 9-bit pass by bucket, then every bucket sorted in LDS
+(sort.h bucket_sort). The order must be the radix passes' order bit for bit
+(key descending, ties by splat index). A bucket of more than 4096 keys (a
+fine bin of more than about 3072 equal or near-equal keys) overflows: the
+frame is rendered again with the radix passes and the next frames keep them
+for a while; the result must still be exact.
 """
 
 import sys
@@ -48,10 +50,13 @@ def both_modes(abi, monkeypatch, scene, cam, w, h):
     return out
 
 
-@pytest.mark.parametrize("which", ["640x480-16k", "1080p-120k", "1080p-300k"])
+@pytest.mark.parametrize("which", ["640x480-16k", "640x480-16k-outliers", "1080p-120k", "1080p-300k"])
 def test_bucket_sort_equals_radix_passes(gpu, monkeypatch, which):
-    if which == "640x480-16k":
+    if which.startswith("640x480-16k"):
         scene, view = synth.plain_small()
+        if which.endswith("outliers"):
+            # a few splats far out stretch the key range (fine bins grow coarse)
+            scene.positions[:4] *= np.float32(300.0)
     elif which == "1080p-120k":
         scene, view = synth.plain_1080p(n=120_000, seed_offset=3)
     else:
@@ -77,8 +82,9 @@ def plane_scene(n_plane, n_spread, seed):
 
 
 def test_bucket_overflow_renders_again_exactly(gpu, oracle, monkeypatch):
-    # 16,000 equal keys in one bucket (> 4096): the bucket sort flags the
-    # frame, which is rendered again with the radix passes.
+    # 16,000 equal keys in one fine bin, so one bucket of more than 4096: the
+    # bucket sort flags the frame, which is rendered again with the radix
+    # passes.
     scene = plane_scene(16_000, 4_000, 11)
     cam = gpu.lib().gsplat_camera_default()
     w, h = 320, 240
@@ -104,12 +110,13 @@ def test_bucket_overflow_renders_again_exactly(gpu, oracle, monkeypatch):
 
 
 def test_bucket_sort_ties_below_capacity(gpu, oracle, monkeypatch):
-    # 3,000 equal keys (one bucket below its 4096 capacity) and a spread:
-    # equal keys must come out by splat index.
-    scene = plane_scene(3_000, 9_000, 12)
+    # 2,000 equal keys (one fine bin; with the bins before it in its bucket
+    # at most 1023 + 2000 keys, below the 4096 capacity) and a spread: equal
+    # keys must come out by splat index.
+    scene = plane_scene(2_000, 9_000, 12)
     cam = gpu.lib().gsplat_camera_default()
     monkeypatch.setenv("GSPLAT_SORT_BUCKET", "1")
-    say("[bucket ties] 3000 equal keys")
+    say("[bucket ties] 2000 equal keys")
     ctx = frame(gpu, scene, cam, 320, 240)
     assert ctx.stage_times().depth_sort == SORT_BUCKET
     osc = oracle_scene(oracle, scene)
