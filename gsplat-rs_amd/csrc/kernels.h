@@ -237,15 +237,22 @@ struct RasterOrder {
 //   Rows2     as Unrolled with two pixels per thread (two tile rows per
 //             workgroup, one LDS read of a splat for both pixels); every
 //             view's bin shift must be >= 1 (both rows in one binning tile)
+//   Queue     as Unrolled in a persistent grid (about the workgroups
+//             resident at once) that takes the (view, tile) work items from
+//             queues in the same order and XCD placement; `queue`
+//             (kRasterQueueWords words) must be zero at launch
 // All give bit-identical images. With `evals`, the counting variant also
 // adds to evals[0] the number of (pixel, splat) evaluations that pass
 // coverage before each pixel's early stop (the blend's algorithmic work) and
 // to evals[1] the coverage tests made.
-enum class RasterVariant : uint32_t { Linear = 0, Unrolled = 1, Rows2 = 2 };
+enum class RasterVariant : uint32_t { Linear = 0, Unrolled = 1, Rows2 = 2, Queue = 3 };
+constexpr uint32_t kQueueLanes = 16;   // persistent raster: queues per XCD
+constexpr uint32_t kQueueStride = 16;  // words between two queue counters (their own cache lines)
+constexpr uint32_t kRasterQueueWords = kXcds * kQueueLanes * kQueueStride;
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *rblock_start, uint32_t nviews,
                    uint32_t total_rblocks, const RasterOrder &order, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
                    const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t stream,
-                   unsigned long long *evals = nullptr);
+                   unsigned long long *evals = nullptr, uint32_t *queue = nullptr);
 
 // Device fill of n u32 words (graph-safe replacement of hipMemsetAsync).
 void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t stream);

@@ -1102,23 +1102,20 @@ __device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const 
 // COUNT: also adds to evals[0] the number of (pixel, splat) evaluations that
 // pass coverage before the pixel's early stop (the raster's algorithmic work)
 // and to evals[1] the number of coverage tests made.
+// The work of raster workgroup b (a 16x16 tile, see raster_tile) by the
+// calling workgroup, with the LDS stage s0/s1/s2 and the per-wave counts
+// wkeep.
 template <bool COUNT, int LOOP>
-__global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__ views,
-                                                     const uint32_t *__restrict__ rblock_start, uint32_t nviews,
-                                                     uint32_t total_rblocks, RasterOrder ro, uint32_t blend_min,
-                                                     const float4 *__restrict__ rec, const uint2 *__restrict__ rect,
-                                                     const uint2 *__restrict__ ranges,
-                                                     const uint32_t *__restrict__ list, float4 *__restrict__ img32,
-                                                     uchar4 *__restrict__ img8,
-                                                     unsigned long long *__restrict__ evals) {
-  // the LDS stage, three arrays of stage_capacity(blend_min) records (sized
-  // at launch: a smaller stage leaves room for more workgroups per CU)
-  extern __shared__ float4 stage_lds[];
-  const uint32_t cap = stage_capacity(blend_min);
-  float4 *s0 = stage_lds, *s1 = stage_lds + cap, *s2 = stage_lds + 2 * cap;
-  __shared__ uint32_t wkeep[kWaves];
+__device__ __forceinline__ void raster_workgroup(uint32_t b, const ViewDev *__restrict__ views,
+                                                 const uint32_t *__restrict__ rblock_start, uint32_t nviews,
+                                                 uint32_t total_rblocks, const RasterOrder &ro, uint32_t blend_min,
+                                                 const float4 *__restrict__ rec, const uint2 *__restrict__ rect,
+                                                 const uint2 *__restrict__ ranges, const uint32_t *__restrict__ list,
+                                                 float4 *__restrict__ img32, uchar4 *__restrict__ img8,
+                                                 unsigned long long *__restrict__ evals, float4 *s0, float4 *s1,
+                                                 float4 *s2, uint32_t *wkeep) {
   uint32_t v, tx, ty;
-  if (!raster_tile(views, rblock_start, nviews, total_rblocks, ro, blockIdx.x, v, tx, ty)) return;
+  if (!raster_tile(views, rblock_start, nviews, total_rblocks, ro, b, v, tx, ty)) return;
   const ViewDev &vw = views[v];
   const uint32_t shift = vw.bin_shift;
   const uint32_t width = vw.fp.width, height = vw.fp.height;
@@ -1195,6 +1192,67 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
     }
     if (lane == 0 && wp) atomicAdd(evals, (unsigned long long)wp);
     if (lane == 0 && u) atomicAdd(evals + 1, (unsigned long long)u);
+  }
+}
+
+template <bool COUNT, int LOOP>
+__global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__ views,
+                                                     const uint32_t *__restrict__ rblock_start, uint32_t nviews,
+                                                     uint32_t total_rblocks, RasterOrder ro, uint32_t blend_min,
+                                                     const float4 *__restrict__ rec, const uint2 *__restrict__ rect,
+                                                     const uint2 *__restrict__ ranges,
+                                                     const uint32_t *__restrict__ list, float4 *__restrict__ img32,
+                                                     uchar4 *__restrict__ img8,
+                                                     unsigned long long *__restrict__ evals) {
+  // the LDS stage, three arrays of stage_capacity(blend_min) records (sized
+  // at launch: a smaller stage leaves room for more workgroups per CU)
+  extern __shared__ float4 stage_lds[];
+  const uint32_t cap = stage_capacity(blend_min);
+  __shared__ uint32_t wkeep[kWaves];
+  raster_workgroup<COUNT, LOOP>(blockIdx.x, views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect,
+                                ranges, list, img32, img8, evals, stage_lds, stage_lds + cap, stage_lds + 2 * cap,
+                                wkeep);
+}
+
+// Persistent form of k_raster (the Unrolled loop): a grid of about what the
+// chip holds at once takes the raster workgroups' work items from queues.
+// Workgroup g runs on XCD x = g % kXcds and serves queue (x, y), y = (g /
+// kXcds) % kQueueLanes, which hands out the items b = x + kXcds (y +
+// kQueueLanes ticket), in order: the items keep the XCD placement and the
+// order of the plain launch (the raster tiles of one binning tile on one XCD,
+// heaviest lists first). Tickets are device-scope atomics, served where the
+// XCDs meet, so the counters are spread over kXcds * kQueueLanes cache lines
+// (kQueueStride words apart) and a workgroup asks for its next ticket before
+// working on its current item. The counters must be zero at launch.
+template <bool COUNT>
+__global__ __launch_bounds__(kThreads) void k_raster_queue(const ViewDev *__restrict__ views,
+                                                           const uint32_t *__restrict__ rblock_start, uint32_t nviews,
+                                                           uint32_t total_rblocks, RasterOrder ro, uint32_t blend_min,
+                                                           const float4 *__restrict__ rec,
+                                                           const uint2 *__restrict__ rect,
+                                                           const uint2 *__restrict__ ranges,
+                                                           const uint32_t *__restrict__ list,
+                                                           float4 *__restrict__ img32, uchar4 *__restrict__ img8,
+                                                           unsigned long long *__restrict__ evals,
+                                                           uint32_t *__restrict__ queue) {
+  extern __shared__ float4 stage_lds[];
+  const uint32_t cap = stage_capacity(blend_min);
+  __shared__ uint32_t wkeep[kWaves];
+  __shared__ uint32_t s_ticket;
+  const uint32_t x = blockIdx.x % kXcds, y = (blockIdx.x / kXcds) % kQueueLanes;
+  uint32_t *counter = queue + (size_t)(x * kQueueLanes + y) * kQueueStride;
+  uint32_t next = 0;
+  if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+  for (;;) {
+    if (threadIdx.x == 0) s_ticket = next;
+    __syncthreads();
+    const uint32_t b = x + kXcds * (y + kQueueLanes * s_ticket);
+    __syncthreads();  // every thread has the item before s_ticket changes
+    if (b >= total_rblocks) break;  // uniform: this queue is drained
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);  // the next ticket, fetched meanwhile
+    raster_workgroup<COUNT, 1>(b, views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
+                               list, img32, img8, evals, stage_lds, stage_lds + cap, stage_lds + 2 * cap, wkeep);
+    __syncthreads();  // the stage and wkeep are free for the next item
   }
 }
 
@@ -1504,13 +1562,42 @@ void launch_raster_order(const uint2 *ranges, uint32_t total_ctiles, uint32_t *c
   GS_HIP(hipGetLastError());
 }
 
+// Workgroups of the persistent raster: eight per CU (the raster's registers
+// allow seven resident at once; the eighth of each CU starts as soon as one
+// ends and finds its queue's remaining items or none), in whole runs of
+// kXcds * kQueueLanes so that every queue has workgroups.
+static uint32_t queue_grid() {
+  static thread_local uint32_t cached = 0;
+  if (cached) return cached;
+  int dev = 0, cus = 0;
+  GS_HIP(hipGetDevice(&dev));
+  GS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  constexpr uint32_t run = kXcds * kQueueLanes;
+  cached = std::max<uint32_t>(run, (uint32_t)std::max(cus, 1) * 8u / run * run);
+  return cached;
+}
+
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *rblock_start, uint32_t nviews,
                    uint32_t total_rblocks, const RasterOrder &ro, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
-                   const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t st, unsigned long long *evals) {
+                   const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t st, unsigned long long *evals,
+                   uint32_t *queue) {
   if (total_rblocks == 0 || nviews == 0) return;
   if (blend_min == 0 || blend_min > kThreads) throw Error(Code::InvalidArgument, "blend threshold outside 1..256");
   const dim3 grid(total_rblocks), block(kThreads);
   const size_t lds = 3 * (size_t)stage_capacity(blend_min) * sizeof(float4);
+  if (variant == RasterVariant::Queue) {
+    if (!queue) throw Error(Code::Internal, "persistent raster without its queue words");
+    constexpr uint32_t run = kXcds * kQueueLanes;  // every queue keeps at least one workgroup
+    const dim3 qgrid(std::min(queue_grid(), (total_rblocks + run - 1) / run * run));
+    if (evals)
+      k_raster_queue<true><<<qgrid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec,
+                                                      rect, ranges, list, img32, img8, evals, queue);
+    else
+      k_raster_queue<false><<<qgrid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec,
+                                                       rect, ranges, list, img32, img8, evals, queue);
+    GS_HIP(hipGetLastError());
+    return;
+  }
   if (variant == RasterVariant::Rows2) {  // every view at bin shift >= 1 (the caller's duty)
     if (evals)
       k_raster_rows2<true><<<grid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,

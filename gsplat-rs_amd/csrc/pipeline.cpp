@@ -32,13 +32,15 @@ constexpr size_t kTableBytes = table_layout(kMaxViews, kMaxViews).bytes;
 //   [3] overflow flag, [4..5] (16x16 tile, splat) pairs (u64),
 //   [6 .. 6+kMaxViews) visible count per view,
 //   kEvalsWord: u64 evaluations passing, u64 coverage tests (counting mode),
-//   kOrderWord: class sizes of the raster's heaviest-first order.
+//   kOrderWord: class sizes of the raster's heaviest-first order,
+//   kQueueWord: the persistent raster's queue counters.
 constexpr size_t kSortOverflowWord = 0;
 constexpr size_t kRasterPairsWord = 4;
 constexpr size_t kVisibleWord = 6;
 constexpr size_t kEvalsWord = kVisibleWord + kMaxViews;
 constexpr size_t kOrderWord = kEvalsWord + 4;  // raster order class sizes (kOrderClasses words)
-constexpr size_t kSummaryWords = kOrderWord + kOrderClasses;
+constexpr size_t kQueueWord = kOrderWord + kOrderClasses;  // persistent raster queues (kRasterQueueWords)
+constexpr size_t kSummaryWords = kQueueWord + kRasterQueueWords;
 
 static_assert(kSegmentAlign % kSortTile == 0, "view segments must be whole sort tiles");
 
@@ -529,7 +531,7 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   Clears frame_clears;
   frame_clears.add(info, kVisibleWord + nviews);
   frame_clears.add(reinterpret_cast<uint32_t *>(ranges_.get()), 2 * (size_t)ctiles);
-  frame_clears.add(info + kOrderWord, kOrderClasses);
+  frame_clears.add(info + kOrderWord, kOrderClasses + kRasterQueueWords);  // order class sizes, raster queues
   if (slots == 0) {
     for (int c = 0; c < Clears::kRanges; ++c) launch_fill_u32(frame_clears.ptr[c], 0u, frame_clears.words[c], st);
   }
@@ -630,7 +632,7 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
     variant = RasterVariant::Rows2;
   launch_raster(variant, views_d, rblock_start_d, nviews, rblocks, ro, tuning_.blend_min,
                 rec_.get(), rect_.get(),
-                ranges_.get(), pv, img32_.get(), img8_.get(), st, evals);
+                ranges_.get(), pv, img32_.get(), img8_.get(), st, evals, info + kQueueWord);
   if (stage_events) GS_HIP(hipEventRecord(ev_[4], st));
   for (size_t i = 0; i < host_rgba8_.size(); ++i) {
     if (!host_rgba8_[i] || host_views_[i].out8) continue;  // written by the raster itself
@@ -726,7 +728,8 @@ void Pipeline::set_bin_shift(uint32_t shift) {
 
 void Pipeline::set_tuning(const Tuning &t) {
   if (t.bin_shift > kMaxBinShift) throw Error(Code::InvalidArgument, "bin shift above " + std::to_string(kMaxBinShift));
-  if (t.raster != RasterVariant::Linear && t.raster != RasterVariant::Unrolled && t.raster != RasterVariant::Rows2)
+  if (t.raster != RasterVariant::Linear && t.raster != RasterVariant::Unrolled && t.raster != RasterVariant::Rows2 &&
+      t.raster != RasterVariant::Queue)
     throw Error(Code::InvalidArgument, "unknown raster variant");
   if (t.rts_min_tiles == 0) throw Error(Code::InvalidArgument, "sort mode threshold must be positive");
   if (t.blend_min == 0 || t.blend_min > 256) throw Error(Code::InvalidArgument, "blend threshold outside 1..256");
@@ -741,7 +744,7 @@ std::vector<TuneSample> Pipeline::autotune(const std::vector<ViewSpec> &views, u
   TuneSample best{base, 1e30f};
   for (uint32_t shift = 0; shift <= kMaxBinShift; ++shift)
     for (uint32_t rts : {128u, 256u, 1024u})
-      for (RasterVariant rv : {RasterVariant::Unrolled, RasterVariant::Rows2}) {
+      for (RasterVariant rv : {RasterVariant::Unrolled, RasterVariant::Rows2, RasterVariant::Queue}) {
       if (rv == RasterVariant::Rows2 && shift == 0) continue;  // runs as Unrolled there
       Tuning t = base;
       t.bin_shift = shift;

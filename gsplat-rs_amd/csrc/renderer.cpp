@@ -15,7 +15,10 @@ constexpr uint32_t kMaxDimension = 16384;
 RasterVariant raster_variant_from_env() {
   if (const char *v = std::getenv("GSPLAT_RASTER_VARIANT")) {
     const int k = std::atoi(v);
-    return k == 0 ? RasterVariant::Linear : k == 2 ? RasterVariant::Rows2 : RasterVariant::Unrolled;
+    return k == 0   ? RasterVariant::Linear
+           : k == 2 ? RasterVariant::Rows2
+           : k == 3 ? RasterVariant::Queue
+                    : RasterVariant::Unrolled;
   }
   return RasterVariant::Unrolled;
 }
