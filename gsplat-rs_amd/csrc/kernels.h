@@ -175,15 +175,27 @@ void launch_tile_counts(const ViewDev *views, const uint32_t *seg_start, uint32_
 void launch_block_scan(uint32_t *block_sums, uint32_t blocks, uint32_t capacity, uint32_t *m_out, uint32_t *info,
                        const uint32_t *block_pairs, unsigned long long *raster_pairs, hipStream_t stream);
 
+// Digit histograms of a look-back pair sort of at most kPairHistPasses
+// passes, accumulated by the pair emission instead of a histogram launch
+// (sort.h SortPrep::hist_ready): hist[(stripe * kSortMaxPasses + p) *
+// 2^digit_bits + digit] with digit (key >> shift[p]) & (2^digit_bits - 1)
+// and stripe = emission workgroup mod kSortStripes; hist zero beforehand.
+constexpr uint32_t kPairHistPasses = 2;
+struct PairHist {
+  uint32_t *hist = nullptr;  // null: no histogram
+  uint32_t passes = 0, digit_bits = 8;
+  uint32_t shift[kPairHistPasses] = {0, 0};
+};
+
 // Writes (view << 24 | binning tile of the view, record slot) pairs at the
 // scanned offsets: each view's pairs form one contiguous run, front to back.
 // Pairs at or beyond `capacity` are dropped (the frame is then flagged and
-// rendered again).
+// rendered again). With hist.hist, also counts the written pairs' digits.
 void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
                       const uint32_t *svals, const uint32_t *rect_lo, const uint32_t *rect_hi,
                       const uint32_t *counts_rev,
                       const uint32_t *offsets_rev, const uint32_t *block_sums, uint32_t capacity,
-                      uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t stream);
+                      uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t stream, const PairHist &hist = PairHist());
 
 // Words copied by one workgroup of a kernel, dst[i] = src[i] for i < words
 // (dst may be pinned host memory: the frame's summary, read by the host

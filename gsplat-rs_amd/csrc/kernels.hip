@@ -20,6 +20,7 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "sort.h"
 
 namespace gs {
 
@@ -750,7 +751,8 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
                                                         const uint32_t *__restrict__ offsets_rev,
                                                         const uint32_t *__restrict__ block_sums, uint32_t capacity,
                                                         uint32_t *__restrict__ pair_keys,
-                                                        uint32_t *__restrict__ pair_vals) {
+                                                        uint32_t *__restrict__ pair_vals, PairHist ph) {
+  __shared__ uint32_t s_hist[kPairHistPasses][1u << 9];
   __shared__ uint32_t s_first[kThreads + 1];
   __shared__ uint32_t s_slot[kThreads];
   __shared__ uint32_t s_x0[kThreads], s_y0[kThreads], s_w[kThreads];
@@ -765,6 +767,9 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
   const uint32_t g = g0 + kThreads - 1 - t;
   const uint32_t slot = vw.seg_off + (svals[g] - (v << kLocalBits));  // loaded beside cnt
   const uint2 b = make_uint2(rect_lo[g], rect_hi[g]);                 // in slot order
+  const uint32_t dmask = (1u << ph.digit_bits) - 1u;
+  if (ph.hist)
+    for (uint32_t i = t; i < ph.passes << ph.digit_bits; i += kThreads) s_hist[i >> ph.digit_bits][i & dmask] = 0;
   s_first[t] = first;
   if (t == kThreads - 1) s_first[kThreads] = first + cnt;
   if (cnt) {
@@ -791,8 +796,19 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
     }
     const uint32_t k = p - s_first[lo];
     const uint32_t w = s_w[lo];
-    pair_keys[p] = (v << kPairViewShift) | ((s_y0[lo] + k / w) * tiles_x + s_x0[lo] + k % w);
+    const uint32_t tile = (s_y0[lo] + k / w) * tiles_x + s_x0[lo] + k % w;
+    pair_keys[p] = (v << kPairViewShift) | tile;
     pair_vals[p] = s_slot[lo];
+    if (ph.hist)
+      for (uint32_t q = 0; q < ph.passes; ++q) atomicAdd(&s_hist[q][(tile >> ph.shift[q]) & dmask], 1u);
+  }
+  if (ph.hist) {
+    __syncthreads();
+    uint32_t *dst = ph.hist + (size_t)(blockIdx.x % kSortStripes) * kSortMaxPasses * (dmask + 1);
+    for (uint32_t i = t; i < ph.passes << ph.digit_bits; i += kThreads) {
+      const uint32_t c = s_hist[i >> ph.digit_bits][i & dmask];
+      if (c) atomicAdd(&dst[i], c);
+    }
   }
 }
 
@@ -1524,12 +1540,14 @@ void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t 
                       const uint32_t *svals, const uint32_t *rect_lo, const uint32_t *rect_hi,
                       const uint32_t *counts_rev,
                       const uint32_t *offsets_rev, const uint32_t *block_sums, uint32_t capacity,
-                      uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t st) {
+                      uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t st, const PairHist &ph) {
   if (total_slots == 0) return;
+  if (ph.hist && (ph.passes > kPairHistPasses || ph.digit_bits > 9))
+    throw Error(Code::Internal, "pair histogram of more than two passes or 9-bit digits");
   k_tile_emit<<<total_slots / kThreads, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, svals,
                                                              rect_lo, rect_hi, counts_rev, offsets_rev, block_sums,
                                                              capacity,
-                                                             pair_keys, pair_vals);
+                                                             pair_keys, pair_vals, ph);
   GS_HIP(hipGetLastError());
 }
 

@@ -589,11 +589,22 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
                      offsets_.get(), visible, block_sums_.get(), block_pairs_.get(), ka, va, scan_clears, st);
   launch_block_scan(block_sums_.get(), count_blocks, pair_cap_, m_dev_.get(), info, block_pairs_.get(), raster_pairs,
                     st);
-  launch_tile_emit(views_d, seg_start_d, nviews, slots, v, ka, va, counts_.get(), offsets_.get(),
-                   block_sums_.get(), pair_cap_, pair_keys_.get(), pair_vals_.get(), st);
-  uint32_t *pk = pair_keys_.get(), *pv = pair_vals_.get(), *pka = pair_keys_alt_.get(), *pva = pair_vals_alt_.get();
+  // A look-back pair sort of at most two passes takes its digit histograms
+  // from the emission (its scratch is clean by then: any fill runs first).
   SortPrep pair_prep;
   if (plan.pair_scratch.fill) launch_fill_u32(pair_scratch_.get(), 0u, plan.pair_scratch.fill, st);
+  PairHist pair_hist;
+  const RadixPasses pair_ps = radix_passes(tile_bits, /*descending=*/false, -1);
+  if (pair_lookback && plan.pair_scratch.skip_clear && pair_ps.count <= kPairHistPasses) {
+    pair_hist.hist = pair_scratch_.get();
+    pair_hist.passes = pair_ps.count;
+    pair_hist.digit_bits = pair_ps.digit_bits;
+    for (uint32_t p = 0; p < pair_ps.count; ++p) pair_hist.shift[p] = pair_ps.shift[p];
+    pair_prep.hist_ready = true;
+  }
+  launch_tile_emit(views_d, seg_start_d, nviews, slots, v, ka, va, counts_.get(), offsets_.get(),
+                   block_sums_.get(), pair_cap_, pair_keys_.get(), pair_vals_.get(), st, pair_hist);
+  uint32_t *pk = pair_keys_.get(), *pv = pair_vals_.get(), *pka = pair_keys_alt_.get(), *pva = pair_vals_alt_.get();
   pair_prep.scratch_clean = plan.pair_scratch.skip_clear;
   pair_prep.rts_min_tiles = tuning_.rts_min_tiles;
   onesweep_sort(&pk, &pv, &pka, &pva, pair_cap_, m_dev_.get(), tile_bits, /*descending=*/false,
