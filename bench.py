@@ -462,7 +462,7 @@ def _kernel_short(name):
 # order), k_raster<...> or k_raster_rows2<...> the raster. Sort kernels are shared by the depth and pair sorts, so launch
 # order (not the name) tells them apart.
 STAGES = ["project", "sort", "bin", "raster"]
-STAGE_KERNEL = {"project": "k_project", "sort": "depth radix sort", "bin": "bin(pair sort+emit)",
+STAGE_KERNEL = {"project": "k_project", "sort": "depth sort", "bin": "bin(pair sort+emit)",
                 "raster": "k_raster"}
 
 
@@ -940,7 +940,7 @@ def main():
     rooflines = {
         "k_project": {"bound": "hbm", "achieved": proj_bytes / (med[0] * 1e-3) / 1e9, "peak": MI355X_HBM_GBPS,
                       "unit": "GB/s", "algorithmic": proj_bytes},
-        "depth radix sort": {"bound": "hbm", "achieved": sort_bytes / (med[1] * 1e-3) / 1e9,
+        "depth sort": {"bound": "hbm", "achieved": sort_bytes / (med[1] * 1e-3) / 1e9,
                              "peak": MI355X_HBM_GBPS, "unit": "GB/s", "algorithmic": sort_bytes},
         "bin(pair sort+emit)": {"bound": "hbm", "achieved": pair_bytes / (med[2] * 1e-3) / 1e9,
                                 "peak": MI355X_HBM_GBPS, "unit": "GB/s", "algorithmic": pair_bytes},

@@ -86,7 +86,7 @@ def test_summarize_batch_profiles(tmp_path):
     assert float(tc[5]) == pytest.approx(5.0 * 1024 * wf / 1e6, abs=0.05)
     # per stage (us per batch, fetch MB, write MB): the rows2 raster is the raster stage
     assert float(rows["k_raster"][0]) == pytest.approx(2.0, abs=0.05)
-    assert float(rows["depth radix sort"][0]) == pytest.approx(0.2, abs=0.05)
+    assert float(rows["depth sort"][0]) == pytest.approx(0.2, abs=0.05)
     assert float(rows["bin(pair sort+emit)"][0]) == pytest.approx(0.9, abs=0.05)
     assert float(rows["bin(pair sort+emit)"][1]) == pytest.approx(240.5 * 1024 * ff / 1e6, abs=0.05)
     assert float(rows["k_project"][2]) == pytest.approx(50.0 * 1024 * wf / 1e6, abs=0.05)
