@@ -87,9 +87,10 @@ void onesweep_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32
 //                   non-zero key, largest key) to [kmin, kmax], picks the
 //                   shift s with (kmax - kmin) >> s < 8192 and counts the
 //                   fine bins (kmax - key) >> s of the non-zero keys;
-//   k_os_pass<9>    each workgroup first plans the buckets (consecutive fine
-//                   bins, about max(1024, visible / 510) keys each; key 0 in
-//                   the last), then one stable look-back pass by bucket;
+//   k_os_pass<8>    each workgroup first plans the buckets (consecutive fine
+//                   bins, about max(1024, visible / 254) keys each; key 0 in
+//                   the last of 256), then one stable look-back pass by
+//                   bucket;
 //   k_bucket_local  one workgroup per bucket sorts its run in LDS by
 //                   kmax - key (stable 8-bit radix passes over the bits its
 //                   fine bins span).

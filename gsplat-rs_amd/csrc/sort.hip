@@ -345,11 +345,13 @@ __device__ __forceinline__ void scatter_tile(const uint32_t (&key)[IPT], const u
 // fine bins, fine bin f = (kmax - key) >> s (largest keys first); the plan
 // groups consecutive fine bins into buckets of about `fill` keys (a new bucket
 // opens every fill keys of the running count, fill = max(1024, visible /
-// 510)), so one stable 9-bit pass by bucket leaves runs of at most fill + (one
-// fine bin) keys, and each run is sorted in LDS. Float depth keys are
+// 254)), so one stable 8-bit pass by bucket leaves runs of at most fill + (one
+// fine bin) keys, and each run is sorted in LDS. (An 8-bit pass costs about
+// two thirds of a 9-bit one; at 2^19 keys fill is 2065, still well inside the
+// 4096 keys of a local sort.) Float depth keys are
 // logarithmic in depth, so a few far or near splats stretch the range only
 // mildly.
-constexpr uint32_t BUCKET_BITS = MAX_DIGIT_BITS;
+constexpr uint32_t BUCKET_BITS = 8;
 constexpr uint32_t BUCKETS = 1u << BUCKET_BITS;  // buckets of the visible keys, the last for key 0
 constexpr uint32_t FINE_PER = kFineBins / T;     // fine bins per thread of the plan
 static_assert(FINE_PER % 4 == 0, "fine bins are read four at a time");
@@ -422,11 +424,11 @@ __device__ __forceinline__ uint32_t bucket_plan(const uint32_t *__restrict__ fin
 // Persistent: each workgroup takes tiles from the pass counter until they run
 // past the live element count, so the grid is bounded whatever the capacity.
 // Pass p's status words start at p * (live tiles) * 2^DB.
-// A bucket pass (from_value kRangeDigit, 9-bit digits) reads kmax and the
-// fine shift from range[0..1] and the fine-bin -> bucket table from
-// `bucket_tab` (u16) into LDS, and the workgroup of tile 0 stores every
-// digit's start at range[2 + d] (range[2 + 2^DB] = n).
-template <uint32_t DB>
+// A bucket pass (BUCKET, from_value kRangeDigit, BUCKET_BITS-bit digits)
+// reads kmax and the fine shift from range[0..1], plans the fine-bin ->
+// bucket table into LDS from the fine histogram, and the workgroup of tile 0
+// stores every digit's start at range[2 + d] (range[2 + 2^DB] = n).
+template <uint32_t DB, bool BUCKET = false>
 __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                const uint32_t *__restrict__ n_dev, uint32_t n_cap, uint32_t pass,
@@ -434,10 +436,11 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
                                                const uint32_t *__restrict__ hist, uint32_t *__restrict__ counter,
                                                uint32_t *__restrict__ status_all, uint32_t *__restrict__ range,
                                                const uint32_t *__restrict__ fine) {
-  constexpr bool kBucket = DB == BUCKET_BITS;  // only 9-bit passes carry the bucket plan's LDS
+  constexpr bool kBucket = BUCKET;  // only bucket passes carry the bucket plan's LDS
+  static_assert(!BUCKET || DB == BUCKET_BITS, "a bucket pass has one digit per bucket");
   __shared__ uint16_t tab[kBucket ? kFineBins : 1];
   __shared__ uint32_t plan[kBucket ? 3 * BUCKETS : 1];
-  if (from_value == kRangeDigit) {
+  if (kBucket && from_value == kRangeDigit) {
     flip = range[0];
     shift = range[1];
   }
@@ -476,7 +479,7 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
     for (uint32_t c = 0; c < STRIPES; ++c) hd += hist[(c * MAX_PASSES + pass) * D + t];
   }
   const uint32_t g = block_exclusive_scan(hd, lds4);
-  if (from_value == kRangeDigit && tile == 0) {
+  if (kBucket && from_value == kRangeDigit && tile == 0) {
     if (t < D) range[2 + t] = g;
     if (t == 0) range[2 + D] = n;
   }
@@ -1044,7 +1047,7 @@ void bucket_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t
     k_os_clear<<<std::min<uint32_t>(div_up(l.zero_words, T), 1024), T, 0, st>>>(scratch, l.zero_words, 0, nullptr,
                                                                                   n);
   k_bucket_hist<<<tiles, T, 0, st>>>(*keys, n, key_range, parts, scratch + l.fine, range);
-  k_os_pass<BUCKET_BITS><<<std::min(tiles, kPassGrid), T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, nullptr, n,
+  k_os_pass<BUCKET_BITS, true><<<std::min(tiles, kPassGrid), T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, nullptr, n,
                                                                    0, 0, 0, kRangeDigit, hist, counter, status,
                                                                    range, scratch + l.fine);
   std::swap(*keys, *keys_alt);
