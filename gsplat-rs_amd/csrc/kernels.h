@@ -223,6 +223,12 @@ void launch_raster_tile_counts(const ViewDev *views, uint32_t v, const ViewDev &
 // classes by list length (class 0 longest), order[class * total_ctiles +
 // place] = global binning tile, counts[class] = class size (zero on entry).
 constexpr uint32_t kOrderClasses = 8;
+// Class of a binning tile with `len` list entries: clamp(12 - floor(log2
+// len), 0, kOrderClasses - 1) (class 0: 4096 or more; empty lists last).
+__host__ __device__ inline uint32_t raster_order_class(uint32_t len) {
+  const int lg = len ? 31 - __builtin_clz(len) : 0;
+  return (uint32_t)(lg >= 12 ? 0 : (12 - lg < (int)kOrderClasses ? 12 - lg : (int)kOrderClasses - 1));
+}
 void launch_raster_order(const uint2 *ranges, uint32_t total_ctiles, uint32_t *counts, uint32_t *order,
                          hipStream_t stream);
 
@@ -256,7 +262,10 @@ struct RasterOrder {
 // All give bit-identical images. With `evals`, the counting variant also
 // adds to evals[0] the number of (pixel, splat) evaluations that pass
 // coverage before each pixel's early stop (the blend's algorithmic work) and
-// to evals[1] the coverage tests made.
+// to evals[1] the coverage tests made. `clears` and `copy` are work the
+// launch does for the kernels before it (see raster_prologue): words to zero
+// and the frame summary to hand to pinned memory; neither may overlap what
+// the raster reads.
 enum class RasterVariant : uint32_t { Linear = 0, Unrolled = 1, Rows2 = 2, Queue = 3 };
 constexpr uint32_t kQueueLanes = 16;   // persistent raster: queues per XCD
 constexpr uint32_t kQueueStride = 16;  // words between two queue counters (their own cache lines)
@@ -264,7 +273,8 @@ constexpr uint32_t kRasterQueueWords = kXcds * kQueueLanes * kQueueStride;
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *rblock_start, uint32_t nviews,
                    uint32_t total_rblocks, const RasterOrder &order, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
                    const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t stream,
-                   unsigned long long *evals = nullptr, uint32_t *queue = nullptr);
+                   unsigned long long *evals = nullptr, uint32_t *queue = nullptr, const Clears &clears = Clears(),
+                   const WordCopy &copy = WordCopy());
 
 // Device fill of n u32 words (graph-safe replacement of hipMemsetAsync).
 void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t stream);

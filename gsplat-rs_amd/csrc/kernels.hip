@@ -873,9 +873,7 @@ __global__ __launch_bounds__(kThreads) void k_raster_order(const uint2 *__restri
   uint32_t cls = 0, place = 0;
   if (c < total_ctiles) {
     const uint2 r = ranges[c];
-    const uint32_t len = r.y - r.x;
-    const int lg = len ? 31 - __clz(len) : 0;
-    cls = (uint32_t)min(max(12 - lg, 0), (int)kOrderClasses - 1);
+    cls = raster_order_class(r.y - r.x);
     place = atomicAdd(&local[cls], 1u);
   }
   __syncthreads();
@@ -911,6 +909,16 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
 }
 
 // ------------------------------------------------------------------ raster
+
+// Work a raster launch may take over from the binning kernels it follows:
+// zeroing scratch words for the next frame (clears) and handing the frame
+// summary to pinned host memory (copy, by workgroup 0). Neither touches
+// what the raster reads.
+__device__ __forceinline__ void raster_prologue(const Clears &clears, const WordCopy &copy) {
+  clear_ranges(clears);
+  if (blockIdx.x == 0)
+    for (uint32_t i = threadIdx.x; i < copy.words; i += kThreads) copy.dst[i] = copy.src[i];
+}
 
 // Records a raster stage holds: a blend starts once blend_min are staged, so
 // at most blend_min - 1 are left when up to kThreads more are added; the
@@ -1219,7 +1227,9 @@ __global__ __launch_bounds__(kThreads) void k_raster(const ViewDev *__restrict__
                                                      const uint2 *__restrict__ ranges,
                                                      const uint32_t *__restrict__ list, float4 *__restrict__ img32,
                                                      uchar4 *__restrict__ img8,
-                                                     unsigned long long *__restrict__ evals) {
+                                                     unsigned long long *__restrict__ evals, Clears clears,
+                                                     WordCopy copy) {
+  raster_prologue(clears, copy);
   // the LDS stage, three arrays of stage_capacity(blend_min) records (sized
   // at launch: a smaller stage leaves room for more workgroups per CU)
   extern __shared__ float4 stage_lds[];
@@ -1250,7 +1260,9 @@ __global__ __launch_bounds__(kThreads) void k_raster_queue(const ViewDev *__rest
                                                            const uint32_t *__restrict__ list,
                                                            float4 *__restrict__ img32, uchar4 *__restrict__ img8,
                                                            unsigned long long *__restrict__ evals,
-                                                           uint32_t *__restrict__ queue) {
+                                                           uint32_t *__restrict__ queue, Clears clears,
+                                                           WordCopy copy) {
+  raster_prologue(clears, copy);
   extern __shared__ float4 stage_lds[];
   const uint32_t cap = stage_capacity(blend_min);
   __shared__ uint32_t wkeep[kWaves];
@@ -1341,7 +1353,9 @@ __global__ __launch_bounds__(kThreads) void k_raster_rows2(const ViewDev *__rest
                                                            const uint2 *__restrict__ ranges,
                                                            const uint32_t *__restrict__ list,
                                                            float4 *__restrict__ img32, uchar4 *__restrict__ img8,
-                                                           unsigned long long *__restrict__ evals) {
+                                                           unsigned long long *__restrict__ evals, Clears clears,
+                                                           WordCopy copy) {
+  raster_prologue(clears, copy);
   extern __shared__ float4 stage_lds[];
   const uint32_t cap = stage_capacity(blend_min);
   float4 *s0 = stage_lds, *s1 = stage_lds + cap, *s2 = stage_lds + 2 * cap;
@@ -1598,7 +1612,7 @@ static uint32_t queue_grid() {
 void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *rblock_start, uint32_t nviews,
                    uint32_t total_rblocks, const RasterOrder &ro, uint32_t blend_min, const float4 *rec, const uint2 *rect, const uint2 *ranges,
                    const uint32_t *list, float4 *img32, uchar4 *img8, hipStream_t st, unsigned long long *evals,
-                   uint32_t *queue) {
+                   uint32_t *queue, const Clears &clears, const WordCopy &copy) {
   if (total_rblocks == 0 || nviews == 0) return;
   if (blend_min == 0 || blend_min > kThreads) throw Error(Code::InvalidArgument, "blend threshold outside 1..256");
   const dim3 grid(total_rblocks), block(kThreads);
@@ -1609,36 +1623,36 @@ void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *
     const dim3 qgrid(std::min(queue_grid(), (total_rblocks + run - 1) / run * run));
     if (evals)
       k_raster_queue<true><<<qgrid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec,
-                                                      rect, ranges, list, img32, img8, evals, queue);
+                                                      rect, ranges, list, img32, img8, evals, queue, clears, copy);
     else
       k_raster_queue<false><<<qgrid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec,
-                                                       rect, ranges, list, img32, img8, evals, queue);
+                                                       rect, ranges, list, img32, img8, evals, queue, clears, copy);
     GS_HIP(hipGetLastError());
     return;
   }
   if (variant == RasterVariant::Rows2) {  // every view at bin shift >= 1 (the caller's duty)
     if (evals)
       k_raster_rows2<true><<<grid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
-                                                   list, img32, img8, evals);
+                                                   list, img32, img8, evals, clears, copy);
     else
       k_raster_rows2<false><<<grid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
-                                                    list, img32, img8, evals);
+                                                    list, img32, img8, evals, clears, copy);
     GS_HIP(hipGetLastError());
     return;
   }
   const bool unrolled = variant != RasterVariant::Linear;
   if (unrolled && evals)
     k_raster<true, 1><<<grid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges, list,
-                                              img32, img8, evals);
+                                              img32, img8, evals, clears, copy);
   else if (unrolled)
     k_raster<false, 1><<<grid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
-                                               list, img32, img8, evals);
+                                               list, img32, img8, evals, clears, copy);
   else if (evals)
     k_raster<true, 0><<<grid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges, list,
-                                              img32, img8, evals);
+                                              img32, img8, evals, clears, copy);
   else
     k_raster<false, 0><<<grid, block, lds, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
-                                               list, img32, img8, evals);
+                                               list, img32, img8, evals, clears, copy);
   GS_HIP(hipGetLastError());
 }
 

@@ -607,6 +607,14 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   uint32_t *pk = pair_keys_.get(), *pv = pair_vals_.get(), *pka = pair_keys_alt_.get(), *pva = pair_vals_alt_.get();
   pair_prep.scratch_clean = plan.pair_scratch.skip_clear;
   pair_prep.rts_min_tiles = tuning_.rts_min_tiles;
+  // One view whose pair sort is one look-back pass over emission-fed
+  // histograms: the pass also writes the binning tile ranges and the
+  // heaviest-first order (the histogram is each tile's list length), and the
+  // raster takes over the clears and the summary hand-over, so neither the
+  // ranges kernel nor the order kernel runs.
+  const bool tiles_from_sort = nviews == 1 && pair_prep.hist_ready && pair_ps.count == 1 && !count_evals_ &&
+                               plan.uniform_shift && ctiles <= (1u << pair_ps.digit_bits);
+  if (tiles_from_sort) pair_prep.tiles = TileRangesOut{ranges_.get(), ctiles, order_.get(), info + kOrderWord};
   onesweep_sort(&pk, &pv, &pka, &pva, pair_cap_, m_dev_.get(), tile_bits, /*descending=*/false,
                 pair_scratch_.get(), pair_scratch_.capacity(), st, -1, hint, pair_prep);
   Clears range_clears;  // the pair sort's scratch (and the counting words), for later
@@ -617,12 +625,13 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   // kernel hands them to the pinned summary itself.
   WordCopy summary_copy;
   if (!count_evals_) summary_copy = WordCopy{info, summary_pinned_, (uint32_t)(kVisibleWord + nviews)};
-  launch_tile_ranges(views_d, pk, m_dev_.get(), pair_cap_, ranges_.get(), range_clears, summary_copy, st);
+  if (!tiles_from_sort)
+    launch_tile_ranges(views_d, pk, m_dev_.get(), pair_cap_, ranges_.get(), range_clears, summary_copy, st);
   // Heaviest binning tiles first when every view bins alike (one layout).
   RasterOrder ro;
   uint32_t rblocks = total_rblocks_;
   if (plan.uniform_shift) {
-    launch_raster_order(ranges_.get(), ctiles, info + kOrderWord, order_.get(), st);
+    if (!tiles_from_sort) launch_raster_order(ranges_.get(), ctiles, info + kOrderWord, order_.get(), st);
     ro.order = order_.get();
     ro.counts = info + kOrderWord;
     ro.total_ctiles = ctiles;
@@ -643,7 +652,8 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
     variant = RasterVariant::Rows2;
   launch_raster(variant, views_d, rblock_start_d, nviews, rblocks, ro, tuning_.blend_min,
                 rec_.get(), rect_.get(),
-                ranges_.get(), pv, img32_.get(), img8_.get(), st, evals, info + kQueueWord);
+                ranges_.get(), pv, img32_.get(), img8_.get(), st, evals, info + kQueueWord,
+                tiles_from_sort ? range_clears : Clears(), tiles_from_sort ? summary_copy : WordCopy());
   if (stage_events) GS_HIP(hipEventRecord(ev_[4], st));
   for (size_t i = 0; i < host_rgba8_.size(); ++i) {
     if (!host_rgba8_[i] || host_views_[i].out8) continue;  // written by the raster itself

@@ -49,6 +49,20 @@ size_t radix_lookback_words(uint32_t n_cap, const RadixPasses &ps);
 // the scan ticket and its chunk status words, which carry flag bits).
 size_t radix_rts_words(uint32_t n_cap, const RadixPasses &ps);
 
+// Binning-tile output of a one-pass look-back sort whose keys are tile
+// indices below `count` (the pair sort of one view, digit histograms from
+// the emission): the workgroup that sorts the first tile also writes tile
+// t's range [begin, end) of the sorted keys into ranges[t] (clamped to the
+// key count) and the raster's heaviest-first order, order[class * count +
+// place] = t and order_counts[class] = class size, from the histogram, so
+// that no kernel has to find the range boundaries in the sorted keys.
+struct TileRangesOut {
+  uint2 *ranges = nullptr;
+  uint32_t count = 0;
+  uint32_t *order = nullptr;
+  uint32_t *order_counts = nullptr;
+};
+
 struct SortPrep {
   bool scratch_clean = false;  // the look-back words are zero: skip the clear
   bool hist_ready = false;     // the digit histograms are accumulated: skip k_os_hist
@@ -61,6 +75,8 @@ struct SortPrep {
   // out: the same result in one pass fewer. Look-back passes ignore them.
   const uint32_t *seg_start = nullptr;
   uint32_t segs = 0;
+  // With hist_ready, a sort of one look-back pass writes these (else unused).
+  TileRangesOut tiles;
 };
 
 // u32 words of scratch needed to sort up to n_cap keys on `bits` key bits

@@ -435,7 +435,7 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
                                                uint32_t shift, uint32_t flip, uint32_t from_value,
                                                const uint32_t *__restrict__ hist, uint32_t *__restrict__ counter,
                                                uint32_t *__restrict__ status_all, uint32_t *__restrict__ range,
-                                               const uint32_t *__restrict__ fine) {
+                                               const uint32_t *__restrict__ fine, TileRangesOut tro) {
   constexpr bool kBucket = BUCKET;  // only bucket passes carry the bucket plan's LDS
   static_assert(!BUCKET || DB == BUCKET_BITS, "a bucket pass has one digit per bucket");
   __shared__ uint16_t tab[kBucket ? kFineBins : 1];
@@ -464,6 +464,7 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
   clear_whist<DB>(whist);
   __syncthreads();
   uint32_t tile = s_tile;
+  const uint32_t first_tile = tile;
   size_t base = (size_t)tile * TILE;
   uint32_t count = base < n ? (uint32_t)min((size_t)TILE, (size_t)n - base) : 0u;
   uint32_t key[IPT], val[IPT], rank[IPT];
@@ -551,6 +552,21 @@ __global__ __launch_bounds__(T) void k_os_pass(const uint32_t *__restrict__ kin,
     base = (size_t)tile * TILE;
     count = base < n ? (uint32_t)min((size_t)TILE, (size_t)n - base) : 0u;
     if (base < n) load_tile(kin, vin, base, count, key, val);
+  }
+  if (tro.ranges && first_tile == 0) {  // uniform: the workgroup that sorted tile 0, off the look-back chain
+    __shared__ uint32_t olocal[kOrderClasses];
+    if (t < kOrderClasses) olocal[t] = 0;
+    __syncthreads();
+    uint32_t cls = 0, place = 0;
+    if (t < tro.count) {  // count <= 2^DB: digit t is tile t
+      const uint32_t b0 = min(g, n), b1 = min(g + hd, n);
+      tro.ranges[t] = make_uint2(b0, b1);
+      cls = raster_order_class(b1 - b0);
+      place = atomicAdd(&olocal[cls], 1u);
+    }
+    __syncthreads();
+    if (t < tro.count) tro.order[(size_t)cls * tro.count + place] = t;
+    if (t < kOrderClasses) tro.order_counts[t] = olocal[t];
   }
 }
 
@@ -1008,8 +1024,10 @@ void sort_passes(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t
   if (!prep.hist_ready) k_os_hist<DB><<<hist_blocks(n_cap), T, 0, st>>>(*keys, *vals, n_dev, n_cap, ps, hist);
   const uint32_t grid = std::min(grid_tiles, kPassGrid);
   for (uint32_t p = 0; p < ps.count; ++p) {
+    const bool tiles_out = ps.count == 1 && prep.hist_ready && prep.tiles.ranges;
     k_os_pass<DB><<<grid, T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, n_dev, n_cap, p, ps.shift[p], ps.flip[p],
-                                      ps.from_value[p], hist, counters + p, status, nullptr, nullptr);
+                                      ps.from_value[p], hist, counters + p, status, nullptr, nullptr,
+                                      tiles_out ? prep.tiles : TileRangesOut());
     std::swap(*keys, *keys_alt);
     std::swap(*vals, *vals_alt);
   }
@@ -1049,7 +1067,7 @@ void bucket_sort(uint32_t **keys, uint32_t **vals, uint32_t **keys_alt, uint32_t
   k_bucket_hist<<<tiles, T, 0, st>>>(*keys, n, key_range, parts, scratch + l.fine, range);
   k_os_pass<BUCKET_BITS, true><<<std::min(tiles, kPassGrid), T, 0, st>>>(*keys, *vals, *keys_alt, *vals_alt, nullptr, n,
                                                                    0, 0, 0, kRangeDigit, hist, counter, status,
-                                                                   range, scratch + l.fine);
+                                                                   range, scratch + l.fine, TileRangesOut());
   std::swap(*keys, *keys_alt);
   std::swap(*vals, *vals_alt);
   k_bucket_local<<<BUCKETS - 1, T, 0, st>>>(*keys, *vals, range, range + kBucketFirst, range + kBucketLast,
