@@ -114,9 +114,20 @@ struct Clears {
 // entries), workgroup b (slots [256 b, 256 b + 256)) also stores its smallest
 // non-zero key (~0u when none) and its largest key at key_range[b], for a
 // bucket sort of the keys (sort.h).
+// With `table` (one view), the view table arrives by value with the launch
+// instead of through a device copy: the projection works from it and its
+// first workgroup stores it at table->dst for the kernels after it.
+constexpr uint32_t kInlineTableWords = 64;
+struct InlineTable {
+  alignas(16) uint32_t words[kInlineTableWords];
+  uint32_t count = 0;      // words of the table
+  uint32_t view_word = 0;  // word offset of the view's ViewDev
+  uint32_t *dst = nullptr;
+};
 void launch_project(uint32_t format, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,
                     uint32_t total_slots, uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect,
-                    const Clears &clears, hipStream_t stream, uint2 *key_range = nullptr);
+                    const Clears &clears, hipStream_t stream, uint2 *key_range = nullptr,
+                    const InlineTable *table = nullptr);
 
 // Views of one scene projected together (views of a batch that share a
 // scene, at most kGroupViews per group). Group chunks are runs of 256

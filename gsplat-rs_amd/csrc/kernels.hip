@@ -479,16 +479,19 @@ __device__ __forceinline__ void store_key_range(uint32_t key, uint2 *__restrict_
   }
 }
 
-template <uint32_t FMT>
+template <uint32_t FMT, bool INLINE>
 __global__ __launch_bounds__(kThreads) void k_project(const ViewDev *__restrict__ views,
                                                       const uint32_t *__restrict__ seg_start, uint32_t nviews,
                                                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                       float4 *__restrict__ rec, uint2 *__restrict__ rect,
-                                                      Clears clears, uint2 *__restrict__ key_range) {
+                                                      Clears clears, uint2 *__restrict__ key_range,
+                                                      InlineTable table) {
   clear_ranges(clears);
+  if (INLINE && blockIdx.x == 0)
+    for (uint32_t i = threadIdx.x; i < table.count; i += kThreads) table.dst[i] = table.words[i];
   const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
-  const uint32_t v = find_view(seg_start, nviews, blockIdx.x * kThreads);
-  const ViewDev &vw = views[v];
+  const uint32_t v = INLINE ? 0u : find_view(seg_start, nviews, blockIdx.x * kThreads);
+  const ViewDev &vw = INLINE ? *reinterpret_cast<const ViewDev *>(&table.words[table.view_word]) : views[v];
   if (vw.format != FMT) return;  // uniform: one view per workgroup
   const uint32_t i = g - vw.seg_off;
   uint32_t key = 0;
@@ -1498,12 +1501,19 @@ void with_format(uint32_t fmt, F &&f) {
 
 void launch_project(uint32_t fmt, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,
                     uint32_t total_slots, uint32_t *keys, uint32_t *vals, float4 *rec, uint2 *rect, const Clears &clr,
-                    hipStream_t st, uint2 *key_range) {
+                    hipStream_t st, uint2 *key_range, const InlineTable *table) {
   if (total_slots == 0 || nviews == 0) return;
+  if (table && (nviews != 1 || table->count > kInlineTableWords || !table->dst ||
+                (table->view_word * 4 + sizeof(ViewDev) > table->count * 4) || table->view_word % 4))
+    throw Error(Code::Internal, "inline view table of one view expected");
   const dim3 grid(div_up(total_slots, kThreads)), block(kThreads);
   with_format(fmt, [&](auto F) {
-    k_project<decltype(F)::value><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect, clr,
-                                                          key_range);
+    if (table)
+      k_project<decltype(F)::value, true><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect,
+                                                                  clr, key_range, *table);
+    else
+      k_project<decltype(F)::value, false><<<grid, block, 0, st>>>(views, seg_start, nviews, keys, vals, rec, rect,
+                                                                   clr, key_range, InlineTable());
   });
   GS_HIP(hipGetLastError());
 }
