@@ -774,7 +774,7 @@ namespace {
 gs::Tuning tuning_from(const GsplatTuning &t, gs::Tuning base) {
   gs::Tuning out = base;
   out.bin_shift = t.bin_shift;
-  if (t.raster_variant > 3) throw gs::Error(gs::Code::InvalidArgument, "raster_variant must be 0, 1, 2 or 3");
+  if (t.raster_variant > 4) throw gs::Error(gs::Code::InvalidArgument, "raster_variant must be 0 to 4");
   out.raster = static_cast<gs::RasterVariant>(t.raster_variant);
   out.rts_min_tiles = t.sort_rts_min_tiles;
   out.blend_min = t.blend_min;

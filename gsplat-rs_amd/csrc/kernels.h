@@ -270,6 +270,9 @@ struct RasterOrder {
 //             resident at once) that takes the (view, tile) work items from
 //             queues in the same order and XCD placement; `queue`
 //             (kRasterQueueWords words) must be zero at launch
+//   Blocks8   8x8 raster tiles: each wave of a 16x16 workgroup blends one
+//             8x8 block over the splats whose pixel bounds meet it (the
+//             records staged once per workgroup, a position list per block)
 // All give bit-identical images. With `evals`, the counting variant also
 // adds to evals[0] the number of (pixel, splat) evaluations that pass
 // coverage before each pixel's early stop (the blend's algorithmic work) and
@@ -277,7 +280,7 @@ struct RasterOrder {
 // launch does for the kernels before it (see raster_prologue): words to zero
 // and the frame summary to hand to pinned memory; neither may overlap what
 // the raster reads.
-enum class RasterVariant : uint32_t { Linear = 0, Unrolled = 1, Rows2 = 2, Queue = 3 };
+enum class RasterVariant : uint32_t { Linear = 0, Unrolled = 1, Rows2 = 2, Queue = 3, Blocks8 = 4 };
 constexpr uint32_t kQueueLanes = 16;   // persistent raster: queues per XCD
 constexpr uint32_t kQueueStride = 16;  // words between two queue counters (their own cache lines)
 constexpr uint32_t kRasterQueueWords = kXcds * kQueueLanes * kQueueStride;

@@ -1132,6 +1132,170 @@ __device__ __forceinline__ void blend_stage(const float4 *s0, float4 *s1, const 
 // The work of raster workgroup b (a 16x16 tile, see raster_tile) by the
 // calling workgroup, with the LDS stage s0/s1/s2 and the per-wave counts
 // wkeep.
+// 8x8 raster tiles (raster variant Blocks8): one 8x8 pixel block per wave.
+// The workgroup stages the records of its 16x16 tile's kept splats once
+// (s0/s1/s2) and keeps, for each block, the list of staged positions whose
+// pixel bounds meet the block (lists after s2, cap words each); wave w blends
+// only block w's list with the wave-uniform loop. A splat left out of a
+// block's list covers none of its pixels (the bounds are conservative), so
+// every pixel blends the same splats in the same order as in the other
+// variants.
+template <bool COUNT>
+__device__ __forceinline__ void raster_workgroup_blocks(uint32_t b, const ViewDev *__restrict__ views,
+                                                        const uint32_t *__restrict__ rblock_start, uint32_t nviews,
+                                                        uint32_t total_rblocks, const RasterOrder &ro,
+                                                        uint32_t blend_min, const float4 *__restrict__ rec,
+                                                        const uint2 *__restrict__ rect,
+                                                        const uint2 *__restrict__ ranges,
+                                                        const uint32_t *__restrict__ list, float4 *__restrict__ img32,
+                                                        uchar4 *__restrict__ img8,
+                                                        unsigned long long *__restrict__ evals, float4 *s0, float4 *s1,
+                                                        float4 *s2, uint32_t *wkeep) {
+  __shared__ uint32_t bkeep[4][kWaves];
+  uint32_t v, tx, ty;
+  if (!raster_tile(views, rblock_start, nviews, total_rblocks, ro, b, v, tx, ty)) return;
+  const uint32_t cap = stage_capacity(blend_min);
+  uint32_t *lst = reinterpret_cast<uint32_t *>(s2 + cap);
+  const ViewDev &vw = views[v];
+  const uint32_t shift = vw.bin_shift;
+  const uint32_t width = vw.fp.width, height = vw.fp.height;
+  const uint32_t t = threadIdx.x, w = t / kWave, lane = __lane_id();
+  const uint32_t x = tx * kTile + (w & 1u) * 8u + (lane & 7u), y = ty * kTile + (w >> 1) * 8u + (lane >> 3);
+  const bool inside = x < width && y < height;
+  const float pnx = (2.0f * ((float)x + 0.5f)) / (float)width - 1.0f;
+  const float pny = 1.0f - (2.0f * ((float)y + 0.5f)) / (float)height;
+  const uint2 rg = ranges[vw.ctile_off + (ty >> shift) * vw.fp.ctiles_x + (tx >> shift)];
+  const uint32_t tx0 = tx * kTile, tx1 = tx0 + kTile - 1, ty0 = ty * kTile, ty1 = ty0 + kTile - 1;
+  const float exlo = (2.0f * (float)tx0) / (float)width - 1.0f, exhi = (2.0f * (float)(tx1 + 1)) / (float)width - 1.0f;
+  const float eyhi = 1.0f - (2.0f * (float)ty0) / (float)height, eylo = 1.0f - (2.0f * (float)(ty1 + 1)) / (float)height;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+  float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+  bool done = !inside;
+  uint32_t passed = 0, tested = 0;
+  uint32_t staged = 0;
+  uint32_t bn[4] = {0, 0, 0, 0};  // entries of each block's list
+  for (uint32_t start = rg.x;;) {
+    if (__syncthreads_count(!done) == 0) break;
+    const bool more = start < rg.y;
+    if (more) {
+      const uint32_t k = start + t;
+      uint32_t s = 0, mask = 0;
+      bool keep = false;
+      float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra;
+      if (k < rg.y) {
+        s = list[k];
+        const uint2 r = rect[s];
+        const uint32_t rx0 = r.x & 0xFFFFu, ry0 = r.x >> 16, rx1 = r.y & 0xFFFFu, ry1 = r.y >> 16;
+        keep = rx0 <= tx1 && rx1 >= tx0 && ry0 <= ty1 && ry1 >= ty0;
+        if (keep) {  // drop splats whose ellipse misses the tile
+          ra = rec[3 * (size_t)s + 0];
+          rb = rec[3 * (size_t)s + 1];
+          keep = ellipse_meets_rect(ra, rb, exlo, exhi, eylo, eyhi);
+        }
+        if (keep) {
+          const bool left = rx0 <= tx0 + 7u, right = rx1 >= tx0 + 8u;
+          const bool top = ry0 <= ty0 + 7u, bottom = ry1 >= ty0 + 8u;
+          mask = (left && top ? 1u : 0u) | (right && top ? 2u : 0u) | (left && bottom ? 4u : 0u) |
+                 (right && bottom ? 8u : 0u);
+        }
+      }
+      const uint64_t bal = __ballot(keep);
+      uint64_t bb[4];
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) bb[q] = __ballot((mask >> q) & 1u);
+      if (lane == 0) {
+        wkeep[w] = (uint32_t)__popcll(bal);
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) bkeep[q][w] = (uint32_t)__popcll(bb[q]);
+      }
+      __syncthreads();
+      uint32_t off = staged, kept = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < kWaves; ++q) {
+        off += q < w ? wkeep[q] : 0u;
+        kept += wkeep[q];
+      }
+      if (keep) {
+        const uint32_t pos = off + (uint32_t)__popcll(bal & lt);
+        s0[pos] = ra;
+        s1[pos] = rb;
+        s2[pos] = rec[3 * (size_t)s + 2];
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+          if ((mask >> q) & 1u) {
+            uint32_t bo = bn[q];
+#pragma unroll
+            for (uint32_t p = 0; p < kWaves; ++p) bo += p < w ? bkeep[q][p] : 0u;
+            lst[q * cap + bo + (uint32_t)__popcll(bb[q] & lt)] = pos;
+          }
+        }
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q)
+#pragma unroll
+        for (uint32_t p = 0; p < kWaves; ++p) bn[q] += bkeep[q][p];
+      staged += kept;
+      start += kThreads;
+      __syncthreads();
+    }
+    if (staged >= blend_min || (!more || start >= rg.y)) {
+      const uint32_t n = bn[w];
+      if (n > 0) {
+        constexpr uint32_t U = 4;
+        const uint32_t *bl = lst + w * cap;
+        bool active = !done;
+        for (uint32_t j0 = 0; j0 < n; j0 += U) {
+          if (__ballot(active) == 0) break;  // wave-uniform
+          float4 ga[U], gb[U], gc[U];
+#pragma unroll
+          for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t pos = bl[min(j0 + u, n - 1)];
+            ga[u] = s0[pos];
+            gb[u] = s1[pos];
+            gc[u] = s2[pos];
+          }
+#pragma unroll
+          for (uint32_t u = 0; u < U; ++u) {
+            const bool valid = j0 + u < n;
+            const float4 a = ga[u];
+            const float4 bq = gb[u];
+            const float dx = pnx - a.x;
+            const float dy = pny - a.y;
+            const float lx = fmaf(a.w, dy, a.z * dx);
+            const float ly = fmaf(bq.y, dy, bq.x * dx);
+            const float r2 = fmaf(ly, ly, lx * lx);
+            const bool hit = active && valid && r2 <= bq.z;
+            if (COUNT) {
+              tested += (active && valid) ? 1u : 0u;
+              passed += hit ? 1u : 0u;
+            }
+            if (hit) {
+              blend_hit(r2, bq.w, gc[u], T, cr, cg, cb);
+              active = !(T < 1e-4f);
+            }
+          }
+        }
+        done = !active;
+      }
+      staged = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) bn[q] = 0;
+      if (start >= rg.y) break;
+    }
+  }
+  if (inside) write_pixel(vw, x, y, cr, cg, cb, T, img32, img8);
+  if (COUNT) {
+    uint32_t wp = passed, u = tested;
+#pragma unroll
+    for (uint32_t off = kWave / 2; off > 0; off >>= 1) {
+      wp += __shfl_down(wp, off, kWave);
+      u += __shfl_down(u, off, kWave);
+    }
+    if (lane == 0 && wp) atomicAdd(evals, (unsigned long long)wp);
+    if (lane == 0 && u) atomicAdd(evals + 1, (unsigned long long)u);
+  }
+}
+
 template <bool COUNT, int LOOP>
 __device__ __forceinline__ void raster_workgroup(uint32_t b, const ViewDev *__restrict__ views,
                                                  const uint32_t *__restrict__ rblock_start, uint32_t nviews,
@@ -1141,6 +1305,11 @@ __device__ __forceinline__ void raster_workgroup(uint32_t b, const ViewDev *__re
                                                  float4 *__restrict__ img32, uchar4 *__restrict__ img8,
                                                  unsigned long long *__restrict__ evals, float4 *s0, float4 *s1,
                                                  float4 *s2, uint32_t *wkeep) {
+  if constexpr (LOOP == 2) {
+    raster_workgroup_blocks<COUNT>(b, views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect, ranges,
+                                   list, img32, img8, evals, s0, s1, s2, wkeep);
+    return;
+  }
   uint32_t v, tx, ty;
   if (!raster_tile(views, rblock_start, nviews, total_rblocks, ro, b, v, tx, ty)) return;
   const ViewDev &vw = views[v];
@@ -1627,6 +1796,17 @@ void launch_raster(RasterVariant variant, const ViewDev *views, const uint32_t *
   if (blend_min == 0 || blend_min > kThreads) throw Error(Code::InvalidArgument, "blend threshold outside 1..256");
   const dim3 grid(total_rblocks), block(kThreads);
   const size_t lds = 3 * (size_t)stage_capacity(blend_min) * sizeof(float4);
+  if (variant == RasterVariant::Blocks8) {  // the stage plus four position lists of stage_capacity words
+    const size_t lds8 = lds + (size_t)stage_capacity(blend_min) * sizeof(float4);
+    if (evals)
+      k_raster<true, 2><<<grid, block, lds8, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec, rect,
+                                                  ranges, list, img32, img8, evals, clears, copy);
+    else
+      k_raster<false, 2><<<grid, block, lds8, st>>>(views, rblock_start, nviews, total_rblocks, ro, blend_min, rec,
+                                                   rect, ranges, list, img32, img8, evals, clears, copy);
+    GS_HIP(hipGetLastError());
+    return;
+  }
   if (variant == RasterVariant::Queue) {
     if (!queue) throw Error(Code::Internal, "persistent raster without its queue words");
     constexpr uint32_t run = kXcds * kQueueLanes;  // every queue keeps at least one workgroup

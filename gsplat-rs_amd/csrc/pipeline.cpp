@@ -768,7 +768,7 @@ void Pipeline::set_bin_shift(uint32_t shift) {
 void Pipeline::set_tuning(const Tuning &t) {
   if (t.bin_shift > kMaxBinShift) throw Error(Code::InvalidArgument, "bin shift above " + std::to_string(kMaxBinShift));
   if (t.raster != RasterVariant::Linear && t.raster != RasterVariant::Unrolled && t.raster != RasterVariant::Rows2 &&
-      t.raster != RasterVariant::Queue)
+      t.raster != RasterVariant::Queue && t.raster != RasterVariant::Blocks8)
     throw Error(Code::InvalidArgument, "unknown raster variant");
   if (t.rts_min_tiles == 0) throw Error(Code::InvalidArgument, "sort mode threshold must be positive");
   if (t.blend_min == 0 || t.blend_min > 256) throw Error(Code::InvalidArgument, "blend threshold outside 1..256");
@@ -783,7 +783,8 @@ std::vector<TuneSample> Pipeline::autotune(const std::vector<ViewSpec> &views, u
   TuneSample best{base, 1e30f};
   for (uint32_t shift = 0; shift <= kMaxBinShift; ++shift)
     for (uint32_t rts : {128u, 256u, 1024u})
-      for (RasterVariant rv : {RasterVariant::Unrolled, RasterVariant::Rows2, RasterVariant::Queue}) {
+      for (RasterVariant rv :
+           {RasterVariant::Unrolled, RasterVariant::Rows2, RasterVariant::Queue, RasterVariant::Blocks8}) {
       if (rv == RasterVariant::Rows2 && shift == 0) continue;  // runs as Unrolled there
       Tuning t = base;
       t.bin_shift = shift;

@@ -18,6 +18,7 @@ RasterVariant raster_variant_from_env() {
     return k == 0   ? RasterVariant::Linear
            : k == 2 ? RasterVariant::Rows2
            : k == 3 ? RasterVariant::Queue
+           : k == 4 ? RasterVariant::Blocks8
                     : RasterVariant::Unrolled;
   }
   return RasterVariant::Unrolled;

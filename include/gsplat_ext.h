@@ -343,7 +343,9 @@ int32_t gsplat_batch_set_stage_timing(GsplatBatch *batch, uint32_t on);
  *                       2 wave-uniform loop with two pixels per thread
  *                       (its views bin into tiles of at least 32 pixels),
  *                       3 wave-uniform loop in a persistent grid taking
- *                       the (view, tile) work items from per-XCD queues
+ *                       the (view, tile) work items from per-XCD queues,
+ *                       4 8x8 raster tiles: each wave blends one 8x8 block
+ *                       of a 16x16 workgroup over the splats that meet it
  *   sort_rts_min_tiles  radix sorts of at least this many 4096-key tiles use
  *                       reduce-then-scan passes, shorter ones look-back passes
  *   frame_ms            (autotune output) median device time of a frame
@@ -374,7 +376,7 @@ typedef struct GsplatTuning {
 int32_t gsplat_batch_get_tuning(GsplatBatch *batch, GsplatTuning *out_tuning);
 int32_t gsplat_batch_set_tuning(GsplatBatch *batch, const GsplatTuning *tuning);
 /* Renders `views` `reps` times under each candidate (bin_shift 0..3 x
- * sort_rts_min_tiles {128, 256, 1024} x raster_variant {1, 2, 3}, variant 2
+ * sort_rts_min_tiles {128, 256, 1024} x raster_variant {1, 2, 3, 4}, variant 2
  * from bin_shift 1), keeps the fastest and returns it;
  * the batch's last frame is then those views under the kept tuning. */
 int32_t gsplat_batch_autotune(GsplatBatch *batch, const GsplatViewDesc *views, uint32_t count, uint32_t reps,

@@ -236,7 +236,7 @@ def test_gpu_auto_camera_frames_scene(gpu, oracle):
 
 
 @pytest.mark.parametrize("variant,shift", [(0, 0), (1, 0), (0, 2), (1, 2), (1, 3), (2, 1), (2, 2), (2, 3), (3, 0),
-                                           (3, 2)])
+                                           (3, 2), (4, 0), (4, 2)])
 def test_gpu_raster_variants_match_oracle(gpu, oracle, monkeypatch, variant, shift):
     # Both blend loops, with splats binned straight into 16x16 tiles (shift
     # 0) or into larger binning tiles culled per raster tile, blend the same
@@ -325,7 +325,7 @@ def test_gpu_raster_variants_bit_identical(gpu, monkeypatch):
     scene, view = synth.plain_1080p(n=80_000, seed_offset=5)
     cam = gpu.look_at(view.position, view.target)
     out = {}
-    for variant in (0, 1, 2, 3):
+    for variant in (0, 1, 2, 3, 4):
         for shift in (0, 1, 2, 3):
             monkeypatch.setenv("GSPLAT_RASTER_VARIANT", str(variant))
             monkeypatch.setenv("GSPLAT_BIN_SHIFT", str(shift))
@@ -336,7 +336,7 @@ def test_gpu_raster_variants_bit_identical(gpu, monkeypatch):
             ctx.close()
     # the stage size at which a blend starts changes no pixel either
     monkeypatch.setenv("GSPLAT_BIN_SHIFT", "2")
-    for variant in (1, 2, 3):
+    for variant in (1, 2, 3, 4):
         for blend_min in (1, 64):
             monkeypatch.setenv("GSPLAT_RASTER_VARIANT", str(variant))
             monkeypatch.setenv("GSPLAT_BLEND_MIN", str(blend_min))
