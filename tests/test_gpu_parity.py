@@ -187,6 +187,31 @@ def test_gpu_parity_plain_1080p_300k_sh3(gpu, oracle):
     full_check(gpu, oracle, s, gpu.look_at(v.position, v.target), v.width, v.height, "plain-1080p-300k")
 
 
+def test_gpu_one_view_pair_overflow_renders_again(gpu, oracle, monkeypatch):
+    # One 1080p view in 64 px binning tiles (510 of them: one 9-bit pair pass,
+    # kept in look-back mode so that it also writes the tile ranges) whose
+    # 60,000 large splats make more binning pairs than the initial capacity
+    # of 2^20: the first frame overflows and is rendered again with grown
+    # buffers. It must equal the next frame (buffers large enough from the
+    # start) bit for bit, and its tile counts the oracle's.
+    monkeypatch.setenv("GSPLAT_BIN_SHIFT", "2")
+    monkeypatch.setenv("GSPLAT_SORT_RTS_MIN_TILES", "100000")
+    scene = synth.tiny_scene(60000, 5252, sh_degree=0, spread=0.4, log_scale=-2.2)
+    cam = gpu.look_at((0.0, 0.0, -2.5))
+    say("[one-view pair overflow]")
+    ctx = render(gpu, scene, cam, 1920, 1080)
+    st = ctx.stage_times()
+    assert st.bin_shift == 2 and st.bin_pairs > (1 << 20)
+    first = (ctx.rgba32f().view(np.uint32), ctx.rgba8(), ctx.tile_counts())
+    ctx.render()
+    second = (ctx.rgba32f().view(np.uint32), ctx.rgba8(), ctx.tile_counts())
+    for name, a, b in zip(("fp32", "rgba8", "tile counts"), first, second):
+        assert np.array_equal(a, b), f"overflowed frame differs from the next one: {name}"
+    proj_ref = oracle.project(oracle_scene(oracle, scene), cam_dict(cam), 1920, 1080)
+    assert np.array_equal(first[2], oracle.tile_counts(proj_ref, 1920, 1080))
+    ctx.close()
+
+
 def test_gpu_empty_scene_and_all_culled(gpu):
     empty = synth.Scene(np.zeros((0, 3), np.float32), np.zeros(0, np.float32), np.zeros((0, 3), np.float32),
                         np.zeros((0, 4), np.float32), np.zeros((0, 3), np.float32), 0, None)
