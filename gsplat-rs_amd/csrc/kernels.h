@@ -202,11 +202,25 @@ struct PairHist {
 // scanned offsets: each view's pairs form one contiguous run, front to back.
 // Pairs at or beyond `capacity` are dropped (the frame is then flagged and
 // rendered again). With hist.hist, also counts the written pairs' digits.
+// With totals.blocks (the count blocks, at most kEmitScanMaxBlocks),
+// block_sums holds the blocks' raw totals (launch_block_scan has not run):
+// each workgroup sums the totals before its own block itself, and
+// workgroup 0 writes what the block scan would have (info[1..3], *m_out,
+// *raster_pairs), one launch fewer for frames of few slots.
+constexpr uint32_t kEmitScanMaxBlocks = 1024;
+struct EmitTotals {
+  uint32_t blocks = 0;
+  const uint32_t *block_pairs = nullptr;
+  uint32_t *m_out = nullptr;
+  uint32_t *info = nullptr;
+  unsigned long long *raster_pairs = nullptr;
+};
 void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t nviews, uint32_t total_slots,
                       const uint32_t *svals, const uint32_t *rect_lo, const uint32_t *rect_hi,
                       const uint32_t *counts_rev,
                       const uint32_t *offsets_rev, const uint32_t *block_sums, uint32_t capacity,
-                      uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t stream, const PairHist &hist = PairHist());
+                      uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t stream, const PairHist &hist = PairHist(),
+                      const EmitTotals &totals = EmitTotals());
 
 // Words copied by one workgroup of a kernel, dst[i] = src[i] for i < words
 // (dst may be pinned host memory: the frame's summary, read by the host

@@ -754,19 +754,59 @@ __global__ __launch_bounds__(kThreads) void k_tile_emit(const ViewDev *__restric
                                                         const uint32_t *__restrict__ offsets_rev,
                                                         const uint32_t *__restrict__ block_sums, uint32_t capacity,
                                                         uint32_t *__restrict__ pair_keys,
-                                                        uint32_t *__restrict__ pair_vals, PairHist ph) {
+                                                        uint32_t *__restrict__ pair_vals, PairHist ph,
+                                                        EmitTotals et) {
   __shared__ uint32_t s_hist[kPairHistPasses][1u << 9];
   __shared__ uint32_t s_first[kThreads + 1];
   __shared__ uint32_t s_slot[kThreads];
   __shared__ uint32_t s_x0[kThreads], s_y0[kThreads], s_w[kThreads];
+  __shared__ uint32_t s_red[kWaves];
+  __shared__ unsigned long long s_red64[2][kWaves];
   const uint32_t g0 = blockIdx.x * kThreads;
   const uint32_t v = find_view(seg_start, nviews, g0);
   const ViewDev &vw = views[v];
-  const uint32_t t = threadIdx.x;
-  // Staged entry t is sorted slot g0 + 255 - t (nearest first).
+  const uint32_t t = threadIdx.x, w = t / kWave, lane = __lane_id();
+  // Staged entry t is sorted slot g0 + 255 - t (nearest first); the
+  // workgroup's 256 entries lie in one count block.
   const uint32_t r = total - 1 - (g0 + kThreads - 1 - t);
+  const uint32_t rb = (total - g0 - kThreads) / kCountBlock;
+  uint32_t block_first;
+  if (et.blocks) {  // raw block totals: sum those before this block
+    uint32_t part = 0;
+    for (uint32_t i = t; i < rb; i += kThreads) part += block_sums[i];
+    (void)block_exclusive_scan(part, s_red, &block_first);
+    if (blockIdx.x == 0) {  // what the block scan writes
+      unsigned long long m = 0, pairs = 0;
+      for (uint32_t i = t; i < et.blocks; i += kThreads) {
+        m += block_sums[i];
+        pairs += et.block_pairs[i];
+      }
+      m = wave_sum_u64(m);
+      pairs = wave_sum_u64(pairs);
+      if (lane == 0) {
+        s_red64[0][w] = m;
+        s_red64[1][w] = pairs;
+      }
+      __syncthreads();
+      if (t == 0) {
+        m = pairs = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kWaves; ++q) {
+          m += s_red64[0][q];
+          pairs += s_red64[1][q];
+        }
+        *et.raster_pairs = pairs;
+        *et.m_out = m > capacity ? capacity : (uint32_t)m;
+        et.info[1] = (uint32_t)m;
+        et.info[2] = (uint32_t)(m >> 32);
+        et.info[3] = m > capacity ? 1u : 0u;
+      }
+    }
+  } else {
+    block_first = block_sums[rb];
+  }
   const uint32_t cnt = counts_rev[r];
-  const uint32_t first = block_sums[r / kCountBlock] + offsets_rev[r];
+  const uint32_t first = block_first + offsets_rev[r];
   const uint32_t g = g0 + kThreads - 1 - t;
   const uint32_t slot = vw.seg_off + (svals[g] - (v << kLocalBits));  // loaded beside cnt
   const uint2 b = make_uint2(rect_lo[g], rect_hi[g]);                 // in slot order
@@ -1733,14 +1773,18 @@ void launch_tile_emit(const ViewDev *views, const uint32_t *seg_start, uint32_t 
                       const uint32_t *svals, const uint32_t *rect_lo, const uint32_t *rect_hi,
                       const uint32_t *counts_rev,
                       const uint32_t *offsets_rev, const uint32_t *block_sums, uint32_t capacity,
-                      uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t st, const PairHist &ph) {
+                      uint32_t *pair_keys, uint32_t *pair_vals, hipStream_t st, const PairHist &ph,
+                      const EmitTotals &et) {
   if (total_slots == 0) return;
   if (ph.hist && (ph.passes > kPairHistPasses || ph.digit_bits > 9))
     throw Error(Code::Internal, "pair histogram of more than two passes or 9-bit digits");
+  if (et.blocks && (et.blocks != total_slots / kCountBlock || et.blocks > kEmitScanMaxBlocks || !et.block_pairs ||
+                    !et.m_out || !et.info || !et.raster_pairs))
+    throw Error(Code::Internal, "emission block totals do not match the slots");
   k_tile_emit<<<total_slots / kThreads, kThreads, 0, st>>>(views, seg_start, nviews, total_slots, svals,
                                                              rect_lo, rect_hi, counts_rev, offsets_rev, block_sums,
                                                              capacity,
-                                                             pair_keys, pair_vals, ph);
+                                                             pair_keys, pair_vals, ph, et);
   GS_HIP(hipGetLastError());
 }
 

@@ -605,8 +605,14 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
   // carry the rects in sorted order from the counts to the emission.
   launch_tile_counts(views_d, seg_start_d, nviews, slots, k, v, rect_.get(), counts_.get(),
                      offsets_.get(), visible, block_sums_.get(), block_pairs_.get(), ka, va, scan_clears, st);
-  launch_block_scan(block_sums_.get(), count_blocks, pair_cap_, m_dev_.get(), info, block_pairs_.get(), raster_pairs,
-                    st);
+  // Few count blocks: the emission sums the block totals it needs itself
+  // (one launch fewer); many: the block scan launch.
+  EmitTotals emit_totals;
+  if (count_blocks >= 1 && count_blocks <= kEmitScanMaxBlocks)
+    emit_totals = EmitTotals{count_blocks, block_pairs_.get(), m_dev_.get(), info, raster_pairs};
+  else
+    launch_block_scan(block_sums_.get(), count_blocks, pair_cap_, m_dev_.get(), info, block_pairs_.get(),
+                      raster_pairs, st);
   // A look-back pair sort of at most two passes takes its digit histograms
   // from the emission (its scratch is clean by then: any fill runs first).
   SortPrep pair_prep;
@@ -621,7 +627,7 @@ void Pipeline::record_frame(const FramePlan &plan, bool captured) {
     pair_prep.hist_ready = true;
   }
   launch_tile_emit(views_d, seg_start_d, nviews, slots, v, ka, va, counts_.get(), offsets_.get(),
-                   block_sums_.get(), pair_cap_, pair_keys_.get(), pair_vals_.get(), st, pair_hist);
+                   block_sums_.get(), pair_cap_, pair_keys_.get(), pair_vals_.get(), st, pair_hist, emit_totals);
   uint32_t *pk = pair_keys_.get(), *pv = pair_vals_.get(), *pka = pair_keys_alt_.get(), *pva = pair_vals_alt_.get();
   pair_prep.scratch_clean = plan.pair_scratch.skip_clear;
   pair_prep.rts_min_tiles = tuning_.rts_min_tiles;
