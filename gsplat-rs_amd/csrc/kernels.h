@@ -118,10 +118,11 @@ struct Clears {
 // instead of through a device copy: the projection works from it and its
 // first workgroup stores it at table->dst for the kernels after it.
 constexpr uint32_t kInlineTableWords = 64;
+constexpr uint32_t kInlineViewWord = 8;  // word offset of the view's ViewDev in a one-view table
 struct InlineTable {
   alignas(16) uint32_t words[kInlineTableWords];
   uint32_t count = 0;      // words of the table
-  uint32_t view_word = 0;  // word offset of the view's ViewDev
+  uint32_t view_word = 0;  // word offset of the view's ViewDev (kInlineViewWord)
   uint32_t *dst = nullptr;
 };
 void launch_project(uint32_t format, const ViewDev *views, const uint32_t *seg_start, uint32_t nviews,

@@ -491,7 +491,7 @@ __global__ __launch_bounds__(kThreads) void k_project(const ViewDev *__restrict_
     for (uint32_t i = threadIdx.x; i < table.count; i += kThreads) table.dst[i] = table.words[i];
   const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
   const uint32_t v = INLINE ? 0u : find_view(seg_start, nviews, blockIdx.x * kThreads);
-  const ViewDev &vw = INLINE ? *reinterpret_cast<const ViewDev *>(&table.words[table.view_word]) : views[v];
+  const ViewDev &vw = INLINE ? *reinterpret_cast<const ViewDev *>(&table.words[kInlineViewWord]) : views[v];
   if (vw.format != FMT) return;  // uniform: one view per workgroup
   const uint32_t i = g - vw.seg_off;
   uint32_t key = 0;
@@ -1713,7 +1713,7 @@ void launch_project(uint32_t fmt, const ViewDev *views, const uint32_t *seg_star
                     hipStream_t st, uint2 *key_range, const InlineTable *table) {
   if (total_slots == 0 || nviews == 0) return;
   if (table && (nviews != 1 || table->count > kInlineTableWords || !table->dst ||
-                (table->view_word * 4 + sizeof(ViewDev) > table->count * 4) || table->view_word % 4))
+                table->view_word != kInlineViewWord || kInlineViewWord * 4 + sizeof(ViewDev) > table->count * 4))
     throw Error(Code::Internal, "inline view table of one view expected");
   const dim3 grid(div_up(total_slots, kThreads)), block(kThreads);
   with_format(fmt, [&](auto F) {
