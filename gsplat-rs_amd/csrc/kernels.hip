@@ -958,7 +958,13 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
 // summary to pinned host memory (copy, by workgroup 0). Neither touches
 // what the raster reads.
 __device__ __forceinline__ void raster_prologue(const Clears &clears, const WordCopy &copy) {
-  clear_ranges(clears);
+  // 64 words per workgroup, taken by the workgroups dispatched last (the
+  // lightest tiles), so that the heaviest tiles start without the stores
+  const uint32_t rb = gridDim.x - 1 - blockIdx.x;
+  if (threadIdx.x < 64)
+    for (int r = 0; r < Clears::kRanges; ++r)
+      for (size_t i = (size_t)rb * 64 + threadIdx.x; i < clears.words[r]; i += (size_t)gridDim.x * 64)
+        clears.ptr[r][i] = 0u;
   if (blockIdx.x == 0)
     for (uint32_t i = threadIdx.x; i < copy.words; i += kThreads) copy.dst[i] = copy.src[i];
 }
