@@ -787,6 +787,21 @@ std::vector<TuneSample> Pipeline::autotune(const std::vector<ViewSpec> &views, u
   std::vector<TuneSample> out;
   const Tuning base = tuning_;
   TuneSample best{base, 1e30f};
+  auto measure = [&](const Tuning &t) {
+    set_tuning(t);
+    enqueue(views);  // warm: buffers grow, the pair count settles
+    wait();
+    std::vector<float> ms;
+    for (uint32_t r = 0; r < reps; ++r) {
+      enqueue(views);
+      wait();
+      ms.push_back(times_.total_ms);
+    }
+    std::sort(ms.begin(), ms.end());
+    const TuneSample sample{t, ms[ms.size() / 2]};
+    out.push_back(sample);
+    if (sample.frame_ms < best.frame_ms) best = sample;
+  };
   for (uint32_t shift = 0; shift <= kMaxBinShift; ++shift)
     for (uint32_t rts : {128u, 256u, 1024u})
       for (RasterVariant rv :
@@ -797,20 +812,16 @@ std::vector<TuneSample> Pipeline::autotune(const std::vector<ViewSpec> &views, u
       t.rts_min_tiles = rts;
       t.raster = rv;
       t.rows2_min_tiles = 0;  // the variant is chosen here
-      set_tuning(t);
-      enqueue(views);  // warm: buffers grow, the pair count settles
-      wait();
-      std::vector<float> ms;
-      for (uint32_t r = 0; r < reps; ++r) {
-        enqueue(views);
-        wait();
-        ms.push_back(times_.total_ms);
-      }
-      std::sort(ms.begin(), ms.end());
-      const TuneSample sample{t, ms[ms.size() / 2]};
-      out.push_back(sample);
-      if (sample.frame_ms < best.frame_ms) best = sample;
+      measure(t);
     }
+  // then the LDS stage depth (blend threshold) around the best of those
+  const Tuning grid_best = best.tuning;
+  for (uint32_t blend_min : {32u, 128u, 256u}) {
+    if (blend_min == grid_best.blend_min) continue;
+    Tuning t = grid_best;
+    t.blend_min = blend_min;
+    measure(t);
+  }
   set_tuning(best.tuning);
   enqueue(views);
   wait();

@@ -377,7 +377,8 @@ int32_t gsplat_batch_get_tuning(GsplatBatch *batch, GsplatTuning *out_tuning);
 int32_t gsplat_batch_set_tuning(GsplatBatch *batch, const GsplatTuning *tuning);
 /* Renders `views` `reps` times under each candidate (bin_shift 0..3 x
  * sort_rts_min_tiles {128, 256, 1024} x raster_variant {1, 2, 3, 4}, variant 2
- * from bin_shift 1), keeps the fastest and returns it;
+ * from bin_shift 1, then blend_min {32, 128, 256} around the fastest of
+ * those), keeps the fastest and returns it;
  * the batch's last frame is then those views under the kept tuning. */
 int32_t gsplat_batch_autotune(GsplatBatch *batch, const GsplatViewDesc *views, uint32_t count, uint32_t reps,
                               GsplatTuning *out_best);

@@ -202,9 +202,10 @@ def test_autotune_keeps_images(abi):
     say("[autotune batch]")
     best = b.autotune(vs, reps=2)
     assert best.bin_shift <= 3 and best.sort_rts_min_tiles in (128, 256, 1024) and best.frame_ms > 0
+    assert best.raster_variant in (1, 2, 3, 4) and best.blend_min in (32, 64, 128, 256)
     t = b.tuning()
-    assert (t.bin_shift, t.sort_rts_min_tiles, t.raster_variant) == (best.bin_shift, best.sort_rts_min_tiles,
-                                                                     best.raster_variant)
+    assert (t.bin_shift, t.sort_rts_min_tiles, t.raster_variant, t.blend_min) == (
+        best.bin_shift, best.sort_rts_min_tiles, best.raster_variant, best.blend_min)
     for v in range(len(vs)):
         assert_same(batch_view(b, v, scenes[views[v][0]].count), before[v], f"view {v} after autotune")
     # every setting renders the same frame; rows2_min_tiles 1 with one
