@@ -187,6 +187,19 @@ def test_gpu_parity_plain_1080p_300k_sh3(gpu, oracle):
     full_check(gpu, oracle, s, gpu.look_at(v.position, v.target), v.width, v.height, "plain-1080p-300k")
 
 
+@pytest.mark.parametrize("w,h", [(2048, 1024), (2112, 1024), (1024, 1024), (1088, 1024)])
+def test_gpu_binning_tile_count_boundaries(gpu, oracle, monkeypatch, w, h):
+    # 64 px binning tiles: 256 (1024x1024, 8-bit digits), 272 (1088x1024,
+    # 9-bit) and 512 (2048x1024, 9-bit) binning tiles sort their pairs in one
+    # pass, whose histogram also gives the tile ranges and raster order; 528
+    # (2112x1024) take two passes and the ranges kernel. Every case must
+    # match the oracle.
+    monkeypatch.setenv("GSPLAT_BIN_SHIFT", "2")
+    monkeypatch.setenv("GSPLAT_BIN_AUTO", "0")
+    s = synth.tiny_scene(12000, 4242, sh_degree=1, spread=0.7, log_scale=-3.0)
+    full_check(gpu, oracle, s, gpu.look_at((0.3, -0.2, -2.4), (0.0, 0.0, 0.0)), w, h, f"bin-tiles-{w}x{h}")
+
+
 def test_gpu_one_view_pair_overflow_renders_again(gpu, oracle, monkeypatch):
     # One 1080p view in 64 px binning tiles (510 of them: one 9-bit pair pass,
     # kept in look-back mode so that it also writes the tile ranges) whose
