@@ -143,3 +143,20 @@ def test_bucket_sort_single_key_value_and_culled(gpu, oracle, monkeypatch):
     ctx = frame(gpu, scene, cam, 200, 150)
     assert ctx.stats().visible_count == 0 and not ctx.rgba8().any()
     ctx.close()
+
+
+@pytest.mark.parametrize("n,mode", [(1 << 19, SORT_BUCKET), ((1 << 19) + 1, SORT_LOOKBACK)])
+def test_bucket_sort_size_limit(gpu, oracle, n, mode):
+    # 2^19 slots is the largest one-view frame the bucket sort takes (buckets
+    # of about 2065 keys); one splat more pads the view to 2^19 + 4096 slots
+    # and the radix passes sort it. Both orders are the oracle's.
+    rng = np.random.default_rng(9191)
+    scene = synth.gaussian_scene(rng, n, 0, synth.mixture_means(rng, n))
+    cam = gpu.look_at((0.0, 0.0, -4.0), (0.0, 0.0, 0.0))
+    say(f"[bucket size limit] {n} splats")
+    ctx = frame(gpu, scene, cam, 320, 240)
+    assert ctx.stage_times().depth_sort == mode
+    osc = oracle_scene(oracle, scene)
+    keys_ref, idx_ref = oracle.preprocess(osc, cam_dict(cam))
+    assert np.array_equal(ctx.sorted_indices(scene.count), oracle.sort_values_by_keys(keys_ref, idx_ref))
+    ctx.close()
