@@ -213,7 +213,8 @@ def test_autotune_keeps_images(abi):
     for shift, variant, blend_min, bin_auto, rows2 in ((0, 0, 64, 1, 0), (3, 0, 256, 1, 0), (2, 2, 1, 1, 0),
                                                        (1, 2, 64, 1, 0), (2, 1, 200, 1, 0), (2, 1, 64, 0, 1),
                                                        (1, 1, 64, 0, 1), (0, 1, 64, 0, 1), (2, 3, 64, 1, 0),
-                                                       (0, 3, 1, 0, 0)):
+                                                       (0, 3, 1, 0, 0), (2, 4, 64, 1, 0), (0, 4, 1, 0, 0),
+                                                       (3, 4, 256, 1, 0)):
         t.bin_shift, t.raster_variant, t.blend_min, t.bin_auto, t.rows2_min_tiles = shift, variant, blend_min, \
             bin_auto, rows2
         b.set_tuning(t)
