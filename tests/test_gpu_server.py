@@ -70,6 +70,31 @@ def test_server_results_equal_alone(abi, pipelines):
     srv.close()
 
 
+def test_server_8x8_tiles_equal_default_alone(abi, monkeypatch):
+    # Server pipelines rasterising in 8x8 tiles (raster_variant 4, read at
+    # server creation) writing pinned and plain targets: every image equals
+    # the view rendered alone with the default 16x16 raster.
+    scenes, reqs = workload(abi, 24, 4)
+    dev = [abi.Scene(s) for s in scenes]
+    monkeypatch.setenv("GSPLAT_RASTER_VARIANT", "4")
+    srv = abi.Server(max_batch_views=16, pipelines=2)
+    monkeypatch.delenv("GSPLAT_RASTER_VARIANT")
+    ids = [srv.add_scene(d) for d in dev]
+    outs, tickets = [], []
+    say(f"[server 8x8 tiles] {len(reqs)} requests")
+    for i, (s, cam, w, h) in enumerate(reqs):
+        out = abi.PinnedImage(w, h) if i % 2 == 0 else np.zeros((h, w, 4), np.uint8)
+        outs.append(out)
+        tickets.append(srv.submit(ids[s], cam, w, h, out))
+    results = [srv.wait(t) for t in tickets]
+    for i, ((s, cam, w, h), r, out) in enumerate(zip(reqs, results, outs)):
+        img, vis = alone(abi, dev[s], cam, w, h)
+        got = out.array if isinstance(out, abi.PinnedImage) else out
+        assert r.status == 0 and r.visible_count == vis, f"request {i}"
+        assert np.array_equal(got, img), f"request {i}: 8x8-tile image differs from the default raster alone"
+    srv.close()
+
+
 def test_server_concurrent_submitters(abi):
     scenes, reqs = workload(abi, 64, 6)
     dev = [abi.Scene(s) for s in scenes]
